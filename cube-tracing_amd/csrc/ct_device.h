@@ -1,0 +1,317 @@
+// ct_device.h -- device-side math, RNG, ray-box test and BVH traversal.
+//
+// Every float expression is written as the sequence of single IEEE
+// operations the reference GLSL denotes (dot = (x*x' + y*y') + z*z',
+// normalize = v / length(v), mix = x*(1-a) + y*a, ...).  The library is
+// compiled with -ffp-contract=off and correctly rounded division and
+// square root, so nothing is fused or approximated.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ct_internal.h"
+
+namespace ct {
+namespace dev {
+
+#define CT_D __device__ __forceinline__
+
+struct F3 {
+  float x, y, z;
+};
+CT_D F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+CT_D F3 add(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+CT_D F3 sub(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+CT_D F3 mulv(F3 a, F3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+CT_D F3 muls(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+CT_D F3 divs(F3 a, float s) { return f3(a.x / s, a.y / s, a.z / s); }
+CT_D F3 neg(F3 a) { return f3(-a.x, -a.y, -a.z); }
+CT_D float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+CT_D float length(F3 a) { return sqrtf(dot(a, a)); }
+CT_D F3 normalize(F3 a) { return divs(a, length(a)); }
+CT_D F3 cross(F3 a, F3 b) {
+  return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+CT_D F3 vabs(F3 a) { return f3(fabsf(a.x), fabsf(a.y), fabsf(a.z)); }
+CT_D float gmin(float a, float b) { return fminf(a, b); }
+CT_D float gmax(float a, float b) { return fmaxf(a, b); }
+CT_D float clampf(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
+CT_D float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+CT_D F3 mix3(F3 x, F3 y, float a) { return f3(mixf(x.x, y.x, a), mixf(x.y, y.y, a), mixf(x.z, y.z, a)); }
+CT_D float gsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+CT_D F3 xyz(float4 v) { return f3(v.x, v.y, v.z); }
+
+// Saturating conversions (NaN -> 0).
+CT_D uint32_t f2u_sat(float x) {
+  if (!(x > 0.0f)) return 0u;
+  if (x >= 4294967296.0f) return 0xFFFFFFFFu;
+  return (uint32_t)x;
+}
+CT_D int32_t f2i_sat(float x) {
+  if (isnan(x)) return 0;
+  if (x >= 2147483648.0f) return 0x7FFFFFFF;
+  if (x <= -2147483648.0f) return (int32_t)0x80000000;
+  return (int32_t)x;
+}
+
+// Column-major 4x4 (m[c*4 + r]): out_r = ((m0r*v0 + m1r*v1) + m2r*v2) + m3r*v3.
+CT_D void mat4_vec(const float *m, float v0, float v1, float v2, float v3, float *o) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) o[r] = m[0 * 4 + r] * v0 + m[1 * 4 + r] * v1 + m[2 * 4 + r] * v2 + m[3 * 4 + r] * v3;
+}
+// 3-row packing (m[c*3 + r]) of the upper three rows.
+CT_D F3 xform3(const float *m, F3 p, float w) {
+  return f3(m[0] * p.x + m[3] * p.y + m[6] * p.z + m[9] * w,
+            m[1] * p.x + m[4] * p.y + m[7] * p.z + m[10] * w,
+            m[2] * p.x + m[5] * p.y + m[8] * p.z + m[11] * w);
+}
+
+// ---------------------------------------------------------------- RNG
+// random.glsl:26-40
+CT_D uint32_t tea(uint32_t val0, uint32_t val1) {
+  uint32_t v0 = val0, v1 = val1, s0 = 0;
+#pragma unroll
+  for (uint32_t n = 0; n < 16; n++) {
+    s0 += 0x9e3779b9u;
+    v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+    v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+  }
+  return v0;
+}
+// random.glsl:44-56
+CT_D uint32_t lcg(uint32_t &prev) {
+  prev = 1664525u * prev + 1013904223u;
+  return prev & 0x00FFFFFFu;
+}
+CT_D float rnd(uint32_t &prev) { return (float)lcg(prev) / 16777216.0f; }
+CT_D float rnd_interval(uint32_t &prev, float mn, float mx) { return mn + rnd(prev) * (mx - mn); }
+// random.glsl:69-77
+CT_D uint32_t urnd_interval(uint32_t &prev, uint32_t mn, uint32_t mx) {
+  if (mn == mx) return mn;
+  if (mn > mx) return f2u_sat(rnd_interval(prev, (float)mx, (float)mn));
+  return f2u_sat(rnd_interval(prev, (float)mn, (float)mx));
+}
+
+// ------------------------------------------------------------ geometry
+CT_D float safe_inverse(float x) { return (x == 0.0f) ? 1e12f : (1.0f / x); }
+
+// primitives.glsl:233-257
+CT_D F3 compute_ray_origin(F3 pos, F3 n) {
+  const float origin = 1.f / 32.f, f_scale = 1.f / 65536.f, i_scale = 256.f;
+  int32_t ix = f2i_sat(n.x * i_scale), iy = f2i_sat(n.y * i_scale), iz = f2i_sat(n.z * i_scale);
+  float px = __uint_as_float(__float_as_uint(pos.x) + (uint32_t)(pos.x < 0.0f ? -ix : ix));
+  float py = __uint_as_float(__float_as_uint(pos.y) + (uint32_t)(pos.y < 0.0f ? -iy : iy));
+  float pz = __uint_as_float(__float_as_uint(pos.z) + (uint32_t)(pos.z < 0.0f ? -iz : iz));
+  F3 fo = muls(n, f_scale);
+  return f3(fabsf(pos.x) < origin ? pos.x + fo.x : px, fabsf(pos.y) < origin ? pos.y + fo.y : py,
+            fabsf(pos.z) < origin ? pos.z + fo.z : pz);
+}
+// Box.rotation = mat3(1): rotation * v and v * rotation as written.
+CT_D F3 rot_mul(F3 v) {
+  return f3(1.0f * v.x + 0.0f * v.y + 0.0f * v.z, 0.0f * v.x + 1.0f * v.y + 0.0f * v.z,
+            0.0f * v.x + 0.0f * v.y + 1.0f * v.z);
+}
+CT_D F3 mul_rot(F3 v) {
+  return f3(v.x * 1.0f + v.y * 0.0f + v.z * 0.0f, v.x * 0.0f + v.y * 1.0f + v.z * 0.0f,
+            v.x * 0.0f + v.y * 0.0f + v.z * 1.0f);
+}
+
+// primitives.glsl:273-355 with rotation = identity and radius = h on all
+// axes.  `dr` is the (rotated) direction and `sg` = -sign(dr), both
+// independent of the box.  oriented: divide by dr; else multiply by inv.
+template <bool kCanStart, bool kOriented>
+CT_D bool intersect_box(F3 c, F3 ro, F3 dr, F3 sg, F3 inv, float &dist) {
+  const float r = kHalfVoxel, ir = 64.0f;  // safeInverse(HALF_VOXEL_EXTENT)
+  F3 o = rot_mul(sub(ro, c));
+  float winding = 1.0f;
+  if (kCanStart) {
+    F3 rb = muls(vabs(o), ir);
+    winding = (gmax(rb.x, gmax(rb.y, rb.z)) < 1.0f) ? -1.0f : 1.0f;
+  }
+  float rw = r * winding;
+  F3 dtp = sub(f3(rw * sg.x, rw * sg.y, rw * sg.z), o);
+  if (kOriented)
+    dtp = f3(dtp.x / dr.x, dtp.y / dr.y, dtp.z / dr.z);
+  else
+    dtp = mulv(dtp, inv);
+  bool tx = (dtp.x >= 0.0f) && (fabsf(o.y + dr.y * dtp.x) < r) && (fabsf(o.z + dr.z * dtp.x) < r);
+  bool ty = (dtp.y >= 0.0f) && (fabsf(o.z + dr.z * dtp.y) < r) && (fabsf(o.x + dr.x * dtp.y) < r);
+  bool tz = (dtp.z >= 0.0f) && (fabsf(o.x + dr.x * dtp.z) < r) && (fabsf(o.y + dr.y * dtp.z) < r);
+  float sx = tx ? sg.x : 0.0f;
+  float sy = tx ? 0.0f : (ty ? sg.y : 0.0f);
+  float sz = (tx || ty) ? 0.0f : (tz ? sg.z : 0.0f);
+  dist = (sx != 0.0f) ? dtp.x : ((sy != 0.0f) ? dtp.y : dtp.z);
+  return (sx != 0.0f) || (sy != 0.0f) || (sz != 0.0f);
+}
+
+// primitives.glsl:499-507
+CT_D F3 cube_like_normal(F3 n) {
+  F3 a = vabs(n);
+  float mc = gmax(gmax(a.x, a.y), a.z);
+  return (mc == a.x) ? f3(gsign(n.x), 0, 0) : (mc == a.y) ? f3(0, gsign(n.y), 0) : f3(0, 0, gsign(n.z));
+}
+
+// mat.glsl:111-202
+CT_D F3 sky_grad(F3 dir, F3 a, F3 b) {
+  F3 u = normalize(dir);
+  float t = 0.5f * (u.y + 1.0f);
+  return mix3(a, b, t);
+}
+CT_D F3 day_color(F3 d) { return sky_grad(d, f3(1.0f, 1.0f, 1.0f), f3(0.5f, 0.7f, 1.0f)); }
+CT_D F3 night_color(F3 d) { return sky_grad(d, f3(0.2f, 0.2f, 0.3f), f3(0.2f, 0.2f, 0.5f)); }
+CT_D F3 sunset_color(F3 d) { return sky_grad(d, f3(0.8f, 0.4f, 0.0f), f3(1.0f, 0.4f, 0.0f)); }
+CT_D F3 sunrise_color(F3 d) { return sky_grad(d, f3(0.7f, 0.35f, 0.0f), f3(0.8f, 0.4f, 0.0f)); }
+CT_D F3 sky_color(float t, bool afternoon, F3 dir) {
+  const float night_duration = 0.5f, sunrise_duration = 0.10f, full_daylight_transition = 0.10f,
+              full_night_transition = 0.10f;
+  float sunrise_end = night_duration + sunrise_duration;
+  float sunrise_to_daylight = sunrise_end + full_daylight_transition;
+  float zenith_transition = 1.0f - sunrise_to_daylight;
+  float daylight_end = 1.0f - zenith_transition;
+  float sunset_start = 1.0f - zenith_transition - sunrise_duration;
+  float sunset_end = 1.0f - zenith_transition - sunrise_duration - full_night_transition;
+  if (afternoon) {
+    if (t < night_duration) return night_color(dir);
+    if (t < sunrise_end) return mix3(night_color(dir), sunrise_color(dir), t);
+    if (t < sunrise_to_daylight) return mix3(sunrise_color(dir), sunset_color(dir), t);
+    return mix3(sunset_color(dir), day_color(dir), t);
+  }
+  if (t > daylight_end) return mix3(sunrise_color(dir), day_color(dir), t);
+  if (t > sunset_start) return mix3(sunset_color(dir), sunrise_color(dir), t);
+  if (t > sunset_end) return mix3(night_color(dir), sunset_color(dir), t);
+  return night_color(dir);
+}
+// light.glsl:78-80
+CT_D float luminance(F3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+// ------------------------------------------------------------ traversal
+constexpr uint32_t kMaxTraversalSteps = 1u << 22;
+
+struct TraceCnt {
+  uint32_t nodes = 0, leaves = 0, insts = 0;
+};
+
+CT_D bool slab(float lx, float ly, float lz, float hx, float hy, float hz, F3 o, F3 inv,
+               float limit, float &tn) {
+  float t1x = (lx - o.x) * inv.x, t2x = (hx - o.x) * inv.x;
+  float t1y = (ly - o.y) * inv.y, t2y = (hy - o.y) * inv.y;
+  float t1z = (lz - o.z) * inv.z, t2z = (hz - o.z) * inv.z;
+  float n0 = gmax(gmax(gmin(t1x, t2x), gmin(t1y, t2y)), gmin(t1z, t2z));
+  float f0 = gmin(gmin(gmax(t1x, t2x), gmax(t1y, t2y)), gmax(t1z, t2z));
+  tn = n0;
+  return (n0 <= f0) && (f0 >= 0.0f) && (n0 <= limit);
+}
+
+// Test both children of a node; returns the next ref (or kInvalid to pop)
+// and pushes the far child when both are hit (near first, ties to child 0).
+CT_D uint32_t visit_node(const DNode *nodes, uint32_t idx, F3 o, F3 inv, float limit,
+                         uint32_t *stk, uint32_t stride, int &sp, bool &overflow) {
+  const float4 *p = reinterpret_cast<const float4 *>(nodes + idx);
+  float4 a = p[0], b = p[1], c = p[2];
+  uint4 d = reinterpret_cast<const uint4 *>(nodes + idx)[3];
+  float tn0, tn1;
+  bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, limit, tn0);
+  bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, limit, tn1);
+  if (h0 && h1) {
+    bool swap = tn1 < tn0;
+    if (sp >= kStackSize) {
+      overflow = true;
+      return kInvalid;
+    }
+    stk[(uint32_t)sp * stride] = swap ? d.x : d.y;
+    ++sp;
+    return swap ? d.y : d.x;
+  }
+  if (h0) return d.x;
+  if (h1) return d.y;
+  return kInvalid;
+}
+
+// Two-level traversal (TLAS in world space, each BLAS in object space).
+// kAny: is_vertex_visible semantics, returns true when occluded.
+// Closest: traceRayEXT + rint semantics, smallest t then smallest
+// (instance, primitive).  `stk` is this lane's LDS stack column.
+template <bool kAny, bool kCount>
+CT_D bool traverse(const DScene &S, F3 o, F3 d, float limit, uint32_t ti, uint32_t tp,
+                   float &best_t, uint32_t &best_i, uint32_t &best_p, uint32_t *stk,
+                   uint32_t stride, TraceCnt &cnt, bool &overflow) {
+  const F3 winv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  F3 oo = o, od = d, oinv = winv, dr = d, sg = d, sinv = d;
+  uint32_t first = 0, inst = 0;
+  bool in_blas = false;
+  int base = 0, sp = 0;
+  uint32_t ref = S.tlas_root;
+  for (uint32_t iter = 0;; ++iter) {
+    if (iter > kMaxTraversalSteps) {  // only a malformed tree can get here
+      overflow = true;
+      return false;
+    }
+    uint32_t next = kInvalid;
+    if (!in_blas) {
+      if (ref & kLeaf) {
+        inst = S.tlas_inst[ref & ~kLeaf];
+        if (kCount) cnt.insts++;
+        const DInst &I = S.inst[inst];
+        float m[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) m[k] = I.w2o[k];
+        oo = xform3(m, o, 1.0f);
+        od = xform3(m, d, 0.0f);
+        oinv = f3(1.0f / od.x, 1.0f / od.y, 1.0f / od.z);
+        if (kAny) {
+          dr = od;
+          sinv = f3(safe_inverse(od.x), safe_inverse(od.y), safe_inverse(od.z));
+        } else {
+          dr = mul_rot(od);
+        }
+        sg = f3(-gsign(dr.x), -gsign(dr.y), -gsign(dr.z));
+        first = I.first;
+        in_blas = true;
+        base = sp;
+        ref = I.root;
+        continue;
+      }
+      if (kCount) cnt.nodes++;
+      next = visit_node(S.tlas_nodes, ref, o, winv, kAny ? limit : best_t, stk, stride, sp, overflow);
+    } else {
+      if (ref & kLeaf) {
+        if (kCount) cnt.leaves++;
+        float4 L = S.blas_leaves[first + (ref & ~kLeaf)];
+        uint32_t pid = __float_as_uint(L.w);
+        float t;
+        if (kAny) {
+          bool hit = intersect_box<false, false>(f3(L.x, L.y, L.z), oo, dr, sg, sinv, t);
+          if (hit && (t >= 0.0f) && (t <= limit) && !(inst == ti && pid == tp)) return true;
+        } else {
+          bool hit = intersect_box<true, true>(f3(L.x, L.y, L.z), oo, dr, sg, sinv, t);
+          if (hit && t > 0.0f) {
+            bool better = (t < best_t) || (t == best_t && ((inst < best_i) || (inst == best_i && pid < best_p)));
+            if (better) {
+              best_t = t;
+              best_i = inst;
+              best_p = pid;
+            }
+          }
+        }
+      } else {
+        if (kCount) cnt.nodes++;
+        next = visit_node(S.blas_nodes, first + ref, oo, oinv, kAny ? limit : best_t, stk, stride, sp, overflow);
+      }
+    }
+    if (overflow) return false;
+    if (next != kInvalid) {
+      ref = next;
+      continue;
+    }
+    if (in_blas && sp == base) in_blas = false;
+    if (sp == 0) break;
+    --sp;
+    ref = stk[(uint32_t)sp * stride];
+  }
+  return false;
+}
+
+}  // namespace dev
+}  // namespace ct

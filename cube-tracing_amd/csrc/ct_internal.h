@@ -1,0 +1,220 @@
+// ct_internal.h -- host-side internals shared by the library's translation
+// units: error reporting, HIP checks, the device-side data layouts and the
+// accel/ctx objects behind the opaque handles of include/ct_abi.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <queue>
+#include <string>
+#include <vector>
+
+#include "../../include/ct_abi.h"
+
+namespace ct {
+
+// ------------------------------------------------------------ errors
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define CT_HIP(call)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      return ::ct::fail(CT_E_HIP, std::string(#call) + ": " +                \
+                                      hipGetErrorString(e_));                 \
+  } while (0)
+
+// ------------------------------------------------------ device layouts
+constexpr uint32_t kLeaf = 0x80000000u;
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr float kHalfVoxel = 0.015625f;      // HALF_VOXEL_EXTENT
+constexpr float kLeafPad = 0.000244140625f;  // 2^-12 node padding
+constexpr int kStackSize = 32;               // LDS traversal stack entries/lane
+constexpr uint32_t kRaySlots = 16;           // shadow-ray slots per pixel and pass
+
+// BVH2 node, 64 B: child0 box (lo xyz, hi xyz), child1 box, two refs.
+// A ref with bit 31 set is a leaf (sorted position), else an internal node.
+struct alignas(16) DNode {
+  float4 a;  // c0.lo.xyz, c0.hi.x
+  float4 b;  // c0.hi.yz, c1.lo.xy
+  float4 c;  // c1.lo.z, c1.hi.xyz
+  uint4 d;   // ref0, ref1, -, -
+};
+static_assert(sizeof(DNode) == 64, "node layout");
+
+// Per-instance record the kernels read.
+struct alignas(16) DInst {
+  float w2o[12];    // world-to-object, column c row r at [c*3 + r] (rows 0..2)
+  float model[12];  // rchit.glsl:29-33 model matrix (same packing)
+  uint32_t first;   // first_primitive_index: base of its nodes and leaves
+  uint32_t count;   // primitive_count
+  uint32_t root;    // BLAS root ref (relative to `first`)
+  uint32_t pad;
+};
+
+struct DScene {
+  const DNode *blas_nodes;
+  const float4 *blas_leaves;  // (centre.xyz, prim id bits), Morton order
+  const DInst *inst;
+  const DNode *tlas_nodes;
+  const uint32_t *tlas_inst;  // instance id per TLAS leaf
+  uint32_t tlas_root;
+  uint32_t pad;
+};
+
+// Raw device buffer with RAII.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // Grow to at least `n` bytes (contents are not preserved).
+  int reserve(size_t n) {
+    if (n <= bytes && p) return CT_OK;
+    release();
+    size_t m = n ? n : 16;
+    hipError_t e = hipMalloc(&p, m);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(CT_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    bytes = m;
+    return CT_OK;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Build scratch for the LBVH kernels (reused between builds).
+struct BuildScratch {
+  DevBuf keys[2], vals[2], centers, lo, hi, parent_int, parent_leaf, flags, hist, seg,
+      root_box, depth, sorted_keys32;
+};
+
+// ---------------------------------------------------------- accel
+struct BvhSet {
+  DevBuf nodes, leaves, instances_d, prims, aabbs, cube_lights, inst_aos;
+  DevBuf tlas_nodes, tlas_inst, tlas_keys, tlas_parent;
+  DevBuf blas_keys, blas_ids, blas_parent;  // kept for ct_accel_read_bvh
+  uint32_t tlas_root = kLeaf;
+  uint32_t tlas_count = 0;
+  uint32_t tlas_depth = 0;
+  uint32_t blas_depth_max = 0;
+  uint32_t generation = 0;
+  uint32_t prim_capacity = 0;
+  uint32_t cube_light_count = 0;
+};
+
+struct FreeList {  // first-fit over [0, capacity) (containers/free_list.hpp:103-278)
+  struct Range { uint32_t off, size; };
+  std::vector<Range> free_ranges;
+  void init(uint32_t capacity) { free_ranges.assign(1, Range{0, capacity}); }
+  bool allocate(uint32_t size, uint32_t &off);
+  void deallocate(uint32_t off, uint32_t size);
+};
+
+struct Accel {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t max_instances = 0, max_primitives = 0, max_cube_lights = 0;
+  // host staging (per batch, reset after processing)
+  std::vector<ct_instance> temp_instances;
+  std::vector<ct_primitive> primitives_host;
+  std::vector<ct_aabb> aabb_host;
+  std::vector<ct_light> cube_lights_host;
+  uint32_t temp_instance_count = 0, temp_primitive_count = 0;
+  uint32_t cube_light_count = 0;
+  std::queue<ct_task> tasks;
+  // committed host mirror of the device scene
+  std::vector<ct_instance> instances;       // by instance id
+  std::vector<uint8_t> instance_used;       // free_uuid_list state
+  std::vector<float> w2o;                   // [id][16]
+  std::vector<ct_aabb> aabbs;               // by global primitive index
+  std::vector<ct_primitive> prims;
+  std::vector<float> blas_root_box;         // [id][6]
+  std::vector<uint32_t> blas_depth;
+  FreeList prim_alloc;
+  uint32_t prim_high_water = 0;
+  BvhSet set;
+  BuildScratch scratch;
+  uint32_t status = CT_AS_IDLE;
+  uint32_t generation = 0;
+};
+
+// Builds every BLAS of `ids` and then the TLAS.  Implemented in accel.hip.
+int rebuild_all(Accel &a);
+
+// ------------------------------------------------------------- ctx
+struct FrameBufs {
+  DevBuf di_pos[2], di_nrm[2], di_org[2], di_hit[2];  // DI, current/previous
+  DevBuf resA[2], resB[2], resC[2];                   // intermediate / previous
+  DevBuf velocity, img[2], taa, hit, pidx, prev_res_copy;
+  DevBuf mask, seed2, nbr;                            // per-pixel pass state
+  DevBuf ray_o, ray_d, ray_t, queue, counters, counters_host_staging;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t max_w = 0, max_h = 0;
+  FrameBufs fb;
+  DevBuf materials, point_lights, env_lights, pt, inst_pt;
+  std::vector<ct_material> mats_host;
+  std::vector<ct_light> point_host, env_host;
+  ct_light_config cfg{};
+  int cur = 1;
+  uint32_t instrument = 0;
+  hipEvent_t ev[16] = {};
+  int n_events = 0;
+  ct_frame_stats stats{};
+  uint64_t *counters_pinned = nullptr;
+  bool stats_pending = false;
+  hipStream_t stream_last = nullptr;
+  uint32_t last_w = 0, last_h = 0;
+};
+
+// Kernel-side views used by trace.hip and frame.hip.
+DScene scene_view(const Accel &a);
+
+// Launch the any-hit traversal over a ray queue (frame.hip / trace.hip).
+struct TraceAnyArgs {
+  const float4 *ray_o;   // (origin, distance)
+  const float4 *ray_d;   // (direction, target instance bits)
+  const uint32_t *ray_t; // target primitive
+  const uint32_t *queue; // ray indices, or null: index = queue position
+  const uint32_t *count; // device count (queue length)
+  uint32_t *head;        // device work counter, zeroed by the launch
+  uint32_t *mask;        // per-pixel occlusion bits (index / 16, bit index % 16)
+  uint8_t *occ;          // alternatively one byte per ray (index), may be null
+  unsigned long long *stats;  // nodes, leaves, instances (instrumented launch) or null
+  uint32_t max_rays;
+};
+int launch_trace_any(const DScene &s, const TraceAnyArgs &args, hipStream_t stream);
+
+struct TraceClosestArgs {
+  const float4 *ray_o;   // (origin, -)
+  const float4 *ray_d;   // (direction, -)
+  float tmax;
+  uint32_t n;
+  float4 *hit;           // (t, instance bits, primitive bits, -)
+  unsigned long long *stats;
+};
+int launch_trace_closest(const DScene &s, const TraceClosestArgs &args, hipStream_t stream);
+
+int device_count_gfx950();
+
+}  // namespace ct
+
+struct ct_accel : ct::Accel {};
+struct ct_ctx : ct::Ctx {};
+struct ct_scene;

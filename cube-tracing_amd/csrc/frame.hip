@@ -1,0 +1,1345 @@
+// frame.hip -- the per-frame passes App::draw records (src/main.cpp:1436-1703):
+// the prepass (rgen.glsl:46-320), the shading pass (rgen.glsl:329-510) and
+// TAA (taa.glsl:16-108), on structure-of-arrays buffers in HBM.
+//
+// Wavefront form.  Every pass that traces shadow rays runs its per-pixel
+// logic twice: an EMIT launch writes each needed shadow ray to a fixed slot
+// (pixel * 16 + k) and appends the slot to a compacted queue, the any-hit
+// traversal kernel resolves the queue into per-pixel occlusion bits, and a
+// FINISH launch re-runs the same code reading the bits.  Both launches
+// evaluate the same deterministic sequence of light samples, so the k-th
+// ray of a pixel means the same query in both.  Queries of one pass never
+// depend on each other's outcome (the reference code issues them before
+// any result-dependent branch), which makes the split exact.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "ct_device.h"
+#include "ct_internal.h"
+
+namespace ct {
+using namespace dev;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kMaxInstances = CT_MAX_INSTANCES;
+constexpr uint32_t kMaxPrimitives = CT_MAX_PRIMITIVES;
+constexpr float kInvPi = 0.31830988618379067153776752674503f;
+constexpr float kInv4Pi = 0.079577471545947667884441881686255f;
+constexpr float kMaxDistance = 1e9f;
+constexpr float kMinCosTheta = 1e-6f;
+
+// counters[] slots
+constexpr int kCntQueue = 0, kCntHead = 1, kCntRis = 2, kCntTemporal = 3, kCntSpatial = 4;
+
+struct DFrame {
+  uint32_t W, H, npix, frame, active, afternoon;
+  float time;
+  uint32_t n_cube, n_point, n_env, n_mats;
+  float inv_view[16], inv_proj[16];
+  ct_restir_params rp;
+  ct_light_config cfg;
+  DScene S;
+  const ct_aabb *aabbs;
+  const ct_primitive *prims;
+  const float4 *mats;   // [2 per material]: (diffuse, type), (emission, -)
+  const float4 *cube;   // [3 per light]: (pos, size), (emissive, type), (inst, prim, -, -)
+  const float4 *point;
+  const float4 *env;
+  const float *pt;      // [16 per instance] prev_view_proj * geometry_T
+  float4 *dpos_c, *dnrm_c, *dorg_c;
+  uint4 *dhit_c;
+  float4 *dpos_p, *dnrm_p, *dorg_p;
+  uint4 *dhit_p;
+  uint4 *rA_i; float4 *rB_i; float *rC_i;
+  uint4 *rA_p; float4 *rB_p; float *rC_p;
+  float2 *vel;
+  float4 *img;
+  const float4 *img_prev;
+  float4 *taa;
+  float4 *hit;
+  uint32_t *pidx;
+  uint32_t *mask;
+  float4 *ray_o, *ray_d;
+  uint32_t *ray_t;
+  uint32_t *queue;
+  uint32_t *counters;
+};
+
+struct PLight {
+  F3 pos;
+  float size;
+  F3 emissive;
+  uint32_t type, inst, prim;
+};
+struct Mat {
+  F3 diffuse, emission;
+};
+struct Hit {  // HIT_INFO_INPUT, shared.inl:171-179
+  F3 P, n;
+  float distance;
+  F3 scatter;
+  uint32_t inst, prim, mat;
+};
+struct Res {  // RESERVOIR, shared.inl:368-377
+  uint32_t Y, type, seed;
+  float W_y, W_sum, M;
+  F3 F;
+};
+struct DI {  // DIRECT_ILLUMINATION_INFO, shared.inl:430-440
+  F3 pos;
+  float distance;
+  F3 nrm, org;
+  uint32_t seed, inst, prim, mat;
+  float conf;
+};
+
+CT_D Res res_init() {
+  Res r;
+  r.Y = kInvalid; r.type = CT_LIGHT_NONE; r.seed = 0;
+  r.W_y = 0.0f; r.W_sum = 0.0f; r.M = 0.0f;
+  r.F = f3(0.0f, 0.0f, 0.0f);
+  return r;
+}
+CT_D Res ld_res(const uint4 *A, const float4 *B, const float *C, uint32_t i) {
+  uint4 a = A[i];
+  float4 b = B[i];
+  Res r;
+  r.Y = a.x; r.type = a.y; r.seed = a.z; r.W_y = __uint_as_float(a.w);
+  r.W_sum = b.x; r.M = b.y; r.F = f3(b.z, b.w, C[i]);
+  return r;
+}
+CT_D void st_res(uint4 *A, float4 *B, float *C, uint32_t i, const Res &r) {
+  A[i] = make_uint4(r.Y, r.type, r.seed, __float_as_uint(r.W_y));
+  B[i] = make_float4(r.W_sum, r.M, r.F.x, r.F.y);
+  C[i] = r.F.z;
+}
+CT_D DI ld_di(const float4 *P, const float4 *N, const float4 *O, const uint4 *Hh, uint32_t i) {
+  float4 p = P[i], n = N[i], o = O[i];
+  uint4 h = Hh[i];
+  DI d;
+  d.pos = xyz(p); d.distance = p.w;
+  d.nrm = xyz(n); d.mat = __float_as_uint(n.w);
+  d.org = xyz(o); d.seed = __float_as_uint(o.w);
+  d.inst = h.x; d.prim = h.y; d.conf = __uint_as_float(h.z);
+  return d;
+}
+CT_D void st_di(float4 *P, float4 *N, float4 *O, uint4 *Hh, uint32_t i, const DI &d) {
+  P[i] = make_float4(d.pos.x, d.pos.y, d.pos.z, d.distance);
+  N[i] = make_float4(d.nrm.x, d.nrm.y, d.nrm.z, __uint_as_float(d.mat));
+  O[i] = make_float4(d.org.x, d.org.y, d.org.z, __uint_as_float(d.seed));
+  Hh[i] = make_uint4(d.inst, d.prim, __float_as_uint(d.conf), 0u);
+}
+CT_D PLight ld_light(const float4 *L, uint32_t i) {
+  float4 a = L[3 * i], b = L[3 * i + 1], c = L[3 * i + 2];
+  PLight l;
+  l.pos = xyz(a); l.size = a.w;
+  l.emissive = xyz(b); l.type = __float_as_uint(b.w);
+  l.inst = __float_as_uint(c.x); l.prim = __float_as_uint(c.y);
+  return l;
+}
+CT_D Mat ld_mat(const DFrame &F, uint32_t i) {
+  Mat m;
+  if (i >= F.n_mats) {
+    m.diffuse = f3(0, 0, 0);
+    m.emission = f3(0, 0, 0);
+    return m;
+  }
+  float4 a = F.mats[2 * i], b = F.mats[2 * i + 1];
+  m.diffuse = xyz(a);
+  m.emission = xyz(b);
+  return m;
+}
+CT_D bool res_valid(const Res &r) { return r.M > 0.0f && r.Y != kInvalid && r.type != CT_LIGHT_NONE; }
+CT_D bool light_by_type(const DFrame &F, uint32_t type, uint32_t idx, PLight &l) {
+  if (type == CT_LIGHT_POINT && idx < F.n_point) { l = ld_light(F.point, idx); return true; }
+  if (type == CT_LIGHT_ENV_MAP && idx < F.n_env) { l = ld_light(F.env, idx); return true; }
+  if (type == CT_LIGHT_CUBE && idx < F.n_cube) { l = ld_light(F.cube, idx); return true; }
+  return false;
+}
+CT_D Hit hit_from_di(const DI &d) {
+  Hit h;
+  F3 p = compute_ray_origin(d.pos, d.nrm);
+  h.P = compute_ray_origin(p, d.nrm);
+  h.n = d.nrm;
+  h.distance = d.distance;
+  h.scatter = f3(0, 0, 0);
+  h.inst = d.inst; h.prim = d.prim; h.mat = d.mat;
+  return h;
+}
+
+// reservoir.glsl:31-50
+CT_D bool update_reservoir(Res &r, uint32_t X, uint32_t type, uint32_t rs, F3 Fv, float w, float c, uint32_t &seed) {
+  if (w < 0.0f || isnan(w) || isinf(w)) return false;
+  r.M += c;
+  r.W_sum += w;
+  if (rnd(seed) < (w / r.W_sum)) {
+    r.Y = X; r.type = type; r.seed = rs; r.F = Fv;
+    return true;
+  }
+  return false;
+}
+
+// ------------------------------------------------------- visibility source
+// EMIT: a query that needs a shadow ray writes it to slot (pixel*16 + k)
+// and reports "unoccluded".  FINISH: the k-th query reads bit k.
+struct Vis {
+  bool emit;
+  uint32_t px, slot, bits, emitted;
+};
+CT_D bool occluded(const DFrame &F, Vis &v, F3 o, F3 d, float dist, uint32_t ti, uint32_t tp) {
+  uint32_t k = v.slot++;
+  if (k >= kRaySlots) return false;  // bounded by CT_MAX_NEIGHBORS; never taken
+  if (v.emit) {
+    uint32_t idx = v.px * kRaySlots + k;
+    F.ray_o[idx] = make_float4(o.x, o.y, o.z, dist);
+    F.ray_d[idx] = make_float4(d.x, d.y, d.z, __uint_as_float(ti));
+    F.ray_t[idx] = tp;
+    v.emitted |= 1u << k;
+    return false;
+  }
+  return (v.bits >> k) & 1u;
+}
+
+// mat.glsl:24-27, 58-88; light.glsl:200-229 (Lambertian)
+CT_D float local_z(F3 n, F3 v) { return dot(v, normalize(n)); }
+CT_D float material_pdf(F3 n, F3 wo, F3 wi) {
+  float a = local_z(n, wo), b = local_z(n, wi);
+  if (gmin(a, b) < kMinCosTheta) return 0.0f;
+  return kInvPi * b;
+}
+CT_D F3 evaluate_material(const Mat &m, F3 n, F3 wo, F3 wi) {
+  float a = local_z(n, wo), b = local_z(n, wi);
+  if (gmin(a, b) < kMinCosTheta) return f3(0, 0, 0);
+  return muls(muls(m.diffuse, kInvPi), b);
+}
+CT_D float geom_fact_sa(F3 P, F3 Ps, F3 ns) {
+  F3 dir = normalize(sub(Ps, P));
+  F3 dd = sub(Ps, P);
+  return fabsf(dot(ns, dir)) / dot(dd, dd);
+}
+// prng.glsl:198-252
+CT_D F3 random_cube_normal(uint32_t &seed) {
+  uint32_t face = min(urnd_interval(seed, 0u, 6u), 5u);
+  switch (face) {
+    case 0: return f3(-1, 0, 0);
+    case 1: return f3(1, 0, 0);
+    case 2: return f3(0, -1, 0);
+    case 3: return f3(0, 1, 0);
+    case 4: return f3(0, 0, -1);
+    default: return f3(0, 0, 1);
+  }
+}
+CT_D F3 random_quad(F3 n, F3 p, float sx, float sy, uint32_t &seed) {
+  float u = rnd(seed);
+  float v = rnd(seed);
+  float qx = u * sx, qy = v * sy;
+  F3 t1 = fabsf(n.x) > fabsf(n.y) ? normalize(f3(-n.z, 0.0f, n.x)) : normalize(f3(0.0f, n.z, -n.y));
+  F3 t2 = normalize(cross(n, t1));
+  return add(add(p, muls(t1, qx)), muls(t2, qy));
+}
+CT_D F3 random_on_hemisphere(uint32_t &seed, F3 n) {
+  F3 p = f3(0, 0, 0);
+  for (int k = 0; k < 1024; ++k) {
+    float x = rnd_interval(seed, -1.0f, 1.0f);
+    float y = rnd_interval(seed, -1.0f, 1.0f);
+    float z = rnd_interval(seed, -1.0f, 1.0f);
+    p = f3(x, y, z);
+    if (x * x + y * y + z * z >= 1.0f) continue;
+    break;
+  }
+  F3 u = normalize(p);
+  return dot(u, n) > 0.0f ? u : neg(u);
+}
+
+struct LS {
+  bool ok;      // the sample is valid before the shadow test
+  bool ray;     // a shadow ray would be traced
+  F3 P, dir;
+  float dist;
+  uint32_t ti, tp;
+  F3 l_pos, Le;
+  float G;
+};
+
+// sample_lights, light.glsl:295-399, without the shadow ray.
+CT_D void sample_lights(const DFrame &F, const Hit &hit, const PLight &l, float &pdf, uint32_t &seed, bool calc_pdf,
+                        bool visibility, LS &o) {
+  o.ok = false; o.ray = false; o.G = 0.0f; o.Le = f3(0, 0, 0); o.l_pos = f3(0, 0, 0);
+  if (l.inst == hit.inst && l.prim == hit.prim) return;
+  F3 P = hit.P, n = hit.n;
+  float distance = -1.0f;
+  F3 l_wi = f3(0, 0, 0), l_pos = f3(0, 0, 0), l_nor = f3(0, 0, 0);
+  bool vis = true, check = false;
+  F3 em = l.emissive;
+  if (l.type == CT_LIGHT_CUBE) {
+    float half = l.size * 0.5f, qs = l.size;
+    uint32_t i = 0;
+    do {
+      vis = true;
+      l_nor = random_cube_normal(seed);
+      l_pos = add(l.pos, muls(l_nor, half));
+      l_pos = random_quad(l_nor, l_pos, qs * 0.5f, qs * 0.5f, seed);
+      l_wi = normalize(sub(P, l_pos));
+      vis = vis && (dot(l_wi, l_nor) > 0.0f);
+      vis = vis && (dot(l_wi, n) < 0.0f);
+    } while (!vis && i++ < 10);
+    l_pos = compute_ray_origin(l_pos, l_nor);
+    l_pos = compute_ray_origin(l_pos, l_nor);
+    distance = length(sub(P, l_pos));
+    if (calc_pdf) {
+      float area = qs * qs * 6.0f;
+      pdf /= area;
+    }
+    check = true;
+    o.G = geom_fact_sa(P, l_pos, l_nor);
+  } else if (l.type == CT_LIGHT_POINT) {
+    l_pos = l.pos;
+    l_nor = normalize(sub(P, l_pos));
+    distance = length(sub(P, l_pos));
+    l_wi = l_nor;
+    o.G = geom_fact_sa(P, l_pos, l_nor);
+  } else if (l.type == CT_LIGHT_ENV_MAP) {
+    F3 l_dir = random_on_hemisphere(seed, n);
+    l_pos = add(P, muls(l_dir, kMaxDistance));
+    l_nor = normalize(sub(P, l_pos));
+    distance = length(sub(P, l_pos));
+    l_wi = l_nor;
+    em = mulv(em, sky_color(F.time, F.afternoon != 0, l_dir));
+    if (calc_pdf) pdf /= kInv4Pi;
+    o.G = 1.0f;
+  }
+  o.ray = visibility && vis;
+  o.P = P;
+  o.dir = neg(l_wi);
+  o.dist = distance;
+  o.ti = check ? l.inst : kInvalid;
+  o.tp = check ? l.prim : kInvalid;
+  o.l_pos = l_pos;
+  o.Le = em;
+  o.ok = vis;
+}
+
+// calculate_sampled_light, light.glsl:401-435.  With use_vis the shadow
+// ray goes through `v`.
+CT_D F3 sampled_light(const DFrame &F, F3 ray_d, Hit &hit, const Mat &mat, const PLight &l, float pdf, float &pdf_out,
+                      float &G, uint32_t &seed, bool calc_pdf, bool use_pdf, bool use_vis, Vis *v) {
+  F3 sn = normalize(hit.n);
+  F3 wo = neg(normalize(ray_d));
+  pdf_out = pdf;
+  G = 0.0f;
+  if (l.emissive.x <= 0.0f && l.emissive.y <= 0.0f && l.emissive.z <= 0.0f) return f3(0, 0, 0);
+  LS ls;
+  sample_lights(F, hit, l, pdf_out, seed, calc_pdf, use_vis, ls);
+  G = ls.G;
+  bool ok = ls.ok;
+  if (ls.ray && occluded(F, *v, ls.P, ls.dir, ls.dist, ls.ti, ls.tp)) ok = false;
+  if (!ok) return f3(0, 0, 0);
+  F3 wi = normalize(sub(ls.l_pos, hit.P));
+  F3 brdf = evaluate_material(mat, sn, wo, wi);
+  hit.scatter = wi;
+  F3 bl = mulv(brdf, ls.Le);
+  return use_pdf ? divs(muls(bl, G), pdf_out) : muls(bl, G);
+}
+
+// reservoir_get_radiance, reservoir.glsl:98-112
+CT_D F3 res_radiance(const DFrame &F, const Res &r, F3 ray_d, Hit hit, const Mat &mat, Vis &v) {
+  if (!res_valid(r)) return f3(0, 0, 0);
+  PLight l;
+  if (!light_by_type(F, r.type, r.Y, l)) return f3(0, 0, 0);
+  float pdf_out, G;
+  uint32_t s = r.seed;
+  return sampled_light(F, ray_d, hit, mat, l, 1.0f, pdf_out, G, s, false, false, true, &v);
+}
+
+// reservoir_visibility_pass, reservoir.glsl:76-96, 114-126
+CT_D void visibility_pass(const DFrame &F, Res &r, const Hit &hit, Vis &v) {
+  bool visibility = false;
+  if (res_valid(r)) {
+    PLight l;
+    if (light_by_type(F, r.type, r.Y, l)) {
+      float pdf = 1.0f;
+      uint32_t s = r.seed;
+      LS ls;
+      sample_lights(F, hit, l, pdf, s, false, true, ls);
+      visibility = ls.ok;
+      if (ls.ray && occluded(F, v, ls.P, ls.dir, ls.dist, ls.ti, ls.tp)) visibility = false;
+    }
+  }
+  r.W_y = visibility ? (r.W_sum / luminance(r.F)) : 0.0f;
+  r.F = muls(r.F, visibility ? 1.0f : 0.0f);
+}
+
+// RIS candidate streaming of one light type, restir_resampling.glsl:72-178
+CT_D void ris_strategy(const DFrame &F, Res &target, uint32_t count, uint32_t nsamp, uint32_t type, const float4 *lights,
+                       uint32_t nee, uint32_t n_brdf, F3 ray_d, Hit &hit, const Mat &mat, uint32_t &seed) {
+  float pdf = 1.0f / (float)count;
+  for (uint32_t k = 0; k < nsamp; ++k) {
+    uint32_t idx = min(urnd_interval(seed, 0u, count), count - 1u);
+    PLight light = ld_light(lights, idx);
+    float light_pdf = 1.0f, G = 0.0f;
+    uint32_t current_seed = seed;
+    F3 Fv = sampled_light(F, ray_d, hit, mat, light, pdf, light_pdf, G, seed, true, false, false, nullptr);
+    float brdf_pdf = material_pdf(hit.n, neg(ray_d), hit.scatter) * G;
+    float m_i = 1.0f / gmax((float)nee * light_pdf + (float)n_brdf * brdf_pdf, 1e-6f);
+    float w_i = luminance(Fv) * m_i;
+    update_reservoir(target, idx, type, current_seed, Fv, w_i, 1.0f, seed);
+  }
+}
+
+// RIS, restir_resampling.glsl:19-255 (point, environment, cube strategies)
+CT_D Res ris(const DFrame &F, float confidence, F3 ray_d, Hit &hit, const Mat &mat, uint32_t &seed) {
+  const ct_restir_params &rp = F.rp;
+  confidence = clampf(confidence, 0.0f, 1.0f);
+  const uint32_t pc = F.cfg.point_light_count, ec = F.cfg.env_map_count;
+  uint32_t n_point = max(f2u_sat(gmin((float)rp.max_ris_point_samples * (1.0f - confidence), (float)pc)), rp.min_ris_point_samples);
+  uint32_t n_env = max(f2u_sat(gmin((float)rp.max_ris_env_samples * (1.0f - confidence), (float)ec)), rp.min_ris_env_samples);
+  uint32_t cc = F.cfg.cube_light_count < 1 ? 1u : F.cfg.cube_light_count;
+  uint32_t n_cube = max(f2u_sat(gmin((float)rp.max_ris_cube_samples * (1.0f - confidence), (float)cc)), rp.min_ris_cube_samples);
+  uint32_t n_brdf = F.cfg.brdf_count + n_env;
+  uint32_t nee = n_point + n_cube;
+  Res r = res_init();
+  if ((F.active & CT_RIS_POINT_LIGHT_BIT) && F.n_point > 0)
+    ris_strategy(F, r, pc, n_point, CT_LIGHT_POINT, F.point, nee, n_brdf, ray_d, hit, mat, seed);
+  if ((F.active & CT_RIS_ENV_LIGHT_BIT) && F.n_env > 0) {
+    Res er = res_init();
+    ris_strategy(F, er, ec, n_env, CT_LIGHT_ENV_MAP, F.env, nee, n_brdf, ray_d, hit, mat, seed);
+    update_reservoir(r, er.Y, er.type, er.seed, er.F, er.W_sum, er.M, seed);
+  }
+  if ((F.active & CT_RIS_CUBE_LIGHT_BIT) && F.n_cube > 0) {
+    Res cr = res_init();
+    ris_strategy(F, cr, cc, n_cube, CT_LIGHT_CUBE, F.cube, nee, n_brdf, ray_d, hit, mat, seed);
+    update_reservoir(r, cr.Y, cr.type, cr.seed, cr.F, cr.W_sum, cr.M, seed);
+  }
+  return r;
+}
+
+// TEMPORAL_REUSE, restir_resampling.glsl:311-416 (remapping inactive)
+CT_D void temporal_reuse(const DFrame &F, Res &r, Res prev, uint32_t pidx, F3 ray_d, const Hit &hit, const Mat &mat,
+                         uint32_t &seed, Vis &v) {
+  const float mcap = F.rp.max_influence_from_the_past;
+  prev.M = gmin(prev.M, mcap * r.M);
+  const float prev_M = prev.M;
+  const float new_M = r.M + prev_M;
+  if (r.W_sum > 0.0f) {
+    float tlp = 0.0f;
+    if (luminance(r.F) > 1e-6f) {
+      DI dp = ld_di(F.dpos_p, F.dnrm_p, F.dorg_p, F.dhit_p, pidx);
+      Hit ph = hit_from_di(dp);
+      Mat pm = ld_mat(F, dp.mat);
+      F3 pd = normalize(sub(ph.P, dp.org));
+      tlp = luminance(res_radiance(F, r, pd, ph, pm, v));
+    }
+    const float p_curr = r.M * luminance(r.F);
+    const float m_curr = p_curr / gmax(p_curr + prev_M * tlp, 1e-6f);
+    r.W_sum *= m_curr;
+  }
+  if (prev.Y != kInvalid) {
+    F3 ct = res_radiance(F, prev, ray_d, hit, mat, v);
+    const float lc = luminance(ct);
+    if (lc > 0.0f) {
+      const float lp = luminance(prev.F);
+      const float p_prev = prev.M * lp;
+      const float m_prev = p_prev / gmax(p_prev + r.M * lc, 1e-6f);
+      const float w_prev = m_prev * lc * prev.W_y;
+      update_reservoir(r, prev.Y, prev.type, prev.seed, ct, w_prev, 1.0f, seed);
+    }
+  }
+  float tl = luminance(r.F);
+  r.W_y = tl > 0.0f ? r.W_sum / tl : 0.0f;
+  r.M = new_M;
+}
+
+// SPATIAL_REUSE, restir_resampling.glsl:418-540, with pairwise_mis.glsl:8-114
+CT_D void spatial_reuse(const DFrame &F, Res &r, float confidence, uint32_t x, uint32_t y, F3 ray_d, const Hit &hit,
+                        const Mat &mat, uint32_t &seed, Vis &v) {
+  const ct_restir_params &rp = F.rp;
+  if (r.W_y == 0.0f) confidence = 0.0f;
+  float radius = mixf(rp.max_neighbor_radius, rp.min_neighbor_radius, clampf(hit.distance / rp.max_distance_to_hit, 0.0f, 1.0f));
+  uint32_t k = f2u_sat(mixf((float)rp.max_neighbors, (float)rp.min_neighbors, clampf(confidence, 0.0f, 1.0f)));
+  uint32_t idx[CT_MAX_NEIGHBORS];
+  uint32_t num = 0;
+  for (uint32_t i = 0; i < k; ++i) {
+    float ox = 2.0f * rnd(seed) - 1.0f;
+    float oy = 2.0f * rnd(seed) - 1.0f;
+    int64_t nx = (int64_t)x + f2i_sat(ox * radius);
+    int64_t ny = (int64_t)y + f2i_sat(oy * radius);
+    if (nx < 0 || nx >= (int64_t)F.W || ny < 0 || ny >= (int64_t)F.H) continue;
+    if (nx == (int64_t)x && ny == (int64_t)y) continue;
+    uint32_t l = (uint32_t)ny * F.W + (uint32_t)nx;
+    if (F.dpos_c[l].w <= 0.0f) continue;
+    if (num < CT_MAX_NEIGHBORS) idx[num++] = l;
+  }
+  if (num == 0) return;
+  Res mres = res_init();
+  float m_c = 1.0f;
+  float M_s = r.M;
+  const float fk = (float)num;
+  for (uint32_t i = 0; i < num; ++i) {
+    Res in = ld_res(F.rA_i, F.rB_i, F.rC_i, idx[i]);
+    DI nd = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, idx[i]);
+    Hit nh = hit_from_di(nd);
+    F3 nray_d = normalize(sub(nh.P, nd.org));
+    Mat nm = ld_mat(F, nd.mat);
+    F3 curr_target = f3(0, 0, 0);
+    float m_i = 0.0f;
+    if (res_valid(in)) {
+      curr_target = res_radiance(F, in, ray_d, hit, mat, v);
+      const float tl = luminance(curr_target);
+      const float pii = luminance(in.F);
+      float mm = in.M * pii;
+      float den = mm + (r.M / fk) * tl;
+      m_i = den > 0.0f ? mm / den : 0.0f;
+    }
+    F3 input_target = f3(0, 0, 0);
+    if (res_valid(r)) input_target = res_radiance(F, r, nray_d, nh, nm, v);
+    {
+      const float pic = luminance(input_target);
+      const float pcc = luminance(r.F);
+      const float numer = in.M * pic;
+      const bool den_ok = (pcc + numer) > 0.0f;
+      m_c += 1.0f;
+      m_c -= den_ok ? numer / (numer + (r.M / fk) * pcc) : 0.0f;
+    }
+    if (res_valid(in)) {
+      const float w_i = luminance(curr_target) * in.W_y * m_i;
+      update_reservoir(mres, in.Y, in.type, in.seed, curr_target, w_i, 1.0f, seed);
+    }
+    M_s += in.M;
+  }
+  uint32_t s2 = seed;
+  const float w_c = luminance(r.F) * r.W_y * m_c;
+  update_reservoir(mres, r.Y, r.type, r.seed, r.F, w_c, 1.0f, s2);
+  mres.M = M_s;
+  const float tl = luminance(mres.F);
+  mres.W_y = tl > 0.0f ? mres.W_sum / (tl * (float)(1u + num)) : 0.0f;
+  r = mres;
+}
+
+// Camera ray of pixel (x, y): get_ray_from_current_pixel, rgen.glsl:18-42
+// (jitter 0).
+CT_D void camera_ray(const DFrame &F, uint32_t x, uint32_t y, F3 &o, F3 &d) {
+  float pcx = ((float)x + 0.0f) + 0.5f, pcy = ((float)y + 0.0f) + 0.5f;
+  float ux = pcx / (float)F.W, uy = pcy / (float)F.H;
+  float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
+  float ov[4], t4[4], d4[4];
+  mat4_vec(F.inv_view, 0.0f, 0.0f, 0.0f, 1.0f, ov);
+  mat4_vec(F.inv_proj, dx, dy, 1.0f, 1.0f, t4);
+  F3 nt = normalize(f3(t4[0], t4[1], t4[2]));
+  mat4_vec(F.inv_view, nt.x, nt.y, nt.z, 0.0f, d4);
+  o = f3(ov[0], ov[1], ov[2]);
+  d = f3(d4[0], d4[1], d4[2]);
+}
+
+// ----------------------------------------------------------- queue push
+CT_D void queue_flush(const DFrame &F, uint32_t px, uint32_t emitted, uint32_t nslots, uint32_t counter) {
+  const uint32_t lane = __lane_id();
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  for (uint32_t k = 0; k < nslots; ++k) {
+    bool pred = (emitted >> k) & 1u;
+    unsigned long long m = __ballot(pred);
+    if (m == 0ull) continue;
+    int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) {
+      base = atomicAdd(&F.counters[kCntQueue], (uint32_t)__popcll(m));
+      atomicAdd(&F.counters[counter], (uint32_t)__popcll(m));
+    }
+    base = __shfl(base, leader);
+    if (pred) F.queue[base + (uint32_t)__popcll(m & lt)] = px * kRaySlots + k;
+  }
+}
+
+// ---------------------------------------------------------------- kernels
+// Primary rays: raygen rgen.glsl:18-42 + closest hit; one 8x8 tile per wave.
+template <bool kCount>
+__global__ void __launch_bounds__(kBlock) k_primary(DFrame F, unsigned long long *stats) {
+  __shared__ uint32_t stk[kStackSize * kBlock];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const uint32_t tiles_x = (F.W + 15) / 16;
+  const uint32_t bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
+  const uint32_t x = bx * 16 + (w & 1u) * 8 + (lane & 7u);
+  const uint32_t y = by * 16 + (w >> 1) * 8 + (lane >> 3);
+  TraceCnt cnt;
+  if (x < F.W && y < F.H) {
+    F3 o, d;
+    camera_ray(F, x, y, o, d);
+    float bt = kMaxDistance;
+    uint32_t bi = kInvalid, bp = kInvalid;
+    bool ovf = false;
+    traverse<false, kCount>(F.S, o, d, kMaxDistance, kInvalid, kInvalid, bt, bi, bp, stk + tid, kBlock, cnt, ovf);
+    F.hit[y * F.W + x] = make_float4(bi != kInvalid ? bt : -1.0f, __uint_as_float(bi), __uint_as_float(bp), 0.0f);
+  }
+  if (kCount) {
+    unsigned long long a = cnt.nodes, b = cnt.leaves, c = cnt.insts;
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_down(a, off);
+      b += __shfl_down(b, off);
+      c += __shfl_down(c, off);
+    }
+    if (lane == 0) {
+      atomicAdd(&stats[0], a);
+      atomicAdd(&stats[1], b);
+      atomicAdd(&stats[2], c);
+    }
+  }
+}
+
+// Prepass after the primary hit: rchit (rchit.glsl:18-54), miss
+// (rmiss.glsl:19-29), motion vector, temporal gather, confidence and RIS.
+// EMIT for the RIS visibility ray.
+__global__ void __launch_bounds__(kBlock) k_ris(DFrame F) {
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t emitted = 0;
+  if (px < F.npix) {
+    const uint32_t x = px % F.W, y = px / F.W;
+    float4 h = F.hit[px];
+    uint32_t hi = __float_as_uint(h.y), hp = __float_as_uint(h.z);
+    // The camera ray again: raygen is cheap and the traversal kernel keeps
+    // no per-pixel state besides the hit.
+    F3 ro, rd;
+    camera_ray(F, x, y, ro, rd);
+    DI di;
+    di.seed = tea(y * F.W + x, F.frame);
+    di.org = ro;
+    di.conf = 1.0f;
+    F3 hv = f3(1.0f, 1.0f, 1.0f);
+    di.pos = f3(0, 0, 0);
+    di.nrm = f3(0, 0, 0);
+    di.distance = -1.0f;
+    di.inst = kMaxInstances;
+    di.prim = kMaxPrimitives;
+    di.mat = kInvalid;
+    if (hi != kInvalid) {
+      const DInst &I = F.S.inst[hi];
+      di.pos = add(ro, muls(rd, h.x));
+      uint32_t gp = I.first + hp;
+      const ct_aabb &bb = F.aabbs[gp];
+      F3 c = muls(add(f3(bb.minimum[0], bb.minimum[1], bb.minimum[2]), f3(bb.maximum[0], bb.maximum[1], bb.maximum[2])), 0.5f);
+      float m[12];
+#pragma unroll
+      for (int k = 0; k < 12; ++k) m[k] = I.model[k];
+      c = xform3(m, c, 1.0f);
+      di.nrm = cube_like_normal(normalize(sub(di.pos, c)));
+      di.distance = length(sub(di.pos, ro));
+      di.mat = F.prims[gp].material_index;
+      di.inst = hi;
+      di.prim = hp;
+    } else {
+      hv = mulv(hv, sky_color(F.time, F.afternoon != 0, rd));
+    }
+    if (!(di.distance > 0.0f)) {
+      F.img[px] = make_float4(hv.x, hv.y, hv.z, 1.0f);
+      if (F.active & CT_TAA_BIT) F.taa[px] = make_float4(hv.x, hv.y, hv.z, 1.0f);
+    } else {
+      // motion_vectors.glsl:70-89
+      const float *pt = F.pt + 16 * (size_t)di.inst;
+      float xi_x = (float)x + 0.5f, xi_y = (float)y + 0.5f;
+      float ps[4];
+      mat4_vec(pt, di.pos.x, di.pos.y, di.pos.z, 1.0f, ps);
+      float x1 = ps[0] / ps[3], y1 = ps[1] / ps[3];
+      x1 = 0.5f * x1 + 0.5f;
+      y1 = 0.5f * y1 + 0.5f;
+      float mvx = x1 * (float)F.W - xi_x, mvy = y1 * (float)F.H - xi_y;
+      F.vel[px] = make_float2(mvx, mvy);
+      Mat mat = ld_mat(F, di.mat);
+      uint32_t pxx = f2u_sat(xi_x + mvx), pyy = f2u_sat(xi_y + mvy);
+      Hit hit = hit_from_di(di);
+      Res prev = res_init();
+      uint32_t pidx = pxx + pyy * F.W;
+      if (pidx < F.W * F.H) prev = ld_res(F.rA_p, F.rB_p, F.rC_p, pidx);
+      float predicted = clampf(prev.M / F.rp.max_influence_from_the_past, 0.0f, 1.0f);
+      float confidence = (prev.W_y > 0.0f) ? predicted : 0.0f;
+      di.conf = confidence;
+      Res r = ris(F, confidence, rd, hit, mat, di.seed);
+      Vis v{true, px, 0u, 0u, 0u};
+      Res rv = r;
+      visibility_pass(F, rv, hit, v);
+      emitted = v.emitted;
+      st_res(F.rA_i, F.rB_i, F.rC_i, px, r);
+      F.pidx[px] = pidx;
+    }
+    st_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px, di);
+    F.mask[px] = 0u;
+  }
+  queue_flush(F, px, emitted, 1u, kCntRis);
+}
+
+// RIS visibility result (FINISH) and the temporal reuse rays (EMIT).
+__global__ void __launch_bounds__(kBlock) k_temporal_emit(DFrame F) {
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t emitted = 0;
+  if (px < F.npix) {
+    float4 pd = F.dpos_c[px];
+    if (pd.w > 0.0f) {
+      DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
+      Hit hit = hit_from_di(di);
+      Res r = ld_res(F.rA_i, F.rB_i, F.rC_i, px);
+      Vis v{false, px, 0u, F.mask[px], 0u};
+      visibility_pass(F, r, hit, v);
+      st_res(F.rA_i, F.rB_i, F.rC_i, px, r);
+      uint32_t pidx = F.pidx[px];
+      Res prev = res_init();
+      if (pidx < F.W * F.H) prev = ld_res(F.rA_p, F.rB_p, F.rC_p, pidx);
+      if (prev.W_y > 0.0f) {
+        // TEMPORAL_REUSE takes the prepass camera ray (rgen.glsl:261-262).
+        F3 ro, rd;
+        camera_ray(F, px % F.W, px / F.W, ro, rd);
+        Vis ve{true, px, 0u, 0u, 0u};
+        Mat mat = ld_mat(F, di.mat);
+        uint32_t seed = di.seed;
+        temporal_reuse(F, r, prev, pidx, rd, hit, mat, seed, ve);
+        emitted = ve.emitted;
+      }
+    }
+    F.mask[px] = 0u;
+  }
+  queue_flush(F, px, emitted, 2u, kCntTemporal);
+}
+
+__global__ void __launch_bounds__(kBlock) k_temporal_finish(DFrame F) {
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  if (px >= F.npix) return;
+  float4 pd = F.dpos_c[px];
+  if (!(pd.w > 0.0f)) return;
+  uint32_t pidx = F.pidx[px];
+  Res prev = res_init();
+  if (pidx < F.W * F.H) prev = ld_res(F.rA_p, F.rB_p, F.rC_p, pidx);
+  if (!(prev.W_y > 0.0f)) return;
+  DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
+  Hit hit = hit_from_di(di);
+  Res r = ld_res(F.rA_i, F.rB_i, F.rC_i, px);
+  Mat mat = ld_mat(F, di.mat);
+  F3 ro, rd;
+  camera_ray(F, px % F.W, px / F.W, ro, rd);
+  Vis v{false, px, 0u, F.mask[px], 0u};
+  uint32_t seed = di.seed;
+  temporal_reuse(F, r, prev, pidx, rd, hit, mat, seed, v);
+  st_res(F.rA_i, F.rB_i, F.rC_i, px, r);
+  F.dorg_c[px].w = __uint_as_float(seed);
+}
+
+// Shading pass: spatial reuse (EMIT / FINISH) then shade and store.
+template <bool kEmit>
+__global__ void __launch_bounds__(kBlock) k_spatial(DFrame F) {
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t emitted = 0;
+  if (px < F.npix) {
+    const uint32_t x = px % F.W, y = px / F.W;
+    float4 pd = F.dpos_c[px];
+    if (pd.w > 0.0f) {
+      DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
+      F3 ray_d = normalize(sub(di.pos, di.org));
+      Mat mat = ld_mat(F, di.mat);
+      Hit hit = hit_from_di(di);
+      Res sr = ld_res(F.rA_i, F.rB_i, F.rC_i, px);
+      Vis v{kEmit, px, 0u, kEmit ? 0u : F.mask[px], 0u};
+      uint32_t seed = di.seed;
+      spatial_reuse(F, sr, di.conf, x, y, ray_d, hit, mat, seed, v);
+      emitted = v.emitted;
+      if (!kEmit) {
+        F3 hv = f3(0, 0, 0);
+        hv = add(hv, muls(sr.F, sr.W_y));
+        if (F.rp.direct_emittance) hv = add(hv, mat.emission);
+        if (isinf(hv.x) || isnan(hv.x)) hv.x = 0.0f;
+        if (isinf(hv.y) || isnan(hv.y)) hv.y = 0.0f;
+        if (isinf(hv.z) || isnan(hv.z)) hv.z = 0.0f;
+        float4 fp = make_float4(hv.x, hv.y, hv.z, 1.0f);
+        F.img[px] = fp;
+        if (F.active & CT_TAA_BIT) F.taa[px] = fp;
+        st_res(F.rA_p, F.rB_p, F.rC_p, px, sr);
+        di.seed = seed;
+        st_di(F.dpos_p, F.dnrm_p, F.dorg_p, F.dhit_p, px, di);
+      }
+    } else if (!kEmit) {
+      reinterpret_cast<float *>(&F.dpos_p[px])[3] = 0.0f;
+    }
+    if (kEmit) F.mask[px] = 0u;
+  }
+  if (kEmit) queue_flush(F, px, emitted, 2u * F.rp.max_neighbors, kCntSpatial);
+}
+
+// taa.glsl:16-108 over the dispatch region.
+__global__ void __launch_bounds__(kBlock) k_taa(DFrame F, uint32_t tw, uint32_t th) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= tw * th) return;
+  const uint32_t x = t % tw, y = t / tw;
+  if (x >= F.W || y >= F.H) return;
+  const uint32_t W = F.W, H = F.H, idx = x + y * W;
+  float4 pd = F.dpos_c[idx];
+  if (!(pd.w > 0.0f)) return;
+  float4 pn = F.dnrm_c[idx];
+  F3 dn = xyz(pn);
+  uint32_t dmat = __float_as_uint(pn.w);
+  float dist = pd.w;
+  float b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, b3 = 0.0f;
+  uint32_t count = 0;
+  for (int i = -1; i <= 1; ++i)
+    for (int j = -1; j <= 1; ++j) {
+      int64_t nx = (int64_t)x + i, ny = (int64_t)y + j;
+      if (nx < 0 || nx >= (int64_t)W || ny < 0 || ny >= (int64_t)H) continue;
+      uint32_t ni = (uint32_t)nx + (uint32_t)ny * W;
+      float4 nn = F.dnrm_c[ni];
+      float nd = F.dpos_c[ni].w;
+      if (dist <= 0.0f || dmat != __float_as_uint(nn.w) || (dot(dn, xyz(nn)) < 0.906f) ||
+          (fabsf(dist - nd) > 0.1f * dist))
+        continue;
+      ++count;
+      float4 tv = F.taa[ni];
+      b0 = b0 + tv.x; b1 = b1 + tv.y; b2 = b2 + tv.z; b3 = b3 + tv.w;
+    }
+  float fc = (float)count;
+  b0 = b0 / fc; b1 = b1 / fc; b2 = b2 / fc; b3 = b3 / fc;
+  float2 vv = F.vel[idx];
+  uint32_t s = __float_as_uint(F.dorg_c[idx].w);
+  float jx = rnd(s);
+  float jy = rnd(s);
+  float x1 = ((float)x + vv.x) + jx, y1 = ((float)y + vv.y) + jy;
+  int32_t ix = f2i_sat(x1), iy = f2i_sat(y1);
+  if (ix >= 0 && ix < (int32_t)W && iy >= 0 && iy < (int32_t)H) {
+    uint32_t pi = (uint32_t)ix + (uint32_t)iy * W;
+    float4 pp = F.img_prev[pi];
+    float4 pnn = F.dnrm_p[pi];
+    bool valid = dot(xyz(pnn), dn) >= 0.99f && __float_as_uint(pnn.w) == dmat;
+    if (valid) {
+      const float a = F.rp.taa_alpha, oma = 1.0f - F.rp.taa_alpha;
+      b0 = a * b0 + oma * pp.x; b1 = a * b1 + oma * pp.y; b2 = a * b2 + oma * pp.z; b3 = a * b3 + oma * pp.w;
+    }
+  }
+  F.img[idx] = make_float4(b0, b1, b2, b3);
+}
+
+__global__ void k_reset_counters(uint32_t *c, int first, int n) {
+  int i = threadIdx.x;
+  if (i < n) c[first + i] = 0u;
+}
+
+// Any-hit traversal over a ray queue; each wave fetches 64 rays at a time.
+template <bool kCount>
+__global__ void __launch_bounds__(kBlock) k_trace_any(DScene S, TraceAnyArgs A) {
+  __shared__ uint32_t stk[kStackSize * kBlock];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
+  uint32_t count = *A.count;
+  if (count > A.max_rays) count = A.max_rays;
+  TraceCnt cnt;
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(A.head, 64u);
+    base = __shfl(base, 0);
+    if (base >= count) break;
+    uint32_t q = base + lane;
+    if (q < count) {
+      uint32_t idx = A.queue ? A.queue[q] : q;
+      float4 o = A.ray_o[idx], d = A.ray_d[idx];
+      uint32_t tp = A.ray_t[idx];
+      float bt = 0.0f;
+      uint32_t bi = 0, bp = 0;
+      bool ovf = false;
+      bool occ = traverse<true, kCount>(S, f3(o.x, o.y, o.z), f3(d.x, d.y, d.z), o.w, __float_as_uint(d.w), tp, bt, bi,
+                                        bp, stk + tid, kBlock, cnt, ovf);
+      if (occ && A.mask) atomicOr(&A.mask[idx / kRaySlots], 1u << (idx % kRaySlots));
+      if (A.occ) A.occ[idx] = occ ? 1 : 0;
+    }
+  }
+  if (kCount) {
+    unsigned long long a = cnt.nodes, b = cnt.leaves, c = cnt.insts;
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_down(a, off);
+      b += __shfl_down(b, off);
+      c += __shfl_down(c, off);
+    }
+    if (lane == 0) {
+      atomicAdd(&A.stats[0], a);
+      atomicAdd(&A.stats[1], b);
+      atomicAdd(&A.stats[2], c);
+    }
+  }
+}
+
+template <bool kCount>
+__global__ void __launch_bounds__(kBlock) k_trace_closest(DScene S, TraceClosestArgs A) {
+  __shared__ uint32_t stk[kStackSize * kBlock];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
+  const uint32_t i = blockIdx.x * blockDim.x + tid;
+  TraceCnt cnt;
+  if (i < A.n) {
+    float4 o = A.ray_o[i], d = A.ray_d[i];
+    float bt = A.tmax;
+    uint32_t bi = kInvalid, bp = kInvalid;
+    bool ovf = false;
+    traverse<false, kCount>(S, f3(o.x, o.y, o.z), f3(d.x, d.y, d.z), A.tmax, kInvalid, kInvalid, bt, bi, bp, stk + tid,
+                            kBlock, cnt, ovf);
+    A.hit[i] = make_float4(bi != kInvalid ? bt : -1.0f, __uint_as_float(bi), __uint_as_float(bp), 0.0f);
+  }
+  if (kCount) {
+    unsigned long long a = cnt.nodes, b = cnt.leaves, c = cnt.insts;
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_down(a, off);
+      b += __shfl_down(b, off);
+      c += __shfl_down(c, off);
+    }
+    if (lane == 0) {
+      atomicAdd(&A.stats[0], a);
+      atomicAdd(&A.stats[1], b);
+      atomicAdd(&A.stats[2], c);
+    }
+  }
+}
+
+inline uint32_t nblocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+int trace_grid() {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
+  }
+  return cus * 8;
+}
+
+}  // namespace
+
+int launch_trace_any(const DScene &s, const TraceAnyArgs &a, hipStream_t st) {
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, a.head, 0, 1);
+  uint32_t grid = std::min<uint32_t>(std::max<uint32_t>(nblocks(a.max_rays), 1), (uint32_t)trace_grid());
+  if (a.stats)
+    hipLaunchKernelGGL(k_trace_any<true>, dim3(grid), dim3(kBlock), 0, st, s, a);
+  else
+    hipLaunchKernelGGL(k_trace_any<false>, dim3(grid), dim3(kBlock), 0, st, s, a);
+  CT_HIP(hipGetLastError());
+  return CT_OK;
+}
+
+int launch_trace_closest(const DScene &s, const TraceClosestArgs &a, hipStream_t st) {
+  if (a.n == 0) return CT_OK;
+  if (a.stats)
+    hipLaunchKernelGGL(k_trace_closest<true>, dim3(nblocks(a.n)), dim3(kBlock), 0, st, s, a);
+  else
+    hipLaunchKernelGGL(k_trace_closest<false>, dim3(nblocks(a.n)), dim3(kBlock), 0, st, s, a);
+  CT_HIP(hipGetLastError());
+  return CT_OK;
+}
+
+}  // namespace ct
+
+using namespace ct;
+
+namespace {
+
+int alloc_frame(Ctx &c) {
+  FrameBufs &b = c.fb;
+  const size_t np = (size_t)c.max_w * c.max_h;
+  int rc;
+  for (int k = 0; k < 2; ++k) {
+    if ((rc = b.di_pos[k].reserve(16 * np)) || (rc = b.di_nrm[k].reserve(16 * np)) || (rc = b.di_org[k].reserve(16 * np)) ||
+        (rc = b.di_hit[k].reserve(16 * np)) || (rc = b.resA[k].reserve(16 * np)) || (rc = b.resB[k].reserve(16 * np)) ||
+        (rc = b.resC[k].reserve(4 * np)) || (rc = b.img[k].reserve(16 * np)))
+      return rc;
+  }
+  if ((rc = b.velocity.reserve(8 * np)) || (rc = b.taa.reserve(16 * np)) || (rc = b.hit.reserve(16 * np)) ||
+      (rc = b.pidx.reserve(4 * np)) || (rc = b.mask.reserve(4 * np)) || (rc = b.ray_o.reserve(16 * np * kRaySlots)) ||
+      (rc = b.ray_d.reserve(16 * np * kRaySlots)) || (rc = b.ray_t.reserve(4 * np * kRaySlots)) ||
+      (rc = b.queue.reserve(4 * np * kRaySlots)) || (rc = b.counters.reserve(64 * sizeof(uint32_t))) ||
+      (rc = b.prev_res_copy.reserve(64 * sizeof(unsigned long long))))
+    return rc;
+  return CT_OK;
+}
+
+int zero_frame(Ctx &c, hipStream_t st) {
+  FrameBufs &b = c.fb;
+  DevBuf *bufs[] = {&b.di_pos[0], &b.di_pos[1], &b.di_nrm[0], &b.di_nrm[1], &b.di_org[0], &b.di_org[1],
+                    &b.di_hit[0], &b.di_hit[1], &b.resA[0],   &b.resA[1],   &b.resB[0],   &b.resB[1],
+                    &b.resC[0],   &b.resC[1],   &b.img[0],    &b.img[1],    &b.velocity,  &b.taa,
+                    &b.hit,       &b.pidx,      &b.mask,      &b.counters};
+  for (DevBuf *d : bufs) CT_HIP(hipMemsetAsync(d->p, 0, d->bytes, st));
+  CT_HIP(hipStreamSynchronize(st));
+  return CT_OK;
+}
+
+void pack_lights(const ct_light *L, uint32_t n, std::vector<float4> &out) {
+  out.resize(3 * (size_t)std::max<uint32_t>(n, 1));
+  for (uint32_t i = 0; i < n; ++i) {
+    const ct_light &l = L[i];
+    float tb, ib, pb;
+    std::memcpy(&tb, &l.type, 4);
+    std::memcpy(&ib, &l.instance_info.instance_id, 4);
+    std::memcpy(&pb, &l.instance_info.primitive_id, 4);
+    out[3 * i] = make_float4(l.position[0], l.position[1], l.position[2], l.size);
+    out[3 * i + 1] = make_float4(l.emissive[0], l.emissive[1], l.emissive[2], tb);
+    out[3 * i + 2] = make_float4(ib, pb, 0.0f, 0.0f);
+  }
+}
+
+int upload_lights(Ctx &c, DevBuf &dst, const std::vector<ct_light> &src) {
+  std::vector<float4> pk;
+  pack_lights(src.data(), (uint32_t)src.size(), pk);
+  int rc;
+  if ((rc = dst.reserve(16 * pk.size()))) return rc;
+  CT_HIP(hipMemcpy(dst.p, pk.data(), 16 * pk.size(), hipMemcpyHostToDevice));
+  (void)c;
+  return CT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ct_ctx_create(const ct_ctx_desc *d, ct_ctx **out) {
+  if (!d || !out) return fail(CT_E_INVALID, "ct_ctx_create: null argument");
+  *out = nullptr;
+  if (d->max_width == 0 || d->max_height == 0 || (uint64_t)d->max_width * d->max_height * kRaySlots >= (1ull << 32))
+    return fail(CT_E_INVALID, "ct_ctx_create: bad size");
+  if (device_count_gfx950() == 0) return fail(CT_E_NODEVICE, "ct_ctx_create: no gfx950 device");
+  CT_HIP(hipSetDevice(d->device));
+  ct_ctx *c = new ct_ctx();
+  c->device = d->device;
+  c->max_w = d->max_width;
+  c->max_h = d->max_height;
+  int rc;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(CT_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  if ((rc = alloc_frame(*c)) || (rc = zero_frame(*c, c->stream))) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
+  for (int k = 0; k < 16; ++k) (void)hipEventCreate(&c->ev[k]);
+  void *pin = nullptr;
+  if (hipHostMalloc(&pin, 64 * sizeof(uint64_t), hipHostMallocDefault) == hipSuccess)
+    c->counters_pinned = static_cast<uint64_t *>(pin);
+  std::memset(&c->cfg, 0, sizeof(c->cfg));
+  *out = c;
+  return CT_OK;
+}
+
+int ct_ctx_destroy(ct_ctx *c) {
+  if (!c) return fail(CT_E_INVALID, "ct_ctx_destroy: null handle");
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (int k = 0; k < 16; ++k)
+    if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+  if (c->counters_pinned) (void)hipHostFree(c->counters_pinned);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CT_OK;
+}
+
+int ct_ctx_append_materials(ct_ctx *c, const ct_material *m, uint32_t count, uint32_t *base) {
+  if (!c || (!m && count)) return fail(CT_E_INVALID, "append_materials: null argument");
+  for (uint32_t i = 0; i < count; ++i)
+    if ((m[i].type & CT_MATERIAL_TYPE_MASK) != CT_MATERIAL_LAMBERTIAN)
+      return fail(CT_E_UNSUPPORTED, "only Lambertian materials are supported");
+  if (base) *base = (uint32_t)c->mats_host.size();
+  c->mats_host.insert(c->mats_host.end(), m, m + count);
+  std::vector<float4> pk(2 * std::max<size_t>(c->mats_host.size(), 1));
+  for (size_t i = 0; i < c->mats_host.size(); ++i) {
+    const ct_material &q = c->mats_host[i];
+    float tb;
+    std::memcpy(&tb, &q.type, 4);
+    pk[2 * i] = make_float4(q.diffuse[0], q.diffuse[1], q.diffuse[2], tb);
+    pk[2 * i + 1] = make_float4(q.emission[0], q.emission[1], q.emission[2], 0.0f);
+  }
+  CT_HIP(hipSetDevice(c->device));
+  int rc;
+  if ((rc = c->materials.reserve(16 * pk.size()))) return rc;
+  CT_HIP(hipMemcpy(c->materials.p, pk.data(), 16 * pk.size(), hipMemcpyHostToDevice));
+  return CT_OK;
+}
+
+int ct_ctx_set_materials(ct_ctx *c, const ct_material *m, uint32_t count) {
+  if (!c) return fail(CT_E_INVALID, "set_materials: null handle");
+  c->mats_host.clear();
+  return ct_ctx_append_materials(c, m, count, nullptr);
+}
+
+int ct_ctx_set_point_lights(ct_ctx *c, const ct_light *l, uint32_t n) {
+  if (!c || (!l && n)) return fail(CT_E_INVALID, "set_point_lights: null argument");
+  c->point_host.assign(l, l + n);
+  CT_HIP(hipSetDevice(c->device));
+  return upload_lights(*c, c->point_lights, c->point_host);
+}
+
+int ct_ctx_set_env_lights(ct_ctx *c, const ct_light *l, uint32_t n) {
+  if (!c || (!l && n)) return fail(CT_E_INVALID, "set_env_lights: null argument");
+  c->env_host.assign(l, l + n);
+  CT_HIP(hipSetDevice(c->device));
+  return upload_lights(*c, c->env_lights, c->env_host);
+}
+
+int ct_ctx_set_light_config(ct_ctx *c, const ct_light_config *cfg) {
+  if (!c || !cfg) return fail(CT_E_INVALID, "set_light_config: null argument");
+  c->cfg = *cfg;
+  return CT_OK;
+}
+
+int ct_ctx_get_light_config(ct_ctx *c, ct_light_config *out) {
+  if (!c || !out) return fail(CT_E_INVALID, "get_light_config: null argument");
+  *out = c->cfg;
+  return CT_OK;
+}
+
+int ct_ctx_reset(ct_ctx *c, void *stream) {
+  if (!c) return fail(CT_E_INVALID, "ct_ctx_reset: null handle");
+  CT_HIP(hipSetDevice(c->device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  c->cur = 1;
+  return zero_frame(*c, st);
+}
+
+int ct_ctx_set_instrumentation(ct_ctx *c, uint32_t flags) {
+  if (!c) return fail(CT_E_INVALID, "set_instrumentation: null handle");
+  c->instrument = flags;
+  return CT_OK;
+}
+
+int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stream) {
+  if (!c || !a || !p) return fail(CT_E_INVALID, "ct_frame_render: null argument");
+  if (p->width == 0 || p->height == 0 || p->width > c->max_w || p->height > c->max_h)
+    return fail(CT_E_INVALID, "ct_frame_render: resolution exceeds the context");
+  if (p->is_active & CT_RIS_BRDF_BIT) return fail(CT_E_UNSUPPORTED, "ct_frame_render: BRDF sampling in RIS is not supported");
+  if (p->restir.max_neighbors > CT_MAX_NEIGHBORS || p->restir.min_neighbors > CT_MAX_NEIGHBORS)
+    return fail(CT_E_INVALID, "ct_frame_render: neighbour count above CT_MAX_NEIGHBORS");
+  if ((p->is_active & CT_TAA_BIT) && !p->has_previous)
+    return fail(CT_E_INVALID, "ct_frame_render: TAA needs a previous frame");
+  if (a->set.tlas_count == 0) return fail(CT_E_STATE, "ct_frame_render: empty scene (call update_scene first)");
+  CT_HIP(hipSetDevice(c->device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  // Cube lights from the manager (packed), re-uploaded on a new build.
+  static thread_local std::vector<float4> pk;
+  int rc;
+  const uint32_t ncube = a->set.cube_light_count;
+  pack_lights(a->cube_lights_host.data(), ncube, pk);
+  DevBuf &cube = c->inst_pt;  // reused: [packed cube lights | per-instance matrices]
+  const size_t ni = a->instances.size();
+  const size_t cube_bytes = 16 * pk.size(), pt_bytes = 64 * std::max<size_t>(ni, 1);
+  if ((rc = cube.reserve(cube_bytes + pt_bytes))) return rc;
+  // per-instance prev_view_proj * (prev_model * inverse(model)); the scene
+  // is static between builds so prev_model == model.
+  std::vector<float> ptm(16 * std::max<size_t>(ni, 1), 0.0f);
+  for (size_t i = 0; i < ni; ++i) {
+    float g[16];
+    const float *t = a->instances[i].transform, *w = &a->w2o[16 * i];
+    for (int cc = 0; cc < 4; ++cc)
+      for (int r = 0; r < 4; ++r)
+        g[cc * 4 + r] = t[0 * 4 + r] * w[cc * 4 + 0] + t[1 * 4 + r] * w[cc * 4 + 1] + t[2 * 4 + r] * w[cc * 4 + 2] +
+                        t[3 * 4 + r] * w[cc * 4 + 3];
+    const float *pv = p->prev_view_proj;
+    for (int cc = 0; cc < 4; ++cc)
+      for (int r = 0; r < 4; ++r)
+        ptm[16 * i + cc * 4 + r] = pv[0 * 4 + r] * g[cc * 4 + 0] + pv[1 * 4 + r] * g[cc * 4 + 1] +
+                                   pv[2 * 4 + r] * g[cc * 4 + 2] + pv[3 * 4 + r] * g[cc * 4 + 3];
+  }
+  CT_HIP(hipMemcpyAsync(cube.p, pk.data(), cube_bytes, hipMemcpyHostToDevice, st));
+  CT_HIP(hipMemcpyAsync(static_cast<char *>(cube.p) + cube_bytes, ptm.data(), 16 * sizeof(float) * ni,
+                        hipMemcpyHostToDevice, st));
+  c->cur ^= 1;
+  FrameBufs &b = c->fb;
+  DFrame F;
+  std::memset(&F, 0, sizeof(F));
+  F.W = p->width; F.H = p->height; F.npix = p->width * p->height;
+  F.frame = p->frame_number; F.active = p->is_active; F.afternoon = p->is_afternoon; F.time = p->time;
+  F.n_cube = ncube; F.n_point = (uint32_t)c->point_host.size(); F.n_env = (uint32_t)c->env_host.size();
+  F.n_mats = (uint32_t)c->mats_host.size();
+  std::memcpy(F.inv_view, p->inv_view, sizeof(F.inv_view));
+  std::memcpy(F.inv_proj, p->inv_proj, sizeof(F.inv_proj));
+  F.rp = p->restir;
+  F.cfg = c->cfg;
+  F.S = scene_view(*a);
+  F.aabbs = a->set.aabbs.as<ct_aabb>();
+  F.prims = a->set.prims.as<ct_primitive>();
+  F.mats = c->materials.as<float4>();
+  F.cube = cube.as<float4>();
+  F.point = c->point_lights.as<float4>();
+  F.env = c->env_lights.as<float4>();
+  F.pt = reinterpret_cast<const float *>(static_cast<char *>(cube.p) + cube_bytes);
+  F.dpos_c = b.di_pos[0].as<float4>(); F.dnrm_c = b.di_nrm[0].as<float4>(); F.dorg_c = b.di_org[0].as<float4>();
+  F.dhit_c = b.di_hit[0].as<uint4>();
+  F.dpos_p = b.di_pos[1].as<float4>(); F.dnrm_p = b.di_nrm[1].as<float4>(); F.dorg_p = b.di_org[1].as<float4>();
+  F.dhit_p = b.di_hit[1].as<uint4>();
+  F.rA_i = b.resA[0].as<uint4>(); F.rB_i = b.resB[0].as<float4>(); F.rC_i = b.resC[0].as<float>();
+  F.rA_p = b.resA[1].as<uint4>(); F.rB_p = b.resB[1].as<float4>(); F.rC_p = b.resC[1].as<float>();
+  F.vel = b.velocity.as<float2>();
+  F.img = b.img[c->cur].as<float4>();
+  F.img_prev = b.img[c->cur ^ 1].as<float4>();
+  F.taa = b.taa.as<float4>();
+  F.hit = b.hit.as<float4>();
+  F.pidx = b.pidx.as<uint32_t>();
+  F.mask = b.mask.as<uint32_t>();
+  F.ray_o = b.ray_o.as<float4>(); F.ray_d = b.ray_d.as<float4>(); F.ray_t = b.ray_t.as<uint32_t>();
+  F.queue = b.queue.as<uint32_t>();
+  F.counters = b.counters.as<uint32_t>();
+  unsigned long long *stats = (c->instrument & 2u) ? b.prev_res_copy.as<unsigned long long>() : nullptr;
+  const bool ev = (c->instrument & 1u) != 0;
+  int ne = 0;
+  auto mark = [&]() -> int {
+    if (ev) CT_HIP(hipEventRecord(c->ev[ne++], st));
+    return CT_OK;
+  };
+  const uint32_t np = F.npix;
+  TraceAnyArgs ta;
+  std::memset(&ta, 0, sizeof(ta));
+  ta.ray_o = F.ray_o; ta.ray_d = F.ray_d; ta.ray_t = F.ray_t; ta.queue = F.queue;
+  ta.count = F.counters + kCntQueue; ta.head = F.counters + kCntHead; ta.mask = F.mask; ta.stats = stats;
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, 0, 8);
+  if (stats) CT_HIP(hipMemsetAsync(stats, 0, 6 * sizeof(unsigned long long), st));
+  if ((rc = mark())) return rc;  // 0
+  const uint32_t tiles = ((F.W + 15) / 16) * ((F.H + 15) / 16);
+  if (stats)
+    hipLaunchKernelGGL(k_primary<true>, dim3(tiles), dim3(kBlock), 0, st, F, stats + 3);
+  else
+    hipLaunchKernelGGL(k_primary<false>, dim3(tiles), dim3(kBlock), 0, st, F, stats);
+  if ((rc = mark())) return rc;  // 1
+  hipLaunchKernelGGL(k_ris, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
+  if ((rc = mark())) return rc;  // 2
+  ta.max_rays = np;
+  if ((rc = launch_trace_any(F.S, ta, st))) return rc;
+  if ((rc = mark())) return rc;  // 3
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, kCntQueue, 1);
+  hipLaunchKernelGGL(k_temporal_emit, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
+  if ((rc = mark())) return rc;  // 4
+  ta.max_rays = 2 * np;
+  if ((rc = launch_trace_any(F.S, ta, st))) return rc;
+  if ((rc = mark())) return rc;  // 5
+  hipLaunchKernelGGL(k_temporal_finish, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
+  if ((rc = mark())) return rc;  // 6
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, kCntQueue, 1);
+  hipLaunchKernelGGL(k_spatial<true>, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
+  if ((rc = mark())) return rc;  // 7
+  ta.max_rays = 2 * p->restir.max_neighbors * np;
+  if ((rc = launch_trace_any(F.S, ta, st))) return rc;
+  if ((rc = mark())) return rc;  // 8
+  hipLaunchKernelGGL(k_spatial<false>, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
+  if ((rc = mark())) return rc;  // 9
+  if (p->is_active & CT_TAA_BIT) {
+    uint32_t tw = p->restir.taa_truncate_dispatch ? (F.W / 8) * 8 : F.W;
+    uint32_t th = p->restir.taa_truncate_dispatch ? (F.H / 8) * 8 : F.H;
+    if (tw && th) hipLaunchKernelGGL(k_taa, dim3(nblocks((uint64_t)tw * th)), dim3(kBlock), 0, st, F, tw, th);
+  }
+  if ((rc = mark())) return rc;  // 10
+  CT_HIP(hipGetLastError());
+  c->n_events = ne;
+  // Counters for the statistics (copied once the frame is done).
+  CT_HIP(hipMemcpyAsync(c->counters_pinned, F.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  if (stats) CT_HIP(hipMemcpyAsync(c->counters_pinned + 8, stats, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  c->stats_pending = true;
+  c->stats = ct_frame_stats{};
+  c->stats.primary_rays = np;
+  c->stream_last = st;
+  c->last_w = F.W;
+  c->last_h = F.H;
+  return CT_OK;
+}
+
+int ct_frame_get_stats(ct_ctx *c, ct_frame_stats *out) {
+  if (!c || !out) return fail(CT_E_INVALID, "get_stats: null argument");
+  CT_HIP(hipSetDevice(c->device));
+  if (c->stats_pending) {
+    CT_HIP(hipStreamSynchronize(c->stream_last));
+    const uint32_t *cnt = reinterpret_cast<const uint32_t *>(c->counters_pinned);
+    c->stats.shadow_rays_ris = cnt[kCntRis];
+    c->stats.shadow_rays_temporal = cnt[kCntTemporal];
+    c->stats.shadow_rays_spatial = cnt[kCntSpatial];
+    c->stats.shadow_rays = (uint64_t)cnt[kCntRis] + cnt[kCntTemporal] + cnt[kCntSpatial];
+    if (c->instrument & 2u) {
+      const unsigned long long *s = reinterpret_cast<const unsigned long long *>(c->counters_pinned + 8);
+      c->stats.nodes = s[0];
+      c->stats.leaves = s[1];
+      c->stats.instances = s[2];
+      c->stats.closest_nodes = s[3];
+      c->stats.closest_leaves = s[4];
+      c->stats.closest_instances = s[5];
+    }
+    if ((c->instrument & 1u) && c->n_events >= 11) {
+      float ms[10];
+      for (int k = 0; k < 10; ++k) CT_HIP(hipEventElapsedTime(&ms[k], c->ev[k], c->ev[k + 1]));
+      c->stats.ms_primary = ms[0];
+      c->stats.ms_ris = ms[1];
+      c->stats.ms_temporal = ms[3] + ms[5];
+      c->stats.ms_spatial = ms[6] + ms[8];
+      c->stats.ms_taa = ms[9];
+      c->stats.ms_trace_any = ms[2] + ms[4] + ms[7];
+      c->stats.ms_trace_closest = ms[0];
+      float tot;
+      CT_HIP(hipEventElapsedTime(&tot, c->ev[0], c->ev[10]));
+      c->stats.ms_total = tot;
+    }
+    c->stats_pending = false;
+  }
+  *out = c->stats;
+  return CT_OK;
+}
+
+int ct_frame_read(ct_ctx *c, uint32_t which, void *dst, size_t bytes) {
+  if (!c || !dst) return fail(CT_E_INVALID, "ct_frame_read: null argument");
+  CT_HIP(hipSetDevice(c->device));
+  CT_HIP(hipDeviceSynchronize());
+  const size_t np = (size_t)c->last_w * c->last_h;
+  FrameBufs &b = c->fb;
+  auto need = [&](size_t s) -> bool { return bytes >= s; };
+  switch (which) {
+    case CT_BUF_IMAGE:
+    case CT_BUF_PREV_IMAGE:
+    case CT_BUF_TAA_FRAME: {
+      if (!need(16 * np)) return fail(CT_E_INVALID, "ct_frame_read: buffer too small");
+      const DevBuf &src = which == CT_BUF_TAA_FRAME ? b.taa : b.img[which == CT_BUF_IMAGE ? c->cur : c->cur ^ 1];
+      CT_HIP(hipMemcpy(dst, src.p, 16 * np, hipMemcpyDeviceToHost));
+      return CT_OK;
+    }
+    case CT_BUF_VELOCITY:
+      if (!need(8 * np)) return fail(CT_E_INVALID, "ct_frame_read: buffer too small");
+      CT_HIP(hipMemcpy(dst, b.velocity.p, 8 * np, hipMemcpyDeviceToHost));
+      return CT_OK;
+    case CT_BUF_DI:
+    case CT_BUF_PREV_DI: {
+      if (!need(sizeof(ct_di_info) * np)) return fail(CT_E_INVALID, "ct_frame_read: buffer too small");
+      int k = which == CT_BUF_DI ? 0 : 1;
+      std::vector<float4> P(np), N(np), O(np);
+      std::vector<uint4> H(np);
+      CT_HIP(hipMemcpy(P.data(), b.di_pos[k].p, 16 * np, hipMemcpyDeviceToHost));
+      CT_HIP(hipMemcpy(N.data(), b.di_nrm[k].p, 16 * np, hipMemcpyDeviceToHost));
+      CT_HIP(hipMemcpy(O.data(), b.di_org[k].p, 16 * np, hipMemcpyDeviceToHost));
+      CT_HIP(hipMemcpy(H.data(), b.di_hit[k].p, 16 * np, hipMemcpyDeviceToHost));
+      ct_di_info *o = static_cast<ct_di_info *>(dst);
+      for (size_t i = 0; i < np; ++i) {
+        o[i].position[0] = P[i].x; o[i].position[1] = P[i].y; o[i].position[2] = P[i].z; o[i].distance = P[i].w;
+        o[i].normal[0] = N[i].x; o[i].normal[1] = N[i].y; o[i].normal[2] = N[i].z;
+        std::memcpy(&o[i].mat_index, &N[i].w, 4);
+        o[i].ray_origin[0] = O[i].x; o[i].ray_origin[1] = O[i].y; o[i].ray_origin[2] = O[i].z;
+        std::memcpy(&o[i].seed, &O[i].w, 4);
+        o[i].instance_hit.instance_id = H[i].x;
+        o[i].instance_hit.primitive_id = H[i].y;
+        std::memcpy(&o[i].confidence, &H[i].z, 4);
+      }
+      return CT_OK;
+    }
+    case CT_BUF_INT_RESERVOIR:
+    case CT_BUF_PREV_RESERVOIR: {
+      if (!need(sizeof(ct_reservoir) * np)) return fail(CT_E_INVALID, "ct_frame_read: buffer too small");
+      int k = which == CT_BUF_INT_RESERVOIR ? 0 : 1;
+      std::vector<uint4> A(np);
+      std::vector<float4> B(np);
+      std::vector<float> C(np);
+      CT_HIP(hipMemcpy(A.data(), b.resA[k].p, 16 * np, hipMemcpyDeviceToHost));
+      CT_HIP(hipMemcpy(B.data(), b.resB[k].p, 16 * np, hipMemcpyDeviceToHost));
+      CT_HIP(hipMemcpy(C.data(), b.resC[k].p, 4 * np, hipMemcpyDeviceToHost));
+      ct_reservoir *o = static_cast<ct_reservoir *>(dst);
+      for (size_t i = 0; i < np; ++i) {
+        o[i].Y = A[i].x; o[i].l_type = A[i].y; o[i].seed = A[i].z;
+        std::memcpy(&o[i].W_y, &A[i].w, 4);
+        o[i].W_sum = B[i].x; o[i].M = B[i].y; o[i].F[0] = B[i].z; o[i].F[1] = B[i].w; o[i].F[2] = C[i];
+      }
+      return CT_OK;
+    }
+    default:
+      return fail(CT_E_INVALID, "ct_frame_read: unknown buffer id");
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+"""Workload definitions: scenes, cameras and per-frame parameters.
+
+``small_plain`` is the 64^3 / 256x256 workload the reference's own CPU path
+can run; ``hd_di`` is the 1920x1080 ReSTIR DI workload on a 256^3 grid in 64
+chunks.  ``reference_mode=True`` keeps the reference's constants
+(defines.glsl:34-62, taa.glsl:13, and camera.h:12 where fov = 45 is passed
+to glm::perspectiveRH_ZO as radians); the default uses the workload's own
+values (60 degree FOV, 32 RIS candidates, 5 neighbours at 30 px).
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Callable, Tuple
+
+import numpy as np
+
+__all__ = ["Workload", "small_plain", "hd_di", "frame_params", "scene_desc"]
+
+
+def _lib():
+    from . import lib
+    return lib()
+
+
+@dataclass
+class Workload:
+    name: str
+    grid: Tuple[int, int, int]
+    chunk: Tuple[int, int, int]
+    occupied: int
+    emissive: int
+    le: Tuple[float, float]
+    seed: int
+    width: int
+    height: int
+    frames: int
+    fov_radians: float
+    camera: Callable[[int], Tuple[tuple, tuple, tuple]]
+    features: int
+    taa: bool
+    restir: object = None
+    albedo: Tuple[float, float] = (0.2, 0.8)
+    time: float = 1.0          # Status.time as main.cpp:1289 sets it
+    afternoon: int = 1         # Status.is_afternoon, main.cpp:1290
+    extra: dict = field(default_factory=dict)
+
+
+def scene_desc(w: Workload):
+    from . import SceneDesc, CT_VOXEL_EXTENT
+    d = SceneDesc()
+    d.grid[:] = list(w.grid)
+    d.chunk[:] = list(w.chunk)
+    d.occupied_count = w.occupied
+    d.emissive_count = w.emissive
+    d.le_min, d.le_max = w.le
+    d.albedo_min, d.albedo_max = w.albedo
+    d.seed = w.seed
+    d.palette_size = 0
+    d.voxel_extent = CT_VOXEL_EXTENT
+    return d
+
+
+def _static_camera(pos, fwd, up):
+    return lambda f: (pos, fwd, up)
+
+
+def _orbit_camera(radius, height, deg_per_frame, start_deg=0.0):
+    def cam(f):
+        a = math.radians(start_deg + deg_per_frame * f)
+        pos = (radius * math.sin(a), height, radius * math.cos(a))
+        n = math.sqrt(pos[0] ** 2 + pos[1] ** 2 + pos[2] ** 2)
+        fwd = (-pos[0] / n, -pos[1] / n, -pos[2] / n)
+        return pos, fwd, (0.0, 1.0, 0.0)
+    return cam
+
+
+def small_plain(reference_mode=False, width=256, height=256, frames=4, taa=False):
+    """64^3 voxels, 12 % fill (31,457 voxels), 8 emissive voxels of radiance 10.
+
+    Camera at (0, 0, 2.5) looking down -z (camera.h:8-10), 1 spp, frames 0..3.
+    """
+    from . import CT_RIS_CUBE_LIGHT_BIT, restir_params_reference
+    rp = restir_params_reference()
+    if not reference_mode:
+        rp.min_ris_cube_samples = rp.max_ris_cube_samples = 32
+        rp.min_neighbors = rp.max_neighbors = 5
+        rp.min_neighbor_radius = rp.max_neighbor_radius = 30.0
+    return Workload(
+        name="small_plain" + ("_reference" if reference_mode else ""),
+        grid=(64, 64, 64), chunk=(64, 64, 64), occupied=round(0.12 * 64 ** 3), emissive=8,
+        le=(10.0, 10.0), seed=0xC0, width=width, height=height, frames=frames,
+        fov_radians=45.0 if reference_mode else math.radians(60.0),
+        camera=_static_camera((0.0, 0.0, 2.5), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0)),
+        features=CT_RIS_CUBE_LIGHT_BIT, taa=taa, restir=rp)
+
+
+def hd_di(frames=16, width=1920, height=1080, grid=256, chunk=64):
+    """256^3 voxels in 64 chunks of 64^3, 25 % fill (4,194,304 voxels), 2 %
+    emissive (83,886 lights, radiance U[1, 20]); ReSTIR DI at 1920x1080 with a
+    1 degree/frame camera orbit and TAA at 0.1 blend."""
+    from . import CT_RIS_CUBE_LIGHT_BIT, restir_params_reference
+    rp = restir_params_reference()
+    rp.taa_alpha = 0.1
+    occ = round(0.25 * grid ** 3)
+    extent = grid * 0.03125
+    return Workload(
+        name=f"hd_di_{grid}", grid=(grid, grid, grid), chunk=(chunk, chunk, chunk), occupied=occ,
+        emissive=round(0.02 * occ), le=(1.0, 20.0), seed=0xC1, width=width, height=height, frames=frames,
+        fov_radians=math.radians(60.0), camera=_orbit_camera(1.5 * extent, 0.375 * extent, 1.0),
+        features=CT_RIS_CUBE_LIGHT_BIT, taa=True, restir=rp)
+
+
+def frame_params(w: Workload, f: int):
+    """FrameParams of frame f: camera of f, previous camera of f-1 (the
+    reference copies the matrices into prev_* at the end of draw,
+    main.cpp:1660-1666), TAA only once a previous frame exists
+    (main.cpp:1483)."""
+    from . import FrameParams, camera_matrices, prev_view_proj, CT_TAA_BIT
+    aspect = w.width / w.height
+    pos, fwd, up = w.camera(f)
+    iv, ip = camera_matrices(pos, fwd, up, w.fov_radians, aspect)
+    if f > 0:
+        ppos, pfwd, pup = w.camera(f - 1)
+        piv, pip = camera_matrices(ppos, pfwd, pup, w.fov_radians, aspect)
+    else:
+        piv, pip = iv, ip
+    p = FrameParams()
+    p.width, p.height, p.frame_number = w.width, w.height, f
+    p.is_active = w.features | (CT_TAA_BIT if (w.taa and f > 0) else 0)
+    p.time = w.time
+    p.is_afternoon = w.afternoon
+    p.has_previous = 1 if f > 0 else 0
+    p.inv_view[:] = iv.tolist()
+    p.inv_proj[:] = ip.tolist()
+    p.prev_view_proj[:] = prev_view_proj(piv, pip).tolist()
+    p.restir = w.restir
+    return p

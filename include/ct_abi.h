@@ -1,0 +1,490 @@
+/*
+ * ct_abi.h -- C ABI of the MI355X-native cube-tracing per-frame ray-tracing path.
+ *
+ * Plain C: fixed-width integers, plain pointers and sizes, no HIP or torch
+ * types.  Streams are passed as `void*` holding a hipStream_t (NULL selects
+ * the library's own stream).  Every entry point returns an int status
+ * (CT_OK == 0, negative on error) and never aborts; the message of the last
+ * error on the calling thread is returned by ct_last_error_message().
+ *
+ * The boundary structs below keep the reference's scalar layouts byte for
+ * byte (src/shaders/shared.inl, `layout(scalar)` in
+ * src/shaders/include/defines.glsl:126-158) so that a host that fills the
+ * reference's buffers can fill these unchanged.  Inside the library all
+ * per-pixel and per-primitive data are stored structure-of-arrays.
+ */
+#ifndef CT_ABI_H
+#define CT_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+#define CT_OK 0
+#define CT_E_INVALID (-1)     /* bad argument or handle */
+#define CT_E_CAPACITY (-2)    /* a max_* capacity would be exceeded */
+#define CT_E_HIP (-3)         /* a HIP runtime call failed */
+#define CT_E_UNSUPPORTED (-4) /* feature not implemented by this library */
+#define CT_E_STATE (-5)       /* call not valid in the current state */
+#define CT_E_NODEVICE (-6)    /* no usable GPU */
+
+/* ------------------------------------------------ constants (shared.inl) */
+#define CT_MAX_INSTANCES 983040u        /* shared.inl:18 */
+#define CT_MAX_PRIMITIVES 134217728u    /* shared.inl:19 */
+#define CT_VOXEL_EXTENT 0.03125f        /* shared.inl:45 */
+#define CT_HALF_VOXEL_EXTENT 0.015625f  /* shared.inl:46 */
+#define CT_PRIMITIVE_ALIGNMENT 32u      /* accel_struct_mngr.hpp:22 */
+#define CT_INVALID_INDEX 0xFFFFFFFFu    /* (u32)-1 used as "none" */
+
+/* feature bits of Status.is_active (shared.inl:73-79) */
+#define CT_TAA_BIT (1u << 0)
+#define CT_RIS_POINT_LIGHT_BIT (1u << 1)
+#define CT_RIS_ENV_LIGHT_BIT (1u << 2)
+#define CT_RIS_CUBE_LIGHT_BIT (1u << 3)
+#define CT_RIS_BRDF_BIT (1u << 4)
+#define CT_REMAP_BIT (1u << 30)
+
+/* LIGHT.type (shared.inl:336-342) */
+#define CT_LIGHT_NONE 0u
+#define CT_LIGHT_POINT 1u
+#define CT_LIGHT_CUBE 2u
+#define CT_LIGHT_ENV_MAP 5u
+
+/* MATERIAL.type (shared.inl:295-301) */
+#define CT_MATERIAL_LAMBERTIAN 0u
+#define CT_MATERIAL_TYPE_MASK 0xFFu
+
+/* ------------------------------------------- boundary structs (AoS) */
+typedef struct ct_aabb { /* shared.inl:89-93, 24 B */
+  float minimum[3];
+  float maximum[3];
+} ct_aabb;
+
+typedef struct ct_object_info { /* shared.inl:95-99 */
+  uint32_t instance_id;
+  uint32_t primitive_id; /* index inside the instance's BLAS slice */
+} ct_object_info;
+
+typedef struct ct_instance { /* shared.inl:265-270, 72 B */
+  float transform[16];       /* object-to-world, column-major (glm/daxa) */
+  uint32_t first_primitive_index;
+  uint32_t primitive_count;
+} ct_instance;
+
+typedef struct ct_primitive { /* shared.inl:289-293 */
+  uint32_t material_index;
+  uint32_t light_index; /* CT_INVALID_INDEX when the voxel is not a light */
+} ct_primitive;
+
+typedef struct ct_material { /* shared.inl:305-321, 84 B */
+  uint32_t type;
+  float ambient[3];
+  float diffuse[3];
+  float specular[3];
+  float transmittance[3];
+  float emission[3];
+  float shininess;
+  float roughness;
+  float ior;
+  float dissolve;
+  int32_t illum;
+} ct_material;
+
+typedef struct ct_light { /* shared.inl:344-351, 40 B */
+  float position[3];
+  float emissive[3];
+  ct_object_info instance_info;
+  float size;
+  uint32_t type;
+} ct_light;
+
+typedef struct ct_light_config { /* shared.inl:203-218 */
+  uint32_t point_light_count;
+  float point_light_pdf;
+  uint32_t cube_light_count;
+  float cube_light_pdf;
+  uint32_t sphere_light_count;
+  float sphere_light_pdf;
+  uint32_t analytic_light_count;
+  float analytic_light_pdf;
+  uint32_t env_map_count;
+  float env_map_pdf;
+  uint32_t brdf_count;
+  float brdf_pdf;
+  uint32_t light_count;
+} ct_light_config;
+
+typedef struct ct_reservoir { /* shared.inl:368-377, 36 B */
+  uint32_t Y;
+  uint32_t l_type;
+  uint32_t seed;
+  float W_y;
+  float W_sum;
+  float M;
+  float F[3];
+} ct_reservoir;
+
+typedef struct ct_di_info { /* shared.inl:430-440, 60 B */
+  float position[3];
+  float distance;
+  float normal[3];
+  float ray_origin[3];
+  uint32_t seed;
+  ct_object_info instance_hit;
+  uint32_t mat_index;
+  float confidence;
+} ct_di_info;
+
+/* ------------------------------------------------------------- errors */
+const char *ct_last_error_message(void);
+/* 1 when a gfx950 device is visible to the HIP runtime, else 0. */
+int ct_device_available(void);
+
+/* =================================================================
+ * Acceleration-structure manager.
+ * Replaces ACCEL_STRUCT_MNGR (src/accel_struct_mngr.hpp:18-664).
+ * ================================================================= */
+typedef struct ct_accel ct_accel;
+
+typedef struct ct_accel_desc {
+  uint32_t max_instance_count;   /* create() argument, accel_struct_mngr.hpp:196 */
+  uint32_t max_primitive_count;
+  uint32_t max_cube_light_count;
+  int32_t device;                /* HIP device ordinal */
+} ct_accel_desc;
+
+/* ACCEL_STRUCT_MNGR::create, accel_struct_mngr.cpp:83-248 */
+int ct_accel_create(const ct_accel_desc *desc, ct_accel **out);
+/* ACCEL_STRUCT_MNGR::destroy, accel_struct_mngr.cpp:250 */
+int ct_accel_destroy(ct_accel *accel);
+
+/* Host staging the loader writes before queueing a build.  Mirrors
+ * get_instances / get_primitives / get_aabb_host_address / get_cube_lights /
+ * get_host_instance_count / get_host_primitive_count
+ * (accel_struct_mngr.hpp:265-311).  Pointers stay valid until destroy. */
+typedef struct ct_staging {
+  ct_instance *instances;          /* [max_instance_count] */
+  ct_primitive *primitives;        /* [max_primitive_count] */
+  ct_aabb *aabbs;                  /* [max_primitive_count] */
+  ct_light *cube_lights;           /* [max_cube_light_count] */
+  uint32_t host_instance_count;    /* temp_instance_count */
+  uint32_t host_primitive_count;   /* temp_primitive_count */
+  uint32_t cube_light_count;       /* the value create() gets by pointer */
+  uint32_t max_instance_count;
+  uint32_t max_primitive_count;
+  uint32_t max_cube_light_count;
+} ct_staging;
+int ct_accel_get_staging(ct_accel *accel, ct_staging *out);
+/* light_config->cube_light_count, kept by the manager for the frame. */
+int ct_accel_set_cube_light_count(ct_accel *accel, uint32_t count);
+/* request_aabb_host_buffer_count, accel_struct_mngr.hpp:279-285 */
+int ct_accel_request_aabb_host_buffer_count(ct_accel *accel, uint32_t count,
+                                            ct_aabb **base, uint32_t *offset);
+
+/* TASK, accel_struct_mngr.hpp:40-107 */
+#define CT_TASK_BUILD_BLAS_FROM_CPU 0u
+#define CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_CPU 1u
+#define CT_TASK_DELETE_BLAS_FROM_CPU 2u
+#define CT_TASK_UPDATE_BLAS_FROM_CPU 3u
+#define CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_GPU 4u
+#define CT_TASK_UNDO_OP_CPU 5u
+
+typedef struct ct_task {
+  uint32_t type;
+  union {
+    struct { /* BLAS_BUILD_FROM_CPU */
+      uint32_t instance_count;
+      uint32_t primitive_count;
+      float transform[16];
+    } blas_build_from_cpu;
+    struct { /* BLAS_PRIMITIVE_DELETE_FROM_CPU */
+      uint32_t instance_index;
+      uint32_t del_primitive_index;
+      uint32_t remap_primitive_index;
+      uint32_t del_light_index;
+      uint32_t remap_light_index;
+      uint32_t remap_primitive_light_index;
+    } blas_delete_primitive_from_cpu;
+    struct { /* BLAS_DELETE_FROM_CPU */
+      uint32_t instance_index;
+      uint32_t first_primitive_index;
+      uint32_t deleted_primitive_count;
+    } blas_delete_from_cpu;
+    struct { /* BLAS_UPDATE */
+      uint32_t instance_index;
+      float transform[16];
+      uint32_t primitive_count;
+      uint32_t primitive_index_buf_offset;
+      uint32_t aabb_buf_offset;
+    } blas_update;
+    struct { /* BLAS_DEL_PRIM_FROM_GPU */
+      uint32_t instance_index;
+      uint32_t del_prim_count;
+    } blas_del_prim_gpu;
+  } u;
+} ct_task;
+
+/* task_queue_add, accel_struct_mngr.hpp:321-330 */
+int ct_accel_task_queue_add(ct_accel *accel, const ct_task *task);
+
+/* update_scene, accel_struct_mngr.hpp:332-436.  Runs the pending phase
+ * stream-ordered on `stream`; with synchronize != 0 it returns after the
+ * stream has drained (the reference blocks on synchronize_cv).
+ * *did_work (optional) receives 1 when a phase ran, 0 when idle. */
+int ct_accel_update_scene(ct_accel *accel, int synchronize, void *stream,
+                          int *did_work);
+
+/* AS_MANAGER_STATUS values, accel_struct_mngr.hpp:24-33 */
+#define CT_AS_IDLE 0u
+#define CT_AS_UPDATING 1u
+#define CT_AS_SWITCHING 2u
+#define CT_AS_SETTLING 4u
+int ct_accel_get_status(ct_accel *accel, uint32_t *status);
+/* is_remapping_primitive_active, accel_struct_mngr.hpp:252-254 */
+int ct_accel_is_remapping_active(ct_accel *accel, int *active);
+
+/* Top-level structure handle; get_current_tlas / get_previous_tlas
+ * (accel_struct_mngr.hpp:216-223). Device pointers. */
+typedef struct ct_tlas {
+  const void *nodes;     /* BVH2 nodes, 64 B each */
+  const void *leaves;    /* instance ids, Morton order */
+  uint32_t root_ref;     /* bit 31 set: leaf */
+  uint32_t instance_count;
+  uint32_t depth;
+  uint32_t generation;   /* increments on every TLAS build */
+} ct_tlas;
+int ct_accel_get_tlas(ct_accel *accel, int previous, ct_tlas *out);
+
+/* Device buffers of the WORLD table (shared.inl:272-286) for the current
+ * or previous buffer set; get_{current,previous}_*_buffer,
+ * accel_struct_mngr.hpp:225-262. */
+typedef struct ct_world {
+  const void *instances;   /* ct_instance AoS */
+  const void *primitives;  /* ct_primitive AoS */
+  const void *aabbs;       /* ct_aabb AoS */
+  const void *cube_lights; /* ct_light AoS */
+  uint32_t instance_count;
+  uint32_t primitive_capacity;
+  uint32_t cube_light_count;
+  uint32_t pad;
+} ct_world;
+int ct_accel_get_world(ct_accel *accel, int previous, ct_world *out);
+
+/* Read back one BLAS (instance_index) or the TLAS (instance_index ==
+ * CT_INVALID_INDEX) as built on the GPU, for parity tests.  Arrays are
+ * host buffers the caller sizes with n = primitive (or instance) count:
+ * keys[n], ids[n], child[2*(n-1)], boxes[12*(n-1)] (child0 lo,hi, child1
+ * lo,hi), parent[2n-1] (internal nodes then leaves).  Any may be NULL. */
+typedef struct ct_bvh_arrays {
+  uint32_t *keys;
+  uint32_t *ids;
+  uint32_t *child;
+  float *boxes;
+  uint32_t *parent;
+  uint32_t n;        /* out: leaf count */
+  uint32_t depth;    /* out: longest root-to-leaf path in nodes */
+  uint32_t root_ref; /* out */
+  uint32_t pad;
+} ct_bvh_arrays;
+int ct_accel_read_bvh(ct_accel *accel, uint32_t instance_index,
+                      ct_bvh_arrays *arrays);
+/* Host copy of an instance's world-to-object matrix (inverse of the
+ * transform), as the kernels use it. */
+int ct_accel_get_world_to_object(ct_accel *accel, uint32_t instance_index,
+                                 float out[16]);
+
+/* =================================================================
+ * Per-stage traversal on caller-owned host arrays (tests, tools).
+ * Closest hit follows traceRayEXT + rint.glsl:10-43 + rchit semantics;
+ * any hit follows is_vertex_visible, light.glsl:82-142.
+ * ================================================================= */
+typedef struct ct_trace_stats {
+  uint64_t rays;
+  uint64_t nodes;     /* BVH2 internal nodes fetched (TLAS + BLAS) */
+  uint64_t leaves;    /* primitive leaves tested */
+  uint64_t instances; /* instance records entered */
+} ct_trace_stats;
+
+/* origins, directions: [n][3]; t_out: closest t or -1; ids_out: [n][2]
+ * (instance_id, primitive_id) or CT_INVALID_INDEX pairs on a miss. */
+int ct_trace_closest(ct_accel *accel, uint32_t n, const float *origins,
+                     const float *directions, float tmax, float *t_out,
+                     uint32_t *ids_out, ct_trace_stats *stats);
+/* distances: [n]; targets: [n][2] voxel excluded from the test, or
+ * CT_INVALID_INDEX pairs; occluded_out: [n] bytes, 1 = blocked. */
+int ct_trace_any(ct_accel *accel, uint32_t n, const float *origins,
+                 const float *directions, const float *distances,
+                 const uint32_t *targets, uint8_t *occluded_out,
+                 ct_trace_stats *stats);
+
+/* =================================================================
+ * Frame context: the buffers and passes App::draw records
+ * (src/main.cpp:1436-1703) -- prepass, shading pass, TAA.
+ * ================================================================= */
+typedef struct ct_ctx ct_ctx;
+
+typedef struct ct_ctx_desc {
+  uint32_t max_width;   /* the reference sizes for 3840x2160 (shared.inl:68-71) */
+  uint32_t max_height;
+  int32_t device;
+  uint32_t pad;
+} ct_ctx_desc;
+
+int ct_ctx_create(const ct_ctx_desc *desc, ct_ctx **out);
+int ct_ctx_destroy(ct_ctx *ctx);
+/* load_materials, main.cpp:512-558 (Lambertian materials only). */
+int ct_ctx_set_materials(ct_ctx *ctx, const ct_material *materials,
+                         uint32_t count);
+/* create_point_lights / create_environment_light, main.cpp:348-400 */
+int ct_ctx_set_point_lights(ct_ctx *ctx, const ct_light *lights, uint32_t count);
+int ct_ctx_set_env_lights(ct_ctx *ctx, const ct_light *lights, uint32_t count);
+/* load_materials with an offset: appends after the materials already
+ * loaded and returns the index of the first one in *base. */
+int ct_ctx_append_materials(ct_ctx *ctx, const ct_material *materials,
+                            uint32_t count, uint32_t *base);
+/* load_lights, main.cpp:418-438 */
+int ct_ctx_set_light_config(ct_ctx *ctx, const ct_light_config *config);
+int ct_ctx_get_light_config(ct_ctx *ctx, ct_light_config *out);
+/* load_reservoirs, main.cpp:234-318: zero every reservoir, DI, velocity
+ * and image buffer. */
+int ct_ctx_reset(ct_ctx *ctx, void *stream);
+
+/* ReSTIR tunables: defines.glsl:34-62 and taa.glsl:13.  The reference
+ * values are returned by ct_restir_params_reference(). */
+#define CT_MAX_NEIGHBORS 8u
+typedef struct ct_restir_params {
+  uint32_t min_ris_point_samples;  /* 1 */
+  uint32_t max_ris_point_samples;  /* 4 */
+  uint32_t min_ris_env_samples;    /* 1 */
+  uint32_t max_ris_env_samples;    /* 4 */
+  uint32_t min_ris_cube_samples;   /* 4 */
+  uint32_t max_ris_cube_samples;   /* 32 */
+  float max_influence_from_the_past; /* 20: confidence scale and temporal M cap */
+  uint32_t min_neighbors;          /* 3 */
+  uint32_t max_neighbors;          /* 4, at most CT_MAX_NEIGHBORS */
+  float min_neighbor_radius;       /* 2 px */
+  float max_neighbor_radius;       /* 50 px */
+  float max_distance_to_hit;       /* 1000 */
+  float taa_alpha;                 /* 0.8: weight of the current frame */
+  uint32_t taa_truncate_dispatch;  /* 1: TAA covers (W/8*8)x(H/8*8) like main.cpp:1642-1643 */
+  uint32_t direct_emittance;       /* 1: DIRECT_EMITTANCE_ON */
+  uint32_t pad;
+} ct_restir_params;
+void ct_restir_params_reference(ct_restir_params *out);
+
+typedef struct ct_frame_params {
+  uint32_t width;
+  uint32_t height;
+  uint32_t frame_number;   /* Status.frame_number */
+  uint32_t is_active;      /* Status.is_active feature bits */
+  float time;              /* Status.time */
+  uint32_t is_afternoon;   /* Status.is_afternoon */
+  uint32_t has_previous;   /* a previous frame exists (TAA precondition, main.cpp:1483) */
+  uint32_t pad;
+  float inv_view[16];      /* camera_view.inv_view (column-major) */
+  float inv_proj[16];      /* camera_view.inv_proj with y.y negated (main.cpp:1460) */
+  float prev_view_proj[16];/* inverse(prev_inv_proj) * inverse(prev_inv_view), motion_vectors.glsl:143 */
+  ct_restir_params restir;
+} ct_frame_params;
+
+/* Record the frame's passes on `stream`. */
+int ct_frame_render(ct_ctx *ctx, ct_accel *accel, const ct_frame_params *params,
+                    void *stream);
+
+/* Buffers readable after a frame (host copies, AoS reference layouts). */
+#define CT_BUF_IMAGE 0u          /* float4 per pixel: final image */
+#define CT_BUF_PREV_IMAGE 1u     /* float4: previous final image */
+#define CT_BUF_TAA_FRAME 2u      /* float4: shading output TAA reads */
+#define CT_BUF_DI 3u             /* ct_di_info: current frame DI */
+#define CT_BUF_PREV_DI 4u        /* ct_di_info: previous-frame DI */
+#define CT_BUF_INT_RESERVOIR 5u  /* ct_reservoir: intermediate */
+#define CT_BUF_PREV_RESERVOIR 6u /* ct_reservoir: previous frame */
+#define CT_BUF_VELOCITY 7u       /* float2 */
+int ct_frame_read(ct_ctx *ctx, uint32_t which, void *host_dst, size_t bytes);
+
+typedef struct ct_frame_stats {
+  uint64_t primary_rays;
+  uint64_t shadow_rays;     /* visibility queries of every pass */
+  uint64_t shadow_rays_ris;
+  uint64_t shadow_rays_temporal;
+  uint64_t shadow_rays_spatial;
+  uint64_t nodes;           /* any-hit traversal: BVH2 nodes fetched (counters on) */
+  uint64_t leaves;          /* any-hit traversal: leaves tested */
+  uint64_t instances;       /* any-hit traversal: instance records entered */
+  uint64_t closest_nodes;   /* primary closest-hit traversal, same counters */
+  uint64_t closest_leaves;
+  uint64_t closest_instances;
+  float ms_total;           /* HIP-event time of the frame's kernels */
+  float ms_primary;
+  float ms_ris;
+  float ms_temporal;
+  float ms_spatial;
+  float ms_taa;
+  float ms_trace_any;       /* sum over the any-hit traversal launches */
+  float ms_trace_closest;
+} ct_frame_stats;
+/* Enable per-pass HIP events (1) and traversal node/leaf counters (2). */
+int ct_ctx_set_instrumentation(ct_ctx *ctx, uint32_t flags);
+int ct_frame_get_stats(ct_ctx *ctx, ct_frame_stats *out);
+
+/* =================================================================
+ * Camera (src/camera.h:75-93, main.cpp:1452-1460) and matrices.
+ * ================================================================= */
+int ct_camera_matrices(const float position[3], const float forward[3],
+                       const float up[3], float fov_radians, float aspect,
+                       float z_near, float z_far, float inv_view[16],
+                       float inv_proj[16]);
+int ct_camera_prev_view_proj(const float prev_inv_view[16],
+                             const float prev_inv_proj[16], float out[16]);
+int ct_mat4_inverse(const float m[16], float out[16]);
+
+/* =================================================================
+ * Synthetic voxel scenes (the map loader's role, map_loader.cpp:224-300).
+ * ================================================================= */
+typedef struct ct_scene ct_scene;
+
+typedef struct ct_scene_desc {
+  uint32_t grid[3];          /* voxels per axis */
+  uint32_t chunk[3];         /* voxels per BLAS chunk per axis */
+  uint32_t occupied_count;   /* exact number of occupied voxels */
+  uint32_t emissive_count;   /* voxels that become cube lights */
+  float le_min, le_max;      /* emitted radiance range (white) */
+  float albedo_min, albedo_max;
+  uint32_t seed;
+  uint32_t palette_size;     /* 0: one material per voxel; else shared palette */
+  float voxel_extent;        /* 0.03125 */
+  uint32_t pad;
+} ct_scene_desc;
+
+typedef struct ct_scene_info {
+  uint32_t instance_count;
+  uint32_t primitive_count;     /* occupied voxels */
+  uint32_t material_count;
+  uint32_t cube_light_count;
+  const ct_instance *instances; /* first_primitive_index: host order, contiguous */
+  const ct_aabb *aabbs;         /* [primitive_count] */
+  const ct_primitive *primitives;
+  const ct_material *materials;
+  const ct_light *cube_lights;  /* instance_info = (instance, primitive in BLAS) */
+  float grid_min[3];
+  float grid_max[3];
+} ct_scene_info;
+
+int ct_scene_generate(const ct_scene_desc *desc, ct_scene **out);
+int ct_scene_get_info(const ct_scene *scene, ct_scene_info *out);
+int ct_scene_destroy(ct_scene *scene);
+/* load_scene (main.cpp:1045-1126): write the staging buffers, queue one
+ * BUILD_BLAS_FROM_CPU task, run update_scene(true), upload materials and
+ * the light configuration (cube lights only, brdf_count = 1 as at
+ * main.cpp:1317-1320). */
+int ct_scene_load(const ct_scene *scene, ct_accel *accel, ct_ctx *ctx,
+                  void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CT_ABI_H */

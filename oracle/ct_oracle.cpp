@@ -1,0 +1,1598 @@
+// ct_oracle.cpp -- TEST INFRASTRUCTURE ONLY: the CPU oracle.
+//
+// A plain C++ restatement of cube-tracing's per-frame ray-tracing path,
+// used as the checker for the HIP library and as the CPU baseline.  Built
+// with -ffp-contract=off and IEEE division/sqrt so every float operation is
+// a single correctly rounded IEEE operation in the order written here.
+//
+// Each function cites the reference file:line it restates.  Where the
+// reference leaves a result to the Vulkan driver (BVH layout, traversal
+// order, equal-t ties) the rule used is stated at the function.
+//
+// Pinning (see tests/): RNG against the TEA/LCG published constants and
+// known-answer vectors; the slab test against brute force on analytic
+// boxes; BVH traversal against brute force over every primitive; LBVH
+// arrays against invariants (every leaf once, parent contains child,
+// Karras split = highest differing bit); ReSTIR DI radiance against the
+// closed-form expectation for one unoccluded light and against invariants
+// (k = 0 spatial reuse leaves the reservoir unchanged, static camera
+// reprojects to the own pixel).
+
+#include "ct_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr uint32_t kLeaf = 0x80000000u;
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr float kHalfVoxel = 0.015625f;        // HALF_VOXEL_EXTENT, shared.inl:46
+constexpr float kLeafPad = 0.000244140625f;    // 2^-12 conservative node padding
+constexpr float kInvPi = 0.31830988618379067153776752674503f;   // shared.inl:52
+constexpr float kInv4Pi = 0.079577471545947667884441881686255f; // shared.inl:54
+constexpr float kMaxDistance = 1e9f;           // MAX_DISTANCE, defines.glsl:30
+constexpr float kMinCosTheta = 1e-6f;          // MIN_COS_THETA, defines.glsl:31
+constexpr int kStack = 96;
+
+// ------------------------------------------------------------------ math
+struct V3 { float x, y, z; };
+static inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+static inline V3 add(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V3 sub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V3 mulv(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline V3 muls(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+static inline V3 divs(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+static inline V3 neg(V3 a) { return v3(-a.x, -a.y, -a.z); }
+// GLSL dot/length/normalize/cross, evaluated left to right.
+static inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline float length(V3 a) { return std::sqrt(dot(a, a)); }
+static inline V3 normalize(V3 a) { return divs(a, length(a)); }
+static inline V3 cross(V3 a, V3 b) {
+  return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline V3 vabs(V3 a) { return v3(std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)); }
+static inline V3 load3(const float *p) { return v3(p[0], p[1], p[2]); }
+static inline void store3(float *p, V3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+// GLSL min/max/clamp/mix/sign.
+static inline float gmin(float a, float b) { return std::fmin(a, b); }
+static inline float gmax(float a, float b) { return std::fmax(a, b); }
+static inline float clampf(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
+static inline float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+static inline V3 mix3(V3 x, V3 y, float a) {
+  return v3(mixf(x.x, y.x, a), mixf(x.y, y.y, a), mixf(x.z, y.z, a));
+}
+static inline float gsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+static inline bool finite_or_nan_bad(float w) { return w < 0.0f || std::isnan(w) || std::isinf(w); }
+
+// Column-major 4x4 (m[c*4 + r]); mat * vec sums columns left to right.
+static inline void mat4_mul_vec(const float *m, const float v[4], float out[4]) {
+  for (int r = 0; r < 4; ++r)
+    out[r] = m[0 * 4 + r] * v[0] + m[1 * 4 + r] * v[1] + m[2 * 4 + r] * v[2] +
+             m[3 * 4 + r] * v[3];
+}
+static inline void mat4_mul(const float *a, const float *b, float *out) {
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 4; ++r)
+      out[c * 4 + r] = a[0 * 4 + r] * b[c * 4 + 0] + a[1 * 4 + r] * b[c * 4 + 1] +
+                       a[2 * 4 + r] * b[c * 4 + 2] + a[3 * 4 + r] * b[c * 4 + 3];
+}
+static inline V3 xform_point(const float *m, V3 p) {
+  float v[4] = {p.x, p.y, p.z, 1.0f}, o[4];
+  mat4_mul_vec(m, v, o);
+  return v3(o[0], o[1], o[2]);
+}
+static inline V3 xform_dir(const float *m, V3 p) {
+  float v[4] = {p.x, p.y, p.z, 0.0f}, o[4];
+  mat4_mul_vec(m, v, o);
+  return v3(o[0], o[1], o[2]);
+}
+
+// Saturating float->int conversions (NaN -> 0), the GPU conversion rule.
+static inline uint32_t f2u_sat(float x) {
+  if (!(x > 0.0f)) return 0u;
+  if (x >= 4294967296.0f) return 0xFFFFFFFFu;
+  return (uint32_t)x;
+}
+static inline int32_t f2i_sat(float x) {
+  if (std::isnan(x)) return 0;
+  if (x >= 2147483648.0f) return 0x7FFFFFFF;
+  if (x <= -2147483648.0f) return (int32_t)0x80000000;
+  return (int32_t)x;
+}
+
+// ------------------------------------------------------------------- RNG
+// random.glsl:26-40 (TEA, 16 rounds)
+static inline uint32_t tea(uint32_t val0, uint32_t val1) {
+  uint32_t v0 = val0, v1 = val1, s0 = 0;
+  for (uint32_t n = 0; n < 16; n++) {
+    s0 += 0x9e3779b9u;
+    v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+    v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+  }
+  return v0;
+}
+// random.glsl:44-50 (Numerical Recipes LCG, 24-bit output)
+static inline uint32_t lcg(uint32_t &prev) {
+  prev = 1664525u * prev + 1013904223u;
+  return prev & 0x00FFFFFFu;
+}
+// random.glsl:53-56
+static inline float rnd(uint32_t &prev) { return (float)lcg(prev) / (float)0x01000000; }
+// random.glsl:58-61
+static inline float rnd_interval(uint32_t &prev, float mn, float mx) {
+  return mn + rnd(prev) * (mx - mn);
+}
+// random.glsl:69-77
+static inline uint32_t urnd_interval(uint32_t &prev, uint32_t mn, uint32_t mx) {
+  if (mn == mx) return mn;
+  if (mn > mx) return f2u_sat(rnd_interval(prev, (float)mx, (float)mn));
+  return f2u_sat(rnd_interval(prev, (float)mn, (float)mx));
+}
+
+// ------------------------------------------------------------- geometry
+// Box.glsl:22-23
+static inline float safe_inverse(float x) { return (x == 0.0f) ? 1e12f : (1.0f / x); }
+
+// primitives.glsl:233-257
+static V3 compute_ray_origin(V3 pos, V3 normal) {
+  const float origin = 1.f / 32.f, f_scale = 1.f / 65536.f, i_scale = 256.f;
+  int32_t ix = f2i_sat(normal.x * i_scale), iy = f2i_sat(normal.y * i_scale),
+          iz = f2i_sat(normal.z * i_scale);
+  auto bump = [](float p, int32_t off) {
+    uint32_t bits;
+    std::memcpy(&bits, &p, 4);
+    bits += (uint32_t)(p < 0.0f ? -off : off);
+    float r;
+    std::memcpy(&r, &bits, 4);
+    return r;
+  };
+  V3 i_pos = v3(bump(pos.x, ix), bump(pos.y, iy), bump(pos.z, iz));
+  V3 f_off = muls(normal, f_scale);
+  return v3(std::fabs(pos.x) < origin ? pos.x + f_off.x : i_pos.x,
+            std::fabs(pos.y) < origin ? pos.y + f_off.y : i_pos.y,
+            std::fabs(pos.z) < origin ? pos.z + f_off.z : i_pos.z);
+}
+
+// Multiplication by Box.rotation = mat3(1): rotation * v and v * rotation
+// are evaluated as the matrix products the shader writes.
+static inline V3 rot_mul(V3 v) {
+  return v3(1.0f * v.x + 0.0f * v.y + 0.0f * v.z, 0.0f * v.x + 1.0f * v.y + 0.0f * v.z,
+            0.0f * v.x + 0.0f * v.y + 1.0f * v.z);
+}
+static inline V3 mul_rot(V3 v) {
+  return v3(v.x * 1.0f + v.y * 0.0f + v.z * 0.0f, v.x * 0.0f + v.y * 1.0f + v.z * 0.0f,
+            v.x * 0.0f + v.y * 0.0f + v.z * 1.0f);
+}
+
+// primitives.glsl:273-355 (jcgt 2018 ray-box), rotation = identity.
+static bool intersect_box(V3 center, V3 radius, V3 inv_radius, V3 ro, V3 rd,
+                          float &distance, V3 &normal, bool can_start,
+                          bool oriented, V3 inv_dir) {
+  V3 o = rot_mul(sub(ro, center));
+  V3 d = oriented ? mul_rot(rd) : rd;
+  float winding = 1.0f;
+  if (can_start) {
+    V3 rb = mulv(vabs(o), inv_radius);
+    winding = (gmax(rb.x, gmax(rb.y, rb.z)) < 1.0f) ? -1.0f : 1.0f;
+  }
+  V3 sgn = v3(-gsign(d.x), -gsign(d.y), -gsign(d.z));
+  V3 dtp = sub(mulv(muls(radius, winding), sgn), o);
+  if (oriented)
+    dtp = v3(dtp.x / d.x, dtp.y / d.y, dtp.z / d.z);
+  else
+    dtp = mulv(dtp, inv_dir);
+  bool tx = (dtp.x >= 0.0f) && (std::fabs(o.y + d.y * dtp.x) < radius.y) &&
+            (std::fabs(o.z + d.z * dtp.x) < radius.z);
+  bool ty = (dtp.y >= 0.0f) && (std::fabs(o.z + d.z * dtp.y) < radius.z) &&
+            (std::fabs(o.x + d.x * dtp.y) < radius.x);
+  bool tz = (dtp.z >= 0.0f) && (std::fabs(o.x + d.x * dtp.z) < radius.x) &&
+            (std::fabs(o.y + d.y * dtp.z) < radius.y);
+  sgn = tx ? v3(sgn.x, 0.0f, 0.0f)
+           : (ty ? v3(0.0f, sgn.y, 0.0f) : v3(0.0f, 0.0f, tz ? sgn.z : 0.0f));
+  distance = (sgn.x != 0.0f) ? dtp.x : ((sgn.y != 0.0f) ? dtp.y : dtp.z);
+  normal = oriented ? rot_mul(sgn) : sgn;
+  return (sgn.x != 0.0f) || (sgn.y != 0.0f) || (sgn.z != 0.0f);
+}
+
+// primitives.glsl:499-507
+static V3 cube_like_normal(V3 n) {
+  V3 a = vabs(n);
+  float mc = gmax(gmax(a.x, a.y), a.z);
+  return (mc == a.x) ? v3(gsign(n.x), 0, 0)
+                     : (mc == a.y) ? v3(0, gsign(n.y), 0) : v3(0, 0, gsign(n.z));
+}
+
+// mat.glsl:111-135
+static V3 sky_grad(V3 dir, V3 a, V3 b) {
+  V3 u = normalize(dir);
+  float t = 0.5f * (u.y + 1.0f);
+  return mix3(a, b, t);
+}
+static V3 day_color(V3 d) { return sky_grad(d, v3(1.0f, 1.0f, 1.0f), v3(0.5f, 0.7f, 1.0f)); }
+static V3 night_color(V3 d) { return sky_grad(d, v3(0.2f, 0.2f, 0.3f), v3(0.2f, 0.2f, 0.5f)); }
+static V3 sunset_color(V3 d) { return sky_grad(d, v3(0.8f, 0.4f, 0.0f), v3(1.0f, 0.4f, 0.0f)); }
+static V3 sunrise_color(V3 d) { return sky_grad(d, v3(0.7f, 0.35f, 0.0f), v3(0.8f, 0.4f, 0.0f)); }
+// mat.glsl:137-202 (the blends use `t`, the time, as written).
+static V3 sky_color(float t, bool afternoon, V3 dir) {
+  const float night_duration = 0.5f, sunrise_duration = 0.10f,
+              full_daylight_transition = 0.10f, full_night_transition = 0.10f;
+  float sunrise_end = night_duration + sunrise_duration;
+  float sunrise_to_daylight = sunrise_end + full_daylight_transition;
+  float zenith_transition = 1.0f - sunrise_to_daylight;
+  float daylight_end = 1.0f - zenith_transition;
+  float sunset_start = 1.0f - zenith_transition - sunrise_duration;
+  float sunset_end = 1.0f - zenith_transition - sunrise_duration - full_night_transition;
+  if (afternoon) {
+    if (t < night_duration) return night_color(dir);
+    if (t < sunrise_end) return mix3(night_color(dir), sunrise_color(dir), t);
+    if (t < sunrise_to_daylight) return mix3(sunrise_color(dir), sunset_color(dir), t);
+    return mix3(sunset_color(dir), day_color(dir), t);
+  }
+  if (t > daylight_end) return mix3(sunrise_color(dir), day_color(dir), t);
+  if (t > sunset_start) return mix3(sunset_color(dir), sunrise_color(dir), t);
+  if (t > sunset_end) return mix3(night_color(dir), sunset_color(dir), t);
+  return night_color(dir);
+}
+
+// light.glsl:78-80 (Rec.709)
+static inline float luminance(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+// ------------------------------------------------------------------ LBVH
+static inline uint32_t expand_bits(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+static inline uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  return (expand_bits(x) << 2) | (expand_bits(y) << 1) | expand_bits(z);
+}
+static inline int clz32(uint32_t x) { return x == 0 ? 32 : __builtin_clz(x); }
+
+struct Bvh {
+  uint32_t n = 0;
+  std::vector<uint32_t> keys, ids, child, parent;
+  std::vector<float> box;          // per internal node: child0 lo,hi, child1 lo,hi
+  std::vector<float> leaf_center;  // sorted order (BLAS)
+  uint32_t root_ref = kLeaf;
+  uint32_t depth = 0;
+  float root_lo[3] = {0, 0, 0}, root_hi[3] = {0, 0, 0};
+};
+
+// Morton keys: 10 bits per axis of the centre, quantised over the centres'
+// bounds: q = min(max((c - lo) * (1024 / (hi - lo)), 0), 1023).
+static void morton_keys(uint32_t n, const float *centers, std::vector<uint32_t> &keys) {
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = gmin(lo[a], centers[3 * i + a]);
+      hi[a] = gmax(hi[a], centers[3 * i + a]);
+    }
+  float scale[3];
+  for (int a = 0; a < 3; ++a) {
+    float ext = hi[a] - lo[a];
+    scale[a] = ext > 0.0f ? 1024.0f / ext : 0.0f;
+  }
+  keys.resize(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t q[3];
+    for (int a = 0; a < 3; ++a) {
+      float f = (centers[3 * i + a] - lo[a]) * scale[a];
+      f = gmin(gmax(f, 0.0f), 1023.0f);
+      q[a] = (uint32_t)f;
+    }
+    keys[i] = morton3(q[0], q[1], q[2]);
+  }
+}
+
+// Karras 2012, "Maximizing parallelism in the construction of BVHs",
+// Fig. 4, on keys sorted by (key, original index); equal keys split on
+// the index (delta = 32 + clz(i ^ j)).
+static void karras(uint32_t n, const uint32_t *keys, uint32_t *child) {
+  auto delta = [&](int64_t i, int64_t j) -> int {
+    if (j < 0 || j > (int64_t)n - 1) return -1;
+    uint32_t ki = keys[i], kj = keys[j];
+    if (ki == kj) return 32 + clz32((uint32_t)i ^ (uint32_t)j);
+    return clz32(ki ^ kj);
+  };
+  for (int64_t i = 0; i + 1 < (int64_t)n; ++i) {
+    int d = (delta(i, i + 1) - delta(i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = delta(i, i - d);
+    int64_t lmax = 2;
+    while (delta(i, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+      if (delta(i, i + (l + t) * d) > dmin) l += t;
+    int64_t j = i + l * d;
+    int dnode = delta(i, j);
+    int64_t s = 0, t = l;
+    do {
+      t = (t + 1) >> 1;
+      if (delta(i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int64_t gamma = i + s * d + std::min(d, 0);
+    int64_t lo = std::min(i, j), hi = std::max(i, j);
+    child[2 * i + 0] = (lo == gamma) ? (kLeaf | (uint32_t)gamma) : (uint32_t)gamma;
+    child[2 * i + 1] = (hi == gamma + 1) ? (kLeaf | (uint32_t)(gamma + 1)) : (uint32_t)(gamma + 1);
+  }
+}
+
+static bool build_lbvh(uint32_t n, const float *centers, const float *lo, const float *hi,
+                       Bvh &b) {
+  b = Bvh();
+  b.n = n;
+  if (n == 0) return false;
+  std::vector<uint32_t> raw;
+  morton_keys(n, centers, raw);
+  std::vector<uint32_t> order(n);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t c) { return raw[a] < raw[c]; });
+  b.keys.resize(n);
+  b.ids.resize(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    b.keys[i] = raw[order[i]];
+    b.ids[i] = order[i];
+  }
+  b.parent.assign(2 * (size_t)n - 1, kInvalid);
+  b.child.assign(n > 1 ? 2 * (size_t)(n - 1) : 0, 0);
+  b.box.assign(n > 1 ? 12 * (size_t)(n - 1) : 0, 0.0f);
+  if (n == 1) {
+    b.root_ref = kLeaf | 0u;
+    b.depth = 0;
+    for (int a = 0; a < 3; ++a) {
+      b.root_lo[a] = lo[3 * b.ids[0] + a];
+      b.root_hi[a] = hi[3 * b.ids[0] + a];
+    }
+    return true;
+  }
+  karras(n, b.keys.data(), b.child.data());
+  for (uint32_t i = 0; i + 1 < n; ++i)
+    for (int c = 0; c < 2; ++c) {
+      uint32_t r = b.child[2 * i + c];
+      if (r & kLeaf)
+        b.parent[(n - 1) + (r & ~kLeaf)] = i;
+      else
+        b.parent[r] = i;
+    }
+  b.root_ref = 0;
+  // Refit: node box = union of its children (min/max are exact).
+  std::vector<float> nb(6 * (size_t)(n - 1));
+  std::vector<uint32_t> stack;
+  std::vector<uint8_t> state(n - 1, 0);
+  stack.push_back(0);
+  auto ref_box = [&](uint32_t r, float *out) {
+    if (r & kLeaf) {
+      uint32_t id = b.ids[r & ~kLeaf];
+      for (int a = 0; a < 3; ++a) {
+        out[a] = lo[3 * id + a];
+        out[3 + a] = hi[3 * id + a];
+      }
+    } else {
+      std::memcpy(out, &nb[6 * (size_t)r], 6 * sizeof(float));
+    }
+  };
+  while (!stack.empty()) {
+    uint32_t i = stack.back();
+    if (state[i] == 0) {
+      state[i] = 1;
+      for (int c = 0; c < 2; ++c)
+        if (!(b.child[2 * i + c] & kLeaf)) stack.push_back(b.child[2 * i + c]);
+      continue;
+    }
+    stack.pop_back();
+    float c0[6], c1[6];
+    ref_box(b.child[2 * i + 0], c0);
+    ref_box(b.child[2 * i + 1], c1);
+    std::memcpy(&b.box[12 * (size_t)i], c0, 6 * sizeof(float));
+    std::memcpy(&b.box[12 * (size_t)i + 6], c1, 6 * sizeof(float));
+    for (int a = 0; a < 3; ++a) {
+      nb[6 * (size_t)i + a] = gmin(c0[a], c1[a]);
+      nb[6 * (size_t)i + 3 + a] = gmax(c0[3 + a], c1[3 + a]);
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    b.root_lo[a] = nb[a];
+    b.root_hi[a] = nb[3 + a];
+  }
+  // Depth: internal nodes on the longest root-to-leaf path.
+  std::vector<uint32_t> nd(n - 1, 0);
+  std::vector<uint32_t> order2;
+  order2.push_back(0);
+  nd[0] = 1;
+  uint32_t depth = 1;
+  for (size_t k = 0; k < order2.size(); ++k) {
+    uint32_t i = order2[k];
+    for (int c = 0; c < 2; ++c) {
+      uint32_t r = b.child[2 * i + c];
+      if (!(r & kLeaf)) {
+        nd[r] = nd[i] + 1;
+        depth = std::max(depth, nd[r]);
+        order2.push_back(r);
+      }
+    }
+  }
+  b.depth = depth;
+  return true;
+}
+
+// -------------------------------------------------------------- scene
+struct Scene {
+  std::vector<ct_instance> inst;
+  std::vector<float> w2o;       // [n][16]
+  std::vector<ct_aabb> aabbs;
+  std::vector<ct_primitive> prims;
+  std::vector<ct_material> mats;
+  std::vector<ct_light> cube, point, env;
+  ct_light_config cfg{};
+  std::vector<Bvh> blas;
+  Bvh tlas;
+  uint32_t prim_capacity = 0;
+};
+
+struct TraceCount {
+  uint64_t rays = 0, nodes = 0, leaves = 0, instances = 0;
+};
+
+// Conservative slab test of a BVH child box (the build's own rule, used
+// identically by the HIP traversal).
+static inline bool slab(const float *bx, V3 o, V3 inv, float limit, float &tn) {
+  float t1x = (bx[0] - o.x) * inv.x, t2x = (bx[3] - o.x) * inv.x;
+  float t1y = (bx[1] - o.y) * inv.y, t2y = (bx[4] - o.y) * inv.y;
+  float t1z = (bx[2] - o.z) * inv.z, t2z = (bx[5] - o.z) * inv.z;
+  float n0 = gmax(gmax(gmin(t1x, t2x), gmin(t1y, t2y)), gmin(t1z, t2z));
+  float f0 = gmin(gmin(gmax(t1x, t2x), gmax(t1y, t2y)), gmax(t1z, t2z));
+  tn = n0;
+  return (n0 <= f0) && (f0 >= 0.0f) && (n0 <= limit);
+}
+
+struct ClosestState {
+  float t;
+  uint32_t inst, prim;
+};
+
+// traceRayEXT(tmin 0, tmax) with rint.glsl:10-43 as the intersection
+// shader: a primitive is a candidate when intersect_box (rayCanStartInBox
+// = true, oriented = true) hits with t > 0 and t <= the current tmax.
+// The closest candidate wins; equal t goes to the smaller (instance,
+// primitive) pair (the reference leaves ties to the driver).
+static inline bool better(float t, uint32_t i, uint32_t p, const ClosestState &s) {
+  if (t < s.t) return true;
+  if (t == s.t) return (i < s.inst) || (i == s.inst && p < s.prim);
+  return false;
+}
+
+static void closest_leaf(const Scene &sc, uint32_t ii, uint32_t pid, V3 oo, V3 od,
+                         ClosestState &st) {
+  const ct_instance &in = sc.inst[ii];
+  const ct_aabb &bb = sc.aabbs[in.first_primitive_index + pid];
+  V3 c = muls(add(load3(bb.minimum), load3(bb.maximum)), 0.5f);
+  V3 rad = v3(kHalfVoxel, kHalfVoxel, kHalfVoxel);
+  V3 irad = v3(safe_inverse(kHalfVoxel), safe_inverse(kHalfVoxel), safe_inverse(kHalfVoxel));
+  float t;
+  V3 nrm;
+  V3 inv = v3(safe_inverse(od.x), safe_inverse(od.y), safe_inverse(od.z));
+  bool hit = intersect_box(c, rad, irad, oo, od, t, nrm, true, true, inv);
+  if (hit && t > 0.0f && better(t, ii, pid, st)) {
+    st.t = t;
+    st.inst = ii;
+    st.prim = pid;
+  }
+}
+
+// is_vertex_visible, light.glsl:82-142: a primitive occludes when
+// is_hit_from_ray (rayCanStartInBox = false, non-oriented) hits with
+// 0 <= t <= distance and it is not the target voxel.
+static bool any_leaf(const Scene &sc, uint32_t ii, uint32_t pid, V3 oo, V3 od,
+                     float dist, uint32_t ti, uint32_t tp) {
+  const ct_instance &in = sc.inst[ii];
+  const ct_aabb &bb = sc.aabbs[in.first_primitive_index + pid];
+  V3 c = muls(add(load3(bb.minimum), load3(bb.maximum)), 0.5f);
+  V3 rad = v3(kHalfVoxel, kHalfVoxel, kHalfVoxel);
+  V3 irad = v3(safe_inverse(kHalfVoxel), safe_inverse(kHalfVoxel), safe_inverse(kHalfVoxel));
+  V3 inv = v3(safe_inverse(od.x), safe_inverse(od.y), safe_inverse(od.z));
+  float t;
+  V3 nrm;
+  bool hit = intersect_box(c, rad, irad, oo, od, t, nrm, false, false, inv);
+  if (!hit || !(t >= 0.0f) || !(t <= dist)) return false;
+  if (ii == ti && pid == tp) return false;
+  return true;
+}
+
+// Two-level traversal: TLAS in world space, each BLAS in object space,
+// near child first (ties go to child 0), far child pushed.
+template <bool kAny>
+static bool traverse(const Scene &sc, V3 o, V3 d, float limit, uint32_t ti, uint32_t tp,
+                     ClosestState &st, TraceCount &cnt) {
+  uint32_t stack[kStack];
+  int sp = 0;
+  V3 winv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  uint32_t ref = sc.tlas.root_ref;
+  for (;;) {
+    if (ref & kLeaf) {
+      uint32_t ii = sc.tlas.ids[ref & ~kLeaf];
+      cnt.instances++;
+      const float *m = &sc.w2o[16 * (size_t)ii];
+      V3 oo = xform_point(m, o), od = xform_dir(m, d);
+      V3 oinv = v3(1.0f / od.x, 1.0f / od.y, 1.0f / od.z);
+      const Bvh &b = sc.blas[ii];
+      int base = sp;
+      uint32_t r = b.root_ref;
+      for (;;) {
+        if (r & kLeaf) {
+          cnt.leaves++;
+          uint32_t pid = b.ids[r & ~kLeaf];
+          if (kAny) {
+            if (any_leaf(sc, ii, pid, oo, od, limit, ti, tp)) return true;
+          } else {
+            closest_leaf(sc, ii, pid, oo, od, st);
+          }
+        } else {
+          cnt.nodes++;
+          const float *bx = &b.box[12 * (size_t)r];
+          float lim = kAny ? limit : st.t;
+          float tn0, tn1;
+          bool h0 = slab(bx, oo, oinv, lim, tn0), h1 = slab(bx + 6, oo, oinv, lim, tn1);
+          uint32_t c0 = b.child[2 * r], c1 = b.child[2 * r + 1];
+          if (h0 && h1) {
+            bool swap = tn1 < tn0;
+            stack[sp++] = swap ? c0 : c1;
+            if (sp >= kStack) { std::fprintf(stderr, "oracle stack overflow\n"); std::abort(); }
+            r = swap ? c1 : c0;
+            continue;
+          }
+          if (h0) { r = c0; continue; }
+          if (h1) { r = c1; continue; }
+        }
+        if (sp == base) break;
+        r = stack[--sp];
+      }
+    } else {
+      cnt.nodes++;
+      const float *bx = &sc.tlas.box[12 * (size_t)ref];
+      float lim = kAny ? limit : st.t;
+      float tn0, tn1;
+      bool h0 = slab(bx, o, winv, lim, tn0), h1 = slab(bx + 6, o, winv, lim, tn1);
+      uint32_t c0 = sc.tlas.child[2 * ref], c1 = sc.tlas.child[2 * ref + 1];
+      if (h0 && h1) {
+        bool swap = tn1 < tn0;
+        stack[sp++] = swap ? c0 : c1;
+        ref = swap ? c1 : c0;
+        continue;
+      }
+      if (h0) { ref = c0; continue; }
+      if (h1) { ref = c1; continue; }
+    }
+    if (sp == 0) break;
+    ref = stack[--sp];
+  }
+  return false;
+}
+
+static bool trace_any(const Scene &sc, V3 o, V3 d, float dist, uint32_t ti, uint32_t tp,
+                      TraceCount &cnt) {
+  ClosestState st{0, 0, 0};
+  cnt.rays++;
+  return traverse<true>(sc, o, d, dist, ti, tp, st, cnt);
+}
+
+static ClosestState trace_closest(const Scene &sc, V3 o, V3 d, float tmax, TraceCount &cnt) {
+  ClosestState st{tmax, kInvalid, kInvalid};
+  cnt.rays++;
+  traverse<false>(sc, o, d, tmax, kInvalid, kInvalid, st, cnt);
+  return st;
+}
+
+static void build_scene_bvh(Scene &sc) {
+  const uint32_t ni = (uint32_t)sc.inst.size();
+  sc.blas.resize(ni);
+  std::vector<float> tl_c(3 * (size_t)ni), tl_lo(3 * (size_t)ni), tl_hi(3 * (size_t)ni);
+  const float r = kHalfVoxel + kLeafPad;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t ii = 0; ii < (int64_t)ni; ++ii) {
+    const uint32_t i = (uint32_t)ii;
+    const ct_instance &in = sc.inst[i];
+    uint32_t n = in.primitive_count;
+    std::vector<float> c(3 * (size_t)n), lo(3 * (size_t)n), hi(3 * (size_t)n);
+    for (uint32_t k = 0; k < n; ++k) {
+      const ct_aabb &bb = sc.aabbs[in.first_primitive_index + k];
+      V3 cc = muls(add(load3(bb.minimum), load3(bb.maximum)), 0.5f);
+      store3(&c[3 * k], cc);
+      store3(&lo[3 * k], v3(cc.x - r, cc.y - r, cc.z - r));
+      store3(&hi[3 * k], v3(cc.x + r, cc.y + r, cc.z + r));
+    }
+    build_lbvh(n, c.data(), lo.data(), hi.data(), sc.blas[i]);
+    // Instance world box: the 8 transformed corners of the BLAS root box,
+    // padded once more.
+    const Bvh &b = sc.blas[i];
+    float wl[3] = {INFINITY, INFINITY, INFINITY}, wh[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < 8; ++k) {
+      V3 p = v3((k & 1) ? b.root_hi[0] : b.root_lo[0], (k & 2) ? b.root_hi[1] : b.root_lo[1],
+                (k & 4) ? b.root_hi[2] : b.root_lo[2]);
+      V3 w = xform_point(in.transform, p);
+      wl[0] = gmin(wl[0], w.x); wl[1] = gmin(wl[1], w.y); wl[2] = gmin(wl[2], w.z);
+      wh[0] = gmax(wh[0], w.x); wh[1] = gmax(wh[1], w.y); wh[2] = gmax(wh[2], w.z);
+    }
+    for (int a = 0; a < 3; ++a) {
+      tl_lo[3 * i + a] = wl[a] - kLeafPad;
+      tl_hi[3 * i + a] = wh[a] + kLeafPad;
+      tl_c[3 * i + a] = (tl_lo[3 * i + a] + tl_hi[3 * i + a]) * 0.5f;
+    }
+  }
+  build_lbvh(ni, tl_c.data(), tl_lo.data(), tl_hi.data(), sc.tlas);
+}
+
+// ------------------------------------------------------- DI frame state
+struct Hit { // HIT_INFO_INPUT, shared.inl:171-179
+  V3 world_hit, world_nrm;
+  float distance;
+  V3 scatter_dir;
+  uint32_t inst, prim;
+  uint32_t mat_idx;
+};
+
+struct Res { // RESERVOIR, shared.inl:368-377
+  uint32_t Y = kInvalid, l_type = CT_LIGHT_NONE, seed = 0;
+  float W_y = 0, W_sum = 0, M = 0;
+  V3 F = {0, 0, 0};
+};
+
+static inline Res res_from(const ct_reservoir &r) {
+  Res o;
+  o.Y = r.Y; o.l_type = r.l_type; o.seed = r.seed;
+  o.W_y = r.W_y; o.W_sum = r.W_sum; o.M = r.M;
+  o.F = load3(r.F);
+  return o;
+}
+static inline ct_reservoir res_to(const Res &r) {
+  ct_reservoir o;
+  o.Y = r.Y; o.l_type = r.l_type; o.seed = r.seed;
+  o.W_y = r.W_y; o.W_sum = r.W_sum; o.M = r.M;
+  store3(o.F, r.F);
+  return o;
+}
+
+struct Frame {
+  Scene *sc = nullptr;
+  uint32_t W = 0, H = 0;
+  std::vector<ct_di_info> di_cur, di_prev;
+  std::vector<ct_reservoir> res_int, res_prev;
+  std::vector<float> velocity;   // [W*H][2]
+  std::vector<float> img[2];     // [W*H][4]
+  std::vector<float> taa;        // [W*H][4]
+  int cur = 0;
+  bool first = true;
+  ct_frame_stats stats{};
+};
+
+struct Ctx { // per-frame constants read by every pixel
+  const Scene *sc;
+  Frame *fr;
+  const ct_frame_params *p;
+  std::vector<float> pt; // per instance: prev_view_proj * geometry_T
+};
+
+static inline const ct_material &material(const Ctx &c, uint32_t idx) {
+  return c.sc->mats[idx];
+}
+
+// reservoir.glsl:17-25
+static inline void init_res(Res &r) { r = Res(); }
+// reservoir.glsl:31-50
+static bool update_reservoir(Res &r, uint32_t X, uint32_t l_type, uint32_t random_seed, V3 F,
+                             float w, float c, uint32_t &seed) {
+  if (finite_or_nan_bad(w)) return false;
+  r.M += c;
+  r.W_sum += w;
+  if (rnd(seed) < (w / r.W_sum)) {
+    r.Y = X;
+    r.l_type = l_type;
+    r.seed = random_seed;
+    r.F = F;
+    return true;
+  }
+  return false;
+}
+// reservoir.glsl:72-74
+static inline bool res_valid(const Res &r) {
+  return r.M > 0.0f && r.Y != kInvalid && r.l_type != CT_LIGHT_NONE;
+}
+
+// prng.glsl:198-215
+static V3 random_cube_normal(uint32_t &seed) {
+  uint32_t face = std::min(urnd_interval(seed, 0, 6), 5u);
+  switch (face) {
+    case 0: return v3(-1, 0, 0);
+    case 1: return v3(1, 0, 0);
+    case 2: return v3(0, -1, 0);
+    case 3: return v3(0, 1, 0);
+    case 4: return v3(0, 0, -1);
+    default: return v3(0, 0, 1);
+  }
+}
+// prng.glsl:217-224
+static void orthonormal_basis(V3 n, V3 &t1, V3 &t2) {
+  if (std::fabs(n.x) > std::fabs(n.y))
+    t1 = normalize(v3(-n.z, 0.0f, n.x));
+  else
+    t1 = normalize(v3(0.0f, n.z, -n.y));
+  t2 = normalize(cross(n, t1));
+}
+// prng.glsl:235-252
+static V3 random_quad(V3 n, V3 p, float sx, float sy, uint32_t &seed) {
+  float u = rnd(seed);
+  float v = rnd(seed);
+  float qx = u * sx, qy = v * sy;
+  V3 xa, ya;
+  orthonormal_basis(n, xa, ya);
+  return add(add(p, muls(xa, qx)), muls(ya, qy));
+}
+// prng.glsl:143-161 (the rejection loop is capped at 1024 draws)
+static V3 random_on_hemisphere(uint32_t &seed, V3 n) {
+  V3 p = v3(0, 0, 0);
+  for (int k = 0; k < 1024; ++k) {
+    float x = rnd_interval(seed, -1.0f, 1.0f);
+    float y = rnd_interval(seed, -1.0f, 1.0f);
+    float z = rnd_interval(seed, -1.0f, 1.0f);
+    p = v3(x, y, z);
+    if (x * x + y * y + z * z >= 1.0f) continue;
+    break;
+  }
+  V3 u = normalize(p);
+  return dot(u, n) > 0.0f ? u : neg(u);
+}
+
+// mat.glsl:58-64 (only the z component of to_local is used)
+static inline float local_z(V3 n, V3 v) { return dot(v, normalize(n)); }
+// light.glsl:200-229 (Lambertian, COSINE_HEMISPHERE_SAMPLING = 1)
+static float sample_material_pdf(const ct_material &, V3 n, V3 wo, V3 wi) {
+  float wo_z = local_z(n, wo), wi_z = local_z(n, wi);
+  if (gmin(wo_z, wi_z) < kMinCosTheta) return 0.0f;
+  return kInvPi * wi_z;
+}
+// mat.glsl:24-27, 66-88 (Lambertian)
+static V3 evaluate_material(const ct_material &m, V3 n, V3 wo, V3 wi) {
+  float wo_z = local_z(n, wo), wi_z = local_z(n, wi);
+  if (gmin(wo_z, wi_z) < kMinCosTheta) return v3(0, 0, 0);
+  return muls(muls(load3(m.diffuse), kInvPi), wi_z);
+}
+// light.glsl:160-164
+static float geom_fact_sa(V3 P, V3 Ps, V3 ns) {
+  V3 dir = normalize(sub(Ps, P));
+  V3 dd = sub(Ps, P);
+  float dist2 = dot(dd, dd);
+  return std::fabs(dot(ns, dir)) / dist2;
+}
+
+struct LightSample {
+  bool vis;
+  V3 l_pos, l_nor, Le;
+  float G;
+};
+
+static const ct_light *light_by_type(const Ctx &c, uint32_t type, uint32_t idx) {
+  const Scene &s = *c.sc;
+  switch (type) {
+    case CT_LIGHT_POINT: return idx < s.point.size() ? &s.point[idx] : nullptr;
+    case CT_LIGHT_ENV_MAP: return idx < s.env.size() ? &s.env[idx] : nullptr;
+    case CT_LIGHT_CUBE: return idx < s.cube.size() ? &s.cube[idx] : nullptr;
+    default: return nullptr;
+  }
+}
+
+// sample_lights, light.glsl:295-399.  `visibility` traces the shadow ray.
+// An output the shader leaves unwritten (G on the fast-discard path) is 0.
+static bool sample_lights(const Ctx &c, const Hit &hit, const ct_light &l, float &pdf,
+                          LightSample &out, uint32_t &seed, bool calc_pdf, bool visibility,
+                          TraceCount &cnt) {
+  V3 P = hit.world_hit, n = hit.world_nrm;
+  float distance = -1.0f;
+  V3 l_wi = v3(0, 0, 0), l_pos = v3(0, 0, 0), l_nor = v3(0, 0, 0);
+  out.G = 0.0f;
+  out.l_pos = l_pos;
+  out.l_nor = l_nor;
+  if (l.instance_info.instance_id == hit.inst && l.instance_info.primitive_id == hit.prim) {
+    out.Le = v3(0, 0, 0);
+    out.vis = false;
+    return false;
+  }
+  bool vis = true, check_instance = false;
+  V3 emissive = load3(l.emissive);
+  if (l.type == CT_LIGHT_CUBE) {
+    float half = l.size * 0.5f;
+    float qs = l.size;
+    uint32_t i = 0;
+    do {
+      vis = true;
+      l_nor = random_cube_normal(seed);
+      l_pos = add(load3(l.position), muls(l_nor, half));
+      l_pos = random_quad(l_nor, l_pos, qs * 0.5f, qs * 0.5f, seed);
+      l_wi = normalize(sub(P, l_pos));
+      vis = vis && (dot(l_wi, l_nor) > 0.0f);
+      vis = vis && (dot(l_wi, n) < 0.0f);
+    } while (!vis && i++ < 10);
+    l_pos = compute_ray_origin(l_pos, l_nor);
+    l_pos = compute_ray_origin(l_pos, l_nor);
+    distance = length(sub(P, l_pos));
+    if (calc_pdf) {
+      float area = qs * qs * 6.0f;
+      pdf /= area;
+    }
+    check_instance = true;
+    out.G = geom_fact_sa(P, l_pos, l_nor);
+  } else if (l.type == CT_LIGHT_POINT) {
+    l_pos = load3(l.position);
+    l_nor = normalize(sub(P, l_pos));
+    distance = length(sub(P, l_pos));
+    check_instance = false;
+    l_wi = l_nor;
+    out.G = geom_fact_sa(P, l_pos, l_nor);
+  } else if (l.type == CT_LIGHT_ENV_MAP) {
+    V3 l_dir = random_on_hemisphere(seed, n);
+    l_pos = add(P, muls(l_dir, kMaxDistance));
+    l_nor = normalize(sub(P, l_pos));
+    distance = length(sub(P, l_pos));
+    check_instance = false;
+    l_wi = l_nor;
+    emissive = mulv(emissive, sky_color(c.p->time, c.p->is_afternoon != 0, l_dir));
+    if (calc_pdf) pdf /= kInv4Pi;
+    out.G = 1.0f;
+  }
+  V3 l_v = neg(l_wi);
+  if (visibility && vis) {
+    uint32_t ti = check_instance ? l.instance_info.instance_id : kInvalid;
+    uint32_t tp = check_instance ? l.instance_info.primitive_id : kInvalid;
+    vis = vis && !trace_any(*c.sc, P, l_v, distance, ti, tp, cnt);
+  }
+  out.l_pos = l_pos;
+  out.l_nor = l_nor;
+  out.Le = vis ? emissive : v3(0, 0, 0);
+  out.vis = vis;
+  return vis;
+}
+
+// calculate_sampled_light, light.glsl:401-435
+static V3 calculate_sampled_light(const Ctx &c, V3 ray_d, Hit &hit, const ct_material &mat,
+                                  const ct_light &light, float pdf, float &pdf_out, float &G,
+                                  uint32_t &seed, bool calc_pdf, bool use_pdf, bool use_vis,
+                                  TraceCount &cnt) {
+  V3 surface_normal = normalize(hit.world_nrm);
+  V3 wo = neg(normalize(ray_d));
+  pdf_out = pdf;
+  V3 result = v3(0, 0, 0);
+  G = 0.0f;
+  if (light.emissive[0] <= 0.0f && light.emissive[1] <= 0.0f && light.emissive[2] <= 0.0f)
+    return result;
+  LightSample ls;
+  bool ok = sample_lights(c, hit, light, pdf_out, ls, seed, calc_pdf, use_vis, cnt);
+  G = ls.G;
+  if (ok) {
+    V3 wi = normalize(sub(ls.l_pos, hit.world_hit));
+    V3 brdf = evaluate_material(mat, surface_normal, wo, wi);
+    hit.scatter_dir = wi;
+    V3 bl = mulv(brdf, ls.Le);
+    result = use_pdf ? divs(muls(bl, G), pdf_out) : muls(bl, G);
+  }
+  return result;
+}
+
+// reservoir_get_radiance, reservoir.glsl:98-112
+static V3 reservoir_get_radiance(const Ctx &c, const Res &r, V3 ray_d, Hit hit,
+                                 const ct_material &mat, TraceCount &cnt) {
+  if (!res_valid(r)) return v3(0, 0, 0);
+  const ct_light *l = light_by_type(c, r.l_type, r.Y);
+  if (!l) return v3(0, 0, 0);
+  float pdf_out, G;
+  uint32_t s = r.seed;
+  return calculate_sampled_light(c, ray_d, hit, mat, *l, 1.0f, pdf_out, G, s, false, false,
+                                 true, cnt);
+}
+
+// reservoir_visibility_pass, reservoir.glsl:76-96, 114-126
+static void reservoir_visibility_pass(const Ctx &c, Res &r, const Hit &hit, TraceCount &cnt) {
+  bool visibility = false;
+  if (res_valid(r)) {
+    const ct_light *l = light_by_type(c, r.l_type, r.Y);
+    if (l) {
+      float l_pdf = 1.0f;
+      uint32_t s = r.seed;
+      LightSample ls;
+      visibility = sample_lights(c, hit, *l, l_pdf, ls, s, false, true, cnt);
+    }
+  }
+  r.W_y = visibility ? (r.W_sum / luminance(r.F)) : 0.0f;
+  r.F = muls(r.F, visibility ? 1.0f : 0.0f);
+}
+
+// RIS, restir_resampling.glsl:19-255 (point, environment and cube
+// strategies; the BRDF strategy is rejected by the caller).
+static Res ris(const Ctx &c, float confidence, V3 ray_d, Hit &hit, const ct_material &mat,
+               uint32_t &seed, TraceCount &cnt) {
+  const ct_restir_params &rp = c.p->restir;
+  const ct_light_config &lc = c.sc->cfg;
+  uint32_t active = c.p->is_active;
+  confidence = clampf(confidence, 0.0f, 1.0f);
+  bool point_active = (active & CT_RIS_POINT_LIGHT_BIT) != 0;
+  bool env_active = (active & CT_RIS_ENV_LIGHT_BIT) != 0;
+  bool cube_active = (active & CT_RIS_CUBE_LIGHT_BIT) != 0;
+  uint32_t point_light_count = lc.point_light_count;
+  uint32_t n_point = std::max(
+      f2u_sat(gmin((float)rp.max_ris_point_samples * (1.0f - confidence), (float)point_light_count)),
+      rp.min_ris_point_samples);
+  uint32_t env_light_count = lc.env_map_count;
+  uint32_t n_env = std::max(
+      f2u_sat(gmin((float)rp.max_ris_env_samples * (1.0f - confidence), (float)env_light_count)),
+      rp.min_ris_env_samples);
+  uint32_t cube_light_count = lc.cube_light_count < 1 ? 1 : lc.cube_light_count;
+  uint32_t n_cube = std::max(
+      f2u_sat(gmin((float)rp.max_ris_cube_samples * (1.0f - confidence), (float)cube_light_count)),
+      rp.min_ris_cube_samples);
+  uint32_t brdf_count = lc.brdf_count;
+  uint32_t n_brdf = brdf_count + n_env;
+  uint32_t nee = n_point + n_cube;
+  Res reservoir;
+  init_res(reservoir);
+  auto strategy = [&](Res &target, uint32_t count, uint32_t nsamp, uint32_t type,
+                      const std::vector<ct_light> &lights) {
+    float pdf = 1.0f / (float)count;
+    for (uint32_t l = 0; l < nsamp; ++l) {
+      uint32_t idx = std::min(urnd_interval(seed, 0, count), count - 1);
+      const ct_light &light = lights[idx];
+      float light_pdf = 1.0f, G = 0.0f;
+      uint32_t current_seed = seed;
+      V3 F = calculate_sampled_light(c, ray_d, hit, mat, light, pdf, light_pdf, G, seed, true,
+                                     false, false, cnt);
+      float brdf_pdf = sample_material_pdf(mat, hit.world_nrm, neg(ray_d), hit.scatter_dir) * G;
+      float m_i = 1.0f / gmax((float)nee * light_pdf + (float)n_brdf * brdf_pdf, 1e-6f);
+      float w_i = luminance(F) * m_i;
+      update_reservoir(target, idx, type, current_seed, F, w_i, 1.0f, seed);
+    }
+  };
+  if (point_active && !c.sc->point.empty()) strategy(reservoir, point_light_count, n_point, CT_LIGHT_POINT, c.sc->point);
+  if (env_active && !c.sc->env.empty()) {
+    Res er;
+    init_res(er);
+    strategy(er, env_light_count, n_env, CT_LIGHT_ENV_MAP, c.sc->env);
+    update_reservoir(reservoir, er.Y, er.l_type, er.seed, er.F, er.W_sum, er.M, seed);
+  }
+  if (cube_active && !c.sc->cube.empty()) {
+    Res cr;
+    init_res(cr);
+    strategy(cr, cube_light_count, n_cube, CT_LIGHT_CUBE, c.sc->cube);
+    update_reservoir(reservoir, cr.Y, cr.l_type, cr.seed, cr.F, cr.W_sum, cr.M, seed);
+  }
+  reservoir_visibility_pass(c, reservoir, hit, cnt);
+  return reservoir;
+}
+
+static inline Hit hit_from_di(const ct_di_info &d) {
+  Hit h;
+  V3 n = load3(d.normal);
+  V3 p = compute_ray_origin(load3(d.position), n);
+  p = compute_ray_origin(p, n);
+  h.world_hit = p;
+  h.world_nrm = n;
+  h.distance = d.distance;
+  h.scatter_dir = v3(0, 0, 0);
+  h.inst = d.instance_hit.instance_id;
+  h.prim = d.instance_hit.primitive_id;
+  h.mat_idx = d.mat_index;
+  return h;
+}
+
+// TEMPORAL_REUSE, restir_resampling.glsl:311-416 (remapping inactive)
+static void temporal_reuse(const Ctx &c, Res &r, Res prev, uint32_t pidx, V3 ray_d, Hit &hit,
+                           const ct_material &mat, uint32_t &seed, TraceCount &cnt) {
+  const float mcap = c.p->restir.max_influence_from_the_past;
+  prev.M = gmin(prev.M, mcap * r.M);
+  const float prev_M = prev.M;
+  const float new_M = r.M + prev_M;
+  if (r.W_sum > 0.0f) {
+    float target_lum_at_prev = 0.0f;
+    if (luminance(r.F) > 1e-6f) {
+      const ct_di_info &dp = c.fr->di_prev[pidx];
+      Hit ph = hit_from_di(dp);
+      const ct_material &pm = material(c, dp.mat_index);
+      V3 pd = normalize(sub(ph.world_hit, load3(dp.ray_origin)));
+      target_lum_at_prev = luminance(reservoir_get_radiance(c, r, pd, ph, pm, cnt));
+    }
+    const float p_curr = r.M * luminance(r.F);
+    const float m_curr = p_curr / gmax(p_curr + prev_M * target_lum_at_prev, 1e-6f);
+    r.W_sum *= m_curr;
+  }
+  uint32_t prev_light_index = prev.Y;
+  if (prev_light_index != kInvalid) {
+    V3 current_target = reservoir_get_radiance(c, prev, ray_d, hit, mat, cnt);
+    const float lc = luminance(current_target);
+    if (lc > 0.0f) {
+      const float lp = luminance(prev.F);
+      const float p_prev = prev.M * lp;
+      const float m_prev = p_prev / gmax(p_prev + r.M * lc, 1e-6f);
+      const float w_prev = m_prev * lc * prev.W_y;
+      update_reservoir(r, prev_light_index, prev.l_type, prev.seed, current_target, w_prev, 1.0f,
+                       seed);
+    }
+  }
+  float tl = luminance(r.F);
+  r.W_y = tl > 0.0f ? r.W_sum / tl : 0.0f;
+  r.M = new_M;
+}
+
+// pairwise_mis.glsl:8-114 and SPATIAL_REUSE, restir_resampling.glsl:418-540
+static void spatial_reuse(const Ctx &c, Res &r, float confidence, uint32_t x, uint32_t y,
+                          V3 ray_d, Hit &hit, const ct_material &mat, uint32_t &seed,
+                          TraceCount &cnt) {
+  const ct_restir_params &rp = c.p->restir;
+  const uint32_t W = c.p->width, H = c.p->height;
+  if (r.W_y == 0.0f) confidence = 0.0f;
+  float radius = mixf(rp.max_neighbor_radius, rp.min_neighbor_radius,
+                      clampf(hit.distance / rp.max_distance_to_hit, 0.0f, 1.0f));
+  uint32_t k = f2u_sat(mixf((float)rp.max_neighbors, (float)rp.min_neighbors,
+                            clampf(confidence, 0.0f, 1.0f)));
+  uint32_t idx[CT_MAX_NEIGHBORS];
+  uint32_t num = 0;
+  for (uint32_t i = 0; i < k; ++i) {
+    float ox = 2.0f * rnd(seed) - 1.0f;
+    float oy = 2.0f * rnd(seed) - 1.0f;
+    int64_t nx = (int64_t)x + f2i_sat(ox * radius);
+    int64_t ny = (int64_t)y + f2i_sat(oy * radius);
+    if (nx < 0 || nx >= (int64_t)W || ny < 0 || ny >= (int64_t)H) continue;
+    if (nx == (int64_t)x && ny == (int64_t)y) continue;
+    uint32_t l = (uint32_t)ny * W + (uint32_t)nx;
+    if (c.fr->di_cur[l].distance <= 0.0f) continue;
+    if (num < CT_MAX_NEIGHBORS) idx[num++] = l;
+  }
+  if (num == 0) return;
+  // pairwise_init
+  Res mres;
+  init_res(mres);
+  float m_c = 1.0f;
+  float M_s = r.M;
+  const float fk = (float)num;
+  for (uint32_t i = 0; i < num; ++i) {
+    Res in = res_from(c.fr->res_int[idx[i]]);
+    const ct_di_info &nd = c.fr->di_cur[idx[i]];
+    Hit nh = hit_from_di(nd);
+    V3 nray_d = normalize(sub(nh.world_hit, load3(nd.ray_origin)));
+    const ct_material &nm = material(c, nd.mat_index);
+    // pairwise_stream
+    V3 curr_target = v3(0, 0, 0);
+    float m_i = 0.0f;
+    if (res_valid(in)) {
+      curr_target = reservoir_get_radiance(c, in, ray_d, hit, mat, cnt);
+      const float tl = luminance(curr_target);
+      const float p_i_y_i = luminance(in.F);
+      float mm = in.M * p_i_y_i;
+      float den = mm + (r.M / fk) * tl;
+      m_i = den > 0.0f ? mm / den : 0.0f;
+    }
+    V3 input_target = v3(0, 0, 0);
+    if (res_valid(r)) input_target = reservoir_get_radiance(c, r, nray_d, nh, nm, cnt);
+    {
+      const float p_i_y_c = luminance(input_target);
+      const float p_c_y_c = luminance(r.F);
+      const float numer = in.M * p_i_y_c;
+      const bool den_ok = (p_c_y_c + numer) > 0.0f;
+      m_c += 1.0f;
+      m_c -= den_ok ? numer / (numer + (r.M / fk) * p_c_y_c) : 0.0f;
+    }
+    if (res_valid(in)) {
+      const float w_i = luminance(curr_target) * in.W_y * m_i;
+      update_reservoir(mres, in.Y, in.l_type, in.seed, curr_target, w_i, 1.0f, seed);
+    }
+    M_s += in.M;
+  }
+  // pairwise_end (its seed is a copy)
+  uint32_t s2 = seed;
+  const float w_c = luminance(r.F) * r.W_y * m_c;
+  update_reservoir(mres, r.Y, r.l_type, r.seed, r.F, w_c, 1.0f, s2);
+  mres.M = M_s;
+  const float tl = luminance(mres.F);
+  mres.W_y = tl > 0.0f ? mres.W_sum / (tl * (float)(1u + num)) : 0.0f;
+  r = mres;
+}
+
+// get_ray_from_current_pixel, rgen.glsl:18-42 (jitter = 0)
+static void camera_ray(const ct_frame_params &p, uint32_t x, uint32_t y, V3 &o, V3 &d) {
+  float pcx = ((float)x + 0.0f) + 0.5f, pcy = ((float)y + 0.0f) + 0.5f;
+  float ux = pcx / (float)p.width, uy = pcy / (float)p.height;
+  float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
+  float org[4] = {0, 0, 0, 1}, ov[4];
+  mat4_mul_vec(p.inv_view, org, ov);
+  o = v3(ov[0], ov[1], ov[2]);
+  float tv[4] = {dx, dy, 1.0f, 1.0f}, t4[4];
+  mat4_mul_vec(p.inv_proj, tv, t4);
+  V3 nt = normalize(v3(t4[0], t4[1], t4[2]));
+  float dv[4] = {nt.x, nt.y, nt.z, 0.0f}, d4[4];
+  mat4_mul_vec(p.inv_view, dv, d4);
+  d = v3(d4[0], d4[1], d4[2]);
+}
+
+// The model matrix rchit.glsl:29-33 builds from gl_ObjectToWorldEXT
+// (element [0][1] where [1][0] belongs).
+static void rchit_model(const float *t, float *m) {
+  const float q[16] = {t[0], t[1], t[2], 0.0f,  t[1],  t[5],  t[6],  0.0f,
+                       t[8], t[9], t[10], 0.0f, t[12], t[13], t[14], 1.0f};
+  std::memcpy(m, q, sizeof(q));
+}
+
+// RESTIR_PREPASS_AND_FIRST_VISIBILITY_TEST, rgen.glsl:46-320
+static void prepass_pixel(const Ctx &c, uint32_t x, uint32_t y, TraceCount &cnt, TraceCount &pc) {
+  const ct_frame_params &p = *c.p;
+  Frame &fr = *c.fr;
+  const uint32_t W = p.width, H = p.height;
+  const uint32_t idx = y * W + x;
+  uint32_t seed = tea(y * W + x, p.frame_number);
+  V3 ro, rd;
+  camera_ray(p, x, y, ro, rd);
+  ClosestState hs = trace_closest(*c.sc, ro, rd, kMaxDistance, pc);
+  ct_di_info di{};
+  V3 hv = v3(1.0f, 1.0f, 1.0f);  // prd.hit_value
+  float distance = -1.0f;
+  V3 world_pos = v3(0, 0, 0), world_nrm = v3(0, 0, 0);
+  uint32_t hi = CT_MAX_INSTANCES, hp = CT_MAX_PRIMITIVES, mat_index = kInvalid;
+  if (hs.inst != kInvalid) {
+    // rchit.glsl:18-54 + packed_intersection_info, primitives.glsl:509-534
+    const ct_instance &in = c.sc->inst[hs.inst];
+    float model[16];
+    rchit_model(in.transform, model);
+    world_pos = add(ro, muls(rd, hs.t));
+    uint32_t gp = in.first_primitive_index + hs.prim;
+    const ct_aabb &bb = c.sc->aabbs[gp];
+    V3 center = muls(add(load3(bb.minimum), load3(bb.maximum)), 0.5f);
+    center = xform_point(model, center);
+    world_nrm = cube_like_normal(normalize(sub(world_pos, center)));
+    distance = length(sub(world_pos, ro));
+    mat_index = c.sc->prims[gp].material_index;
+    hi = hs.inst;
+    hp = hs.prim;
+  } else {
+    hv = mulv(hv, sky_color(p.time, p.is_afternoon != 0, rd));  // rmiss.glsl:19-29
+  }
+  store3(di.position, world_pos);
+  di.distance = distance;
+  store3(di.normal, world_nrm);
+  store3(di.ray_origin, ro);
+  di.seed = seed;
+  di.instance_hit.instance_id = hi;
+  di.instance_hit.primitive_id = hp;
+  di.mat_index = mat_index;
+  di.confidence = 1.0f;
+  const bool is_hit = di.distance > 0.0f;
+  float *img = &fr.img[fr.cur][4 * (size_t)idx];
+  if (!is_hit) {
+    store3(img, hv);
+    img[3] = 1.0f;
+    if (p.is_active & CT_TAA_BIT) {
+      float *t = &fr.taa[4 * (size_t)idx];
+      store3(t, hv);
+      t[3] = 1.0f;
+    }
+  } else {
+    // motion_vectors.glsl:70-89
+    const float *pt = &c.pt[16 * (size_t)di.instance_hit.instance_id];
+    float xi_x = (float)x + 0.5f, xi_y = (float)y + 0.5f;
+    float S[4] = {di.position[0], di.position[1], di.position[2], 1.0f}, ps[4];
+    mat4_mul_vec(pt, S, ps);
+    float x1 = ps[0] / ps[3], y1 = ps[1] / ps[3];
+    x1 = 0.5f * x1 + 0.5f;
+    y1 = 0.5f * y1 + 0.5f;
+    float mvx = x1 * (float)W - xi_x, mvy = y1 * (float)H - xi_y;
+    fr.velocity[2 * (size_t)idx] = mvx;
+    fr.velocity[2 * (size_t)idx + 1] = mvy;
+    const ct_material &mat = material(c, di.mat_index);
+    uint32_t px = f2u_sat(xi_x + mvx), py = f2u_sat(xi_y + mvy);
+    Hit hit = hit_from_di(di);
+    // GATHER_TEMPORAL_RESERVOIR, restir_resampling.glsl:269-309
+    Res prev;
+    init_res(prev);
+    uint32_t pidx = px + py * W;
+    if (pidx < W * H) prev = res_from(fr.res_prev[pidx]);
+    float predicted = clampf(prev.M / p.restir.max_influence_from_the_past, 0.0f, 1.0f);
+    float confidence = (prev.W_y > 0.0f) ? predicted : 0.0f;
+    di.confidence = confidence;
+    Res r = ris(c, confidence, rd, hit, mat, di.seed, cnt);
+    if (prev.W_y > 0.0f) temporal_reuse(c, r, prev, pidx, rd, hit, mat, di.seed, cnt);
+    fr.res_int[idx] = res_to(r);
+  }
+  fr.di_cur[idx] = di;
+}
+
+// THIRD_VISIBILITY_TEST_AND_SHADING_PASS, rgen.glsl:329-510
+static void shade_pixel(const Ctx &c, uint32_t x, uint32_t y, TraceCount &cnt) {
+  const ct_frame_params &p = *c.p;
+  Frame &fr = *c.fr;
+  const uint32_t idx = y * p.width + x;
+  ct_di_info di = fr.di_cur[idx];
+  if (di.distance > 0.0f) {
+    V3 ray_d = normalize(sub(load3(di.position), load3(di.ray_origin)));
+    const ct_material &mat = material(c, di.mat_index);
+    Hit hit = hit_from_di(di);
+    Res r = res_from(fr.res_int[idx]);
+    float confidence = di.confidence;
+    Res sr = r;
+    spatial_reuse(c, sr, confidence, x, y, ray_d, hit, mat, di.seed, cnt);
+    V3 hv = v3(0, 0, 0);
+    hv = add(hv, muls(sr.F, sr.W_y));
+    if (p.restir.direct_emittance) hv = add(hv, load3(mat.emission));
+    if (std::isinf(hv.x) || std::isnan(hv.x)) hv.x = 0.0f;
+    if (std::isinf(hv.y) || std::isnan(hv.y)) hv.y = 0.0f;
+    if (std::isinf(hv.z) || std::isnan(hv.z)) hv.z = 0.0f;
+    float *img = &fr.img[fr.cur][4 * (size_t)idx];
+    store3(img, hv);
+    img[3] = 1.0f;
+    if (p.is_active & CT_TAA_BIT) {
+      float *t = &fr.taa[4 * (size_t)idx];
+      store3(t, hv);
+      t[3] = 1.0f;
+    }
+    fr.res_prev[idx] = res_to(sr);
+    fr.di_prev[idx] = di;
+  } else {
+    fr.di_prev[idx].distance = 0.0f;
+  }
+}
+
+// taa.glsl:16-108
+static void taa_pixel(const Ctx &c, uint32_t x, uint32_t y) {
+  const ct_frame_params &p = *c.p;
+  Frame &fr = *c.fr;
+  const uint32_t W = p.width, H = p.height;
+  const uint32_t idx = x + y * W;
+  const ct_di_info &di = fr.di_cur[idx];
+  if (!(di.distance > 0.0f)) return;
+  float b[4] = {0, 0, 0, 0};
+  uint32_t count = 0;
+  V3 dn = load3(di.normal);
+  for (int i = -1; i <= 1; ++i)
+    for (int j = -1; j <= 1; ++j) {
+      int64_t nx = (int64_t)x + i, ny = (int64_t)y + j;
+      if (nx < 0 || nx >= (int64_t)W || ny < 0 || ny >= (int64_t)H) continue;
+      uint32_t ni = (uint32_t)nx + (uint32_t)ny * W;
+      const ct_di_info &nd = fr.di_cur[ni];
+      if (di.distance <= 0.0f || di.mat_index != nd.mat_index ||
+          (dot(dn, load3(nd.normal)) < 0.906f) ||
+          (std::fabs(di.distance - nd.distance) > 0.1f * di.distance))
+        continue;
+      ++count;
+      const float *t = &fr.taa[4 * (size_t)ni];
+      for (int k = 0; k < 4; ++k) b[k] = b[k] + t[k];
+    }
+  for (int k = 0; k < 4; ++k) b[k] = b[k] / (float)count;
+  float vx = fr.velocity[2 * (size_t)idx], vy = fr.velocity[2 * (size_t)idx + 1];
+  uint32_t s = di.seed;
+  float jx = rnd(s);
+  float jy = rnd(s);
+  float x1 = ((float)x + vx) + jx, y1 = ((float)y + vy) + jy;
+  int32_t ix = f2i_sat(x1), iy = f2i_sat(y1);
+  if (ix >= 0 && ix < (int32_t)W && iy >= 0 && iy < (int32_t)H) {
+    uint32_t pi = (uint32_t)ix + (uint32_t)iy * W;
+    const float *pp = &fr.img[fr.cur ^ 1][4 * (size_t)pi];
+    const ct_di_info &pd = fr.di_prev[pi];
+    bool valid = dot(load3(pd.normal), dn) >= 0.99f && pd.mat_index == di.mat_index;
+    if (valid) {
+      const float a = p.restir.taa_alpha, oma = 1.0f - p.restir.taa_alpha;
+      for (int k = 0; k < 4; ++k) b[k] = a * b[k] + oma * pp[k];
+    }
+  }
+  float *o = &fr.img[fr.cur][4 * (size_t)idx];
+  for (int k = 0; k < 4; ++k) o[k] = b[k];
+}
+
+}  // namespace
+
+// ================================================================ C API
+struct oc_scene : Scene {};
+struct oc_frame : Frame {};
+
+extern "C" {
+
+uint32_t oc_tea(uint32_t v0, uint32_t v1) { return tea(v0, v1); }
+uint32_t oc_lcg(uint32_t *state) { return lcg(*state); }
+float oc_rnd(uint32_t *state) { return rnd(*state); }
+uint32_t oc_urnd_interval(uint32_t *state, uint32_t mn, uint32_t mx) {
+  return urnd_interval(*state, mn, mx);
+}
+void oc_compute_ray_origin(const float pos[3], const float nrm[3], float out[3]) {
+  store3(out, compute_ray_origin(load3(pos), load3(nrm)));
+}
+int oc_intersect_box(const float center[3], float radius, const float origin[3],
+                     const float direction[3], int can_start_in_box, int oriented, float *t,
+                     float normal[3]) {
+  V3 rd = load3(direction);
+  V3 inv = v3(safe_inverse(rd.x), safe_inverse(rd.y), safe_inverse(rd.z));
+  V3 nrm;
+  float tt = 0.0f;
+  bool hit = intersect_box(load3(center), v3(radius, radius, radius),
+                           v3(safe_inverse(radius), safe_inverse(radius), safe_inverse(radius)),
+                           load3(origin), rd, tt, nrm, can_start_in_box != 0, oriented != 0, inv);
+  *t = tt;
+  store3(normal, nrm);
+  return hit ? 1 : 0;
+}
+void oc_cube_like_normal(const float n[3], float out[3]) {
+  store3(out, cube_like_normal(load3(n)));
+}
+void oc_sky_color(float time, int is_afternoon, const float dir[3], float out[3]) {
+  store3(out, sky_color(time, is_afternoon != 0, load3(dir)));
+}
+uint32_t oc_morton3(uint32_t qx, uint32_t qy, uint32_t qz) { return morton3(qx, qy, qz); }
+
+static void export_bvh(const Bvh &b, ct_bvh_arrays *a) {
+  uint32_t n = b.n;
+  if (a->keys) std::memcpy(a->keys, b.keys.data(), n * sizeof(uint32_t));
+  if (a->ids) std::memcpy(a->ids, b.ids.data(), n * sizeof(uint32_t));
+  if (n > 1) {
+    if (a->child) std::memcpy(a->child, b.child.data(), b.child.size() * sizeof(uint32_t));
+    if (a->boxes) std::memcpy(a->boxes, b.box.data(), b.box.size() * sizeof(float));
+  }
+  if (a->parent) std::memcpy(a->parent, b.parent.data(), b.parent.size() * sizeof(uint32_t));
+  a->n = n;
+  a->depth = b.depth;
+  a->root_ref = b.root_ref;
+}
+
+int oc_lbvh_build(uint32_t n, const float *centers, const float *lo, const float *hi,
+                  uint32_t *keys, uint32_t *ids, uint32_t *child, float *boxes,
+                  uint32_t *parent) {
+  Bvh b;
+  if (!build_lbvh(n, centers, lo, hi, b)) return -1;
+  ct_bvh_arrays a{keys, ids, child, boxes, parent, 0, 0, 0, 0};
+  export_bvh(b, &a);
+  return (int)b.depth;
+}
+
+oc_scene *oc_scene_create(uint32_t n_inst, const ct_instance *instances,
+                          const float *world_to_object, uint32_t prim_capacity,
+                          const ct_aabb *aabbs, const ct_primitive *prims, uint32_t n_mat,
+                          const ct_material *materials, uint32_t n_cube,
+                          const ct_light *cube_lights, uint32_t n_point,
+                          const ct_light *point_lights, uint32_t n_env,
+                          const ct_light *env_lights, const ct_light_config *config) {
+  if (n_inst == 0 || !instances || !world_to_object || !aabbs || !prims || !config) return nullptr;
+  for (uint32_t i = 0; i < n_mat; ++i)
+    if ((materials[i].type & CT_MATERIAL_TYPE_MASK) != CT_MATERIAL_LAMBERTIAN) return nullptr;
+  oc_scene *s = new oc_scene();
+  s->inst.assign(instances, instances + n_inst);
+  s->w2o.assign(world_to_object, world_to_object + 16 * (size_t)n_inst);
+  s->prim_capacity = prim_capacity;
+  s->aabbs.assign(aabbs, aabbs + prim_capacity);
+  s->prims.assign(prims, prims + prim_capacity);
+  if (n_mat) s->mats.assign(materials, materials + n_mat);
+  if (n_cube) s->cube.assign(cube_lights, cube_lights + n_cube);
+  if (n_point) s->point.assign(point_lights, point_lights + n_point);
+  if (n_env) s->env.assign(env_lights, env_lights + n_env);
+  s->cfg = *config;
+  for (uint32_t i = 0; i < n_inst; ++i)
+    if (instances[i].primitive_count == 0 ||
+        (uint64_t)instances[i].first_primitive_index + instances[i].primitive_count >
+            prim_capacity) {
+      delete s;
+      return nullptr;
+    }
+  build_scene_bvh(*s);
+  return s;
+}
+
+void oc_scene_destroy(oc_scene *scene) { delete scene; }
+
+int oc_scene_read_bvh(oc_scene *scene, uint32_t instance, ct_bvh_arrays *arrays) {
+  if (!scene || !arrays) return CT_E_INVALID;
+  if (instance == kInvalid) {
+    export_bvh(scene->tlas, arrays);
+    return CT_OK;
+  }
+  if (instance >= scene->blas.size()) return CT_E_INVALID;
+  export_bvh(scene->blas[instance], arrays);
+  return CT_OK;
+}
+
+static void brute_closest(const Scene &sc, V3 o, V3 d, ClosestState &st) {
+  for (uint32_t i = 0; i < sc.inst.size(); ++i) {
+    const float *m = &sc.w2o[16 * (size_t)i];
+    V3 oo = xform_point(m, o), od = xform_dir(m, d);
+    for (uint32_t k = 0; k < sc.inst[i].primitive_count; ++k) closest_leaf(sc, i, k, oo, od, st);
+  }
+}
+static bool brute_any(const Scene &sc, V3 o, V3 d, float dist, uint32_t ti, uint32_t tp) {
+  for (uint32_t i = 0; i < sc.inst.size(); ++i) {
+    const float *m = &sc.w2o[16 * (size_t)i];
+    V3 oo = xform_point(m, o), od = xform_dir(m, d);
+    for (uint32_t k = 0; k < sc.inst[i].primitive_count; ++k)
+      if (any_leaf(sc, i, k, oo, od, dist, ti, tp)) return true;
+  }
+  return false;
+}
+
+void oc_trace_closest(oc_scene *scene, uint32_t n, const float *origins, const float *directions,
+                      float tmax, int brute, float *t_out, uint32_t *ids_out,
+                      ct_trace_stats *stats) {
+  uint64_t rays = 0, nodes = 0, leaves = 0, insts = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : rays, nodes, leaves, insts)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    V3 o = load3(&origins[3 * i]), d = load3(&directions[3 * i]);
+    ClosestState st{tmax, kInvalid, kInvalid};
+    TraceCount cnt;
+    if (brute) {
+      brute_closest(*scene, o, d, st);
+      cnt.rays = 1;
+    } else {
+      st = trace_closest(*scene, o, d, tmax, cnt);
+    }
+    bool hit = st.inst != kInvalid;
+    t_out[i] = hit ? st.t : -1.0f;
+    ids_out[2 * i] = hit ? st.inst : kInvalid;
+    ids_out[2 * i + 1] = hit ? st.prim : kInvalid;
+    rays += cnt.rays; nodes += cnt.nodes; leaves += cnt.leaves; insts += cnt.instances;
+  }
+  if (stats) {
+    stats->rays = rays; stats->nodes = nodes; stats->leaves = leaves; stats->instances = insts;
+  }
+}
+
+void oc_trace_any(oc_scene *scene, uint32_t n, const float *origins, const float *directions,
+                  const float *distances, const uint32_t *targets, int brute,
+                  uint8_t *occluded_out, ct_trace_stats *stats) {
+  uint64_t rays = 0, nodes = 0, leaves = 0, insts = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : rays, nodes, leaves, insts)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    V3 o = load3(&origins[3 * i]), d = load3(&directions[3 * i]);
+    uint32_t ti = targets ? targets[2 * i] : kInvalid, tp = targets ? targets[2 * i + 1] : kInvalid;
+    TraceCount cnt;
+    bool occ;
+    if (brute) {
+      occ = brute_any(*scene, o, d, distances[i], ti, tp);
+      cnt.rays = 1;
+    } else {
+      occ = trace_any(*scene, o, d, distances[i], ti, tp, cnt);
+    }
+    occluded_out[i] = occ ? 1 : 0;
+    rays += cnt.rays; nodes += cnt.nodes; leaves += cnt.leaves; insts += cnt.instances;
+  }
+  if (stats) {
+    stats->rays = rays; stats->nodes = nodes; stats->leaves = leaves; stats->instances = insts;
+  }
+}
+
+oc_frame *oc_frame_create(oc_scene *scene, uint32_t width, uint32_t height) {
+  if (!scene || width == 0 || height == 0) return nullptr;
+  oc_frame *f = new oc_frame();
+  f->sc = scene;
+  f->W = width;
+  f->H = height;
+  size_t np = (size_t)width * height;
+  f->di_cur.assign(np, ct_di_info{});
+  f->di_prev.assign(np, ct_di_info{});
+  f->res_int.assign(np, ct_reservoir{});
+  f->res_prev.assign(np, ct_reservoir{});
+  f->velocity.assign(2 * np, 0.0f);
+  f->img[0].assign(4 * np, 0.0f);
+  f->img[1].assign(4 * np, 0.0f);
+  f->taa.assign(4 * np, 0.0f);
+  f->cur = 1;
+  return f;
+}
+
+void oc_frame_destroy(oc_frame *frame) { delete frame; }
+
+int oc_frame_render(oc_frame *frame, const ct_frame_params *params, int threads,
+                    uint32_t row_begin, uint32_t row_end) {
+  if (!frame || !params) return CT_E_INVALID;
+  const ct_frame_params &p = *params;
+  if (p.width != frame->W || p.height != frame->H) return CT_E_INVALID;
+  if (p.is_active & CT_RIS_BRDF_BIT) return CT_E_UNSUPPORTED;
+  if (p.restir.max_neighbors > CT_MAX_NEIGHBORS || p.restir.min_neighbors > CT_MAX_NEIGHBORS)
+    return CT_E_INVALID;
+  if ((p.is_active & CT_TAA_BIT) && !p.has_previous) return CT_E_INVALID;
+  if (row_end > p.height || row_begin > row_end) return CT_E_INVALID;
+  Scene &sc = *frame->sc;
+  Ctx c;
+  c.sc = &sc;
+  c.fr = frame;
+  c.p = params;
+  const size_t ni = sc.inst.size();
+  c.pt.resize(16 * ni);
+  for (size_t i = 0; i < ni; ++i) {
+    float g[16];
+    // geometry_T = previous_model * inverse(instance_model), motion_vectors.glsl:146
+    mat4_mul(sc.inst[i].transform, &sc.w2o[16 * i], g);
+    mat4_mul(p.prev_view_proj, g, &c.pt[16 * i]);
+  }
+  frame->cur ^= 1;
+#ifdef _OPENMP
+  int nt = threads > 0 ? threads : omp_get_max_threads();
+#else
+  int nt = 1;
+  (void)threads;
+#endif
+  const uint32_t W = p.width;
+  const int64_t rows = (int64_t)(row_end - row_begin);
+  const int64_t np = rows * (int64_t)W;
+  uint64_t nodes = 0, leaves = 0, insts = 0, cnodes = 0, cleaves = 0, cinsts = 0;
+  uint64_t prim_rays = 0, sh_prepass = 0, sh_shade = 0;
+#pragma omp parallel num_threads(nt) reduction(+ : nodes, leaves, insts, cnodes, cleaves, cinsts, prim_rays, sh_prepass)
+  {
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t k = 0; k < np; ++k) {
+      uint32_t y = row_begin + (uint32_t)(k / W), x = (uint32_t)(k % W);
+      TraceCount cnt, pc;
+      prepass_pixel(c, x, y, cnt, pc);
+      prim_rays += 1;
+      sh_prepass += cnt.rays;
+      nodes += cnt.nodes; leaves += cnt.leaves; insts += cnt.instances;
+      cnodes += pc.nodes; cleaves += pc.leaves; cinsts += pc.instances;
+    }
+  }
+#pragma omp parallel num_threads(nt) reduction(+ : nodes, leaves, insts, sh_shade)
+  {
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t k = 0; k < np; ++k) {
+      uint32_t y = row_begin + (uint32_t)(k / W), x = (uint32_t)(k % W);
+      TraceCount cnt;
+      shade_pixel(c, x, y, cnt);
+      sh_shade += cnt.rays;
+      nodes += cnt.nodes; leaves += cnt.leaves; insts += cnt.instances;
+    }
+  }
+  if (p.is_active & CT_TAA_BIT) {
+    uint32_t tw = p.restir.taa_truncate_dispatch ? (p.width / 8) * 8 : p.width;
+    uint32_t th = p.restir.taa_truncate_dispatch ? (p.height / 8) * 8 : p.height;
+    th = std::min(th, row_end);
+    uint32_t tb = row_begin;
+    const int64_t tn = (int64_t)(th > tb ? th - tb : 0) * tw;
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 64)
+    for (int64_t k = 0; k < tn; ++k) taa_pixel(c, (uint32_t)(k % tw), tb + (uint32_t)(k / tw));
+  }
+  ct_frame_stats &s = frame->stats;
+  s = ct_frame_stats{};
+  s.primary_rays = prim_rays;
+  s.shadow_rays = sh_prepass + sh_shade;
+  s.shadow_rays_spatial = sh_shade;
+  s.nodes = nodes;
+  s.leaves = leaves;
+  s.instances = insts;
+  s.closest_nodes = cnodes;
+  s.closest_leaves = cleaves;
+  s.closest_instances = cinsts;
+  return CT_OK;
+}
+
+int oc_frame_read(oc_frame *frame, uint32_t which, void *dst, uint64_t bytes) {
+  if (!frame || !dst) return CT_E_INVALID;
+  size_t np = (size_t)frame->W * frame->H;
+  const void *src = nullptr;
+  size_t sz = 0;
+  switch (which) {
+    case CT_BUF_IMAGE: src = frame->img[frame->cur].data(); sz = 16 * np; break;
+    case CT_BUF_PREV_IMAGE: src = frame->img[frame->cur ^ 1].data(); sz = 16 * np; break;
+    case CT_BUF_TAA_FRAME: src = frame->taa.data(); sz = 16 * np; break;
+    case CT_BUF_DI: src = frame->di_cur.data(); sz = sizeof(ct_di_info) * np; break;
+    case CT_BUF_PREV_DI: src = frame->di_prev.data(); sz = sizeof(ct_di_info) * np; break;
+    case CT_BUF_INT_RESERVOIR: src = frame->res_int.data(); sz = sizeof(ct_reservoir) * np; break;
+    case CT_BUF_PREV_RESERVOIR: src = frame->res_prev.data(); sz = sizeof(ct_reservoir) * np; break;
+    case CT_BUF_VELOCITY: src = frame->velocity.data(); sz = 8 * np; break;
+    default: return CT_E_INVALID;
+  }
+  if (bytes < sz) return CT_E_INVALID;
+  std::memcpy(dst, src, sz);
+  return CT_OK;
+}
+
+int oc_frame_get_stats(oc_frame *frame, ct_frame_stats *out) {
+  if (!frame || !out) return CT_E_INVALID;
+  *out = frame->stats;
+  return CT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+/*
+ * ct_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * C API of the CPU oracle: a plain-C++ restatement of cube-tracing's
+ * per-frame ray-tracing path (the GLSL shaders under src/shaders/glsl and
+ * the host logic of src/main.cpp / src/accel_struct_mngr.cpp).  Only the
+ * tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it,
+ * and only as the checker; the product library never links or calls it.
+ *
+ * Boundary structs (ct_instance, ct_aabb, ct_reservoir, ...) come from
+ * include/ct_abi.h so both sides agree on the reference's scalar layouts.
+ */
+#ifndef CT_ORACLE_H
+#define CT_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/ct_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* random.glsl:26-77, prng.glsl:198-215 */
+uint32_t oc_tea(uint32_t v0, uint32_t v1);
+uint32_t oc_lcg(uint32_t *state);
+float oc_rnd(uint32_t *state);
+uint32_t oc_urnd_interval(uint32_t *state, uint32_t mn, uint32_t mx);
+
+/* primitives.glsl:233-257 */
+void oc_compute_ray_origin(const float pos[3], const float nrm[3], float out[3]);
+/* primitives.glsl:273-355 with a unit-rotation Box; returns hit (0/1). */
+int oc_intersect_box(const float center[3], float radius, const float origin[3],
+                     const float direction[3], int can_start_in_box,
+                     int oriented, float *t, float normal[3]);
+/* primitives.glsl:499-507 */
+void oc_cube_like_normal(const float n[3], float out[3]);
+/* mat.glsl:137-202 */
+void oc_sky_color(float time, int is_afternoon, const float dir[3], float out[3]);
+
+/* 30-bit Morton code of three 10-bit cells. */
+uint32_t oc_morton3(uint32_t qx, uint32_t qy, uint32_t qz);
+
+/* LBVH over n items: keys from the item centres, bounds from lo/hi.
+ * Outputs (each may be NULL): keys[n], ids[n], child[2(n-1)],
+ * boxes[12(n-1)], parent[2n-1]; returns depth (internal nodes on the
+ * longest root-to-leaf path) or -1 on bad input. */
+int oc_lbvh_build(uint32_t n, const float *centers, const float *lo,
+                  const float *hi, uint32_t *keys, uint32_t *ids,
+                  uint32_t *child, float *boxes, uint32_t *parent);
+
+typedef struct oc_scene oc_scene;
+/* world_to_object: [n_inst][16]; prims/aabbs indexed by
+ * first_primitive_index + primitive id (capacity prim_capacity). */
+oc_scene *oc_scene_create(uint32_t n_inst, const ct_instance *instances,
+                          const float *world_to_object, uint32_t prim_capacity,
+                          const ct_aabb *aabbs, const ct_primitive *prims,
+                          uint32_t n_mat, const ct_material *materials,
+                          uint32_t n_cube, const ct_light *cube_lights,
+                          uint32_t n_point, const ct_light *point_lights,
+                          uint32_t n_env, const ct_light *env_lights,
+                          const ct_light_config *config);
+void oc_scene_destroy(oc_scene *scene);
+/* Arrays of one BLAS (instance) or of the TLAS (CT_INVALID_INDEX). */
+int oc_scene_read_bvh(oc_scene *scene, uint32_t instance, ct_bvh_arrays *arrays);
+
+/* brute != 0: test every primitive (no BVH).  stats may be NULL. */
+void oc_trace_closest(oc_scene *scene, uint32_t n, const float *origins,
+                      const float *directions, float tmax, int brute,
+                      float *t_out, uint32_t *ids_out, ct_trace_stats *stats);
+void oc_trace_any(oc_scene *scene, uint32_t n, const float *origins,
+                  const float *directions, const float *distances,
+                  const uint32_t *targets, int brute, uint8_t *occluded_out,
+                  ct_trace_stats *stats);
+
+typedef struct oc_frame oc_frame;
+oc_frame *oc_frame_create(oc_scene *scene, uint32_t width, uint32_t height);
+void oc_frame_destroy(oc_frame *frame);
+/* One frame (prepass, shading, TAA) with up to `threads` OpenMP threads
+ * (0 = default).  Returns CT_OK or a negative CT_E_* code.
+ * row_begin/row_end restrict the prepass and shading passes to a band of
+ * rows (used only for the bounded CPU-baseline sample); pass 0 and height
+ * for a full frame. */
+int oc_frame_render(oc_frame *frame, const ct_frame_params *params,
+                    int threads, uint32_t row_begin, uint32_t row_end);
+/* Same buffer ids and layouts as ct_frame_read. */
+int oc_frame_read(oc_frame *frame, uint32_t which, void *dst, uint64_t bytes);
+int oc_frame_get_stats(oc_frame *frame, ct_frame_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,282 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the CPU oracle (liboracle.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
+this module, as the checker.  The boundary struct layouts are the ones of
+include/ct_abi.h (taken from the ctamd binding, which mirrors that header).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_ROOT = os.path.dirname(_HERE)
+sys.path.insert(0, os.path.join(_ROOT, "cube-tracing_amd"))
+import ctamd  # noqa: E402  (struct layouts only)
+from ctamd import (Aabb, Instance, Primitive, Material, Light, LightConfig, FrameParams,  # noqa: E402
+                   FrameStats, TraceStats, BvhArrays, Reservoir, DiInfo)
+
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+f32, u32 = C.c_float, C.c_uint32
+
+
+def build():
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        sig = {
+            "oc_tea": ([u32, u32], u32),
+            "oc_lcg": ([P(u32)], u32),
+            "oc_rnd": ([P(u32)], f32),
+            "oc_urnd_interval": ([P(u32), u32, u32], u32),
+            "oc_compute_ray_origin": ([P(f32), P(f32), P(f32)], None),
+            "oc_intersect_box": ([P(f32), f32, P(f32), P(f32), C.c_int, C.c_int, P(f32), P(f32)], C.c_int),
+            "oc_cube_like_normal": ([P(f32), P(f32)], None),
+            "oc_sky_color": ([f32, C.c_int, P(f32), P(f32)], None),
+            "oc_morton3": ([u32, u32, u32], u32),
+            "oc_lbvh_build": ([u32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_void_p], C.c_int),
+            "oc_scene_create": ([u32, P(Instance), C.c_void_p, u32, C.c_void_p, C.c_void_p, u32, P(Material),
+                                 u32, P(Light), u32, P(Light), u32, P(Light), P(LightConfig)], C.c_void_p),
+            "oc_scene_destroy": ([C.c_void_p], None),
+            "oc_scene_read_bvh": ([C.c_void_p, u32, P(BvhArrays)], C.c_int),
+            "oc_trace_closest": ([C.c_void_p, u32, C.c_void_p, C.c_void_p, f32, C.c_int, C.c_void_p, C.c_void_p,
+                                  P(TraceStats)], None),
+            "oc_trace_any": ([C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                              C.c_void_p, P(TraceStats)], None),
+            "oc_frame_create": ([C.c_void_p, u32, u32], C.c_void_p),
+            "oc_frame_destroy": ([C.c_void_p], None),
+            "oc_frame_render": ([C.c_void_p, P(FrameParams), C.c_int, u32, u32], C.c_int),
+            "oc_frame_read": ([C.c_void_p, u32, C.c_void_p, C.c_uint64], C.c_int),
+            "oc_frame_get_stats": ([C.c_void_p, P(FrameStats)], C.c_int),
+        }
+        for name, (a, r) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = a
+            fn.restype = r
+        _lib = L
+    return _lib
+
+
+def _f3(v):
+    return (f32 * 3)(*[float(x) for x in v])
+
+
+def tea(v0, v1):
+    return lib().oc_tea(v0 & 0xFFFFFFFF, v1 & 0xFFFFFFFF)
+
+
+def lcg(state):
+    s = u32(state & 0xFFFFFFFF)
+    r = lib().oc_lcg(C.byref(s))
+    return r, s.value
+
+
+def rnd(state):
+    s = u32(state & 0xFFFFFFFF)
+    r = lib().oc_rnd(C.byref(s))
+    return r, s.value
+
+
+def urnd_interval(state, mn, mx):
+    s = u32(state & 0xFFFFFFFF)
+    r = lib().oc_urnd_interval(C.byref(s), mn, mx)
+    return r, s.value
+
+
+def compute_ray_origin(pos, nrm):
+    o = (f32 * 3)()
+    lib().oc_compute_ray_origin(_f3(pos), _f3(nrm), o)
+    return np.array(o[:], np.float32)
+
+
+def intersect_box(center, radius, origin, direction, can_start=True, oriented=True):
+    t = f32()
+    n = (f32 * 3)()
+    hit = lib().oc_intersect_box(_f3(center), radius, _f3(origin), _f3(direction), int(can_start), int(oriented),
+                                 C.byref(t), n)
+    return bool(hit), t.value, np.array(n[:], np.float32)
+
+
+def cube_like_normal(n):
+    o = (f32 * 3)()
+    lib().oc_cube_like_normal(_f3(n), o)
+    return np.array(o[:], np.float32)
+
+
+def sky_color(time, afternoon, d):
+    o = (f32 * 3)()
+    lib().oc_sky_color(time, int(afternoon), _f3(d), o)
+    return np.array(o[:], np.float32)
+
+
+def morton3(x, y, z):
+    return lib().oc_morton3(x, y, z)
+
+
+def lbvh_build(centers, lo, hi):
+    c = np.ascontiguousarray(centers, np.float32)
+    lo = np.ascontiguousarray(lo, np.float32)
+    hi = np.ascontiguousarray(hi, np.float32)
+    n = c.shape[0]
+    keys = np.zeros(n, np.uint32)
+    ids = np.zeros(n, np.uint32)
+    child = np.zeros(max(2 * (n - 1), 1), np.uint32)
+    boxes = np.zeros(max(12 * (n - 1), 1), np.float32)
+    parent = np.zeros(2 * n - 1, np.uint32)
+    depth = lib().oc_lbvh_build(n, c.ctypes.data, lo.ctypes.data, hi.ctypes.data, keys.ctypes.data, ids.ctypes.data,
+                                child.ctypes.data, boxes.ctypes.data, parent.ctypes.data)
+    return {"keys": keys, "ids": ids, "child": child[: 2 * (n - 1)], "boxes": boxes[: 12 * (n - 1)],
+            "parent": parent, "depth": depth}
+
+
+def light_config_for(n_cube, n_point=0, n_env=0):
+    """load_lights (main.cpp:418-438) with brdf_count = 1 when cube lights
+    exist (main.cpp:1317-1320); pdfs use the reference's integer division."""
+    c = LightConfig()
+    c.point_light_count, c.cube_light_count, c.env_map_count = n_point, n_cube, n_env
+    c.brdf_count = 1 if n_cube > 0 else 0
+    c.light_count = n_point + n_cube + n_env
+    ipdf = (lambda k: 0.0 if k == 0 else float(k // c.light_count))
+    c.point_light_pdf, c.cube_light_pdf, c.env_map_pdf = ipdf(n_point), ipdf(n_cube), ipdf(n_env)
+    c.brdf_pdf = 0.0 if (c.brdf_count == 0 or n_cube == 0) else float(c.brdf_count // n_cube)
+    return c
+
+
+class OScene:
+    """Oracle scene: instances, voxels, materials, lights and its own LBVH."""
+
+    def __init__(self, instances, world_to_object, aabbs, primitives, materials, cube_lights,
+                 point_lights=(), env_lights=(), config=None):
+        self.n_inst = len(instances)
+        self._inst = (Instance * self.n_inst)(*instances)
+        self._w2o = np.ascontiguousarray(world_to_object, np.float32).reshape(self.n_inst, 16)
+        self._aabbs = np.ascontiguousarray(aabbs, np.float32).reshape(-1, 6)
+        self._prims = np.ascontiguousarray(primitives, np.uint32).reshape(-1, 2)
+        self._mats = (Material * len(materials))(*materials)
+        self._cube = (Light * max(len(cube_lights), 1))(*cube_lights)
+        self._point = (Light * max(len(point_lights), 1))(*point_lights)
+        self._env = (Light * max(len(env_lights), 1))(*env_lights)
+        self.config = config if config is not None else light_config_for(len(cube_lights), len(point_lights),
+                                                                          len(env_lights))
+        self.handle = lib().oc_scene_create(
+            self.n_inst, self._inst, self._w2o.ctypes.data, self._aabbs.shape[0], self._aabbs.ctypes.data,
+            self._prims.ctypes.data, len(materials), self._mats, len(cube_lights), self._cube, len(point_lights),
+            self._point, len(env_lights), self._env, C.byref(self.config))
+        if not self.handle:
+            raise ValueError("oc_scene_create rejected the scene")
+
+    @classmethod
+    def from_scene(cls, scene, world_to_object=None, **kw):
+        ni = scene.info.instance_count
+        w2o = world_to_object
+        if w2o is None:
+            w2o = np.tile(np.eye(4, dtype=np.float32).reshape(16), (ni, 1))
+        return cls(list(scene.instances), w2o, scene.aabbs, scene.primitives, list(scene.materials),
+                   list(scene.lights), **kw)
+
+    def read_bvh(self, instance, n):
+        keys = np.zeros(n, np.uint32)
+        ids = np.zeros(n, np.uint32)
+        child = np.zeros(max(2 * (n - 1), 1), np.uint32)
+        boxes = np.zeros(max(12 * (n - 1), 1), np.float32)
+        parent = np.zeros(2 * n - 1, np.uint32)
+        a = BvhArrays(keys.ctypes.data_as(C.POINTER(u32)), ids.ctypes.data_as(C.POINTER(u32)),
+                      child.ctypes.data_as(C.POINTER(u32)), boxes.ctypes.data_as(C.POINTER(f32)),
+                      parent.ctypes.data_as(C.POINTER(u32)), 0, 0, 0, 0)
+        rc = lib().oc_scene_read_bvh(self.handle, instance, C.byref(a))
+        assert rc == 0
+        return {"keys": keys, "ids": ids, "child": child[: 2 * (n - 1)], "boxes": boxes[: 12 * (n - 1)],
+                "parent": parent, "depth": a.depth, "root_ref": a.root_ref, "n": a.n}
+
+    def trace_closest(self, origins, directions, tmax=1e9, brute=False, stats=False):
+        o = np.ascontiguousarray(origins, np.float32)
+        d = np.ascontiguousarray(directions, np.float32)
+        n = o.shape[0]
+        t = np.zeros(n, np.float32)
+        ids = np.zeros((n, 2), np.uint32)
+        st = TraceStats()
+        lib().oc_trace_closest(self.handle, n, o.ctypes.data, d.ctypes.data, tmax, int(brute), t.ctypes.data,
+                               ids.ctypes.data, C.byref(st))
+        return (t, ids, st) if stats else (t, ids)
+
+    def trace_any(self, origins, directions, distances, targets=None, brute=False, stats=False):
+        o = np.ascontiguousarray(origins, np.float32)
+        d = np.ascontiguousarray(directions, np.float32)
+        dist = np.ascontiguousarray(distances, np.float32)
+        n = o.shape[0]
+        tg = None if targets is None else np.ascontiguousarray(targets, np.uint32)
+        occ = np.zeros(n, np.uint8)
+        st = TraceStats()
+        lib().oc_trace_any(self.handle, n, o.ctypes.data, d.ctypes.data, dist.ctypes.data,
+                           None if tg is None else tg.ctypes.data, int(brute), occ.ctypes.data, C.byref(st))
+        return (occ, st) if stats else occ
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().oc_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class OFrame:
+    def __init__(self, scene: OScene, w, h):
+        self.scene = scene
+        self.w, self.h = w, h
+        self.handle = lib().oc_frame_create(scene.handle, w, h)
+
+    def render(self, params, threads=0, rows=None):
+        r0, r1 = (0, self.h) if rows is None else rows
+        rc = lib().oc_frame_render(self.handle, C.byref(params), threads, r0, r1)
+        if rc != 0:
+            raise RuntimeError(f"oc_frame_render failed: {rc}")
+
+    def read(self, which):
+        n = self.w * self.h
+        if which in (ctamd.CT_BUF_IMAGE, ctamd.CT_BUF_PREV_IMAGE, ctamd.CT_BUF_TAA_FRAME):
+            out = np.zeros((n, 4), np.float32)
+        elif which == ctamd.CT_BUF_VELOCITY:
+            out = np.zeros((n, 2), np.float32)
+        elif which in (ctamd.CT_BUF_DI, ctamd.CT_BUF_PREV_DI):
+            out = (DiInfo * n)()
+        else:
+            out = (Reservoir * n)()
+        nbytes = out.nbytes if isinstance(out, np.ndarray) else C.sizeof(out)
+        ptr = out.ctypes.data if isinstance(out, np.ndarray) else C.addressof(out)
+        rc = lib().oc_frame_read(self.handle, which, ptr, nbytes)
+        assert rc == 0
+        return out
+
+    def stats(self):
+        s = FrameStats()
+        lib().oc_frame_get_stats(self.handle, C.byref(s))
+        return s
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().oc_frame_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

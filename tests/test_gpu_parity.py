@@ -1,0 +1,113 @@
+"""HIP path against the oracle, bit for bit (integer ids, sorted arrays and
+fp32 values compared as bit patterns)."""
+import numpy as np
+import pytest
+
+import ctamd
+import oracle as O
+from helpers import bits, gpu_scene, oracle_scene, random_rays, di_array, res_array
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare_bvh(g, o, label):
+    n = o["n"]
+    assert g["n"] == n, label
+    assert np.array_equal(g["keys"], o["keys"]), f"{label}: sorted Morton keys differ"
+    assert np.array_equal(g["ids"], o["ids"]), f"{label}: sorted primitive ids differ"
+    assert np.array_equal(g["child"], o["child"]), f"{label}: Karras topology differs"
+    assert np.array_equal(bits(g["boxes"]), bits(o["boxes"])), f"{label}: refit boxes differ"
+    assert np.array_equal(g["parent"], o["parent"]), f"{label}: parents differ"
+    assert g["depth"] == o["depth"] and g["root_ref"] == o["root_ref"], label
+
+
+def test_lbvh_arrays_small(gpu):
+    w = ctamd.small_plain(width=32, height=32)
+    sc, acc, ctx = gpu_scene(w)
+    osc = oracle_scene(sc, acc)
+    n = sc.info.primitive_count
+    _compare_bvh(acc.read_bvh(0, n), osc.read_bvh(0, n), "blas 0")
+    _compare_bvh(acc.read_bvh(ctamd.CT_INVALID_INDEX, 1), osc.read_bvh(ctamd.CT_INVALID_INDEX, 1), "tlas")
+
+
+def test_lbvh_arrays_chunked(gpu):
+    w = ctamd.hd_di(width=64, height=36, grid=96, chunk=32)
+    sc, acc, ctx = gpu_scene(w)
+    osc = oracle_scene(sc, acc)
+    ni = sc.info.instance_count
+    assert ni == 27
+    for i in range(ni):
+        n = sc.instances[i].primitive_count
+        _compare_bvh(acc.read_bvh(i, n), osc.read_bvh(i, n), f"blas {i}")
+    _compare_bvh(acc.read_bvh(ctamd.CT_INVALID_INDEX, ni), osc.read_bvh(ctamd.CT_INVALID_INDEX, ni), "tlas")
+
+
+@pytest.mark.parametrize("grid,chunk", [(64, 64), (96, 32)])
+def test_trace_closest_bit_exact(gpu, grid, chunk):
+    w = ctamd.hd_di(width=32, height=32, grid=grid, chunk=chunk)
+    sc, acc, ctx = gpu_scene(w)
+    osc = oracle_scene(sc, acc)
+    rng = np.random.default_rng(7)
+    o, d = random_rays(rng, 20000, sc.grid_min, sc.grid_max)
+    tg, ig, sg = acc.trace_closest(o, d, stats=True)
+    to, io, so = osc.trace_closest(o, d, stats=True)
+    assert np.array_equal(ig, io), "hit ids differ"
+    assert np.array_equal(bits(tg), bits(to)), "hit t differs"
+    assert (sg.nodes, sg.leaves, sg.instances) == (so.nodes, so.leaves, so.instances)
+    # brute force over every primitive on a subset
+    tb, ib = osc.trace_closest(o[:400], d[:400], brute=True)
+    assert np.array_equal(ib, ig[:400]) and np.array_equal(bits(tb), bits(tg[:400]))
+
+
+def test_trace_any_bit_exact(gpu):
+    w = ctamd.hd_di(width=32, height=32, grid=96, chunk=32)
+    sc, acc, ctx = gpu_scene(w)
+    osc = oracle_scene(sc, acc)
+    rng = np.random.default_rng(11)
+    o, d = random_rays(rng, 20000, sc.grid_min, sc.grid_max, inside_frac=0.5)
+    dist = (rng.random(o.shape[0]) * 3.0).astype(np.float32)
+    tg = np.full((o.shape[0], 2), ctamd.CT_INVALID_INDEX, np.uint32)
+    lights = np.array([[l.instance_info.instance_id, l.instance_info.primitive_id] for l in sc.lights], np.uint32)
+    tg[::3] = lights[rng.integers(0, len(lights), size=tg[::3].shape[0])]
+    g, sg = acc.trace_any(o, d, dist, tg, stats=True)
+    r, so = osc.trace_any(o, d, dist, tg, stats=True)
+    assert np.array_equal(g, r)
+    assert (sg.nodes, sg.leaves, sg.instances) == (so.nodes, so.leaves, so.instances)
+    b = osc.trace_any(o[:400], d[:400], dist[:400], tg[:400], brute=True)
+    assert np.array_equal(b, g[:400])
+    assert 0 < g.mean() < 1
+
+
+def _frames_match(w, frames):
+    sc, acc, ctx = gpu_scene(w)
+    assert ctx.light_config().cube_light_count == sc.info.cube_light_count
+    osc = oracle_scene(sc, acc)
+    ofr = O.OFrame(osc, w.width, w.height)
+    for f in range(frames):
+        p = ctamd.frame_params(w, f)
+        ctx.render(acc, p)
+        ofr.render(p)
+        gi, oi = ctx.read(ctamd.CT_BUF_IMAGE, w.width, w.height), ofr.read(ctamd.CT_BUF_IMAGE)
+        for which in (ctamd.CT_BUF_DI, ctamd.CT_BUF_PREV_DI):
+            assert np.array_equal(di_array(ctx.read(which, w.width, w.height)), di_array(ofr.read(which))), (f, which)
+        for which in (ctamd.CT_BUF_INT_RESERVOIR, ctamd.CT_BUF_PREV_RESERVOIR):
+            ga, oa = res_array(ctx.read(which, w.width, w.height)), res_array(ofr.read(which))
+            hit = di_array(ofr.read(ctamd.CT_BUF_DI))[:, 3].view(np.float32) > 0
+            assert np.array_equal(ga[hit], oa[hit]), (f, which)
+        assert np.array_equal(bits(gi), bits(oi)), f"frame {f}: radiance differs"
+        gs, os_ = ctx.stats(), ofr.stats()
+        assert gs.primary_rays == os_.primary_rays
+        assert gs.shadow_rays == os_.shadow_rays, (f, gs.shadow_rays, os_.shadow_rays)
+    return sc
+
+
+def test_frame_parity_small(gpu):
+    _frames_match(ctamd.small_plain(width=64, height=64, frames=4), 4)
+
+
+def test_frame_parity_reference_mode_taa(gpu):
+    _frames_match(ctamd.small_plain(reference_mode=True, width=72, height=40, frames=4, taa=True), 4)
+
+
+def test_frame_parity_chunked_orbit(gpu):
+    _frames_match(ctamd.hd_di(width=96, height=64, grid=96, chunk=32, frames=3), 3)
