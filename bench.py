@@ -78,7 +78,18 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy (rocprof counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the per-frame path is single-GPU "
               f"and multi-GPU sharding is not part of this build", file=sys.stderr)
@@ -111,8 +122,10 @@ def main():
     ist = ctx.stats()
     bytes_any = 36 * ist.shadow_rays + 64 * ist.nodes + 28 * ist.leaves + 64 * ist.instances
     bytes_per_shadow_ray = bytes_any / max(ist.shadow_rays, 1)
-    log(f"instrumented frame: {ist.shadow_rays} shadow rays, {ist.nodes} nodes, {ist.leaves} leaves, "
-        f"{ist.instances} instance entries; {bytes_per_shadow_ray:.0f} algorithmic B/ray")
+    log(f"instrumented frame: {ist.shadow_rays} shadow rays ({ist.shadow_rays_ris} RIS, "
+        f"{ist.shadow_rays_temporal} temporal, {ist.shadow_rays_spatial} spatial), {ist.nodes} nodes, "
+        f"{ist.leaves} leaves, {ist.instances} instance entries; {bytes_per_shadow_ray:.0f} algorithmic B/ray, "
+        f"{bytes_any / 1e6:.1f} MB per frame; closest: {ist.closest_nodes} nodes, {ist.closest_leaves} leaves")
     # Timed region: K frames, events per pass.
     ctx.set_instrumentation(1)
     import ctypes

@@ -1,0 +1,74 @@
+"""Writes the golden fixtures under tests/golden/ from the oracle.
+
+  python tests/golden/make_golden.py
+
+rng_kat.npz      TEA-16 and LCG/rnd known-answer vectors (random.glsl:26-77)
+lbvh_300.npz     LBVH arrays (sorted Morton keys/ids, Karras children,
+                 refit boxes, parents) for 300 seeded voxel centres
+frames_32.npz    32x32 ReSTIR DI images, DI and reservoirs of frames 0-3 of
+                 the small workload, and of its reference-constant variant
+                 with TAA
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "cube-tracing_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import ctamd  # noqa: E402
+import oracle as O  # noqa: E402
+
+
+def rng_kat():
+    rng = np.random.default_rng(2024)
+    pairs = rng.integers(0, 2 ** 32, size=(64, 2), dtype=np.uint64).astype(np.uint32)
+    tea = np.array([O.tea(int(a), int(b)) for a, b in pairs], np.uint32)
+    s, lcg_seq, rnd_seq = 0xC0FFEE, [], []
+    for _ in range(64):
+        r, s = O.rnd(s)
+        rnd_seq.append(r)
+        lcg_seq.append(s)
+    return {"pairs": pairs, "tea": tea, "lcg_state": np.array(lcg_seq, np.uint32),
+            "rnd": np.array(rnd_seq, np.float32), "seed": np.uint32(0xC0FFEE)}
+
+
+def lbvh_300():
+    rng = np.random.default_rng(300)
+    cells = rng.choice(40 ** 3, size=300, replace=False)
+    c = (np.stack([cells % 40, (cells // 40) % 40, cells // 1600], 1).astype(np.float32) - 20.0) * np.float32(0.03125)
+    c = c + np.float32(0.015625)
+    r = np.float32(0.015625 + 0.000244140625)
+    b = O.lbvh_build(c, c - r, c + r)
+    return {"centers": c, "keys": b["keys"], "ids": b["ids"], "child": b["child"], "boxes": b["boxes"],
+            "parent": b["parent"], "depth": np.uint32(b["depth"])}
+
+
+def frames(w):
+    sc = ctamd.Scene(ctamd.scene_desc(w))
+    osc = O.OScene.from_scene(sc)
+    fr = O.OFrame(osc, w.width, w.height)
+    img, di, res = [], [], []
+    for f in range(w.frames):
+        fr.render(ctamd.frame_params(w, f), threads=1)
+        img.append(fr.read(ctamd.CT_BUF_IMAGE))
+        di.append(np.frombuffer(bytes(fr.read(ctamd.CT_BUF_DI)), np.uint32).reshape(-1, 15))
+        res.append(np.frombuffer(bytes(fr.read(ctamd.CT_BUF_PREV_RESERVOIR)), np.uint32).reshape(-1, 9))
+    return np.stack(img), np.stack(di), np.stack(res)
+
+
+def main():
+    np.savez_compressed(os.path.join(HERE, "rng_kat.npz"), **rng_kat())
+    np.savez_compressed(os.path.join(HERE, "lbvh_300.npz"), **lbvh_300())
+    a = frames(ctamd.small_plain(width=32, height=32, frames=4))
+    b = frames(ctamd.small_plain(reference_mode=True, width=32, height=32, frames=4, taa=True))
+    np.savez_compressed(os.path.join(HERE, "frames_32.npz"), img=a[0], di=a[1], res=a[2],
+                        ref_img=b[0], ref_di=b[1], ref_res=b[2])
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
