@@ -44,28 +44,30 @@ def cpu_baseline(w, sc, acc, budget_s):
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         pass
-    rows = 8
-    total_rays, total_t, sampled_rows = 0, 0.0, 0
+    # Whole frames (or, when one frame is too long, a centred band of rows)
+    # until about `budget_s` seconds of CPU work have been timed.
+    rows = w.height
+    total_rays, total_t, sampled_rows, frames = 0, 0.0, 0, 0
     mid = w.height // 2
-    for f in range(2):
+    for f in range(64):
         p = ctamd.frame_params(w, f)
-        while True:
-            r0 = max(0, mid - rows // 2)
-            r1 = min(w.height, r0 + rows)
-            t1 = time.perf_counter()
-            fr.render(p, threads=cores, rows=(r0, r1))
-            dt = time.perf_counter() - t1
-            st = fr.stats()
-            if dt < budget_s / 8 and rows < w.height:
-                rows = min(w.height, rows * 2)
-                continue
-            total_rays += st.primary_rays + st.shadow_rays
-            total_t += dt
-            sampled_rows += r1 - r0
+        r0 = max(0, mid - rows // 2)
+        r1 = min(w.height, r0 + rows)
+        t1 = time.perf_counter()
+        fr.render(p, threads=cores, rows=(r0, r1))
+        dt = time.perf_counter() - t1
+        st = fr.stats()
+        total_rays += st.primary_rays + st.shadow_rays
+        total_t += dt
+        sampled_rows += r1 - r0
+        frames += 1
+        if dt > budget_s / 2 and rows > 16:
+            rows //= 2
+        if total_t >= budget_s:
             break
     return {"value": round(total_rays / total_t / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
-            "sample": f"frames 0-1 of the same workload, {sampled_rows} rows in total around the image centre "
-                      f"({total_rays} traversal queries in {total_t:.1f}s, OpenMP {cores} threads)"}
+            "sample": f"{frames} consecutive frames of the same workload from frame 0, {sampled_rows} rows in "
+                      f"total ({total_rays} traversal queries in {total_t:.1f}s, OpenMP {cores} threads)"}
 
 
 def main():
@@ -76,7 +78,7 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (rocprof counter calibration)")

@@ -87,30 +87,75 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n) {
   if (i < n) p[i] = v;
 }
 
-// Item centres of voxel AABBs and per-segment centre bounds.
-__global__ void k_blas_items(SegView sv, const ct_aabb *aabbs, float4 *centers, uint32_t *bounds) {
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= sv.n) return;
-  uint32_t s = find_seg(sv.off, sv.nseg, k);
-  uint32_t g = sv.src[s] + (k - sv.off[s]);
-  const ct_aabb &b = aabbs[g];
-  float cx = (b.minimum[0] + b.maximum[0]) * 0.5f;
-  float cy = (b.minimum[1] + b.maximum[1]) * 0.5f;
-  float cz = (b.minimum[2] + b.maximum[2]) * 0.5f;
-  centers[k] = make_float4(cx, cy, cz, 0.0f);
-  uint32_t *bb = bounds + 6 * s;
-  atomicMin(bb + 0, fenc(cx)); atomicMin(bb + 1, fenc(cy)); atomicMin(bb + 2, fenc(cz));
-  atomicMax(bb + 3, fenc(cx)); atomicMax(bb + 4, fenc(cy)); atomicMax(bb + 5, fenc(cz));
+// Per-segment centre bounds: one set of 6 atomics per block when the
+// block's items belong to one segment (the common case), else per item.
+__device__ void bounds_accumulate(SegView sv, uint32_t k, bool valid, uint32_t s, float cx, float cy, float cz,
+                                  uint32_t *bounds) {
+  __shared__ uint32_t red[6][kBlock / 64];
+  const uint32_t first = blockIdx.x * blockDim.x;
+  const uint32_t last = min(first + blockDim.x, sv.n) - 1u;
+  const bool uniform = find_seg(sv.off, sv.nseg, first) == find_seg(sv.off, sv.nseg, last);
+  if (!uniform) {
+    if (valid) {
+      uint32_t *bb = bounds + 6 * s;
+      atomicMin(bb + 0, fenc(cx)); atomicMin(bb + 1, fenc(cy)); atomicMin(bb + 2, fenc(cz));
+      atomicMax(bb + 3, fenc(cx)); atomicMax(bb + 4, fenc(cy)); atomicMax(bb + 5, fenc(cz));
+    }
+    return;
+  }
+  uint32_t v[6] = {valid ? fenc(cx) : 0xFFFFFFFFu, valid ? fenc(cy) : 0xFFFFFFFFu, valid ? fenc(cz) : 0xFFFFFFFFu,
+                   valid ? fenc(cx) : 0u, valid ? fenc(cy) : 0u, valid ? fenc(cz) : 0u};
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+      uint32_t o = __shfl_xor(v[a], off);
+      v[a] = a < 3 ? min(v[a], o) : max(v[a], o);
+    }
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int a = 0; a < 6; ++a) red[a][w] = v[a];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t sfirst = find_seg(sv.off, sv.nseg, first);
+    uint32_t *bb = bounds + 6 * sfirst;
+    for (int a = 0; a < 6; ++a) {
+      uint32_t r = red[a][0];
+      for (int q = 1; q < (int)(blockDim.x / 64); ++q) r = a < 3 ? min(r, red[a][q]) : max(r, red[a][q]);
+      if (a < 3) atomicMin(bb + a, r); else atomicMax(bb + a, r);
+    }
+  }
 }
 
-__global__ void k_item_bounds(SegView sv, const float4 *centers, uint32_t *bounds) {
+// Item centres of voxel AABBs and per-segment centre bounds.
+__global__ void __launch_bounds__(kBlock) k_blas_items(SegView sv, const ct_aabb *aabbs, float4 *centers,
+                                                        uint32_t *bounds) {
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= sv.n) return;
-  uint32_t s = find_seg(sv.off, sv.nseg, k);
-  float4 c = centers[k];
-  uint32_t *bb = bounds + 6 * s;
-  atomicMin(bb + 0, fenc(c.x)); atomicMin(bb + 1, fenc(c.y)); atomicMin(bb + 2, fenc(c.z));
-  atomicMax(bb + 3, fenc(c.x)); atomicMax(bb + 4, fenc(c.y)); atomicMax(bb + 5, fenc(c.z));
+  bool valid = k < sv.n;
+  uint32_t s = 0;
+  float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+  if (valid) {
+    s = find_seg(sv.off, sv.nseg, k);
+    uint32_t g = sv.src[s] + (k - sv.off[s]);
+    const ct_aabb &b = aabbs[g];
+    cx = (b.minimum[0] + b.maximum[0]) * 0.5f;
+    cy = (b.minimum[1] + b.maximum[1]) * 0.5f;
+    cz = (b.minimum[2] + b.maximum[2]) * 0.5f;
+    centers[k] = make_float4(cx, cy, cz, 0.0f);
+  }
+  bounds_accumulate(sv, k, valid, s, cx, cy, cz, bounds);
+}
+
+__global__ void __launch_bounds__(kBlock) k_item_bounds(SegView sv, const float4 *centers, uint32_t *bounds) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  bool valid = k < sv.n;
+  uint32_t s = 0;
+  float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (valid) {
+    s = find_seg(sv.off, sv.nseg, k);
+    c = centers[k];
+  }
+  bounds_accumulate(sv, k, valid, s, c.x, c.y, c.z, bounds);
 }
 
 __device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
@@ -307,7 +352,8 @@ __device__ __forceinline__ float ld_relaxed(const float *p) {
 // child boxes (min/max are exact, so the result is order independent).
 __global__ void k_refit(SegView sv, const uint32_t *svals, const float4 *centers, const float4 *lo_in,
                         const float4 *hi_in, float pad_r, DNode *nodes, const uint32_t *parent_int,
-                        const uint32_t *parent_leaf, uint32_t *flags, float *root_box) {
+                        const uint32_t *parent_leaf, uint32_t *flags, uint32_t *heights, float *root_box,
+                        uint32_t *depth) {
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= sv.n) return;
   uint32_t s = find_seg(sv.off, sv.nseg, k);
@@ -330,6 +376,7 @@ __global__ void k_refit(SegView sv, const uint32_t *svals, const float4 *centers
   }
   uint32_t ref = kLeaf | j;
   uint32_t node = parent_leaf[nb + j];
+  uint32_t h = 0;
   for (int level = 0; node != kInvalid && node + 1 < n && level < 256; ++level) {
     DNode *nd = nodes + nb + node;
     float *f = reinterpret_cast<float *>(nd);
@@ -349,27 +396,17 @@ __global__ void k_refit(SegView sv, const uint32_t *svals, const float4 *centers
       b[a] = fminf(c0[a], c1[a]);
       b[3 + a] = fmaxf(c0[3 + a], c1[3 + a]);
     }
+    // height = 1 + the taller child (leaves have height 0)
+    uint32_t r0c = nd->d.x, r1c = nd->d.y;
+    uint32_t h0 = (r0c & kLeaf) ? 0u : __hip_atomic_load(heights + nb + r0c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t h1 = (r1c & kLeaf) ? 0u : __hip_atomic_load(heights + nb + r1c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    h = max(h0, h1) + 1u;
+    heights[nb + node] = h;
     ref = node;
     node = parent_int[nb + node];
   }
   for (int a = 0; a < 6; ++a) root_box[6 * s + a] = b[a];
-}
-
-__global__ void k_depth(SegView sv, const uint32_t *parent_int, const uint32_t *parent_leaf,
-                        uint32_t *depth) {
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= sv.n) return;
-  uint32_t s = find_seg(sv.off, sv.nseg, k);
-  uint32_t j = k - sv.off[s];
-  uint32_t nb = sv.base[s];
-  if (sv.off[s + 1] - sv.off[s] == 1) return;
-  const uint32_t n = sv.off[s + 1] - sv.off[s];
-  uint32_t c = 0, node = parent_leaf[nb + j];
-  while (node != kInvalid && node + 1 < n && c < 256) {
-    ++c;
-    node = parent_int[nb + node];
-  }
-  atomicMax(&depth[s], c);
+  depth[s] = h;
 }
 
 // Structural check of the built topology: every child ref in range, the
@@ -433,6 +470,7 @@ int build_segments(Accel &a, hipStream_t st, const std::vector<uint32_t> &off,
   if ((rc = S.keys[0].reserve(8 * (size_t)n)) || (rc = S.keys[1].reserve(8 * (size_t)n)) ||
       (rc = S.vals[0].reserve(4 * (size_t)n)) || (rc = S.vals[1].reserve(4 * (size_t)n)) ||
       (rc = S.centers.reserve(16 * (size_t)n)) || (rc = S.flags.reserve(4 * (size_t)node_capacity)) ||
+      (rc = S.heights.reserve(4 * (size_t)node_capacity)) ||
       (rc = S.root_box.reserve(24 * (size_t)nseg)) || (rc = S.depth.reserve(4 * (size_t)nseg)) ||
       (rc = S.sorted_keys32.reserve(4 * (size_t)n)) || (rc = S.hist.reserve(4 * 256 * (size_t)(blocks(n, kRsTile) + 1))))
     return rc;
@@ -475,8 +513,8 @@ int build_segments(Accel &a, hipStream_t st, const std::vector<uint32_t> &off,
   hipLaunchKernelGGL(k_karras, dim3(blocks(n)), dim3(kBlock), 0, st, sv, d_k32, d_nodes, d_parent_int, d_parent_leaf);
   hipLaunchKernelGGL(k_leaves, dim3(blocks(n)), dim3(kBlock), 0, st, sv, k0, v0, d_centers, d_leaves, d_keys_out, d_ids_out, d_payload);
   hipLaunchKernelGGL(k_refit, dim3(blocks(n)), dim3(kBlock), 0, st, sv, v0, d_centers, d_lo, d_hi, pad_r, d_nodes,
-                     d_parent_int, d_parent_leaf, S.flags.as<uint32_t>(), S.root_box.as<float>());
-  hipLaunchKernelGGL(k_depth, dim3(blocks(n)), dim3(kBlock), 0, st, sv, d_parent_int, d_parent_leaf, S.depth.as<uint32_t>());
+                     d_parent_int, d_parent_leaf, S.flags.as<uint32_t>(), S.heights.as<uint32_t>(),
+                     S.root_box.as<float>(), S.depth.as<uint32_t>());
   uint32_t *d_err = S.flags.as<uint32_t>();  // flags are free again after the refit
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, st, d_err, 0u, 1u);
   hipLaunchKernelGGL(k_validate, dim3(blocks(n)), dim3(kBlock), 0, st, sv, d_nodes, d_parent_int, d_parent_leaf, d_err);

@@ -97,7 +97,7 @@ struct DevBuf {
 
 // Build scratch for the LBVH kernels (reused between builds).
 struct BuildScratch {
-  DevBuf keys[2], vals[2], centers, lo, hi, parent_int, parent_leaf, flags, hist, seg,
+  DevBuf keys[2], vals[2], centers, lo, hi, parent_int, parent_leaf, flags, heights, hist, seg,
       root_box, depth, sorted_keys32;
 };
 
@@ -159,7 +159,7 @@ struct FrameBufs {
   DevBuf di_pos[2], di_nrm[2], di_org[2], di_hit[2];  // DI, current/previous
   DevBuf resA[2], resB[2], resC[2];                   // intermediate / previous
   DevBuf velocity, img[2], taa, hit, pidx, prev_res_copy;
-  DevBuf mask, seed2, nbr;                            // per-pixel pass state
+  DevBuf mask, gather;                                // per-pixel pass state
   DevBuf ray_o, ray_d, ray_t, queue, counters, counters_host_staging;
 };
 

@@ -68,6 +68,7 @@ struct DFrame {
   uint32_t *ray_t;
   uint32_t *queue;
   uint32_t *counters;
+  float4 *gat;  // [8 per pixel] neighbour record, see st_gather
 };
 
 struct PLight {
@@ -118,6 +119,33 @@ CT_D void st_res(uint4 *A, float4 *B, float *C, uint32_t i, const Res &r) {
   B[i] = make_float4(r.W_sum, r.M, r.F.x, r.F.y);
   C[i] = r.F.z;
 }
+// Neighbour record, one 128 B line per pixel: everything spatial reuse reads
+// from a neighbour (its DI minus seed/confidence, and its intermediate
+// reservoir).  Written once the temporal pass is done.
+constexpr uint32_t kGather = 8;
+CT_D void st_gather(float4 *G, uint32_t i, const float4 &pos, const float4 &nrm, const float4 &org, uint32_t inst,
+                    uint32_t prim, const Res &r) {
+  float4 *g = G + (size_t)kGather * i;
+  g[0] = pos;
+  g[1] = nrm;
+  g[2] = make_float4(org.x, org.y, org.z, __uint_as_float(inst));
+  g[3] = make_float4(__uint_as_float(prim), __uint_as_float(r.Y), __uint_as_float(r.type), __uint_as_float(r.seed));
+  g[4] = make_float4(r.W_y, r.W_sum, r.M, r.F.x);
+  g[5] = make_float4(r.F.y, r.F.z, 0.0f, 0.0f);
+}
+CT_D void ld_gather(const float4 *G, uint32_t i, DI &d, Res &r) {
+  const float4 *g = G + (size_t)kGather * i;
+  float4 a = g[0], b = g[1], c = g[2], e = g[3], f = g[4], h = g[5];
+  d.pos = xyz(a); d.distance = a.w;
+  d.nrm = xyz(b); d.mat = __float_as_uint(b.w);
+  d.org = xyz(c); d.inst = __float_as_uint(c.w);
+  d.prim = __float_as_uint(e.x);
+  d.seed = 0; d.conf = 0.0f;
+  r.Y = __float_as_uint(e.y); r.type = __float_as_uint(e.z); r.seed = __float_as_uint(e.w);
+  r.W_y = f.x; r.W_sum = f.y; r.M = f.z; r.F = f3(f.w, h.x, h.y);
+}
+CT_D float gather_distance(const float4 *G, uint32_t i) { return G[(size_t)kGather * i].w; }
+
 CT_D DI ld_di(const float4 *P, const float4 *N, const float4 *O, const uint4 *Hh, uint32_t i) {
   float4 p = P[i], n = N[i], o = O[i];
   uint4 h = Hh[i];
@@ -256,6 +284,45 @@ CT_D F3 random_on_hemisphere(uint32_t &seed, F3 n) {
   return dot(u, n) > 0.0f ? u : neg(u);
 }
 
+// LCG state after 33 steps: s * a^33 + c * (a^32 + ... + 1) (mod 2^32).
+constexpr uint32_t lcg_mul_pow(int k) {
+  uint32_t r = 1u;
+  for (int i = 0; i < k; ++i) r *= 1664525u;
+  return r;
+}
+constexpr uint32_t lcg_add_sum(int k) {
+  uint32_t r = 0u, p = 1u;
+  for (int i = 0; i < k; ++i) {
+    r += p;
+    p *= 1664525u;
+  }
+  return r * 1013904223u;
+}
+CT_D uint32_t lcg_skip33(uint32_t s) {
+  constexpr uint32_t A = lcg_mul_pow(33), C = lcg_add_sum(33);
+  return A * s + C;
+}
+
+// True when the light voxel lies wholly on the back side of the surface at
+// P (axis-aligned normal n): then dot(normalize(P - x), n) >= 0 for every
+// sample point x of the voxel, light.glsl:343 rejects all draws.  The
+// 1e-4 margin covers the rounding of the sample position; near-boundary
+// cases take the full loop.
+CT_D bool light_behind_surface(F3 P, F3 n, F3 c, float half) {
+  const float m = half + 1e-4f;
+  if (n.y == 0.0f && n.z == 0.0f) {
+    if (n.x == 1.0f) return P.x >= c.x + m;
+    if (n.x == -1.0f) return P.x <= c.x - m;
+  } else if (n.x == 0.0f && n.z == 0.0f) {
+    if (n.y == 1.0f) return P.y >= c.y + m;
+    if (n.y == -1.0f) return P.y <= c.y - m;
+  } else if (n.x == 0.0f && n.y == 0.0f) {
+    if (n.z == 1.0f) return P.z >= c.z + m;
+    if (n.z == -1.0f) return P.z <= c.z - m;
+  }
+  return false;
+}
+
 struct LS {
   bool ok;      // the sample is valid before the shadow test
   bool ray;     // a shadow ray would be traced
@@ -278,6 +345,13 @@ CT_D void sample_lights(const DFrame &F, const Hit &hit, const PLight &l, float 
   F3 em = l.emissive;
   if (l.type == CT_LIGHT_CUBE) {
     float half = l.size * 0.5f, qs = l.size;
+    if (light_behind_surface(P, n, l.pos, half)) {
+      // Every one of the 11 draws fails the "in front of the surface" test,
+      // so the result is a failed sample (its G and pdf only ever multiply
+      // a zero contribution) after 11 x 3 = 33 LCG steps.
+      seed = lcg_skip33(seed);
+      return;
+    }
     uint32_t i = 0;
     do {
       vis = true;
@@ -471,7 +545,7 @@ CT_D void spatial_reuse(const DFrame &F, Res &r, float confidence, uint32_t x, u
     if (nx < 0 || nx >= (int64_t)F.W || ny < 0 || ny >= (int64_t)F.H) continue;
     if (nx == (int64_t)x && ny == (int64_t)y) continue;
     uint32_t l = (uint32_t)ny * F.W + (uint32_t)nx;
-    if (F.dpos_c[l].w <= 0.0f) continue;
+    if (gather_distance(F.gat, l) <= 0.0f) continue;
     if (num < CT_MAX_NEIGHBORS) idx[num++] = l;
   }
   if (num == 0) return;
@@ -480,8 +554,9 @@ CT_D void spatial_reuse(const DFrame &F, Res &r, float confidence, uint32_t x, u
   float M_s = r.M;
   const float fk = (float)num;
   for (uint32_t i = 0; i < num; ++i) {
-    Res in = ld_res(F.rA_i, F.rB_i, F.rC_i, idx[i]);
-    DI nd = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, idx[i]);
+    Res in;
+    DI nd;
+    ld_gather(F.gat, idx[i], nd, in);
     Hit nh = hit_from_di(nd);
     F3 nray_d = normalize(sub(nh.P, nd.org));
     Mat nm = ld_mat(F, nd.mat);
@@ -536,21 +611,37 @@ CT_D void camera_ray(const DFrame &F, uint32_t x, uint32_t y, F3 &o, F3 &d) {
 }
 
 // ----------------------------------------------------------- queue push
-CT_D void queue_flush(const DFrame &F, uint32_t px, uint32_t emitted, uint32_t nslots, uint32_t counter) {
-  const uint32_t lane = __lane_id();
-  const unsigned long long lt = (1ull << lane) - 1ull;
-  for (uint32_t k = 0; k < nslots; ++k) {
-    bool pred = (emitted >> k) & 1u;
-    unsigned long long m = __ballot(pred);
-    if (m == 0ull) continue;
-    int leader = __ffsll((long long)m) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader) {
-      base = atomicAdd(&F.counters[kCntQueue], (uint32_t)__popcll(m));
-      atomicAdd(&F.counters[counter], (uint32_t)__popcll(m));
+// Appends this thread's emitted slots (bit k -> ray pixel*16 + k) to the
+// ray queue with one atomic per block.  Every thread of the block calls it.
+CT_D void queue_flush(const DFrame &F, uint32_t px, uint32_t emitted, uint32_t counter) {
+  __shared__ uint32_t s_wsum[kBlock / 64];
+  __shared__ uint32_t s_base;
+  const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+  const uint32_t c = (uint32_t)__popc(emitted);
+  uint32_t incl = c;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t v = __shfl_up(incl, off);
+    if ((int)lane >= off) incl += v;
+  }
+  if (lane == 63) s_wsum[w] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int i = 0; i < kBlock / 64; ++i) {
+      uint32_t t = s_wsum[i];
+      s_wsum[i] = tot;
+      tot += t;
     }
-    base = __shfl(base, leader);
-    if (pred) F.queue[base + (uint32_t)__popcll(m & lt)] = px * kRaySlots + k;
+    s_base = tot ? atomicAdd(&F.counters[kCntQueue], tot) : 0u;
+    if (tot) atomicAdd(&F.counters[counter], tot);
+  }
+  __syncthreads();
+  uint32_t pos = s_base + s_wsum[w] + incl - c;
+  while (emitted) {
+    uint32_t k = (uint32_t)(__ffs(emitted) - 1);
+    emitted &= emitted - 1u;
+    F.queue[pos++] = px * kRaySlots + k;
   }
 }
 
@@ -666,7 +757,7 @@ __global__ void __launch_bounds__(kBlock) k_ris(DFrame F) {
     st_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px, di);
     F.mask[px] = 0u;
   }
-  queue_flush(F, px, emitted, 1u, kCntRis);
+  queue_flush(F, px, emitted, kCntRis);
 }
 
 // RIS visibility result (FINISH) and the temporal reuse rays (EMIT).
@@ -698,29 +789,36 @@ __global__ void __launch_bounds__(kBlock) k_temporal_emit(DFrame F) {
     }
     F.mask[px] = 0u;
   }
-  queue_flush(F, px, emitted, 2u, kCntTemporal);
+  queue_flush(F, px, emitted, kCntTemporal);
 }
 
 __global__ void __launch_bounds__(kBlock) k_temporal_finish(DFrame F) {
   const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
   if (px >= F.npix) return;
   float4 pd = F.dpos_c[px];
-  if (!(pd.w > 0.0f)) return;
+  if (!(pd.w > 0.0f)) {
+    F.gat[(size_t)kGather * px] = pd;  // only the distance of a miss is ever read
+    return;
+  }
+  float4 pn = F.dnrm_c[px], po = F.dorg_c[px];
+  uint4 ph = F.dhit_c[px];
+  Res r = ld_res(F.rA_i, F.rB_i, F.rC_i, px);
   uint32_t pidx = F.pidx[px];
   Res prev = res_init();
   if (pidx < F.W * F.H) prev = ld_res(F.rA_p, F.rB_p, F.rC_p, pidx);
-  if (!(prev.W_y > 0.0f)) return;
-  DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
-  Hit hit = hit_from_di(di);
-  Res r = ld_res(F.rA_i, F.rB_i, F.rC_i, px);
-  Mat mat = ld_mat(F, di.mat);
-  F3 ro, rd;
-  camera_ray(F, px % F.W, px / F.W, ro, rd);
-  Vis v{false, px, 0u, F.mask[px], 0u};
-  uint32_t seed = di.seed;
-  temporal_reuse(F, r, prev, pidx, rd, hit, mat, seed, v);
-  st_res(F.rA_i, F.rB_i, F.rC_i, px, r);
-  F.dorg_c[px].w = __uint_as_float(seed);
+  if (prev.W_y > 0.0f) {
+    DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
+    Hit hit = hit_from_di(di);
+    Mat mat = ld_mat(F, di.mat);
+    F3 ro, rd;
+    camera_ray(F, px % F.W, px / F.W, ro, rd);
+    Vis v{false, px, 0u, F.mask[px], 0u};
+    uint32_t seed = di.seed;
+    temporal_reuse(F, r, prev, pidx, rd, hit, mat, seed, v);
+    st_res(F.rA_i, F.rB_i, F.rC_i, px, r);
+    F.dorg_c[px].w = __uint_as_float(seed);
+  }
+  st_gather(F.gat, px, pd, pn, po, ph.x, ph.y, r);
 }
 
 // Shading pass: spatial reuse (EMIT / FINISH) then shade and store.
@@ -760,7 +858,7 @@ __global__ void __launch_bounds__(kBlock) k_spatial(DFrame F) {
     }
     if (kEmit) F.mask[px] = 0u;
   }
-  if (kEmit) queue_flush(F, px, emitted, 2u * F.rp.max_neighbors, kCntSpatial);
+  if (kEmit) queue_flush(F, px, emitted, kCntSpatial);
 }
 
 // taa.glsl:16-108 over the dispatch region.
@@ -944,6 +1042,7 @@ int alloc_frame(Ctx &c) {
       (rc = b.pidx.reserve(4 * np)) || (rc = b.mask.reserve(4 * np)) || (rc = b.ray_o.reserve(16 * np * kRaySlots)) ||
       (rc = b.ray_d.reserve(16 * np * kRaySlots)) || (rc = b.ray_t.reserve(4 * np * kRaySlots)) ||
       (rc = b.queue.reserve(4 * np * kRaySlots)) || (rc = b.counters.reserve(64 * sizeof(uint32_t))) ||
+      (rc = b.gather.reserve(16 * kGather * np)) ||
       (rc = b.prev_res_copy.reserve(64 * sizeof(unsigned long long))))
     return rc;
   return CT_OK;
@@ -1175,6 +1274,7 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
   F.ray_o = b.ray_o.as<float4>(); F.ray_d = b.ray_d.as<float4>(); F.ray_t = b.ray_t.as<uint32_t>();
   F.queue = b.queue.as<uint32_t>();
   F.counters = b.counters.as<uint32_t>();
+  F.gat = b.gather.as<float4>();
   unsigned long long *stats = (c->instrument & 2u) ? b.prev_res_copy.as<unsigned long long>() : nullptr;
   const bool ev = (c->instrument & 1u) != 0;
   int ne = 0;
