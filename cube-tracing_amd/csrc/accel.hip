@@ -540,7 +540,7 @@ DScene scene_view(const Accel &a) {
   s.tlas_nodes = a.set.tlas_nodes.as<DNode>();
   s.tlas_inst = a.set.tlas_inst.as<uint32_t>();
   s.tlas_root = a.set.tlas_root;
-  s.pad = 0;
+  s.n_inst = (uint32_t)a.instances.size();
   return s;
 }
 

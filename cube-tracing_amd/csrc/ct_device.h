@@ -122,7 +122,7 @@ CT_D F3 mul_rot(F3 v) {
 // axes.  `dr` is the (rotated) direction and `sg` = -sign(dr), both
 // independent of the box.  oriented: divide by dr; else multiply by inv.
 template <bool kCanStart, bool kOriented>
-CT_D bool intersect_box(F3 c, F3 ro, F3 dr, F3 sg, F3 inv, float &dist) {
+CT_D bool intersect_box(F3 c, F3 ro, F3 dr, F3 sg, F3 inv, float &dist, F3 *normal = nullptr) {
   const float r = kHalfVoxel, ir = 64.0f;  // safeInverse(HALF_VOXEL_EXTENT)
   F3 o = rot_mul(sub(ro, c));
   float winding = 1.0f;
@@ -143,6 +143,7 @@ CT_D bool intersect_box(F3 c, F3 ro, F3 dr, F3 sg, F3 inv, float &dist) {
   float sy = tx ? 0.0f : (ty ? sg.y : 0.0f);
   float sz = (tx || ty) ? 0.0f : (tz ? sg.z : 0.0f);
   dist = (sx != 0.0f) ? dtp.x : ((sy != 0.0f) ? dtp.y : dtp.z);
+  if (normal) *normal = kOriented ? rot_mul(f3(sx, sy, sz)) : f3(sx, sy, sz);
   return (sx != 0.0f) || (sy != 0.0f) || (sz != 0.0f);
 }
 
@@ -186,6 +187,28 @@ CT_D F3 sky_color(float t, bool afternoon, F3 dir) {
 // light.glsl:78-80
 CT_D float luminance(F3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 
+// sin and cos of x in [0, 2*pi).  GLSL leaves their precision to the
+// implementation, so the library fixes one: Cody-Waite reduction by pi/2
+// (three-part constant) and the single-precision minimax polynomials of
+// the Cephes library (S. L. Moshier, sinf.c / cosf.c) on [-pi/4, pi/4].
+// Only IEEE add/mul and floor are used, so the CPU restatement computes
+// the same bits.
+CT_D void sincos_ct(float x, float &s, float &c) {
+  const float j = floorf(x * 0.636619772367581343f + 0.5f);
+  const int q = (int)j;
+  const float r = ((x - j * 1.5703125f) - j * 4.837512969970703125e-4f) - j * 7.54978995489188216e-8f;
+  const float z = r * r;
+  const float sp = ((-1.9515295891e-4f * z + 8.3321608736e-3f) * z - 1.6666654611e-1f) * z * r + r;
+  const float cp = ((2.443315711809948e-5f * z - 1.388731625493765e-3f) * z + 4.166664568298827e-2f) * z * z -
+                   0.5f * z + 1.0f;
+  switch (q & 3) {
+    case 0: s = sp; c = cp; break;
+    case 1: s = cp; c = -sp; break;
+    case 2: s = -sp; c = -cp; break;
+    default: s = -cp; c = sp; break;
+  }
+}
+
 // ------------------------------------------------------------ traversal
 constexpr uint32_t kMaxTraversalSteps = 1u << 22;
 
@@ -204,10 +227,46 @@ CT_D bool slab(float lx, float ly, float lz, float hx, float hy, float hz, F3 o,
   return (n0 <= f0) && (f0 >= 0.0f) && (n0 <= limit);
 }
 
+// Per-lane traversal stack: the newest kStackReg entries in registers
+// (r[0] is the top), older ones in this lane's LDS column.
+struct Stack {
+  uint32_t r[kStackReg > 0 ? kStackReg : 1];
+  uint32_t *lds;
+  uint32_t stride;
+  int n;
+  CT_D void push(uint32_t v, bool &overflow) {
+    if (kStackReg == 0) {
+      if (n >= kStackLds) { overflow = true; return; }
+      lds[(uint32_t)n * stride] = v;
+      ++n;
+      return;
+    }
+    if (n >= kStackReg) {
+      if (n - kStackReg >= kStackLds) { overflow = true; return; }
+      lds[(uint32_t)(n - kStackReg) * stride] = r[kStackReg - 1];
+    }
+#pragma unroll
+    for (int i = kStackReg - 1; i > 0; --i) r[i] = r[i - 1];
+    r[0] = v;
+    ++n;
+  }
+  CT_D uint32_t pop() {
+    if (kStackReg == 0) {
+      --n;
+      return lds[(uint32_t)n * stride];
+    }
+    uint32_t v = r[0];
+#pragma unroll
+    for (int i = 0; i < kStackReg - 1; ++i) r[i] = r[i + 1];
+    --n;
+    if (n >= kStackReg) r[kStackReg - 1] = lds[(uint32_t)(n - kStackReg) * stride];
+    return v;
+  }
+};
+
 // Test both children of a node; returns the next ref (or kInvalid to pop)
 // and pushes the far child when both are hit (near first, ties to child 0).
-CT_D uint32_t visit_node(const DNode *nodes, uint32_t idx, F3 o, F3 inv, float limit,
-                         uint32_t *stk, uint32_t stride, int &sp, bool &overflow) {
+CT_D uint32_t visit_node(const DNode *nodes, uint32_t idx, F3 o, F3 inv, float limit, Stack &stk, bool &overflow) {
   const float4 *p = reinterpret_cast<const float4 *>(nodes + idx);
   float4 a = p[0], b = p[1], c = p[2];
   uint4 d = reinterpret_cast<const uint4 *>(nodes + idx)[3];
@@ -216,12 +275,7 @@ CT_D uint32_t visit_node(const DNode *nodes, uint32_t idx, F3 o, F3 inv, float l
   bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, limit, tn1);
   if (h0 && h1) {
     bool swap = tn1 < tn0;
-    if (sp >= kStackSize) {
-      overflow = true;
-      return kInvalid;
-    }
-    stk[(uint32_t)sp * stride] = swap ? d.x : d.y;
-    ++sp;
+    stk.push(swap ? d.x : d.y, overflow);
     return swap ? d.y : d.x;
   }
   if (h0) return d.x;
@@ -229,19 +283,36 @@ CT_D uint32_t visit_node(const DNode *nodes, uint32_t idx, F3 o, F3 inv, float l
   return kInvalid;
 }
 
+// Leaf semantics of a traversal.
+//  kTraceAny:     is_vertex_visible (light.glsl:82-142): is_hit_from_ray
+//                 with rayCanStartInBox = false; occluded when a non-target
+//                 voxel has 0 <= t <= limit.
+//  kTraceClosest: traceRayEXT + rint.glsl:10-43 (rayCanStartInBox = true,
+//                 oriented): closest t > 0.
+//  kTraceBounce:  intersect() of bounce.glsl:9-130 with its is_hit_from_ray
+//                 box test (rayCanStartInBox = true, non-oriented): closest
+//                 t >= 0.  The reference commits the first generated hit
+//                 (bounce.glsl:118); the closest one is taken here.
+// Closest modes break equal t by the smaller (instance, primitive).
+constexpr int kTraceAny = 0, kTraceClosest = 1, kTraceBounce = 2;
+
 // Two-level traversal (TLAS in world space, each BLAS in object space).
-// kAny: is_vertex_visible semantics, returns true when occluded.
-// Closest: traceRayEXT + rint semantics, smallest t then smallest
-// (instance, primitive).  `stk` is this lane's LDS stack column.
-template <bool kAny, bool kCount>
-CT_D bool traverse(const DScene &S, F3 o, F3 d, float limit, uint32_t ti, uint32_t tp,
-                   float &best_t, uint32_t &best_i, uint32_t &best_p, uint32_t *stk,
-                   uint32_t stride, TraceCnt &cnt, bool &overflow) {
+// Returns true when an any-hit query is occluded.  `stk` is this lane's LDS
+// stack column.
+template <int kMode, bool kCount>
+CT_D bool traverse_mode(const DScene &S, F3 o, F3 d, float limit, uint32_t ti, uint32_t tp,
+                        float &best_t, uint32_t &best_i, uint32_t &best_p, uint32_t *stk,
+                        uint32_t stride, TraceCnt &cnt, bool &overflow) {
+  constexpr bool kAny = kMode == kTraceAny;
   const F3 winv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
   F3 oo = o, od = d, oinv = winv, dr = d, sg = d, sinv = d;
   uint32_t first = 0, inst = 0;
   bool in_blas = false;
-  int base = 0, sp = 0;
+  int base = 0;
+  Stack st;
+  st.lds = stk;
+  st.stride = stride;
+  st.n = 0;
   uint32_t ref = S.tlas_root;
   for (uint32_t iter = 0;; ++iter) {
     if (iter > kMaxTraversalSteps) {  // only a malformed tree can get here
@@ -260,7 +331,7 @@ CT_D bool traverse(const DScene &S, F3 o, F3 d, float limit, uint32_t ti, uint32
         oo = xform3(m, o, 1.0f);
         od = xform3(m, d, 0.0f);
         oinv = f3(1.0f / od.x, 1.0f / od.y, 1.0f / od.z);
-        if (kAny) {
+        if (kMode != kTraceClosest) {
           dr = od;
           sinv = f3(safe_inverse(od.x), safe_inverse(od.y), safe_inverse(od.z));
         } else {
@@ -269,12 +340,12 @@ CT_D bool traverse(const DScene &S, F3 o, F3 d, float limit, uint32_t ti, uint32
         sg = f3(-gsign(dr.x), -gsign(dr.y), -gsign(dr.z));
         first = I.first;
         in_blas = true;
-        base = sp;
+        base = st.n;
         ref = I.root;
         continue;
       }
       if (kCount) cnt.nodes++;
-      next = visit_node(S.tlas_nodes, ref, o, winv, kAny ? limit : best_t, stk, stride, sp, overflow);
+      next = visit_node(S.tlas_nodes, ref, o, winv, kAny ? limit : best_t, st, overflow);
     } else {
       if (ref & kLeaf) {
         if (kCount) cnt.leaves++;
@@ -285,8 +356,9 @@ CT_D bool traverse(const DScene &S, F3 o, F3 d, float limit, uint32_t ti, uint32
           bool hit = intersect_box<false, false>(f3(L.x, L.y, L.z), oo, dr, sg, sinv, t);
           if (hit && (t >= 0.0f) && (t <= limit) && !(inst == ti && pid == tp)) return true;
         } else {
-          bool hit = intersect_box<true, true>(f3(L.x, L.y, L.z), oo, dr, sg, sinv, t);
-          if (hit && t > 0.0f) {
+          bool hit = (kMode == kTraceClosest) ? intersect_box<true, true>(f3(L.x, L.y, L.z), oo, dr, sg, sinv, t)
+                                              : intersect_box<true, false>(f3(L.x, L.y, L.z), oo, dr, sg, sinv, t);
+          if (hit && ((kMode == kTraceClosest) ? (t > 0.0f) : (t >= 0.0f))) {
             bool better = (t < best_t) || (t == best_t && ((inst < best_i) || (inst == best_i && pid < best_p)));
             if (better) {
               best_t = t;
@@ -297,7 +369,7 @@ CT_D bool traverse(const DScene &S, F3 o, F3 d, float limit, uint32_t ti, uint32
         }
       } else {
         if (kCount) cnt.nodes++;
-        next = visit_node(S.blas_nodes, first + ref, oo, oinv, kAny ? limit : best_t, stk, stride, sp, overflow);
+        next = visit_node(S.blas_nodes, first + ref, oo, oinv, kAny ? limit : best_t, st, overflow);
       }
     }
     if (overflow) return false;
@@ -305,12 +377,19 @@ CT_D bool traverse(const DScene &S, F3 o, F3 d, float limit, uint32_t ti, uint32
       ref = next;
       continue;
     }
-    if (in_blas && sp == base) in_blas = false;
-    if (sp == 0) break;
-    --sp;
-    ref = stk[(uint32_t)sp * stride];
+    if (in_blas && st.n == base) in_blas = false;
+    if (st.n == 0) break;
+    ref = st.pop();
   }
   return false;
+}
+
+template <bool kAny, bool kCount>
+CT_D bool traverse(const DScene &S, F3 o, F3 d, float limit, uint32_t ti, uint32_t tp,
+                   float &best_t, uint32_t &best_i, uint32_t &best_p, uint32_t *stk,
+                   uint32_t stride, TraceCnt &cnt, bool &overflow) {
+  return traverse_mode<kAny ? kTraceAny : kTraceClosest, kCount>(S, o, d, limit, ti, tp, best_t, best_i, best_p, stk,
+                                                                  stride, cnt, overflow);
 }
 
 }  // namespace dev

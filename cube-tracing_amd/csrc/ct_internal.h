@@ -33,7 +33,17 @@ constexpr uint32_t kLeaf = 0x80000000u;
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 constexpr float kHalfVoxel = 0.015625f;      // HALF_VOXEL_EXTENT
 constexpr float kLeafPad = 0.000244140625f;  // 2^-12 node padding
-constexpr int kStackSize = 32;               // LDS traversal stack entries/lane
+// Traversal stack per lane: the top CT_STACK_REG entries live in registers
+// (a shift register), deeper ones spill to CT_STACK_LDS LDS slots.
+#ifndef CT_STACK_REG
+#define CT_STACK_REG 4
+#endif
+#ifndef CT_STACK_LDS
+#define CT_STACK_LDS 28
+#endif
+constexpr int kStackReg = CT_STACK_REG;
+constexpr int kStackLds = CT_STACK_LDS;
+constexpr int kStackSize = kStackReg + kStackLds;  // capacity the build's depth must fit
 constexpr uint32_t kRaySlots = 16;           // shadow-ray slots per pixel and pass
 
 // BVH2 node, 64 B: child0 box (lo xyz, hi xyz), child1 box, two refs.
@@ -63,7 +73,7 @@ struct DScene {
   const DNode *tlas_nodes;
   const uint32_t *tlas_inst;  // instance id per TLAS leaf
   uint32_t tlas_root;
-  uint32_t pad;
+  uint32_t n_inst;            // entries of `inst` (instance slots)
 };
 
 // Raw device buffer with RAII.
@@ -161,6 +171,8 @@ struct FrameBufs {
   DevBuf velocity, img[2], taa, hit, pidx, prev_res_copy;
   DevBuf mask, gather;                                // per-pixel pass state
   DevBuf ray_o, ray_d, ray_t, queue, counters, counters_host_staging;
+  DevBuf pres[2], indirect;  // ReSTIR PT: output / temporal path reservoirs, indirect colour
+  bool pt_ready = false;     // allocated (and zeroed) on the first frame with indirect light
 };
 
 struct Ctx {

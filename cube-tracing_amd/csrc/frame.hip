@@ -33,8 +33,15 @@ constexpr float kInv4Pi = 0.079577471545947667884441881686255f;
 constexpr float kMaxDistance = 1e9f;
 constexpr float kMinCosTheta = 1e-6f;
 
+// Event marks of a frame (Ctx::ev indices).
+enum {
+  kMarkStart, kMarkPrimary, kMarkRis, kMarkTraceRis, kMarkTemporalEmit, kMarkTraceTemporal, kMarkTemporalFinish,
+  kMarkPt, kMarkSpatialEmit, kMarkTraceSpatial, kMarkSpatialFinish, kMarkPtSpatial, kMarkTaa, kMarkCount
+};
+
 // counters[] slots
-constexpr int kCntQueue = 0, kCntHead = 1, kCntRis = 2, kCntTemporal = 3, kCntSpatial = 4;
+constexpr int kCntQueue = 0, kCntHead = 1, kCntRis = 2, kCntTemporal = 3, kCntSpatial = 4, kCntPtBounce = 5,
+              kCntPtShadow = 6;
 
 struct DFrame {
   uint32_t W, H, npix, frame, active, afternoon;
@@ -46,7 +53,7 @@ struct DFrame {
   DScene S;
   const ct_aabb *aabbs;
   const ct_primitive *prims;
-  const float4 *mats;   // [2 per material]: (diffuse, type), (emission, -)
+  const float4 *mats;   // [2 per material]: (diffuse, type), (emission, roughness)
   const float4 *cube;   // [3 per light]: (pos, size), (emissive, type), (inst, prim, -, -)
   const float4 *point;
   const float4 *env;
@@ -69,6 +76,9 @@ struct DFrame {
   uint32_t *queue;
   uint32_t *counters;
   float4 *gat;  // [8 per pixel] neighbour record, see st_gather
+  float4 *pres_o;    // [6 per pixel] output path reservoirs (this frame)
+  float4 *pres_t;    // [6 per pixel] temporal path reservoirs (history)
+  float4 *indirect;  // indirect colour of the first pass
 };
 
 struct PLight {
@@ -79,6 +89,7 @@ struct PLight {
 };
 struct Mat {
   F3 diffuse, emission;
+  float roughness;
 };
 struct Hit {  // HIT_INFO_INPUT, shared.inl:171-179
   F3 P, n;
@@ -175,11 +186,13 @@ CT_D Mat ld_mat(const DFrame &F, uint32_t i) {
   if (i >= F.n_mats) {
     m.diffuse = f3(0, 0, 0);
     m.emission = f3(0, 0, 0);
+    m.roughness = 0.0f;
     return m;
   }
   float4 a = F.mats[2 * i], b = F.mats[2 * i + 1];
   m.diffuse = xyz(a);
   m.emission = xyz(b);
+  m.roughness = b.w;
   return m;
 }
 CT_D bool res_valid(const Res &r) { return r.M > 0.0f && r.Y != kInvalid && r.type != CT_LIGHT_NONE; }
@@ -610,6 +623,8 @@ CT_D void camera_ray(const DFrame &F, uint32_t x, uint32_t y, F3 &o, F3 &d) {
   d = f3(d4[0], d4[1], d4[2]);
 }
 
+#include "restir_pt.h"
+
 // ----------------------------------------------------------- queue push
 // Appends this thread's emitted slots (bit k -> ray pixel*16 + k) to the
 // ray queue with one atomic per block.  Every thread of the block calls it.
@@ -649,7 +664,7 @@ CT_D void queue_flush(const DFrame &F, uint32_t px, uint32_t emitted, uint32_t c
 // Primary rays: raygen rgen.glsl:18-42 + closest hit; one 8x8 tile per wave.
 template <bool kCount>
 __global__ void __launch_bounds__(kBlock) k_primary(DFrame F, unsigned long long *stats) {
-  __shared__ uint32_t stk[kStackSize * kBlock];
+  __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   const uint32_t tiles_x = (F.W + 15) / 16;
   const uint32_t bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
@@ -839,7 +854,15 @@ __global__ void __launch_bounds__(kBlock) k_spatial(DFrame F) {
       uint32_t seed = di.seed;
       spatial_reuse(F, sr, di.conf, x, y, ray_d, hit, mat, seed, v);
       emitted = v.emitted;
-      if (!kEmit) {
+      if (!kEmit && F.rp.indirect) {
+        // The indirect term, emission and the NaN scrub follow in k_pt_spatial.
+        F3 hv = f3(0, 0, 0);
+        hv = add(hv, muls(sr.F, sr.W_y));
+        F.img[px] = make_float4(hv.x, hv.y, hv.z, 1.0f);
+        st_res(F.rA_p, F.rB_p, F.rC_p, px, sr);
+        di.seed = seed;
+        st_di(F.dpos_p, F.dnrm_p, F.dorg_p, F.dhit_p, px, di);
+      } else if (!kEmit) {
         F3 hv = f3(0, 0, 0);
         hv = add(hv, muls(sr.F, sr.W_y));
         if (F.rp.direct_emittance) hv = add(hv, mat.emission);
@@ -862,6 +885,135 @@ __global__ void __launch_bounds__(kBlock) k_spatial(DFrame F) {
 }
 
 // taa.glsl:16-108 over the dispatch region.
+// Sums a per-lane count over the wave; lane 0 adds it to `dst`.
+CT_D void wave_add(uint32_t v, uint32_t *dst) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  if (__lane_id() == 0 && v) atomicAdd(dst, v);
+}
+CT_D void wave_add_cnt(const TraceCnt &c, unsigned long long *dst) {
+  unsigned long long a = c.nodes, b = c.leaves, d = c.insts;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off);
+    b += __shfl_down(b, off);
+    d += __shfl_down(d, off);
+  }
+  if (__lane_id() == 0) {
+    atomicAdd(&dst[0], a);
+    atomicAdd(&dst[1], b);
+    atomicAdd(&dst[2], d);
+  }
+}
+
+// Indirect illumination of the first pass (rgen.glsl:271-315): initial
+// paths (trace_restir_path_tracing) and temporal path reuse
+// (indirect_illumination.glsl:454-475), after the DI temporal reuse.
+template <bool kCount>
+__global__ void __launch_bounds__(kBlock) k_pt(DFrame F, unsigned long long *stats) {
+  __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  PtTrace tr;
+  tr.stk = stk + threadIdx.x;
+  tr.bounce_rays = 0u;
+  tr.shadow_rays = 0u;
+  tr.overflow = false;
+  if (px < F.npix && F.dpos_c[px].w > 0.0f) {
+    const uint32_t x = px % F.W, y = px / F.W;
+    const DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
+    const Isect ci = isect_from_di(F, di);
+    uint32_t seed = di.seed;
+    F3 ind;
+    PRes out = pt_initial<kCount>(F, ci, seed, ind, tr);
+    if (F.rp.pt_temporal) {
+      // temporal_path_get_reprojected, indirect_illumination.glsl:29-57
+      const float2 v = F.vel[px];
+      const float r1 = rnd(seed);
+      const float r2 = rnd(seed);
+      const uint32_t pcx = f2u_sat(((float)x + v.x) + r1), pcy = f2u_sat(((float)y + v.y) + r2);
+      const uint32_t prev = min(F.npix - 1u, pcx + pcy * F.W);
+      const PRes temporal = ld_pres(F.pres_t, prev);
+      const Isect pi = isect_from_di_raw(F, ld_di(F.dpos_p, F.dnrm_p, F.dorg_p, F.dhit_p, prev));
+      out = pt_temporal<kCount>(F, out, temporal, ci, pi, seed, tr);
+      ind = muls(out.F, out.weight);
+    }
+    if (has_nan_inf(ind)) ind = f3(0, 0, 0);  // rgen.glsl:293-294
+    st_pres(F.pres_o, px, out);
+    F.indirect[px] = make_float4(ind.x, ind.y, ind.z, 0.0f);
+    F.dorg_c[px].w = __uint_as_float(seed);
+  }
+  wave_add(tr.bounce_rays, &F.counters[kCntPtBounce]);
+  wave_add(tr.shadow_rays, &F.counters[kCntPtShadow]);
+  if (kCount) wave_add_cnt(tr.cnt, stats + 6);
+}
+
+// Spatial path reuse (indirect_illumination_spatial_reuse, rgen.glsl:427-452)
+// and the rest of the shading pass: indirect term, emission, NaN scrub,
+// image store (rgen.glsl:449-489).
+template <bool kCount>
+__global__ void __launch_bounds__(kBlock) k_pt_spatial(DFrame F, unsigned long long *stats) {
+  __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  PtTrace tr;
+  tr.stk = stk + threadIdx.x;
+  tr.bounce_rays = 0u;
+  tr.shadow_rays = 0u;
+  tr.overflow = false;
+  if (px < F.npix && F.dpos_c[px].w > 0.0f) {
+    const uint32_t x = px % F.W, y = px / F.W;
+    const DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
+    uint32_t seed = __float_as_uint(F.dorg_p[px].w);  // after the DI spatial reuse
+    F3 ind;
+    if (F.rp.pt_spatial) {
+      const Isect ci = isect_from_di(F, di);
+      const F3 cam = f3(F.inv_view[12], F.inv_view[13], F.inv_view[14]);
+      const PRes fin = pt_spatial<kCount>(F, x, y, ci, cam, seed, tr);
+      st_pres(F.pres_t, px, fin);
+      ind = muls(fin.F, fin.weight);
+    } else {
+      // RESTIR_PT_SPATIAL_ON == 0: the first pass's reservoir is the history
+      // (indirect_illumination.glsl:270-272), copied here so no pixel of the
+      // first pass reads a history entry another pixel rewrites.
+      if (F.rp.pt_temporal) st_pres(F.pres_t, px, ld_pres(F.pres_o, px));
+      ind = xyz(F.indirect[px]);
+    }
+    const float4 h = F.img[px];
+    F3 hv = add(f3(h.x, h.y, h.z), ind);
+    if (F.rp.direct_emittance) hv = add(hv, ld_mat(F, di.mat).emission);
+    if (isinf(hv.x) || isnan(hv.x)) hv.x = 0.0f;
+    if (isinf(hv.y) || isnan(hv.y)) hv.y = 0.0f;
+    if (isinf(hv.z) || isnan(hv.z)) hv.z = 0.0f;
+    const float4 fp = make_float4(hv.x, hv.y, hv.z, 1.0f);
+    F.img[px] = fp;
+    if (F.active & CT_TAA_BIT) F.taa[px] = fp;
+    F.dorg_p[px].w = __uint_as_float(seed);
+  }
+  wave_add(tr.shadow_rays, &F.counters[kCntPtShadow]);
+  if (kCount) wave_add_cnt(tr.cnt, stats + 6);
+}
+
+// Path segments on caller rays (ct_trace_bounce).
+template <bool kCount>
+__global__ void __launch_bounds__(kBlock) k_trace_bounce(DFrame F, const float4 *ro, const float4 *rd, uint32_t n,
+                                                        float4 *pos_dist, float4 *nrm_ids, float4 *wo_prim,
+                                                        unsigned long long *stats) {
+  __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  PtTrace tr;
+  tr.stk = stk + threadIdx.x;
+  tr.bounce_rays = 0u;
+  tr.shadow_rays = 0u;
+  tr.overflow = false;
+  if (i < n) {
+    const Isect h = bounce_intersect<kCount>(F, xyz(ro[i]), xyz(rd[i]), tr);
+    const uint32_t inst = h.is_hit ? h.inst : kInvalid, prim = h.is_hit ? h.prim : kInvalid;
+    pos_dist[i] = make_float4(h.P.x, h.P.y, h.P.z, h.distance);
+    nrm_ids[i] = make_float4(h.n.x, h.n.y, h.n.z, __uint_as_float(inst));
+    wo_prim[i] = make_float4(h.wo.x, h.wo.y, h.wo.z, __uint_as_float(prim));
+  }
+  if (kCount) wave_add_cnt(tr.cnt, stats);
+}
+
 __global__ void __launch_bounds__(kBlock) k_taa(DFrame F, uint32_t tw, uint32_t th) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= tw * th) return;
@@ -919,7 +1071,7 @@ __global__ void k_reset_counters(uint32_t *c, int first, int n) {
 // Any-hit traversal over a ray queue; each wave fetches 64 rays at a time.
 template <bool kCount>
 __global__ void __launch_bounds__(kBlock) k_trace_any(DScene S, TraceAnyArgs A) {
-  __shared__ uint32_t stk[kStackSize * kBlock];
+  __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
   const uint32_t tid = threadIdx.x, lane = tid & 63u;
   uint32_t count = *A.count;
   if (count > A.max_rays) count = A.max_rays;
@@ -960,7 +1112,7 @@ __global__ void __launch_bounds__(kBlock) k_trace_any(DScene S, TraceAnyArgs A) 
 
 template <bool kCount>
 __global__ void __launch_bounds__(kBlock) k_trace_closest(DScene S, TraceClosestArgs A) {
-  __shared__ uint32_t stk[kStackSize * kBlock];
+  __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
   const uint32_t tid = threadIdx.x, lane = tid & 63u;
   const uint32_t i = blockIdx.x * blockDim.x + tid;
   TraceCnt cnt;
@@ -1055,7 +1207,23 @@ int zero_frame(Ctx &c, hipStream_t st) {
                     &b.resC[0],   &b.resC[1],   &b.img[0],    &b.img[1],    &b.velocity,  &b.taa,
                     &b.hit,       &b.pidx,      &b.mask,      &b.counters};
   for (DevBuf *d : bufs) CT_HIP(hipMemsetAsync(d->p, 0, d->bytes, st));
+  if (b.pt_ready)
+    for (DevBuf *d : {&b.pres[0], &b.pres[1], &b.indirect}) CT_HIP(hipMemsetAsync(d->p, 0, d->bytes, st));
   CT_HIP(hipStreamSynchronize(st));
+  return CT_OK;
+}
+
+// Path reservoirs exist only for contexts that render indirect light; they
+// start zeroed like every other frame buffer (main.cpp:234-318).
+int alloc_pt(Ctx &c, hipStream_t st) {
+  FrameBufs &b = c.fb;
+  if (b.pt_ready) return CT_OK;
+  const size_t np = (size_t)c.max_w * c.max_h;
+  int rc;
+  if ((rc = b.pres[0].reserve(96 * np)) || (rc = b.pres[1].reserve(96 * np)) || (rc = b.indirect.reserve(16 * np)))
+    return rc;
+  for (DevBuf *d : {&b.pres[0], &b.pres[1], &b.indirect}) CT_HIP(hipMemsetAsync(d->p, 0, d->bytes, st));
+  b.pt_ready = true;
   return CT_OK;
 }
 
@@ -1143,7 +1311,7 @@ int ct_ctx_append_materials(ct_ctx *c, const ct_material *m, uint32_t count, uin
     float tb;
     std::memcpy(&tb, &q.type, 4);
     pk[2 * i] = make_float4(q.diffuse[0], q.diffuse[1], q.diffuse[2], tb);
-    pk[2 * i + 1] = make_float4(q.emission[0], q.emission[1], q.emission[2], 0.0f);
+    pk[2 * i + 1] = make_float4(q.emission[0], q.emission[1], q.emission[2], q.roughness);
   }
   CT_HIP(hipSetDevice(c->device));
   int rc;
@@ -1208,11 +1376,19 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
   if ((p->is_active & CT_TAA_BIT) && !p->has_previous)
     return fail(CT_E_INVALID, "ct_frame_render: TAA needs a previous frame");
   if (a->set.tlas_count == 0) return fail(CT_E_STATE, "ct_frame_render: empty scene (call update_scene first)");
+  const bool pt = p->restir.indirect != 0;
+  if (pt && (p->restir.max_depth == 0 || p->restir.max_depth > CT_MAX_PT_DEPTH))
+    return fail(CT_E_INVALID, "ct_frame_render: max_depth must be in [1, CT_MAX_PT_DEPTH]");
+  if (pt && p->restir.pt_neighbor_count > CT_MAX_PT_NEIGHBORS)
+    return fail(CT_E_INVALID, "ct_frame_render: pt_neighbor_count above CT_MAX_PT_NEIGHBORS");
+  if (pt && p->restir.pt_shift_mapping != CT_SHIFT_RECONNECTION && p->restir.pt_shift_mapping != CT_SHIFT_HYBRID)
+    return fail(CT_E_UNSUPPORTED, "ct_frame_render: only the reconnection and hybrid shifts exist");
   CT_HIP(hipSetDevice(c->device));
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  int rc;
+  if (pt && (rc = alloc_pt(*c, st))) return rc;
   // Cube lights from the manager (packed), re-uploaded on a new build.
   static thread_local std::vector<float4> pk;
-  int rc;
   const uint32_t ncube = a->set.cube_light_count;
   pack_lights(a->cube_lights_host.data(), ncube, pk);
   DevBuf &cube = c->inst_pt;  // reused: [packed cube lights | per-instance matrices]
@@ -1275,6 +1451,9 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
   F.queue = b.queue.as<uint32_t>();
   F.counters = b.counters.as<uint32_t>();
   F.gat = b.gather.as<float4>();
+  F.pres_o = b.pres[0].as<float4>();
+  F.pres_t = b.pres[1].as<float4>();
+  F.indirect = b.indirect.as<float4>();
   unsigned long long *stats = (c->instrument & 2u) ? b.prev_res_copy.as<unsigned long long>() : nullptr;
   const bool ev = (c->instrument & 1u) != 0;
   int ne = 0;
@@ -1288,52 +1467,125 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
   ta.ray_o = F.ray_o; ta.ray_d = F.ray_d; ta.ray_t = F.ray_t; ta.queue = F.queue;
   ta.count = F.counters + kCntQueue; ta.head = F.counters + kCntHead; ta.mask = F.mask; ta.stats = stats;
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, 0, 8);
-  if (stats) CT_HIP(hipMemsetAsync(stats, 0, 6 * sizeof(unsigned long long), st));
-  if ((rc = mark())) return rc;  // 0
+  if (stats) CT_HIP(hipMemsetAsync(stats, 0, 9 * sizeof(unsigned long long), st));
+  if ((rc = mark())) return rc;  // kMarkStart
   const uint32_t tiles = ((F.W + 15) / 16) * ((F.H + 15) / 16);
   if (stats)
     hipLaunchKernelGGL(k_primary<true>, dim3(tiles), dim3(kBlock), 0, st, F, stats + 3);
   else
     hipLaunchKernelGGL(k_primary<false>, dim3(tiles), dim3(kBlock), 0, st, F, stats);
-  if ((rc = mark())) return rc;  // 1
+  if ((rc = mark())) return rc;  // kMarkPrimary
   hipLaunchKernelGGL(k_ris, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
-  if ((rc = mark())) return rc;  // 2
+  if ((rc = mark())) return rc;  // kMarkRis
   ta.max_rays = np;
   if ((rc = launch_trace_any(F.S, ta, st))) return rc;
-  if ((rc = mark())) return rc;  // 3
+  if ((rc = mark())) return rc;  // kMarkTraceRis
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, kCntQueue, 1);
   hipLaunchKernelGGL(k_temporal_emit, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
-  if ((rc = mark())) return rc;  // 4
+  if ((rc = mark())) return rc;  // kMarkTemporalEmit
   ta.max_rays = 2 * np;
   if ((rc = launch_trace_any(F.S, ta, st))) return rc;
-  if ((rc = mark())) return rc;  // 5
+  if ((rc = mark())) return rc;  // kMarkTraceTemporal
   hipLaunchKernelGGL(k_temporal_finish, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
-  if ((rc = mark())) return rc;  // 6
+  if ((rc = mark())) return rc;  // kMarkTemporalFinish
+  if (pt) {
+    if (stats)
+      hipLaunchKernelGGL(k_pt<true>, dim3(nblocks(np)), dim3(kBlock), 0, st, F, stats);
+    else
+      hipLaunchKernelGGL(k_pt<false>, dim3(nblocks(np)), dim3(kBlock), 0, st, F, stats);
+  }
+  if ((rc = mark())) return rc;  // kMarkPt
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, kCntQueue, 1);
   hipLaunchKernelGGL(k_spatial<true>, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
-  if ((rc = mark())) return rc;  // 7
+  if ((rc = mark())) return rc;  // kMarkSpatialEmit
   ta.max_rays = 2 * p->restir.max_neighbors * np;
   if ((rc = launch_trace_any(F.S, ta, st))) return rc;
-  if ((rc = mark())) return rc;  // 8
+  if ((rc = mark())) return rc;  // kMarkTraceSpatial
   hipLaunchKernelGGL(k_spatial<false>, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
-  if ((rc = mark())) return rc;  // 9
+  if ((rc = mark())) return rc;  // kMarkSpatialFinish
+  if (pt) {
+    if (stats)
+      hipLaunchKernelGGL(k_pt_spatial<true>, dim3(nblocks(np)), dim3(kBlock), 0, st, F, stats);
+    else
+      hipLaunchKernelGGL(k_pt_spatial<false>, dim3(nblocks(np)), dim3(kBlock), 0, st, F, stats);
+  }
+  if ((rc = mark())) return rc;  // kMarkPtSpatial
   if (p->is_active & CT_TAA_BIT) {
     uint32_t tw = p->restir.taa_truncate_dispatch ? (F.W / 8) * 8 : F.W;
     uint32_t th = p->restir.taa_truncate_dispatch ? (F.H / 8) * 8 : F.H;
     if (tw && th) hipLaunchKernelGGL(k_taa, dim3(nblocks((uint64_t)tw * th)), dim3(kBlock), 0, st, F, tw, th);
   }
-  if ((rc = mark())) return rc;  // 10
+  if ((rc = mark())) return rc;  // kMarkTaa
   CT_HIP(hipGetLastError());
   c->n_events = ne;
   // Counters for the statistics (copied once the frame is done).
   CT_HIP(hipMemcpyAsync(c->counters_pinned, F.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  if (stats) CT_HIP(hipMemcpyAsync(c->counters_pinned + 8, stats, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  if (stats) CT_HIP(hipMemcpyAsync(c->counters_pinned + 8, stats, 9 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
   c->stats_pending = true;
   c->stats = ct_frame_stats{};
   c->stats.primary_rays = np;
   c->stream_last = st;
   c->last_w = F.W;
   c->last_h = F.H;
+  return CT_OK;
+}
+
+int ct_trace_bounce(ct_accel *a, uint32_t n, const float *origins, const float *directions, uint32_t *ids_out,
+                    float *pos_out, float *nrm_out, float *dist_out, float *wo_out, ct_trace_stats *stats) {
+  if (!a || (n && (!origins || !directions || !ids_out || !pos_out || !nrm_out || !dist_out || !wo_out)))
+    return fail(CT_E_INVALID, "ct_trace_bounce: null argument");
+  if (a->set.tlas_count == 0) return fail(CT_E_STATE, "ct_trace_bounce: empty scene");
+  if (n == 0) return CT_OK;
+  CT_HIP(hipSetDevice(a->device));
+  DevBuf o, d, r0, r1, r2, st;
+  int rc;
+  if ((rc = o.reserve(16 * (size_t)n)) || (rc = d.reserve(16 * (size_t)n)) || (rc = r0.reserve(16 * (size_t)n)) ||
+      (rc = r1.reserve(16 * (size_t)n)) || (rc = r2.reserve(16 * (size_t)n)) ||
+      (rc = st.reserve(3 * sizeof(unsigned long long))))
+    return rc;
+  std::vector<float4> ho(n), hd(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    ho[i] = make_float4(origins[3 * i], origins[3 * i + 1], origins[3 * i + 2], 0.0f);
+    hd[i] = make_float4(directions[3 * i], directions[3 * i + 1], directions[3 * i + 2], 0.0f);
+  }
+  CT_HIP(hipMemcpy(o.p, ho.data(), 16 * (size_t)n, hipMemcpyHostToDevice));
+  CT_HIP(hipMemcpy(d.p, hd.data(), 16 * (size_t)n, hipMemcpyHostToDevice));
+  CT_HIP(hipMemset(st.p, 0, 3 * sizeof(unsigned long long)));
+  DFrame F;
+  std::memset(&F, 0, sizeof(F));
+  F.S = scene_view(*a);
+  F.aabbs = a->set.aabbs.as<ct_aabb>();
+  F.prims = a->set.prims.as<ct_primitive>();
+  F.n_mats = 0;  // materials live in a frame context; the records here carry indices only
+  if (stats)
+    hipLaunchKernelGGL(k_trace_bounce<true>, dim3(nblocks(n)), dim3(kBlock), 0, a->stream, F, o.as<float4>(),
+                       d.as<float4>(), n, r0.as<float4>(), r1.as<float4>(), r2.as<float4>(),
+                       st.as<unsigned long long>());
+  else
+    hipLaunchKernelGGL(k_trace_bounce<false>, dim3(nblocks(n)), dim3(kBlock), 0, a->stream, F, o.as<float4>(),
+                       d.as<float4>(), n, r0.as<float4>(), r1.as<float4>(), r2.as<float4>(), nullptr);
+  CT_HIP(hipGetLastError());
+  CT_HIP(hipStreamSynchronize(a->stream));
+  std::vector<float4> a0(n), a1(n), a2(n);
+  CT_HIP(hipMemcpy(a0.data(), r0.p, 16 * (size_t)n, hipMemcpyDeviceToHost));
+  CT_HIP(hipMemcpy(a1.data(), r1.p, 16 * (size_t)n, hipMemcpyDeviceToHost));
+  CT_HIP(hipMemcpy(a2.data(), r2.p, 16 * (size_t)n, hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < n; ++i) {
+    pos_out[3 * i] = a0[i].x; pos_out[3 * i + 1] = a0[i].y; pos_out[3 * i + 2] = a0[i].z;
+    dist_out[i] = a0[i].w;
+    nrm_out[3 * i] = a1[i].x; nrm_out[3 * i + 1] = a1[i].y; nrm_out[3 * i + 2] = a1[i].z;
+    std::memcpy(&ids_out[2 * i], &a1[i].w, 4);
+    wo_out[3 * i] = a2[i].x; wo_out[3 * i + 1] = a2[i].y; wo_out[3 * i + 2] = a2[i].z;
+    std::memcpy(&ids_out[2 * i + 1], &a2[i].w, 4);
+  }
+  if (stats) {
+    unsigned long long v[3];
+    CT_HIP(hipMemcpy(v, st.p, sizeof(v), hipMemcpyDeviceToHost));
+    stats->rays = n;
+    stats->nodes = v[0];
+    stats->leaves = v[1];
+    stats->instances = v[2];
+  }
   return CT_OK;
 }
 
@@ -1346,7 +1598,9 @@ int ct_frame_get_stats(ct_ctx *c, ct_frame_stats *out) {
     c->stats.shadow_rays_ris = cnt[kCntRis];
     c->stats.shadow_rays_temporal = cnt[kCntTemporal];
     c->stats.shadow_rays_spatial = cnt[kCntSpatial];
-    c->stats.shadow_rays = (uint64_t)cnt[kCntRis] + cnt[kCntTemporal] + cnt[kCntSpatial];
+    c->stats.shadow_rays_pt = cnt[kCntPtShadow];
+    c->stats.bounce_rays = cnt[kCntPtBounce];
+    c->stats.shadow_rays = (uint64_t)cnt[kCntRis] + cnt[kCntTemporal] + cnt[kCntSpatial] + cnt[kCntPtShadow];
     if (c->instrument & 2u) {
       const unsigned long long *s = reinterpret_cast<const unsigned long long *>(c->counters_pinned + 8);
       c->stats.nodes = s[0];
@@ -1355,20 +1609,27 @@ int ct_frame_get_stats(ct_ctx *c, ct_frame_stats *out) {
       c->stats.closest_nodes = s[3];
       c->stats.closest_leaves = s[4];
       c->stats.closest_instances = s[5];
+      c->stats.pt_nodes = s[6];
+      c->stats.pt_leaves = s[7];
+      c->stats.pt_instances = s[8];
     }
-    if ((c->instrument & 1u) && c->n_events >= 11) {
-      float ms[10];
-      for (int k = 0; k < 10; ++k) CT_HIP(hipEventElapsedTime(&ms[k], c->ev[k], c->ev[k + 1]));
-      c->stats.ms_primary = ms[0];
-      c->stats.ms_ris = ms[1];
-      c->stats.ms_temporal = ms[3] + ms[5];
-      c->stats.ms_spatial = ms[6] + ms[8];
-      c->stats.ms_taa = ms[9];
-      c->stats.ms_trace_any = ms[2] + ms[4] + ms[7];
-      c->stats.ms_trace_closest = ms[0];
-      float tot;
-      CT_HIP(hipEventElapsedTime(&tot, c->ev[0], c->ev[10]));
-      c->stats.ms_total = tot;
+    if ((c->instrument & 1u) && c->n_events >= kMarkCount) {
+      auto ms = [&](int a, int b) -> float {
+        float v = 0.0f;
+        (void)hipEventElapsedTime(&v, c->ev[a], c->ev[b]);
+        return v;
+      };
+      c->stats.ms_primary = ms(kMarkStart, kMarkPrimary);
+      c->stats.ms_ris = ms(kMarkPrimary, kMarkRis);
+      c->stats.ms_temporal = ms(kMarkTraceRis, kMarkTemporalEmit) + ms(kMarkTraceTemporal, kMarkTemporalFinish);
+      c->stats.ms_pt = ms(kMarkTemporalFinish, kMarkPt);
+      c->stats.ms_spatial = ms(kMarkPt, kMarkSpatialEmit) + ms(kMarkTraceSpatial, kMarkSpatialFinish);
+      c->stats.ms_pt_spatial = ms(kMarkSpatialFinish, kMarkPtSpatial);
+      c->stats.ms_taa = ms(kMarkPtSpatial, kMarkTaa);
+      c->stats.ms_trace_any = ms(kMarkRis, kMarkTraceRis) + ms(kMarkTemporalEmit, kMarkTraceTemporal) +
+                              ms(kMarkSpatialEmit, kMarkTraceSpatial);
+      c->stats.ms_trace_closest = c->stats.ms_primary;
+      c->stats.ms_total = ms(kMarkStart, kMarkTaa);
     }
     c->stats_pending = false;
   }
@@ -1434,6 +1695,47 @@ int ct_frame_read(ct_ctx *c, uint32_t which, void *dst, size_t bytes) {
         o[i].Y = A[i].x; o[i].l_type = A[i].y; o[i].seed = A[i].z;
         std::memcpy(&o[i].W_y, &A[i].w, 4);
         o[i].W_sum = B[i].x; o[i].M = B[i].y; o[i].F[0] = B[i].z; o[i].F[1] = B[i].w; o[i].F[2] = C[i];
+      }
+      return CT_OK;
+    }
+    case CT_BUF_PATH_RESERVOIR:
+    case CT_BUF_TEMPORAL_PATH_RESERVOIR: {
+      if (!need(sizeof(ct_path_reservoir) * np)) return fail(CT_E_INVALID, "ct_frame_read: buffer too small");
+      ct_path_reservoir *o = static_cast<ct_path_reservoir *>(dst);
+      if (!b.pt_ready) {
+        std::memset(o, 0, sizeof(ct_path_reservoir) * np);
+        return CT_OK;
+      }
+      std::vector<float4> Q(6 * np);
+      CT_HIP(hipMemcpy(Q.data(), b.pres[which == CT_BUF_PATH_RESERVOIR ? 0 : 1].p, 96 * np, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < np; ++i) {
+        const float4 *q = &Q[6 * i];
+        ct_path_reservoir &r = o[i];
+        r.M = q[0].x; r.weight = q[0].y;
+        std::memcpy(&r.path_flags, &q[0].z, 4);
+        std::memcpy(&r.rc_random_seed, &q[0].w, 4);
+        r.F[0] = q[1].x; r.F[1] = q[1].y; r.F[2] = q[1].z; r.light_pdf = q[1].w;
+        r.cached_jacobian[0] = q[2].x; r.cached_jacobian[1] = q[2].y; r.cached_jacobian[2] = q[2].z;
+        std::memcpy(&r.init_random_seed, &q[2].w, 4);
+        std::memcpy(&r.rc_vertex_hit.object.instance_id, &q[3].x, 4);
+        std::memcpy(&r.rc_vertex_hit.object.primitive_id, &q[3].y, 4);
+        r.rc_vertex_hit.hit[0] = q[3].z; r.rc_vertex_hit.hit[1] = q[3].w; r.rc_vertex_hit.hit[2] = q[4].x;
+        r.rc_vertex_wi[0] = q[4].y; r.rc_vertex_wi[1] = q[4].z; r.rc_vertex_wi[2] = q[4].w;
+        r.rc_vertex_irradiance[0] = q[5].x; r.rc_vertex_irradiance[1] = q[5].y; r.rc_vertex_irradiance[2] = q[5].z;
+      }
+      return CT_OK;
+    }
+    case CT_BUF_INDIRECT: {
+      if (!need(12 * np)) return fail(CT_E_INVALID, "ct_frame_read: buffer too small");
+      float *o = static_cast<float *>(dst);
+      if (!b.pt_ready) {
+        std::memset(o, 0, 12 * np);
+        return CT_OK;
+      }
+      std::vector<float4> Q(np);
+      CT_HIP(hipMemcpy(Q.data(), b.indirect.p, 16 * np, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < np; ++i) {
+        o[3 * i] = Q[i].x; o[3 * i + 1] = Q[i].y; o[3 * i + 2] = Q[i].z;
       }
       return CT_OK;
     }

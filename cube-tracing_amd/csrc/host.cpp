@@ -137,6 +137,17 @@ void ct_restir_params_reference(ct_restir_params *o) {
   o->taa_alpha = 0.8f;            // taa.glsl:13
   o->taa_truncate_dispatch = 1;   // main.cpp:1642-1643
   o->direct_emittance = 1;        // defines.glsl:19
+  o->indirect = 0;                // INDIRECT_ILLUMINATION_ON, defines.glsl:20
+  o->max_depth = 3;               // MAX_DEPTH, shared.inl:61 (main.cpp:1330)
+  o->pt_temporal = 1;             // RESTIR_PT_TEMPORAL_ON, defines.glsl:15
+  o->pt_spatial = 1;              // RESTIR_PT_SPATIAL_ON, defines.glsl:16
+  o->pt_shift_mapping = CT_SHIFT_RECONNECTION;  // rgen.glsl:282
+  o->pt_neighbor_count = 3;       // NEIGHBOR_COUNT, defines.glsl:61
+  o->pt_neighbor_radius = 1;      // NEIGHBOR_RADIUS, defines.glsl:62
+  o->pt_m_cap = 20.0f;            // MAX_INFLUENCE_FROM_THE_PAST_THRESHOLD_PT, defines.glsl:58
+  o->near_field_distance = 0.015625f * 0.5f;  // NEAR_FIELD_DISTANCE, defines.glsl:53
+  o->roughness_threshold = 0.2f;  // SPECULAR_ROUGHNESS_THRESHOLD, defines.glsl:55
+  o->pt_russian_roulette = 0;     // USE_RUSSIAN_ROULETTE, defines.glsl:59
 }
 
 int ct_mat4_inverse(const float m[16], float out[16]) {

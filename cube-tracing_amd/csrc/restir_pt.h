@@ -1,0 +1,820 @@
+// restir_pt.h -- ReSTIR PT on the device: the initial path tracer
+// (trace_restir_path_tracing, path_tracer.glsl:854-887), the bounce
+// closest hit it traces (intersect(), bounce.glsl:9-130), the reconnection
+// shift (shift.glsl:25-350, 445-542), temporal path reuse
+// (indirect_illumination.glsl:147-279) and spatial path reuse (:312-452).
+//
+// Included by frame.hip inside its anonymous namespace, after the DI
+// helpers (Mat, Hit, sample_lights, material_pdf, evaluate_material).
+// Each function restates the reference function named beside it for
+// Lambertian materials; the arithmetic is written in the GLSL order.
+//
+// Traversals run inline (closest hit for path segments, any hit for the
+// reconnection visibility), with the lane's traversal stack passed in.
+#pragma once
+
+constexpr float kHlfMax = 6.5504e+4f;         // HLF_MAX, shared.inl:56
+constexpr float kHlfMin = 1.175494351e-38f;   // HLF_MIN, shared.inl:57
+constexpr float kPi = 3.1415926535897932384626433832795f;
+constexpr float kVoxelExtent = 0.03125f;      // VOXEL_EXTENT, shared.inl:45
+constexpr uint32_t kMaxMaterials = 10000u;    // MAX_MATERIALS, shared.inl:20
+constexpr uint32_t kGeomLightCube = 2u, kGeomLightAnalytic = 4u, kGeomLightEnv = 5u;  // shared.inl:336-341
+// PATH_FLAG_*, defines.glsl:64-76
+constexpr uint32_t kPathActive = 0x1u, kPathHit = 0x2u, kPathTransmission = 0x4u, kPathDelta = 0x10u,
+                   kPathLightUpper = 0x80u, kPathLightLower = 0x100u, kPathDiffusePrimary = 0x200u;
+
+struct ObjHit {  // OBJECT_HIT
+  uint32_t inst, prim;
+  F3 hit;
+};
+struct Isect {  // INTERSECT, shared.inl:323-334
+  bool is_hit;
+  float distance;
+  F3 P, n, wo, wi;
+  uint32_t inst, prim, mat_idx;
+  Mat mat;
+};
+struct PRes {  // PATH_RESERVOIR, shared.inl:405-423
+  float M, weight;
+  uint32_t flags, rc_seed;
+  F3 F;
+  float light_pdf;
+  F3 cj;
+  uint32_t init_seed;
+  ObjHit rc;
+  F3 rc_wi, rc_irr;
+};
+struct PBuilder {  // PATH_BUILDER, path_builder.glsl:8-17
+  ObjHit rc;
+  F3 rc_wi;
+  uint32_t cached_seed, seed, rc_len, flags;
+  F3 cj;
+};
+struct PState {  // PATH_STATE, path_state.glsl:23-78 (fields the Lambertian path reads)
+  uint32_t max_depth, flags, path_length, bounce_counters;
+  float scene_length;
+  F3 origin, dir;
+  float pdf;
+  F3 normal;
+  ObjHit hit;
+  F3 thp, prefix_thp, L;
+  float rr_pdf, prev_scatter_pdf;
+  uint32_t seed;
+  PBuilder pb;
+  PRes res;
+  bool last_rough;
+};
+
+// Traversal context of one lane.
+struct PtTrace {
+  uint32_t *stk;
+  TraceCnt cnt;
+  uint32_t bounce_rays, shadow_rays;
+  bool overflow;
+};
+
+// ---------------------------------------------------------------- helpers
+CT_D ObjHit objhit_invalid() {
+  ObjHit h;
+  h.inst = kMaxInstances;
+  h.prim = kMaxPrimitives;
+  h.hit = f3(0, 0, 0);
+  return h;
+}
+CT_D bool instance_hit_exists(uint32_t inst, uint32_t prim) {  // primitives.glsl:542-545
+  return inst < kMaxInstances && prim < kMaxPrimitives;
+}
+// path_reservoir.glsl:64-147
+CT_D uint32_t pf_path_length(uint32_t f) { return f & 0xFu; }
+CT_D uint32_t pf_rc_length(uint32_t f) { return (f >> 4) & 0xFu; }
+CT_D void pf_set_path_length(uint32_t &f, uint32_t l) { f = (f & ~0xFu) | (l & 0xFu); }
+CT_D void pf_set_rc_length(uint32_t &f, uint32_t l) { f = (f & ~0xF0u) | ((l & 0xFu) << 4); }
+CT_D bool pf_delta(uint32_t f, bool before) { return ((f >> (before ? 20 : 21)) & 1u) != 0u; }
+CT_D bool pf_transmission(uint32_t f, bool before) { return ((f >> (before ? 22 : 23)) & 1u) != 0u; }
+CT_D void pf_set_last_nee(uint32_t &f, bool b) { f = (f & ~0x10000u) | ((b ? 1u : 0u) << 16); }
+CT_D bool pf_last_nee(uint32_t f) { return ((f >> 16) & 1u) != 0u; }
+// insert_light_type keeps two bits, so GEOMETRY_LIGHT_ENV_MAP (5) reads
+// back as 1 (path_reservoir.glsl:137-145).
+CT_D void pf_set_light_type(uint32_t &f, uint32_t t) { f = (f & ~0xc0000u) | ((t & 3u) << 18); }
+CT_D uint32_t pf_light_type(uint32_t f) { return (f >> 18) & 3u; }
+
+CT_D float lum601(F3 c) { return c.x * 0.299f + c.y * 0.587f + c.z * 0.114f; }  // path_F_to_scalar
+// eval_mis with the power heuristic, light.glsl:171-191
+CT_D float eval_mis(float f_pdf, float g_pdf) {
+  const float a = 1.0f * f_pdf, b = 1.0f * g_pdf;
+  const float f = a * a, g = b * b;
+  return f / (f + g);
+}
+CT_D bool has_nan_inf(F3 v) {
+  return isnan(v.x) || isnan(v.y) || isnan(v.z) || isinf(v.x) || isinf(v.y) || isinf(v.z);
+}
+CT_D bool jacobian_invalid(float j) { return j <= 0.0f || isnan(j) || isinf(j); }  // shift.glsl:12-14
+
+CT_D PRes pres_init(uint32_t max_depth) {  // path_reservoir_initialise, path_reservoir.glsl:151-165
+  PRes r;
+  r.M = 0.0f;
+  r.weight = 0.0f;
+  r.flags = 0u;
+  pf_set_rc_length(r.flags, max_depth);
+  r.rc_seed = 0u;
+  r.F = f3(0, 0, 0);
+  r.light_pdf = 0.0f;
+  r.cj = f3(0, 0, 0);
+  r.init_seed = 0u;
+  r.rc = objhit_invalid();
+  r.rc_wi = f3(0, 0, 0);
+  r.rc_irr = f3(0, 0, 0);
+  return r;
+}
+// 6 x float4 per pixel
+CT_D void st_pres(float4 *B, uint32_t i, const PRes &r) {
+  float4 *q = B + 6 * (size_t)i;
+  q[0] = make_float4(r.M, r.weight, __uint_as_float(r.flags), __uint_as_float(r.rc_seed));
+  q[1] = make_float4(r.F.x, r.F.y, r.F.z, r.light_pdf);
+  q[2] = make_float4(r.cj.x, r.cj.y, r.cj.z, __uint_as_float(r.init_seed));
+  q[3] = make_float4(__uint_as_float(r.rc.inst), __uint_as_float(r.rc.prim), r.rc.hit.x, r.rc.hit.y);
+  q[4] = make_float4(r.rc.hit.z, r.rc_wi.x, r.rc_wi.y, r.rc_wi.z);
+  q[5] = make_float4(r.rc_irr.x, r.rc_irr.y, r.rc_irr.z, 0.0f);
+}
+CT_D PRes ld_pres(const float4 *B, uint32_t i) {
+  const float4 *q = B + 6 * (size_t)i;
+  float4 a = q[0], b = q[1], c = q[2], d = q[3], e = q[4], g = q[5];
+  PRes r;
+  r.M = a.x; r.weight = a.y; r.flags = __float_as_uint(a.z); r.rc_seed = __float_as_uint(a.w);
+  r.F = f3(b.x, b.y, b.z); r.light_pdf = b.w;
+  r.cj = f3(c.x, c.y, c.z); r.init_seed = __float_as_uint(c.w);
+  r.rc.inst = __float_as_uint(d.x); r.rc.prim = __float_as_uint(d.y); r.rc.hit = f3(d.z, d.w, e.x);
+  r.rc_wi = f3(e.y, e.z, e.w);
+  r.rc_irr = f3(g.x, g.y, g.z);
+  return r;
+}
+
+// path_reservoir_update, path_reservoir.glsl:183-202
+CT_D bool pres_update(PRes &r, F3 in_F, float p, uint32_t &seed) {
+  r.M += 1.0f;
+  const float w = lum601(in_F) / p;
+  if (isnan(w) || w == 0.0f) return false;
+  r.weight += w;
+  if (rnd(seed) * r.weight <= w) {
+    r.F = in_F;
+    return true;
+  }
+  return false;
+}
+CT_D void pres_take(PRes &r, const PRes &in, F3 in_F) {
+  r.flags = in.flags;
+  r.rc_seed = in.rc_seed;
+  r.init_seed = in.init_seed;
+  r.cj = in.cj;
+  r.light_pdf = in.light_pdf;
+  r.rc_wi = in.rc_wi;
+  r.rc = in.rc;
+  r.rc_irr = in.rc_irr;
+  r.F = in_F;
+}
+// path_reservoir_merge, path_reservoir.glsl:204-232
+CT_D bool pres_merge(PRes &r, F3 in_F, float jac, const PRes &in, uint32_t &seed, float mis, bool force) {
+  const float w = lum601(in_F) * jac * in.M * in.weight * mis;
+  r.M += in.M;
+  if (isnan(w) || w == 0.0f) return false;
+  r.weight += w;
+  if (force || rnd(seed) * r.weight <= w) {
+    pres_take(r, in, in_F);
+    return true;
+  }
+  return false;
+}
+// path_reservoir_merge_with_resampling_MIS, path_reservoir.glsl:234-263
+CT_D bool pres_merge_mis(PRes &r, F3 in_F, float jac, const PRes &in, uint32_t &seed, float mis) {
+  const float w = lum601(in_F) * jac * in.weight * mis;
+  r.M += in.M;
+  if (isnan(w) || w == 0.0f) return false;
+  r.weight += w;
+  if (rnd(seed) * r.weight <= w) {
+    pres_take(r, in, in_F);
+    return true;
+  }
+  return false;
+}
+CT_D void pres_finalize_ris(PRes &r) {  // path_reservoir.glsl:265-271
+  const float p_hat = lum601(r.F);
+  r.weight = (p_hat == 0.0f || r.M == 0.0f) ? 0.0f : r.weight / (p_hat * r.M);
+}
+CT_D void pres_finalize_gris(PRes &r) {  // path_reservoir.glsl:273-279
+  const float p_hat = lum601(r.F);
+  r.weight = (p_hat == 0.0f) ? 0.0f : r.weight / p_hat;
+}
+
+// ---------------------------------------------------------------- geometry
+CT_D Isect isect_empty() {  // intersect_initiliaze, primitives.glsl:197-209
+  Isect i;
+  i.is_hit = false;
+  i.distance = 0.0f;
+  i.P = f3(0, 0, 0); i.n = f3(0, 0, 0); i.wo = f3(0, 0, 0); i.wi = f3(0, 0, 0);
+  i.inst = kMaxInstances; i.prim = kMaxPrimitives; i.mat_idx = kMaxMaterials;
+  i.mat.diffuse = f3(0, 0, 0); i.mat.emission = f3(0, 0, 0); i.mat.roughness = 0.0f;
+  return i;
+}
+// transpose(world2obj) * vec4(n, 0) for the 3-row packing; the fourth row
+// of an affine world-to-object matrix is (0, 0, 0, 1).
+CT_D F3 xform_normal(const float *w, F3 n) {
+  return f3(((w[0] * n.x + w[1] * n.y) + w[2] * n.z) + 0.0f * 0.0f,
+            ((w[3] * n.x + w[4] * n.y) + w[5] * n.z) + 0.0f * 0.0f,
+            ((w[6] * n.x + w[7] * n.y) + w[8] * n.z) + 0.0f * 0.0f);
+}
+CT_D F3 voxel_center(const DFrame &F, uint32_t gp) {
+  const ct_aabb &bb = F.aabbs[gp];
+  return muls(add(f3(bb.minimum[0], bb.minimum[1], bb.minimum[2]), f3(bb.maximum[0], bb.maximum[1], bb.maximum[2])),
+              0.5f);
+}
+// is_hit_from_ray (primitives.glsl:412-453) with rayCanStartInBox = true
+// on the current instance buffer; world-space hit position and normal.
+CT_D bool hit_from_ray(const DFrame &F, uint32_t inst, uint32_t prim, F3 ro, F3 rd, float &t, F3 &pos, F3 &nrm) {
+  if (inst >= F.S.n_inst) return false;
+  const DInst &I = F.S.inst[inst];
+  if (prim >= I.count) return false;
+  float w[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) w[k] = I.w2o[k];
+  F3 oo = xform3(w, ro, 1.0f), od = xform3(w, rd, 0.0f);
+  F3 sinv = f3(safe_inverse(od.x), safe_inverse(od.y), safe_inverse(od.z));
+  F3 sg = f3(-gsign(od.x), -gsign(od.y), -gsign(od.z));
+  F3 on;
+  bool h = intersect_box<true, false>(voxel_center(F, I.first + prim), oo, od, sg, sinv, t, &on);
+  pos = add(ro, muls(rd, t));
+  nrm = xform_normal(w, on);
+  return h;
+}
+CT_D uint32_t material_of(const DFrame &F, uint32_t inst, uint32_t prim) {
+  return F.prims[F.S.inst[inst].first + prim].material_index;
+}
+
+// intersect(), bounce.glsl:9-130: the box tests start at the origin pushed
+// along the ray (compute_ray_origin(origin, dir)); distance and wo use the
+// un-pushed origin.
+template <bool kCount>
+CT_D Isect bounce_intersect(const DFrame &F, F3 origin, F3 dir, PtTrace &tr) {
+  const F3 ro = compute_ray_origin(origin, dir);
+  float bt = kMaxDistance;
+  uint32_t bi = kInvalid, bp = kInvalid;
+  ++tr.bounce_rays;
+  traverse_mode<kTraceBounce, kCount>(F.S, ro, dir, kMaxDistance, kInvalid, kInvalid, bt, bi, bp, tr.stk, kBlock,
+                                      tr.cnt, tr.overflow);
+  Isect i = isect_empty();
+  i.mat_idx = 0u;
+  F3 int_hit = f3(0, 0, 0);
+  if (bi != kInvalid) {
+    float t;
+    F3 pos, nrm;
+    hit_from_ray(F, bi, bp, ro, dir, t, pos, nrm);
+    int_hit = pos;
+    i.is_hit = true;
+    i.P = pos;
+    i.n = nrm;
+    i.inst = bi;
+    i.prim = bp;
+    i.mat_idx = material_of(F, bi, bp);
+    i.mat = ld_mat(F, i.mat_idx);
+    i.distance = length(sub(origin, int_hit));
+  }
+  i.wo = normalize(sub(origin, int_hit));
+  return i;
+}
+
+// is_vertex_visible (light.glsl:82-142) on the current TLAS.
+template <bool kCount>
+CT_D bool vertex_visible(const DFrame &F, F3 o, F3 d, float dist, uint32_t ti, uint32_t tp, PtTrace &tr) {
+  float bt = 0.0f;
+  uint32_t bi = kInvalid, bp = kInvalid;
+  ++tr.shadow_rays;
+  return !traverse_mode<kTraceAny, kCount>(F.S, o, d, dist, ti, tp, bt, bi, bp, tr.stk, kBlock, tr.cnt, tr.overflow);
+}
+
+// load_intersection_data_vertex_position, primitives.glsl:567-611
+CT_D Isect load_vertex(const DFrame &F, const ObjHit &h, F3 world_pos) {
+  F3 rd = sub(h.hit, world_pos);
+  float t;
+  F3 pos, nor;
+  if (!hit_from_ray(F, h.inst, h.prim, world_pos, rd, t, pos, nor)) return isect_empty();
+  pos = compute_ray_origin(pos, nor);
+  pos = compute_ray_origin(pos, nor);
+  Isect i;
+  i.is_hit = true;
+  i.distance = length(sub(world_pos, pos));
+  i.P = pos;
+  i.n = nor;
+  i.wo = normalize(sub(world_pos, pos));
+  i.wi = f3(0, 0, 0);
+  i.inst = h.inst;
+  i.prim = h.prim;
+  i.mat_idx = material_of(F, h.inst, h.prim);
+  i.mat = ld_mat(F, i.mat_idx);
+  return i;
+}
+// compute_new_ray_origin, primitives.glsl:259-264
+CT_D F3 new_ray_origin(F3 pos, F3 n, bool view_side) {
+  F3 nn = view_side ? n : neg(n);
+  return compute_ray_origin(compute_ray_origin(pos, nn), nn);
+}
+
+// ---------------------------------------------------------------- sampling
+CT_D bool normal_near_zero(F3 v) {  // prng.glsl:17-21
+  const float s = 1e-8f;
+  return fabsf(v.x) < s && fabsf(v.y) < s && fabsf(v.z) < s;
+}
+// random_cosine_direction, prng.glsl:84-108
+CT_D F3 random_cosine_direction(uint32_t &seed, F3 n) {
+  const float r1 = rnd(seed);
+  const float r2 = rnd(seed);
+  const float z = sqrtf(1.0f - r2);
+  const float phi = 2.0f * kPi * r1;
+  float sp, cp;
+  sincos_ct(phi, sp, cp);
+  const float x = cp * sqrtf(r2);
+  const float y = sp * sqrtf(r2);
+  F3 u = fabsf(n.x) > 0.1f ? f3(0.0f, 1.0f, 0.0f) : f3(1.0f, 0.0f, 0.0f);
+  u = normalize(cross(n, u));
+  F3 v = cross(n, u);
+  return add(add(muls(u, x), muls(v, y)), muls(n, z));
+}
+
+// path_generate_scatter_ray, path_tracer.glsl:59-112, with the Lambertian
+// scatter callable (rcall_scatter.glsl:73-84).
+CT_D bool pt_scatter(PState &p, Isect &i) {
+  F3 sd = random_cosine_direction(p.seed, i.n);
+  if (normal_near_zero(sd)) sd = i.n;
+  i.wi = sd;
+  F3 weight = evaluate_material(i.mat, i.n, i.wo, i.wi);
+  const float pdf = material_pdf(i.n, i.wo, i.wi);
+  weight = divs(weight, pdf);
+  p.dir = i.wi;
+  if (p.path_length == 0u) p.flags |= kPathDiffusePrimary;
+  p.thp = mulv(p.thp, weight);
+  p.pdf = pdf;
+  if (p.path_length == p.pb.rc_len) p.pb.cj.y = pdf;
+  p.prev_scatter_pdf = pdf;
+  if (pdf <= 0.0f) return false;
+  p.bounce_counters += 1u;
+  p.normal = i.n;
+  const bool valid = p.thp.x > 0.0f || p.thp.y > 0.0f || p.thp.z > 0.0f;
+  if (valid && p.path_length <= p.pb.rc_len) {
+    p.prefix_thp = p.thp;  // path_record_prefix_thp
+    p.thp = f3(1.0f, 1.0f, 1.0f);
+  }
+  return valid;
+}
+
+// path_builder_add_escape_vertex, path_builder.glsl:36-75
+CT_D void add_escape_vertex(PBuilder &pb, uint32_t plen, F3 wi, F3 path_weight, F3 postfix, float rr_pdf, float mis,
+                            float light_pdf, uint32_t light_type, PRes &res) {
+  if (plen < 1u) return;
+  const bool is_rc = plen == pb.rc_len;
+  if (!pres_update(res, path_weight, rr_pdf, pb.seed)) return;
+  res.flags = pb.flags;
+  pf_set_path_length(res.flags, plen);
+  pf_set_rc_length(res.flags, pb.rc_len);
+  pf_set_last_nee(res.flags, false);
+  res.rc_irr = postfix;
+  res.F = path_weight;
+  res.rc_wi = pb.rc_wi;
+  res.rc = pb.rc;
+  res.cj = pb.cj;
+  res.rc_seed = pb.cached_seed;
+  if (is_rc) {
+    res.rc_wi = wi;
+    res.rc_irr = divs(res.rc_irr, mis);
+  }
+  res.light_pdf = light_pdf;
+  pf_set_light_type(res.flags, light_type);
+}
+
+// eval_voxel_pdf, path_tracer.glsl:185-211
+CT_D float eval_voxel_pdf(F3 pos, const Isect &i) {
+  const F3 to_light = sub(i.P, pos);
+  const float d2 = dot(to_light, to_light);
+  if (d2 <= kHlfMin) return 0.0f;
+  const F3 L = divs(to_light, sqrtf(d2));
+  const float cos_theta = dot(i.n, neg(L));
+  if (cos_theta <= 0.0f) return 0.0f;
+  const float area = kVoxelExtent * kVoxelExtent;
+  const float denom = gmax(kHlfMin, cos_theta * area);
+  return d2 / denom;
+}
+
+// path_handle_emissive_hit, path_tracer.glsl:258-337 (not a primary hit,
+// no replay, reconnection shift).
+CT_D void pt_emissive(const DFrame &F, PState &p, const Isect &i) {
+  const float light_pdf = (1.0f / (float)F.n_cube) * eval_voxel_pdf(p.origin, i);
+  const float mis = eval_mis(p.pdf, light_pdf);
+  const F3 cur = mulv(p.thp, p.prefix_thp);
+  const F3 Lr = muls(mulv(cur, i.mat.emission), mis);
+  const F3 att = mulv(cur, i.mat.emission);
+  if (p.path_length > 1u) p.L = add(p.L, Lr);
+  const F3 postfix = muls(mulv(p.thp, att), mis);
+  const uint32_t plen = p.path_length >= 1u ? p.path_length - 1u : 0u;
+  add_escape_vertex(p.pb, plen, p.dir, Lr, postfix, p.rr_pdf, mis, light_pdf, kGeomLightCube, p.res);
+}
+
+// path_generate_light_sample (path_tracer.glsl:222-256) as far as it is
+// observable.  The sample's MIS weight is eval_mis(1, ls.pdf, ...) with
+// ls.pdf never assigned (0), so every NEE contribution is 0 or NaN and
+// path_reservoir_update rejects it without drawing; only the random numbers
+// the light sample consumes reach the outputs.  Its shadow ray is not
+// traced.
+CT_D void pt_light_sample(const DFrame &F, const Isect &i, uint32_t &seed) {
+  const uint32_t n = F.n_cube;
+  const uint32_t li = min(urnd_interval(seed, 0u, n), n - 1u);
+  if (n == 0u) return;
+  const PLight l = ld_light(F.cube, li);
+  Hit h;
+  h.P = i.P;
+  h.n = i.n;
+  h.distance = i.distance;
+  h.scatter = neg(i.wo);
+  h.inst = i.inst;
+  h.prim = i.prim;
+  h.mat = i.mat_idx;
+  float pdf = 1.0f / (float)n;
+  LS ls;
+  sample_lights(F, h, l, pdf, seed, true, false, ls);
+}
+
+// path_handle_hit, path_tracer.glsl:599-727 (no replay, no hybrid shift:
+// generate_path fixes use_hybrid_shift = false, path_state.glsl:245-247).
+CT_D void pt_handle_hit(const DFrame &F, PState &p, Isect &i) {
+  const ct_restir_params &rp = F.rp;
+  const bool compute_emissive =
+      !(p.path_length == 1u && (p.flags & kPathTransmission) == 0u && (p.flags & kPathDelta) == 0u);
+  if (compute_emissive && (i.mat.emission.x > 0.0f || i.mat.emission.y > 0.0f || i.mat.emission.z > 0.0f))
+    pt_emissive(F, p, i);
+  // path_has_finished_surface_bounces, path_state.glsl:151-159
+  if ((p.bounce_counters & 0xFFu) > p.max_depth) {
+    p.flags &= ~kPathActive;
+    return;
+  }
+  const F3 prev_origin = p.origin;
+  p.origin = compute_ray_origin(compute_ray_origin(i.P, i.n), i.n);
+  const bool cur_rough = i.mat.roughness > rp.roughness_threshold;  // classify_as_rough
+  // path_check_rc_vertex, path_tracer.glsl:484-597 (not a replay): the
+  // reconnection vertex is the first secondary hit.
+  if (p.path_length == 1u) {
+    p.pb.rc = p.hit;
+    p.pb.cj.x = p.prev_scatter_pdf;
+    const F3 disp = sub(prev_origin, i.P);
+    p.pb.cj.z = fabsf(dot(i.n, i.wo)) / dot(disp, disp);
+    if (isnan(p.pb.cj.z)) p.pb.cj.z = 0.0f;
+    p.pb.rc_len = p.path_length;
+  }
+  p.flags &= ~(kPathLightUpper | kPathLightLower);
+  // path_handle_sample_light, path_tracer.glsl:339-438
+  if (p.path_length == p.pb.rc_len) p.pb.cached_seed = p.seed;
+  pt_light_sample(F, i, p.seed);
+  p.flags = (p.flags | kPathLightUpper) & ~kPathLightLower;
+  // Russian roulette, terminate_path_by_russian_roulette (path_tracer.glsl:165-177)
+  if (rp.pt_russian_roulette && p.path_length >= 1u) {
+    const float rr = luminance(mulv(p.thp, p.prefix_thp));
+    const float prob = gmax(0.0f, 1.0f - rr);
+    if (rnd(p.seed) < prob) {
+      p.flags &= ~kPathActive;
+      return;
+    }
+    p.rr_pdf *= 1.0f - prob;
+  }
+  bool valid = pt_scatter(p, i);
+  if (p.path_length == p.pb.rc_len) p.pb.cached_seed = p.seed;
+  p.pb.rc_wi = p.dir;
+  p.last_rough = cur_rough;
+  if ((p.flags & kPathActive) == 0u) return;
+  if ((p.bounce_counters & 0xFFu) > p.max_depth) valid = false;
+  if (!valid) p.flags &= ~kPathActive;
+}
+
+// path_handle_miss, path_tracer.glsl:729-815 (environment light on)
+CT_D void pt_handle_miss(const DFrame &F, PState &p) {
+  const float light_pdf = kInv4Pi;
+  const float mis = eval_mis(p.pdf, light_pdf);
+  const F3 Le = sky_color(F.time, F.afternoon != 0, p.dir);
+  const F3 Lr = muls(mulv(mulv(p.thp, p.prefix_thp), Le), mis);
+  if (p.path_length > 0u) p.L = add(p.L, Lr);
+  add_escape_vertex(p.pb, p.path_length, p.dir, Lr, muls(mulv(p.thp, Le), mis), p.rr_pdf, mis, light_pdf,
+                    kGeomLightEnv, p.res);
+  p.flags &= ~kPathActive;
+}
+
+// trace_restir_path_tracing, path_tracer.glsl:854-887, with generate_path
+// (path_state.glsl:208-249) and path_handle_primary_hit (:440-469).
+// Returns the finalized reservoir; `color` is path_output's colour.
+template <bool kCount>
+CT_D PRes pt_initial(const DFrame &F, Isect i, uint32_t &seed, F3 &color, PtTrace &tr) {
+  PState p;
+  p.max_depth = F.rp.max_depth;
+  p.flags = kPathActive | kPathHit;
+  p.path_length = 0u;
+  p.bounce_counters = 0u;
+  p.scene_length = 0.0f;
+  p.origin = f3(0, 0, 0);
+  p.dir = f3(0, 0, 0);
+  p.pdf = 0.0f;
+  p.normal = f3(0, 0, 0);
+  p.hit.inst = i.inst;
+  p.hit.prim = i.prim;
+  p.hit.hit = i.P;
+  p.thp = f3(1.0f, 1.0f, 1.0f);
+  p.prefix_thp = f3(1.0f, 1.0f, 1.0f);
+  p.L = f3(0, 0, 0);
+  p.rr_pdf = 1.0f;
+  p.prev_scatter_pdf = 1.0f;
+  p.seed = seed;
+  p.pb.seed = seed;
+  p.pb.rc = objhit_invalid();
+  p.pb.rc_wi = f3(0, 0, 0);
+  p.pb.cached_seed = 0u;
+  p.pb.rc_len = F.rp.max_depth;
+  p.pb.flags = 0u;
+  p.pb.cj = f3(0, 0, 0);
+  p.res = pres_init(F.rp.max_depth);
+  p.last_rough = true;
+  // path_handle_primary_hit
+  p.origin = compute_ray_origin(compute_ray_origin(i.P, i.n), i.n);
+  p.flags &= ~(kPathLightUpper | kPathLightLower);
+  p.res.init_seed = p.seed;
+  bool valid = pt_scatter(p, i);
+  if (p.path_length == p.pb.rc_len) p.pb.cached_seed = p.seed;
+  p.pb.rc_wi = p.dir;
+  if (valid) {
+    do {
+      // path_next_vertex, path_tracer.glsl:136-149
+      i = bounce_intersect<kCount>(F, p.origin, p.dir, tr);
+      if (i.is_hit) {
+        p.hit.inst = i.inst;
+        p.hit.prim = i.prim;
+        p.hit.hit = i.P;
+        p.flags |= kPathHit;
+        p.path_length++;
+        p.scene_length += i.distance;
+        pt_handle_hit(F, p, i);
+      } else {
+        p.flags &= ~kPathHit;
+        p.scene_length = kHlfMax;
+        pt_handle_miss(F, p);
+      }
+    } while ((p.flags & kPathActive) != 0u && !tr.overflow);
+  }
+  p.res.M = 1.0f;  // path_builder_finilize
+  pres_finalize_ris(p.res);
+  color = muls(p.res.F, p.res.weight);
+  if (has_nan_inf(color)) color = f3(0, 0, 0);
+  seed = p.seed;
+  return p.res;
+}
+
+// compute_shifted_integrand_reconnection, shift.glsl:25-350, for the
+// reconnection shift (rc vertex 1, no cached Jacobian).  `dst` and `src`
+// are the primary intersections; `s` the source reservoir.
+template <bool kCount>
+CT_D F3 shift_reconnection(const DFrame &F, float &dst_jac, Isect dst, Isect src, const PRes &s, bool eval_vis,
+                           PtTrace &tr) {
+  dst_jac = 0.0f;
+  const uint32_t rc_len = 1u;
+  const F3 irr = s.rc_irr, wi = s.rc_wi;
+  const bool exists = instance_hit_exists(s.rc.inst, s.rc.prim);
+  const uint32_t plen = pf_path_length(s.flags);
+  const bool nee = pf_last_nee(s.flags);
+  const bool trans = pf_transmission(s.flags, true);
+  src.P = new_ray_origin(src.P, src.n, !trans);
+  dst.P = new_ray_origin(dst.P, dst.n, !trans);
+  if (!exists) {
+    F3 integrand = f3(0, 0, 0);
+    if (pf_light_type(s.flags) == kGeomLightEnv && plen + 1u == rc_len && !nee) {
+      if (vertex_visible<kCount>(F, dst.P, wi, kMaxDistance, kMaxInstances, kMaxPrimitives, tr)) {
+        const float src_pdf1 = material_pdf(src.n, src.wo, wi);
+        const float dst_pdf1 = material_pdf(dst.n, dst.wo, wi);
+        const F3 dst_f1 = evaluate_material(dst.mat, dst.n, dst.wo, wi);
+        const float mis = eval_mis(dst_pdf1, s.light_pdf);
+        integrand = mulv(muls(divs(dst_f1, dst_pdf1), mis), irr);
+        dst_jac = dst_pdf1 / src_pdf1;
+      }
+    }
+    if (jacobian_invalid(dst_jac)) dst_jac = 0.0f;
+    if (has_nan_inf(integrand)) return f3(0, 0, 0);
+    return integrand;
+  }
+  const bool is_final = plen == rc_len;
+  const bool escaped = plen + 1u == rc_len && !nee;
+  const bool rc_nee = is_final && nee;
+  if (pf_delta(s.flags, true) || pf_delta(s.flags, false)) return f3(0, 0, 0);
+  const Isect rc = load_vertex(F, s.rc, dst.P);
+  const F3 dst_cv = neg(rc.wo);
+  const F3 src_cv = normalize(sub(rc.P, src.P));
+  const F3 sdisp = sub(rc.P, dst.P);
+  const float sd2 = dot(sdisp, sdisp);
+  const float scos = fabsf(dot(rc.n, neg(dst_cv)));
+  const float cjz = scos / sd2;
+  const F3 odisp = sub(rc.P, src.P);
+  const float od2 = dot(odisp, odisp);
+  const float ocos = fabsf(dot(rc.n, neg(src_cv)));
+  float jac = cjz * od2 / ocos;
+  if (jacobian_invalid(jac)) return f3(0, 0, 0);
+  const float dst_pdf1 = material_pdf(dst.n, dst.wo, dst_cv);
+  const float src_pdf1 = material_pdf(src.n, src.wo, src_cv);
+  jac *= dst_pdf1 / src_pdf1;
+  if (jacobian_invalid(jac)) return f3(0, 0, 0);
+  const F3 dst_f1 = evaluate_material(dst.mat, dst.n, dst.wo, dst_cv);
+  float dst_rc_all = 0.0f, dst_pdf2 = 1.0f, dst_rc = 1.0f, src_rc = 1.0f;
+  if (!escaped) {
+    dst_rc = material_pdf(rc.n, rc.wo, wi);
+    dst_rc_all = dst_rc;
+    src_rc = material_pdf(rc.n, neg(src_cv), wi);
+    dst_pdf2 = !rc_nee ? dst_rc : s.light_pdf;
+  }
+  F3 dst_f2 = f3(1.0f, 1.0f, 1.0f);
+  if (!escaped) dst_f2 = evaluate_material(rc.mat, rc.n, rc.wo, wi);
+  if ((dst_f1.x == 0.0f && dst_f1.y == 0.0f && dst_f1.z == 0.0f) ||
+      (dst_f2.x == 0.0f && dst_f2.y == 0.0f && dst_f2.z == 0.0f))
+    return f3(0, 0, 0);
+  F3 no_f1 = mulv(divs(dst_f2, dst_pdf2), irr);
+  F3 integrand = mulv(divs(dst_f1, dst_pdf1), no_f1);
+  if (escaped) integrand = muls(integrand, eval_mis(dst_pdf1, s.light_pdf));
+  if (is_final && pf_light_type(s.flags) != kGeomLightAnalytic) {
+    const float lp = s.light_pdf;
+    const float mis = eval_mis(rc_nee ? lp : dst_rc_all, rc_nee ? dst_rc_all : lp);
+    integrand = muls(integrand, mis);
+    no_f1 = muls(no_f1, mis);
+    if (!rc_nee) jac *= dst_rc / src_rc;
+  }
+  if (!is_final && !escaped) jac *= dst_rc / src_rc;
+  if (jacobian_invalid(jac)) return f3(0, 0, 0);
+  if (eval_vis) {
+    const F3 dir = normalize(sub(rc.P, dst.P));
+    const float dist = length(sub(dst.P, rc.P));
+    if (!vertex_visible<kCount>(F, dst.P, dir, dist, rc.inst, rc.prim, tr)) return f3(0, 0, 0);
+  }
+  if (has_nan_inf(integrand)) return f3(0, 0, 0);
+  dst_jac = jac;
+  return integrand;
+}
+
+// compute_shifted_integrand_ (shift.glsl:446-483), reconnection or hybrid.
+// The hybrid shift reads RECONNECTION_DATA, whose storage is stubbed in
+// the reference (path_reservoir.glsl:37-58 returns an invalid record);
+// with it the hybrid integrand keeps dst_jacobian = 0 (shift.glsl:363 and
+// the skipped reconnection branch), so every hybrid shift is rejected.
+template <bool kCount>
+CT_D F3 shifted_integrand(const DFrame &F, float &dst_jac, const Isect &dst, const Isect &src, const PRes &s,
+                          bool eval_vis, PtTrace &tr) {
+  dst_jac = 0.0f;
+  if (s.weight == 0.0f) return f3(0, 0, 0);
+  if (F.rp.pt_shift_mapping == CT_SHIFT_RECONNECTION) return shift_reconnection<kCount>(F, dst_jac, dst, src, s, eval_vis, tr);
+  if (F.rp.pt_shift_mapping == CT_SHIFT_HYBRID) {
+    // compute_shifted_integrand_hybrid, shift.glsl:352-444, with the stubbed
+    // reconnection data: rc length 1 gives tp = 1 and an invalid previous
+    // vertex, longer ones tp = 0; the reconnection branch is never entered.
+    // (ROUGHNESS_BASED_REJECTION is false, defines.glsl:54.)
+    return pf_rc_length(s.flags) == 1u ? f3(1.0f, 1.0f, 1.0f) : f3(0, 0, 0);
+  }
+  return f3(1.0f, 1.0f, 1.0f);
+}
+
+// shift_and_merge_reservoir, shift.glsl:485-510 (force_merge, mis 1)
+template <bool kCount>
+CT_D bool shift_and_merge(const DFrame &F, float &dst_jac, const Isect &dst, PRes &dst_res, const Isect &src,
+                          const PRes &src_res, uint32_t &seed, PtTrace &tr) {
+  const F3 integrand = shifted_integrand<kCount>(F, dst_jac, dst, src, src_res, true, tr);
+  const bool selected = pres_merge(dst_res, integrand, dst_jac, src_res, seed, 1.0f, true);
+  if (!selected) dst_res.F = f3(0, 0, 0);
+  dst_res.M = src_res.M;
+  dst_res.weight = src_res.weight;
+  return selected;
+}
+
+// temporal_path_reuse, indirect_illumination.glsl:147-279 (Talbot RMIS)
+template <bool kCount>
+CT_D PRes pt_temporal(const DFrame &F, const PRes &central, PRes temporal, const Isect &cur_i, const Isect &prev_i,
+                      uint32_t &seed, PtTrace &tr) {
+  PRes dest = pres_init(F.rp.max_depth);
+  const float current_M = central.M;
+  temporal.M = gmin(F.rp.pt_m_cap * current_M, temporal.M);
+  float dst_jac = 0.0f;
+  for (int k = -1; k <= 0; ++k) {
+    float p_sum = 0.0f, p_self = 0.0f;
+    PRes tdst = dest;
+    bool possible;
+    if (k == -1) {
+      tdst = central;
+      dst_jac = 1.0f;
+      possible = tdst.weight > 0.0f;
+    } else {
+      possible = shift_and_merge<kCount>(F, dst_jac, cur_i, tdst, prev_i, temporal, seed, tr);
+    }
+    if (possible) {
+      // j = -1
+      const float cur_p = lum601(tdst.F) * current_M;
+      p_sum += cur_p;
+      if (k == -1) p_self = cur_p;
+      // j = 0
+      if (k == 0) {
+        p_self = lum601(temporal.F) / dst_jac * temporal.M;
+        p_sum += p_self;
+      } else {
+        float tj = 0.0f;
+        const F3 ti = shifted_integrand<kCount>(F, tj, prev_i, cur_i, tdst, true, tr);
+        const float pp = lum601(ti) * tj;
+        p_sum += pp * temporal.M;
+      }
+    }
+    const float mis = p_sum == 0.0f ? 0.0f : p_self / p_sum;
+    pres_merge_mis(dest, tdst.F, dst_jac, tdst, seed, mis);
+  }
+  if (dest.weight > 0.0f) pres_finalize_gris(dest);
+  if (dest.weight < 0.0f || isinf(dest.weight) || isnan(dest.weight)) dest.weight = 0.0f;
+  return dest;
+}
+
+// get_next_neighbor_pixel, prng.glsl:272-296
+CT_D void pt_neighbor(int32_t px, int32_t py, int32_t radius, uint32_t &seed, int32_t &nx, int32_t &ny) {
+  const float ox = 2.0f * rnd(seed) - 1.0f;
+  const float oy = 2.0f * rnd(seed) - 1.0f;
+  nx = px + f2i_sat(ox * (float)radius);
+  ny = py + f2i_sat(oy * (float)radius);
+  if (nx == px && ny == py) {
+    nx = -1;
+    ny = -1;
+  }
+}
+
+// The primary intersection of a pixel from a DI record
+// (temporal_path_get_reprojected_primary_hit / path_get_neighbor_primary_hit,
+// indirect_illumination.glsl:59-107): wo is not normalized there.
+CT_D Isect isect_from_di_raw(const DFrame &F, const DI &d) {
+  Isect i;
+  i.is_hit = d.distance > 0.0f;
+  i.distance = d.distance;
+  i.P = d.pos;
+  i.n = d.nrm;
+  i.wo = sub(d.org, d.pos);
+  i.wi = f3(0, 0, 0);
+  i.inst = d.inst;
+  i.prim = d.prim;
+  i.mat_idx = d.mat;
+  i.mat = ld_mat(F, d.mat);
+  return i;
+}
+// The primary intersection rgen.glsl builds (rgen.glsl:275-278, 431-434).
+CT_D Isect isect_from_di(const DFrame &F, const DI &d) {
+  Isect i = isect_from_di_raw(F, d);
+  i.wo = normalize(sub(d.org, d.pos));
+  return i;
+}
+
+// indirect_illumination_spatial_reuse, indirect_illumination.glsl:312-452
+template <bool kCount>
+CT_D PRes pt_spatial(const DFrame &F, uint32_t x, uint32_t y, const Isect &ci, F3 camera_pos, uint32_t &seed,
+                     PtTrace &tr) {
+  const uint32_t cur = y * F.W + x;
+  const PRes central = ld_pres(F.pres_o, cur);
+  PRes dest = pres_init(F.rp.max_depth);
+  const int32_t ncount = (int32_t)F.rp.pt_neighbor_count;
+  int32_t valid = 0;
+  float canonical = 1.0f;
+  for (int32_t n = 0; n < ncount; ++n) {
+    int32_t nx, ny;
+    pt_neighbor((int32_t)x, (int32_t)y, F.rp.pt_neighbor_radius, seed, nx, ny);
+    if (!(nx >= 0 && ny >= 0 && nx < (int32_t)F.W && ny < (int32_t)F.H)) continue;  // is_valid_screen_region
+    const uint32_t ni = (uint32_t)ny * F.W + (uint32_t)nx;
+    const DI nd = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, ni);
+    const Isect nis = isect_from_di_raw(F, nd);
+    if (!instance_hit_exists(nis.inst, nis.prim)) continue;
+    {  // is_valid_geometry, primitives.glsl:547-556
+      const float cd = length(sub(camera_pos, ci.P));
+      const float nd2 = length(sub(camera_pos, nis.P));
+      if (!(dot(ci.n, nis.n) >= 0.5f && fabsf(cd - nd2) < 0.1f * cd)) continue;
+    }
+    const PRes nres = ld_pres(F.pres_o, ni);
+    valid++;
+    float prefix_jac = 0.0f;
+    const F3 prefix = shifted_integrand<kCount>(F, prefix_jac, nis, ci, central, true, tr);
+    const float prefix_pdf = lum601(prefix) * prefix_jac;
+    canonical += 1.0f;
+    if (prefix_pdf > 0.0f)
+      canonical -= prefix_pdf * nres.M /
+                   (prefix_pdf * nres.M + central.M * lum601(central.F) / (float)ncount);
+    PRes tdst = dest;
+    float dst_jac = 0.0f;
+    const bool possible = shift_and_merge<kCount>(F, dst_jac, ci, tdst, nis, nres, seed, tr);
+    float nw = 0.0f;
+    if (possible) {
+      nw = lum601(nres.F) / dst_jac * nres.M /
+           ((lum601(nres.F) / dst_jac) * nres.M + lum601(tdst.F) * central.M / (float)ncount);
+      if (isnan(nw) || isinf(nw)) nw = 0.0f;
+    }
+    pres_merge_mis(dest, tdst.F, dst_jac, tdst, seed, nw);
+  }
+  pres_merge_mis(dest, central.F, 1.0f, central, seed, canonical);
+  if (dest.weight > 0.0f) {
+    pres_finalize_gris(dest);
+    dest.weight /= (float)(valid + 1);
+  }
+  if (dest.weight < 0.0f) dest.weight = 0.0f;
+  if (isnan(dest.weight) || isinf(dest.weight)) dest.weight = 0.0f;
+  return dest;
+}

@@ -52,6 +52,11 @@ CT_TASK_UNDO_OP_CPU = 5
 
 CT_BUF_IMAGE, CT_BUF_PREV_IMAGE, CT_BUF_TAA_FRAME, CT_BUF_DI = 0, 1, 2, 3
 CT_BUF_PREV_DI, CT_BUF_INT_RESERVOIR, CT_BUF_PREV_RESERVOIR, CT_BUF_VELOCITY = 4, 5, 6, 7
+CT_BUF_PATH_RESERVOIR, CT_BUF_TEMPORAL_PATH_RESERVOIR, CT_BUF_INDIRECT = 8, 9, 10
+
+CT_SHIFT_RECONNECTION, CT_SHIFT_HYBRID = 0, 2
+CT_MAX_PT_NEIGHBORS = 8
+CT_MAX_PT_DEPTH = 14
 
 f32, u32, i32 = C.c_float, C.c_uint32, C.c_int32
 
@@ -176,7 +181,21 @@ class RestirParams(C.Structure):
                 ("min_ris_cube_samples", u32), ("max_ris_cube_samples", u32),
                 ("max_influence_from_the_past", f32), ("min_neighbors", u32), ("max_neighbors", u32),
                 ("min_neighbor_radius", f32), ("max_neighbor_radius", f32), ("max_distance_to_hit", f32),
-                ("taa_alpha", f32), ("taa_truncate_dispatch", u32), ("direct_emittance", u32), ("pad", u32)]
+                ("taa_alpha", f32), ("taa_truncate_dispatch", u32), ("direct_emittance", u32),
+                ("indirect", u32), ("max_depth", u32), ("pt_temporal", u32), ("pt_spatial", u32),
+                ("pt_shift_mapping", u32), ("pt_neighbor_count", u32), ("pt_neighbor_radius", i32),
+                ("pt_m_cap", f32), ("near_field_distance", f32), ("roughness_threshold", f32),
+                ("pt_russian_roulette", u32), ("pad", u32)]
+
+
+class ObjectHit(C.Structure):
+    _fields_ = [("object", ObjectInfo), ("hit", f32 * 3)]
+
+
+class PathReservoir(C.Structure):
+    _fields_ = [("M", f32), ("weight", f32), ("path_flags", u32), ("rc_random_seed", u32), ("F", f32 * 3),
+                ("light_pdf", f32), ("cached_jacobian", f32 * 3), ("init_random_seed", u32),
+                ("rc_vertex_hit", ObjectHit), ("rc_vertex_wi", f32 * 3), ("rc_vertex_irradiance", f32 * 3)]
 
 
 class FrameParams(C.Structure):
@@ -191,7 +210,9 @@ class FrameStats(C.Structure):
                 ("nodes", C.c_uint64), ("leaves", C.c_uint64), ("instances", C.c_uint64),
                 ("closest_nodes", C.c_uint64), ("closest_leaves", C.c_uint64), ("closest_instances", C.c_uint64),
                 ("ms_total", f32), ("ms_primary", f32), ("ms_ris", f32), ("ms_temporal", f32),
-                ("ms_spatial", f32), ("ms_taa", f32), ("ms_trace_any", f32), ("ms_trace_closest", f32)]
+                ("ms_spatial", f32), ("ms_taa", f32), ("ms_trace_any", f32), ("ms_trace_closest", f32),
+                ("bounce_rays", C.c_uint64), ("shadow_rays_pt", C.c_uint64), ("pt_nodes", C.c_uint64),
+                ("pt_leaves", C.c_uint64), ("pt_instances", C.c_uint64), ("ms_pt", f32), ("ms_pt_spatial", f32)]
 
 
 class SceneDesc(C.Structure):
@@ -209,7 +230,8 @@ class SceneInfo(C.Structure):
 
 STRUCT_SIZES = {"ct_aabb": (Aabb, 24), "ct_instance": (Instance, 72), "ct_primitive": (Primitive, 8),
                 "ct_material": (Material, 84), "ct_light": (Light, 40), "ct_light_config": (LightConfig, 52),
-                "ct_reservoir": (Reservoir, 36), "ct_di_info": (DiInfo, 60)}
+                "ct_reservoir": (Reservoir, 36), "ct_di_info": (DiInfo, 60), "ct_path_reservoir": (PathReservoir, 92),
+                "ct_restir_params": (RestirParams, 108)}
 
 
 # ----------------------------------------------------------------- loader
@@ -241,6 +263,8 @@ _SIGS = {
                           C.POINTER(TraceStats)], C.c_int),
     "ct_trace_any": ([C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                       C.POINTER(TraceStats)], C.c_int),
+    "ct_trace_bounce": ([C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                         C.c_void_p, C.c_void_p, C.POINTER(TraceStats)], C.c_int),
     "ct_ctx_create": ([C.POINTER(CtxDesc), C.POINTER(C.c_void_p)], C.c_int),
     "ct_ctx_destroy": ([C.c_void_p], C.c_int),
     "ct_ctx_set_materials": ([C.c_void_p, C.POINTER(Material), u32], C.c_int),
@@ -436,6 +460,25 @@ class Accel:
                                  C.byref(st) if stats else None))
         return (occ, st) if stats else occ
 
+    def trace_bounce(self, origins, directions, stats=False):
+        """intersect() of bounce.glsl: ids, position, normal, distance, wo per ray."""
+        o = np.ascontiguousarray(origins, np.float32)
+        d = np.ascontiguousarray(directions, np.float32)
+        n = o.shape[0]
+        ids = np.zeros((n, 2), np.uint32)
+        pos = np.zeros((n, 3), np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        dist = np.zeros(n, np.float32)
+        wo = np.zeros((n, 3), np.float32)
+        st = TraceStats()
+        check(lib().ct_trace_bounce(self.handle, n, o.ctypes.data, d.ctypes.data, ids.ctypes.data, pos.ctypes.data,
+                                    nrm.ctypes.data, dist.ctypes.data, wo.ctypes.data,
+                                    C.byref(st) if stats else None))
+        out = {"ids": ids, "pos": pos, "nrm": nrm, "dist": dist, "wo": wo}
+        if stats:
+            out["stats"] = st
+        return out
+
     def close(self):
         if self.handle:
             lib().ct_accel_destroy(self.handle)
@@ -484,6 +527,10 @@ class Ctx:
             out = np.zeros((n, 2), np.float32)
         elif which in (CT_BUF_DI, CT_BUF_PREV_DI):
             out = (DiInfo * n)()
+        elif which in (CT_BUF_PATH_RESERVOIR, CT_BUF_TEMPORAL_PATH_RESERVOIR):
+            out = (PathReservoir * n)()
+        elif which == CT_BUF_INDIRECT:
+            out = np.zeros((n, 3), np.float32)
         else:
             out = (Reservoir * n)()
         nbytes = out.nbytes if isinstance(out, np.ndarray) else C.sizeof(out)

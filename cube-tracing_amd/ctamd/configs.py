@@ -1,8 +1,8 @@
 """Workload definitions: scenes, cameras and per-frame parameters.
 
 ``small_plain`` is the 64^3 / 256x256 workload the reference's own CPU path
-can run; ``hd_di`` is the 1920x1080 ReSTIR DI workload on a 256^3 grid in 64
-chunks.  ``reference_mode=True`` keeps the reference's constants
+can run; ``hd_di`` is the 1920x1080 ReSTIR DI + 3-bounce path workload on a
+256^3 grid in 64 chunks.  ``reference_mode=True`` keeps the reference's constants
 (defines.glsl:34-62, taa.glsl:13, and camera.h:12 where fov = 45 is passed
 to glm::perspectiveRH_ZO as radians); the default uses the workload's own
 values (60 degree FOV, 32 RIS candidates, 5 neighbours at 30 px).
@@ -76,10 +76,17 @@ def _orbit_camera(radius, height, deg_per_frame, start_deg=0.0):
     return cam
 
 
-def small_plain(reference_mode=False, width=256, height=256, frames=4, taa=False):
+def _indirect(rp, indirect, max_depth):
+    rp.indirect = 1 if indirect else 0
+    rp.max_depth = max_depth
+
+
+def small_plain(reference_mode=False, width=256, height=256, frames=4, taa=False, indirect=None, max_depth=1):
     """64^3 voxels, 12 % fill (31,457 voxels), 8 emissive voxels of radiance 10.
 
     Camera at (0, 0, 2.5) looking down -z (camera.h:8-10), 1 spp, frames 0..3.
+    The workload's own mode adds one diffuse bounce (ReSTIR PT with
+    max_depth 1); reference mode keeps INDIRECT_ILLUMINATION_ON = 0.
     """
     from . import CT_RIS_CUBE_LIGHT_BIT, restir_params_reference
     rp = restir_params_reference()
@@ -87,6 +94,7 @@ def small_plain(reference_mode=False, width=256, height=256, frames=4, taa=False
         rp.min_ris_cube_samples = rp.max_ris_cube_samples = 32
         rp.min_neighbors = rp.max_neighbors = 5
         rp.min_neighbor_radius = rp.max_neighbor_radius = 30.0
+    _indirect(rp, (not reference_mode) if indirect is None else indirect, max_depth)
     return Workload(
         name="small_plain" + ("_reference" if reference_mode else ""),
         grid=(64, 64, 64), chunk=(64, 64, 64), occupied=round(0.12 * 64 ** 3), emissive=8,
@@ -96,13 +104,15 @@ def small_plain(reference_mode=False, width=256, height=256, frames=4, taa=False
         features=CT_RIS_CUBE_LIGHT_BIT, taa=taa, restir=rp)
 
 
-def hd_di(frames=16, width=1920, height=1080, grid=256, chunk=64):
+def hd_di(frames=16, width=1920, height=1080, grid=256, chunk=64, indirect=True, max_depth=3):
     """256^3 voxels in 64 chunks of 64^3, 25 % fill (4,194,304 voxels), 2 %
-    emissive (83,886 lights, radiance U[1, 20]); ReSTIR DI at 1920x1080 with a
+    emissive (83,886 lights, radiance U[1, 20]); ReSTIR DI plus a 3-bounce
+    diffuse path (ReSTIR PT, max_depth 3 = MAX_DEPTH) at 1920x1080 with a
     1 degree/frame camera orbit and TAA at 0.1 blend."""
     from . import CT_RIS_CUBE_LIGHT_BIT, restir_params_reference
     rp = restir_params_reference()
     rp.taa_alpha = 0.1
+    _indirect(rp, indirect, max_depth)
     occ = round(0.25 * grid ** 3)
     extent = grid * 0.03125
     return Workload(

@@ -321,6 +321,17 @@ int ct_trace_any(ct_accel *accel, uint32_t n, const float *origins,
                  const uint32_t *targets, uint8_t *occluded_out,
                  ct_trace_stats *stats);
 
+/* Path-segment closest hit, intersect() of bounce.glsl:9-130: the box
+ * tests start at compute_ray_origin(origin, direction) and the closest
+ * voxel is returned (the shader commits the first one its query
+ * generates).  Outputs per ray: ids [n][2] (CT_INVALID_INDEX on a miss),
+ * world hit position and normal [n][3], distance to the un-pushed origin
+ * [n] and wo [n][3], as the INTERSECT record holds them. */
+int ct_trace_bounce(ct_accel *accel, uint32_t n, const float *origins,
+                    const float *directions, uint32_t *ids_out, float *pos_out,
+                    float *nrm_out, float *dist_out, float *wo_out,
+                    ct_trace_stats *stats);
+
 /* =================================================================
  * Frame context: the buffers and passes App::draw records
  * (src/main.cpp:1436-1703) -- prepass, shading pass, TAA.
@@ -372,8 +383,44 @@ typedef struct ct_restir_params {
   float taa_alpha;                 /* 0.8: weight of the current frame */
   uint32_t taa_truncate_dispatch;  /* 1: TAA covers (W/8*8)x(H/8*8) like main.cpp:1642-1643 */
   uint32_t direct_emittance;       /* 1: DIRECT_EMITTANCE_ON */
+  /* Indirect illumination by ReSTIR PT (INDIRECT_ILLUMINATION_ON with
+   * RESTIR_PT_ON, rgen.glsl:271-315, 427-452; SCENE_PARAMS rgen.glsl:280-287). */
+  uint32_t indirect;               /* 0: INDIRECT_ILLUMINATION_ON (defines.glsl:20) */
+  uint32_t max_depth;              /* 3: Status.max_depth = MAX_DEPTH (shared.inl:61) */
+  uint32_t pt_temporal;            /* 1: RESTIR_PT_TEMPORAL_ON (defines.glsl:15) */
+  uint32_t pt_spatial;             /* 1: RESTIR_PT_SPATIAL_ON */
+  uint32_t pt_shift_mapping;       /* CT_SHIFT_RECONNECTION (rgen.glsl:282) */
+  uint32_t pt_neighbor_count;      /* 3: NEIGHBOR_COUNT, at most CT_MAX_PT_NEIGHBORS */
+  int32_t pt_neighbor_radius;      /* 1: NEIGHBOR_RADIUS (pixels) */
+  float pt_m_cap;                  /* 20: MAX_INFLUENCE_FROM_THE_PAST_THRESHOLD_PT */
+  float near_field_distance;       /* 0.0078125: NEAR_FIELD_DISTANCE */
+  float roughness_threshold;       /* 0.2: SPECULAR_ROUGHNESS_THRESHOLD */
+  uint32_t pt_russian_roulette;    /* 0: USE_RUSSIAN_ROULETTE */
   uint32_t pad;
 } ct_restir_params;
+#define CT_SHIFT_RECONNECTION 0u  /* SHIFT_MAPPING_RECONNECTION, defines.glsl:85 */
+#define CT_SHIFT_HYBRID 2u        /* SHIFT_MAPPING_HYBRID, defines.glsl:87 */
+#define CT_MAX_PT_NEIGHBORS 8u
+#define CT_MAX_PT_DEPTH 14u       /* path lengths are stored in 4 bits */
+
+/* PATH_RESERVOIR (shared.inl:405-423), scalar layout, 92 bytes. */
+typedef struct ct_object_hit { /* OBJECT_HIT, shared.inl:101-105 */
+  ct_object_info object;
+  float hit[3];
+} ct_object_hit;
+typedef struct ct_path_reservoir {
+  float M;
+  float weight;
+  uint32_t path_flags;
+  uint32_t rc_random_seed;
+  float F[3];
+  float light_pdf;
+  float cached_jacobian[3];
+  uint32_t init_random_seed;
+  ct_object_hit rc_vertex_hit;
+  float rc_vertex_wi[3];
+  float rc_vertex_irradiance[3];
+} ct_path_reservoir;
 void ct_restir_params_reference(ct_restir_params *out);
 
 typedef struct ct_frame_params {
@@ -404,6 +451,9 @@ int ct_frame_render(ct_ctx *ctx, ct_accel *accel, const ct_frame_params *params,
 #define CT_BUF_INT_RESERVOIR 5u  /* ct_reservoir: intermediate */
 #define CT_BUF_PREV_RESERVOIR 6u /* ct_reservoir: previous frame */
 #define CT_BUF_VELOCITY 7u       /* float2 */
+#define CT_BUF_PATH_RESERVOIR 8u          /* ct_path_reservoir: output (current frame) */
+#define CT_BUF_TEMPORAL_PATH_RESERVOIR 9u /* ct_path_reservoir: temporal (next frame's history) */
+#define CT_BUF_INDIRECT 10u               /* float3: indirect colour of the first pass */
 int ct_frame_read(ct_ctx *ctx, uint32_t which, void *host_dst, size_t bytes);
 
 typedef struct ct_frame_stats {
@@ -426,6 +476,13 @@ typedef struct ct_frame_stats {
   float ms_taa;
   float ms_trace_any;       /* sum over the any-hit traversal launches */
   float ms_trace_closest;
+  uint64_t bounce_rays;     /* ReSTIR PT path segments (closest hit) */
+  uint64_t shadow_rays_pt;  /* ReSTIR PT reconnection visibility queries */
+  uint64_t pt_nodes;        /* ReSTIR PT traversals (closest + any), counters on */
+  uint64_t pt_leaves;
+  uint64_t pt_instances;
+  float ms_pt;              /* initial paths + temporal path reuse */
+  float ms_pt_spatial;      /* spatial path reuse + final composition */
 } ct_frame_stats;
 /* Enable per-pass HIP events (1) and traversal node/leaf counters (2). */
 int ct_ctx_set_instrumentation(ct_ctx *ctx, uint32_t flags);

@@ -443,6 +443,7 @@ struct Scene {
 
 struct TraceCount {
   uint64_t rays = 0, nodes = 0, leaves = 0, instances = 0;
+  uint64_t bounce = 0;  // closest-hit path segments (counted apart from `rays`)
 };
 
 // Conservative slab test of a BVH child box (the build's own rule, used
@@ -473,6 +474,10 @@ static inline bool better(float t, uint32_t i, uint32_t p, const ClosestState &s
   return false;
 }
 
+// kBounce: the box test of intersect() (bounce.glsl:86-98, is_hit_from_ray
+// with rayCanStartInBox = true, non-oriented) accepting t >= 0.  The
+// reference commits the first generated hit; the closest one is taken.
+template <bool kBounce = false>
 static void closest_leaf(const Scene &sc, uint32_t ii, uint32_t pid, V3 oo, V3 od,
                          ClosestState &st) {
   const ct_instance &in = sc.inst[ii];
@@ -483,8 +488,8 @@ static void closest_leaf(const Scene &sc, uint32_t ii, uint32_t pid, V3 oo, V3 o
   float t;
   V3 nrm;
   V3 inv = v3(safe_inverse(od.x), safe_inverse(od.y), safe_inverse(od.z));
-  bool hit = intersect_box(c, rad, irad, oo, od, t, nrm, true, true, inv);
-  if (hit && t > 0.0f && better(t, ii, pid, st)) {
+  bool hit = intersect_box(c, rad, irad, oo, od, t, nrm, true, !kBounce, inv);
+  if (hit && (kBounce ? t >= 0.0f : t > 0.0f) && better(t, ii, pid, st)) {
     st.t = t;
     st.inst = ii;
     st.prim = pid;
@@ -512,7 +517,7 @@ static bool any_leaf(const Scene &sc, uint32_t ii, uint32_t pid, V3 oo, V3 od,
 
 // Two-level traversal: TLAS in world space, each BLAS in object space,
 // near child first (ties go to child 0), far child pushed.
-template <bool kAny>
+template <bool kAny, bool kBounce = false>
 static bool traverse(const Scene &sc, V3 o, V3 d, float limit, uint32_t ti, uint32_t tp,
                      ClosestState &st, TraceCount &cnt) {
   uint32_t stack[kStack];
@@ -536,7 +541,7 @@ static bool traverse(const Scene &sc, V3 o, V3 d, float limit, uint32_t ti, uint
           if (kAny) {
             if (any_leaf(sc, ii, pid, oo, od, limit, ti, tp)) return true;
           } else {
-            closest_leaf(sc, ii, pid, oo, od, st);
+            closest_leaf<kBounce>(sc, ii, pid, oo, od, st);
           }
         } else {
           cnt.nodes++;
@@ -591,6 +596,13 @@ static ClosestState trace_closest(const Scene &sc, V3 o, V3 d, float tmax, Trace
   ClosestState st{tmax, kInvalid, kInvalid};
   cnt.rays++;
   traverse<false>(sc, o, d, tmax, kInvalid, kInvalid, st, cnt);
+  return st;
+}
+
+static ClosestState trace_bounce(const Scene &sc, V3 o, V3 d, TraceCount &cnt) {
+  ClosestState st{kMaxDistance, kInvalid, kInvalid};
+  cnt.bounce++;
+  traverse<false, true>(sc, o, d, kMaxDistance, kInvalid, kInvalid, st, cnt);
   return st;
 }
 
@@ -671,6 +683,8 @@ struct Frame {
   std::vector<float> velocity;   // [W*H][2]
   std::vector<float> img[2];     // [W*H][4]
   std::vector<float> taa;        // [W*H][4]
+  std::vector<ct_path_reservoir> pres_o, pres_t;  // output / temporal path reservoirs
+  std::vector<float> indirect;   // [W*H][3] indirect colour of the first pass
   int cur = 0;
   bool first = true;
   ct_frame_stats stats{};
@@ -1103,6 +1117,737 @@ static void spatial_reuse(const Ctx &c, Res &r, float confidence, uint32_t x, ui
   r = mres;
 }
 
+// ================================================================ ReSTIR PT
+// Indirect illumination (INDIRECT_ILLUMINATION_ON with RESTIR_PT_ON):
+// trace_restir_path_tracing (path_tracer.glsl:854-887), intersect()
+// (bounce.glsl:9-130), the reconnection shift (shift.glsl:25-350) and the
+// temporal / spatial path reuse (indirect_illumination.glsl:147-452), for
+// Lambertian materials.
+
+constexpr float kHlfMax = 6.5504e+4f;        // shared.inl:56
+constexpr float kHlfMin = 1.175494351e-38f;  // shared.inl:57
+constexpr float kPiF = 3.1415926535897932384626433832795f;  // DAXA_PI, shared.inl:49
+constexpr float kVoxelExtent = 0.03125f;     // shared.inl:45
+
+// sin/cos of x in [0, 2*pi).  GLSL leaves their accuracy to the
+// implementation; this build fixes one: Cody-Waite reduction by pi/2 and
+// the Cephes single-precision minimax polynomials (S. L. Moshier,
+// sinf.c / cosf.c) on [-pi/4, pi/4].
+static void sincos_ct(float x, float &s, float &c) {
+  const float j = std::floor(x * 0.636619772367581343f + 0.5f);
+  const int q = (int)j;
+  const float r = ((x - j * 1.5703125f) - j * 4.837512969970703125e-4f) - j * 7.54978995489188216e-8f;
+  const float z = r * r;
+  const float sp = ((-1.9515295891e-4f * z + 8.3321608736e-3f) * z - 1.6666654611e-1f) * z * r + r;
+  const float cp = ((2.443315711809948e-5f * z - 1.388731625493765e-3f) * z + 4.166664568298827e-2f) * z * z -
+                   0.5f * z + 1.0f;
+  switch (q & 3) {
+    case 0: s = sp; c = cp; break;
+    case 1: s = cp; c = -sp; break;
+    case 2: s = -sp; c = -cp; break;
+    default: s = -cp; c = sp; break;
+  }
+}
+
+struct Isect {  // INTERSECT, shared.inl:323-334
+  bool is_hit = false;
+  float distance = 0.0f;
+  V3 P{0, 0, 0}, n{0, 0, 0}, wo{0, 0, 0}, wi{0, 0, 0};
+  uint32_t inst = CT_MAX_INSTANCES, prim = CT_MAX_PRIMITIVES, mat_idx = 10000u;  // MAX_MATERIALS
+  ct_material mat{};
+};
+
+static inline uint32_t pf_path_length(uint32_t f) { return f & 0xFu; }
+static inline uint32_t pf_rc_length(uint32_t f) { return (f >> 4) & 0xFu; }
+static inline void pf_set_path_length(uint32_t &f, uint32_t l) { f &= ~0xFu; f |= (l & 0xFu); }
+static inline void pf_set_rc_length(uint32_t &f, uint32_t l) { f &= ~0xF0u; f |= (l & 0xFu) << 4; }
+static inline bool pf_delta(uint32_t f, bool before) { return (f >> (before ? 20 : 21)) & 1u; }
+static inline bool pf_transmission(uint32_t f, bool before) { return (f >> (before ? 22 : 23)) & 1u; }
+static inline void pf_set_last_nee(uint32_t &f, bool b) { f &= ~0x10000u; f |= (uint32_t)(b ? 1 : 0) << 16; }
+static inline bool pf_last_nee(uint32_t f) { return (f >> 16) & 1u; }
+static inline void pf_set_light_type(uint32_t &f, uint32_t t) { f &= ~0xc0000u; f |= (t & 3u) << 18; }
+static inline uint32_t pf_light_type(uint32_t f) { return (f >> 18) & 3u; }
+
+// path_F_to_scalar (Rec. 601 weights), path_reservoir.glsl:160-162
+static inline float lum601(V3 c) { return dot(c, v3(0.299f, 0.587f, 0.114f)); }
+// eval_mis / power_heuristic, light.glsl:171-191
+static inline float eval_mis(float nf, float f_pdf, float ng, float g_pdf) {
+  float a = nf * f_pdf, b = ng * g_pdf;
+  float f = a * a, g = b * b;
+  return f / (f + g);
+}
+static inline bool bad3(V3 v) {
+  return std::isnan(v.x) || std::isnan(v.y) || std::isnan(v.z) || std::isinf(v.x) || std::isinf(v.y) ||
+         std::isinf(v.z);
+}
+static inline bool jac_invalid(float j) { return j <= 0.0f || std::isnan(j) || std::isinf(j); }
+
+static void pres_init(ct_path_reservoir &r, uint32_t max_depth) {  // path_reservoir.glsl:151-165
+  r = ct_path_reservoir{};
+  r.M = 0.0f;
+  r.weight = 0.0f;
+  r.path_flags = 0;
+  pf_set_rc_length(r.path_flags, max_depth);
+  r.rc_vertex_hit.object.instance_id = CT_MAX_INSTANCES;
+  r.rc_vertex_hit.object.primitive_id = CT_MAX_PRIMITIVES;
+}
+static bool pres_update(ct_path_reservoir &r, V3 in_F, float p, uint32_t &seed) {  // :183-202
+  r.M += 1.0f;
+  float w = lum601(in_F) / p;
+  if (std::isnan(w) || w == 0.0f) return false;
+  r.weight += w;
+  if (rnd(seed) * r.weight <= w) {
+    store3(r.F, in_F);
+    return true;
+  }
+  return false;
+}
+static void pres_copy_from(ct_path_reservoir &r, const ct_path_reservoir &in, V3 in_F) {
+  r.path_flags = in.path_flags;
+  r.rc_random_seed = in.rc_random_seed;
+  r.init_random_seed = in.init_random_seed;
+  std::memcpy(r.cached_jacobian, in.cached_jacobian, sizeof(r.cached_jacobian));
+  r.light_pdf = in.light_pdf;
+  std::memcpy(r.rc_vertex_wi, in.rc_vertex_wi, sizeof(r.rc_vertex_wi));
+  r.rc_vertex_hit = in.rc_vertex_hit;
+  std::memcpy(r.rc_vertex_irradiance, in.rc_vertex_irradiance, sizeof(r.rc_vertex_irradiance));
+  store3(r.F, in_F);
+}
+// path_reservoir_merge, path_reservoir.glsl:204-232
+static bool pres_merge(ct_path_reservoir &r, V3 in_F, float jac, const ct_path_reservoir &in, uint32_t &seed,
+                       float mis, bool force) {
+  float w = lum601(in_F) * jac * in.M * in.weight * mis;
+  r.M += in.M;
+  if (std::isnan(w) || w == 0.0f) return false;
+  r.weight += w;
+  if (force || rnd(seed) * r.weight <= w) {
+    pres_copy_from(r, in, in_F);
+    return true;
+  }
+  return false;
+}
+// path_reservoir_merge_with_resampling_MIS, path_reservoir.glsl:234-263
+static bool pres_merge_mis(ct_path_reservoir &r, V3 in_F, float jac, const ct_path_reservoir &in, uint32_t &seed,
+                           float mis) {
+  float w = lum601(in_F) * jac * in.weight * mis;
+  r.M += in.M;
+  if (std::isnan(w) || w == 0.0f) return false;
+  r.weight += w;
+  if (rnd(seed) * r.weight <= w) {
+    pres_copy_from(r, in, in_F);
+    return true;
+  }
+  return false;
+}
+
+// transpose(world2obj) * vec4(n, 0)
+static V3 normal_to_world(const float *m, V3 n) {
+  V3 o;
+  float *po = &o.x;
+  for (int r = 0; r < 3; ++r)
+    po[r] = m[4 * r + 0] * n.x + m[4 * r + 1] * n.y + m[4 * r + 2] * n.z + m[4 * r + 3] * 0.0f;
+  return o;
+}
+static V3 voxel_center(const Scene &sc, uint32_t gp) {
+  const ct_aabb &bb = sc.aabbs[gp];
+  return muls(add(load3(bb.minimum), load3(bb.maximum)), 0.5f);
+}
+// is_hit_from_ray, primitives.glsl:412-453, rayCanStartInBox = true
+// (its `oriented` argument is not forwarded: intersect_box runs
+// non-oriented with safeInverse of the object-space direction).
+static bool hit_from_ray(const Scene &sc, uint32_t ii, uint32_t pid, V3 ro, V3 rd, float &t, V3 &pos, V3 &nrm) {
+  if (ii >= sc.inst.size() || pid >= sc.inst[ii].primitive_count) return false;
+  const float *m = &sc.w2o[16 * (size_t)ii];
+  V3 oo = xform_point(m, ro), od = xform_dir(m, rd);
+  V3 rad = v3(kHalfVoxel, kHalfVoxel, kHalfVoxel);
+  V3 irad = v3(safe_inverse(kHalfVoxel), safe_inverse(kHalfVoxel), safe_inverse(kHalfVoxel));
+  V3 inv = v3(safe_inverse(od.x), safe_inverse(od.y), safe_inverse(od.z));
+  V3 on;
+  bool h = intersect_box(voxel_center(sc, sc.inst[ii].first_primitive_index + pid), rad, irad, oo, od, t, on, true,
+                         false, inv);
+  pos = add(ro, muls(rd, t));
+  nrm = normal_to_world(m, on);
+  return h;
+}
+static uint32_t material_index_of(const Scene &sc, uint32_t ii, uint32_t pid) {
+  return sc.prims[sc.inst[ii].first_primitive_index + pid].material_index;
+}
+static const ct_material &material_or_zero(const Scene &sc, uint32_t idx) {
+  static const ct_material zero{};
+  return idx < sc.mats.size() ? sc.mats[idx] : zero;
+}
+
+// intersect(), bounce.glsl:9-130
+static Isect intersect_bounce(const Ctx &c, V3 origin, V3 dir, TraceCount &cnt) {
+  const Scene &sc = *c.sc;
+  V3 ro = compute_ray_origin(origin, dir);
+  ClosestState st = trace_bounce(sc, ro, dir, cnt);
+  Isect i;
+  i.mat_idx = 0;
+  V3 int_hit = v3(0, 0, 0);
+  if (st.inst != kInvalid) {
+    float t;
+    V3 pos, nrm;
+    hit_from_ray(sc, st.inst, st.prim, ro, dir, t, pos, nrm);
+    int_hit = pos;
+    i.is_hit = true;
+    i.P = pos;
+    i.n = nrm;
+    i.inst = st.inst;
+    i.prim = st.prim;
+    i.mat_idx = material_index_of(sc, st.inst, st.prim);
+    i.mat = material_or_zero(sc, i.mat_idx);
+    i.distance = length(sub(origin, int_hit));
+  }
+  i.wo = normalize(sub(origin, int_hit));
+  return i;
+}
+
+// is_vertex_visible, light.glsl:82-142
+static bool vertex_visible(const Ctx &c, V3 o, V3 d, float dist, uint32_t ti, uint32_t tp, TraceCount &cnt) {
+  return !trace_any(*c.sc, o, d, dist, ti, tp, cnt);
+}
+
+// load_intersection_data_vertex_position, primitives.glsl:567-611
+static Isect load_vertex(const Ctx &c, const ct_object_hit &h, V3 world_pos) {
+  V3 rd = sub(load3(h.hit), world_pos);
+  float t;
+  V3 pos, nor;
+  Isect i;
+  if (!hit_from_ray(*c.sc, h.object.instance_id, h.object.primitive_id, world_pos, rd, t, pos, nor)) return i;
+  pos = compute_ray_origin(pos, nor);
+  pos = compute_ray_origin(pos, nor);
+  i.is_hit = true;
+  i.distance = length(sub(world_pos, pos));
+  i.P = pos;
+  i.n = nor;
+  i.wo = normalize(sub(world_pos, pos));
+  i.inst = h.object.instance_id;
+  i.prim = h.object.primitive_id;
+  i.mat_idx = material_index_of(*c.sc, i.inst, i.prim);
+  i.mat = material_or_zero(*c.sc, i.mat_idx);
+  return i;
+}
+
+// random_cosine_direction, prng.glsl:84-108
+static V3 random_cosine_direction(uint32_t &seed, V3 normal) {
+  float r1 = rnd(seed);
+  float r2 = rnd(seed);
+  float z = std::sqrt(1.0f - r2);
+  float phi = 2.0f * kPiF * r1;
+  float sp, cp;
+  sincos_ct(phi, sp, cp);
+  float x = cp * std::sqrt(r2);
+  float y = sp * std::sqrt(r2);
+  V3 u = std::fabs(normal.x) > 0.1f ? v3(0.0f, 1.0f, 0.0f) : v3(1.0f, 0.0f, 0.0f);
+  u = normalize(cross(normal, u));
+  V3 v = cross(normal, u);
+  return add(add(muls(u, x), muls(v, y)), muls(normal, z));
+}
+
+struct PathBuilder {  // path_builder.glsl:8-17
+  ct_object_hit rc_vertex_hit{};
+  V3 rc_vertex_wi{0, 0, 0};
+  uint32_t cached_random_seed = 0, seed = 0, rc_vertex_length = 0, path_flags = 0;
+  V3 cached_jacobian{0, 0, 0};
+};
+struct PathState {  // path_state.glsl:23-78
+  uint32_t max_depth = 0, flags = 0, path_length = 0, bounce_counters = 0;
+  float scene_length = 0.0f;
+  V3 origin{0, 0, 0}, dir{0, 0, 0};
+  float pdf = 0.0f;
+  V3 normal{0, 0, 0};
+  ct_object_hit hit{};
+  V3 thp{1, 1, 1}, prefix_thp{1, 1, 1}, L{0, 0, 0};
+  float rr_pdf = 1.0f, prev_scatter_pdf = 1.0f;
+  uint32_t seed = 0;
+  PathBuilder pb;
+  ct_path_reservoir res{};
+  bool last_rough = true;
+};
+constexpr uint32_t kPfActive = 0x1, kPfHit = 0x2, kPfTransmission = 0x4, kPfDelta = 0x10, kPfUpper = 0x80,
+                   kPfLower = 0x100, kPfDiffusePrimary = 0x200;  // defines.glsl:64-76
+
+static inline void path_set_flag(PathState &p, uint32_t f, bool v) {  // path_state.glsl:81-91
+  if (v) p.flags |= f;
+  else p.flags &= ~f;
+}
+static inline void set_objhit(ct_object_hit &h, uint32_t i, uint32_t pr, V3 pos) {
+  h.object.instance_id = i;
+  h.object.primitive_id = pr;
+  store3(h.hit, pos);
+}
+
+// generate_scatter_ray (Lambertian callable, rcall_scatter.glsl:73-84) and
+// path_generate_scatter_ray, path_tracer.glsl:59-112
+static bool path_generate_scatter_ray(PathState &p, Isect &i) {
+  V3 sd = random_cosine_direction(p.seed, i.n);
+  if (std::fabs(sd.x) < 1e-8f && std::fabs(sd.y) < 1e-8f && std::fabs(sd.z) < 1e-8f) sd = i.n;
+  i.wi = sd;
+  V3 weight = evaluate_material(i.mat, i.n, i.wo, i.wi);
+  float pdf = sample_material_pdf(i.mat, i.n, i.wo, i.wi);
+  weight = divs(weight, pdf);
+  p.dir = i.wi;
+  if (p.path_length == 0) path_set_flag(p, kPfDiffusePrimary, true);
+  p.thp = mulv(p.thp, weight);
+  p.pdf = pdf;
+  if (p.path_length == p.pb.rc_vertex_length) p.pb.cached_jacobian.y = pdf;
+  p.prev_scatter_pdf = pdf;
+  if (pdf <= 0.0f) return false;
+  p.bounce_counters += 1u << 0;  // BOUNCE_TYPE_DIFFUSE
+  p.normal = i.n;
+  bool valid = p.thp.x > 0.0f || p.thp.y > 0.0f || p.thp.z > 0.0f;
+  if (valid && p.path_length <= p.pb.rc_vertex_length) {
+    p.prefix_thp = p.thp;
+    p.thp = v3(1.0f, 1.0f, 1.0f);
+  }
+  return valid;
+}
+
+// path_builder_add_escape_vertex, path_builder.glsl:36-75
+static void add_escape_vertex(PathBuilder &pb, uint32_t path_length, V3 wi, V3 path_weight, V3 postfix,
+                              float rr_pdf, float mis_weight, float light_pdf, uint32_t light_type,
+                              ct_path_reservoir &res) {
+  if (path_length < 1) return;
+  bool is_rc = path_length == pb.rc_vertex_length;
+  if (!pres_update(res, path_weight, rr_pdf, pb.seed)) return;
+  res.path_flags = pb.path_flags;
+  pf_set_path_length(res.path_flags, path_length);
+  pf_set_rc_length(res.path_flags, pb.rc_vertex_length);
+  pf_set_last_nee(res.path_flags, false);
+  store3(res.rc_vertex_irradiance, postfix);
+  store3(res.F, path_weight);
+  store3(res.rc_vertex_wi, pb.rc_vertex_wi);
+  res.rc_vertex_hit = pb.rc_vertex_hit;
+  store3(res.cached_jacobian, pb.cached_jacobian);
+  res.rc_random_seed = pb.cached_random_seed;
+  if (is_rc) {
+    store3(res.rc_vertex_wi, wi);
+    store3(res.rc_vertex_irradiance, divs(load3(res.rc_vertex_irradiance), mis_weight));
+  }
+  res.light_pdf = light_pdf;
+  pf_set_light_type(res.path_flags, light_type);
+}
+
+// eval_voxel_pdf, path_tracer.glsl:185-211
+static float eval_voxel_pdf(V3 pos_W, const Isect &i) {
+  V3 to_light = sub(i.P, pos_W);
+  const float dist_sqr = dot(to_light, to_light);
+  if (dist_sqr <= kHlfMin) return 0.0f;
+  V3 L = divs(to_light, std::sqrt(dist_sqr));
+  float cos_theta = dot(i.n, neg(L));
+  if (cos_theta <= 0.0f) return 0.0f;
+  const float side = kVoxelExtent;
+  const float area = side * side;
+  float denom = gmax(kHlfMin, cos_theta * area);
+  return dist_sqr / denom;
+}
+
+// path_handle_emissive_hit, path_tracer.glsl:258-337
+static void path_handle_emissive_hit(const Ctx &c, PathState &p, const Isect &i) {
+  float light_pdf = 0.0f, mis = 1.0f;
+  V3 Lr = v3(0, 0, 0);
+  V3 em = load3(i.mat.emission);
+  if (p.path_length != 0) {
+    float light_count = (float)c.sc->cube.size();
+    light_pdf = (1.0f / light_count) * eval_voxel_pdf(p.origin, i);
+    mis = eval_mis(1, p.pdf, 1, light_pdf);
+    Lr = muls(mulv(mulv(p.thp, p.prefix_thp), em), mis);
+  }
+  V3 att = mulv(mulv(p.thp, p.prefix_thp), em);
+  if (p.path_length > 1) p.L = add(p.L, Lr);
+  V3 postfix = muls(mulv(p.thp, att), mis);
+  uint32_t plen = (uint32_t)std::max(0, (int)p.path_length - 1);
+  add_escape_vertex(p.pb, plen, p.dir, Lr, postfix, p.rr_pdf, mis, light_pdf, CT_LIGHT_CUBE, p.res);
+}
+
+// path_generate_light_sample, path_tracer.glsl:222-256: the light sample
+// draws its random numbers; its MIS weight uses ls.pdf, which the shader
+// never assigns (0), so the NEE term it feeds to the reservoir is 0 or NaN
+// and is rejected without a draw (path_reservoir.glsl:189-190).  The light
+// list is the cube lights (the shader's `light_count` is undeclared in the
+// prepass, rgen.glsl:281).
+static void path_light_sample(const Ctx &c, const Isect &i, uint32_t &seed) {
+  const uint32_t n = (uint32_t)c.sc->cube.size();
+  uint32_t li = std::min(urnd_interval(seed, 0, n), n - 1u);
+  if (n == 0) return;
+  Hit h;
+  h.world_hit = i.P;
+  h.world_nrm = i.n;
+  h.distance = i.distance;
+  h.scatter_dir = neg(i.wo);
+  h.inst = i.inst;
+  h.prim = i.prim;
+  h.mat_idx = i.mat_idx;
+  float pdf = 1.0f / (float)n;
+  LightSample ls;
+  TraceCount none;
+  sample_lights(c, h, c.sc->cube[li], pdf, ls, seed, true, false, none);
+}
+
+// path_handle_hit, path_tracer.glsl:599-727 (use_hybrid_shift = false,
+// path_state.glsl:245-247)
+static void path_handle_hit(const Ctx &c, PathState &p, Isect &i) {
+  const ct_restir_params &rp = c.p->restir;
+  bool compute_emissive = true;
+  if (p.path_length == 1 && !(p.flags & kPfTransmission) && !(p.flags & kPfDelta)) compute_emissive = false;
+  V3 em = load3(i.mat.emission);
+  if (compute_emissive && (em.x > 0.0f || em.y > 0.0f || em.z > 0.0f)) path_handle_emissive_hit(c, p, i);
+  if ((p.bounce_counters & 0xffu) > p.max_depth) {
+    path_set_flag(p, kPfActive, false);
+    return;
+  }
+  V3 prev_origin = p.origin;
+  p.origin = compute_ray_origin(i.P, i.n);
+  p.origin = compute_ray_origin(p.origin, i.n);
+  bool cur_rough = i.mat.roughness > rp.roughness_threshold;
+  // path_check_rc_vertex, path_tracer.glsl:484-597 (!use_hybrid_shift)
+  if (p.path_length == 1) {
+    p.pb.rc_vertex_hit = p.hit;
+    p.pb.cached_jacobian.x = p.prev_scatter_pdf;
+    V3 disp = sub(prev_origin, i.P);
+    p.pb.cached_jacobian.z = std::fabs(dot(i.n, i.wo)) / dot(disp, disp);
+    if (std::isnan(p.pb.cached_jacobian.z)) p.pb.cached_jacobian.z = 0.0f;
+    p.pb.rc_vertex_length = p.path_length;
+  }
+  path_set_flag(p, kPfUpper, false);
+  path_set_flag(p, kPfLower, false);
+  // path_handle_sample_light, path_tracer.glsl:339-438
+  if (p.path_length == p.pb.rc_vertex_length) p.pb.cached_random_seed = p.seed;
+  path_light_sample(c, i, p.seed);
+  path_set_flag(p, kPfUpper, true);
+  path_set_flag(p, kPfLower, false);
+  if (rp.pt_russian_roulette && p.path_length >= 1) {
+    // terminate_path_by_russian_roulette, path_tracer.glsl:165-177
+    float rr = luminance(mulv(p.thp, p.prefix_thp));
+    float prob = gmax(0.0f, 1.0f - rr);
+    if (rnd(p.seed) < prob) {
+      path_set_flag(p, kPfActive, false);
+      return;
+    }
+    p.rr_pdf *= 1.0f - prob;
+  }
+  bool valid = path_generate_scatter_ray(p, i);
+  if (p.path_length == p.pb.rc_vertex_length) p.pb.cached_random_seed = p.seed;
+  p.pb.rc_vertex_wi = p.dir;
+  p.last_rough = cur_rough;
+  if (!(p.flags & kPfActive)) return;
+  if ((p.bounce_counters & 0xffu) > p.max_depth) valid = false;
+  if (!valid) path_set_flag(p, kPfActive, false);
+}
+
+// path_handle_miss, path_tracer.glsl:729-815
+static void path_handle_miss(const Ctx &c, PathState &p) {
+  float light_pdf = kInv4Pi;  // env_map_sampler_eval_pdf, light.glsl:71
+  float mis = eval_mis(1, p.pdf, 1, light_pdf);
+  V3 Le = sky_color(c.p->time, c.p->is_afternoon != 0, p.dir);
+  V3 Lr = muls(mulv(mulv(p.thp, p.prefix_thp), Le), mis);
+  if (p.path_length > 0) p.L = add(p.L, Lr);
+  add_escape_vertex(p.pb, p.path_length, p.dir, Lr, muls(mulv(p.thp, Le), mis), p.rr_pdf, mis, light_pdf,
+                    CT_LIGHT_ENV_MAP, p.res);
+  path_set_flag(p, kPfActive, false);
+}
+
+// trace_restir_path_tracing, path_tracer.glsl:854-887
+static ct_path_reservoir trace_restir_path(const Ctx &c, Isect i, uint32_t &seed, V3 &color, TraceCount &bcnt) {
+  const uint32_t D = c.p->restir.max_depth;
+  PathState p;
+  // generate_path, path_state.glsl:208-249
+  p.max_depth = D;
+  p.flags = 0;
+  p.path_length = 0;
+  path_set_flag(p, kPfActive, true);
+  p.bounce_counters = 0;
+  set_objhit(p.hit, i.inst, i.prim, i.P);
+  path_set_flag(p, kPfHit, true);
+  p.seed = seed;
+  p.pb.seed = seed;
+  p.pb.rc_vertex_hit.object.instance_id = CT_MAX_INSTANCES;
+  p.pb.rc_vertex_hit.object.primitive_id = CT_MAX_PRIMITIVES;
+  p.pb.rc_vertex_length = D;
+  pres_init(p.res, D);
+  // path_handle_primary_hit, path_tracer.glsl:440-469
+  p.origin = compute_ray_origin(i.P, i.n);
+  p.origin = compute_ray_origin(p.origin, i.n);
+  path_set_flag(p, kPfUpper, false);
+  path_set_flag(p, kPfLower, false);
+  p.res.init_random_seed = p.seed;
+  bool valid = path_generate_scatter_ray(p, i);
+  if (p.path_length == p.pb.rc_vertex_length) p.pb.cached_random_seed = p.seed;
+  p.pb.rc_vertex_wi = p.dir;
+  if (valid) {
+    do {
+      i = intersect_bounce(c, p.origin, p.dir, bcnt);  // path_next_vertex, :136-149
+      if (i.is_hit) {
+        set_objhit(p.hit, i.inst, i.prim, i.P);
+        path_set_flag(p, kPfHit, true);
+        p.path_length++;
+        p.scene_length += i.distance;
+        path_handle_hit(c, p, i);
+      } else {
+        path_set_flag(p, kPfHit, false);
+        p.scene_length = kHlfMax;
+        path_handle_miss(c, p);
+      }
+    } while (p.flags & kPfActive);
+  }
+  p.res.M = 1.0f;  // path_builder_finilize
+  // path_output, path_tracer.glsl:822-835 (path_reservoir_finalize_RIS)
+  float p_hat = lum601(load3(p.res.F));
+  if (p_hat == 0.0f || p.res.M == 0.0f)
+    p.res.weight = 0.0f;
+  else
+    p.res.weight = p.res.weight / (p_hat * p.res.M);
+  color = muls(load3(p.res.F), p.res.weight);
+  if (bad3(color)) color = v3(0, 0, 0);
+  seed = p.seed;
+  return p.res;
+}
+
+// compute_new_ray_origin, primitives.glsl:259-264
+static V3 compute_new_ray_origin(V3 pos, V3 n, bool view_side) {
+  pos = compute_ray_origin(pos, view_side ? n : neg(n));
+  return compute_ray_origin(pos, view_side ? n : neg(n));
+}
+
+// compute_shifted_integrand_reconnection, shift.glsl:25-350
+// (use_hybrid_shift = false, use_cached_jacobian = false)
+static V3 shift_reconnection(const Ctx &c, float &dst_jacobian, Isect dst, Isect src, const ct_path_reservoir &s,
+                             bool eval_visibility, TraceCount &acnt) {
+  dst_jacobian = 0.0f;
+  const uint32_t rc_vertex_length = 1;
+  const ct_object_hit &rch = s.rc_vertex_hit;
+  V3 irr = load3(s.rc_vertex_irradiance), wi = load3(s.rc_vertex_wi);
+  bool exists = rch.object.instance_id < CT_MAX_INSTANCES && rch.object.primitive_id < CT_MAX_PRIMITIVES;
+  uint32_t path_length = pf_path_length(s.path_flags);
+  bool last_nee = pf_last_nee(s.path_flags);
+  bool trans = pf_transmission(s.path_flags, true);
+  src.P = compute_new_ray_origin(src.P, src.n, !trans);
+  dst.P = compute_new_ray_origin(dst.P, dst.n, !trans);
+  if (!exists) {
+    V3 integrand = v3(0, 0, 0);
+    if (pf_light_type(s.path_flags) == CT_LIGHT_ENV_MAP && path_length + 1 == rc_vertex_length && !last_nee) {
+      if (vertex_visible(c, dst.P, wi, kMaxDistance, kInvalid, kInvalid, acnt)) {  // is_segment_visible
+        float src_pdf1 = sample_material_pdf(src.mat, src.n, src.wo, wi);
+        float dst_pdf1 = sample_material_pdf(dst.mat, dst.n, dst.wo, wi);
+        V3 dst_f1 = evaluate_material(dst.mat, dst.n, dst.wo, wi);
+        float mis = eval_mis(1, dst_pdf1, 1, s.light_pdf);
+        integrand = mulv(muls(divs(dst_f1, dst_pdf1), mis), irr);
+        dst_jacobian = dst_pdf1 / src_pdf1;
+      }
+    }
+    if (jac_invalid(dst_jacobian)) dst_jacobian = 0.0f;
+    if (bad3(integrand)) return v3(0, 0, 0);
+    return integrand;
+  }
+  bool is_final = path_length == rc_vertex_length;
+  bool escaped = path_length + 1 == rc_vertex_length && !last_nee;
+  bool rc_nee = is_final && last_nee;
+  if (pf_delta(s.path_flags, true) || pf_delta(s.path_flags, false)) return v3(0, 0, 0);
+  Isect rc = load_vertex(c, rch, dst.P);
+  V3 dst_cv = neg(rc.wo);
+  V3 src_cv = normalize(sub(rc.P, src.P));
+  V3 shifted_disp = sub(rc.P, dst.P);
+  float shifted_dist2 = dot(shifted_disp, shifted_disp);
+  float shifted_cosine = std::fabs(dot(rc.n, neg(dst_cv)));
+  float cj_z = shifted_cosine / shifted_dist2;
+  V3 original_disp = sub(rc.P, src.P);
+  float original_dist2 = dot(original_disp, original_disp);
+  float original_cosine = std::fabs(dot(rc.n, neg(src_cv)));
+  float jacobian = cj_z * original_dist2 / original_cosine;
+  if (jac_invalid(jacobian)) return v3(0, 0, 0);
+  float dst_pdf1 = sample_material_pdf(dst.mat, dst.n, dst.wo, dst_cv);
+  float dst_pdf1_all = dst_pdf1;
+  float src_pdf1 = sample_material_pdf(src.mat, src.n, src.wo, src_cv);
+  jacobian *= dst_pdf1 / src_pdf1;
+  if (jac_invalid(jacobian)) return v3(0, 0, 0);
+  V3 dst_f1 = evaluate_material(dst.mat, dst.n, dst.wo, dst_cv);
+  float dst_rc_all = 0.0f, dst_pdf2 = 1.0f, dst_rc = 1.0f, src_rc = 1.0f;
+  if (!escaped) {
+    dst_rc = sample_material_pdf(rc.mat, rc.n, rc.wo, wi);
+    dst_rc_all = dst_rc;
+    src_rc = sample_material_pdf(rc.mat, rc.n, neg(src_cv), wi);
+    dst_pdf2 = !rc_nee ? dst_rc : s.light_pdf;
+  }
+  V3 dst_f2 = v3(1.0f, 1.0f, 1.0f);
+  if (!escaped) dst_f2 = evaluate_material(rc.mat, rc.n, rc.wo, wi);
+  if ((dst_f1.x == 0.0f && dst_f1.y == 0.0f && dst_f1.z == 0.0f) ||
+      (dst_f2.x == 0.0f && dst_f2.y == 0.0f && dst_f2.z == 0.0f))
+    return v3(0, 0, 0);
+  V3 no_f1 = mulv(divs(dst_f2, dst_pdf2), irr);
+  V3 integrand = mulv(divs(dst_f1, dst_pdf1), no_f1);
+  if (escaped) integrand = muls(integrand, eval_mis(1, dst_pdf1_all, 1, s.light_pdf));
+  if (is_final) {
+    if (pf_light_type(s.path_flags) != 4u) {  // GEOMETRY_LIGHT_ANALITIC
+      float lp = s.light_pdf;
+      float mis = eval_mis(1, rc_nee ? lp : dst_rc_all, 1, rc_nee ? dst_rc_all : lp);
+      integrand = muls(integrand, mis);
+      no_f1 = muls(no_f1, mis);
+      if (!rc_nee) jacobian *= dst_rc / src_rc;
+    }
+  }
+  if (!is_final && !escaped) jacobian *= dst_rc / src_rc;
+  if (jac_invalid(jacobian)) return v3(0, 0, 0);
+  if (eval_visibility) {
+    V3 dir = normalize(sub(rc.P, dst.P));
+    float distance = length(sub(dst.P, rc.P));
+    if (!vertex_visible(c, dst.P, dir, distance, rc.inst, rc.prim, acnt)) return v3(0, 0, 0);
+  }
+  if (bad3(integrand)) return v3(0, 0, 0);
+  dst_jacobian = jacobian;
+  return integrand;
+}
+
+// compute_shifted_integrand_, shift.glsl:446-483.  The hybrid shift runs on
+// RECONNECTION_DATA whose storage the reference stubs
+// (path_reservoir.glsl:37-58): with it compute_shifted_integrand_hybrid
+// (shift.glsl:352-444) returns tp = 1 for reconnection length 1, tp = 0
+// otherwise, and leaves dst_jacobian = 0.
+static V3 shifted_integrand(const Ctx &c, float &dst_jacobian, const Isect &dst, const Isect &src,
+                            const ct_path_reservoir &s, bool eval_visibility, TraceCount &acnt) {
+  dst_jacobian = 0.0f;
+  if (s.weight == 0.0f) return v3(0, 0, 0);
+  const uint32_t shift = c.p->restir.pt_shift_mapping;
+  if (shift == CT_SHIFT_RECONNECTION) return shift_reconnection(c, dst_jacobian, dst, src, s, eval_visibility, acnt);
+  if (shift == CT_SHIFT_HYBRID) return pf_rc_length(s.path_flags) == 1 ? v3(1, 1, 1) : v3(0, 0, 0);
+  return v3(1, 1, 1);
+}
+
+// shift_and_merge_reservoir, shift.glsl:485-510 (force_merge)
+static bool shift_and_merge(const Ctx &c, float &dst_jacobian, const Isect &dst, ct_path_reservoir &dst_res,
+                            const Isect &src, const ct_path_reservoir &src_res, uint32_t &seed, TraceCount &acnt) {
+  V3 integrand = shifted_integrand(c, dst_jacobian, dst, src, src_res, true, acnt);
+  bool selected = pres_merge(dst_res, integrand, dst_jacobian, src_res, seed, 1.0f, true);
+  if (!selected) store3(dst_res.F, v3(0, 0, 0));
+  dst_res.M = src_res.M;
+  dst_res.weight = src_res.weight;
+  return selected;
+}
+
+static void pres_finalize_gris(ct_path_reservoir &r) {  // path_reservoir.glsl:273-279
+  float p_hat = lum601(load3(r.F));
+  if (p_hat == 0.0f) r.weight = 0.0f;
+  else r.weight = r.weight / p_hat;
+}
+
+// temporal_path_reuse, indirect_illumination.glsl:147-279
+static ct_path_reservoir temporal_path_reuse(const Ctx &c, const ct_path_reservoir &central,
+                                             ct_path_reservoir temporal, const Isect &cur_i, const Isect &prev_i,
+                                             uint32_t &seed, TraceCount &acnt) {
+  ct_path_reservoir dest;
+  pres_init(dest, c.p->restir.max_depth);
+  const float current_M = central.M;
+  temporal.M = gmin(c.p->restir.pt_m_cap * current_M, temporal.M);
+  float dst_jacobian = 0.0f;
+  for (int i = -1; i <= 0; ++i) {
+    float p_sum = 0.0f, p_self = 0.0f;
+    ct_path_reservoir temp_dst = dest;
+    bool possible = false;
+    if (i == -1) {
+      temp_dst = central;
+      dst_jacobian = 1.0f;
+      possible = temp_dst.weight > 0.0f;
+    } else {
+      possible = shift_and_merge(c, dst_jacobian, cur_i, temp_dst, prev_i, temporal, seed, acnt);
+    }
+    if (possible) {
+      for (int j = -1; j <= 0; ++j) {
+        if (j == -1) {
+          float cur_p = lum601(load3(temp_dst.F)) * current_M;
+          p_sum += cur_p;
+          if (i == -1) p_self = cur_p;
+        } else {
+          if (i == j) {
+            p_self = lum601(load3(temporal.F)) / dst_jacobian * temporal.M;
+            p_sum += p_self;
+            continue;
+          }
+          float tnj = 0.0f;
+          V3 tni = shifted_integrand(c, tnj, prev_i, cur_i, temp_dst, true, acnt);
+          float pp = lum601(tni) * tnj;
+          p_sum += pp * temporal.M;
+        }
+      }
+    }
+    float mis = p_sum == 0.0f ? 0.0f : p_self / p_sum;
+    pres_merge_mis(dest, load3(temp_dst.F), dst_jacobian, temp_dst, seed, mis);
+  }
+  if (dest.weight > 0.0f) pres_finalize_gris(dest);
+  if (dest.weight < 0.0f || std::isinf(dest.weight) || std::isnan(dest.weight)) dest.weight = 0.0f;
+  return dest;
+}
+
+static Isect isect_from_di(const Ctx &c, const ct_di_info &d, bool normalize_wo) {
+  Isect i;
+  i.is_hit = d.distance > 0.0f;
+  i.distance = d.distance;
+  i.P = load3(d.position);
+  i.n = load3(d.normal);
+  i.wo = sub(load3(d.ray_origin), i.P);
+  if (normalize_wo) i.wo = normalize(i.wo);
+  i.wi = v3(0, 0, 0);
+  i.inst = d.instance_hit.instance_id;
+  i.prim = d.instance_hit.primitive_id;
+  i.mat_idx = d.mat_index;
+  i.mat = material_or_zero(*c.sc, d.mat_index);
+  return i;
+}
+
+// indirect_illumination_spatial_reuse, indirect_illumination.glsl:312-452
+static ct_path_reservoir spatial_path_reuse(const Ctx &c, uint32_t x, uint32_t y, const Isect &ci, V3 camera_pos,
+                                            uint32_t &seed, TraceCount &acnt) {
+  const ct_frame_params &p = *c.p;
+  const uint32_t W = p.width, H = p.height;
+  const ct_path_reservoir &central = c.fr->pres_o[y * W + x];
+  ct_path_reservoir dest;
+  pres_init(dest, p.restir.max_depth);
+  const int ncount = (int)p.restir.pt_neighbor_count;
+  int valid = 0;
+  float canonical = 1.0f;
+  for (int i = 0; i < ncount; ++i) {
+    // get_next_neighbor_pixel, prng.glsl:272-296
+    float ox = 2.0f * rnd(seed) - 1.0f;
+    float oy = 2.0f * rnd(seed) - 1.0f;
+    int nx = (int)x + f2i_sat(ox * (float)p.restir.pt_neighbor_radius);
+    int ny = (int)y + f2i_sat(oy * (float)p.restir.pt_neighbor_radius);
+    if (nx == (int)x && ny == (int)y) nx = ny = -1;
+    if (nx < 0 || ny < 0 || nx >= (int)W || ny >= (int)H) continue;
+    const uint32_t ni = (uint32_t)ny * W + (uint32_t)nx;
+    Isect nis = isect_from_di(c, c.fr->di_cur[ni], false);
+    if (!(nis.inst < CT_MAX_INSTANCES && nis.prim < CT_MAX_PRIMITIVES)) continue;
+    // is_valid_geometry, primitives.glsl:547-556
+    float cd = length(sub(camera_pos, ci.P)), nd = length(sub(camera_pos, nis.P));
+    if (!(dot(ci.n, nis.n) >= 0.5f && std::fabs(cd - nd) < 0.1f * cd)) continue;
+    const ct_path_reservoir &nres = c.fr->pres_o[ni];
+    valid++;
+    float prefix_jac = 0.0f;
+    V3 prefix = shifted_integrand(c, prefix_jac, nis, ci, central, true, acnt);
+    float prefix_pdf = lum601(prefix) * prefix_jac;
+    canonical += 1.0f;
+    if (prefix_pdf > 0.0f)
+      canonical -= prefix_pdf * nres.M /
+                   (prefix_pdf * nres.M + central.M * lum601(load3(central.F)) / (float)ncount);
+    ct_path_reservoir tdst = dest;
+    float dst_jacobian = 0.0f;
+    bool possible = shift_and_merge(c, dst_jacobian, ci, tdst, nis, nres, seed, acnt);
+    float nw = 0.0f;
+    if (possible) {
+      float a = lum601(load3(nres.F)), b = lum601(load3(tdst.F));
+      nw = a / dst_jacobian * nres.M / ((a / dst_jacobian) * nres.M + b * central.M / (float)ncount);
+      if (std::isnan(nw) || std::isinf(nw)) nw = 0.0f;
+    }
+    pres_merge_mis(dest, load3(tdst.F), dst_jacobian, tdst, seed, nw);
+  }
+  pres_merge_mis(dest, load3(central.F), 1.0f, central, seed, canonical);
+  if (dest.weight > 0.0f) {
+    pres_finalize_gris(dest);
+    dest.weight /= (float)(valid + 1);
+  }
+  if (dest.weight < 0.0f) dest.weight = 0.0f;
+  if (std::isnan(dest.weight) || std::isinf(dest.weight)) dest.weight = 0.0f;
+  return dest;
+}
+
 // get_ray_from_current_pixel, rgen.glsl:18-42 (jitter = 0)
 static void camera_ray(const ct_frame_params &p, uint32_t x, uint32_t y, V3 &o, V3 &d) {
   float pcx = ((float)x + 0.0f) + 0.5f, pcy = ((float)y + 0.0f) + 0.5f;
@@ -1128,7 +1873,8 @@ static void rchit_model(const float *t, float *m) {
 }
 
 // RESTIR_PREPASS_AND_FIRST_VISIBILITY_TEST, rgen.glsl:46-320
-static void prepass_pixel(const Ctx &c, uint32_t x, uint32_t y, TraceCount &cnt, TraceCount &pc) {
+static void prepass_pixel(const Ctx &c, uint32_t x, uint32_t y, TraceCount &cnt, TraceCount &pc,
+                          TraceCount &ptc) {
   const ct_frame_params &p = *c.p;
   Frame &fr = *c.fr;
   const uint32_t W = p.width, H = p.height;
@@ -1205,12 +1951,33 @@ static void prepass_pixel(const Ctx &c, uint32_t x, uint32_t y, TraceCount &cnt,
     Res r = ris(c, confidence, rd, hit, mat, di.seed, cnt);
     if (prev.W_y > 0.0f) temporal_reuse(c, r, prev, pidx, rd, hit, mat, di.seed, cnt);
     fr.res_int[idx] = res_to(r);
+    if (p.restir.indirect) {
+      // rgen.glsl:271-315: indirect_illumination_restir_path_tracing
+      // (indirect_illumination.glsl:454-475)
+      const Isect ci = isect_from_di(c, di, true);
+      V3 ind;
+      ct_path_reservoir out = trace_restir_path(c, ci, di.seed, ind, ptc);
+      if (p.restir.pt_temporal) {
+        // temporal_path_get_reprojected, indirect_illumination.glsl:29-57
+        const float vx = fr.velocity[2 * (size_t)idx], vy = fr.velocity[2 * (size_t)idx + 1];
+        const float r1 = rnd(di.seed);
+        const float r2 = rnd(di.seed);
+        uint32_t pcx = f2u_sat(((float)x + vx) + r1), pcy = f2u_sat(((float)y + vy) + r2);
+        uint32_t prev_idx = std::min(W * H - 1u, pcx + pcy * W);
+        const Isect pi = isect_from_di(c, fr.di_prev[prev_idx], false);
+        out = temporal_path_reuse(c, out, fr.pres_t[prev_idx], ci, pi, di.seed, ptc);
+        ind = muls(load3(out.F), out.weight);
+      }
+      if (bad3(ind)) ind = v3(0, 0, 0);
+      fr.pres_o[idx] = out;
+      store3(&fr.indirect[3 * (size_t)idx], ind);
+    }
   }
   fr.di_cur[idx] = di;
 }
 
 // THIRD_VISIBILITY_TEST_AND_SHADING_PASS, rgen.glsl:329-510
-static void shade_pixel(const Ctx &c, uint32_t x, uint32_t y, TraceCount &cnt) {
+static void shade_pixel(const Ctx &c, uint32_t x, uint32_t y, TraceCount &cnt, TraceCount &ptc) {
   const ct_frame_params &p = *c.p;
   Frame &fr = *c.fr;
   const uint32_t idx = y * p.width + x;
@@ -1225,6 +1992,21 @@ static void shade_pixel(const Ctx &c, uint32_t x, uint32_t y, TraceCount &cnt) {
     spatial_reuse(c, sr, confidence, x, y, ray_d, hit, mat, di.seed, cnt);
     V3 hv = v3(0, 0, 0);
     hv = add(hv, muls(sr.F, sr.W_y));
+    if (p.restir.indirect) {
+      // rgen.glsl:427-452
+      V3 ind;
+      if (p.restir.pt_spatial) {
+        const Isect ci = isect_from_di(c, di, true);
+        const V3 cam = v3(p.inv_view[12], p.inv_view[13], p.inv_view[14]);
+        ct_path_reservoir fin = spatial_path_reuse(c, x, y, ci, cam, di.seed, ptc);
+        fr.pres_t[idx] = fin;
+        ind = muls(load3(fin.F), fin.weight);
+      } else {
+        if (p.restir.pt_temporal) fr.pres_t[idx] = fr.pres_o[idx];
+        ind = load3(&fr.indirect[3 * (size_t)idx]);
+      }
+      hv = add(hv, ind);
+    }
     if (p.restir.direct_emittance) hv = add(hv, load3(mat.emission));
     if (std::isinf(hv.x) || std::isnan(hv.x)) hv.x = 0.0f;
     if (std::isinf(hv.y) || std::isnan(hv.y)) hv.y = 0.0f;
@@ -1441,6 +2223,66 @@ void oc_trace_closest(oc_scene *scene, uint32_t n, const float *origins, const f
   }
 }
 
+void oc_trace_bounce(oc_scene *scene, uint32_t n, const float *origins, const float *directions, int brute,
+                     uint32_t *ids_out, float *pos_out, float *nrm_out, float *dist_out, float *wo_out,
+                     ct_trace_stats *stats) {
+  uint64_t rays = 0, nodes = 0, leaves = 0, insts = 0;
+  Ctx c;
+  c.sc = scene;
+  c.fr = nullptr;
+  c.p = nullptr;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : rays, nodes, leaves, insts)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    V3 o = load3(&origins[3 * i]), d = load3(&directions[3 * i]);
+    TraceCount cnt;
+    Isect h;
+    if (brute) {
+      // every voxel, same box test and tie rule, then intersect()'s record
+      V3 ro = compute_ray_origin(o, d);
+      ClosestState st{kMaxDistance, kInvalid, kInvalid};
+      for (uint32_t k = 0; k < scene->inst.size(); ++k) {
+        const float *m = &scene->w2o[16 * (size_t)k];
+        V3 oo = xform_point(m, ro), od = xform_dir(m, d);
+        for (uint32_t q = 0; q < scene->inst[k].primitive_count; ++q) closest_leaf<true>(*scene, k, q, oo, od, st);
+      }
+      h.mat_idx = 0;
+      V3 int_hit = v3(0, 0, 0);
+      if (st.inst != kInvalid) {
+        float t;
+        V3 pos, nrm;
+        hit_from_ray(*scene, st.inst, st.prim, ro, d, t, pos, nrm);
+        int_hit = pos;
+        h.is_hit = true;
+        h.P = pos;
+        h.n = nrm;
+        h.inst = st.inst;
+        h.prim = st.prim;
+        h.distance = length(sub(o, int_hit));
+      }
+      h.wo = normalize(sub(o, int_hit));
+      cnt.bounce = 1;
+    } else {
+      h = intersect_bounce(c, o, d, cnt);
+    }
+    ids_out[2 * i] = h.is_hit ? h.inst : kInvalid;
+    ids_out[2 * i + 1] = h.is_hit ? h.prim : kInvalid;
+    store3(&pos_out[3 * i], h.P);
+    store3(&nrm_out[3 * i], h.n);
+    dist_out[i] = h.distance;
+    store3(&wo_out[3 * i], h.wo);
+    rays += cnt.bounce; nodes += cnt.nodes; leaves += cnt.leaves; insts += cnt.instances;
+  }
+  if (stats) {
+    stats->rays = rays; stats->nodes = nodes; stats->leaves = leaves; stats->instances = insts;
+  }
+}
+
+void oc_sincos(float x, float *s, float *c) { sincos_ct(x, *s, *c); }
+
+void oc_random_cosine_direction(uint32_t *seed, const float n[3], float out[3]) {
+  store3(out, random_cosine_direction(*seed, load3(n)));
+}
+
 void oc_trace_any(oc_scene *scene, uint32_t n, const float *origins, const float *directions,
                   const float *distances, const uint32_t *targets, int brute,
                   uint8_t *occluded_out, ct_trace_stats *stats) {
@@ -1480,6 +2322,9 @@ oc_frame *oc_frame_create(oc_scene *scene, uint32_t width, uint32_t height) {
   f->img[0].assign(4 * np, 0.0f);
   f->img[1].assign(4 * np, 0.0f);
   f->taa.assign(4 * np, 0.0f);
+  f->pres_o.assign(np, ct_path_reservoir{});
+  f->pres_t.assign(np, ct_path_reservoir{});
+  f->indirect.assign(3 * np, 0.0f);
   f->cur = 1;
   return f;
 }
@@ -1496,6 +2341,9 @@ int oc_frame_render(oc_frame *frame, const ct_frame_params *params, int threads,
     return CT_E_INVALID;
   if ((p.is_active & CT_TAA_BIT) && !p.has_previous) return CT_E_INVALID;
   if (row_end > p.height || row_begin > row_end) return CT_E_INVALID;
+  if (p.restir.indirect && (p.restir.max_depth == 0 || p.restir.max_depth > CT_MAX_PT_DEPTH ||
+                            p.restir.pt_neighbor_count > CT_MAX_PT_NEIGHBORS))
+    return CT_E_INVALID;
   Scene &sc = *frame->sc;
   Ctx c;
   c.sc = &sc;
@@ -1520,29 +2368,33 @@ int oc_frame_render(oc_frame *frame, const ct_frame_params *params, int threads,
   const int64_t rows = (int64_t)(row_end - row_begin);
   const int64_t np = rows * (int64_t)W;
   uint64_t nodes = 0, leaves = 0, insts = 0, cnodes = 0, cleaves = 0, cinsts = 0;
-  uint64_t prim_rays = 0, sh_prepass = 0, sh_shade = 0;
-#pragma omp parallel num_threads(nt) reduction(+ : nodes, leaves, insts, cnodes, cleaves, cinsts, prim_rays, sh_prepass)
+  uint64_t prim_rays = 0, sh_prepass = 0, sh_shade = 0, pt_nodes = 0, pt_leaves = 0, pt_insts = 0;
+  uint64_t bounce = 0, pt_shadow = 0;
+#pragma omp parallel num_threads(nt) reduction(+ : nodes, leaves, insts, cnodes, cleaves, cinsts, prim_rays, sh_prepass, bounce, pt_shadow, pt_nodes, pt_leaves, pt_insts)
   {
 #pragma omp for schedule(dynamic, 64)
     for (int64_t k = 0; k < np; ++k) {
       uint32_t y = row_begin + (uint32_t)(k / W), x = (uint32_t)(k % W);
-      TraceCount cnt, pc;
-      prepass_pixel(c, x, y, cnt, pc);
+      TraceCount cnt, pc, ptc;
+      prepass_pixel(c, x, y, cnt, pc, ptc);
       prim_rays += 1;
       sh_prepass += cnt.rays;
       nodes += cnt.nodes; leaves += cnt.leaves; insts += cnt.instances;
       cnodes += pc.nodes; cleaves += pc.leaves; cinsts += pc.instances;
+      bounce += ptc.bounce; pt_shadow += ptc.rays;
+      pt_nodes += ptc.nodes; pt_leaves += ptc.leaves; pt_insts += ptc.instances;
     }
   }
-#pragma omp parallel num_threads(nt) reduction(+ : nodes, leaves, insts, sh_shade)
+#pragma omp parallel num_threads(nt) reduction(+ : nodes, leaves, insts, sh_shade, pt_shadow, pt_nodes, pt_leaves, pt_insts)
   {
 #pragma omp for schedule(dynamic, 64)
     for (int64_t k = 0; k < np; ++k) {
       uint32_t y = row_begin + (uint32_t)(k / W), x = (uint32_t)(k % W);
-      TraceCount cnt;
-      shade_pixel(c, x, y, cnt);
+      TraceCount cnt, ptc;
+      shade_pixel(c, x, y, cnt, ptc);
       sh_shade += cnt.rays;
       nodes += cnt.nodes; leaves += cnt.leaves; insts += cnt.instances;
+      pt_shadow += ptc.rays; pt_nodes += ptc.nodes; pt_leaves += ptc.leaves; pt_insts += ptc.instances;
     }
   }
   if (p.is_active & CT_TAA_BIT) {
@@ -1565,6 +2417,12 @@ int oc_frame_render(oc_frame *frame, const ct_frame_params *params, int threads,
   s.closest_nodes = cnodes;
   s.closest_leaves = cleaves;
   s.closest_instances = cinsts;
+  s.bounce_rays = bounce;
+  s.shadow_rays_pt = pt_shadow;
+  s.shadow_rays += pt_shadow;
+  s.pt_nodes = pt_nodes;
+  s.pt_leaves = pt_leaves;
+  s.pt_instances = pt_insts;
   return CT_OK;
 }
 
@@ -1582,6 +2440,9 @@ int oc_frame_read(oc_frame *frame, uint32_t which, void *dst, uint64_t bytes) {
     case CT_BUF_INT_RESERVOIR: src = frame->res_int.data(); sz = sizeof(ct_reservoir) * np; break;
     case CT_BUF_PREV_RESERVOIR: src = frame->res_prev.data(); sz = sizeof(ct_reservoir) * np; break;
     case CT_BUF_VELOCITY: src = frame->velocity.data(); sz = 8 * np; break;
+    case CT_BUF_PATH_RESERVOIR: src = frame->pres_o.data(); sz = sizeof(ct_path_reservoir) * np; break;
+    case CT_BUF_TEMPORAL_PATH_RESERVOIR: src = frame->pres_t.data(); sz = sizeof(ct_path_reservoir) * np; break;
+    case CT_BUF_INDIRECT: src = frame->indirect.data(); sz = 12 * np; break;
     default: return CT_E_INVALID;
   }
   if (bytes < sz) return CT_E_INVALID;

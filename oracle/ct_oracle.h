@@ -73,6 +73,16 @@ void oc_trace_any(oc_scene *scene, uint32_t n, const float *origins,
                   const uint32_t *targets, int brute, uint8_t *occluded_out,
                   ct_trace_stats *stats);
 
+/* intersect() (bounce.glsl:9-130), same outputs as ct_trace_bounce;
+ * brute != 0 tests every voxel instead of traversing. */
+void oc_trace_bounce(oc_scene *scene, uint32_t n, const float *origins,
+                     const float *directions, int brute, uint32_t *ids_out,
+                     float *pos_out, float *nrm_out, float *dist_out,
+                     float *wo_out, ct_trace_stats *stats);
+/* The build's sin/cos (Cephes polynomials) and random_cosine_direction. */
+void oc_sincos(float x, float *s, float *c);
+void oc_random_cosine_direction(uint32_t *seed, const float n[3], float out[3]);
+
 typedef struct oc_frame oc_frame;
 oc_frame *oc_frame_create(oc_scene *scene, uint32_t width, uint32_t height);
 void oc_frame_destroy(oc_frame *frame);

@@ -19,7 +19,7 @@ _ROOT = os.path.dirname(_HERE)
 sys.path.insert(0, os.path.join(_ROOT, "cube-tracing_amd"))
 import ctamd  # noqa: E402  (struct layouts only)
 from ctamd import (Aabb, Instance, Primitive, Material, Light, LightConfig, FrameParams,  # noqa: E402
-                   FrameStats, TraceStats, BvhArrays, Reservoir, DiInfo)
+                   FrameStats, TraceStats, BvhArrays, Reservoir, DiInfo, PathReservoir)
 
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
@@ -57,6 +57,10 @@ def lib():
                                   P(TraceStats)], None),
             "oc_trace_any": ([C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                               C.c_void_p, P(TraceStats)], None),
+            "oc_trace_bounce": ([C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                 C.c_void_p, C.c_void_p, C.c_void_p, P(TraceStats)], None),
+            "oc_sincos": ([f32, P(f32), P(f32)], None),
+            "oc_random_cosine_direction": ([P(u32), P(f32), P(f32)], None),
             "oc_frame_create": ([C.c_void_p, u32, u32], C.c_void_p),
             "oc_frame_destroy": ([C.c_void_p], None),
             "oc_frame_render": ([C.c_void_p, P(FrameParams), C.c_int, u32, u32], C.c_int),
@@ -121,6 +125,19 @@ def sky_color(time, afternoon, d):
     o = (f32 * 3)()
     lib().oc_sky_color(time, int(afternoon), _f3(d), o)
     return np.array(o[:], np.float32)
+
+
+def sincos(x):
+    sv, cv = f32(), f32()
+    lib().oc_sincos(float(x), C.byref(sv), C.byref(cv))
+    return sv.value, cv.value
+
+
+def random_cosine_direction(state, n):
+    s = u32(state)
+    o = (f32 * 3)()
+    lib().oc_random_cosine_direction(C.byref(s), _f3(n), o)
+    return np.array(o[:], np.float32), s.value
 
 
 def morton3(x, y, z):
@@ -225,6 +242,23 @@ class OScene:
                            None if tg is None else tg.ctypes.data, int(brute), occ.ctypes.data, C.byref(st))
         return (occ, st) if stats else occ
 
+    def trace_bounce(self, origins, directions, brute=False, stats=False):
+        o = np.ascontiguousarray(origins, np.float32)
+        d = np.ascontiguousarray(directions, np.float32)
+        n = o.shape[0]
+        ids = np.zeros((n, 2), np.uint32)
+        pos = np.zeros((n, 3), np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        dist = np.zeros(n, np.float32)
+        wo = np.zeros((n, 3), np.float32)
+        st = TraceStats()
+        lib().oc_trace_bounce(self.handle, n, o.ctypes.data, d.ctypes.data, int(brute), ids.ctypes.data,
+                              pos.ctypes.data, nrm.ctypes.data, dist.ctypes.data, wo.ctypes.data, C.byref(st))
+        out = {"ids": ids, "pos": pos, "nrm": nrm, "dist": dist, "wo": wo}
+        if stats:
+            out["stats"] = st
+        return out
+
     def close(self):
         if getattr(self, "handle", None):
             lib().oc_scene_destroy(self.handle)
@@ -257,6 +291,10 @@ class OFrame:
             out = np.zeros((n, 2), np.float32)
         elif which in (ctamd.CT_BUF_DI, ctamd.CT_BUF_PREV_DI):
             out = (DiInfo * n)()
+        elif which in (ctamd.CT_BUF_PATH_RESERVOIR, ctamd.CT_BUF_TEMPORAL_PATH_RESERVOIR):
+            out = (PathReservoir * n)()
+        elif which == ctamd.CT_BUF_INDIRECT:
+            out = np.zeros((n, 3), np.float32)
         else:
             out = (Reservoir * n)()
         nbytes = out.nbytes if isinstance(out, np.ndarray) else C.sizeof(out)

@@ -5,9 +5,12 @@
 rng_kat.npz      TEA-16 and LCG/rnd known-answer vectors (random.glsl:26-77)
 lbvh_300.npz     LBVH arrays (sorted Morton keys/ids, Karras children,
                  refit boxes, parents) for 300 seeded voxel centres
-frames_32.npz    32x32 ReSTIR DI images, DI and reservoirs of frames 0-3 of
-                 the small workload, and of its reference-constant variant
-                 with TAA
+frames_32.npz    32x32 images, DI and reservoirs of frames 0-3 of the small
+                 workload (ReSTIR DI + one diffuse bounce), and of its
+                 reference-constant variant with TAA (DI only)
+pt_32.npz        32x32 frames 0-2 of the chunked 96^3 workload with ReSTIR PT
+                 (max_depth 3): images, indirect colour, output and temporal
+                 path reservoirs
 """
 import os
 import sys
@@ -60,6 +63,25 @@ def frames(w):
     return np.stack(img), np.stack(di), np.stack(res)
 
 
+def pt_frames(w):
+    sc = ctamd.Scene(ctamd.scene_desc(w))
+    osc = O.OScene.from_scene(sc)
+    fr = O.OFrame(osc, w.width, w.height)
+    out = {"img": [], "ind": [], "pres": [], "tres": []}
+    for f in range(w.frames):
+        fr.render(ctamd.frame_params(w, f), threads=1)
+        out["img"].append(fr.read(ctamd.CT_BUF_IMAGE))
+        out["ind"].append(fr.read(ctamd.CT_BUF_INDIRECT))
+        out["pres"].append(np.frombuffer(bytes(fr.read(ctamd.CT_BUF_PATH_RESERVOIR)), np.uint32).reshape(-1, 23))
+        out["tres"].append(np.frombuffer(bytes(fr.read(ctamd.CT_BUF_TEMPORAL_PATH_RESERVOIR)),
+                                         np.uint32).reshape(-1, 23))
+    return {k: np.stack(v) for k, v in out.items()}
+
+
+def pt_workload():
+    return ctamd.hd_di(width=32, height=32, grid=96, chunk=32, frames=3, max_depth=3)
+
+
 def main():
     np.savez_compressed(os.path.join(HERE, "rng_kat.npz"), **rng_kat())
     np.savez_compressed(os.path.join(HERE, "lbvh_300.npz"), **lbvh_300())
@@ -67,6 +89,7 @@ def main():
     b = frames(ctamd.small_plain(reference_mode=True, width=32, height=32, frames=4, taa=True))
     np.savez_compressed(os.path.join(HERE, "frames_32.npz"), img=a[0], di=a[1], res=a[2],
                         ref_img=b[0], ref_di=b[1], ref_res=b[2])
+    np.savez_compressed(os.path.join(HERE, "pt_32.npz"), **pt_frames(pt_workload()))
     print("golden fixtures written to", HERE)
 
 

@@ -54,3 +54,7 @@ def di_array(arr):
 
 def res_array(arr):
     return np.frombuffer(bytes(arr), np.uint32).reshape(len(arr), 9)
+
+
+def pres_array(arr):
+    return np.frombuffer(bytes(arr), np.uint32).reshape(len(arr), 23)

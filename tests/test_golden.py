@@ -66,3 +66,36 @@ def test_frames_fixture_gpu(gpu, variant):
         di = np.frombuffer(bytes(ctx.read(ctamd.CT_BUF_DI, 32, 32)), np.uint32).reshape(-1, 15)
         assert np.array_equal(img.view(np.uint32), k[variant + "img"][f].view(np.uint32)), f
         assert np.array_equal(di, k[variant + "di"][f]), f
+
+
+def test_pt_fixture_oracle():
+    import sys
+    sys.path.insert(0, os.path.dirname(G))
+    from golden.make_golden import pt_frames, pt_workload
+    k = load("pt_32.npz")
+    out = pt_frames(pt_workload())
+    for name in ("img", "ind"):
+        assert np.array_equal(out[name].view(np.uint32), k[name].view(np.uint32)), name
+    for name in ("pres", "tres"):
+        assert np.array_equal(out[name], k[name]), name
+
+
+@pytest.mark.gpu
+def test_pt_fixture_gpu(gpu):
+    import sys
+    sys.path.insert(0, os.path.dirname(G))
+    from golden.make_golden import pt_workload
+    from helpers import gpu_scene
+    k = load("pt_32.npz")
+    w = pt_workload()
+    sc, acc, ctx = gpu_scene(w)
+    for f in range(w.frames):
+        ctx.render(acc, ctamd.frame_params(w, f))
+        img = ctx.read(ctamd.CT_BUF_IMAGE, 32, 32)
+        ind = ctx.read(ctamd.CT_BUF_INDIRECT, 32, 32)
+        pres = np.frombuffer(bytes(ctx.read(ctamd.CT_BUF_PATH_RESERVOIR, 32, 32)), np.uint32).reshape(-1, 23)
+        tres = np.frombuffer(bytes(ctx.read(ctamd.CT_BUF_TEMPORAL_PATH_RESERVOIR, 32, 32)), np.uint32).reshape(-1, 23)
+        assert np.array_equal(img.view(np.uint32), k["img"][f].view(np.uint32)), f
+        assert np.array_equal(ind.view(np.uint32), k["ind"][f].view(np.uint32)), f
+        assert np.array_equal(pres, k["pres"][f]), f
+        assert np.array_equal(tres, k["tres"][f]), f

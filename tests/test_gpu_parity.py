@@ -5,7 +5,7 @@ import pytest
 
 import ctamd
 import oracle as O
-from helpers import bits, gpu_scene, oracle_scene, random_rays, di_array, res_array
+from helpers import bits, gpu_scene, oracle_scene, random_rays, di_array, res_array, pres_array
 
 pytestmark = pytest.mark.gpu
 
@@ -94,10 +94,20 @@ def _frames_match(w, frames):
             ga, oa = res_array(ctx.read(which, w.width, w.height)), res_array(ofr.read(which))
             hit = di_array(ofr.read(ctamd.CT_BUF_DI))[:, 3].view(np.float32) > 0
             assert np.array_equal(ga[hit], oa[hit]), (f, which)
+        if w.restir.indirect:
+            for which in (ctamd.CT_BUF_PATH_RESERVOIR, ctamd.CT_BUF_TEMPORAL_PATH_RESERVOIR):
+                gp = pres_array(ctx.read(which, w.width, w.height))
+                op = pres_array(ofr.read(which))
+                bad = np.nonzero((gp != op).any(1))[0]
+                assert bad.size == 0, (f, which, bad[:8], gp[bad[:2]], op[bad[:2]])
+            gind, oind = ctx.read(ctamd.CT_BUF_INDIRECT, w.width, w.height), ofr.read(ctamd.CT_BUF_INDIRECT)
+            assert np.array_equal(bits(gind), bits(oind)), f"frame {f}: indirect colour differs"
         assert np.array_equal(bits(gi), bits(oi)), f"frame {f}: radiance differs"
         gs, os_ = ctx.stats(), ofr.stats()
         assert gs.primary_rays == os_.primary_rays
         assert gs.shadow_rays == os_.shadow_rays, (f, gs.shadow_rays, os_.shadow_rays)
+        assert gs.bounce_rays == os_.bounce_rays, (f, gs.bounce_rays, os_.bounce_rays)
+        assert gs.shadow_rays_pt == os_.shadow_rays_pt, (f, gs.shadow_rays_pt, os_.shadow_rays_pt)
     return sc
 
 
@@ -111,3 +121,43 @@ def test_frame_parity_reference_mode_taa(gpu):
 
 def test_frame_parity_chunked_orbit(gpu):
     _frames_match(ctamd.hd_di(width=96, height=64, grid=96, chunk=32, frames=3), 3)
+
+
+@pytest.mark.parametrize("grid,chunk", [(64, 64), (96, 32)])
+def test_trace_bounce_bit_exact(gpu, grid, chunk):
+    """intersect() path segments: ids, hit record and node counts."""
+    w = ctamd.hd_di(width=32, height=32, grid=grid, chunk=chunk)
+    sc, acc, ctx = gpu_scene(w)
+    osc = oracle_scene(sc, acc)
+    rng = np.random.default_rng(13)
+    o, d = random_rays(rng, 20000, sc.grid_min, sc.grid_max, inside_frac=0.5)
+    g = acc.trace_bounce(o, d, stats=True)
+    r = osc.trace_bounce(o, d, stats=True)
+    assert np.array_equal(g["ids"], r["ids"]), "hit ids differ"
+    for k in ("pos", "nrm", "dist", "wo"):
+        assert np.array_equal(bits(g[k]), bits(r[k])), k
+    gs, rs = g["stats"], r["stats"]
+    assert (gs.nodes, gs.leaves, gs.instances) == (rs.nodes, rs.leaves, rs.instances)
+    b = osc.trace_bounce(o[:400], d[:400], brute=True)
+    assert np.array_equal(b["ids"], g["ids"][:400])
+    assert np.array_equal(bits(b["pos"]), bits(g["pos"][:400]))
+    assert 0.2 < (g["ids"][:, 0] != ctamd.CT_INVALID_INDEX).mean() < 1.0
+
+
+def _pt_workload(max_depth=3, radius=1, shift=ctamd.CT_SHIFT_RECONNECTION, rr=0, temporal=1, spatial=1):
+    w = ctamd.hd_di(width=48, height=40, grid=96, chunk=32, frames=3, max_depth=max_depth)
+    w.restir.pt_neighbor_radius = radius
+    w.restir.pt_shift_mapping = shift
+    w.restir.pt_russian_roulette = rr
+    w.restir.pt_temporal = temporal
+    w.restir.pt_spatial = spatial
+    return w
+
+
+@pytest.mark.parametrize("kw", [dict(max_depth=1), dict(max_depth=2), dict(max_depth=4, radius=3),
+                                dict(shift=ctamd.CT_SHIFT_HYBRID), dict(rr=1), dict(temporal=0),
+                                dict(spatial=0)],
+                         ids=["depth1", "depth2", "depth4_radius3", "hybrid", "russian_roulette", "no_temporal",
+                              "no_spatial"])
+def test_frame_parity_pt_variants(gpu, kw):
+    _frames_match(_pt_workload(**kw), 3)
