@@ -1,12 +1,14 @@
 #!/usr/bin/env python3
-"""Benchmark: ReSTIR DI frames on one MI355X.
+"""Benchmark: ReSTIR DI + 3-bounce path frames on one MI355X.
 
-Workload (default): 1920x1080 ReSTIR DI on a 256^3 voxel grid split into 64
-BLAS chunks of 64^3 under one TLAS (4,194,304 voxels at 25 % fill, 83,886
-emissive voxels), camera orbiting 1 degree/frame, TAA at 0.1 blend.  A step
-is one frame: primary rays, RIS + visibility, temporal reuse, spatial reuse
-with pairwise MIS, shading and TAA.  The scene is resident in HBM before the
-timed region.
+Workload (default): 1920x1080 ReSTIR DI plus a 3-bounce diffuse path
+(ReSTIR PT, reconnection shift, max_depth 3) on a 256^3 voxel grid split
+into 64 BLAS chunks of 64^3 under one TLAS (4,194,304 voxels at 25 % fill,
+83,886 emissive voxels), camera orbiting 1 degree/frame, TAA at 0.1 blend.
+A step is one frame: primary rays, RIS + visibility, temporal reuse, path
+segments and temporal path reuse, spatial reuse with pairwise MIS, spatial
+path reuse, shading and TAA.  The scene is resident in HBM before the timed
+region.
 
 Prints one JSON line: Mrays/s (traversal queries per second), frame ms, the
 roofline of the any-hit traversal kernel and the CPU oracle timed on the
@@ -57,7 +59,7 @@ def cpu_baseline(w, sc, acc, budget_s):
         fr.render(p, threads=cores, rows=(r0, r1))
         dt = time.perf_counter() - t1
         st = fr.stats()
-        total_rays += st.primary_rays + st.shadow_rays
+        total_rays += st.primary_rays + st.shadow_rays + st.bounce_rays
         total_t += dt
         sampled_rows += r1 - r0
         frames += 1
@@ -78,6 +80,7 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--depth", type=int, default=3, help="path bounces (ReSTIR PT max_depth); 0 = DI only")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -100,7 +103,8 @@ def main():
     if not ctamd.device_available():
         raise SystemExit("bench: no gfx950 device visible")
     K, Wm = args.steps, args.warmup
-    w = ctamd.hd_di(frames=Wm + K + 1, width=args.width, height=args.height, grid=args.grid)
+    w = ctamd.hd_di(frames=Wm + K + 1, width=args.width, height=args.height, grid=args.grid,
+                    indirect=args.depth > 0, max_depth=max(args.depth, 1))
     t0 = time.time()
     sc = ctamd.Scene(ctamd.scene_desc(w))
     log(f"scene: {sc.info.instance_count} instances, {sc.info.primitive_count} voxels, "
@@ -122,12 +126,18 @@ def main():
     ctx.render(acc, ctamd.frame_params(w, f))
     f += 1
     ist = ctx.stats()
-    bytes_any = 36 * ist.shadow_rays + 64 * ist.nodes + 28 * ist.leaves + 64 * ist.instances
-    bytes_per_shadow_ray = bytes_any / max(ist.shadow_rays, 1)
-    log(f"instrumented frame: {ist.shadow_rays} shadow rays ({ist.shadow_rays_ris} RIS, "
-        f"{ist.shadow_rays_temporal} temporal, {ist.shadow_rays_spatial} spatial), {ist.nodes} nodes, "
-        f"{ist.leaves} leaves, {ist.instances} instance entries; {bytes_per_shadow_ray:.0f} algorithmic B/ray, "
-        f"{bytes_any / 1e6:.1f} MB per frame; closest: {ist.closest_nodes} nodes, {ist.closest_leaves} leaves")
+    # Algorithmic bytes per query: a shadow ray reads its
+    # 36 B record, a path segment reads 32 B and writes a 16 B hit; every BVH2
+    # node fetched is 64 B, every leaf 28 B (centre + id + instance record
+    # share), every instance entry 64 B.
+    n_any = ist.shadow_rays_ris + ist.shadow_rays_temporal + ist.shadow_rays_spatial
+    bytes_any = 36 * n_any + 64 * ist.nodes + 28 * ist.leaves + 64 * ist.instances
+    bytes_seg = 48 * ist.bounce_rays + 64 * ist.pt_nodes + 28 * ist.pt_leaves + 64 * ist.pt_instances
+    b_any = bytes_any / max(n_any, 1)
+    b_seg = bytes_seg / max(ist.bounce_rays, 1)
+    log(f"instrumented frame: {n_any} shadow rays ({ist.shadow_rays_ris} RIS, {ist.shadow_rays_temporal} temporal, "
+        f"{ist.shadow_rays_spatial} spatial) {b_any:.0f} B/ray; {ist.bounce_rays} path segments {b_seg:.0f} B/segment; "
+        f"{ist.shadow_rays_pt} path-reuse visibility rays; closest: {ist.closest_nodes} nodes")
     # Timed region: K frames, events per pass.
     ctx.set_instrumentation(1)
     import ctypes
@@ -135,39 +145,56 @@ def main():
     hip.hipDeviceSynchronize()
     t_start = time.perf_counter()
     rays = 0
-    ms_any = 0.0
-    ms_frames = []
+    ms_any = ms_seg = 0.0
+    n_any_t = n_seg_t = 0
+    ms_frames, ms_pt = [], []
     for _ in range(K):
         ctx.render(acc, ctamd.frame_params(w, f))
         f += 1
         st = ctx.stats()  # drains the stream, reads the frame's counters and events
-        rays += st.primary_rays + st.shadow_rays
+        rays += st.primary_rays + st.shadow_rays + st.bounce_rays
         ms_any += st.ms_trace_any
+        ms_seg += st.ms_pt_segments
+        n_any_t += st.shadow_rays_ris + st.shadow_rays_temporal + st.shadow_rays_spatial
+        n_seg_t += st.bounce_rays
         ms_frames.append(st.ms_total)
+        ms_pt.append(st.ms_pt)
     hip.hipDeviceSynchronize()
     wall = time.perf_counter() - t_start
     ms_per_step = 1000.0 * wall / K
     mrays = rays / wall / 1e6
-    shadow_per_frame = (rays / K) - w.width * w.height
-    achieved = bytes_per_shadow_ray * shadow_per_frame * K / (ms_any / 1000.0) / 1e9
+    kern = {
+        "k_trace_any": {"desc": "k_trace_any (shadow-ray any-hit traversal)", "ms": ms_any,
+                        "achieved": b_any * n_any_t / (ms_any / 1000.0) / 1e9 if ms_any > 0 else 0.0},
+        "k_trace_segments": {"desc": "k_trace_segments (path-segment closest-hit traversal)", "ms": ms_seg,
+                             "achieved": b_seg * n_seg_t / (ms_seg / 1000.0) / 1e9 if ms_seg > 0 else 0.0},
+    }
+    order = sorted(kern, key=lambda k: -kern[k]["ms"])
+
+    def roof(k):
+        v = kern[k]
+        return {"bound": "hbm", "achieved": round(v["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(v["achieved"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": v["desc"],
+                "ms_per_frame": round(v["ms"] / K, 3)}
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, sc, acc, args.cpu_budget)
     ms_sorted = sorted(ms_frames)
+    depth = w.restir.max_depth if w.restir.indirect else 0
     out = {
         "metric": "Mrays/s + 1080p frame ms (ReSTIR DI/PT), 1 MI355X; batched/adjoint vs plain",
         "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": K, "warmup": Wm,
         "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"restir_di_{w.width}x{w.height}_grid{args.grid}_64blas_taa",
+        "config": {"workload": f"restir_di_pt{depth}_{w.width}x{w.height}_grid{args.grid}_64blas_taa",
                    "width": w.width, "height": w.height, "grid": args.grid, "blas": sc.info.instance_count,
                    "voxels": sc.info.primitive_count, "lights": sc.info.cube_light_count,
                    "frame_ms_gpu_events_median": round(ms_sorted[len(ms_sorted) // 2], 3),
-                   "rays_per_frame": rays // K, "indirect_bounces": 0},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_trace_any (shadow-ray any-hit traversal)",
-                     "ms_per_frame": round(ms_any / K, 3)},
+                   "pt_ms_gpu_events_median": round(sorted(ms_pt)[len(ms_pt) // 2], 3),
+                   "rays_per_frame": rays // K, "indirect_bounces": depth},
+        "roofline": roof(order[0]),
+        "roofline_next_kernel": roof(order[1]),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -934,6 +934,17 @@ int ct_accel_get_world_to_object(ct_accel *a, uint32_t id, float out[16]) {
   return CT_OK;
 }
 
+int ct_accel_read_instances(ct_accel *a, ct_instance *out, uint32_t capacity, uint32_t *count) {
+  if (!a || !count || (capacity && !out)) return fail(CT_E_INVALID, "read_instances: null argument");
+  const uint32_t n = (uint32_t)a->instances.size();
+  *count = n;
+  for (uint32_t i = 0; i < n && i < capacity; ++i) {
+    out[i] = a->instances[i];
+    if (i >= a->instance_used.size() || !a->instance_used[i]) out[i].primitive_count = 0;
+  }
+  return CT_OK;
+}
+
 int ct_accel_read_bvh(ct_accel *a, uint32_t id, ct_bvh_arrays *arr) {
   if (!a || !arr) return fail(CT_E_INVALID, "read_bvh: null argument");
   CT_HIP(hipSetDevice(a->device));

@@ -209,6 +209,29 @@ CT_D void sincos_ct(float x, float &s, float &c) {
   }
 }
 
+constexpr float kPi = 3.1415926535897932384626433832795f;  // DAXA_PI, shared.inl:49
+
+CT_D bool normal_near_zero(F3 v) {  // prng.glsl:17-21
+  const float s = 1e-8f;
+  return fabsf(v.x) < s && fabsf(v.y) < s && fabsf(v.z) < s;
+}
+// random_cosine_direction, prng.glsl:84-108
+CT_D F3 random_cosine_direction(uint32_t &seed, F3 n) {
+  const float r1 = rnd(seed);
+  const float r2 = rnd(seed);
+  const float z = sqrtf(1.0f - r2);
+  const float phi = 2.0f * kPi * r1;
+  float sp, cp;
+  sincos_ct(phi, sp, cp);
+  const float x = cp * sqrtf(r2);
+  const float y = sp * sqrtf(r2);
+  F3 u = fabsf(n.x) > 0.1f ? f3(0.0f, 1.0f, 0.0f) : f3(1.0f, 0.0f, 0.0f);
+  u = normalize(cross(n, u));
+  F3 v = cross(n, u);
+  return add(add(muls(u, x), muls(v, y)), muls(n, z));
+}
+
+
 // ------------------------------------------------------------ traversal
 constexpr uint32_t kMaxTraversalSteps = 1u << 22;
 

@@ -172,6 +172,7 @@ struct FrameBufs {
   DevBuf mask, gather;                                // per-pixel pass state
   DevBuf ray_o, ray_d, ray_t, queue, counters, counters_host_staging;
   DevBuf pres[2], indirect;  // ReSTIR PT: output / temporal path reservoirs, indirect colour
+  DevBuf pstate, ptq[2];     // ReSTIR PT wavefront: path state, segment queues
   bool pt_ready = false;     // allocated (and zeroed) on the first frame with indirect light
 };
 
@@ -186,7 +187,9 @@ struct Ctx {
   ct_light_config cfg{};
   int cur = 1;
   uint32_t instrument = 0;
-  hipEvent_t ev[16] = {};
+  hipEvent_t ev[48] = {};
+  int ev_seg[2 * 16] = {};  // event pairs around each path-segment traversal
+  int n_seg = 0;
   int n_events = 0;
   ct_frame_stats stats{};
   uint64_t *counters_pinned = nullptr;

@@ -41,7 +41,7 @@ enum {
 
 // counters[] slots
 constexpr int kCntQueue = 0, kCntHead = 1, kCntRis = 2, kCntTemporal = 3, kCntSpatial = 4, kCntPtBounce = 5,
-              kCntPtShadow = 6;
+              kCntPtShadow = 6, kCntPtQueue = 8 /* and 9 */, kCntPtHead = 10;
 
 struct DFrame {
   uint32_t W, H, npix, frame, active, afternoon;
@@ -79,6 +79,8 @@ struct DFrame {
   float4 *pres_o;    // [6 per pixel] output path reservoirs (this frame)
   float4 *pres_t;    // [6 per pixel] temporal path reservoirs (history)
   float4 *indirect;  // indirect colour of the first pass
+  float4 *pst;       // path state between the wavefront kernels (14 float4 per pixel)
+  uint32_t *ptq[2];  // path-segment queues (pixel ids) of even / odd depths
 };
 
 struct PLight {
@@ -906,11 +908,116 @@ CT_D void wave_add_cnt(const TraceCnt &c, unsigned long long *dst) {
   }
 }
 
-// Indirect illumination of the first pass (rgen.glsl:271-315): initial
-// paths (trace_restir_path_tracing) and temporal path reuse
-// (indirect_illumination.glsl:454-475), after the DI temporal reuse.
+// Appends this thread's item (when `active`) to a queue with one atomic per
+// block; returns its position.  Every thread of the block calls it.
+CT_D uint32_t queue_push(bool active, uint32_t *count, uint32_t *also) {
+  __shared__ uint32_t s_wsum[kBlock / 64];
+  __shared__ uint32_t s_base;
+  const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+  const unsigned long long b = __ballot(active);
+  const uint32_t before = (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+  if (lane == 0) s_wsum[w] = (uint32_t)__popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int k = 0; k < kBlock / 64; ++k) {
+      const uint32_t t = s_wsum[k];
+      s_wsum[k] = tot;
+      tot += t;
+    }
+    s_base = tot ? atomicAdd(count, tot) : 0u;
+    if (tot && also) atomicAdd(also, tot);
+  }
+  __syncthreads();
+  return active ? s_base + s_wsum[w] + before : kInvalid;
+}
+
+// ReSTIR PT in wavefront form (rgen.glsl:271-315 for the first pass).
+// k_pt_begin starts every hit pixel's path (generate_path + primary
+// scatter) and queues its first segment; per depth, k_trace_segments finds
+// each queued segment's closest voxel and k_pt_step runs one iteration of
+// the path loop, queueing the next segment; k_pt_end finalizes the path
+// reservoir and runs temporal path reuse.  Counters: kCntPtQueue +
+// (depth & 1) holds the length of the queue of that depth.
+__global__ void __launch_bounds__(kBlock) k_pt_begin(DFrame F) {
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  bool more = false;
+  F3 ro, rd;
+  if (px < F.npix && F.dpos_c[px].w > 0.0f) {
+    const DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
+    PState p;
+    more = pt_begin(F, p, isect_from_di(F, di), di.seed);
+    st_pstate(F.pst, F.npix, px, p);
+    ro = compute_ray_origin(p.origin, p.dir);
+    rd = p.dir;
+  }
+  const uint32_t pos = queue_push(more, &F.counters[kCntPtQueue], &F.counters[kCntPtBounce]);
+  if (more) {
+    F.ptq[0][pos] = px;
+    F.ray_o[pos] = make_float4(ro.x, ro.y, ro.z, 0.0f);
+    F.ray_d[pos] = make_float4(rd.x, rd.y, rd.z, 0.0f);
+  }
+}
+
+// Closest voxel of each queued path segment (intersect() semantics), one
+// 64-ray fetch per wave.
 template <bool kCount>
-__global__ void __launch_bounds__(kBlock) k_pt(DFrame F, unsigned long long *stats) {
+__global__ void __launch_bounds__(kBlock) k_trace_segments(DScene S, const float4 *ray_o, const float4 *ray_d,
+                                                           const uint32_t *count, uint32_t *head, float4 *hit,
+                                                           unsigned long long *stats) {
+  __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
+  const uint32_t n = *count;
+  TraceCnt cnt;
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(head, 64u);
+    base = __shfl(base, 0);
+    if (base >= n) break;
+    const uint32_t q = base + lane;
+    if (q < n) {
+      const float4 o = ray_o[q], d = ray_d[q];
+      float bt = kMaxDistance;
+      uint32_t bi = kInvalid, bp = kInvalid;
+      bool ovf = false;
+      traverse_mode<kTraceBounce, kCount>(S, xyz(o), xyz(d), kMaxDistance, kInvalid, kInvalid, bt, bi, bp,
+                                          stk + tid, kBlock, cnt, ovf);
+      hit[q] = make_float4(bt, __uint_as_float(bi), __uint_as_float(bp), 0.0f);
+    }
+  }
+  if (kCount) wave_add_cnt(cnt, stats);
+}
+
+// One path-loop iteration for every path of queue `d & 1`.
+__global__ void __launch_bounds__(kBlock) k_pt_step(DFrame F, uint32_t d) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = F.counters[kCntPtQueue + (d & 1u)];
+  bool more = false;
+  uint32_t px = 0;
+  F3 ro, rd;
+  if (q < n) {
+    px = F.ptq[d & 1u][q];
+    PState p;
+    ld_pstate(F.pst, F.npix, px, p);
+    const float4 h = F.hit[q];
+    const F3 box_o = compute_ray_origin(p.origin, p.dir);
+    Isect i = segment_record(F, p.origin, box_o, p.dir, __float_as_uint(h.y), __float_as_uint(h.z));
+    more = pt_step(F, p, i);
+    st_pstate(F.pst, F.npix, px, p);
+    ro = compute_ray_origin(p.origin, p.dir);
+    rd = p.dir;
+  }
+  const uint32_t pos = queue_push(more, &F.counters[kCntPtQueue + ((d + 1u) & 1u)], &F.counters[kCntPtBounce]);
+  if (more) {
+    F.ptq[(d + 1u) & 1u][pos] = px;
+    F.ray_o[pos] = make_float4(ro.x, ro.y, ro.z, 0.0f);
+    F.ray_d[pos] = make_float4(rd.x, rd.y, rd.z, 0.0f);
+  }
+}
+
+// path_output and temporal path reuse (indirect_illumination.glsl:454-475).
+template <bool kCount>
+__global__ void __launch_bounds__(kBlock) k_pt_end(DFrame F, unsigned long long *stats) {
   __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
   const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
   PtTrace tr;
@@ -920,13 +1027,16 @@ __global__ void __launch_bounds__(kBlock) k_pt(DFrame F, unsigned long long *sta
   tr.overflow = false;
   if (px < F.npix && F.dpos_c[px].w > 0.0f) {
     const uint32_t x = px % F.W, y = px / F.W;
-    const DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
-    const Isect ci = isect_from_di(F, di);
-    uint32_t seed = di.seed;
+    PState p;
+    ld_pstate(F.pst, F.npix, px, p);
     F3 ind;
-    PRes out = pt_initial<kCount>(F, ci, seed, ind, tr);
+    pt_end(p, ind);
+    PRes out = p.res;
+    uint32_t seed = p.seed;
     if (F.rp.pt_temporal) {
       // temporal_path_get_reprojected, indirect_illumination.glsl:29-57
+      const DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
+      const Isect ci = isect_from_di(F, di);
       const float2 v = F.vel[px];
       const float r1 = rnd(seed);
       const float r2 = rnd(seed);
@@ -942,9 +1052,8 @@ __global__ void __launch_bounds__(kBlock) k_pt(DFrame F, unsigned long long *sta
     F.indirect[px] = make_float4(ind.x, ind.y, ind.z, 0.0f);
     F.dorg_c[px].w = __uint_as_float(seed);
   }
-  wave_add(tr.bounce_rays, &F.counters[kCntPtBounce]);
   wave_add(tr.shadow_rays, &F.counters[kCntPtShadow]);
-  if (kCount) wave_add_cnt(tr.cnt, stats + 6);
+  if (kCount) wave_add_cnt(tr.cnt, stats + 9);
 }
 
 // Spatial path reuse (indirect_illumination_spatial_reuse, rgen.glsl:427-452)
@@ -989,7 +1098,7 @@ __global__ void __launch_bounds__(kBlock) k_pt_spatial(DFrame F, unsigned long l
     F.dorg_p[px].w = __uint_as_float(seed);
   }
   wave_add(tr.shadow_rays, &F.counters[kCntPtShadow]);
-  if (kCount) wave_add_cnt(tr.cnt, stats + 6);
+  if (kCount) wave_add_cnt(tr.cnt, stats + 9);
 }
 
 // Path segments on caller rays (ct_trace_bounce).
@@ -1220,7 +1329,9 @@ int alloc_pt(Ctx &c, hipStream_t st) {
   if (b.pt_ready) return CT_OK;
   const size_t np = (size_t)c.max_w * c.max_h;
   int rc;
-  if ((rc = b.pres[0].reserve(96 * np)) || (rc = b.pres[1].reserve(96 * np)) || (rc = b.indirect.reserve(16 * np)))
+  if ((rc = b.pres[0].reserve(96 * np)) || (rc = b.pres[1].reserve(96 * np)) || (rc = b.indirect.reserve(16 * np)) ||
+      (rc = b.pstate.reserve(16 * kPathStateVec * np)) || (rc = b.ptq[0].reserve(4 * np)) ||
+      (rc = b.ptq[1].reserve(4 * np)))
     return rc;
   for (DevBuf *d : {&b.pres[0], &b.pres[1], &b.indirect}) CT_HIP(hipMemsetAsync(d->p, 0, d->bytes, st));
   b.pt_ready = true;
@@ -1277,7 +1388,7 @@ int ct_ctx_create(const ct_ctx_desc *d, ct_ctx **out) {
     delete c;
     return rc;
   }
-  for (int k = 0; k < 16; ++k) (void)hipEventCreate(&c->ev[k]);
+  for (int k = 0; k < 48; ++k) (void)hipEventCreate(&c->ev[k]);
   void *pin = nullptr;
   if (hipHostMalloc(&pin, 64 * sizeof(uint64_t), hipHostMallocDefault) == hipSuccess)
     c->counters_pinned = static_cast<uint64_t *>(pin);
@@ -1290,7 +1401,7 @@ int ct_ctx_destroy(ct_ctx *c) {
   if (!c) return fail(CT_E_INVALID, "ct_ctx_destroy: null handle");
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  for (int k = 0; k < 16; ++k)
+  for (int k = 0; k < 48; ++k)
     if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
   if (c->counters_pinned) (void)hipHostFree(c->counters_pinned);
   (void)hipStreamDestroy(c->stream);
@@ -1454,11 +1565,22 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
   F.pres_o = b.pres[0].as<float4>();
   F.pres_t = b.pres[1].as<float4>();
   F.indirect = b.indirect.as<float4>();
+  F.pst = b.pstate.as<float4>();
+  F.ptq[0] = b.ptq[0].as<uint32_t>();
+  F.ptq[1] = b.ptq[1].as<uint32_t>();
   unsigned long long *stats = (c->instrument & 2u) ? b.prev_res_copy.as<unsigned long long>() : nullptr;
   const bool ev = (c->instrument & 1u) != 0;
-  int ne = 0;
+  int ne = 0, nseg = kMarkCount;
   auto mark = [&]() -> int {
     if (ev) CT_HIP(hipEventRecord(c->ev[ne++], st));
+    return CT_OK;
+  };
+  c->n_seg = 0;
+  auto seg_mark = [&]() -> int {
+    if (ev && nseg < 48 && c->n_seg < 32) {
+      CT_HIP(hipEventRecord(c->ev[nseg], st));
+      c->ev_seg[c->n_seg++] = nseg++;
+    }
     return CT_OK;
   };
   const uint32_t np = F.npix;
@@ -1467,7 +1589,7 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
   ta.ray_o = F.ray_o; ta.ray_d = F.ray_d; ta.ray_t = F.ray_t; ta.queue = F.queue;
   ta.count = F.counters + kCntQueue; ta.head = F.counters + kCntHead; ta.mask = F.mask; ta.stats = stats;
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, 0, 8);
-  if (stats) CT_HIP(hipMemsetAsync(stats, 0, 9 * sizeof(unsigned long long), st));
+  if (stats) CT_HIP(hipMemsetAsync(stats, 0, 12 * sizeof(unsigned long long), st));
   if ((rc = mark())) return rc;  // kMarkStart
   const uint32_t tiles = ((F.W + 15) / 16) * ((F.H + 15) / 16);
   if (stats)
@@ -1489,10 +1611,27 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
   hipLaunchKernelGGL(k_temporal_finish, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
   if ((rc = mark())) return rc;  // kMarkTemporalFinish
   if (pt) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, kCntPtQueue, 3);
+    hipLaunchKernelGGL(k_pt_begin, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
+    const uint32_t grid = std::min<uint32_t>(nblocks(np), (uint32_t)trace_grid());
+    for (uint32_t d = 0; d < p->restir.max_depth; ++d) {
+      // the next queue and the fetch counter start at zero
+      hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, kCntPtQueue + (int)((d + 1) & 1u), 1);
+      hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, kCntPtHead, 1);
+      if ((rc = seg_mark())) return rc;
+      if (stats)
+        hipLaunchKernelGGL(k_trace_segments<true>, dim3(grid), dim3(kBlock), 0, st, F.S, F.ray_o, F.ray_d,
+                           F.counters + kCntPtQueue + (d & 1u), F.counters + kCntPtHead, F.hit, stats + 6);
+      else
+        hipLaunchKernelGGL(k_trace_segments<false>, dim3(grid), dim3(kBlock), 0, st, F.S, F.ray_o, F.ray_d,
+                           F.counters + kCntPtQueue + (d & 1u), F.counters + kCntPtHead, F.hit, stats);
+      if ((rc = seg_mark())) return rc;
+      hipLaunchKernelGGL(k_pt_step, dim3(nblocks(np)), dim3(kBlock), 0, st, F, d);
+    }
     if (stats)
-      hipLaunchKernelGGL(k_pt<true>, dim3(nblocks(np)), dim3(kBlock), 0, st, F, stats);
+      hipLaunchKernelGGL(k_pt_end<true>, dim3(nblocks(np)), dim3(kBlock), 0, st, F, stats);
     else
-      hipLaunchKernelGGL(k_pt<false>, dim3(nblocks(np)), dim3(kBlock), 0, st, F, stats);
+      hipLaunchKernelGGL(k_pt_end<false>, dim3(nblocks(np)), dim3(kBlock), 0, st, F, stats);
   }
   if ((rc = mark())) return rc;  // kMarkPt
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, kCntQueue, 1);
@@ -1520,7 +1659,7 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
   c->n_events = ne;
   // Counters for the statistics (copied once the frame is done).
   CT_HIP(hipMemcpyAsync(c->counters_pinned, F.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  if (stats) CT_HIP(hipMemcpyAsync(c->counters_pinned + 8, stats, 9 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  if (stats) CT_HIP(hipMemcpyAsync(c->counters_pinned + 8, stats, 12 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
   c->stats_pending = true;
   c->stats = ct_frame_stats{};
   c->stats.primary_rays = np;
@@ -1630,6 +1769,8 @@ int ct_frame_get_stats(ct_ctx *c, ct_frame_stats *out) {
                               ms(kMarkSpatialEmit, kMarkTraceSpatial);
       c->stats.ms_trace_closest = c->stats.ms_primary;
       c->stats.ms_total = ms(kMarkStart, kMarkTaa);
+      c->stats.ms_pt_segments = 0.0f;
+      for (int k = 0; k + 1 < c->n_seg; k += 2) c->stats.ms_pt_segments += ms(c->ev_seg[k], c->ev_seg[k + 1]);
     }
     c->stats_pending = false;
   }

@@ -15,7 +15,6 @@
 
 constexpr float kHlfMax = 6.5504e+4f;         // HLF_MAX, shared.inl:56
 constexpr float kHlfMin = 1.175494351e-38f;   // HLF_MIN, shared.inl:57
-constexpr float kPi = 3.1415926535897932384626433832795f;
 constexpr float kVoxelExtent = 0.03125f;      // VOXEL_EXTENT, shared.inl:45
 constexpr uint32_t kMaxMaterials = 10000u;    // MAX_MATERIALS, shared.inl:20
 constexpr uint32_t kGeomLightCube = 2u, kGeomLightAnalytic = 4u, kGeomLightEnv = 5u;  // shared.inl:336-341
@@ -318,26 +317,6 @@ CT_D F3 new_ray_origin(F3 pos, F3 n, bool view_side) {
 }
 
 // ---------------------------------------------------------------- sampling
-CT_D bool normal_near_zero(F3 v) {  // prng.glsl:17-21
-  const float s = 1e-8f;
-  return fabsf(v.x) < s && fabsf(v.y) < s && fabsf(v.z) < s;
-}
-// random_cosine_direction, prng.glsl:84-108
-CT_D F3 random_cosine_direction(uint32_t &seed, F3 n) {
-  const float r1 = rnd(seed);
-  const float r2 = rnd(seed);
-  const float z = sqrtf(1.0f - r2);
-  const float phi = 2.0f * kPi * r1;
-  float sp, cp;
-  sincos_ct(phi, sp, cp);
-  const float x = cp * sqrtf(r2);
-  const float y = sp * sqrtf(r2);
-  F3 u = fabsf(n.x) > 0.1f ? f3(0.0f, 1.0f, 0.0f) : f3(1.0f, 0.0f, 0.0f);
-  u = normalize(cross(n, u));
-  F3 v = cross(n, u);
-  return add(add(muls(u, x), muls(v, y)), muls(n, z));
-}
-
 // path_generate_scatter_ray, path_tracer.glsl:59-112, with the Lambertian
 // scatter callable (rcall_scatter.glsl:73-84).
 CT_D bool pt_scatter(PState &p, Isect &i) {
@@ -501,12 +480,14 @@ CT_D void pt_handle_miss(const DFrame &F, PState &p) {
   p.flags &= ~kPathActive;
 }
 
-// trace_restir_path_tracing, path_tracer.glsl:854-887, with generate_path
-// (path_state.glsl:208-249) and path_handle_primary_hit (:440-469).
-// Returns the finalized reservoir; `color` is path_output's colour.
-template <bool kCount>
-CT_D PRes pt_initial(const DFrame &F, Isect i, uint32_t &seed, F3 &color, PtTrace &tr) {
-  PState p;
+// trace_restir_path_tracing (path_tracer.glsl:854-887) split at its
+// path_next_vertex calls, so each path segment can be traced by the
+// wavefront closest-hit kernel: pt_begin is generate_path
+// (path_state.glsl:208-249) plus path_handle_primary_hit (:440-469);
+// pt_step is one loop iteration after the segment's hit is known; pt_end
+// is path_finalize + path_output (:817-835).  Returns whether the path
+// traces another segment.
+CT_D bool pt_begin(const DFrame &F, PState &p, Isect i, uint32_t seed) {
   p.max_depth = F.rp.max_depth;
   p.flags = kPathActive | kPathHit;
   p.path_length = 0u;
@@ -534,38 +515,122 @@ CT_D PRes pt_initial(const DFrame &F, Isect i, uint32_t &seed, F3 &color, PtTrac
   p.pb.cj = f3(0, 0, 0);
   p.res = pres_init(F.rp.max_depth);
   p.last_rough = true;
-  // path_handle_primary_hit
   p.origin = compute_ray_origin(compute_ray_origin(i.P, i.n), i.n);
   p.flags &= ~(kPathLightUpper | kPathLightLower);
   p.res.init_seed = p.seed;
-  bool valid = pt_scatter(p, i);
+  const bool valid = pt_scatter(p, i);
   if (p.path_length == p.pb.rc_len) p.pb.cached_seed = p.seed;
   p.pb.rc_wi = p.dir;
-  if (valid) {
-    do {
-      // path_next_vertex, path_tracer.glsl:136-149
-      i = bounce_intersect<kCount>(F, p.origin, p.dir, tr);
-      if (i.is_hit) {
-        p.hit.inst = i.inst;
-        p.hit.prim = i.prim;
-        p.hit.hit = i.P;
-        p.flags |= kPathHit;
-        p.path_length++;
-        p.scene_length += i.distance;
-        pt_handle_hit(F, p, i);
-      } else {
-        p.flags &= ~kPathHit;
-        p.scene_length = kHlfMax;
-        pt_handle_miss(F, p);
-      }
-    } while ((p.flags & kPathActive) != 0u && !tr.overflow);
+  return valid;
+}
+
+// The INTERSECT record intersect() (bounce.glsl:9-130) builds once the
+// segment from `origin` (boxes tested from `ro` = origin pushed along dir)
+// has its closest voxel (inst = kInvalid on a miss).
+CT_D Isect segment_record(const DFrame &F, F3 origin, F3 ro, F3 dir, uint32_t inst, uint32_t prim) {
+  Isect i = isect_empty();
+  i.mat_idx = 0u;
+  F3 int_hit = f3(0, 0, 0);
+  if (inst != kInvalid) {
+    float t;
+    F3 pos, nrm;
+    hit_from_ray(F, inst, prim, ro, dir, t, pos, nrm);
+    int_hit = pos;
+    i.is_hit = true;
+    i.P = pos;
+    i.n = nrm;
+    i.inst = inst;
+    i.prim = prim;
+    i.mat_idx = material_of(F, inst, prim);
+    i.mat = ld_mat(F, i.mat_idx);
+    i.distance = length(sub(origin, int_hit));
   }
+  i.wo = normalize(sub(origin, int_hit));
+  return i;
+}
+
+// One iteration of the path loop after path_next_vertex found `i`.
+CT_D bool pt_step(const DFrame &F, PState &p, Isect &i) {
+  if (i.is_hit) {
+    p.hit.inst = i.inst;
+    p.hit.prim = i.prim;
+    p.hit.hit = i.P;
+    p.flags |= kPathHit;
+    p.path_length++;
+    p.scene_length += i.distance;
+    pt_handle_hit(F, p, i);
+  } else {
+    p.flags &= ~kPathHit;
+    p.scene_length = kHlfMax;
+    pt_handle_miss(F, p);
+  }
+  return (p.flags & kPathActive) != 0u;
+}
+
+CT_D void pt_end(PState &p, F3 &color) {
   p.res.M = 1.0f;  // path_builder_finilize
   pres_finalize_ris(p.res);
   color = muls(p.res.F, p.res.weight);
   if (has_nan_inf(color)) color = f3(0, 0, 0);
-  seed = p.seed;
-  return p.res;
+}
+
+// Path state between the wavefront kernels: 14 float4 per path, stored
+// field-major (field k of path p at S[k * np + p]).  Fields the outputs
+// never read (L, normal, scene_length, hit) are not kept.
+constexpr int kPathStateVec = 14;
+CT_D void st_pstate(float4 *S, uint32_t np, uint32_t p, const PState &s) {
+  const uint32_t packed = (s.path_length & 0xFFu) | ((s.bounce_counters & 0xFFu) << 8) | ((s.pb.rc_len & 0xFFu) << 16) |
+                          ((s.last_rough ? 1u : 0u) << 24) | ((s.max_depth & 0x7Fu) << 25);
+  S[0 * (size_t)np + p] = make_float4(__uint_as_float(s.seed), __uint_as_float(s.pb.seed), __uint_as_float(s.flags),
+                                      __uint_as_float(packed));
+  S[1 * (size_t)np + p] = make_float4(s.origin.x, s.origin.y, s.origin.z, s.pdf);
+  S[2 * (size_t)np + p] = make_float4(s.dir.x, s.dir.y, s.dir.z, s.prev_scatter_pdf);
+  S[3 * (size_t)np + p] = make_float4(s.thp.x, s.thp.y, s.thp.z, s.rr_pdf);
+  S[4 * (size_t)np + p] = make_float4(s.prefix_thp.x, s.prefix_thp.y, s.prefix_thp.z, __uint_as_float(s.pb.cached_seed));
+  S[5 * (size_t)np + p] = make_float4(__uint_as_float(s.pb.rc.inst), __uint_as_float(s.pb.rc.prim), s.pb.rc.hit.x,
+                                      s.pb.rc.hit.y);
+  S[6 * (size_t)np + p] = make_float4(s.pb.rc.hit.z, s.pb.rc_wi.x, s.pb.rc_wi.y, s.pb.rc_wi.z);
+  S[7 * (size_t)np + p] = make_float4(s.pb.cj.x, s.pb.cj.y, s.pb.cj.z, __uint_as_float(s.pb.flags));
+  const PRes &r = s.res;
+  S[8 * (size_t)np + p] = make_float4(r.M, r.weight, __uint_as_float(r.flags), __uint_as_float(r.rc_seed));
+  S[9 * (size_t)np + p] = make_float4(r.F.x, r.F.y, r.F.z, r.light_pdf);
+  S[10 * (size_t)np + p] = make_float4(r.cj.x, r.cj.y, r.cj.z, __uint_as_float(r.init_seed));
+  S[11 * (size_t)np + p] = make_float4(__uint_as_float(r.rc.inst), __uint_as_float(r.rc.prim), r.rc.hit.x, r.rc.hit.y);
+  S[12 * (size_t)np + p] = make_float4(r.rc.hit.z, r.rc_wi.x, r.rc_wi.y, r.rc_wi.z);
+  S[13 * (size_t)np + p] = make_float4(r.rc_irr.x, r.rc_irr.y, r.rc_irr.z, 0.0f);
+}
+CT_D void ld_pstate(const float4 *S, uint32_t np, uint32_t p, PState &s) {
+  float4 q[kPathStateVec];
+#pragma unroll
+  for (int k = 0; k < kPathStateVec; ++k) q[k] = S[(size_t)k * np + p];
+  s.seed = __float_as_uint(q[0].x);
+  s.pb.seed = __float_as_uint(q[0].y);
+  s.flags = __float_as_uint(q[0].z);
+  const uint32_t packed = __float_as_uint(q[0].w);
+  s.path_length = packed & 0xFFu;
+  s.bounce_counters = (packed >> 8) & 0xFFu;
+  s.pb.rc_len = (packed >> 16) & 0xFFu;
+  s.last_rough = ((packed >> 24) & 1u) != 0u;
+  s.max_depth = (packed >> 25) & 0x7Fu;
+  s.origin = f3(q[1].x, q[1].y, q[1].z); s.pdf = q[1].w;
+  s.dir = f3(q[2].x, q[2].y, q[2].z); s.prev_scatter_pdf = q[2].w;
+  s.thp = f3(q[3].x, q[3].y, q[3].z); s.rr_pdf = q[3].w;
+  s.prefix_thp = f3(q[4].x, q[4].y, q[4].z); s.pb.cached_seed = __float_as_uint(q[4].w);
+  s.pb.rc.inst = __float_as_uint(q[5].x); s.pb.rc.prim = __float_as_uint(q[5].y);
+  s.pb.rc.hit = f3(q[5].z, q[5].w, q[6].x);
+  s.pb.rc_wi = f3(q[6].y, q[6].z, q[6].w);
+  s.pb.cj = f3(q[7].x, q[7].y, q[7].z); s.pb.flags = __float_as_uint(q[7].w);
+  PRes &r = s.res;
+  r.M = q[8].x; r.weight = q[8].y; r.flags = __float_as_uint(q[8].z); r.rc_seed = __float_as_uint(q[8].w);
+  r.F = f3(q[9].x, q[9].y, q[9].z); r.light_pdf = q[9].w;
+  r.cj = f3(q[10].x, q[10].y, q[10].z); r.init_seed = __float_as_uint(q[10].w);
+  r.rc.inst = __float_as_uint(q[11].x); r.rc.prim = __float_as_uint(q[11].y); r.rc.hit = f3(q[11].z, q[11].w, q[12].x);
+  r.rc_wi = f3(q[12].y, q[12].z, q[12].w);
+  r.rc_irr = f3(q[13].x, q[13].y, q[13].z);
+  s.scene_length = 0.0f;
+  s.normal = f3(0, 0, 0);
+  s.L = f3(0, 0, 0);
+  s.hit = objhit_invalid();
 }
 
 // compute_shifted_integrand_reconnection, shift.glsl:25-350, for the

@@ -57,6 +57,7 @@ CT_BUF_PATH_RESERVOIR, CT_BUF_TEMPORAL_PATH_RESERVOIR, CT_BUF_INDIRECT = 8, 9, 1
 CT_SHIFT_RECONNECTION, CT_SHIFT_HYBRID = 0, 2
 CT_MAX_PT_NEIGHBORS = 8
 CT_MAX_PT_DEPTH = 14
+CT_ADJOINT_MAX_DEPTH = 7
 
 f32, u32, i32 = C.c_float, C.c_uint32, C.c_int32
 
@@ -212,7 +213,16 @@ class FrameStats(C.Structure):
                 ("ms_total", f32), ("ms_primary", f32), ("ms_ris", f32), ("ms_temporal", f32),
                 ("ms_spatial", f32), ("ms_taa", f32), ("ms_trace_any", f32), ("ms_trace_closest", f32),
                 ("bounce_rays", C.c_uint64), ("shadow_rays_pt", C.c_uint64), ("pt_nodes", C.c_uint64),
-                ("pt_leaves", C.c_uint64), ("pt_instances", C.c_uint64), ("ms_pt", f32), ("ms_pt_spatial", f32)]
+                ("pt_leaves", C.c_uint64), ("pt_instances", C.c_uint64), ("ms_pt", f32), ("ms_pt_spatial", f32),
+                ("ms_pt_segments", f32), ("pad", f32)]
+
+
+class AdjointView(C.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("inv_view", f32 * 16), ("inv_proj", f32 * 16)]
+
+
+class AdjointDesc(C.Structure):
+    _fields_ = [("n_views", u32), ("spp", u32), ("max_depth", u32), ("seed", u32)]
 
 
 class SceneDesc(C.Structure):
@@ -231,7 +241,8 @@ class SceneInfo(C.Structure):
 STRUCT_SIZES = {"ct_aabb": (Aabb, 24), "ct_instance": (Instance, 72), "ct_primitive": (Primitive, 8),
                 "ct_material": (Material, 84), "ct_light": (Light, 40), "ct_light_config": (LightConfig, 52),
                 "ct_reservoir": (Reservoir, 36), "ct_di_info": (DiInfo, 60), "ct_path_reservoir": (PathReservoir, 92),
-                "ct_restir_params": (RestirParams, 108)}
+                "ct_restir_params": (RestirParams, 108), "ct_adjoint_view": (AdjointView, 136),
+                "ct_adjoint_desc": (AdjointDesc, 16)}
 
 
 # ----------------------------------------------------------------- loader
@@ -263,6 +274,11 @@ _SIGS = {
                           C.POINTER(TraceStats)], C.c_int),
     "ct_trace_any": ([C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                       C.POINTER(TraceStats)], C.c_int),
+    "ct_accel_read_instances": ([C.c_void_p, C.POINTER(Instance), u32, C.POINTER(u32)], C.c_int),
+    "ct_adjoint_render": ([C.c_void_p, C.POINTER(AdjointDesc), C.POINTER(AdjointView), C.c_void_p, C.c_void_p,
+                           C.c_void_p, C.c_void_p], C.c_int),
+    "ct_adjoint_backward": ([C.c_void_p, C.POINTER(AdjointDesc), C.POINTER(AdjointView), C.c_void_p, C.c_void_p,
+                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "ct_trace_bounce": ([C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                          C.c_void_p, C.c_void_p, C.POINTER(TraceStats)], C.c_int),
     "ct_ctx_create": ([C.POINTER(CtxDesc), C.POINTER(C.c_void_p)], C.c_int),
@@ -296,6 +312,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise CtError(CT_E_NODEVICE, f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        # PyTorch-ROCm ships its own HIP runtime.  Two HIP runtimes in one
+        # process do not share devices, so when torch is installed it is
+        # loaded first and the library binds to the same runtime (device
+        # pointers then pass freely between torch tensors and the ABI).
+        try:
+            import torch  # noqa: F401
+        except Exception:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (args, res) in _SIGS.items():
             fn = getattr(L, name)
@@ -422,6 +446,13 @@ class Accel:
         out = np.zeros(16, np.float32)
         check(lib().ct_accel_get_world_to_object(self.handle, instance, _fp(out)))
         return out
+
+    def read_instances(self):
+        cnt = u32(0)
+        check(lib().ct_accel_read_instances(self.handle, None, 0, C.byref(cnt)))
+        out = (Instance * max(cnt.value, 1))()
+        check(lib().ct_accel_read_instances(self.handle, out, cnt.value, C.byref(cnt)))
+        return out[: cnt.value]
 
     def read_bvh(self, instance: int, n: int) -> dict:
         keys = np.zeros(n, np.uint32)

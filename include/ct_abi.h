@@ -294,6 +294,12 @@ int ct_accel_read_bvh(ct_accel *accel, uint32_t instance_index,
                       ct_bvh_arrays *arrays);
 /* Host copy of an instance's world-to-object matrix (inverse of the
  * transform), as the kernels use it. */
+/* Host copy of the committed instance table, the contents of
+ * get_current_instance_buffer (accel_struct_mngr.hpp:225-230), by instance
+ * id: count = number of slots; an unused slot has primitive_count 0.  At
+ * most `capacity` records are written. */
+int ct_accel_read_instances(ct_accel *accel, ct_instance *out, uint32_t capacity,
+                            uint32_t *count);
 int ct_accel_get_world_to_object(ct_accel *accel, uint32_t instance_index,
                                  float out[16]);
 
@@ -478,11 +484,13 @@ typedef struct ct_frame_stats {
   float ms_trace_closest;
   uint64_t bounce_rays;     /* ReSTIR PT path segments (closest hit) */
   uint64_t shadow_rays_pt;  /* ReSTIR PT reconnection visibility queries */
-  uint64_t pt_nodes;        /* ReSTIR PT traversals (closest + any), counters on */
+  uint64_t pt_nodes;        /* path-segment traversal (counters on) */
   uint64_t pt_leaves;
   uint64_t pt_instances;
   float ms_pt;              /* initial paths + temporal path reuse */
   float ms_pt_spatial;      /* spatial path reuse + final composition */
+  float ms_pt_segments;     /* sum over the path-segment traversal launches */
+  float pad;
 } ct_frame_stats;
 /* Enable per-pass HIP events (1) and traversal node/leaf counters (2). */
 int ct_ctx_set_instrumentation(ct_ctx *ctx, uint32_t flags);
@@ -540,6 +548,55 @@ int ct_scene_destroy(ct_scene *scene);
  * main.cpp:1317-1320). */
 int ct_scene_load(const ct_scene *scene, ct_accel *accel, ct_ctx *ctx,
                   void *stream);
+
+/* =================================================================
+ * Differentiable rendering.  No counterpart in the reference (its
+ * renderer has no derivatives); the build's own adjoint of radiance with
+ * respect to per-voxel albedo and emission.
+ *
+ * The forward model is a unidirectional path tracer over the accel's
+ * voxels: every segment is intersect() of bounce.glsl:9-130 (closest
+ * voxel, boxes tested from the origin pushed along the ray), scattering
+ * is cosine-weighted around the face normal (random_cosine_direction,
+ * prng.glsl:84-108, whose Lambertian weight f*cos/pdf is the albedo),
+ * emission is collected at every vertex and a ray that leaves the scene
+ * carries nothing.  A sample of pixel (x, y) is
+ *   L = sum_k beta_k * e(v_k),  beta_0 = 1,  beta_{k+1} = beta_k * a(v_k)
+ * over vertices v_0..v_K, K <= max_depth.  The camera ray passes through
+ * (x + u, y + v), u, v the first two numbers of the sample's stream
+ * tea(tea(global pixel, seed), sample).
+ *
+ * The adjoint replays each path from its seed (nothing is stored between
+ * the passes; the <= max_depth+1 vertices of the path being replayed stay
+ * in registers) and adds dLoss/da and dLoss/de into the gradient buffers
+ * with atomics.
+ * ================================================================= */
+#define CT_ADJOINT_MAX_DEPTH 7u
+typedef struct ct_adjoint_view {
+  uint32_t width;
+  uint32_t height;
+  float inv_view[16]; /* as ct_frame_params */
+  float inv_proj[16];
+} ct_adjoint_view;
+typedef struct ct_adjoint_desc {
+  uint32_t n_views;
+  uint32_t spp;       /* samples per pixel; the image is their mean */
+  uint32_t max_depth; /* scattering events after the primary hit, <= CT_ADJOINT_MAX_DEPTH */
+  uint32_t seed;
+} ct_adjoint_desc;
+/* Device pointers.  albedo [slots][3] and emission [slots] are indexed by
+ * the voxel's primitive slot (first_primitive_index + primitive id);
+ * image holds the views one after another, [H][W][3] each.  Calls are
+ * ordered on `stream` (null: the accel's stream). */
+int ct_adjoint_render(ct_accel *accel, const ct_adjoint_desc *desc,
+                      const ct_adjoint_view *views, const float *albedo,
+                      const float *emission, float *image, void *stream);
+/* grad_image has image's layout; the gradients are added into
+ * grad_albedo [slots][3] and grad_emission [slots] (zero them first). */
+int ct_adjoint_backward(ct_accel *accel, const ct_adjoint_desc *desc,
+                        const ct_adjoint_view *views, const float *albedo,
+                        const float *emission, const float *grad_image,
+                        float *grad_albedo, float *grad_emission, void *stream);
 
 #ifdef __cplusplus
 }

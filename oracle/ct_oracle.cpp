@@ -1874,7 +1874,7 @@ static void rchit_model(const float *t, float *m) {
 
 // RESTIR_PREPASS_AND_FIRST_VISIBILITY_TEST, rgen.glsl:46-320
 static void prepass_pixel(const Ctx &c, uint32_t x, uint32_t y, TraceCount &cnt, TraceCount &pc,
-                          TraceCount &ptc) {
+                          TraceCount &ptc, TraceCount &pac) {
   const ct_frame_params &p = *c.p;
   Frame &fr = *c.fr;
   const uint32_t W = p.width, H = p.height;
@@ -1965,7 +1965,7 @@ static void prepass_pixel(const Ctx &c, uint32_t x, uint32_t y, TraceCount &cnt,
         uint32_t pcx = f2u_sat(((float)x + vx) + r1), pcy = f2u_sat(((float)y + vy) + r2);
         uint32_t prev_idx = std::min(W * H - 1u, pcx + pcy * W);
         const Isect pi = isect_from_di(c, fr.di_prev[prev_idx], false);
-        out = temporal_path_reuse(c, out, fr.pres_t[prev_idx], ci, pi, di.seed, ptc);
+        out = temporal_path_reuse(c, out, fr.pres_t[prev_idx], ci, pi, di.seed, pac);
         ind = muls(load3(out.F), out.weight);
       }
       if (bad3(ind)) ind = v3(0, 0, 0);
@@ -2073,6 +2073,60 @@ static void taa_pixel(const Ctx &c, uint32_t x, uint32_t y) {
 }
 
 }  // namespace
+
+// ================================================================ adjoint
+// Differentiable voxel path tracer (the build's own; ct_abi.h documents
+// the model) and its adjoint, restated for the CPU.
+
+struct AdjSample {
+  uint32_t g[8];
+  V3 beta[8];
+  uint32_t nv = 0;
+};
+
+static V3 adj_trace(const Scene &sc, const ct_adjoint_desc &d, const ct_adjoint_view &V, uint32_t x, uint32_t y,
+                    uint32_t seed, const float *albedo, const float *emission, AdjSample *path) {
+  // camera ray through (x + u, y + v)
+  const float jx = rnd(seed);
+  const float jy = rnd(seed);
+  float pcx = (float)x + jx, pcy = (float)y + jy;
+  float ux = pcx / (float)V.width, uy = pcy / (float)V.height;
+  float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
+  float org[4] = {0, 0, 0, 1}, ov[4];
+  mat4_mul_vec(V.inv_view, org, ov);
+  float tv[4] = {dx, dy, 1.0f, 1.0f}, t4[4];
+  mat4_mul_vec(V.inv_proj, tv, t4);
+  V3 nt = normalize(v3(t4[0], t4[1], t4[2]));
+  float dv[4] = {nt.x, nt.y, nt.z, 0.0f}, d4[4];
+  mat4_mul_vec(V.inv_view, dv, d4);
+  V3 o = v3(ov[0], ov[1], ov[2]), dir = v3(d4[0], d4[1], d4[2]);
+  V3 beta = v3(1, 1, 1), L = v3(0, 0, 0);
+  uint32_t nv = 0;
+  for (uint32_t k = 0; k <= d.max_depth; ++k) {
+    V3 ro = compute_ray_origin(o, dir);
+    TraceCount cnt;
+    ClosestState st = trace_bounce(sc, ro, dir, cnt);
+    if (st.inst == kInvalid) break;
+    float t;
+    V3 pos, nrm;
+    hit_from_ray(sc, st.inst, st.prim, ro, dir, t, pos, nrm);
+    uint32_t g = sc.inst[st.inst].first_primitive_index + st.prim;
+    if (path) {
+      path->g[k] = g;
+      path->beta[k] = beta;
+    }
+    nv = k + 1;
+    L = add(L, muls(beta, emission[g]));
+    if (k == d.max_depth) break;
+    beta = mulv(beta, load3(&albedo[3 * (size_t)g]));
+    V3 sd = random_cosine_direction(seed, nrm);
+    if (std::fabs(sd.x) < 1e-8f && std::fabs(sd.y) < 1e-8f && std::fabs(sd.z) < 1e-8f) sd = nrm;
+    dir = sd;
+    o = compute_ray_origin(compute_ray_origin(pos, nrm), nrm);
+  }
+  if (path) path->nv = nv;
+  return L;
+}
 
 // ================================================================ C API
 struct oc_scene : Scene {};
@@ -2279,6 +2333,71 @@ void oc_trace_bounce(oc_scene *scene, uint32_t n, const float *origins, const fl
 
 void oc_sincos(float x, float *s, float *c) { sincos_ct(x, *s, *c); }
 
+int oc_adjoint_render(oc_scene *scene, const ct_adjoint_desc *d, const ct_adjoint_view *views,
+                      const float *albedo, const float *emission, float *image) {
+  if (!scene || !d || !views || d->spp == 0 || d->max_depth > CT_ADJOINT_MAX_DEPTH) return CT_E_INVALID;
+  uint64_t base = 0;
+  for (uint32_t v = 0; v < d->n_views; ++v) {
+    const ct_adjoint_view &V = views[v];
+    const int64_t np = (int64_t)V.width * V.height;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t p = 0; p < np; ++p) {
+      const uint32_t x = (uint32_t)(p % V.width), y = (uint32_t)(p / V.width), gpix = (uint32_t)(base + p);
+      V3 sum = v3(0, 0, 0);
+      for (uint32_t s = 0; s < d->spp; ++s)
+        sum = add(sum, adj_trace(*scene, *d, V, x, y, tea(tea(gpix, d->seed), s), albedo, emission, nullptr));
+      store3(&image[3 * (size_t)gpix], divs(sum, (float)d->spp));
+    }
+    base += (uint64_t)np;
+  }
+  return CT_OK;
+}
+
+// Gradients accumulated in double, then added into the float outputs.
+int oc_adjoint_backward(oc_scene *scene, const ct_adjoint_desc *d, const ct_adjoint_view *views,
+                        const float *albedo, const float *emission, const float *grad_image,
+                        float *grad_albedo, float *grad_emission) {
+  if (!scene || !d || !views || d->spp == 0 || d->max_depth > CT_ADJOINT_MAX_DEPTH) return CT_E_INVALID;
+  const size_t slots = scene->prim_capacity;
+  std::vector<double> ga(3 * slots, 0.0), ge(slots, 0.0);
+  uint64_t base = 0;
+  for (uint32_t v = 0; v < d->n_views; ++v) {
+    const ct_adjoint_view &V = views[v];
+    const int64_t np = (int64_t)V.width * V.height;
+    for (int64_t p = 0; p < np; ++p) {
+      const uint32_t x = (uint32_t)(p % V.width), y = (uint32_t)(p / V.width), gpix = (uint32_t)(base + p);
+      const double inv = 1.0 / (double)d->spp;
+      const double G[3] = {grad_image[3 * (size_t)gpix] * inv, grad_image[3 * (size_t)gpix + 1] * inv,
+                           grad_image[3 * (size_t)gpix + 2] * inv};
+      for (uint32_t s = 0; s < d->spp; ++s) {
+        AdjSample path;
+        adj_trace(*scene, *d, V, x, y, tea(tea(gpix, d->seed), s), albedo, emission, &path);
+        double S[3] = {0, 0, 0};
+        for (int k = (int)path.nv - 1; k >= 0; --k) {
+          const uint32_t g = path.g[k];
+          const double b[3] = {path.beta[k].x, path.beta[k].y, path.beta[k].z};
+          const double e = emission[g];
+          ge[g] += G[0] * b[0] + G[1] * b[1] + G[2] * b[2];
+          if ((uint32_t)k < d->max_depth) {
+            for (int c = 0; c < 3; ++c) {
+              ga[3 * g + c] += G[c] * b[c] * S[c];
+              S[c] = e + (double)albedo[3 * (size_t)g + c] * S[c];
+            }
+          } else {
+            S[0] = S[1] = S[2] = e;
+          }
+        }
+      }
+    }
+    base += (uint64_t)np;
+  }
+  for (size_t i = 0; i < slots; ++i) {
+    grad_emission[i] += (float)ge[i];
+    for (int c = 0; c < 3; ++c) grad_albedo[3 * i + c] += (float)ga[3 * i + c];
+  }
+  return CT_OK;
+}
+
 void oc_random_cosine_direction(uint32_t *seed, const float n[3], float out[3]) {
   store3(out, random_cosine_direction(*seed, load3(n)));
 }
@@ -2375,17 +2494,17 @@ int oc_frame_render(oc_frame *frame, const ct_frame_params *params, int threads,
 #pragma omp for schedule(dynamic, 64)
     for (int64_t k = 0; k < np; ++k) {
       uint32_t y = row_begin + (uint32_t)(k / W), x = (uint32_t)(k % W);
-      TraceCount cnt, pc, ptc;
-      prepass_pixel(c, x, y, cnt, pc, ptc);
+      TraceCount cnt, pc, ptc, pac;
+      prepass_pixel(c, x, y, cnt, pc, ptc, pac);
       prim_rays += 1;
       sh_prepass += cnt.rays;
       nodes += cnt.nodes; leaves += cnt.leaves; insts += cnt.instances;
       cnodes += pc.nodes; cleaves += pc.leaves; cinsts += pc.instances;
-      bounce += ptc.bounce; pt_shadow += ptc.rays;
+      bounce += ptc.bounce; pt_shadow += pac.rays;
       pt_nodes += ptc.nodes; pt_leaves += ptc.leaves; pt_insts += ptc.instances;
     }
   }
-#pragma omp parallel num_threads(nt) reduction(+ : nodes, leaves, insts, sh_shade, pt_shadow, pt_nodes, pt_leaves, pt_insts)
+#pragma omp parallel num_threads(nt) reduction(+ : nodes, leaves, insts, sh_shade, pt_shadow)
   {
 #pragma omp for schedule(dynamic, 64)
     for (int64_t k = 0; k < np; ++k) {
@@ -2394,7 +2513,7 @@ int oc_frame_render(oc_frame *frame, const ct_frame_params *params, int threads,
       shade_pixel(c, x, y, cnt, ptc);
       sh_shade += cnt.rays;
       nodes += cnt.nodes; leaves += cnt.leaves; insts += cnt.instances;
-      pt_shadow += ptc.rays; pt_nodes += ptc.nodes; pt_leaves += ptc.leaves; pt_insts += ptc.instances;
+      pt_shadow += ptc.rays;
     }
   }
   if (p.is_active & CT_TAA_BIT) {

@@ -83,6 +83,16 @@ void oc_trace_bounce(oc_scene *scene, uint32_t n, const float *origins,
 void oc_sincos(float x, float *s, float *c);
 void oc_random_cosine_direction(uint32_t *seed, const float n[3], float out[3]);
 
+/* The differentiable path tracer of ct_adjoint_render / _backward on host
+ * arrays (gradients accumulated in double, then added). */
+int oc_adjoint_render(oc_scene *scene, const ct_adjoint_desc *desc,
+                      const ct_adjoint_view *views, const float *albedo,
+                      const float *emission, float *image);
+int oc_adjoint_backward(oc_scene *scene, const ct_adjoint_desc *desc,
+                        const ct_adjoint_view *views, const float *albedo,
+                        const float *emission, const float *grad_image,
+                        float *grad_albedo, float *grad_emission);
+
 typedef struct oc_frame oc_frame;
 oc_frame *oc_frame_create(oc_scene *scene, uint32_t width, uint32_t height);
 void oc_frame_destroy(oc_frame *frame);

@@ -19,7 +19,8 @@ _ROOT = os.path.dirname(_HERE)
 sys.path.insert(0, os.path.join(_ROOT, "cube-tracing_amd"))
 import ctamd  # noqa: E402  (struct layouts only)
 from ctamd import (Aabb, Instance, Primitive, Material, Light, LightConfig, FrameParams,  # noqa: E402
-                   FrameStats, TraceStats, BvhArrays, Reservoir, DiInfo, PathReservoir)
+                   FrameStats, TraceStats, BvhArrays, Reservoir, DiInfo, PathReservoir, AdjointDesc,
+                   AdjointView)
 
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
@@ -60,6 +61,10 @@ def lib():
             "oc_trace_bounce": ([C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.c_void_p, P(TraceStats)], None),
             "oc_sincos": ([f32, P(f32), P(f32)], None),
+            "oc_adjoint_render": ([C.c_void_p, P(AdjointDesc), P(AdjointView), C.c_void_p, C.c_void_p, C.c_void_p],
+                                  C.c_int),
+            "oc_adjoint_backward": ([C.c_void_p, P(AdjointDesc), P(AdjointView), C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
             "oc_random_cosine_direction": ([P(u32), P(f32), P(f32)], None),
             "oc_frame_create": ([C.c_void_p, u32, u32], C.c_void_p),
             "oc_frame_destroy": ([C.c_void_p], None),
@@ -258,6 +263,29 @@ class OScene:
         if stats:
             out["stats"] = st
         return out
+
+    def adjoint_render(self, desc, views, albedo, emission):
+        """The differentiable path tracer: image [sum of H*W][3]."""
+        a = np.ascontiguousarray(albedo, np.float32)
+        e = np.ascontiguousarray(emission, np.float32)
+        n = sum(v.width * v.height for v in views)
+        img = np.zeros((n, 3), np.float32)
+        vv = (AdjointView * len(views))(*views)
+        rc = lib().oc_adjoint_render(self.handle, C.byref(desc), vv, a.ctypes.data, e.ctypes.data, img.ctypes.data)
+        assert rc == 0
+        return img
+
+    def adjoint_backward(self, desc, views, albedo, emission, grad_image):
+        a = np.ascontiguousarray(albedo, np.float32)
+        e = np.ascontiguousarray(emission, np.float32)
+        g = np.ascontiguousarray(grad_image, np.float32)
+        ga = np.zeros_like(a)
+        ge = np.zeros_like(e)
+        vv = (AdjointView * len(views))(*views)
+        rc = lib().oc_adjoint_backward(self.handle, C.byref(desc), vv, a.ctypes.data, e.ctypes.data, g.ctypes.data,
+                                       ga.ctypes.data, ge.ctypes.data)
+        assert rc == 0
+        return ga, ge
 
     def close(self):
         if getattr(self, "handle", None):
