@@ -615,6 +615,23 @@ int rebuild_all(Accel &a) {
   }
   S.blas_depth_max = dmax;
   CT_HIP(hipMemcpyAsync(S.instances_d.p, dinst.data(), sizeof(DInst) * a.instances.size(), hipMemcpyHostToDevice, st));
+  // geometry_T = previous_model * inverse(model) per instance
+  // (motion_vectors.glsl:146); the scene is static between builds, so
+  // previous_model = model.  Frames multiply it by their prev_view_proj.
+  {
+    std::vector<float> geo(16 * std::max<size_t>(a.instances.size(), 1), 0.0f);
+    for (size_t i = 0; i < a.instances.size(); ++i) {
+      if (i >= a.instance_used.size() || !a.instance_used[i]) continue;
+      const float *t = a.instances[i].transform, *w = &a.w2o[16 * i];
+      for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r)
+          geo[16 * i + c * 4 + r] = t[0 * 4 + r] * w[c * 4 + 0] + t[1 * 4 + r] * w[c * 4 + 1] +
+                                    t[2 * 4 + r] * w[c * 4 + 2] + t[3 * 4 + r] * w[c * 4 + 3];
+    }
+    if ((rc = S.geo_t.reserve(sizeof(float) * geo.size()))) return rc;
+    CT_HIP(hipMemcpyAsync(S.geo_t.p, geo.data(), sizeof(float) * geo.size(), hipMemcpyHostToDevice, st));
+    CT_HIP(hipStreamSynchronize(st));
+  }
   // ---- TLAS over the instances' padded world boxes, in slot order.
   const uint32_t ni = (uint32_t)ids.size();
   std::vector<float4> tc(ni), tlo(ni), thi(ni);

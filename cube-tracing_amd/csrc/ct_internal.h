@@ -116,6 +116,7 @@ struct BvhSet {
   DevBuf nodes, leaves, instances_d, prims, aabbs, cube_lights, inst_aos;
   DevBuf tlas_nodes, tlas_inst, tlas_keys, tlas_parent;
   DevBuf blas_keys, blas_ids, blas_parent;  // kept for ct_accel_read_bvh
+  DevBuf geo_t;                             // per instance: transform * world-to-object (16 floats)
   uint32_t tlas_root = kLeaf;
   uint32_t tlas_count = 0;
   uint32_t tlas_depth = 0;
@@ -196,6 +197,11 @@ struct Ctx {
   bool stats_pending = false;
   hipStream_t stream_last = nullptr;
   uint32_t last_w = 0, last_h = 0;
+  DevBuf frame_dev, blk_dev;    // the frame record and its block tables on the device
+  uint32_t blk_key[4] = {0, 0, 0, 0};
+  uint32_t blk_counts[3] = {0, 0, 0};
+  const void *lights_accel = nullptr;  // accel and build whose cube lights were uploaded
+  uint32_t lights_generation = 0;
 };
 
 // Kernel-side views used by trace.hip and frame.hip.

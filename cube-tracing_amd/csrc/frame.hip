@@ -57,7 +57,8 @@ struct DFrame {
   const float4 *cube;   // [3 per light]: (pos, size), (emissive, type), (inst, prim, -, -)
   const float4 *point;
   const float4 *env;
-  const float *pt;      // [16 per instance] prev_view_proj * geometry_T
+  const float *geo_t;   // [16 per instance] geometry_T = model * inverse(model)
+  float prev_vp[16];    // inverse(prev_inv_proj) * inverse(prev_inv_view)
   float4 *dpos_c, *dnrm_c, *dorg_c;
   uint4 *dhit_c;
   float4 *dpos_p, *dnrm_p, *dorg_p;
@@ -80,7 +81,18 @@ struct DFrame {
   float4 *pres_t;    // [6 per pixel] temporal path reservoirs (history)
   float4 *indirect;  // indirect colour of the first pass
   float4 *pst;       // path state between the wavefront kernels (14 float4 per pixel)
-  uint32_t *ptq[2];  // path-segment queues (pixel ids) of even / odd depths
+  uint32_t *ptq[2];  // path-segment queues (tagged pixel ids) of even / odd depths
+  float4 *seg_o, *seg_d, *seg_hit;  // path segments and their hits by queue position
+  uint32_t qtag;     // frame index << kViewShift when frames are batched, else 0
+  uint32_t taa_w, taa_h;  // TAA dispatch size (main.cpp:1642-1643 truncation applied)
+};
+
+// Several frames in one launch: every block knows its frame and first
+// pixel (or tile); queue entries carry the frame index in their top bits.
+constexpr uint32_t kViewShift = 26, kSlotMask = (1u << kViewShift) - 1u;
+struct FrameSet {
+  const DFrame *fb;   // device array of frames
+  const uint2 *blk;   // per block: (frame, first pixel or tile)
 };
 
 struct PLight {
@@ -658,18 +670,20 @@ CT_D void queue_flush(const DFrame &F, uint32_t px, uint32_t emitted, uint32_t c
   while (emitted) {
     uint32_t k = (uint32_t)(__ffs(emitted) - 1);
     emitted &= emitted - 1u;
-    F.queue[pos++] = px * kRaySlots + k;
+    F.queue[pos++] = F.qtag | (px * kRaySlots + k);
   }
 }
 
 // ---------------------------------------------------------------- kernels
 // Primary rays: raygen rgen.glsl:18-42 + closest hit; one 8x8 tile per wave.
 template <bool kCount>
-__global__ void __launch_bounds__(kBlock) k_primary(DFrame F, unsigned long long *stats) {
+__global__ void __launch_bounds__(kBlock) k_primary(FrameSet FS, unsigned long long *stats) {
   __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
+  const uint2 blk_ = FS.blk[blockIdx.x];
+  const DFrame &F = FS.fb[blk_.x];
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   const uint32_t tiles_x = (F.W + 15) / 16;
-  const uint32_t bx = blockIdx.x % tiles_x, by = blockIdx.x / tiles_x;
+  const uint32_t bx = blk_.y % tiles_x, by = blk_.y / tiles_x;
   const uint32_t x = bx * 16 + (w & 1u) * 8 + (lane & 7u);
   const uint32_t y = by * 16 + (w >> 1) * 8 + (lane >> 3);
   TraceCnt cnt;
@@ -700,8 +714,10 @@ __global__ void __launch_bounds__(kBlock) k_primary(DFrame F, unsigned long long
 // Prepass after the primary hit: rchit (rchit.glsl:18-54), miss
 // (rmiss.glsl:19-29), motion vector, temporal gather, confidence and RIS.
 // EMIT for the RIS visibility ray.
-__global__ void __launch_bounds__(kBlock) k_ris(DFrame F) {
-  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_ris(FrameSet FS) {
+  const uint2 blk_ = FS.blk[blockIdx.x];
+  const DFrame &F = FS.fb[blk_.x];
+  const uint32_t px = blk_.y + threadIdx.x;
   uint32_t emitted = 0;
   if (px < F.npix) {
     const uint32_t x = px % F.W, y = px / F.W;
@@ -744,8 +760,15 @@ __global__ void __launch_bounds__(kBlock) k_ris(DFrame F) {
       F.img[px] = make_float4(hv.x, hv.y, hv.z, 1.0f);
       if (F.active & CT_TAA_BIT) F.taa[px] = make_float4(hv.x, hv.y, hv.z, 1.0f);
     } else {
-      // motion_vectors.glsl:70-89
-      const float *pt = F.pt + 16 * (size_t)di.inst;
+      // motion_vectors.glsl:70-89: P = prev_view_proj * geometry_T
+      const float *g = F.geo_t + 16 * (size_t)di.inst;
+      float pt[16];
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          pt[cc * 4 + r] = F.prev_vp[0 * 4 + r] * g[cc * 4 + 0] + F.prev_vp[1 * 4 + r] * g[cc * 4 + 1] +
+                           F.prev_vp[2 * 4 + r] * g[cc * 4 + 2] + F.prev_vp[3 * 4 + r] * g[cc * 4 + 3];
       float xi_x = (float)x + 0.5f, xi_y = (float)y + 0.5f;
       float ps[4];
       mat4_vec(pt, di.pos.x, di.pos.y, di.pos.z, 1.0f, ps);
@@ -778,8 +801,10 @@ __global__ void __launch_bounds__(kBlock) k_ris(DFrame F) {
 }
 
 // RIS visibility result (FINISH) and the temporal reuse rays (EMIT).
-__global__ void __launch_bounds__(kBlock) k_temporal_emit(DFrame F) {
-  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_temporal_emit(FrameSet FS) {
+  const uint2 blk_ = FS.blk[blockIdx.x];
+  const DFrame &F = FS.fb[blk_.x];
+  const uint32_t px = blk_.y + threadIdx.x;
   uint32_t emitted = 0;
   if (px < F.npix) {
     float4 pd = F.dpos_c[px];
@@ -809,8 +834,10 @@ __global__ void __launch_bounds__(kBlock) k_temporal_emit(DFrame F) {
   queue_flush(F, px, emitted, kCntTemporal);
 }
 
-__global__ void __launch_bounds__(kBlock) k_temporal_finish(DFrame F) {
-  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_temporal_finish(FrameSet FS) {
+  const uint2 blk_ = FS.blk[blockIdx.x];
+  const DFrame &F = FS.fb[blk_.x];
+  const uint32_t px = blk_.y + threadIdx.x;
   if (px >= F.npix) return;
   float4 pd = F.dpos_c[px];
   if (!(pd.w > 0.0f)) {
@@ -840,8 +867,10 @@ __global__ void __launch_bounds__(kBlock) k_temporal_finish(DFrame F) {
 
 // Shading pass: spatial reuse (EMIT / FINISH) then shade and store.
 template <bool kEmit>
-__global__ void __launch_bounds__(kBlock) k_spatial(DFrame F) {
-  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_spatial(FrameSet FS) {
+  const uint2 blk_ = FS.blk[blockIdx.x];
+  const DFrame &F = FS.fb[blk_.x];
+  const uint32_t px = blk_.y + threadIdx.x;
   uint32_t emitted = 0;
   if (px < F.npix) {
     const uint32_t x = px % F.W, y = px / F.W;
@@ -939,8 +968,10 @@ CT_D uint32_t queue_push(bool active, uint32_t *count, uint32_t *also) {
 // the path loop, queueing the next segment; k_pt_end finalizes the path
 // reservoir and runs temporal path reuse.  Counters: kCntPtQueue +
 // (depth & 1) holds the length of the queue of that depth.
-__global__ void __launch_bounds__(kBlock) k_pt_begin(DFrame F) {
-  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_pt_begin(FrameSet FS) {
+  const uint2 blk_ = FS.blk[blockIdx.x];
+  const DFrame &F = FS.fb[blk_.x];
+  const uint32_t px = blk_.y + threadIdx.x;
   bool more = false;
   F3 ro, rd;
   if (px < F.npix && F.dpos_c[px].w > 0.0f) {
@@ -953,53 +984,57 @@ __global__ void __launch_bounds__(kBlock) k_pt_begin(DFrame F) {
   }
   const uint32_t pos = queue_push(more, &F.counters[kCntPtQueue], &F.counters[kCntPtBounce]);
   if (more) {
-    F.ptq[0][pos] = px;
-    F.ray_o[pos] = make_float4(ro.x, ro.y, ro.z, 0.0f);
-    F.ray_d[pos] = make_float4(rd.x, rd.y, rd.z, 0.0f);
+    F.ptq[0][pos] = F.qtag | px;
+    F.seg_o[pos] = make_float4(ro.x, ro.y, ro.z, 0.0f);
+    F.seg_d[pos] = make_float4(rd.x, rd.y, rd.z, 0.0f);
   }
 }
 
 // Closest voxel of each queued path segment (intersect() semantics), one
-// 64-ray fetch per wave.
-template <bool kCount>
-__global__ void __launch_bounds__(kBlock) k_trace_segments(DScene S, const float4 *ray_o, const float4 *ray_d,
-                                                           const uint32_t *count, uint32_t *head, float4 *hit,
-                                                           unsigned long long *stats) {
+// 64-ray fetch per wave.  kMulti: segments of several frames (scenes).
+template <bool kCount, bool kMulti>
+__global__ void __launch_bounds__(kBlock) k_trace_segments(FrameSet FS, uint32_t d, unsigned long long *stats) {
   __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
+  const DFrame &F0 = FS.fb[0];
   const uint32_t tid = threadIdx.x, lane = tid & 63u;
-  const uint32_t n = *count;
+  const uint32_t n = F0.counters[kCntPtQueue + (d & 1u)];
   TraceCnt cnt;
   for (;;) {
     uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(head, 64u);
+    if (lane == 0) base = atomicAdd(&F0.counters[kCntPtHead], 64u);
     base = __shfl(base, 0);
     if (base >= n) break;
     const uint32_t q = base + lane;
     if (q < n) {
-      const float4 o = ray_o[q], d = ray_d[q];
+      const uint32_t view = kMulti ? (F0.ptq[d & 1u][q] >> kViewShift) : 0u;
+      const float4 o = F0.seg_o[q], dd = F0.seg_d[q];
       float bt = kMaxDistance;
       uint32_t bi = kInvalid, bp = kInvalid;
       bool ovf = false;
-      traverse_mode<kTraceBounce, kCount>(S, xyz(o), xyz(d), kMaxDistance, kInvalid, kInvalid, bt, bi, bp,
-                                          stk + tid, kBlock, cnt, ovf);
-      hit[q] = make_float4(bt, __uint_as_float(bi), __uint_as_float(bp), 0.0f);
+      traverse_mode<kTraceBounce, kCount>(FS.fb[view].S, xyz(o), xyz(dd), kMaxDistance, kInvalid, kInvalid, bt, bi,
+                                          bp, stk + tid, kBlock, cnt, ovf);
+      F0.seg_hit[q] = make_float4(bt, __uint_as_float(bi), __uint_as_float(bp), 0.0f);
     }
   }
   if (kCount) wave_add_cnt(cnt, stats);
 }
 
 // One path-loop iteration for every path of queue `d & 1`.
-__global__ void __launch_bounds__(kBlock) k_pt_step(DFrame F, uint32_t d) {
+template <bool kMulti>
+__global__ void __launch_bounds__(kBlock) k_pt_step(FrameSet FS, uint32_t d) {
+  const DFrame &F0 = FS.fb[0];
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t n = F.counters[kCntPtQueue + (d & 1u)];
+  const uint32_t n = F0.counters[kCntPtQueue + (d & 1u)];
   bool more = false;
-  uint32_t px = 0;
+  uint32_t entry = 0;
   F3 ro, rd;
   if (q < n) {
-    px = F.ptq[d & 1u][q];
+    entry = F0.ptq[d & 1u][q];
+    const DFrame &F = FS.fb[kMulti ? (entry >> kViewShift) : 0u];
+    const uint32_t px = kMulti ? (entry & kSlotMask) : entry;
     PState p;
     ld_pstate(F.pst, F.npix, px, p);
-    const float4 h = F.hit[q];
+    const float4 h = F0.seg_hit[q];
     const F3 box_o = compute_ray_origin(p.origin, p.dir);
     Isect i = segment_record(F, p.origin, box_o, p.dir, __float_as_uint(h.y), __float_as_uint(h.z));
     more = pt_step(F, p, i);
@@ -1007,19 +1042,21 @@ __global__ void __launch_bounds__(kBlock) k_pt_step(DFrame F, uint32_t d) {
     ro = compute_ray_origin(p.origin, p.dir);
     rd = p.dir;
   }
-  const uint32_t pos = queue_push(more, &F.counters[kCntPtQueue + ((d + 1u) & 1u)], &F.counters[kCntPtBounce]);
+  const uint32_t pos = queue_push(more, &F0.counters[kCntPtQueue + ((d + 1u) & 1u)], &F0.counters[kCntPtBounce]);
   if (more) {
-    F.ptq[(d + 1u) & 1u][pos] = px;
-    F.ray_o[pos] = make_float4(ro.x, ro.y, ro.z, 0.0f);
-    F.ray_d[pos] = make_float4(rd.x, rd.y, rd.z, 0.0f);
+    F0.ptq[(d + 1u) & 1u][pos] = entry;
+    F0.seg_o[pos] = make_float4(ro.x, ro.y, ro.z, 0.0f);
+    F0.seg_d[pos] = make_float4(rd.x, rd.y, rd.z, 0.0f);
   }
 }
 
 // path_output and temporal path reuse (indirect_illumination.glsl:454-475).
 template <bool kCount>
-__global__ void __launch_bounds__(kBlock) k_pt_end(DFrame F, unsigned long long *stats) {
+__global__ void __launch_bounds__(kBlock) k_pt_end(FrameSet FS, unsigned long long *stats) {
   __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
-  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint2 blk_ = FS.blk[blockIdx.x];
+  const DFrame &F = FS.fb[blk_.x];
+  const uint32_t px = blk_.y + threadIdx.x;
   PtTrace tr;
   tr.stk = stk + threadIdx.x;
   tr.bounce_rays = 0u;
@@ -1060,9 +1097,11 @@ __global__ void __launch_bounds__(kBlock) k_pt_end(DFrame F, unsigned long long 
 // and the rest of the shading pass: indirect term, emission, NaN scrub,
 // image store (rgen.glsl:449-489).
 template <bool kCount>
-__global__ void __launch_bounds__(kBlock) k_pt_spatial(DFrame F, unsigned long long *stats) {
+__global__ void __launch_bounds__(kBlock) k_pt_spatial(FrameSet FS, unsigned long long *stats) {
   __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
-  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint2 blk_ = FS.blk[blockIdx.x];
+  const DFrame &F = FS.fb[blk_.x];
+  const uint32_t px = blk_.y + threadIdx.x;
   PtTrace tr;
   tr.stk = stk + threadIdx.x;
   tr.bounce_rays = 0u;
@@ -1123,8 +1162,12 @@ __global__ void __launch_bounds__(kBlock) k_trace_bounce(DFrame F, const float4 
   if (kCount) wave_add_cnt(tr.cnt, stats);
 }
 
-__global__ void __launch_bounds__(kBlock) k_taa(DFrame F, uint32_t tw, uint32_t th) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_taa(FrameSet FS) {
+  const uint2 blk_ = FS.blk[blockIdx.x];
+  const DFrame &F = FS.fb[blk_.x];
+  if (!(F.active & CT_TAA_BIT)) return;
+  const uint32_t tw = F.taa_w, th = F.taa_h;
+  const uint32_t t = blk_.y + threadIdx.x;
   if (t >= tw * th) return;
   const uint32_t x = t % tw, y = t / tw;
   if (x >= F.W || y >= F.H) return;
@@ -1176,6 +1219,17 @@ __global__ void k_reset_counters(uint32_t *c, int first, int n) {
   int i = threadIdx.x;
   if (i < n) c[first + i] = 0u;
 }
+__global__ void k_reset_u64(unsigned long long *c, int n) {
+  int i = threadIdx.x;
+  if (i < n) c[i] = 0ull;
+}
+// The frame record travels as a kernel argument and is stored for the
+// pass kernels (no host staging buffer to keep alive).
+__global__ void k_store_frame(DFrame F, DFrame *dst) {
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(&F);
+  uint32_t *d = reinterpret_cast<uint32_t *>(dst);
+  for (uint32_t i = threadIdx.x; i < sizeof(DFrame) / 4; i += blockDim.x) d[i] = src[i];
+}
 
 // Any-hit traversal over a ray queue; each wave fetches 64 rays at a time.
 template <bool kCount>
@@ -1217,6 +1271,40 @@ __global__ void __launch_bounds__(kBlock) k_trace_any(DScene S, TraceAnyArgs A) 
       atomicAdd(&A.stats[2], c);
     }
   }
+}
+
+// The frame path's any-hit traversal: the queue holds tagged slots of the
+// frames' ray buffers.
+template <bool kCount, bool kMulti>
+__global__ void __launch_bounds__(kBlock) k_trace_any_frames(FrameSet FS, uint32_t max_rays,
+                                                             unsigned long long *stats) {
+  __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
+  const DFrame &F0 = FS.fb[0];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
+  uint32_t count = F0.counters[kCntQueue];
+  if (count > max_rays) count = max_rays;
+  TraceCnt cnt;
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&F0.counters[kCntHead], 64u);
+    base = __shfl(base, 0);
+    if (base >= count) break;
+    const uint32_t q = base + lane;
+    if (q < count) {
+      const uint32_t e = F0.queue[q];
+      const DFrame &F = FS.fb[kMulti ? (e >> kViewShift) : 0u];
+      const uint32_t idx = kMulti ? (e & kSlotMask) : e;
+      const float4 o = F.ray_o[idx], d = F.ray_d[idx];
+      const uint32_t tp = F.ray_t[idx];
+      float bt = 0.0f;
+      uint32_t bi = 0, bp = 0;
+      bool ovf = false;
+      const bool occ = traverse<true, kCount>(F.S, xyz(o), xyz(d), o.w, __float_as_uint(d.w), tp, bt, bi, bp,
+                                              stk + tid, kBlock, cnt, ovf);
+      if (occ) atomicOr(&F.mask[idx / kRaySlots], 1u << (idx % kRaySlots));
+    }
+  }
+  if (kCount) wave_add_cnt(cnt, stats);
 }
 
 template <bool kCount>
@@ -1362,6 +1450,220 @@ int upload_lights(Ctx &c, DevBuf &dst, const std::vector<ct_light> &src) {
   return CT_OK;
 }
 
+// ------------------------------------------------------------ frame plans
+int check_frame(const Ctx *c, const Accel *a, const ct_frame_params *p) {
+  if (!c || !a || !p) return fail(CT_E_INVALID, "ct_frame_render: null argument");
+  if (p->width == 0 || p->height == 0 || p->width > c->max_w || p->height > c->max_h)
+    return fail(CT_E_INVALID, "ct_frame_render: resolution exceeds the context");
+  if (p->is_active & CT_RIS_BRDF_BIT) return fail(CT_E_UNSUPPORTED, "ct_frame_render: BRDF sampling in RIS is not supported");
+  if (p->restir.max_neighbors > CT_MAX_NEIGHBORS || p->restir.min_neighbors > CT_MAX_NEIGHBORS)
+    return fail(CT_E_INVALID, "ct_frame_render: neighbour count above CT_MAX_NEIGHBORS");
+  if ((p->is_active & CT_TAA_BIT) && !p->has_previous)
+    return fail(CT_E_INVALID, "ct_frame_render: TAA needs a previous frame");
+  if (a->set.tlas_count == 0) return fail(CT_E_STATE, "ct_frame_render: empty scene (call update_scene first)");
+  if (c->device != a->device) return fail(CT_E_INVALID, "ct_frame_render: context and accel on different devices");
+  const bool pt = p->restir.indirect != 0;
+  if (pt && (p->restir.max_depth == 0 || p->restir.max_depth > CT_MAX_PT_DEPTH))
+    return fail(CT_E_INVALID, "ct_frame_render: max_depth must be in [1, CT_MAX_PT_DEPTH]");
+  if (pt && p->restir.pt_neighbor_count > CT_MAX_PT_NEIGHBORS)
+    return fail(CT_E_INVALID, "ct_frame_render: pt_neighbor_count above CT_MAX_PT_NEIGHBORS");
+  if (pt && p->restir.pt_shift_mapping != CT_SHIFT_RECONNECTION && p->restir.pt_shift_mapping != CT_SHIFT_HYBRID)
+    return fail(CT_E_UNSUPPORTED, "ct_frame_render: only the reconnection and hybrid shifts exist");
+  return CT_OK;
+}
+
+// Uploads the frame's light and motion tables and fills its record with
+// the context's own buffers (a batch later points the shared ones at its
+// arena).  Flips the image ping-pong.
+int prepare_frame(Ctx *c, Accel *a, const ct_frame_params *p, hipStream_t st, DFrame &F) {
+  int rc;
+  const bool pt = p->restir.indirect != 0;
+  if (pt && (rc = alloc_pt(*c, st))) return rc;
+  // Cube lights from the manager (packed), uploaded after each build.
+  const uint32_t ncube = a->set.cube_light_count;
+  DevBuf &cube = c->inst_pt;
+  if (c->lights_accel != a || c->lights_generation != a->set.generation) {
+    CT_HIP(hipStreamSynchronize(st));  // earlier frames may still read the old table
+    std::vector<float4> pk;
+    pack_lights(a->cube_lights_host.data(), ncube, pk);
+    if ((rc = cube.reserve(16 * pk.size()))) return rc;
+    CT_HIP(hipMemcpy(cube.p, pk.data(), 16 * pk.size(), hipMemcpyHostToDevice));
+    c->lights_accel = a;
+    c->lights_generation = a->set.generation;
+  }
+  c->cur ^= 1;
+  FrameBufs &b = c->fb;
+  std::memset(&F, 0, sizeof(F));
+  F.W = p->width; F.H = p->height; F.npix = p->width * p->height;
+  F.frame = p->frame_number; F.active = p->is_active; F.afternoon = p->is_afternoon; F.time = p->time;
+  F.n_cube = ncube; F.n_point = (uint32_t)c->point_host.size(); F.n_env = (uint32_t)c->env_host.size();
+  F.n_mats = (uint32_t)c->mats_host.size();
+  std::memcpy(F.inv_view, p->inv_view, sizeof(F.inv_view));
+  std::memcpy(F.inv_proj, p->inv_proj, sizeof(F.inv_proj));
+  F.rp = p->restir;
+  F.cfg = c->cfg;
+  F.S = scene_view(*a);
+  F.aabbs = a->set.aabbs.as<ct_aabb>();
+  F.prims = a->set.prims.as<ct_primitive>();
+  F.mats = c->materials.as<float4>();
+  F.cube = cube.as<float4>();
+  F.point = c->point_lights.as<float4>();
+  F.env = c->env_lights.as<float4>();
+  F.geo_t = a->set.geo_t.as<float>();
+  std::memcpy(F.prev_vp, p->prev_view_proj, sizeof(F.prev_vp));
+  F.dpos_c = b.di_pos[0].as<float4>(); F.dnrm_c = b.di_nrm[0].as<float4>(); F.dorg_c = b.di_org[0].as<float4>();
+  F.dhit_c = b.di_hit[0].as<uint4>();
+  F.dpos_p = b.di_pos[1].as<float4>(); F.dnrm_p = b.di_nrm[1].as<float4>(); F.dorg_p = b.di_org[1].as<float4>();
+  F.dhit_p = b.di_hit[1].as<uint4>();
+  F.rA_i = b.resA[0].as<uint4>(); F.rB_i = b.resB[0].as<float4>(); F.rC_i = b.resC[0].as<float>();
+  F.rA_p = b.resA[1].as<uint4>(); F.rB_p = b.resB[1].as<float4>(); F.rC_p = b.resC[1].as<float>();
+  F.vel = b.velocity.as<float2>();
+  F.img = b.img[c->cur].as<float4>();
+  F.img_prev = b.img[c->cur ^ 1].as<float4>();
+  F.taa = b.taa.as<float4>();
+  F.hit = b.hit.as<float4>();
+  F.pidx = b.pidx.as<uint32_t>();
+  F.mask = b.mask.as<uint32_t>();
+  F.ray_o = b.ray_o.as<float4>(); F.ray_d = b.ray_d.as<float4>(); F.ray_t = b.ray_t.as<uint32_t>();
+  F.queue = b.queue.as<uint32_t>();
+  F.counters = b.counters.as<uint32_t>();
+  F.gat = b.gather.as<float4>();
+  F.pres_o = b.pres[0].as<float4>();
+  F.pres_t = b.pres[1].as<float4>();
+  F.indirect = b.indirect.as<float4>();
+  F.pst = b.pstate.as<float4>();
+  F.ptq[0] = b.ptq[0].as<uint32_t>();
+  F.ptq[1] = b.ptq[1].as<uint32_t>();
+  // path segments reuse the primary-hit and shadow-ray buffers, which are
+  // idle while paths are traced
+  F.seg_o = b.ray_o.as<float4>();
+  F.seg_d = b.ray_d.as<float4>();
+  F.seg_hit = b.hit.as<float4>();
+  F.qtag = 0u;
+  F.taa_w = p->restir.taa_truncate_dispatch ? (F.W / 8) * 8 : F.W;
+  F.taa_h = p->restir.taa_truncate_dispatch ? (F.H / 8) * 8 : F.H;
+  return CT_OK;
+}
+
+// Block tables of a set of frames, one after another: primary-ray tiles
+// (16x16 pixels per block), pixel blocks, TAA blocks.
+void build_tables(const DFrame *fr, uint32_t n, std::vector<uint2> &out, uint32_t counts[3]) {
+  out.clear();
+  for (uint32_t v = 0; v < n; ++v) {
+    const uint32_t tiles = ((fr[v].W + 15) / 16) * ((fr[v].H + 15) / 16);
+    for (uint32_t t = 0; t < tiles; ++t) out.push_back(make_uint2(v, t));
+  }
+  counts[0] = (uint32_t)out.size();
+  for (uint32_t v = 0; v < n; ++v)
+    for (uint32_t q = 0; q < fr[v].npix; q += kBlock) out.push_back(make_uint2(v, q));
+  counts[1] = (uint32_t)out.size() - counts[0];
+  for (uint32_t v = 0; v < n; ++v)
+    for (uint32_t q = 0; q < fr[v].taa_w * fr[v].taa_h; q += kBlock) out.push_back(make_uint2(v, q));
+  counts[2] = (uint32_t)out.size() - counts[0] - counts[1];
+}
+
+// Everything the launch sequence of a set of frames needs.
+struct Launch {
+  const DFrame *fb = nullptr;  // device frame records
+  const uint2 *blk = nullptr;  // [tiles | pixels | taa] block tables
+  uint32_t blocks[3] = {0, 0, 0};
+  uint32_t total_px = 0;       // pixels over all frames
+  uint32_t pt_depth = 0;       // deepest path of the set (0: no indirect light)
+  uint32_t max_neighbors = 0;  // most spatial neighbours of the set
+  uint32_t trace_blocks = 0;   // grid of the persistent traversal kernels
+  bool multi = false;          // queue entries carry the frame index
+  uint32_t *counters = nullptr;
+  unsigned long long *stats = nullptr;
+};
+
+template <bool kCount, bool kMulti>
+void launch_any_frames(const Launch &L, FrameSet fs, uint32_t max_rays, hipStream_t st) {
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntHead, 1);
+  const uint32_t grid = std::min<uint32_t>(std::max<uint32_t>(nblocks(max_rays), 1), L.trace_blocks);
+  hipLaunchKernelGGL((k_trace_any_frames<kCount, kMulti>), dim3(grid), dim3(kBlock), 0, st, fs, max_rays, L.stats);
+}
+void launch_any(const Launch &L, FrameSet fs, uint32_t max_rays, hipStream_t st) {
+  if (L.stats)
+    L.multi ? launch_any_frames<true, true>(L, fs, max_rays, st) : launch_any_frames<true, false>(L, fs, max_rays, st);
+  else
+    L.multi ? launch_any_frames<false, true>(L, fs, max_rays, st) : launch_any_frames<false, false>(L, fs, max_rays, st);
+}
+
+// The command list of App::draw (main.cpp:1436-1703) for every frame of the
+// set: the same kernels, each launched once over all frames.
+template <class Mark, class SegMark>
+int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark) {
+  int rc;
+  const FrameSet tiles{L.fb, L.blk}, pix{L.fb, L.blk + L.blocks[0]}, taa{L.fb, L.blk + L.blocks[0] + L.blocks[1]};
+  const uint32_t np = L.total_px, nb = L.blocks[1];
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, 0, 8);
+  if (L.stats) hipLaunchKernelGGL(k_reset_u64, dim3(1), dim3(64), 0, st, L.stats, 12);
+  if ((rc = mark())) return rc;  // kMarkStart
+  if (L.stats)
+    hipLaunchKernelGGL(k_primary<true>, dim3(L.blocks[0]), dim3(kBlock), 0, st, tiles, L.stats + 3);
+  else
+    hipLaunchKernelGGL(k_primary<false>, dim3(L.blocks[0]), dim3(kBlock), 0, st, tiles, L.stats);
+  if ((rc = mark())) return rc;  // kMarkPrimary
+  hipLaunchKernelGGL(k_ris, dim3(nb), dim3(kBlock), 0, st, pix);
+  if ((rc = mark())) return rc;  // kMarkRis
+  launch_any(L, pix, np, st);
+  if ((rc = mark())) return rc;  // kMarkTraceRis
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntQueue, 1);
+  hipLaunchKernelGGL(k_temporal_emit, dim3(nb), dim3(kBlock), 0, st, pix);
+  if ((rc = mark())) return rc;  // kMarkTemporalEmit
+  launch_any(L, pix, 2 * np, st);
+  if ((rc = mark())) return rc;  // kMarkTraceTemporal
+  hipLaunchKernelGGL(k_temporal_finish, dim3(nb), dim3(kBlock), 0, st, pix);
+  if ((rc = mark())) return rc;  // kMarkTemporalFinish
+  if (L.pt_depth) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntPtQueue, 3);
+    hipLaunchKernelGGL(k_pt_begin, dim3(nb), dim3(kBlock), 0, st, pix);
+    const uint32_t grid = std::min<uint32_t>(nblocks(np), L.trace_blocks);
+    for (uint32_t d = 0; d < L.pt_depth; ++d) {
+      // the next queue and the fetch counter start at zero
+      hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntPtQueue + (int)((d + 1) & 1u), 1);
+      hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntPtHead, 1);
+      if ((rc = seg_mark())) return rc;
+      if (L.stats && L.multi)
+        hipLaunchKernelGGL((k_trace_segments<true, true>), dim3(grid), dim3(kBlock), 0, st, pix, d, L.stats + 6);
+      else if (L.stats)
+        hipLaunchKernelGGL((k_trace_segments<true, false>), dim3(grid), dim3(kBlock), 0, st, pix, d, L.stats + 6);
+      else if (L.multi)
+        hipLaunchKernelGGL((k_trace_segments<false, true>), dim3(grid), dim3(kBlock), 0, st, pix, d, L.stats);
+      else
+        hipLaunchKernelGGL((k_trace_segments<false, false>), dim3(grid), dim3(kBlock), 0, st, pix, d, L.stats);
+      if ((rc = seg_mark())) return rc;
+      if (L.multi)
+        hipLaunchKernelGGL(k_pt_step<true>, dim3(nblocks(np)), dim3(kBlock), 0, st, pix, d);
+      else
+        hipLaunchKernelGGL(k_pt_step<false>, dim3(nblocks(np)), dim3(kBlock), 0, st, pix, d);
+    }
+    if (L.stats)
+      hipLaunchKernelGGL(k_pt_end<true>, dim3(nb), dim3(kBlock), 0, st, pix, L.stats);
+    else
+      hipLaunchKernelGGL(k_pt_end<false>, dim3(nb), dim3(kBlock), 0, st, pix, L.stats);
+  }
+  if ((rc = mark())) return rc;  // kMarkPt
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntQueue, 1);
+  hipLaunchKernelGGL(k_spatial<true>, dim3(nb), dim3(kBlock), 0, st, pix);
+  if ((rc = mark())) return rc;  // kMarkSpatialEmit
+  launch_any(L, pix, 2 * L.max_neighbors * np, st);
+  if ((rc = mark())) return rc;  // kMarkTraceSpatial
+  hipLaunchKernelGGL(k_spatial<false>, dim3(nb), dim3(kBlock), 0, st, pix);
+  if ((rc = mark())) return rc;  // kMarkSpatialFinish
+  if (L.pt_depth) {
+    if (L.stats)
+      hipLaunchKernelGGL(k_pt_spatial<true>, dim3(nb), dim3(kBlock), 0, st, pix, L.stats);
+    else
+      hipLaunchKernelGGL(k_pt_spatial<false>, dim3(nb), dim3(kBlock), 0, st, pix, L.stats);
+  }
+  if ((rc = mark())) return rc;  // kMarkPtSpatial
+  if (L.blocks[2]) hipLaunchKernelGGL(k_taa, dim3(L.blocks[2]), dim3(kBlock), 0, st, taa);
+  if ((rc = mark())) return rc;  // kMarkTaa
+  CT_HIP(hipGetLastError());
+  return CT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1471,6 +1773,254 @@ int ct_ctx_reset(ct_ctx *c, void *stream) {
   return zero_frame(*c, st);
 }
 
+struct ct_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<ct_ctx *> ctxs;
+  std::vector<ct_accel *> accels;
+  DevBuf frames, blk, queue, counters, seg_o, seg_d, seg_hit, ptq[2];
+  uint32_t capacity_px = 0;
+  std::vector<uint32_t> tables_key, graph_key;
+  uint32_t blocks[3] = {0, 0, 0};
+  DFrame *pinned = nullptr;  // [2][n] frame records being uploaded
+  hipEvent_t slot_free[2] = {};
+  bool slot_used[2] = {false, false};
+  int slot = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t blocks_per_cu = 8;
+  uint32_t *counters_pinned = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  hipStream_t last_stream = nullptr;
+  bool pending = false;
+  uint64_t primary_rays = 0;
+};
+
+namespace {
+int cu_count(int device) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, device) == hipSuccess && p.multiProcessorCount > 0) return p.multiProcessorCount;
+  return 256;
+}
+void drop_graph(ct_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_key.clear();
+}
+}  // namespace
+
+int ct_batch_create(uint32_t n, ct_ctx *const *ctxs, ct_accel *const *accels, ct_batch **out) {
+  if (!ctxs || !accels || !out || n == 0) return fail(CT_E_INVALID, "ct_batch_create: null argument or no views");
+  *out = nullptr;
+  if (n > (1u << (32 - kViewShift))) return fail(CT_E_INVALID, "ct_batch_create: at most 64 views");
+  uint64_t cap = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!ctxs[i] || !accels[i]) return fail(CT_E_INVALID, "ct_batch_create: null view");
+    if (ctxs[i]->device != ctxs[0]->device || accels[i]->device != ctxs[0]->device)
+      return fail(CT_E_INVALID, "ct_batch_create: views on different devices");
+    for (uint32_t j = 0; j < i; ++j)
+      if (ctxs[j] == ctxs[i]) return fail(CT_E_INVALID, "ct_batch_create: a context appears twice");
+    const uint64_t np = (uint64_t)ctxs[i]->max_w * ctxs[i]->max_h;
+    if (np * kRaySlots > kSlotMask) return fail(CT_E_INVALID, "ct_batch_create: a view above 2^22 pixels");
+    cap += np;
+  }
+  if (cap * kRaySlots >= (1ull << 32)) return fail(CT_E_INVALID, "ct_batch_create: too many pixels");
+  CT_HIP(hipSetDevice(ctxs[0]->device));
+  ct_batch *b = new ct_batch();
+  b->device = ctxs[0]->device;
+  b->ctxs.assign(ctxs, ctxs + n);
+  b->accels.assign(accels, accels + n);
+  b->capacity_px = (uint32_t)cap;
+  int rc = CT_OK;
+  void *pin = nullptr, *cpin = nullptr;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      (rc = b->frames.reserve(sizeof(DFrame) * n)) || (rc = b->queue.reserve(4 * cap * kRaySlots)) ||
+      (rc = b->counters.reserve(64 * sizeof(uint32_t))) || (rc = b->seg_o.reserve(16 * cap)) ||
+      (rc = b->seg_d.reserve(16 * cap)) || (rc = b->seg_hit.reserve(16 * cap)) || (rc = b->ptq[0].reserve(4 * cap)) ||
+      (rc = b->ptq[1].reserve(4 * cap)) ||
+      hipHostMalloc(&pin, 2 * sizeof(DFrame) * n, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&cpin, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+      hipEventCreate(&b->slot_free[0]) != hipSuccess || hipEventCreate(&b->slot_free[1]) != hipSuccess ||
+      hipEventCreate(&b->t0) != hipSuccess || hipEventCreate(&b->t1) != hipSuccess) {
+    b->pinned = static_cast<DFrame *>(pin);
+    b->counters_pinned = static_cast<uint32_t *>(cpin);
+    ct_batch_destroy(b);
+    return rc ? rc : fail(CT_E_HIP, "ct_batch_create: HIP allocation failed");
+  }
+  b->pinned = static_cast<DFrame *>(pin);
+  b->counters_pinned = static_cast<uint32_t *>(cpin);
+  b->blocks_per_cu = 8;
+  *out = b;
+  return CT_OK;
+}
+
+int ct_batch_destroy(ct_batch *b) {
+  if (!b) return fail(CT_E_INVALID, "ct_batch_destroy: null handle");
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->last_stream) (void)hipStreamSynchronize(b->last_stream);
+  drop_graph(b);
+  for (hipEvent_t e : {b->slot_free[0], b->slot_free[1], b->t0, b->t1})
+    if (e) (void)hipEventDestroy(e);
+  if (b->pinned) (void)hipHostFree(b->pinned);
+  if (b->counters_pinned) (void)hipHostFree(b->counters_pinned);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return CT_OK;
+}
+
+int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, void *stream) {
+  if (!b || !params) return fail(CT_E_INVALID, "ct_batch_render: null argument");
+  const uint32_t n = (uint32_t)b->ctxs.size();
+  int rc;
+  for (uint32_t i = 0; i < n; ++i)
+    if ((rc = check_frame(b->ctxs[i], b->accels[i], &params[i]))) return rc;
+  CT_HIP(hipSetDevice(b->device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : b->stream;
+  // the host records of this slot were uploaded two renders ago
+  if (b->slot_used[b->slot]) CT_HIP(hipEventSynchronize(b->slot_free[b->slot]));
+  DFrame *rec = b->pinned + (size_t)b->slot * n;
+  uint32_t pt_depth = 0, max_nb = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    DFrame &F = rec[i];
+    if ((rc = prepare_frame(b->ctxs[i], b->accels[i], &params[i], st, F))) return rc;
+    F.queue = b->queue.as<uint32_t>();
+    F.counters = b->counters.as<uint32_t>();
+    F.seg_o = b->seg_o.as<float4>();
+    F.seg_d = b->seg_d.as<float4>();
+    F.seg_hit = b->seg_hit.as<float4>();
+    F.ptq[0] = b->ptq[0].as<uint32_t>();
+    F.ptq[1] = b->ptq[1].as<uint32_t>();
+    F.qtag = n > 1 ? (i << kViewShift) : 0u;
+    if (F.rp.indirect) pt_depth = std::max(pt_depth, F.rp.max_depth);
+    max_nb = std::max(max_nb, F.rp.max_neighbors);
+  }
+  std::vector<uint32_t> tkey;
+  for (uint32_t i = 0; i < n; ++i) {
+    tkey.push_back(rec[i].W);
+    tkey.push_back(rec[i].H);
+    tkey.push_back(rec[i].taa_w);
+    tkey.push_back(rec[i].taa_h);
+  }
+  if (tkey != b->tables_key) {
+    std::vector<uint2> tbl;
+    build_tables(rec, n, tbl, b->blocks);
+    CT_HIP(hipStreamSynchronize(st));
+    if ((rc = b->blk.reserve(sizeof(uint2) * tbl.size()))) return rc;
+    CT_HIP(hipMemcpy(b->blk.p, tbl.data(), sizeof(uint2) * tbl.size(), hipMemcpyHostToDevice));
+    b->tables_key = tkey;
+    drop_graph(b);
+  }
+  CT_HIP(hipMemcpyAsync(b->frames.p, rec, sizeof(DFrame) * n, hipMemcpyHostToDevice, st));
+  CT_HIP(hipEventRecord(b->slot_free[b->slot], st));
+  b->slot_used[b->slot] = true;
+  b->slot ^= 1;
+  Launch L;
+  L.fb = b->frames.as<DFrame>();
+  L.blk = b->blk.as<uint2>();
+  std::memcpy(L.blocks, b->blocks, sizeof(L.blocks));
+  L.total_px = 0;
+  for (uint32_t i = 0; i < n; ++i) L.total_px += rec[i].npix;
+  L.pt_depth = pt_depth;
+  L.max_neighbors = max_nb;
+  L.trace_blocks = (uint32_t)cu_count(b->device) * b->blocks_per_cu;
+  L.multi = n > 1;
+  L.counters = b->counters.as<uint32_t>();
+  L.stats = nullptr;
+  auto none = []() -> int { return CT_OK; };
+  CT_HIP(hipEventRecord(b->t0, st));
+  if (use_graph) {
+    std::vector<uint32_t> gkey = tkey;
+    gkey.insert(gkey.end(), {L.total_px, pt_depth, max_nb, L.trace_blocks, (uint32_t)L.multi});
+    if (!b->exec || gkey != b->graph_key) {
+      drop_graph(b);
+      CT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_frames(L, st, none, none);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(st, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(CT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      b->graph = g;
+      CT_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->graph_key = gkey;
+    }
+    CT_HIP(hipGraphLaunch(b->exec, st));
+  } else if ((rc = enqueue_frames(L, st, none, none))) {
+    return rc;
+  }
+  CT_HIP(hipEventRecord(b->t1, st));
+  CT_HIP(hipMemcpyAsync(b->counters_pinned, b->counters.p, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  b->pending = true;
+  b->last_stream = st;
+  b->primary_rays = L.total_px;
+  for (uint32_t i = 0; i < n; ++i) {
+    ct_ctx *c = b->ctxs[i];
+    c->stats_pending = false;
+    c->stats = ct_frame_stats{};
+    c->stream_last = st;
+    c->last_w = rec[i].W;
+    c->last_h = rec[i].H;
+  }
+  return CT_OK;
+}
+
+int ct_batch_get_stats(ct_batch *b, ct_frame_stats *out) {
+  if (!b || !out) return fail(CT_E_INVALID, "ct_batch_get_stats: null argument");
+  *out = ct_frame_stats{};
+  if (!b->pending) return CT_OK;
+  CT_HIP(hipSetDevice(b->device));
+  CT_HIP(hipStreamSynchronize(b->last_stream));
+  const uint32_t *cnt = b->counters_pinned;
+  out->primary_rays = b->primary_rays;
+  out->shadow_rays_ris = cnt[kCntRis];
+  out->shadow_rays_temporal = cnt[kCntTemporal];
+  out->shadow_rays_spatial = cnt[kCntSpatial];
+  out->shadow_rays_pt = cnt[kCntPtShadow];
+  out->bounce_rays = cnt[kCntPtBounce];
+  out->shadow_rays = (uint64_t)cnt[kCntRis] + cnt[kCntTemporal] + cnt[kCntSpatial] + cnt[kCntPtShadow];
+  float ms = 0.0f;
+  CT_HIP(hipEventElapsedTime(&ms, b->t0, b->t1));
+  out->ms_total = ms;
+  return CT_OK;
+}
+
+int ct_batch_autotune(ct_batch *b, const ct_frame_params *params, uint32_t repeats, ct_batch_tuning *out) {
+  if (!b || !params || repeats == 0) return fail(CT_E_INVALID, "ct_batch_autotune: bad argument");
+  const uint32_t cand[] = {2, 4, 8, 16, 32};
+  float best = 0.0f;
+  uint32_t best_c = b->blocks_per_cu;
+  int rc;
+  for (uint32_t c : cand) {
+    b->blocks_per_cu = c;
+    float tot = 0.0f;
+    for (uint32_t r = 0; r < repeats + 1; ++r) {  // the first render captures the graph
+      if ((rc = ct_batch_render(b, params, 1, nullptr))) return rc;
+      ct_frame_stats s;
+      if ((rc = ct_batch_get_stats(b, &s))) return rc;
+      if (r > 0) tot += s.ms_total;
+    }
+    const float ms = tot / (float)repeats;
+    if (best == 0.0f || ms < best) {
+      best = ms;
+      best_c = c;
+    }
+  }
+  b->blocks_per_cu = best_c;
+  drop_graph(b);
+  if (out) {
+    out->blocks_per_cu = best_c;
+    out->ms = best;
+    out->candidates = (uint32_t)(sizeof(cand) / sizeof(cand[0]));
+    out->pad = 0;
+  }
+  return CT_OK;
+}
+
 int ct_ctx_set_instrumentation(ct_ctx *c, uint32_t flags) {
   if (!c) return fail(CT_E_INVALID, "set_instrumentation: null handle");
   c->instrument = flags;
@@ -1478,104 +2028,44 @@ int ct_ctx_set_instrumentation(ct_ctx *c, uint32_t flags) {
 }
 
 int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stream) {
-  if (!c || !a || !p) return fail(CT_E_INVALID, "ct_frame_render: null argument");
-  if (p->width == 0 || p->height == 0 || p->width > c->max_w || p->height > c->max_h)
-    return fail(CT_E_INVALID, "ct_frame_render: resolution exceeds the context");
-  if (p->is_active & CT_RIS_BRDF_BIT) return fail(CT_E_UNSUPPORTED, "ct_frame_render: BRDF sampling in RIS is not supported");
-  if (p->restir.max_neighbors > CT_MAX_NEIGHBORS || p->restir.min_neighbors > CT_MAX_NEIGHBORS)
-    return fail(CT_E_INVALID, "ct_frame_render: neighbour count above CT_MAX_NEIGHBORS");
-  if ((p->is_active & CT_TAA_BIT) && !p->has_previous)
-    return fail(CT_E_INVALID, "ct_frame_render: TAA needs a previous frame");
-  if (a->set.tlas_count == 0) return fail(CT_E_STATE, "ct_frame_render: empty scene (call update_scene first)");
-  const bool pt = p->restir.indirect != 0;
-  if (pt && (p->restir.max_depth == 0 || p->restir.max_depth > CT_MAX_PT_DEPTH))
-    return fail(CT_E_INVALID, "ct_frame_render: max_depth must be in [1, CT_MAX_PT_DEPTH]");
-  if (pt && p->restir.pt_neighbor_count > CT_MAX_PT_NEIGHBORS)
-    return fail(CT_E_INVALID, "ct_frame_render: pt_neighbor_count above CT_MAX_PT_NEIGHBORS");
-  if (pt && p->restir.pt_shift_mapping != CT_SHIFT_RECONNECTION && p->restir.pt_shift_mapping != CT_SHIFT_HYBRID)
-    return fail(CT_E_UNSUPPORTED, "ct_frame_render: only the reconnection and hybrid shifts exist");
+  int rc;
+  if ((rc = check_frame(c, a, p))) return rc;
   CT_HIP(hipSetDevice(c->device));
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
-  int rc;
-  if (pt && (rc = alloc_pt(*c, st))) return rc;
-  // Cube lights from the manager (packed), re-uploaded on a new build.
-  static thread_local std::vector<float4> pk;
-  const uint32_t ncube = a->set.cube_light_count;
-  pack_lights(a->cube_lights_host.data(), ncube, pk);
-  DevBuf &cube = c->inst_pt;  // reused: [packed cube lights | per-instance matrices]
-  const size_t ni = a->instances.size();
-  const size_t cube_bytes = 16 * pk.size(), pt_bytes = 64 * std::max<size_t>(ni, 1);
-  if ((rc = cube.reserve(cube_bytes + pt_bytes))) return rc;
-  // per-instance prev_view_proj * (prev_model * inverse(model)); the scene
-  // is static between builds so prev_model == model.
-  std::vector<float> ptm(16 * std::max<size_t>(ni, 1), 0.0f);
-  for (size_t i = 0; i < ni; ++i) {
-    float g[16];
-    const float *t = a->instances[i].transform, *w = &a->w2o[16 * i];
-    for (int cc = 0; cc < 4; ++cc)
-      for (int r = 0; r < 4; ++r)
-        g[cc * 4 + r] = t[0 * 4 + r] * w[cc * 4 + 0] + t[1 * 4 + r] * w[cc * 4 + 1] + t[2 * 4 + r] * w[cc * 4 + 2] +
-                        t[3 * 4 + r] * w[cc * 4 + 3];
-    const float *pv = p->prev_view_proj;
-    for (int cc = 0; cc < 4; ++cc)
-      for (int r = 0; r < 4; ++r)
-        ptm[16 * i + cc * 4 + r] = pv[0 * 4 + r] * g[cc * 4 + 0] + pv[1 * 4 + r] * g[cc * 4 + 1] +
-                                   pv[2 * 4 + r] * g[cc * 4 + 2] + pv[3 * 4 + r] * g[cc * 4 + 3];
-  }
-  CT_HIP(hipMemcpyAsync(cube.p, pk.data(), cube_bytes, hipMemcpyHostToDevice, st));
-  CT_HIP(hipMemcpyAsync(static_cast<char *>(cube.p) + cube_bytes, ptm.data(), 16 * sizeof(float) * ni,
-                        hipMemcpyHostToDevice, st));
-  c->cur ^= 1;
-  FrameBufs &b = c->fb;
   DFrame F;
-  std::memset(&F, 0, sizeof(F));
-  F.W = p->width; F.H = p->height; F.npix = p->width * p->height;
-  F.frame = p->frame_number; F.active = p->is_active; F.afternoon = p->is_afternoon; F.time = p->time;
-  F.n_cube = ncube; F.n_point = (uint32_t)c->point_host.size(); F.n_env = (uint32_t)c->env_host.size();
-  F.n_mats = (uint32_t)c->mats_host.size();
-  std::memcpy(F.inv_view, p->inv_view, sizeof(F.inv_view));
-  std::memcpy(F.inv_proj, p->inv_proj, sizeof(F.inv_proj));
-  F.rp = p->restir;
-  F.cfg = c->cfg;
-  F.S = scene_view(*a);
-  F.aabbs = a->set.aabbs.as<ct_aabb>();
-  F.prims = a->set.prims.as<ct_primitive>();
-  F.mats = c->materials.as<float4>();
-  F.cube = cube.as<float4>();
-  F.point = c->point_lights.as<float4>();
-  F.env = c->env_lights.as<float4>();
-  F.pt = reinterpret_cast<const float *>(static_cast<char *>(cube.p) + cube_bytes);
-  F.dpos_c = b.di_pos[0].as<float4>(); F.dnrm_c = b.di_nrm[0].as<float4>(); F.dorg_c = b.di_org[0].as<float4>();
-  F.dhit_c = b.di_hit[0].as<uint4>();
-  F.dpos_p = b.di_pos[1].as<float4>(); F.dnrm_p = b.di_nrm[1].as<float4>(); F.dorg_p = b.di_org[1].as<float4>();
-  F.dhit_p = b.di_hit[1].as<uint4>();
-  F.rA_i = b.resA[0].as<uint4>(); F.rB_i = b.resB[0].as<float4>(); F.rC_i = b.resC[0].as<float>();
-  F.rA_p = b.resA[1].as<uint4>(); F.rB_p = b.resB[1].as<float4>(); F.rC_p = b.resC[1].as<float>();
-  F.vel = b.velocity.as<float2>();
-  F.img = b.img[c->cur].as<float4>();
-  F.img_prev = b.img[c->cur ^ 1].as<float4>();
-  F.taa = b.taa.as<float4>();
-  F.hit = b.hit.as<float4>();
-  F.pidx = b.pidx.as<uint32_t>();
-  F.mask = b.mask.as<uint32_t>();
-  F.ray_o = b.ray_o.as<float4>(); F.ray_d = b.ray_d.as<float4>(); F.ray_t = b.ray_t.as<uint32_t>();
-  F.queue = b.queue.as<uint32_t>();
-  F.counters = b.counters.as<uint32_t>();
-  F.gat = b.gather.as<float4>();
-  F.pres_o = b.pres[0].as<float4>();
-  F.pres_t = b.pres[1].as<float4>();
-  F.indirect = b.indirect.as<float4>();
-  F.pst = b.pstate.as<float4>();
-  F.ptq[0] = b.ptq[0].as<uint32_t>();
-  F.ptq[1] = b.ptq[1].as<uint32_t>();
+  if ((rc = prepare_frame(c, a, p, st, F))) return rc;
+  FrameBufs &b = c->fb;
   unsigned long long *stats = (c->instrument & 2u) ? b.prev_res_copy.as<unsigned long long>() : nullptr;
+  // the frame record and its block tables on the device
+  if ((rc = c->frame_dev.reserve(sizeof(DFrame)))) return rc;
+  const uint32_t key[4] = {F.W, F.H, F.taa_w, F.taa_h};
+  if (std::memcmp(key, c->blk_key, sizeof(key)) != 0) {
+    std::vector<uint2> tbl;
+    build_tables(&F, 1, tbl, c->blk_counts);
+    if ((rc = c->blk_dev.reserve(sizeof(uint2) * tbl.size()))) return rc;
+    CT_HIP(hipStreamSynchronize(st));
+    CT_HIP(hipMemcpy(c->blk_dev.p, tbl.data(), sizeof(uint2) * tbl.size(), hipMemcpyHostToDevice));
+    std::memcpy(c->blk_key, key, sizeof(key));
+  }
+  hipLaunchKernelGGL(k_store_frame, dim3(1), dim3(64), 0, st, F, c->frame_dev.as<DFrame>());
+  Launch L;
+  L.fb = c->frame_dev.as<DFrame>();
+  L.blk = c->blk_dev.as<uint2>();
+  std::memcpy(L.blocks, c->blk_counts, sizeof(L.blocks));
+  L.total_px = F.npix;
+  L.pt_depth = F.rp.indirect ? F.rp.max_depth : 0u;
+  L.max_neighbors = F.rp.max_neighbors;
+  L.trace_blocks = (uint32_t)trace_grid();
+  L.multi = false;
+  L.counters = F.counters;
+  L.stats = stats;
   const bool ev = (c->instrument & 1u) != 0;
   int ne = 0, nseg = kMarkCount;
+  c->n_seg = 0;
   auto mark = [&]() -> int {
     if (ev) CT_HIP(hipEventRecord(c->ev[ne++], st));
     return CT_OK;
   };
-  c->n_seg = 0;
   auto seg_mark = [&]() -> int {
     if (ev && nseg < 48 && c->n_seg < 32) {
       CT_HIP(hipEventRecord(c->ev[nseg], st));
@@ -1583,86 +2073,14 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
     }
     return CT_OK;
   };
-  const uint32_t np = F.npix;
-  TraceAnyArgs ta;
-  std::memset(&ta, 0, sizeof(ta));
-  ta.ray_o = F.ray_o; ta.ray_d = F.ray_d; ta.ray_t = F.ray_t; ta.queue = F.queue;
-  ta.count = F.counters + kCntQueue; ta.head = F.counters + kCntHead; ta.mask = F.mask; ta.stats = stats;
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, 0, 8);
-  if (stats) CT_HIP(hipMemsetAsync(stats, 0, 12 * sizeof(unsigned long long), st));
-  if ((rc = mark())) return rc;  // kMarkStart
-  const uint32_t tiles = ((F.W + 15) / 16) * ((F.H + 15) / 16);
-  if (stats)
-    hipLaunchKernelGGL(k_primary<true>, dim3(tiles), dim3(kBlock), 0, st, F, stats + 3);
-  else
-    hipLaunchKernelGGL(k_primary<false>, dim3(tiles), dim3(kBlock), 0, st, F, stats);
-  if ((rc = mark())) return rc;  // kMarkPrimary
-  hipLaunchKernelGGL(k_ris, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
-  if ((rc = mark())) return rc;  // kMarkRis
-  ta.max_rays = np;
-  if ((rc = launch_trace_any(F.S, ta, st))) return rc;
-  if ((rc = mark())) return rc;  // kMarkTraceRis
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, kCntQueue, 1);
-  hipLaunchKernelGGL(k_temporal_emit, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
-  if ((rc = mark())) return rc;  // kMarkTemporalEmit
-  ta.max_rays = 2 * np;
-  if ((rc = launch_trace_any(F.S, ta, st))) return rc;
-  if ((rc = mark())) return rc;  // kMarkTraceTemporal
-  hipLaunchKernelGGL(k_temporal_finish, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
-  if ((rc = mark())) return rc;  // kMarkTemporalFinish
-  if (pt) {
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, kCntPtQueue, 3);
-    hipLaunchKernelGGL(k_pt_begin, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
-    const uint32_t grid = std::min<uint32_t>(nblocks(np), (uint32_t)trace_grid());
-    for (uint32_t d = 0; d < p->restir.max_depth; ++d) {
-      // the next queue and the fetch counter start at zero
-      hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, kCntPtQueue + (int)((d + 1) & 1u), 1);
-      hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, kCntPtHead, 1);
-      if ((rc = seg_mark())) return rc;
-      if (stats)
-        hipLaunchKernelGGL(k_trace_segments<true>, dim3(grid), dim3(kBlock), 0, st, F.S, F.ray_o, F.ray_d,
-                           F.counters + kCntPtQueue + (d & 1u), F.counters + kCntPtHead, F.hit, stats + 6);
-      else
-        hipLaunchKernelGGL(k_trace_segments<false>, dim3(grid), dim3(kBlock), 0, st, F.S, F.ray_o, F.ray_d,
-                           F.counters + kCntPtQueue + (d & 1u), F.counters + kCntPtHead, F.hit, stats);
-      if ((rc = seg_mark())) return rc;
-      hipLaunchKernelGGL(k_pt_step, dim3(nblocks(np)), dim3(kBlock), 0, st, F, d);
-    }
-    if (stats)
-      hipLaunchKernelGGL(k_pt_end<true>, dim3(nblocks(np)), dim3(kBlock), 0, st, F, stats);
-    else
-      hipLaunchKernelGGL(k_pt_end<false>, dim3(nblocks(np)), dim3(kBlock), 0, st, F, stats);
-  }
-  if ((rc = mark())) return rc;  // kMarkPt
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, F.counters, kCntQueue, 1);
-  hipLaunchKernelGGL(k_spatial<true>, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
-  if ((rc = mark())) return rc;  // kMarkSpatialEmit
-  ta.max_rays = 2 * p->restir.max_neighbors * np;
-  if ((rc = launch_trace_any(F.S, ta, st))) return rc;
-  if ((rc = mark())) return rc;  // kMarkTraceSpatial
-  hipLaunchKernelGGL(k_spatial<false>, dim3(nblocks(np)), dim3(kBlock), 0, st, F);
-  if ((rc = mark())) return rc;  // kMarkSpatialFinish
-  if (pt) {
-    if (stats)
-      hipLaunchKernelGGL(k_pt_spatial<true>, dim3(nblocks(np)), dim3(kBlock), 0, st, F, stats);
-    else
-      hipLaunchKernelGGL(k_pt_spatial<false>, dim3(nblocks(np)), dim3(kBlock), 0, st, F, stats);
-  }
-  if ((rc = mark())) return rc;  // kMarkPtSpatial
-  if (p->is_active & CT_TAA_BIT) {
-    uint32_t tw = p->restir.taa_truncate_dispatch ? (F.W / 8) * 8 : F.W;
-    uint32_t th = p->restir.taa_truncate_dispatch ? (F.H / 8) * 8 : F.H;
-    if (tw && th) hipLaunchKernelGGL(k_taa, dim3(nblocks((uint64_t)tw * th)), dim3(kBlock), 0, st, F, tw, th);
-  }
-  if ((rc = mark())) return rc;  // kMarkTaa
-  CT_HIP(hipGetLastError());
+  if ((rc = enqueue_frames(L, st, mark, seg_mark))) return rc;
   c->n_events = ne;
   // Counters for the statistics (copied once the frame is done).
   CT_HIP(hipMemcpyAsync(c->counters_pinned, F.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   if (stats) CT_HIP(hipMemcpyAsync(c->counters_pinned + 8, stats, 12 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
   c->stats_pending = true;
   c->stats = ct_frame_stats{};
-  c->stats.primary_rays = np;
+  c->stats.primary_rays = F.npix;
   c->stream_last = st;
   c->last_w = F.W;
   c->last_h = F.H;

@@ -115,6 +115,10 @@ struct ct_scene {
   float gmin[3] = {0, 0, 0}, gmax[3] = {0, 0, 0};
 };
 
+namespace {
+int add_cube_lights(ct_ctx *ctx, uint32_t nl);
+}  // namespace
+
 extern "C" {
 
 const char *ct_last_error_message(void) { return ct::g_error.c_str(); }
@@ -413,8 +417,25 @@ int ct_scene_load(const ct_scene *s, ct_accel *a, ct_ctx *ctx, void *stream) {
   for (int k = 0; k < 16; ++k) t.u.blas_build_from_cpu.transform[k] = (k % 5 == 0) ? 1.0f : 0.0f;
   if ((rc = ct_accel_task_queue_add(a, &t))) return rc;
   if ((rc = ct_accel_update_scene(a, 1, stream, nullptr))) return rc;
-  // load_lights, main.cpp:418-438 (integer-division pdfs) with
-  // brdf_count = BRDF_SAMPLING_COUNT when cube lights exist (main.cpp:1317-1320).
+  return add_cube_lights(ctx, nl);
+}
+
+int ct_scene_bind(const ct_scene *s, ct_ctx *ctx) {
+  if (!s || !ctx) return fail(CT_E_INVALID, "ct_scene_bind: null argument");
+  if (!ctx->mats_host.empty()) return fail(CT_E_STATE, "ct_scene_bind: the context already holds materials");
+  uint32_t base = 0;
+  int rc;
+  if ((rc = ct_ctx_append_materials(ctx, s->mats.data(), (uint32_t)s->mats.size(), &base))) return rc;
+  return add_cube_lights(ctx, (uint32_t)s->lights.size());
+}
+
+}  // extern "C"
+
+namespace {
+// load_lights, main.cpp:418-438 (integer-division pdfs) with
+// brdf_count = BRDF_SAMPLING_COUNT when cube lights exist (main.cpp:1317-1320).
+int add_cube_lights(ct_ctx *ctx, uint32_t nl) {
+  int rc;
   ct_light_config cfg;
   if ((rc = ct_ctx_get_light_config(ctx, &cfg))) return rc;
   cfg.cube_light_count += nl;
@@ -431,5 +452,4 @@ int ct_scene_load(const ct_scene *s, ct_accel *a, ct_ctx *ctx, void *stream) {
   cfg.brdf_pdf = cfg.brdf_count == 0 || geo == 0 ? 0.0f : (float)(cfg.brdf_count / geo);
   return ct_ctx_set_light_config(ctx, &cfg);
 }
-
-}  // extern "C"
+}  // namespace

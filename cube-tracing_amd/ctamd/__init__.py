@@ -225,6 +225,10 @@ class AdjointDesc(C.Structure):
     _fields_ = [("n_views", u32), ("spp", u32), ("max_depth", u32), ("seed", u32)]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("blocks_per_cu", u32), ("ms", f32), ("candidates", u32), ("pad", u32)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("grid", u32 * 3), ("chunk", u32 * 3), ("occupied_count", u32), ("emissive_count", u32),
                 ("le_min", f32), ("le_max", f32), ("albedo_min", f32), ("albedo_max", f32), ("seed", u32),
@@ -242,7 +246,7 @@ STRUCT_SIZES = {"ct_aabb": (Aabb, 24), "ct_instance": (Instance, 72), "ct_primit
                 "ct_material": (Material, 84), "ct_light": (Light, 40), "ct_light_config": (LightConfig, 52),
                 "ct_reservoir": (Reservoir, 36), "ct_di_info": (DiInfo, 60), "ct_path_reservoir": (PathReservoir, 92),
                 "ct_restir_params": (RestirParams, 108), "ct_adjoint_view": (AdjointView, 136),
-                "ct_adjoint_desc": (AdjointDesc, 16)}
+                "ct_adjoint_desc": (AdjointDesc, 16), "ct_batch_tuning": (BatchTuning, 16)}
 
 
 # ----------------------------------------------------------------- loader
@@ -295,6 +299,11 @@ _SIGS = {
     "ct_frame_read": ([C.c_void_p, u32, C.c_void_p, C.c_size_t], C.c_int),
     "ct_ctx_set_instrumentation": ([C.c_void_p, u32], C.c_int),
     "ct_frame_get_stats": ([C.c_void_p, C.POINTER(FrameStats)], C.c_int),
+    "ct_batch_create": ([u32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)], C.c_int),
+    "ct_batch_destroy": ([C.c_void_p], C.c_int),
+    "ct_batch_render": ([C.c_void_p, C.POINTER(FrameParams), C.c_int, C.c_void_p], C.c_int),
+    "ct_batch_get_stats": ([C.c_void_p, C.POINTER(FrameStats)], C.c_int),
+    "ct_batch_autotune": ([C.c_void_p, C.POINTER(FrameParams), u32, C.POINTER(BatchTuning)], C.c_int),
     "ct_camera_matrices": ([C.POINTER(f32), C.POINTER(f32), C.POINTER(f32), f32, f32, f32, f32,
                             C.POINTER(f32), C.POINTER(f32)], C.c_int),
     "ct_camera_prev_view_proj": ([C.POINTER(f32), C.POINTER(f32), C.POINTER(f32)], C.c_int),
@@ -303,6 +312,7 @@ _SIGS = {
     "ct_scene_get_info": ([C.c_void_p, C.POINTER(SceneInfo)], C.c_int),
     "ct_scene_destroy": ([C.c_void_p], C.c_int),
     "ct_scene_load": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "ct_scene_bind": ([C.c_void_p, C.c_void_p], C.c_int),
 }
 
 
@@ -581,9 +591,53 @@ class Ctx:
             pass
 
 
+class Batch:
+    """Several views (context + accel each) rendered with one launch per pass."""
+
+    def __init__(self, ctxs, accels):
+        n = len(ctxs)
+        self.ctxs, self.accels = list(ctxs), list(accels)
+        c = (C.c_void_p * n)(*[x.handle for x in ctxs])
+        a = (C.c_void_p * n)(*[x.handle for x in accels])
+        h = C.c_void_p()
+        check(lib().ct_batch_create(n, c, a, C.byref(h)))
+        self.handle = h
+
+    def render(self, params, use_graph=True):
+        arr = (FrameParams * len(params))(*params)
+        check(lib().ct_batch_render(self.handle, arr, 1 if use_graph else 0, None))
+
+    def stats(self) -> FrameStats:
+        s = FrameStats()
+        check(lib().ct_batch_get_stats(self.handle, C.byref(s)))
+        return s
+
+    def autotune(self, params, repeats=3) -> BatchTuning:
+        arr = (FrameParams * len(params))(*params)
+        t = BatchTuning()
+        check(lib().ct_batch_autotune(self.handle, arr, repeats, C.byref(t)))
+        return t
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().ct_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def load_scene(scene: Scene, accel: Accel, ctx: Ctx):
     """load_scene (main.cpp:1045-1126): staging, BUILD_BLAS_FROM_CPU, update_scene(true)."""
     check(lib().ct_scene_load(scene.handle, accel.handle, ctx.handle, None))
+
+
+def bind_scene(scene: Scene, ctx: Ctx):
+    """Materials and light configuration of a loaded scene for another (fresh) context."""
+    check(lib().ct_scene_bind(scene.handle, ctx.handle))
 
 
 from .configs import *  # noqa: E402,F401,F403

@@ -16,7 +16,7 @@ from typing import Callable, Tuple
 
 import numpy as np
 
-__all__ = ["Workload", "small_plain", "hd_di", "frame_params", "scene_desc"]
+__all__ = ["Workload", "small_plain", "hd_di", "frame_params", "scene_desc", "batched_views"]
 
 
 def _lib():
@@ -147,3 +147,39 @@ def frame_params(w: Workload, f: int):
     p.prev_view_proj[:] = prev_view_proj(piv, pip).tolist()
     p.restir = w.restir
     return p
+
+
+def batched_views(n_views=64, seed=0xC3, grids=(64, 128, 256, 384), max_depth=2, size_scale=1.0):
+    """Four scenes (64^3 .. 384^3, noise fill U[10 %, 30 %], 2 % emissive)
+    and n_views cameras on seeded orbits with resolutions U[128, 1024] x
+    U[128, 768]; every view runs ReSTIR DI + a max_depth-bounce path, 1 spp.
+    Returns (scene workloads, [(scene index, view workload)])."""
+    from . import CT_RIS_CUBE_LIGHT_BIT, restir_params_reference
+    rng = np.random.default_rng(seed)
+    scenes = []
+    for k, g in enumerate(grids):
+        fill = float(rng.uniform(0.10, 0.30))
+        occ = round(fill * g ** 3)
+        rp = restir_params_reference()
+        rp.taa_alpha = 0.1
+        _indirect(rp, max_depth > 0, max(max_depth, 1))
+        scenes.append(Workload(name=f"batch_scene_{g}", grid=(g, g, g), chunk=(min(g, 64),) * 3, occupied=occ,
+                               emissive=round(0.02 * occ), le=(1.0, 20.0), seed=seed + k, width=0, height=0,
+                               frames=1, fov_radians=math.radians(60.0), camera=None,
+                               features=CT_RIS_CUBE_LIGHT_BIT, taa=True, restir=rp))
+    views = []
+    for v in range(n_views):
+        k = int(rng.integers(0, len(grids)))
+        w = max(8, int(round(int(rng.integers(128, 1025)) * size_scale)))
+        h = max(8, int(round(int(rng.integers(128, 769)) * size_scale)))
+        extent = grids[k] * 0.03125
+        radius = float(rng.uniform(1.2, 2.0)) * extent
+        height = float(rng.uniform(0.1, 0.5)) * extent
+        start = float(rng.uniform(0.0, 360.0))
+        sw = scenes[k]
+        views.append((k, Workload(name=f"view_{v}", grid=sw.grid, chunk=sw.chunk, occupied=sw.occupied,
+                                  emissive=sw.emissive, le=sw.le, seed=sw.seed, width=w, height=h, frames=16,
+                                  fov_radians=math.radians(60.0),
+                                  camera=_orbit_camera(radius, height, 1.0, start_deg=start),
+                                  features=sw.features, taa=True, restir=sw.restir)))
+    return scenes, views

@@ -497,6 +497,41 @@ int ct_ctx_set_instrumentation(ct_ctx *ctx, uint32_t flags);
 int ct_frame_get_stats(ct_ctx *ctx, ct_frame_stats *out);
 
 /* =================================================================
+ * Batched frames.  No counterpart in the reference, which renders one
+ * view per App::draw (main.cpp:1436); for callers with many views.
+ *
+ * A batch binds n (context, accel) pairs: views of one or several scenes,
+ * each view with its own context (its frame history) and resolution.
+ * ct_batch_render runs frame params[i] of every view with one launch per
+ * pass over all views' pixels (the passes and results of ct_frame_render,
+ * view by view).  With use_graph the launch sequence is captured once into
+ * a hipGraph and replayed; it is captured again when resolutions, path
+ * depth, neighbour count or the tuned grid change.  Views must differ in
+ * context; contexts and accels must share one device; a context of a
+ * batch holds at most 2^22 pixels.
+ * ================================================================= */
+typedef struct ct_batch ct_batch;
+int ct_batch_create(uint32_t n_views, ct_ctx *const *ctxs, ct_accel *const *accels,
+                    ct_batch **out);
+int ct_batch_destroy(ct_batch *batch);
+int ct_batch_render(ct_batch *batch, const ct_frame_params *params, int use_graph,
+                    void *stream);
+/* Totals over the views of the last batch render: rays and ray counters,
+ * ms_total = HIP-event time of the whole batch (other times are 0). */
+int ct_batch_get_stats(ct_batch *batch, ct_frame_stats *out);
+/* Launch-shape autotuner: renders the given frames `repeats` times for
+ * each candidate grid of the persistent traversal kernels (blocks per CU)
+ * and keeps the fastest; reports the choice. */
+typedef struct ct_batch_tuning {
+  uint32_t blocks_per_cu;
+  float ms;            /* mean batch time of the choice */
+  uint32_t candidates; /* number of shapes tried */
+  uint32_t pad;
+} ct_batch_tuning;
+int ct_batch_autotune(ct_batch *batch, const ct_frame_params *params, uint32_t repeats,
+                      ct_batch_tuning *out);
+
+/* =================================================================
  * Camera (src/camera.h:75-93, main.cpp:1452-1460) and matrices.
  * ================================================================= */
 int ct_camera_matrices(const float position[3], const float forward[3],
@@ -548,6 +583,10 @@ int ct_scene_destroy(ct_scene *scene);
  * main.cpp:1317-1320). */
 int ct_scene_load(const ct_scene *scene, ct_accel *accel, ct_ctx *ctx,
                   void *stream);
+/* The context half of ct_scene_load, for more contexts rendering a scene
+ * already loaded into an accel with a fresh context: the same materials
+ * and light configuration.  The context must hold no materials yet. */
+int ct_scene_bind(const ct_scene *scene, ct_ctx *ctx);
 
 /* =================================================================
  * Differentiable rendering.  No counterpart in the reference (its

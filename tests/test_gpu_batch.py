@@ -1,0 +1,78 @@
+"""Batched frames: one launch per pass over several views of several
+scenes (with and without hipGraph capture) gives, view by view, the same
+bits as rendering each view on its own."""
+import numpy as np
+import pytest
+
+import ctamd
+from helpers import bits, di_array, pres_array
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(w):
+    sc = ctamd.Scene(ctamd.scene_desc(w))
+    acc = ctamd.Accel(max_instances=sc.info.instance_count + 8,
+                      max_primitives=sc.info.primitive_count + 32 * sc.info.instance_count + 1024,
+                      max_cube_lights=sc.info.cube_light_count + 8)
+    first = ctamd.Ctx(8, 8)
+    ctamd.load_scene(sc, acc, first)  # the accel holds the scene; contexts bind to it
+    return sc, acc
+
+
+def _views():
+    scenes, views = ctamd.batched_views(n_views=5, seed=0xC3, grids=(64, 96), max_depth=2, size_scale=0.08)
+    built = [_scene(s) for s in scenes]
+    return scenes, views, built
+
+
+def _ctx(sc, acc, w):
+    ctx = ctamd.Ctx(w.width, w.height)
+    ctamd.bind_scene(sc, ctx)
+    return ctx
+
+
+def _read(ctx, w):
+    out = {"img": bits(ctx.read(ctamd.CT_BUF_IMAGE, w.width, w.height)),
+           "di": di_array(ctx.read(ctamd.CT_BUF_DI, w.width, w.height)),
+           "pdi": di_array(ctx.read(ctamd.CT_BUF_PREV_DI, w.width, w.height))}
+    if w.restir.indirect:
+        out["pres"] = pres_array(ctx.read(ctamd.CT_BUF_TEMPORAL_PATH_RESERVOIR, w.width, w.height))
+    return out
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_equals_per_view(gpu, use_graph):
+    scenes, views, built = _views()
+    plain = [_ctx(*built[k], w) for k, w in views]
+    # fresh contexts for the batch: both sides start with the same history
+    batch_ctx = [_ctx(*built[k], w) for k, w in views]
+    b = ctamd.Batch(batch_ctx, [built[k][1] for k, _ in views])
+    for f in range(3):
+        params = [ctamd.frame_params(w, f) for _, w in views]
+        b.render(params, use_graph=use_graph)
+        bs = b.stats()
+        tot = {"shadow": 0, "bounce": 0, "pt": 0, "prim": 0}
+        for (k, w), ctx, p in zip(views, plain, params):
+            ctx.render(built[k][1], p)
+            st = ctx.stats()
+            tot["shadow"] += st.shadow_rays
+            tot["bounce"] += st.bounce_rays
+            tot["pt"] += st.shadow_rays_pt
+            tot["prim"] += st.primary_rays
+        for (k, w), pc, bc in zip(views, plain, batch_ctx):
+            a, c = _read(pc, w), _read(bc, w)
+            for key in a:
+                assert np.array_equal(a[key], c[key]), (f, w.name, key)
+        assert (bs.shadow_rays, bs.bounce_rays, bs.shadow_rays_pt, bs.primary_rays) == \
+            (tot["shadow"], tot["bounce"], tot["pt"], tot["prim"])
+        assert bs.ms_total > 0
+
+
+def test_batch_autotune_keeps_results(gpu):
+    scenes, views, built = _views()
+    ctxs = [_ctx(*built[k], w) for k, w in views]
+    b = ctamd.Batch(ctxs, [built[k][1] for k, _ in views])
+    params = [ctamd.frame_params(w, 0) for _, w in views]
+    t = b.autotune(params, repeats=1)
+    assert t.blocks_per_cu in (2, 4, 8, 16, 32) and t.ms > 0 and t.candidates == 5
