@@ -72,6 +72,110 @@ def cpu_baseline(w, sc, acc, budget_s):
                       f"total ({total_rays} traversal queries in {total_t:.1f}s, OpenMP {cores} threads)"}
 
 
+def batched_bench(frames=5, warmup=2, n_views=64):
+    """Batched views (4 scenes, 64 views of uneven size): per-view
+    ct_frame_render against one batched, graph-replayed render."""
+    import ctamd
+    t0 = time.time()
+    scenes, views = ctamd.batched_views(n_views=n_views)
+    built = []
+    for sw in scenes:
+        sc = ctamd.Scene(ctamd.scene_desc(sw))
+        acc = ctamd.Accel(max_instances=sc.info.instance_count + 8,
+                          max_primitives=sc.info.primitive_count + 32 * sc.info.instance_count + 1024,
+                          max_cube_lights=sc.info.cube_light_count + 8)
+        tmp = ctamd.Ctx(8, 8)
+        ctamd.load_scene(sc, acc, tmp)
+        built.append((sc, acc))
+    ctx_plain, ctx_batch = [], []
+    for k, w in views:
+        for lst in (ctx_plain, ctx_batch):
+            c = ctamd.Ctx(w.width, w.height)
+            ctamd.bind_scene(built[k][0], c)
+            lst.append(c)
+    accs = [built[k][1] for k, _ in views]
+    b = ctamd.Batch(ctx_batch, accs)
+    log(f"batched: {len(views)} views over {len(scenes)} scenes "
+        f"({sum(sc.info.primitive_count for sc, _ in built)} voxels, "
+        f"{sum(w.width * w.height for _, w in views)} pixels) set up in {time.time() - t0:.1f}s")
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    params = [[ctamd.frame_params(w, f) for _, w in views] for f in range(warmup + frames)]
+    tune = b.autotune(params[0], repeats=2)
+    log(f"batched: autotuned traversal grid {tune.blocks_per_cu} blocks/CU ({tune.ms:.2f} ms)")
+    # plain: one ct_frame_render per view and frame
+    t_plain, t_batch, rays = 0.0, 0.0, 0
+    for f in range(warmup + frames):
+        hip.hipDeviceSynchronize()
+        t1 = time.perf_counter()
+        for (k, w), c, p in zip(views, ctx_plain, params[f]):
+            c.render(built[k][1], p)
+        hip.hipDeviceSynchronize()
+        if f >= warmup:
+            t_plain += time.perf_counter() - t1
+    for f in range(warmup + frames):
+        hip.hipDeviceSynchronize()
+        t1 = time.perf_counter()
+        b.render(params[f], use_graph=True)
+        hip.hipDeviceSynchronize()
+        if f >= warmup:
+            t_batch += time.perf_counter() - t1
+            st = b.stats()
+            rays += st.primary_rays + st.shadow_rays + st.bounce_rays
+    return {"views": len(views), "scenes": len(scenes), "pixels": sum(w.width * w.height for _, w in views),
+            "indirect_bounces": views[0][1].restir.max_depth, "frames": frames,
+            "plain_ms_per_frame": round(1000 * t_plain / frames, 3),
+            "batched_ms_per_frame": round(1000 * t_batch / frames, 3),
+            "speedup": round(t_plain / t_batch, 2), "batched_mrays_s": round(rays / t_batch / 1e6, 1),
+            "graph": True, "tuned_blocks_per_cu": tune.blocks_per_cu}
+
+
+def adjoint_bench(steps=3):
+    """Inverse-rendering step on 128^3 voxels, 8 views of 512^2 at 16 spp,
+    3 bounces: forward render vs forward + path-replay adjoint."""
+    import torch
+    import ctamd
+    from ctamd import adjoint as AD
+    w = AD.AdjointWorkload()
+    sc, alb, emi = AD.build_scene(w)
+    acc = ctamd.Accel(max_instances=sc.info.instance_count + 8,
+                      max_primitives=sc.info.primitive_count + 32 * sc.info.instance_count + 1024,
+                      max_cube_lights=max(sc.info.cube_light_count, 1) + 8)
+    AD._load_geometry(sc, acc)
+    slots = AD.VoxelSlots(sc, acc)
+    extent = w.grid * ctamd.CT_VOXEL_EXTENT
+    views = AD.orbit_views(w.views, w.width, w.height, (0.0, 0.0, 0.0), 1.6 * extent, 0.4 * extent)
+    r = AD.DiffRenderer(acc, views, w.spp, w.max_depth, w.target_seed)
+    dev = torch.device("cuda")
+    target = r.forward(torch.from_numpy(slots.scatter(alb)).to(dev), torch.from_numpy(slots.scatter(emi)).to(dev))
+    a0, e0 = AD.initial_parameters(w, alb, emi)
+    a = torch.from_numpy(slots.scatter(a0)).to(dev).requires_grad_(True)
+    e = torch.from_numpy(slots.scatter(e0)).to(dev).requires_grad_(True)
+    opt = torch.optim.Adam([{"params": [a], "lr": 0.02}, {"params": [e], "lr": 0.5}])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        with torch.no_grad():
+            r.forward(a, e)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t1) / steps
+    losses = []
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        opt.zero_grad()
+        loss = ((AD.render(r, a, e) - target) ** 2).mean()
+        loss.backward()
+        opt.step()
+        losses.append(float(loss.detach()))
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t1) / steps
+    paths = w.views * w.width * w.height * w.spp
+    return {"voxels": sc.info.primitive_count, "views": w.views, "resolution": [w.width, w.height], "spp": w.spp,
+            "bounces": w.max_depth, "paths_per_render": paths, "forward_ms": round(1000 * t_fwd, 2),
+            "adam_step_ms": round(1000 * t_step, 2), "step_over_forward": round(t_step / t_fwd, 2),
+            "forward_mpaths_s": round(paths / t_fwd / 1e6, 1), "losses": [round(x, 6) for x in losses]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +187,8 @@ def main():
     ap.add_argument("--depth", type=int, default=3, help="path bounces (ReSTIR PT max_depth); 0 = DI only")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-view batched comparison")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the inverse-rendering step timing")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (rocprof counter calibration)")
     args = ap.parse_args()
@@ -177,6 +283,13 @@ def main():
                 "frac": round(v["achieved"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": v["desc"],
                 "ms_per_frame": round(v["ms"] / K, 3)}
 
+    batched = adjoint = None
+    if not args.no_batched:
+        batched = batched_bench()
+        log(f"batched: {batched}")
+    if not args.no_adjoint:
+        adjoint = adjoint_bench()
+        log(f"adjoint: {adjoint}")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, sc, acc, args.cpu_budget)
@@ -196,6 +309,8 @@ def main():
         "roofline": roof(order[0]),
         "roofline_next_kernel": roof(order[1]),
         "cpu_baseline": cpu,
+        "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 

@@ -394,7 +394,9 @@ def _arr(ptr, ctype, n):
 
 
 class Scene:
-    """A synthetic voxel scene generated by the library (host side)."""
+    """A synthetic voxel scene generated by the library (host side).  The
+    instance, AABB, primitive, material and light arrays are copied out of
+    the library on first use."""
 
     def __init__(self, desc: SceneDesc):
         self.desc = desc
@@ -404,14 +406,32 @@ class Scene:
         info = SceneInfo()
         check(lib().ct_scene_get_info(h, C.byref(info)))
         self.info = info
-        ni, np_, nm, nl = info.instance_count, info.primitive_count, info.material_count, info.cube_light_count
-        self.instances = (Instance * ni).from_buffer_copy(_arr(info.instances, Instance, ni))
-        self.aabbs = np.frombuffer(bytes(_arr(info.aabbs, Aabb, np_)), np.float32).reshape(np_, 6).copy()
-        self.primitives = np.frombuffer(bytes(_arr(info.primitives, Primitive, np_)), np.uint32).reshape(np_, 2).copy()
-        self.materials = (Material * nm).from_buffer_copy(_arr(info.materials, Material, nm))
-        self.lights = (Light * nl).from_buffer_copy(_arr(info.cube_lights, Light, nl)) if nl else (Light * 0)()
         self.grid_min = np.array(info.grid_min[:], np.float32)
         self.grid_max = np.array(info.grid_max[:], np.float32)
+        self._cache = {}
+
+    def _get(self, name):
+        if name not in self._cache:
+            info = self.info
+            ni, np_, nm, nl = info.instance_count, info.primitive_count, info.material_count, info.cube_light_count
+            if name == "instances":
+                v = (Instance * ni).from_buffer_copy(_arr(info.instances, Instance, ni))
+            elif name == "aabbs":
+                v = np.ctypeslib.as_array(C.cast(info.aabbs, C.POINTER(f32)), shape=(np_ * 6,)).reshape(np_, 6).copy()
+            elif name == "primitives":
+                v = np.ctypeslib.as_array(C.cast(info.primitives, C.POINTER(u32)), shape=(np_ * 2,)).reshape(np_, 2).copy()
+            elif name == "materials":
+                v = (Material * nm).from_buffer_copy(_arr(info.materials, Material, nm))
+            else:
+                v = (Light * nl).from_buffer_copy(_arr(info.cube_lights, Light, nl)) if nl else (Light * 0)()
+            self._cache[name] = v
+        return self._cache[name]
+
+    instances = property(lambda self: self._get("instances"))
+    aabbs = property(lambda self: self._get("aabbs"))
+    primitives = property(lambda self: self._get("primitives"))
+    materials = property(lambda self: self._get("materials"))
+    lights = property(lambda self: self._get("lights"))
 
     def close(self):
         if self.handle:
