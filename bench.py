@@ -30,6 +30,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_trace_hd_di_pt3.md")
+
+
+def _traffic_summary():
+    """Measured HBM bytes per launch of the traversal kernels on the default
+    workload: the `traffic_bytes_per_launch:` line of the counter summary."""
+    try:
+        with open(TRAFFIC_SUMMARY) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return {k: int(v) for k, v in (kv.split("=") for kv in line.split(":", 1)[1].split())}
+    except OSError:
+        pass
+    return {}
+
+
+def depth_of(w):
+    return w.restir.max_depth if w.restir.indirect else 0
+
+
 def cpu_baseline(w, sc, acc, budget_s):
     """Oracle (OpenMP, all host cores) on bands of rows of frames 0 and 1."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -276,12 +296,19 @@ def main():
                              "achieved": b_seg * n_seg_t / (ms_seg / 1000.0) / 1e9 if ms_seg > 0 else 0.0},
     }
     order = sorted(kern, key=lambda k: -kern[k]["ms"])
+    # HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE counter
+    # runs of this workload (rocprofv3 cannot run inside the timed process);
+    # the algorithmic bytes per launch of this run are reported beside them.
+    measured = _traffic_summary() if (args.width, args.height, args.grid, args.depth) == (1920, 1080, 256, 3) else {}
+    launches = {"k_trace_any": 3, "k_trace_segments": max(depth_of(w), 1)}
+    algo = {"k_trace_any": b_any * n_any_t / K, "k_trace_segments": b_seg * n_seg_t / K}
 
     def roof(k):
         v = kern[k]
         return {"bound": "hbm", "achieved": round(v["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(v["achieved"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": v["desc"],
-                "ms_per_frame": round(v["ms"] / K, 3)}
+                "frac": round(v["achieved"] / HBM_PEAK_GBS, 4), "traffic": measured.get(k),
+                "algorithmic_bytes_per_launch": round(algo[k] / launches[k]),
+                "kernel": v["desc"], "ms_per_frame": round(v["ms"] / K, 3)}
 
     batched = adjoint = None
     if not args.no_batched:
