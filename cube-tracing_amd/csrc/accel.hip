@@ -435,6 +435,37 @@ __global__ void k_validate(SegView sv, const DNode *nodes, const uint32_t *paren
   }
 }
 
+// Parent links of the segments' slot ranges, reset before their build (other
+// instances keep theirs for later refits).
+__global__ void k_reset_parents(SegView sv, uint32_t *parent_int, uint32_t *parent_leaf) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= sv.n) return;
+  uint32_t s = find_seg(sv.off, sv.nseg, k);
+  uint32_t j = sv.base[s] + (k - sv.off[s]);
+  parent_int[j] = kInvalid;
+  parent_leaf[j] = kInvalid;
+}
+
+// Refit inputs of a built BLAS: its leaf order (local ids by Morton rank) in
+// item order, and its leaf records rewritten with the voxels' current centres.
+__global__ void k_refit_prep(SegView sv, const uint32_t *blas_ids, const float4 *centers, uint32_t *svals,
+                             float4 *leaves) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= sv.n) return;
+  uint32_t s = find_seg(sv.off, sv.nseg, k);
+  uint32_t j = k - sv.off[s];
+  uint32_t p = blas_ids[sv.base[s] + j];
+  svals[k] = p;
+  float4 c = centers[sv.off[s] + p];
+  leaves[sv.base[s] + j] = make_float4(c.x, c.y, c.z, __uint_as_float(p));
+}
+
+// dst[w.x] = w.y for each (index, value) pair; indices are checked on the host.
+__global__ void k_scatter_u32(const uint2 *w, uint32_t n, uint32_t *dst) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[w[i].x] = w[i].y;
+}
+
 inline uint32_t blocks(uint64_t n, uint32_t b = kBlock) { return (uint32_t)((n + b - 1) / b); }
 
 // Host copy of a column-major mat4 * (x, y, z, 1), summed as the kernels do.
@@ -506,8 +537,7 @@ int build_segments(Accel &a, hipStream_t st, const std::vector<uint32_t> &off,
   }
   uint32_t *d_k32 = S.sorted_keys32.as<uint32_t>();
   hipLaunchKernelGGL(k_split_keys, dim3(blocks(n)), dim3(kBlock), 0, st, k0, d_k32, n);
-  hipLaunchKernelGGL(k_fill_u32, dim3(blocks(node_capacity)), dim3(kBlock), 0, st, d_parent_int, kInvalid, node_capacity);
-  hipLaunchKernelGGL(k_fill_u32, dim3(blocks(node_capacity)), dim3(kBlock), 0, st, d_parent_leaf, kInvalid, node_capacity);
+  hipLaunchKernelGGL(k_reset_parents, dim3(blocks(n)), dim3(kBlock), 0, st, sv, d_parent_int, d_parent_leaf);
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks(node_capacity)), dim3(kBlock), 0, st, S.flags.as<uint32_t>(), 0u, node_capacity);
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks(nseg)), dim3(kBlock), 0, st, S.depth.as<uint32_t>(), 0u, nseg);
   hipLaunchKernelGGL(k_karras, dim3(blocks(n)), dim3(kBlock), 0, st, sv, d_k32, d_nodes, d_parent_int, d_parent_leaf);
@@ -544,65 +574,46 @@ DScene scene_view(const Accel &a) {
   return s;
 }
 
-// Rebuild every BLAS and the TLAS of the committed scene.
-int rebuild_all(Accel &a) {
-  hipStream_t st = a.stream;
+// What an update changed: instances whose BLAS is rebuilt from scratch,
+// instances refitted in place, slot ranges whose AABBs/primitives are
+// uploaded, and whether buffers must grow (everything is then rebuilt).
+struct Dirty {
+  bool full = false;
+  std::vector<uint32_t> rebuild, refit;
+  std::vector<std::pair<uint32_t, uint32_t>> ranges;  // (first slot, count)
+};
+
+namespace {
+
+std::vector<uint32_t> used_ids(const Accel &a) {
+  std::vector<uint32_t> ids;
+  for (uint32_t i = 0; i < a.instance_used.size(); ++i)
+    if (a.instance_used[i]) ids.push_back(i);
+  return ids;
+}
+
+// Instance tables (DInst for traversal, the ct_instance AoS of the WORLD
+// buffer, geometry_T = previous transform * world-to-object for motion
+// vectors, motion_vectors.glsl:72-81, and the previous first indices of the
+// remapping lookup) and the cube light list, from the host mirror.
+int upload_tables(Accel &a, hipStream_t st) {
   BvhSet &S = a.set;
   int rc;
-  std::vector<uint32_t> ids;
-  for (uint32_t i = 0; i < a.max_instances && i < a.instance_used.size(); ++i)
-    if (a.instance_used[i]) ids.push_back(i);
-  const uint32_t cap = std::max<uint32_t>(a.prim_high_water, 1);
-  S.prim_capacity = cap;
-  if ((rc = S.aabbs.reserve(sizeof(ct_aabb) * (size_t)cap)) || (rc = S.prims.reserve(sizeof(ct_primitive) * (size_t)cap)) ||
-      (rc = S.nodes.reserve(sizeof(DNode) * (size_t)cap)) || (rc = S.leaves.reserve(16 * (size_t)cap)) ||
-      (rc = S.blas_keys.reserve(4 * (size_t)cap)) || (rc = S.blas_ids.reserve(4 * (size_t)cap)) ||
-      (rc = S.blas_parent.reserve(8 * (size_t)cap)) ||
-      (rc = S.inst_aos.reserve(sizeof(ct_instance) * (size_t)std::max<size_t>(a.instances.size(), 1))) ||
-      (rc = S.instances_d.reserve(sizeof(DInst) * (size_t)std::max<size_t>(a.instances.size(), 1))) ||
-      (rc = S.cube_lights.reserve(sizeof(ct_light) * (size_t)std::max<uint32_t>(a.cube_light_count, 1))))
+  a.light_high_water = std::max(a.light_high_water, a.cube_light_count);
+  const size_t ni = std::max<size_t>(a.instances.size(), 1);
+  if ((rc = S.inst_aos.reserve(sizeof(ct_instance) * ni)) || (rc = S.instances_d.reserve(sizeof(DInst) * ni)) ||
+      (rc = S.geo_t.reserve(sizeof(float) * 16 * ni)) || (rc = S.prev_first.reserve(sizeof(uint32_t) * ni)) ||
+      (rc = S.cube_lights.reserve(sizeof(ct_light) * (size_t)std::max<uint32_t>(a.light_high_water, 1))))
     return rc;
-  CT_HIP(hipMemcpyAsync(S.aabbs.p, a.aabbs.data(), sizeof(ct_aabb) * (size_t)a.prim_high_water, hipMemcpyHostToDevice, st));
-  CT_HIP(hipMemcpyAsync(S.prims.p, a.prims.data(), sizeof(ct_primitive) * (size_t)a.prim_high_water, hipMemcpyHostToDevice, st));
-  CT_HIP(hipMemcpyAsync(S.inst_aos.p, a.instances.data(), sizeof(ct_instance) * a.instances.size(), hipMemcpyHostToDevice, st));
-  if (a.cube_light_count)
-    CT_HIP(hipMemcpyAsync(S.cube_lights.p, a.cube_lights_host.data(), sizeof(ct_light) * a.cube_light_count,
-                          hipMemcpyHostToDevice, st));
-  S.cube_light_count = a.cube_light_count;
-  a.blas_root_box.assign(6 * a.instances.size(), 0.0f);
-  a.blas_depth.assign(a.instances.size(), 0);
-  std::vector<DInst> dinst(std::max<size_t>(a.instances.size(), 1));
+  std::vector<DInst> dinst(ni);
   std::memset(dinst.data(), 0, sizeof(DInst) * dinst.size());
-  S.tlas_count = (uint32_t)ids.size();
-  if (ids.empty()) {
-    S.tlas_root = kInvalid;
-    S.tlas_depth = 0;
-    S.blas_depth_max = 0;
-    S.generation = ++a.generation;
-    CT_HIP(hipStreamSynchronize(st));
-    return CT_OK;
-  }
-  // ---- BLASes: one segment per instance.
-  std::vector<uint32_t> off(1, 0), base, src;
-  for (uint32_t id : ids) {
-    base.push_back(a.instances[id].first_primitive_index);
-    src.push_back(a.instances[id].first_primitive_index);
-    off.push_back(off.back() + a.instances[id].primitive_count);
-  }
-  BuildOut bo;
-  uint32_t *par = S.blas_parent.as<uint32_t>();
-  rc = build_segments(a, st, off, base, src, true, S.aabbs.as<ct_aabb>(), nullptr, nullptr, nullptr, nullptr,
-                      kHalfVoxel + kLeafPad, cap, S.nodes.as<DNode>(), S.leaves.as<float4>(),
-                      S.blas_keys.as<uint32_t>(), S.blas_ids.as<uint32_t>(), par, par + cap, bo);
-  if (rc) return rc;
-  uint32_t dmax = 0;
-  for (size_t q = 0; q < ids.size(); ++q) {
-    uint32_t id = ids[q];
-    for (int k = 0; k < 6; ++k) a.blas_root_box[6 * id + k] = bo.root_box[6 * q + k];
-    a.blas_depth[id] = bo.depth[q];
-    dmax = std::max(dmax, bo.depth[q]);
+  std::vector<float> geo(16 * ni, 0.0f);
+  std::vector<uint32_t> pf(ni, 0u);
+  for (size_t id = 0; id < a.instances.size(); ++id) {
+    pf[id] = a.prev_first[id];  // a deleted instance keeps its previous range until the settle
+    if (!a.instance_used[id]) continue;
     DInst &d = dinst[id];
-    const float *w = &a.w2o[16 * (size_t)id];
+    const float *w = &a.w2o[16 * id];
     const float *t = a.instances[id].transform;
     for (int c = 0; c < 4; ++c)
       for (int r = 0; r < 3; ++r) d.w2o[c * 3 + r] = w[c * 4 + r];
@@ -612,27 +623,127 @@ int rebuild_all(Accel &a) {
     d.first = a.instances[id].first_primitive_index;
     d.count = a.instances[id].primitive_count;
     d.root = d.count == 1 ? kLeaf : 0u;
+    const float *pt = &a.prev_transform[16 * id];
+    for (int c = 0; c < 4; ++c)
+      for (int r = 0; r < 4; ++r)
+        geo[16 * id + c * 4 + r] = pt[0 * 4 + r] * w[c * 4 + 0] + pt[1 * 4 + r] * w[c * 4 + 1] +
+                                   pt[2 * 4 + r] * w[c * 4 + 2] + pt[3 * 4 + r] * w[c * 4 + 3];
   }
+  std::vector<ct_instance> aos(a.instances);
+  for (size_t id = 0; id < aos.size(); ++id)
+    if (!a.instance_used[id]) aos[id] = ct_instance{};
+  if (!a.instances.empty()) {
+    CT_HIP(hipMemcpyAsync(S.instances_d.p, dinst.data(), sizeof(DInst) * a.instances.size(), hipMemcpyHostToDevice, st));
+    CT_HIP(hipMemcpyAsync(S.inst_aos.p, aos.data(), sizeof(ct_instance) * aos.size(), hipMemcpyHostToDevice, st));
+    CT_HIP(hipMemcpyAsync(S.geo_t.p, geo.data(), sizeof(float) * 16 * a.instances.size(), hipMemcpyHostToDevice, st));
+    CT_HIP(hipMemcpyAsync(S.prev_first.p, pf.data(), sizeof(uint32_t) * a.instances.size(), hipMemcpyHostToDevice, st));
+  }
+  // Records past the count stay, as the reference's light buffer keeps them.
+  if (a.light_high_water)
+    CT_HIP(hipMemcpyAsync(S.cube_lights.p, a.cube_lights_host.data(), sizeof(ct_light) * a.light_high_water,
+                          hipMemcpyHostToDevice, st));
+  S.cube_light_count = a.cube_light_count;
+  S.cube_light_hw = a.light_high_water;
+  CT_HIP(hipStreamSynchronize(st));  // the host vectors above go out of scope
+  return CT_OK;
+}
+
+// Fresh LBVH of each listed BLAS (one segment per instance).
+int build_blas_set(Accel &a, hipStream_t st, const std::vector<uint32_t> &ids) {
+  if (ids.empty()) return CT_OK;
+  BvhSet &S = a.set;
+  const uint32_t cap = S.prim_capacity;
+  std::vector<uint32_t> off(1, 0), base, src;
+  for (uint32_t id : ids) {
+    base.push_back(a.instances[id].first_primitive_index);
+    src.push_back(a.instances[id].first_primitive_index);
+    off.push_back(off.back() + a.instances[id].primitive_count);
+  }
+  BuildOut bo;
+  uint32_t *par = S.blas_parent.as<uint32_t>();
+  int rc = build_segments(a, st, off, base, src, true, S.aabbs.as<ct_aabb>(), nullptr, nullptr, nullptr, nullptr,
+                          kHalfVoxel + kLeafPad, cap, S.nodes.as<DNode>(), S.leaves.as<float4>(),
+                          S.blas_keys.as<uint32_t>(), S.blas_ids.as<uint32_t>(), par, par + cap, bo);
+  if (rc) return rc;
+  for (size_t q = 0; q < ids.size(); ++q) {
+    for (int k = 0; k < 6; ++k) a.blas_root_box[6 * ids[q] + k] = bo.root_box[6 * q + k];
+    a.blas_depth[ids[q]] = bo.depth[q];
+  }
+  return CT_OK;
+}
+
+// In-place refit of each listed BLAS (update_blases with src_blas = the
+// BLAS itself, accel_struct_mngr.cpp:1672-1852): the leaf order and node
+// topology stay; leaf centres are re-read from the AABBs and every node box
+// is recomputed bottom up.
+int refit_blas_set(Accel &a, hipStream_t st, const std::vector<uint32_t> &ids) {
+  if (ids.empty()) return CT_OK;
+  BvhSet &S = a.set;
+  BuildScratch &B = a.scratch;
+  const uint32_t cap = S.prim_capacity;
+  const uint32_t nseg = (uint32_t)ids.size();
+  std::vector<uint32_t> off(1, 0), base;
+  for (uint32_t id : ids) {
+    base.push_back(a.instances[id].first_primitive_index);
+    off.push_back(off.back() + a.instances[id].primitive_count);
+  }
+  const uint32_t n = off.back();
+  int rc;
+  if ((rc = B.seg.reserve(sizeof(uint32_t) * (3 * (size_t)nseg + 1))) || (rc = B.centers.reserve(16 * (size_t)n)) ||
+      (rc = B.vals[0].reserve(4 * (size_t)n)) || (rc = B.flags.reserve(4 * (size_t)cap)) ||
+      (rc = B.heights.reserve(4 * (size_t)cap)) || (rc = B.root_box.reserve(24 * (size_t)nseg)) ||
+      (rc = B.depth.reserve(4 * (size_t)nseg)) || (rc = B.lo.reserve(24 * (size_t)nseg)))
+    return rc;
+  uint32_t *d_off = B.seg.as<uint32_t>(), *d_base = d_off + nseg + 1, *d_src = d_base + nseg;
+  CT_HIP(hipMemcpyAsync(d_off, off.data(), sizeof(uint32_t) * (nseg + 1), hipMemcpyHostToDevice, st));
+  CT_HIP(hipMemcpyAsync(d_base, base.data(), sizeof(uint32_t) * nseg, hipMemcpyHostToDevice, st));
+  CT_HIP(hipMemcpyAsync(d_src, base.data(), sizeof(uint32_t) * nseg, hipMemcpyHostToDevice, st));
+  SegView sv{d_off, d_base, d_src, nseg, n};
+  std::vector<uint32_t> binit(6 * (size_t)nseg);
+  for (uint32_t s = 0; s < nseg; ++s)
+    for (int q = 0; q < 6; ++q) binit[6 * s + q] = q < 3 ? 0xFFFFFFFFu : 0u;
+  CT_HIP(hipMemcpyAsync(B.lo.p, binit.data(), 24 * (size_t)nseg, hipMemcpyHostToDevice, st));
+  float4 *d_centers = B.centers.as<float4>();
+  uint32_t *d_vals = B.vals[0].as<uint32_t>();
+  uint32_t *par = S.blas_parent.as<uint32_t>();
+  hipLaunchKernelGGL(k_blas_items, dim3(blocks(n)), dim3(kBlock), 0, st, sv, S.aabbs.as<ct_aabb>(), d_centers,
+                     B.lo.as<uint32_t>());
+  hipLaunchKernelGGL(k_refit_prep, dim3(blocks(n)), dim3(kBlock), 0, st, sv, S.blas_ids.as<uint32_t>(), d_centers,
+                     d_vals, S.leaves.as<float4>());
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks(cap)), dim3(kBlock), 0, st, B.flags.as<uint32_t>(), 0u, cap);
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks(nseg)), dim3(kBlock), 0, st, B.depth.as<uint32_t>(), 0u, nseg);
+  hipLaunchKernelGGL(k_refit, dim3(blocks(n)), dim3(kBlock), 0, st, sv, d_vals, d_centers, nullptr, nullptr,
+                     kHalfVoxel + kLeafPad, S.nodes.as<DNode>(), par, par + cap, B.flags.as<uint32_t>(),
+                     B.heights.as<uint32_t>(), B.root_box.as<float>(), B.depth.as<uint32_t>());
+  CT_HIP(hipGetLastError());
+  std::vector<float> rb(6 * (size_t)nseg);
+  std::vector<uint32_t> dp(nseg);
+  CT_HIP(hipMemcpyAsync(rb.data(), B.root_box.p, 24 * (size_t)nseg, hipMemcpyDeviceToHost, st));
+  CT_HIP(hipMemcpyAsync(dp.data(), B.depth.p, 4 * (size_t)nseg, hipMemcpyDeviceToHost, st));
+  CT_HIP(hipStreamSynchronize(st));
+  for (uint32_t q = 0; q < nseg; ++q) {
+    for (int k = 0; k < 6; ++k) a.blas_root_box[6 * ids[q] + k] = rb[6 * q + k];
+    a.blas_depth[ids[q]] = dp[q];  // unchanged topology: same height
+  }
+  return CT_OK;
+}
+
+// TLAS over the instances' padded world boxes, in slot order (build_tlas,
+// accel_struct_mngr.cpp:1854-1973).
+int build_tlas(Accel &a, hipStream_t st) {
+  BvhSet &S = a.set;
+  int rc;
+  const std::vector<uint32_t> ids = used_ids(a);
+  S.tlas_count = (uint32_t)ids.size();
+  uint32_t dmax = 0;
+  for (uint32_t id : ids) dmax = std::max(dmax, a.blas_depth[id]);
   S.blas_depth_max = dmax;
-  CT_HIP(hipMemcpyAsync(S.instances_d.p, dinst.data(), sizeof(DInst) * a.instances.size(), hipMemcpyHostToDevice, st));
-  // geometry_T = previous_model * inverse(model) per instance
-  // (motion_vectors.glsl:146); the scene is static between builds, so
-  // previous_model = model.  Frames multiply it by their prev_view_proj.
-  {
-    std::vector<float> geo(16 * std::max<size_t>(a.instances.size(), 1), 0.0f);
-    for (size_t i = 0; i < a.instances.size(); ++i) {
-      if (i >= a.instance_used.size() || !a.instance_used[i]) continue;
-      const float *t = a.instances[i].transform, *w = &a.w2o[16 * i];
-      for (int c = 0; c < 4; ++c)
-        for (int r = 0; r < 4; ++r)
-          geo[16 * i + c * 4 + r] = t[0 * 4 + r] * w[c * 4 + 0] + t[1 * 4 + r] * w[c * 4 + 1] +
-                                    t[2 * 4 + r] * w[c * 4 + 2] + t[3 * 4 + r] * w[c * 4 + 3];
-    }
-    if ((rc = S.geo_t.reserve(sizeof(float) * geo.size()))) return rc;
-    CT_HIP(hipMemcpyAsync(S.geo_t.p, geo.data(), sizeof(float) * geo.size(), hipMemcpyHostToDevice, st));
-    CT_HIP(hipStreamSynchronize(st));
+  if (ids.empty()) {
+    S.tlas_root = kInvalid;
+    S.tlas_depth = 0;
+    S.generation = ++a.generation;
+    return CT_OK;
   }
-  // ---- TLAS over the instances' padded world boxes, in slot order.
   const uint32_t ni = (uint32_t)ids.size();
   std::vector<float4> tc(ni), tlo(ni), thi(ni);
   for (uint32_t q = 0; q < ni; ++q) {
@@ -675,10 +786,9 @@ int rebuild_all(Accel &a) {
   uint32_t *tk = S.tlas_keys.as<uint32_t>();
   uint32_t *tpar = S.tlas_parent.as<uint32_t>();
   BuildOut to;
-  rc = build_segments(a, st, toff, tbase, tsrc, false, nullptr, dc, dl, dh, dpay, 0.0f, ni,
-                      S.tlas_nodes.as<DNode>(), tleaves.as<float4>(), tk, tk + ni, tpar, tpar + ni, to);
+  rc = build_segments(a, st, toff, tbase, tsrc, false, nullptr, dc, dl, dh, dpay, 0.0f, ni, S.tlas_nodes.as<DNode>(),
+                      tleaves.as<float4>(), tk, tk + ni, tpar, tpar + ni, to);
   if (rc) return rc;
-  // Extract the instance-id column of the TLAS leaves.
   std::vector<float4> hl(ni);
   CT_HIP(hipMemcpy(hl.data(), tleaves.p, 16 * (size_t)ni, hipMemcpyDeviceToHost));
   std::vector<uint32_t> tinst(ni);
@@ -691,6 +801,59 @@ int rebuild_all(Accel &a) {
     return fail(CT_E_UNSUPPORTED, "BVH depth " + std::to_string(S.tlas_depth + S.blas_depth_max) +
                                       " exceeds the traversal stack (" + std::to_string(kStackSize) + ")");
   return CT_OK;
+}
+
+}  // namespace
+
+// Brings the device scene in line with the host mirror after an update:
+// everything when buffers grow (or on the first build), else the touched
+// slot ranges, the listed rebuilds and refits, the tables and the TLAS.
+int commit(Accel &a, const Dirty &d) {
+  hipStream_t st = a.stream;
+  BvhSet &S = a.set;
+  int rc;
+  a.blas_root_box.resize(6 * a.instances.size(), 0.0f);
+  a.blas_depth.resize(a.instances.size(), 0);
+  const uint32_t cap = std::max<uint32_t>(a.prim_high_water, 1);
+  const bool full = d.full || S.prim_capacity != cap || !S.nodes.p;
+  if (full) {
+    S.prim_capacity = cap;
+    if ((rc = S.aabbs.reserve(sizeof(ct_aabb) * (size_t)cap)) || (rc = S.prims.reserve(sizeof(ct_primitive) * (size_t)cap)) ||
+        (rc = S.nodes.reserve(sizeof(DNode) * (size_t)cap)) || (rc = S.leaves.reserve(16 * (size_t)cap)) ||
+        (rc = S.blas_keys.reserve(4 * (size_t)cap)) || (rc = S.blas_ids.reserve(4 * (size_t)cap)) ||
+        (rc = S.blas_parent.reserve(8 * (size_t)cap)) || (rc = S.remap_prim.reserve(4 * (size_t)cap)))
+      return rc;
+    if (a.prim_high_water)
+      CT_HIP(hipMemcpyAsync(S.aabbs.p, a.aabbs.data(), sizeof(ct_aabb) * (size_t)a.prim_high_water,
+                            hipMemcpyHostToDevice, st));
+    if (a.prim_high_water)
+      CT_HIP(hipMemcpyAsync(S.prims.p, a.prims.data(), sizeof(ct_primitive) * (size_t)a.prim_high_water,
+                            hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks(cap)), dim3(kBlock), 0, st, S.remap_prim.as<uint32_t>(), 0u, cap);
+    S.remap_prim_cap = cap;
+  } else {
+    for (const auto &r : d.ranges) {
+      if (r.second == 0) continue;
+      CT_HIP(hipMemcpyAsync(S.aabbs.as<ct_aabb>() + r.first, &a.aabbs[r.first], sizeof(ct_aabb) * (size_t)r.second,
+                            hipMemcpyHostToDevice, st));
+      CT_HIP(hipMemcpyAsync(S.prims.as<ct_primitive>() + r.first, &a.prims[r.first],
+                            sizeof(ct_primitive) * (size_t)r.second, hipMemcpyHostToDevice, st));
+    }
+  }
+  const uint32_t lcap = std::max<uint32_t>(a.max_cube_lights, 1);
+  if (S.remap_light_cap != lcap) {
+    if ((rc = S.remap_light.reserve(4 * (size_t)lcap))) return rc;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks(lcap)), dim3(kBlock), 0, st, S.remap_light.as<uint32_t>(), 0u, lcap);
+    S.remap_light_cap = lcap;
+  }
+  if (full) {
+    if ((rc = build_blas_set(a, st, used_ids(a)))) return rc;
+  } else {
+    if ((rc = build_blas_set(a, st, d.rebuild))) return rc;
+    if ((rc = refit_blas_set(a, st, d.refit))) return rc;
+  }
+  if ((rc = upload_tables(a, st))) return rc;
+  return build_tlas(a, st);
 }
 
 int device_count_gfx950() {
@@ -812,6 +975,17 @@ int ct_accel_request_aabb_host_buffer_count(ct_accel *a, uint32_t count, ct_aabb
   return CT_OK;
 }
 
+int ct_accel_request_primitive_index_host_buffer_count(ct_accel *a, uint32_t count, uint32_t **base, uint32_t *offset) {
+  if (!a || !base || !offset) return fail(CT_E_INVALID, "request_primitive_index_host_buffer_count: null argument");
+  if ((uint64_t)a->temp_prim_index_count + count > a->max_primitives)
+    return fail(CT_E_CAPACITY, "request_primitive_index_host_buffer_count: exceeds max_primitive_count");
+  if (a->prim_index_host.size() < a->max_primitives) a->prim_index_host.resize(a->max_primitives, 0u);
+  *offset = a->temp_prim_index_count;
+  *base = a->prim_index_host.data();
+  a->temp_prim_index_count += count;
+  return CT_OK;
+}
+
 int ct_accel_task_queue_add(ct_accel *a, const ct_task *task) {
   if (!a || !task) return fail(CT_E_INVALID, "task_queue_add: null argument");
   if (task->type > CT_TASK_UNDO_OP_CPU) return fail(CT_E_INVALID, "task_queue_add: unknown task type");
@@ -825,84 +999,422 @@ int ct_accel_task_queue_add(ct_accel *a, const ct_task *task) {
   return CT_OK;
 }
 
+}  // extern "C"
+
+namespace ct {
+namespace {
+
+constexpr size_t kMaxDone = 1u << 16;  // undo depth kept
+
+bool used(const Accel &a, uint32_t id) { return id < a.instance_used.size() && a.instance_used[id]; }
+
+void mark(std::vector<uint32_t> &v, uint32_t id) {
+  if (std::find(v.begin(), v.end(), id) == v.end()) v.push_back(id);
+}
+
+// Column-major product m * t with the operation order of
+// daxa_f32mat4x4_mult (math.inl:92-117).
+void mat4_mul_host(const float *m, const float *t, float *out) {
+  float r[16];
+  for (int c = 0; c < 4; ++c)
+    for (int row = 0; row < 4; ++row)
+      r[c * 4 + row] = m[0 * 4 + row] * t[c * 4 + 0] + m[1 * 4 + row] * t[c * 4 + 1] + m[2 * 4 + row] * t[c * 4 + 2] +
+                       m[3 * 4 + row] * t[c * 4 + 3];
+  std::memcpy(out, r, sizeof(r));
+}
+
+// Per-update state: remapping writes and what the commit must redo.
+struct Update {
+  Dirty dirty;
+  std::vector<uint2> remap_prims, remap_lights;  // (index, value)
+  std::vector<DoneTask> completed;
+  int first_error = CT_OK;
+  std::string message;
+  void error(int code, const std::string &m) {
+    if (first_error == CT_OK) {
+      first_error = code;
+      message = m;
+    }
+  }
+};
+
+void touch(Accel &a, Update &u, uint32_t id, uint32_t count) {
+  u.dirty.ranges.emplace_back(a.instances[id].first_primitive_index, count);
+}
+
+void free_instance(Accel &a, Update &u, uint32_t id) {
+  a.prim_alloc.deallocate(a.instances[id].first_primitive_index, align32(a.instances[id].primitive_count));
+  a.instance_used[id] = 0;
+  a.instances[id] = ct_instance{};
+  auto drop = [&](std::vector<uint32_t> &v) { v.erase(std::remove(v.begin(), v.end(), id), v.end()); };
+  drop(u.dirty.rebuild);
+  drop(u.dirty.refit);
+}
+
+// The light at index `l` records its owner (instance, primitive); make the
+// owner's PRIMITIVE.light_index point back at `l` (the owner may sit in
+// another instance: its slot is uploaded too).
+void relink_light_owner(Accel &a, Update &u, uint32_t l) {
+  const ct_object_info o = a.cube_lights_host[l].instance_info;
+  if (used(a, o.instance_id) && o.primitive_id < a.instances[o.instance_id].primitive_count) {
+    const uint32_t slot = a.instances[o.instance_id].first_primitive_index + o.primitive_id;
+    a.prims[slot].light_index = l;
+    u.dirty.ranges.emplace_back(slot, 1u);
+  }
+}
+
+void task_build(Accel &a, Update &u, const ct_task &t, uint32_t &cursor) {
+  for (uint32_t q = 0; q < t.u.blas_build_from_cpu.instance_count; ++q, ++cursor) {
+    if (cursor >= a.max_instances) return u.error(CT_E_CAPACITY, "update_scene: staging cursor out of range");
+    const ct_instance &ti = a.temp_instances[cursor];
+    uint32_t id = 0;
+    while (id < a.instance_used.size() && a.instance_used[id]) ++id;
+    if (id >= a.max_instances) return u.error(CT_E_CAPACITY, "update_scene: no free instance id");
+    if (ti.primitive_count == 0) {
+      u.error(CT_E_INVALID, "update_scene: instance without primitives");
+      continue;
+    }
+    if ((uint64_t)ti.first_primitive_index + ti.primitive_count > a.max_primitives) {
+      u.error(CT_E_INVALID, "update_scene: staged primitive range out of bounds");
+      continue;
+    }
+    uint32_t off = 0;
+    if (!a.prim_alloc.allocate(align32(ti.primitive_count), off))
+      return u.error(CT_E_CAPACITY, "update_scene: primitive free list exhausted");
+    if (id >= a.instance_used.size()) {
+      a.instance_used.resize(id + 1, 0);
+      a.instances.resize(id + 1, ct_instance{});
+      a.w2o.resize(16 * (size_t)(id + 1), 0.0f);
+      a.prev_transform.resize(16 * (size_t)(id + 1), 0.0f);
+      a.prev_first.resize(id + 1, 0u);
+    }
+    a.instance_used[id] = 1;
+    uint32_t end = off + align32(ti.primitive_count);
+    if (end > a.prim_high_water) {
+      a.prim_high_water = end;
+      a.aabbs.resize(end);
+      a.prims.resize(end);
+      u.dirty.full = true;
+    }
+    std::memcpy(&a.aabbs[off], &a.aabb_host[ti.first_primitive_index], sizeof(ct_aabb) * ti.primitive_count);
+    std::memcpy(&a.prims[off], &a.primitives_host[ti.first_primitive_index], sizeof(ct_primitive) * ti.primitive_count);
+    ct_instance &in = a.instances[id];
+    std::memcpy(in.transform, t.u.blas_build_from_cpu.transform, sizeof(in.transform));
+    in.first_primitive_index = off;
+    in.primitive_count = ti.primitive_count;
+    host_inverse(in.transform, &a.w2o[16 * (size_t)id]);
+    // A new instance has no earlier frame: its previous state is its own.
+    std::memcpy(&a.prev_transform[16 * (size_t)id], in.transform, sizeof(in.transform));
+    a.prev_first[id] = off;
+    mark(u.dirty.rebuild, id);
+    touch(a, u, id, in.primitive_count);
+  }
+}
+
+// delete_aabb_device_buffer + update_remapping_buffer +
+// update_light_remapping_buffer + delete_light_device_buffer
+// (accel_struct_mngr.cpp:607-817, 1155-1196).
+bool task_delete_primitive(Accel &a, Update &u, ct_task &t, DoneTask &d) {
+  const uint32_t id = t.u.blas_delete_primitive_from_cpu.instance_index;
+  const uint32_t del = t.u.blas_delete_primitive_from_cpu.del_primitive_index;
+  if (!used(a, id)) return u.error(CT_E_INVALID, "update_scene: DELETE_PRIMITIVE of an unused instance"), false;
+  ct_instance &in = a.instances[id];
+  if (del >= in.primitive_count)
+    return u.error(CT_E_INVALID, "update_scene: DELETE_PRIMITIVE index out of range"), false;
+  if (in.primitive_count == 1)
+    return u.error(CT_E_INVALID, "update_scene: DELETE_PRIMITIVE would empty the instance (use DELETE_BLAS)"), false;
+  const uint32_t first = in.first_primitive_index, last = in.primitive_count - 1;
+  touch(a, u, id, in.primitive_count);
+  d.task = t;
+  d.last = last;
+  d.aabb = a.aabbs[first + del];
+  d.prim = a.prims[first + del];
+  uint32_t L = a.prims[first + del].light_index, Lx = kInvalid;
+  if (L != kInvalid && L >= a.cube_light_count) L = kInvalid;  // a stale link names no light
+  if (L != kInvalid) {
+    d.light_rec = a.cube_lights_host[L];
+    Lx = a.cube_light_count - 1;
+    if (Lx == L) Lx = kInvalid;
+    if (Lx != kInvalid) {
+      a.cube_lights_host[L] = a.cube_lights_host[Lx];
+      relink_light_owner(a, u, L);
+    }
+    --a.cube_light_count;
+  }
+  uint32_t Lm = kInvalid;
+  if (last != del) {
+    a.aabbs[first + del] = a.aabbs[first + last];
+    a.prims[first + del] = a.prims[first + last];
+    Lm = a.prims[first + del].light_index;
+    if (Lm != kInvalid && Lm < a.cube_light_count) a.cube_lights_host[Lm].instance_info.primitive_id = del;
+  }
+  in.primitive_count = last;
+  d.light = L;
+  d.light_moved = Lx;
+  t.u.blas_delete_primitive_from_cpu.remap_primitive_index = last;
+  t.u.blas_delete_primitive_from_cpu.del_light_index = L;
+  t.u.blas_delete_primitive_from_cpu.remap_light_index = Lx;
+  t.u.blas_delete_primitive_from_cpu.remap_primitive_light_index = Lm;
+  d.task = t;
+  u.remap_prims.push_back(make_uint2(first + del, kInvalid));
+  if (last != del) u.remap_prims.push_back(make_uint2(first + last, del));
+  if (L != kInvalid) u.remap_lights.push_back(make_uint2(L, kInvalid));
+  if (Lx != kInvalid) u.remap_lights.push_back(make_uint2(Lx, L));
+  mark(u.dirty.rebuild, id);
+  return true;
+}
+
+// Exact inverse of task_delete_primitive (the reference's own undo,
+// restore_aabb_device_buffer / restore_remapping_buffer /
+// restore_cube_light_remapping_buffer, accel_struct_mngr.cpp:915-1083).
+void undo_delete_primitive(Accel &a, Update &u, const DoneTask &d) {
+  const uint32_t id = d.task.u.blas_delete_primitive_from_cpu.instance_index;
+  const uint32_t del = d.task.u.blas_delete_primitive_from_cpu.del_primitive_index;
+  if (!used(a, id) || a.instances[id].primitive_count != d.last)
+    return u.error(CT_E_STATE, "update_scene: UNDO of a primitive deletion whose instance has changed since");
+  if (d.light != kInvalid && a.cube_light_count >= a.max_cube_lights)
+    return u.error(CT_E_CAPACITY, "update_scene: UNDO needs a cube light slot");
+  ct_instance &in = a.instances[id];
+  const uint32_t first = in.first_primitive_index, last = d.last;
+  in.primitive_count = last + 1;
+  touch(a, u, id, in.primitive_count);
+  if (last != del) {
+    a.aabbs[first + last] = a.aabbs[first + del];
+    a.prims[first + last] = a.prims[first + del];
+    const uint32_t Lm = a.prims[first + last].light_index;
+    if (Lm != kInvalid && Lm < a.cube_light_count) a.cube_lights_host[Lm].instance_info.primitive_id = last;
+    u.remap_prims.push_back(make_uint2(first + del, last));
+  }
+  a.aabbs[first + del] = d.aabb;
+  a.prims[first + del] = d.prim;
+  if (d.light != kInvalid) {
+    ++a.cube_light_count;
+    if (d.light_moved != kInvalid) {
+      a.cube_lights_host[d.light_moved] = a.cube_lights_host[d.light];
+      relink_light_owner(a, u, d.light_moved);
+      u.remap_lights.push_back(make_uint2(d.light, d.light_moved));
+    }
+    a.cube_lights_host[d.light] = d.light_rec;
+  }
+  mark(u.dirty.rebuild, id);
+}
+
+void task_update(Accel &a, Update &u, const ct_task &t, bool &touched) {
+  const uint32_t id = t.u.blas_update.instance_index;
+  if (!used(a, id)) return u.error(CT_E_INVALID, "update_scene: UPDATE_BLAS of an unused instance");
+  const uint32_t n = t.u.blas_update.primitive_count;
+  const uint32_t pio = t.u.blas_update.primitive_index_buf_offset, ao = t.u.blas_update.aabb_buf_offset;
+  if (n && ((uint64_t)pio + n > a.prim_index_host.size() || (uint64_t)ao + n > a.aabb_host.size()))
+    return u.error(CT_E_INVALID, "update_scene: UPDATE_BLAS staging range out of bounds");
+  ct_instance &in = a.instances[id];
+  mat4_mul_host(in.transform, t.u.blas_update.transform, in.transform);
+  host_inverse(in.transform, &a.w2o[16 * (size_t)id]);
+  touched = true;
+  bool moved = false;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t p = a.prim_index_host[pio + i];
+    if (p >= in.primitive_count) continue;  // update_aabb_device_buffer skips these (:532-538)
+    a.aabbs[in.first_primitive_index + p] = a.aabb_host[ao + i];
+    moved = true;
+  }
+  if (moved) {
+    touch(a, u, id, in.primitive_count);
+    if (std::find(u.dirty.rebuild.begin(), u.dirty.rebuild.end(), id) == u.dirty.rebuild.end()) mark(u.dirty.refit, id);
+  }
+}
+
+// DELETE_PRIMITIVE_BLAS_FROM_GPU (:2258-2290).  The device slots already
+// hold the compacted primitives; the host mirror was refreshed from them.
+void task_delete_from_gpu(Accel &a, Update &u, const ct_task &t) {
+  const uint32_t id = t.u.blas_del_prim_gpu.instance_index;
+  if (!used(a, id)) return u.error(CT_E_INVALID, "update_scene: DELETE_PRIMITIVE_FROM_GPU of an unused instance");
+  ct_instance &in = a.instances[id];
+  if (in.primitive_count <= t.u.blas_del_prim_gpu.del_prim_count) {
+    free_instance(a, u, id);
+    return;
+  }
+  in.primitive_count -= t.u.blas_del_prim_gpu.del_prim_count;
+  touch(a, u, id, in.primitive_count);
+  mark(u.dirty.rebuild, id);
+}
+
+// process_settling_task_queue (:2586-2680): the remapping tables are
+// cleared and the previous instance state becomes the current one.
+int settle(Accel &a, hipStream_t st) {
+  BvhSet &S = a.set;
+  for (const auto &r : a.settle_prims) {
+    if (!S.remap_prim.p || r.first >= S.remap_prim_cap) continue;
+    const uint32_t n = std::min(r.second, S.remap_prim_cap - r.first);
+    if (n) hipLaunchKernelGGL(k_fill_u32, dim3(blocks(n)), dim3(kBlock), 0, st, S.remap_prim.as<uint32_t>() + r.first, 0u, n);
+  }
+  for (uint32_t l : a.settle_lights)
+    if (S.remap_light.p && l < S.remap_light_cap)
+      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, st, S.remap_light.as<uint32_t>() + l, 0u, 1u);
+  CT_HIP(hipGetLastError());
+  a.settle_prims.clear();
+  a.settle_lights.clear();
+  for (size_t id = 0; id < a.instances.size(); ++id) {
+    std::memcpy(&a.prev_transform[16 * id], a.instances[id].transform, 16 * sizeof(float));
+    a.prev_first[id] = a.instances[id].first_primitive_index;
+  }
+  return upload_tables(a, st);
+}
+
+int scatter(hipStream_t st, const std::vector<uint2> &w, uint32_t *dst) {
+  if (w.empty()) return CT_OK;
+  DevBuf buf;
+  int rc;
+  if ((rc = buf.reserve(sizeof(uint2) * w.size()))) return rc;
+  CT_HIP(hipMemcpyAsync(buf.p, w.data(), sizeof(uint2) * w.size(), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_scatter_u32, dim3(blocks(w.size())), dim3(kBlock), 0, st, buf.as<uint2>(), (uint32_t)w.size(), dst);
+  CT_HIP(hipGetLastError());
+  CT_HIP(hipStreamSynchronize(st));  // `buf` is freed on return
+  return CT_OK;
+}
+
+}  // namespace
+}  // namespace ct
+
+extern "C" {
+
 int ct_accel_update_scene(ct_accel *a, int synchronize, void *stream, int *did_work) {
   if (!a) return fail(CT_E_INVALID, "update_scene: null handle");
   if (did_work) *did_work = 0;
-  if (a->tasks.empty()) return CT_OK;
+  if (a->tasks.empty() && a->status != CT_AS_SETTLING) return CT_OK;
   CT_HIP(hipSetDevice(a->device));
   if (stream) CT_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
-  // process_task_queue, accel_struct_mngr.cpp:2115-2443
-  uint32_t cursor = 0;
+  hipStream_t st = a->stream;
+  int rc;
+  // A caller's compaction pass ahead of DELETE_PRIMITIVE_BLAS_FROM_GPU has
+  // written device slots and remapping entries: take them before the
+  // settle clears the tables.
+  std::vector<ct_task> queue;
   while (!a->tasks.empty()) {
-    ct_task t = a->tasks.front();
+    queue.push_back(a->tasks.front());
     a->tasks.pop();
+  }
+  std::vector<std::pair<uint32_t, std::vector<uint32_t>>> gpu_remap;
+  for (const ct_task &t : queue) {
+    if (t.type != CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_GPU) continue;
+    const uint32_t id = t.u.blas_del_prim_gpu.instance_index;
+    if (!used(*a, id) || !a->set.aabbs.p) continue;
+    const ct_instance &in = a->instances[id];
+    const uint32_t f = in.first_primitive_index, n = in.primitive_count;
+    CT_HIP(hipMemcpyAsync(&a->aabbs[f], a->set.aabbs.as<ct_aabb>() + f, sizeof(ct_aabb) * n, hipMemcpyDeviceToHost, st));
+    CT_HIP(hipMemcpyAsync(&a->prims[f], a->set.prims.as<ct_primitive>() + f, sizeof(ct_primitive) * n,
+                          hipMemcpyDeviceToHost, st));
+    std::vector<uint32_t> rm(n);
+    CT_HIP(hipMemcpyAsync(rm.data(), a->set.remap_prim.as<uint32_t>() + f, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
+    CT_HIP(hipStreamSynchronize(st));
+    gpu_remap.emplace_back(f, std::move(rm));
+  }
+  bool worked = false;
+  if (a->status == CT_AS_SETTLING) {
+    if ((rc = settle(*a, st))) return rc;
+    a->status = CT_AS_IDLE;
+    worked = true;
+  }
+  if (queue.empty()) {
+    if (did_work) *did_work = worked ? 1 : 0;
+    return CT_OK;
+  }
+  // process_task_queue, accel_struct_mngr.cpp:2115-2443
+  a->status = CT_AS_UPDATING;
+  Update u;
+  uint32_t cursor = 0;
+  bool tables = false;
+  for (ct_task &t : queue) {
     switch (t.type) {
-      case CT_TASK_BUILD_BLAS_FROM_CPU: {
-        for (uint32_t q = 0; q < t.u.blas_build_from_cpu.instance_count; ++q, ++cursor) {
-          if (cursor >= a->max_instances) return fail(CT_E_CAPACITY, "update_scene: staging cursor out of range");
-          const ct_instance &ti = a->temp_instances[cursor];
-          uint32_t id = 0;
-          while (id < a->instance_used.size() && a->instance_used[id]) ++id;
-          if (id >= a->max_instances) return fail(CT_E_CAPACITY, "update_scene: no free instance id");
-          if (ti.primitive_count == 0) return fail(CT_E_INVALID, "update_scene: instance without primitives");
-          if ((uint64_t)ti.first_primitive_index + ti.primitive_count > a->max_primitives)
-            return fail(CT_E_INVALID, "update_scene: staged primitive range out of bounds");
-          uint32_t off = 0;
-          if (!a->prim_alloc.allocate(align32(ti.primitive_count), off))
-            return fail(CT_E_CAPACITY, "update_scene: primitive free list exhausted");
-          if (id >= a->instance_used.size()) {
-            a->instance_used.resize(id + 1, 0);
-            a->instances.resize(id + 1, ct_instance{});
-            a->w2o.resize(16 * (size_t)(id + 1), 0.0f);
-          }
-          a->instance_used[id] = 1;
-          uint32_t end = off + align32(ti.primitive_count);
-          if (end > a->prim_high_water) {
-            a->prim_high_water = end;
-            a->aabbs.resize(end);
-            a->prims.resize(end);
-          }
-          std::memcpy(&a->aabbs[off], &a->aabb_host[ti.first_primitive_index], sizeof(ct_aabb) * ti.primitive_count);
-          std::memcpy(&a->prims[off], &a->primitives_host[ti.first_primitive_index], sizeof(ct_primitive) * ti.primitive_count);
-          ct_instance &in = a->instances[id];
-          std::memcpy(in.transform, t.u.blas_build_from_cpu.transform, sizeof(in.transform));
-          in.first_primitive_index = off;
-          in.primitive_count = ti.primitive_count;
-          host_inverse(in.transform, &a->w2o[16 * (size_t)id]);
+      case CT_TASK_BUILD_BLAS_FROM_CPU:
+        task_build(*a, u, t, cursor);
+        tables = true;
+        break;
+      case CT_TASK_DELETE_BLAS_FROM_CPU: {
+        const uint32_t id = t.u.blas_delete_from_cpu.instance_index;
+        if (!used(*a, id)) {
+          u.error(CT_E_INVALID, "update_scene: DELETE_BLAS of an unused instance");
+          continue;
         }
+        t.u.blas_delete_from_cpu.first_primitive_index = a->instances[id].first_primitive_index;
+        t.u.blas_delete_from_cpu.deleted_primitive_count = a->instances[id].primitive_count;
+        free_instance(*a, u, id);
+        tables = true;
+        DoneTask d;
+        d.task = t;
+        u.completed.push_back(d);
         break;
       }
-      case CT_TASK_DELETE_BLAS_FROM_CPU: {
-        uint32_t id = t.u.blas_delete_from_cpu.instance_index;
-        if (id >= a->instance_used.size() || !a->instance_used[id])
-          return fail(CT_E_INVALID, "update_scene: DELETE_BLAS of an unused instance");
-        a->prim_alloc.deallocate(a->instances[id].first_primitive_index, align32(a->instances[id].primitive_count));
-        a->instance_used[id] = 0;
-        a->instances[id] = ct_instance{};
+      case CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_CPU: {
+        DoneTask d;
+        if (task_delete_primitive(*a, u, t, d)) u.completed.push_back(d);
+        tables = true;
+        break;
+      }
+      case CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_GPU: {
+        task_delete_from_gpu(*a, u, t);
+        tables = true;
+        DoneTask d;
+        d.task = t;
+        u.completed.push_back(d);
         break;
       }
       case CT_TASK_UPDATE_BLAS_FROM_CPU: {
-        uint32_t id = t.u.blas_update.instance_index;
-        if (id >= a->instance_used.size() || !a->instance_used[id])
-          return fail(CT_E_INVALID, "update_scene: UPDATE_BLAS of an unused instance");
-        if (t.u.blas_update.primitive_count != 0)
-          return fail(CT_E_UNSUPPORTED, "update_scene: UPDATE_BLAS with primitive changes is not supported");
-        std::memcpy(a->instances[id].transform, t.u.blas_update.transform, sizeof(float) * 16);
-        host_inverse(a->instances[id].transform, &a->w2o[16 * (size_t)id]);
+        bool touched = false;
+        task_update(*a, u, t, touched);
+        if (touched) {
+          tables = true;
+          DoneTask d;
+          d.task = t;
+          u.completed.push_back(d);
+        }
+        break;
+      }
+      case CT_TASK_UNDO_OP_CPU: {
+        if (a->done.empty()) break;
+        DoneTask d = a->done.back();
+        a->done.pop_back();
+        if (d.task.type == CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_CPU) undo_delete_primitive(*a, u, d);
+        tables = true;
         break;
       }
       default:
-        return fail(CT_E_UNSUPPORTED, "update_scene: task type " + std::to_string(t.type) + " is not supported");
+        break;
     }
   }
   a->temp_instance_count = 0;
   a->temp_primitive_count = 0;
-  a->status = CT_AS_UPDATING;
-  int rc = rebuild_all(*a);
-  a->status = CT_AS_IDLE;
-  if (rc) return rc;
-  if (synchronize) CT_HIP(hipStreamSynchronize(a->stream));
+  a->temp_prim_index_count = 0;
+  if (tables) {
+    if ((rc = commit(*a, u.dirty))) {
+      a->status = CT_AS_IDLE;
+      return rc;
+    }
+  }
+  // Remapping writes of this update (and the caller's, re-applied), then
+  // remembered for the settle.
+  for (auto &g : gpu_remap) {
+    for (uint32_t k = 0; k < g.second.size(); ++k)
+      if (g.second[k] != 0u) u.remap_prims.push_back(make_uint2(g.first + k, g.second[k]));
+  }
+  for (const uint2 &w : u.remap_prims)
+    if (w.x < a->set.remap_prim_cap) a->settle_prims.emplace_back(w.x, 1u);
+  for (const uint2 &w : u.remap_lights)
+    if (w.x < a->set.remap_light_cap) a->settle_lights.push_back(w.x);
+  std::vector<uint2> wp, wl;
+  for (const uint2 &w : u.remap_prims)
+    if (w.x < a->set.remap_prim_cap) wp.push_back(w);
+  for (const uint2 &w : u.remap_lights)
+    if (w.x < a->set.remap_light_cap) wl.push_back(w);
+  if ((rc = scatter(st, wp, a->set.remap_prim.as<uint32_t>())) || (rc = scatter(st, wl, a->set.remap_light.as<uint32_t>())))
+    return rc;
+  for (DoneTask &d : u.completed) {
+    a->done.push_back(d);
+    if (a->done.size() > kMaxDone) a->done.pop_front();
+  }
+  a->status = CT_AS_SETTLING;
+  if (synchronize) CT_HIP(hipStreamSynchronize(st));
   if (did_work) *did_work = 1;
+  if (u.first_error != CT_OK) return fail(u.first_error, u.message);
   return CT_OK;
 }
 
@@ -937,10 +1449,32 @@ int ct_accel_get_world(ct_accel *a, int previous, ct_world *o) {
   o->primitives = a->set.prims.p;
   o->aabbs = a->set.aabbs.p;
   o->cube_lights = a->set.cube_lights.p;
+  o->remapped_primitives = a->set.remap_prim.p;
+  o->remapped_cube_lights = a->set.remap_light.p;
   o->instance_count = (uint32_t)a->instances.size();
   o->primitive_capacity = a->set.prim_capacity;
   o->cube_light_count = a->set.cube_light_count;
-  o->pad = 0;
+  o->cube_light_capacity = a->set.remap_light_cap;
+  return CT_OK;
+}
+
+int ct_accel_read_world(ct_accel *a, ct_aabb *aabbs, ct_primitive *prims, uint32_t prim_capacity, ct_light *lights,
+                        uint32_t light_capacity, uint32_t *remap_prims, uint32_t *remap_lights) {
+  if (!a) return fail(CT_E_INVALID, "read_world: null handle");
+  CT_HIP(hipSetDevice(a->device));
+  CT_HIP(hipStreamSynchronize(a->stream));
+  const BvhSet &S = a->set;
+  const size_t np = std::min<size_t>(prim_capacity, S.prim_capacity);
+  const size_t nl = std::min<size_t>(light_capacity, S.cube_light_hw);
+  if (aabbs && np && S.aabbs.p) CT_HIP(hipMemcpy(aabbs, S.aabbs.p, sizeof(ct_aabb) * np, hipMemcpyDeviceToHost));
+  if (prims && np && S.prims.p) CT_HIP(hipMemcpy(prims, S.prims.p, sizeof(ct_primitive) * np, hipMemcpyDeviceToHost));
+  if (lights && nl && S.cube_lights.p) CT_HIP(hipMemcpy(lights, S.cube_lights.p, sizeof(ct_light) * nl, hipMemcpyDeviceToHost));
+  const size_t nr = std::min<size_t>(prim_capacity, S.remap_prim_cap);
+  if (remap_prims && nr && S.remap_prim.p)
+    CT_HIP(hipMemcpy(remap_prims, S.remap_prim.p, 4 * nr, hipMemcpyDeviceToHost));
+  const size_t nrl = std::min<size_t>(light_capacity, S.remap_light_cap);
+  if (remap_lights && nrl && S.remap_light.p)
+    CT_HIP(hipMemcpy(remap_lights, S.remap_light.p, 4 * nrl, hipMemcpyDeviceToHost));
   return CT_OK;
 }
 

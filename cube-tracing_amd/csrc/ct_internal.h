@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <memory>
+#include <deque>
 #include <queue>
 #include <string>
 #include <vector>
@@ -116,7 +117,10 @@ struct BvhSet {
   DevBuf nodes, leaves, instances_d, prims, aabbs, cube_lights, inst_aos;
   DevBuf tlas_nodes, tlas_inst, tlas_keys, tlas_parent;
   DevBuf blas_keys, blas_ids, blas_parent;  // kept for ct_accel_read_bvh
-  DevBuf geo_t;                             // per instance: transform * world-to-object (16 floats)
+  DevBuf geo_t;                             // per instance: previous transform * world-to-object (16 floats)
+  DevBuf prev_first;                        // per instance: first_primitive_index before the last update
+  DevBuf remap_prim, remap_light;           // remapping tables (u32 per slot / per cube light)
+  uint32_t remap_prim_cap = 0, remap_light_cap = 0;
   uint32_t tlas_root = kLeaf;
   uint32_t tlas_count = 0;
   uint32_t tlas_depth = 0;
@@ -124,6 +128,7 @@ struct BvhSet {
   uint32_t generation = 0;
   uint32_t prim_capacity = 0;
   uint32_t cube_light_count = 0;
+  uint32_t cube_light_hw = 0;   // entries held: deleted lights leave stale records past the count
 };
 
 struct FreeList {  // first-fit over [0, capacity) (containers/free_list.hpp:103-278)
@@ -132,6 +137,18 @@ struct FreeList {  // first-fit over [0, capacity) (containers/free_list.hpp:103
   void init(uint32_t capacity) { free_ranges.assign(1, Range{0, capacity}); }
   bool allocate(uint32_t size, uint32_t &off);
   void deallocate(uint32_t off, uint32_t size);
+};
+
+// A task completed by update_scene, kept for UNDO_OP_CPU.  For a primitive
+// deletion it holds what the undo needs: the instance's last index before
+// the deletion, the deleted light and the light moved into its place, and
+// the deleted records.
+struct DoneTask {
+  ct_task task{};
+  uint32_t last = 0, light = 0xFFFFFFFFu, light_moved = 0xFFFFFFFFu;
+  ct_aabb aabb{};
+  ct_primitive prim{};
+  ct_light light_rec{};
 };
 
 struct Accel {
@@ -143,13 +160,20 @@ struct Accel {
   std::vector<ct_primitive> primitives_host;
   std::vector<ct_aabb> aabb_host;
   std::vector<ct_light> cube_lights_host;
-  uint32_t temp_instance_count = 0, temp_primitive_count = 0;
+  std::vector<uint32_t> prim_index_host;    // UPDATE_BLAS primitive indices
+  uint32_t temp_instance_count = 0, temp_primitive_count = 0, temp_prim_index_count = 0;
   uint32_t cube_light_count = 0;
+  uint32_t light_high_water = 0;            // most cube lights ever held
   std::queue<ct_task> tasks;
   // committed host mirror of the device scene
   std::vector<ct_instance> instances;       // by instance id
   std::vector<uint8_t> instance_used;       // free_uuid_list state
   std::vector<float> w2o;                   // [id][16]
+  std::vector<float> prev_transform;        // [id][16]: transforms before the last update
+  std::vector<uint32_t> prev_first;         // [id]: first_primitive_index before the last update
+  std::deque<DoneTask> done;                // completed tasks, most recent last
+  std::vector<std::pair<uint32_t, uint32_t>> settle_prims;  // remap slot ranges to clear
+  std::vector<uint32_t> settle_lights;                      // remap light entries to clear
   std::vector<ct_aabb> aabbs;               // by global primitive index
   std::vector<ct_primitive> prims;
   std::vector<float> blas_root_box;         // [id][6]
@@ -162,8 +186,6 @@ struct Accel {
   uint32_t generation = 0;
 };
 
-// Builds every BLAS of `ids` and then the TLAS.  Implemented in accel.hip.
-int rebuild_all(Accel &a);
 
 // ------------------------------------------------------------- ctx
 struct FrameBufs {

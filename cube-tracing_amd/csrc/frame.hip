@@ -57,7 +57,9 @@ struct DFrame {
   const float4 *cube;   // [3 per light]: (pos, size), (emissive, type), (inst, prim, -, -)
   const float4 *point;
   const float4 *env;
-  const float *geo_t;   // [16 per instance] geometry_T = model * inverse(model)
+  const float *geo_t;   // [16 per instance] geometry_T = previous model * inverse(model)
+  const uint32_t *remap_prim, *remap_light, *prev_first;  // remapping tables, previous first indices
+  uint32_t remap_cap, remap_light_cap;
   float prev_vp[16];    // inverse(prev_inv_proj) * inverse(prev_inv_view)
   float4 *dpos_c, *dnrm_c, *dorg_c;
   uint4 *dhit_c;
@@ -519,10 +521,14 @@ CT_D Res ris(const DFrame &F, float confidence, F3 ray_d, Hit &hit, const Mat &m
   return r;
 }
 
-// TEMPORAL_REUSE, restir_resampling.glsl:311-416 (remapping inactive)
+// TEMPORAL_REUSE, restir_resampling.glsl:311-416.  With CT_REMAP_BIT the
+// previous frame's hit primitive and cube light go through the remapping
+// tables (:338-345, :380-387): a deleted primitive ends the reuse, a moved
+// one or a moved light takes its current index.
 CT_D void temporal_reuse(const DFrame &F, Res &r, Res prev, uint32_t pidx, F3 ray_d, const Hit &hit, const Mat &mat,
                          uint32_t &seed, Vis &v) {
   const float mcap = F.rp.max_influence_from_the_past;
+  const bool remap = (F.active & CT_REMAP_BIT) != 0u;
   prev.M = gmin(prev.M, mcap * r.M);
   const float prev_M = prev.M;
   const float new_M = r.M + prev_M;
@@ -530,6 +536,12 @@ CT_D void temporal_reuse(const DFrame &F, Res &r, Res prev, uint32_t pidx, F3 ra
     float tlp = 0.0f;
     if (luminance(r.F) > 1e-6f) {
       DI dp = ld_di(F.dpos_p, F.dnrm_p, F.dorg_p, F.dhit_p, pidx);
+      if (remap) {
+        const uint32_t slot = (dp.inst < F.S.n_inst ? F.prev_first[dp.inst] : 0u) + dp.prim;
+        const uint32_t m = slot < F.remap_cap ? F.remap_prim[slot] : 0u;
+        if (m == kInvalid) return;
+        if (m != 0u) dp.prim = m;
+      }
       Hit ph = hit_from_di(dp);
       Mat pm = ld_mat(F, dp.mat);
       F3 pd = normalize(sub(ph.P, dp.org));
@@ -538,6 +550,10 @@ CT_D void temporal_reuse(const DFrame &F, Res &r, Res prev, uint32_t pidx, F3 ra
     const float p_curr = r.M * luminance(r.F);
     const float m_curr = p_curr / gmax(p_curr + prev_M * tlp, 1e-6f);
     r.W_sum *= m_curr;
+  }
+  if (remap && prev.type == CT_LIGHT_CUBE && prev.Y < F.remap_light_cap) {
+    const uint32_t m = F.remap_light[prev.Y];
+    if (m != 0u) prev.Y = m;
   }
   if (prev.Y != kInvalid) {
     F3 ct = res_radiance(F, prev, ray_d, hit, mat, v);
@@ -1502,6 +1518,7 @@ int prepare_frame(Ctx *c, Accel *a, const ct_frame_params *p, hipStream_t st, DF
   std::memcpy(F.inv_proj, p->inv_proj, sizeof(F.inv_proj));
   F.rp = p->restir;
   F.cfg = c->cfg;
+  F.cfg.cube_light_count = ncube;  // the manager owns this count (accel_struct_mngr.cpp:2580)
   F.S = scene_view(*a);
   F.aabbs = a->set.aabbs.as<ct_aabb>();
   F.prims = a->set.prims.as<ct_primitive>();
@@ -1510,6 +1527,11 @@ int prepare_frame(Ctx *c, Accel *a, const ct_frame_params *p, hipStream_t st, DF
   F.point = c->point_lights.as<float4>();
   F.env = c->env_lights.as<float4>();
   F.geo_t = a->set.geo_t.as<float>();
+  F.remap_prim = a->set.remap_prim.as<uint32_t>();
+  F.remap_light = a->set.remap_light.as<uint32_t>();
+  F.prev_first = a->set.prev_first.as<uint32_t>();
+  F.remap_cap = a->set.remap_prim_cap;
+  F.remap_light_cap = a->set.remap_light_cap;
   std::memcpy(F.prev_vp, p->prev_view_proj, sizeof(F.prev_vp));
   F.dpos_c = b.di_pos[0].as<float4>(); F.dnrm_c = b.di_nrm[0].as<float4>(); F.dorg_c = b.di_org[0].as<float4>();
   F.dhit_c = b.di_hit[0].as<uint4>();

@@ -40,6 +40,9 @@ CT_RIS_POINT_LIGHT_BIT = 1 << 1
 CT_RIS_ENV_LIGHT_BIT = 1 << 2
 CT_RIS_CUBE_LIGHT_BIT = 1 << 3
 CT_RIS_BRDF_BIT = 1 << 4
+CT_REMAP_BIT = 1 << 30
+
+CT_AS_IDLE, CT_AS_UPDATING, CT_AS_SWITCHING, CT_AS_SETTLING = 0, 1, 2, 4
 
 CT_LIGHT_NONE, CT_LIGHT_POINT, CT_LIGHT_CUBE, CT_LIGHT_ENV_MAP = 0, 1, 2, 5
 
@@ -158,8 +161,9 @@ class Tlas(C.Structure):
 
 class World(C.Structure):
     _fields_ = [("instances", C.c_void_p), ("primitives", C.c_void_p), ("aabbs", C.c_void_p),
-                ("cube_lights", C.c_void_p), ("instance_count", u32), ("primitive_capacity", u32),
-                ("cube_light_count", u32), ("pad", u32)]
+                ("cube_lights", C.c_void_p), ("remapped_primitives", C.c_void_p),
+                ("remapped_cube_lights", C.c_void_p), ("instance_count", u32), ("primitive_capacity", u32),
+                ("cube_light_count", u32), ("cube_light_capacity", u32)]
 
 
 class BvhArrays(C.Structure):
@@ -246,7 +250,8 @@ STRUCT_SIZES = {"ct_aabb": (Aabb, 24), "ct_instance": (Instance, 72), "ct_primit
                 "ct_material": (Material, 84), "ct_light": (Light, 40), "ct_light_config": (LightConfig, 52),
                 "ct_reservoir": (Reservoir, 36), "ct_di_info": (DiInfo, 60), "ct_path_reservoir": (PathReservoir, 92),
                 "ct_restir_params": (RestirParams, 108), "ct_adjoint_view": (AdjointView, 136),
-                "ct_adjoint_desc": (AdjointDesc, 16), "ct_batch_tuning": (BatchTuning, 16)}
+                "ct_adjoint_desc": (AdjointDesc, 16), "ct_batch_tuning": (BatchTuning, 16),
+                "ct_task": (Task, 84), "ct_world": (World, 64)}
 
 
 # ----------------------------------------------------------------- loader
@@ -266,6 +271,10 @@ _SIGS = {
     "ct_accel_get_staging": ([C.c_void_p, C.POINTER(Staging)], C.c_int),
     "ct_accel_set_cube_light_count": ([C.c_void_p, u32], C.c_int),
     "ct_accel_request_aabb_host_buffer_count": ([C.c_void_p, u32, C.POINTER(C.POINTER(Aabb)), C.POINTER(u32)], C.c_int),
+    "ct_accel_request_primitive_index_host_buffer_count": ([C.c_void_p, u32, C.POINTER(C.POINTER(u32)),
+                                                            C.POINTER(u32)], C.c_int),
+    "ct_accel_read_world": ([C.c_void_p, C.POINTER(Aabb), C.POINTER(Primitive), u32, C.POINTER(Light), u32,
+                             C.POINTER(u32), C.POINTER(u32)], C.c_int),
     "ct_accel_task_queue_add": ([C.c_void_p, C.POINTER(Task)], C.c_int),
     "ct_accel_update_scene": ([C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_int)], C.c_int),
     "ct_accel_get_status": ([C.c_void_p, C.POINTER(u32)], C.c_int),
@@ -483,6 +492,90 @@ class Accel:
         out = (Instance * max(cnt.value, 1))()
         check(lib().ct_accel_read_instances(self.handle, out, cnt.value, C.byref(cnt)))
         return out[: cnt.value]
+
+    def status(self) -> int:
+        s = u32(0)
+        check(lib().ct_accel_get_status(self.handle, C.byref(s)))
+        return s.value
+
+    def remapping_active(self) -> bool:
+        a = C.c_int(0)
+        check(lib().ct_accel_is_remapping_active(self.handle, C.byref(a)))
+        return bool(a.value)
+
+    def world(self) -> World:
+        w = World()
+        check(lib().ct_accel_get_world(self.handle, 0, C.byref(w)))
+        return w
+
+    def read_world(self) -> dict:
+        """Device contents of the WORLD buffers: aabbs [cap, 6], primitives
+        [cap, 2] by slot, the cube lights held (stale records past the
+        count included), and the two remapping tables."""
+        w = self.world()
+        cap, lcap = w.primitive_capacity, max(w.cube_light_capacity, 1)
+        aabbs = np.zeros((max(cap, 1), 6), np.float32)
+        prims = np.zeros((max(cap, 1), 2), np.uint32)
+        lights = (Light * lcap)()
+        rp = np.zeros(max(cap, 1), np.uint32)
+        rl = np.zeros(lcap, np.uint32)
+        check(lib().ct_accel_read_world(self.handle, aabbs.ctypes.data_as(C.POINTER(Aabb)),
+                                        prims.ctypes.data_as(C.POINTER(Primitive)), cap, lights, lcap,
+                                        rp.ctypes.data_as(C.POINTER(u32)), rl.ctypes.data_as(C.POINTER(u32))))
+        return {"aabbs": aabbs[:cap], "prims": prims[:cap], "lights": list(lights), "light_count": w.cube_light_count,
+                "remap_prims": rp[:cap], "remap_lights": rl, "capacity": cap}
+
+    # -- tasks (TASK kinds, accel_struct_mngr.hpp:40-107)
+    def delete_blas(self, instance: int):
+        t = Task()
+        t.type = CT_TASK_DELETE_BLAS_FROM_CPU
+        t.u.blas_delete_from_cpu.instance_index = instance
+        self.task_queue_add(t)
+
+    def delete_primitive(self, instance: int, primitive: int):
+        t = Task()
+        t.type = CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_CPU
+        t.u.blas_delete_primitive_from_cpu.instance_index = instance
+        t.u.blas_delete_primitive_from_cpu.del_primitive_index = primitive
+        self.task_queue_add(t)
+
+    def delete_primitives_from_gpu(self, instance: int, count: int):
+        t = Task()
+        t.type = CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_GPU
+        t.u.blas_del_prim_gpu.instance_index = instance
+        t.u.blas_del_prim_gpu.del_prim_count = count
+        self.task_queue_add(t)
+
+    def update_blas(self, instance: int, transform=None, edits=()):
+        """transform (column-major 16, multiplied onto the instance's) and
+        edits [(primitive index, aabb min+max)] staged for the task."""
+        t = Task()
+        t.type = CT_TASK_UPDATE_BLAS_FROM_CPU
+        t.u.blas_update.instance_index = instance
+        tr = np.eye(4, dtype=np.float32).reshape(16) if transform is None else np.asarray(transform, np.float32)
+        t.u.blas_update.transform[:] = tr.reshape(16).tolist()
+        edits = list(edits)
+        if edits:
+            pbase, poff = C.POINTER(u32)(), u32(0)
+            check(lib().ct_accel_request_primitive_index_host_buffer_count(self.handle, len(edits), C.byref(pbase),
+                                                                            C.byref(poff)))
+            abase, aoff = C.POINTER(Aabb)(), u32(0)
+            check(lib().ct_accel_request_aabb_host_buffer_count(self.handle, len(edits), C.byref(abase),
+                                                                 C.byref(aoff)))
+            for k, (p, box) in enumerate(edits):
+                pbase[poff.value + k] = int(p)
+                b = abase[aoff.value + k]
+                b.minimum[:] = [float(v) for v in box[:3]]
+                b.maximum[:] = [float(v) for v in box[3:6]]
+            t.u.blas_update.primitive_count = len(edits)
+            t.u.blas_update.primitive_index_buf_offset = poff.value
+            t.u.blas_update.aabb_buf_offset = aoff.value
+        self.task_queue_add(t)
+
+    def undo(self):
+        t = Task()
+        t.type = CT_TASK_UNDO_OP_CPU
+        self.task_queue_add(t)
 
     def read_bvh(self, instance: int, n: int) -> dict:
         keys = np.zeros(n, np.uint32)

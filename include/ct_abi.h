@@ -184,8 +184,50 @@ int ct_accel_set_cube_light_count(ct_accel *accel, uint32_t count);
 /* request_aabb_host_buffer_count, accel_struct_mngr.hpp:279-285 */
 int ct_accel_request_aabb_host_buffer_count(ct_accel *accel, uint32_t count,
                                             ct_aabb **base, uint32_t *offset);
+/* request_primitive_index_host_buffer_count, accel_struct_mngr.hpp:293-299:
+ * staging for the primitive indices (within the instance) whose AABBs an
+ * UPDATE_BLAS_FROM_CPU task replaces. */
+int ct_accel_request_primitive_index_host_buffer_count(ct_accel *accel, uint32_t count,
+                                                       uint32_t **base, uint32_t *offset);
 
-/* TASK, accel_struct_mngr.hpp:40-107 */
+/* TASK, accel_struct_mngr.hpp:40-107; processed by update_scene in queue
+ * order (process_task_queue, accel_struct_mngr.cpp:2115-2443):
+ *
+ * BUILD_BLAS_FROM_CPU: instance_count staged instances, each in the lowest
+ *   free instance id, its primitives in a 32-aligned slot range.
+ * DELETE_BLAS_FROM_CPU: frees the instance id and its slot range (its
+ *   cube lights stay in the light list, as in delete_blas_process).
+ * DELETE_PRIMITIVE_BLAS_FROM_CPU: removes primitive del_primitive_index of
+ *   the instance by moving the instance's last primitive into its place;
+ *   if it was a cube light, the last light of the list moves into the
+ *   light's place.  The back-references (PRIMITIVE.light_index and
+ *   LIGHT.instance_info) of everything moved are rewritten.  The BLAS is
+ *   rebuilt.  (delete_aabb_device_buffer :607-732, delete_light_device_buffer
+ *   :1155-1196; the build applies the swap the same way whether or not the
+ *   deleted primitive is the instance's last, and resets the light state per
+ *   task, where the reference carries it over between tasks.)
+ * DELETE_PRIMITIVE_BLAS_FROM_GPU: the caller's own pass has already
+ *   compacted the instance's primitives on the device (and written the
+ *   remapping table); the instance loses its last del_prim_count primitives
+ *   and is rebuilt, or is deleted when that leaves none (:2258-2290).
+ * UPDATE_BLAS_FROM_CPU: transform = transform * task transform (column-major
+ *   product, :2303-2304); then, for primitive_count > 0, the AABBs staged at
+ *   aabb_buf_offset replace those of the primitive indices staged at
+ *   primitive_index_buf_offset (indices >= the instance's count are
+ *   skipped), and the BLAS is refitted in place: same topology, bounds
+ *   recomputed bottom up (update_blases :1672-1852).
+ * UNDO_OP_CPU: reverts the most recent task completed by an earlier
+ *   update_scene call.  A primitive deletion is restored exactly (primitive,
+ *   AABB, light and every moved record back in place); other kinds are
+ *   popped without effect, as process_undo_task_queue does (:853-913).
+ *
+ * Remapping tables (REMAPPED_PRIMITIVE_BUFFER / REMAPPED_CUBE_LIGHT_BUFFER,
+ * shared.inl:280-281), one u32 per primitive slot and per cube light, map a
+ * previous frame's primitive (by slot) or light (by index) to the current
+ * primitive index within its instance, or light index: 0 = unchanged,
+ * CT_INVALID_INDEX = deleted.  They are filled by deletions and undos and
+ * cleared by the next update_scene call (process_settling_task_queue
+ * :2586-2680). */
 #define CT_TASK_BUILD_BLAS_FROM_CPU 0u
 #define CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_CPU 1u
 #define CT_TASK_DELETE_BLAS_FROM_CPU 2u
@@ -231,10 +273,20 @@ typedef struct ct_task {
 /* task_queue_add, accel_struct_mngr.hpp:321-330 */
 int ct_accel_task_queue_add(ct_accel *accel, const ct_task *task);
 
-/* update_scene, accel_struct_mngr.hpp:332-436.  Runs the pending phase
- * stream-ordered on `stream`; with synchronize != 0 it returns after the
- * stream has drained (the reference blocks on synchronize_cv).
- * *did_work (optional) receives 1 when a phase ran, 0 when idle. */
+/* update_scene, accel_struct_mngr.hpp:332-436, called once per frame before
+ * the frame as App::update does (main.cpp:1420).  The reference spreads an
+ * update over three calls (UPDATING, SWITCHING, SETTLING); here one call
+ * settles the previous update (clears the remapping tables, makes the
+ * previous instance transforms equal the current ones: status IDLE), then
+ * processes the whole queue and rebuilds what changed (status SETTLING
+ * until the next call).  A frame rendered while SETTLING sees the instance
+ * transforms from before the update as the previous frame's
+ * (get_geometry_previous_transform_from_instance_id, motion_vectors.glsl:72)
+ * and, with CT_REMAP_BIT set, applies the remapping tables in temporal
+ * reuse (restir_resampling.glsl:338-387).  The work is ordered on the
+ * accel's stream after `stream` has drained; with synchronize != 0 the call
+ * returns after the build.  *did_work (optional) receives 1 when a task or
+ * a settle ran, 0 when idle. */
 int ct_accel_update_scene(ct_accel *accel, int synchronize, void *stream,
                           int *did_work);
 
@@ -264,15 +316,26 @@ int ct_accel_get_tlas(ct_accel *accel, int previous, ct_tlas *out);
  * accel_struct_mngr.hpp:225-262. */
 typedef struct ct_world {
   const void *instances;   /* ct_instance AoS */
-  const void *primitives;  /* ct_primitive AoS */
-  const void *aabbs;       /* ct_aabb AoS */
+  void *primitives;        /* ct_primitive AoS by slot */
+  void *aabbs;             /* ct_aabb AoS by slot */
   const void *cube_lights; /* ct_light AoS */
+  void *remapped_primitives;  /* u32 by slot (remapped_primitive_address) */
+  void *remapped_cube_lights; /* u32 by light (remapped_cube_light_address) */
   uint32_t instance_count;
   uint32_t primitive_capacity;
   uint32_t cube_light_count;
-  uint32_t pad;
+  uint32_t cube_light_capacity;
 } ct_world;
+/* Writable members are for a caller's own compaction pass ahead of a
+ * DELETE_PRIMITIVE_BLAS_FROM_GPU task (rearrangement.slang writes the same
+ * buffers). */
 int ct_accel_get_world(ct_accel *accel, int previous, ct_world *out);
+/* Host copies of the committed WORLD buffers as the device holds them:
+ * aabbs and primitives by slot [prim_capacity], cube lights
+ * [light_capacity], and the two remapping tables.  Any pointer may be NULL. */
+int ct_accel_read_world(ct_accel *accel, ct_aabb *aabbs, ct_primitive *prims,
+                        uint32_t prim_capacity, ct_light *lights, uint32_t light_capacity,
+                        uint32_t *remapped_primitives, uint32_t *remapped_cube_lights);
 
 /* Read back one BLAS (instance_index) or the TLAS (instance_index ==
  * CT_INVALID_INDEX) as built on the GPU, for parity tests.  Arrays are

@@ -438,7 +438,13 @@ struct Scene {
   ct_light_config cfg{};
   std::vector<Bvh> blas;
   Bvh tlas;
+  std::vector<uint32_t> tlas_inst;  // TLAS item -> instance id (instances with primitives)
   uint32_t prim_capacity = 0;
+  // The previous frame's scene as the frame after an update sees it:
+  // instance transforms and first indices, and the remapping tables
+  // (0 = unchanged, kInvalid = deleted).  Empty = same as the current.
+  std::vector<float> prev_transform;
+  std::vector<uint32_t> prev_first, remap_prim, remap_light;
 };
 
 struct TraceCount {
@@ -524,9 +530,10 @@ static bool traverse(const Scene &sc, V3 o, V3 d, float limit, uint32_t ti, uint
   int sp = 0;
   V3 winv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
   uint32_t ref = sc.tlas.root_ref;
+  if (sc.tlas.n == 0) return false;
   for (;;) {
     if (ref & kLeaf) {
-      uint32_t ii = sc.tlas.ids[ref & ~kLeaf];
+      uint32_t ii = sc.tlas_inst[sc.tlas.ids[ref & ~kLeaf]];
       cnt.instances++;
       const float *m = &sc.w2o[16 * (size_t)ii];
       V3 oo = xform_point(m, o), od = xform_dir(m, d);
@@ -608,7 +615,7 @@ static ClosestState trace_bounce(const Scene &sc, V3 o, V3 d, TraceCount &cnt) {
 
 static void build_scene_bvh(Scene &sc) {
   const uint32_t ni = (uint32_t)sc.inst.size();
-  sc.blas.resize(ni);
+  sc.blas.assign(ni, Bvh{});
   std::vector<float> tl_c(3 * (size_t)ni), tl_lo(3 * (size_t)ni), tl_hi(3 * (size_t)ni);
   const float r = kHalfVoxel + kLeafPad;
 #pragma omp parallel for schedule(dynamic, 1)
@@ -616,6 +623,7 @@ static void build_scene_bvh(Scene &sc) {
     const uint32_t i = (uint32_t)ii;
     const ct_instance &in = sc.inst[i];
     uint32_t n = in.primitive_count;
+    if (n == 0) continue;  // an unused instance id
     std::vector<float> c(3 * (size_t)n), lo(3 * (size_t)n), hi(3 * (size_t)n);
     for (uint32_t k = 0; k < n; ++k) {
       const ct_aabb &bb = sc.aabbs[in.first_primitive_index + k];
@@ -642,7 +650,19 @@ static void build_scene_bvh(Scene &sc) {
       tl_c[3 * i + a] = (tl_lo[3 * i + a] + tl_hi[3 * i + a]) * 0.5f;
     }
   }
-  build_lbvh(ni, tl_c.data(), tl_lo.data(), tl_hi.data(), sc.tlas);
+  // TLAS over the instances that hold primitives, in id order.
+  sc.tlas_inst.clear();
+  std::vector<float> uc, ul, uh;
+  for (uint32_t i = 0; i < ni; ++i) {
+    if (sc.inst[i].primitive_count == 0) continue;
+    sc.tlas_inst.push_back(i);
+    for (int a = 0; a < 3; ++a) {
+      uc.push_back(tl_c[3 * i + a]);
+      ul.push_back(tl_lo[3 * i + a]);
+      uh.push_back(tl_hi[3 * i + a]);
+    }
+  }
+  build_lbvh((uint32_t)sc.tlas_inst.size(), uc.data(), ul.data(), uh.data(), sc.tlas);
 }
 
 // ------------------------------------------------------- DI frame state
@@ -1005,17 +1025,34 @@ static inline Hit hit_from_di(const ct_di_info &d) {
   return h;
 }
 
-// TEMPORAL_REUSE, restir_resampling.glsl:311-416 (remapping inactive)
+// The remapping tables' entry for an index, 0 (unchanged) outside them.
+static uint32_t remap_entry(const std::vector<uint32_t> &t, uint64_t i) { return i < t.size() ? t[i] : 0u; }
+
+// TEMPORAL_REUSE, restir_resampling.glsl:311-416.  With REMAP_BIT the
+// previous hit's primitive (:338-345, looked up by the previous instance
+// table's first index, primitives.glsl:133-137) and the previous cube
+// light (:380-387) go through the remapping tables; the light is evaluated
+// at its current index.
 static void temporal_reuse(const Ctx &c, Res &r, Res prev, uint32_t pidx, V3 ray_d, Hit &hit,
                            const ct_material &mat, uint32_t &seed, TraceCount &cnt) {
   const float mcap = c.p->restir.max_influence_from_the_past;
+  const bool remap = (c.p->is_active & CT_REMAP_BIT) != 0;
+  const Scene &sc = *c.sc;
   prev.M = gmin(prev.M, mcap * r.M);
   const float prev_M = prev.M;
   const float new_M = r.M + prev_M;
   if (r.W_sum > 0.0f) {
     float target_lum_at_prev = 0.0f;
     if (luminance(r.F) > 1e-6f) {
-      const ct_di_info &dp = c.fr->di_prev[pidx];
+      ct_di_info dp = c.fr->di_prev[pidx];
+      if (remap) {
+        const uint32_t pi = dp.instance_hit.instance_id;
+        uint64_t first = 0;
+        if (pi < sc.inst.size()) first = sc.prev_first.empty() ? sc.inst[pi].first_primitive_index : sc.prev_first[pi];
+        const uint32_t m = remap_entry(sc.remap_prim, first + dp.instance_hit.primitive_id);
+        if (m == kInvalid) return;
+        if (m != 0u) dp.instance_hit.primitive_id = m;
+      }
       Hit ph = hit_from_di(dp);
       const ct_material &pm = material(c, dp.mat_index);
       V3 pd = normalize(sub(ph.world_hit, load3(dp.ray_origin)));
@@ -1024,6 +1061,10 @@ static void temporal_reuse(const Ctx &c, Res &r, Res prev, uint32_t pidx, V3 ray
     const float p_curr = r.M * luminance(r.F);
     const float m_curr = p_curr / gmax(p_curr + prev_M * target_lum_at_prev, 1e-6f);
     r.W_sum *= m_curr;
+  }
+  if (remap && prev.l_type == CT_LIGHT_CUBE) {
+    const uint32_t m = remap_entry(sc.remap_light, prev.Y);
+    if (m != 0u) prev.Y = m;
   }
   uint32_t prev_light_index = prev.Y;
   if (prev_light_index != kInvalid) {
@@ -2210,15 +2251,34 @@ oc_scene *oc_scene_create(uint32_t n_inst, const ct_instance *instances,
   if (n_point) s->point.assign(point_lights, point_lights + n_point);
   if (n_env) s->env.assign(env_lights, env_lights + n_env);
   s->cfg = *config;
-  for (uint32_t i = 0; i < n_inst; ++i)
-    if (instances[i].primitive_count == 0 ||
-        (uint64_t)instances[i].first_primitive_index + instances[i].primitive_count >
-            prim_capacity) {
+  uint32_t n_used = 0;
+  for (uint32_t i = 0; i < n_inst; ++i) {
+    n_used += instances[i].primitive_count > 0;
+    if ((uint64_t)instances[i].first_primitive_index + instances[i].primitive_count > prim_capacity) {
       delete s;
       return nullptr;
     }
+  }
+  if (n_used == 0) {
+    delete s;
+    return nullptr;
+  }
   build_scene_bvh(*s);
   return s;
+}
+
+int oc_scene_set_previous(oc_scene *s, const float *prev_transforms, const uint32_t *prev_first,
+                          const uint32_t *remap_prims, uint32_t n_remap_prims,
+                          const uint32_t *remap_lights, uint32_t n_remap_lights) {
+  if (!s) return CT_E_INVALID;
+  const size_t ni = s->inst.size();
+  s->prev_transform.clear();
+  s->prev_first.clear();
+  if (prev_transforms) s->prev_transform.assign(prev_transforms, prev_transforms + 16 * ni);
+  if (prev_first) s->prev_first.assign(prev_first, prev_first + ni);
+  s->remap_prim.assign(remap_prims ? remap_prims : nullptr, remap_prims ? remap_prims + n_remap_prims : nullptr);
+  s->remap_light.assign(remap_lights ? remap_lights : nullptr, remap_lights ? remap_lights + n_remap_lights : nullptr);
+  return CT_OK;
 }
 
 void oc_scene_destroy(oc_scene *scene) { delete scene; }
@@ -2426,6 +2486,12 @@ void oc_trace_any(oc_scene *scene, uint32_t n, const float *origins, const float
   }
 }
 
+int oc_frame_set_scene(oc_frame *frame, oc_scene *scene) {
+  if (!frame || !scene) return CT_E_INVALID;
+  frame->sc = scene;
+  return CT_OK;
+}
+
 oc_frame *oc_frame_create(oc_scene *scene, uint32_t width, uint32_t height) {
   if (!scene || width == 0 || height == 0) return nullptr;
   oc_frame *f = new oc_frame();
@@ -2473,7 +2539,8 @@ int oc_frame_render(oc_frame *frame, const ct_frame_params *params, int threads,
   for (size_t i = 0; i < ni; ++i) {
     float g[16];
     // geometry_T = previous_model * inverse(instance_model), motion_vectors.glsl:146
-    mat4_mul(sc.inst[i].transform, &sc.w2o[16 * i], g);
+    const float *prev_model = sc.prev_transform.empty() ? sc.inst[i].transform : &sc.prev_transform[16 * i];
+    mat4_mul(prev_model, &sc.w2o[16 * i], g);
     mat4_mul(p.prev_view_proj, g, &c.pt[16 * i]);
   }
   frame->cur ^= 1;

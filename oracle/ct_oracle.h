@@ -51,7 +51,8 @@ int oc_lbvh_build(uint32_t n, const float *centers, const float *lo,
 
 typedef struct oc_scene oc_scene;
 /* world_to_object: [n_inst][16]; prims/aabbs indexed by
- * first_primitive_index + primitive id (capacity prim_capacity). */
+ * first_primitive_index + primitive id (capacity prim_capacity).  An
+ * instance with primitive_count 0 is an unused id (not in the TLAS). */
 oc_scene *oc_scene_create(uint32_t n_inst, const ct_instance *instances,
                           const float *world_to_object, uint32_t prim_capacity,
                           const ct_aabb *aabbs, const ct_primitive *prims,
@@ -61,6 +62,14 @@ oc_scene *oc_scene_create(uint32_t n_inst, const ct_instance *instances,
                           uint32_t n_env, const ct_light *env_lights,
                           const ct_light_config *config);
 void oc_scene_destroy(oc_scene *scene);
+/* The previous frame's scene for the frame after an update: instance
+ * transforms [n_inst][16] and first primitive indices [n_inst] before the
+ * update (NULL: the current ones), and the remapping tables by slot and by
+ * cube light (0 = unchanged, CT_INVALID_INDEX = deleted), read when the
+ * frame's is_active has CT_REMAP_BIT. */
+int oc_scene_set_previous(oc_scene *scene, const float *prev_transforms, const uint32_t *prev_first,
+                          const uint32_t *remap_prims, uint32_t n_remap_prims,
+                          const uint32_t *remap_lights, uint32_t n_remap_lights);
 /* Arrays of one BLAS (instance) or of the TLAS (CT_INVALID_INDEX). */
 int oc_scene_read_bvh(oc_scene *scene, uint32_t instance, ct_bvh_arrays *arrays);
 
@@ -95,6 +104,9 @@ int oc_adjoint_backward(oc_scene *scene, const ct_adjoint_desc *desc,
 
 typedef struct oc_frame oc_frame;
 oc_frame *oc_frame_create(oc_scene *scene, uint32_t width, uint32_t height);
+/* Later frames render `scene` (the scene after an update) while keeping
+ * the frame history. */
+int oc_frame_set_scene(oc_frame *frame, oc_scene *scene);
 void oc_frame_destroy(oc_frame *frame);
 /* One frame (prepass, shading, TAA) with up to `threads` OpenMP threads
  * (0 = default).  Returns CT_OK or a negative CT_E_* code.

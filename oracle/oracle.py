@@ -67,6 +67,9 @@ def lib():
                                      C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
             "oc_random_cosine_direction": ([P(u32), P(f32), P(f32)], None),
             "oc_frame_create": ([C.c_void_p, u32, u32], C.c_void_p),
+            "oc_frame_set_scene": ([C.c_void_p, C.c_void_p], C.c_int),
+            "oc_scene_set_previous": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, u32, C.c_void_p, u32],
+                                      C.c_int),
             "oc_frame_destroy": ([C.c_void_p], None),
             "oc_frame_render": ([C.c_void_p, P(FrameParams), C.c_int, u32, u32], C.c_int),
             "oc_frame_read": ([C.c_void_p, u32, C.c_void_p, C.c_uint64], C.c_int),
@@ -210,6 +213,20 @@ class OScene:
         return cls(list(scene.instances), w2o, scene.aabbs, scene.primitives, list(scene.materials),
                    list(scene.lights), **kw)
 
+    def set_previous(self, prev_transforms=None, prev_first=None, remap_prims=None, remap_lights=None):
+        """The previous frame's instance transforms [n, 16] and first
+        indices [n] (None: the current ones) and the remapping tables."""
+        self._prev_t = None if prev_transforms is None else np.ascontiguousarray(prev_transforms, np.float32)
+        self._prev_f = None if prev_first is None else np.ascontiguousarray(prev_first, np.uint32)
+        self._rp = np.ascontiguousarray(remap_prims if remap_prims is not None else [], np.uint32)
+        self._rl = np.ascontiguousarray(remap_lights if remap_lights is not None else [], np.uint32)
+        rc = lib().oc_scene_set_previous(self.handle, None if self._prev_t is None else self._prev_t.ctypes.data,
+                                         None if self._prev_f is None else self._prev_f.ctypes.data,
+                                         self._rp.ctypes.data if self._rp.size else None, self._rp.size,
+                                         self._rl.ctypes.data if self._rl.size else None, self._rl.size)
+        if rc != 0:
+            raise ValueError(f"oc_scene_set_previous: {rc}")
+
     def read_bvh(self, instance, n):
         keys = np.zeros(n, np.uint32)
         ids = np.zeros(n, np.uint32)
@@ -304,6 +321,12 @@ class OFrame:
         self.scene = scene
         self.w, self.h = w, h
         self.handle = lib().oc_frame_create(scene.handle, w, h)
+
+    def set_scene(self, scene: OScene):
+        """Render the following frames with `scene`, keeping the history."""
+        self.scene = scene
+        if lib().oc_frame_set_scene(self.handle, scene.handle) != 0:
+            raise ValueError("oc_frame_set_scene rejected the scene")
 
     def render(self, params, threads=0, rows=None):
         r0, r1 = (0, self.h) if rows is None else rows

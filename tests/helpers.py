@@ -58,3 +58,43 @@ def res_array(arr):
 
 def pres_array(arr):
     return np.frombuffer(bytes(arr), np.uint32).reshape(len(arr), 23)
+
+
+_HIP = None
+
+
+def hip_runtime():
+    """The HIP runtime library this process already runs (the one torch
+    and libctamd.so share), for raw device copies in tests."""
+    global _HIP
+    if _HIP is None:
+        ctamd.lib()
+        path = None
+        with open("/proc/self/maps") as fh:
+            for line in fh:
+                if "libamdhip64.so" in line:
+                    path = line.split()[-1]
+                    break
+        assert path, "no HIP runtime loaded"
+        _HIP = C.CDLL(path)
+        _HIP.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _HIP.hipMemcpy.restype = C.c_int
+    return _HIP
+
+
+def to_device(dst_ptr, host_array):
+    a = np.ascontiguousarray(host_array)
+    assert hip_runtime().hipMemcpy(dst_ptr, a.ctypes.data, a.nbytes, 1) == 0  # hipMemcpyHostToDevice
+
+
+def oracle_scene_from_accel(acc, materials, config):
+    """Oracle scene over the library's committed records (its slot layout,
+    unused instance ids included with no primitives)."""
+    inst = acc.read_instances()
+    W = acc.read_world()
+    w2o = np.tile(np.eye(4, dtype=np.float32).reshape(16), (len(inst), 1))
+    for i, r in enumerate(inst):
+        if r.primitive_count:
+            w2o[i] = acc.world_to_object(i)
+    lights = W["lights"][: W["light_count"]]
+    return O.OScene(list(inst), w2o, W["aabbs"], W["prims"], list(materials), lights, config=config), inst, W
