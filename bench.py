@@ -256,14 +256,14 @@ def main():
     # 36 B record, a path segment reads 32 B and writes a 16 B hit; every BVH2
     # node fetched is 64 B, every leaf 28 B (centre + id + instance record
     # share), every instance entry 64 B.
-    n_any = ist.shadow_rays_ris + ist.shadow_rays_temporal + ist.shadow_rays_spatial
+    n_any = ist.shadow_rays_ris + ist.shadow_rays_temporal + ist.shadow_rays_pt + ist.shadow_rays_spatial
     bytes_any = 36 * n_any + 64 * ist.nodes + 28 * ist.leaves + 64 * ist.instances
     bytes_seg = 48 * ist.bounce_rays + 64 * ist.pt_nodes + 28 * ist.pt_leaves + 64 * ist.pt_instances
     b_any = bytes_any / max(n_any, 1)
     b_seg = bytes_seg / max(ist.bounce_rays, 1)
     log(f"instrumented frame: {n_any} shadow rays ({ist.shadow_rays_ris} RIS, {ist.shadow_rays_temporal} temporal, "
-        f"{ist.shadow_rays_spatial} spatial) {b_any:.0f} B/ray; {ist.bounce_rays} path segments {b_seg:.0f} B/segment; "
-        f"{ist.shadow_rays_pt} path-reuse visibility rays; closest: {ist.closest_nodes} nodes")
+        f"{ist.shadow_rays_pt} path reuse, {ist.shadow_rays_spatial} spatial) {b_any:.0f} B/ray; "
+        f"{ist.bounce_rays} path segments {b_seg:.0f} B/segment; closest: {ist.closest_nodes} nodes")
     # Timed region: K frames, events per pass.
     ctx.set_instrumentation(1)
     import ctypes
@@ -281,7 +281,7 @@ def main():
         rays += st.primary_rays + st.shadow_rays + st.bounce_rays
         ms_any += st.ms_trace_any
         ms_seg += st.ms_pt_segments
-        n_any_t += st.shadow_rays_ris + st.shadow_rays_temporal + st.shadow_rays_spatial
+        n_any_t += st.shadow_rays_ris + st.shadow_rays_temporal + st.shadow_rays_pt + st.shadow_rays_spatial
         n_seg_t += st.bounce_rays
         ms_frames.append(st.ms_total)
         ms_pt.append(st.ms_pt)
@@ -300,7 +300,8 @@ def main():
     # runs of this workload (rocprofv3 cannot run inside the timed process);
     # the algorithmic bytes per launch of this run are reported beside them.
     measured = _traffic_summary() if (args.width, args.height, args.grid, args.depth) == (1920, 1080, 256, 3) else {}
-    launches = {"k_trace_any": 3, "k_trace_segments": max(depth_of(w), 1)}
+    # any-hit launches per frame: RIS visibility, temporal, path-reuse visibility (with paths), spatial
+    launches = {"k_trace_any": 4 if depth_of(w) else 3, "k_trace_segments": max(depth_of(w), 1)}
     algo = {"k_trace_any": b_any * n_any_t / K, "k_trace_segments": b_seg * n_seg_t / K}
 
     def roof(k):

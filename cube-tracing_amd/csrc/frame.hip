@@ -36,7 +36,8 @@ constexpr float kMinCosTheta = 1e-6f;
 // Event marks of a frame (Ctx::ev indices).
 enum {
   kMarkStart, kMarkPrimary, kMarkRis, kMarkTraceRis, kMarkTemporalEmit, kMarkTraceTemporal, kMarkTemporalFinish,
-  kMarkPt, kMarkSpatialEmit, kMarkTraceSpatial, kMarkSpatialFinish, kMarkPtSpatial, kMarkTaa, kMarkCount
+  kMarkPtEmit, kMarkTracePt, kMarkPt, kMarkSpatialEmit, kMarkTraceSpatial, kMarkSpatialFinish, kMarkPtSpatial, kMarkTaa,
+  kMarkCount
 };
 
 // counters[] slots
@@ -1066,47 +1067,58 @@ __global__ void __launch_bounds__(kBlock) k_pt_step(FrameSet FS, uint32_t d) {
   }
 }
 
-// path_output and temporal path reuse (indirect_illumination.glsl:454-475).
-template <bool kCount>
-__global__ void __launch_bounds__(kBlock) k_pt_end(FrameSet FS, unsigned long long *stats) {
-  __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
+// path_output and temporal path reuse (indirect_illumination.glsl:454-475)
+// in two passes around one any-hit launch: kVisEmit queues the reuse's
+// visibility rays (taken as visible), kVisFinish recomputes with their
+// results and stores.  No traversal here keeps the kernels' registers low.
+template <int kVis>
+__global__ void __launch_bounds__(kBlock) k_pt_end(FrameSet FS) {
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
   const uint32_t px = blk_.y + threadIdx.x;
   PtTrace tr;
-  tr.stk = stk + threadIdx.x;
+  tr.stk = nullptr;
   tr.bounce_rays = 0u;
   tr.shadow_rays = 0u;
   tr.overflow = false;
-  if (px < F.npix && F.dpos_c[px].w > 0.0f) {
-    const uint32_t x = px % F.W, y = px / F.W;
-    PState p;
-    ld_pstate(F.pst, F.npix, px, p);
-    F3 ind;
-    pt_end(p, ind);
-    PRes out = p.res;
-    uint32_t seed = p.seed;
-    if (F.rp.pt_temporal) {
-      // temporal_path_get_reprojected, indirect_illumination.glsl:29-57
-      const DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
-      const Isect ci = isect_from_di(F, di);
-      const float2 v = F.vel[px];
-      const float r1 = rnd(seed);
-      const float r2 = rnd(seed);
-      const uint32_t pcx = f2u_sat(((float)x + v.x) + r1), pcy = f2u_sat(((float)y + v.y) + r2);
-      const uint32_t prev = min(F.npix - 1u, pcx + pcy * F.W);
-      const PRes temporal = ld_pres(F.pres_t, prev);
-      const Isect pi = isect_from_di_raw(F, ld_di(F.dpos_p, F.dnrm_p, F.dorg_p, F.dhit_p, prev));
-      out = pt_temporal<kCount>(F, out, temporal, ci, pi, seed, tr);
-      ind = muls(out.F, out.weight);
+  tr.px = px;
+  tr.slot = 0u;
+  tr.bits = 0u;
+  tr.emitted = 0u;
+  if (px < F.npix) {
+    if (F.dpos_c[px].w > 0.0f) {
+      const uint32_t x = px % F.W, y = px / F.W;
+      if (kVis == kVisFinish) tr.bits = F.mask[px];
+      PState p;
+      ld_pstate(F.pst, F.npix, px, p);
+      F3 ind;
+      pt_end(p, ind);
+      PRes out = p.res;
+      uint32_t seed = p.seed;
+      if (F.rp.pt_temporal) {
+        // temporal_path_get_reprojected, indirect_illumination.glsl:29-57
+        const DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
+        const Isect ci = isect_from_di(F, di);
+        const float2 v = F.vel[px];
+        const float r1 = rnd(seed);
+        const float r2 = rnd(seed);
+        const uint32_t pcx = f2u_sat(((float)x + v.x) + r1), pcy = f2u_sat(((float)y + v.y) + r2);
+        const uint32_t prev = min(F.npix - 1u, pcx + pcy * F.W);
+        const PRes temporal = ld_pres(F.pres_t, prev);
+        const Isect pi = isect_from_di_raw(F, ld_di(F.dpos_p, F.dnrm_p, F.dorg_p, F.dhit_p, prev));
+        out = pt_temporal<false, kVis>(F, out, temporal, ci, pi, seed, tr);
+        ind = muls(out.F, out.weight);
+      }
+      if (kVis == kVisFinish) {
+        if (has_nan_inf(ind)) ind = f3(0, 0, 0);  // rgen.glsl:293-294
+        st_pres(F.pres_o, px, out);
+        F.indirect[px] = make_float4(ind.x, ind.y, ind.z, 0.0f);
+        F.dorg_c[px].w = __uint_as_float(seed);
+      }
     }
-    if (has_nan_inf(ind)) ind = f3(0, 0, 0);  // rgen.glsl:293-294
-    st_pres(F.pres_o, px, out);
-    F.indirect[px] = make_float4(ind.x, ind.y, ind.z, 0.0f);
-    F.dorg_c[px].w = __uint_as_float(seed);
+    if (kVis == kVisEmit) F.mask[px] = 0u;
   }
-  wave_add(tr.shadow_rays, &F.counters[kCntPtShadow]);
-  if (kCount) wave_add_cnt(tr.cnt, stats + 9);
+  if (kVis == kVisEmit) queue_flush(F, px, tr.emitted, kCntPtShadow);
 }
 
 // Spatial path reuse (indirect_illumination_spatial_reuse, rgen.glsl:427-452)
@@ -1660,11 +1672,13 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark)
       else
         hipLaunchKernelGGL(k_pt_step<false>, dim3(nblocks(np)), dim3(kBlock), 0, st, pix, d);
     }
-    if (L.stats)
-      hipLaunchKernelGGL(k_pt_end<true>, dim3(nb), dim3(kBlock), 0, st, pix, L.stats);
-    else
-      hipLaunchKernelGGL(k_pt_end<false>, dim3(nb), dim3(kBlock), 0, st, pix, L.stats);
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntQueue, 1);
+    hipLaunchKernelGGL(k_pt_end<kVisEmit>, dim3(nb), dim3(kBlock), 0, st, pix);
   }
+  if ((rc = mark())) return rc;  // kMarkPtEmit
+  if (L.pt_depth) launch_any(L, pix, 2 * np, st);
+  if ((rc = mark())) return rc;  // kMarkTracePt
+  if (L.pt_depth) hipLaunchKernelGGL(k_pt_end<kVisFinish>, dim3(nb), dim3(kBlock), 0, st, pix);
   if ((rc = mark())) return rc;  // kMarkPt
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntQueue, 1);
   hipLaunchKernelGGL(k_spatial<true>, dim3(nb), dim3(kBlock), 0, st, pix);
@@ -2206,7 +2220,7 @@ int ct_frame_get_stats(ct_ctx *c, ct_frame_stats *out) {
       c->stats.ms_pt_spatial = ms(kMarkSpatialFinish, kMarkPtSpatial);
       c->stats.ms_taa = ms(kMarkPtSpatial, kMarkTaa);
       c->stats.ms_trace_any = ms(kMarkRis, kMarkTraceRis) + ms(kMarkTemporalEmit, kMarkTraceTemporal) +
-                              ms(kMarkSpatialEmit, kMarkTraceSpatial);
+                              ms(kMarkPtEmit, kMarkTracePt) + ms(kMarkSpatialEmit, kMarkTraceSpatial);
       c->stats.ms_trace_closest = c->stats.ms_primary;
       c->stats.ms_total = ms(kMarkStart, kMarkTaa);
       c->stats.ms_pt_segments = 0.0f;

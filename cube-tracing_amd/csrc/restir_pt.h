@@ -64,12 +64,15 @@ struct PState {  // PATH_STATE, path_state.glsl:23-78 (fields the Lambertian pat
   bool last_rough;
 };
 
-// Traversal context of one lane.
+// Traversal context of one lane.  The visibility tests of a pass that
+// defers them to the any-hit queue use px / slot / bits / emitted as the DI
+// passes' Vis does.
 struct PtTrace {
   uint32_t *stk;
   TraceCnt cnt;
   uint32_t bounce_rays, shadow_rays;
   bool overflow;
+  uint32_t px, slot, bits, emitted;
 };
 
 // ---------------------------------------------------------------- helpers
@@ -280,13 +283,31 @@ CT_D Isect bounce_intersect(const DFrame &F, F3 origin, F3 dir, PtTrace &tr) {
   return i;
 }
 
+// Visibility modes of a pass: traversed in place, queued (taken as visible
+// while the ray is recorded in the pixel's next slot), or read back from the
+// pixel's occlusion bits after the any-hit launch.
+constexpr int kVisInline = 0, kVisEmit = 1, kVisFinish = 2;
+
 // is_vertex_visible (light.glsl:82-142) on the current TLAS.
-template <bool kCount>
+template <bool kCount, int kVis = kVisInline>
 CT_D bool vertex_visible(const DFrame &F, F3 o, F3 d, float dist, uint32_t ti, uint32_t tp, PtTrace &tr) {
-  float bt = 0.0f;
-  uint32_t bi = kInvalid, bp = kInvalid;
-  ++tr.shadow_rays;
-  return !traverse_mode<kTraceAny, kCount>(F.S, o, d, dist, ti, tp, bt, bi, bp, tr.stk, kBlock, tr.cnt, tr.overflow);
+  if (kVis == kVisInline) {
+    float bt = 0.0f;
+    uint32_t bi = kInvalid, bp = kInvalid;
+    ++tr.shadow_rays;
+    return !traverse_mode<kTraceAny, kCount>(F.S, o, d, dist, ti, tp, bt, bi, bp, tr.stk, kBlock, tr.cnt, tr.overflow);
+  }
+  const uint32_t k = tr.slot++;
+  if (k >= kRaySlots) return true;  // at most two tests per pixel; never taken
+  if (kVis == kVisEmit) {
+    const uint32_t idx = tr.px * kRaySlots + k;
+    F.ray_o[idx] = make_float4(o.x, o.y, o.z, dist);
+    F.ray_d[idx] = make_float4(d.x, d.y, d.z, __uint_as_float(ti));
+    F.ray_t[idx] = tp;
+    tr.emitted |= 1u << k;
+    return true;
+  }
+  return ((tr.bits >> k) & 1u) == 0u;
 }
 
 // load_intersection_data_vertex_position, primitives.glsl:567-611
@@ -636,7 +657,7 @@ CT_D void ld_pstate(const float4 *S, uint32_t np, uint32_t p, PState &s) {
 // compute_shifted_integrand_reconnection, shift.glsl:25-350, for the
 // reconnection shift (rc vertex 1, no cached Jacobian).  `dst` and `src`
 // are the primary intersections; `s` the source reservoir.
-template <bool kCount>
+template <bool kCount, int kVis = kVisInline>
 CT_D F3 shift_reconnection(const DFrame &F, float &dst_jac, Isect dst, Isect src, const PRes &s, bool eval_vis,
                            PtTrace &tr) {
   dst_jac = 0.0f;
@@ -651,7 +672,7 @@ CT_D F3 shift_reconnection(const DFrame &F, float &dst_jac, Isect dst, Isect src
   if (!exists) {
     F3 integrand = f3(0, 0, 0);
     if (pf_light_type(s.flags) == kGeomLightEnv && plen + 1u == rc_len && !nee) {
-      if (vertex_visible<kCount>(F, dst.P, wi, kMaxDistance, kMaxInstances, kMaxPrimitives, tr)) {
+      if (vertex_visible<kCount, kVis>(F, dst.P, wi, kMaxDistance, kMaxInstances, kMaxPrimitives, tr)) {
         const float src_pdf1 = material_pdf(src.n, src.wo, wi);
         const float dst_pdf1 = material_pdf(dst.n, dst.wo, wi);
         const F3 dst_f1 = evaluate_material(dst.mat, dst.n, dst.wo, wi);
@@ -712,7 +733,7 @@ CT_D F3 shift_reconnection(const DFrame &F, float &dst_jac, Isect dst, Isect src
   if (eval_vis) {
     const F3 dir = normalize(sub(rc.P, dst.P));
     const float dist = length(sub(dst.P, rc.P));
-    if (!vertex_visible<kCount>(F, dst.P, dir, dist, rc.inst, rc.prim, tr)) return f3(0, 0, 0);
+    if (!vertex_visible<kCount, kVis>(F, dst.P, dir, dist, rc.inst, rc.prim, tr)) return f3(0, 0, 0);
   }
   if (has_nan_inf(integrand)) return f3(0, 0, 0);
   dst_jac = jac;
@@ -724,12 +745,13 @@ CT_D F3 shift_reconnection(const DFrame &F, float &dst_jac, Isect dst, Isect src
 // the reference (path_reservoir.glsl:37-58 returns an invalid record);
 // with it the hybrid integrand keeps dst_jacobian = 0 (shift.glsl:363 and
 // the skipped reconnection branch), so every hybrid shift is rejected.
-template <bool kCount>
+template <bool kCount, int kVis = kVisInline>
 CT_D F3 shifted_integrand(const DFrame &F, float &dst_jac, const Isect &dst, const Isect &src, const PRes &s,
                           bool eval_vis, PtTrace &tr) {
   dst_jac = 0.0f;
   if (s.weight == 0.0f) return f3(0, 0, 0);
-  if (F.rp.pt_shift_mapping == CT_SHIFT_RECONNECTION) return shift_reconnection<kCount>(F, dst_jac, dst, src, s, eval_vis, tr);
+  if (F.rp.pt_shift_mapping == CT_SHIFT_RECONNECTION)
+    return shift_reconnection<kCount, kVis>(F, dst_jac, dst, src, s, eval_vis, tr);
   if (F.rp.pt_shift_mapping == CT_SHIFT_HYBRID) {
     // compute_shifted_integrand_hybrid, shift.glsl:352-444, with the stubbed
     // reconnection data: rc length 1 gives tp = 1 and an invalid previous
@@ -741,10 +763,10 @@ CT_D F3 shifted_integrand(const DFrame &F, float &dst_jac, const Isect &dst, con
 }
 
 // shift_and_merge_reservoir, shift.glsl:485-510 (force_merge, mis 1)
-template <bool kCount>
+template <bool kCount, int kVis = kVisInline>
 CT_D bool shift_and_merge(const DFrame &F, float &dst_jac, const Isect &dst, PRes &dst_res, const Isect &src,
                           const PRes &src_res, uint32_t &seed, PtTrace &tr) {
-  const F3 integrand = shifted_integrand<kCount>(F, dst_jac, dst, src, src_res, true, tr);
+  const F3 integrand = shifted_integrand<kCount, kVis>(F, dst_jac, dst, src, src_res, true, tr);
   const bool selected = pres_merge(dst_res, integrand, dst_jac, src_res, seed, 1.0f, true);
   if (!selected) dst_res.F = f3(0, 0, 0);
   dst_res.M = src_res.M;
@@ -752,8 +774,12 @@ CT_D bool shift_and_merge(const DFrame &F, float &dst_jac, const Isect &dst, PRe
   return selected;
 }
 
-// temporal_path_reuse, indirect_illumination.glsl:147-279 (Talbot RMIS)
-template <bool kCount>
+// temporal_path_reuse, indirect_illumination.glsl:147-279 (Talbot RMIS).
+// Its two visibility tests (the central path shifted to the previous pixel,
+// then the history shifted here) exist or not by data alone, never by an
+// earlier test's outcome, so a queued pass records the same rays in the
+// same slots as the finishing pass reads.
+template <bool kCount, int kVis = kVisInline>
 CT_D PRes pt_temporal(const DFrame &F, const PRes &central, PRes temporal, const Isect &cur_i, const Isect &prev_i,
                       uint32_t &seed, PtTrace &tr) {
   PRes dest = pres_init(F.rp.max_depth);
@@ -769,7 +795,7 @@ CT_D PRes pt_temporal(const DFrame &F, const PRes &central, PRes temporal, const
       dst_jac = 1.0f;
       possible = tdst.weight > 0.0f;
     } else {
-      possible = shift_and_merge<kCount>(F, dst_jac, cur_i, tdst, prev_i, temporal, seed, tr);
+      possible = shift_and_merge<kCount, kVis>(F, dst_jac, cur_i, tdst, prev_i, temporal, seed, tr);
     }
     if (possible) {
       // j = -1
@@ -782,7 +808,7 @@ CT_D PRes pt_temporal(const DFrame &F, const PRes &central, PRes temporal, const
         p_sum += p_self;
       } else {
         float tj = 0.0f;
-        const F3 ti = shifted_integrand<kCount>(F, tj, prev_i, cur_i, tdst, true, tr);
+        const F3 ti = shifted_integrand<kCount, kVis>(F, tj, prev_i, cur_i, tdst, true, tr);
         const float pp = lum601(ti) * tj;
         p_sum += pp * temporal.M;
       }
