@@ -275,6 +275,37 @@ CT_D F3 evaluate_material(const Mat &m, F3 n, F3 wo, F3 wi) {
   if (gmin(a, b) < kMinCosTheta) return f3(0, 0, 0);
   return muls(muls(m.diffuse, kInvPi), b);
 }
+
+// What calculate_sampled_light derives from the shading point alone, once
+// per point instead of once per light sample: sn = normalize(hit.n), wo =
+// -normalize(ray_d), the normal evaluate_material normalizes again and its
+// cosine with wo, and the cosine material_pdf(hit.n, -ray_d, .) uses.  The
+// same operations on the same inputs: values are unchanged.
+struct Shade {
+  F3 sn, nsn, wo;
+  float a_eval, a_pdf;
+};
+CT_D Shade shade_of(F3 ray_d, const Hit &hit) {
+  Shade s;
+  s.sn = normalize(hit.n);
+  s.nsn = normalize(s.sn);
+  s.wo = neg(normalize(ray_d));
+  s.a_eval = dot(s.wo, s.nsn);
+  s.a_pdf = dot(neg(ray_d), s.sn);
+  return s;
+}
+// evaluate_material(m, sn, wo, wi) with the shading point's terms precomputed
+CT_D F3 evaluate_material(const Mat &m, const Shade &sh, F3 wi) {
+  const float b = dot(wi, sh.nsn);
+  if (gmin(sh.a_eval, b) < kMinCosTheta) return f3(0, 0, 0);
+  return muls(muls(m.diffuse, kInvPi), b);
+}
+// material_pdf(hit.n, -ray_d, wi) with the shading point's terms precomputed
+CT_D float material_pdf(const Shade &sh, F3 wi) {
+  const float b = dot(wi, sh.sn);
+  if (gmin(sh.a_pdf, b) < kMinCosTheta) return 0.0f;
+  return kInvPi * b;
+}
 CT_D float geom_fact_sa(F3 P, F3 Ps, F3 ns) {
   F3 dir = normalize(sub(Ps, P));
   F3 dd = sub(Ps, P);
@@ -382,16 +413,39 @@ CT_D void sample_lights(const DFrame &F, const Hit &hit, const PLight &l, float 
       seed = lcg_skip33(seed);
       return;
     }
+    // light.glsl:333-345.  The face normal is an axis vector, so
+    // random_quad's tangents are the exact axis vectors t1, t2 (their
+    // normalize divides by 1), and dot(normalize(P - x), axis) has the sign
+    // of (P - x) on that axis: the draw is tested by signs and only the
+    // last one is normalized.  A normal n off the axes takes the full test.
+    int na = -1;
+    float ns = 0.0f;
+    if (n.y == 0.0f && n.z == 0.0f && fabsf(n.x) == 1.0f) { na = 0; ns = n.x; }
+    else if (n.x == 0.0f && n.z == 0.0f && fabsf(n.y) == 1.0f) { na = 1; ns = n.y; }
+    else if (n.x == 0.0f && n.y == 0.0f && fabsf(n.z) == 1.0f) { na = 2; ns = n.z; }
+    const float hq = qs * 0.5f;
     uint32_t i = 0;
     do {
-      vis = true;
       l_nor = random_cube_normal(seed);
       l_pos = add(l.pos, muls(l_nor, half));
-      l_pos = random_quad(l_nor, l_pos, qs * 0.5f, qs * 0.5f, seed);
-      l_wi = normalize(sub(P, l_pos));
-      vis = vis && (dot(l_wi, l_nor) > 0.0f);
-      vis = vis && (dot(l_wi, n) < 0.0f);
+      const float u = rnd(seed);
+      const float v = rnd(seed);
+      const float qx = u * hq, qy = v * hq;
+      const F3 t1 = fabsf(l_nor.x) > fabsf(l_nor.y) ? f3(-l_nor.z, 0.0f, l_nor.x) : f3(0.0f, l_nor.z, -l_nor.y);
+      const F3 t2 = cross(l_nor, t1);
+      l_pos = add(add(l_pos, muls(t1, qx)), muls(t2, qy));
+      const F3 w = sub(P, l_pos);
+      const float wn = l_nor.x != 0.0f ? w.x * l_nor.x : (l_nor.y != 0.0f ? w.y * l_nor.y : w.z * l_nor.z);
+      vis = wn > 0.0f;
+      if (vis) {
+        if (na >= 0) {
+          vis = (na == 0 ? w.x : (na == 1 ? w.y : w.z)) * ns < 0.0f;
+        } else {
+          vis = dot(normalize(w), n) < 0.0f;
+        }
+      }
     } while (!vis && i++ < 10);
+    l_wi = normalize(sub(P, l_pos));
     l_pos = compute_ray_origin(l_pos, l_nor);
     l_pos = compute_ray_origin(l_pos, l_nor);
     distance = length(sub(P, l_pos));
@@ -430,10 +484,8 @@ CT_D void sample_lights(const DFrame &F, const Hit &hit, const PLight &l, float 
 
 // calculate_sampled_light, light.glsl:401-435.  With use_vis the shadow
 // ray goes through `v`.
-CT_D F3 sampled_light(const DFrame &F, F3 ray_d, Hit &hit, const Mat &mat, const PLight &l, float pdf, float &pdf_out,
-                      float &G, uint32_t &seed, bool calc_pdf, bool use_pdf, bool use_vis, Vis *v) {
-  F3 sn = normalize(hit.n);
-  F3 wo = neg(normalize(ray_d));
+CT_D F3 sampled_light(const DFrame &F, const Shade &sh, Hit &hit, const Mat &mat, const PLight &l, float pdf,
+                      float &pdf_out, float &G, uint32_t &seed, bool calc_pdf, bool use_pdf, bool use_vis, Vis *v) {
   pdf_out = pdf;
   G = 0.0f;
   if (l.emissive.x <= 0.0f && l.emissive.y <= 0.0f && l.emissive.z <= 0.0f) return f3(0, 0, 0);
@@ -444,20 +496,24 @@ CT_D F3 sampled_light(const DFrame &F, F3 ray_d, Hit &hit, const Mat &mat, const
   if (ls.ray && occluded(F, *v, ls.P, ls.dir, ls.dist, ls.ti, ls.tp)) ok = false;
   if (!ok) return f3(0, 0, 0);
   F3 wi = normalize(sub(ls.l_pos, hit.P));
-  F3 brdf = evaluate_material(mat, sn, wo, wi);
+  F3 brdf = evaluate_material(mat, sh, wi);
   hit.scatter = wi;
   F3 bl = mulv(brdf, ls.Le);
   return use_pdf ? divs(muls(bl, G), pdf_out) : muls(bl, G);
 }
 
 // reservoir_get_radiance, reservoir.glsl:98-112
-CT_D F3 res_radiance(const DFrame &F, const Res &r, F3 ray_d, Hit hit, const Mat &mat, Vis &v) {
+CT_D F3 res_radiance(const DFrame &F, const Res &r, const Shade &sh, Hit hit, const Mat &mat, Vis &v) {
   if (!res_valid(r)) return f3(0, 0, 0);
   PLight l;
   if (!light_by_type(F, r.type, r.Y, l)) return f3(0, 0, 0);
   float pdf_out, G;
   uint32_t s = r.seed;
-  return sampled_light(F, ray_d, hit, mat, l, 1.0f, pdf_out, G, s, false, false, true, &v);
+  return sampled_light(F, sh, hit, mat, l, 1.0f, pdf_out, G, s, false, false, true, &v);
+}
+CT_D F3 res_radiance(const DFrame &F, const Res &r, F3 ray_d, Hit hit, const Mat &mat, Vis &v) {
+  if (!res_valid(r)) return f3(0, 0, 0);
+  return res_radiance(F, r, shade_of(ray_d, hit), hit, mat, v);
 }
 
 // reservoir_visibility_pass, reservoir.glsl:76-96, 114-126
@@ -480,15 +536,15 @@ CT_D void visibility_pass(const DFrame &F, Res &r, const Hit &hit, Vis &v) {
 
 // RIS candidate streaming of one light type, restir_resampling.glsl:72-178
 CT_D void ris_strategy(const DFrame &F, Res &target, uint32_t count, uint32_t nsamp, uint32_t type, const float4 *lights,
-                       uint32_t nee, uint32_t n_brdf, F3 ray_d, Hit &hit, const Mat &mat, uint32_t &seed) {
+                       uint32_t nee, uint32_t n_brdf, const Shade &sh, Hit &hit, const Mat &mat, uint32_t &seed) {
   float pdf = 1.0f / (float)count;
   for (uint32_t k = 0; k < nsamp; ++k) {
     uint32_t idx = min(urnd_interval(seed, 0u, count), count - 1u);
     PLight light = ld_light(lights, idx);
     float light_pdf = 1.0f, G = 0.0f;
     uint32_t current_seed = seed;
-    F3 Fv = sampled_light(F, ray_d, hit, mat, light, pdf, light_pdf, G, seed, true, false, false, nullptr);
-    float brdf_pdf = material_pdf(hit.n, neg(ray_d), hit.scatter) * G;
+    F3 Fv = sampled_light(F, sh, hit, mat, light, pdf, light_pdf, G, seed, true, false, false, nullptr);
+    float brdf_pdf = material_pdf(sh, hit.scatter) * G;
     float m_i = 1.0f / gmax((float)nee * light_pdf + (float)n_brdf * brdf_pdf, 1e-6f);
     float w_i = luminance(Fv) * m_i;
     update_reservoir(target, idx, type, current_seed, Fv, w_i, 1.0f, seed);
@@ -506,17 +562,18 @@ CT_D Res ris(const DFrame &F, float confidence, F3 ray_d, Hit &hit, const Mat &m
   uint32_t n_cube = max(f2u_sat(gmin((float)rp.max_ris_cube_samples * (1.0f - confidence), (float)cc)), rp.min_ris_cube_samples);
   uint32_t n_brdf = F.cfg.brdf_count + n_env;
   uint32_t nee = n_point + n_cube;
+  const Shade sh = shade_of(ray_d, hit);
   Res r = res_init();
   if ((F.active & CT_RIS_POINT_LIGHT_BIT) && F.n_point > 0)
-    ris_strategy(F, r, pc, n_point, CT_LIGHT_POINT, F.point, nee, n_brdf, ray_d, hit, mat, seed);
+    ris_strategy(F, r, pc, n_point, CT_LIGHT_POINT, F.point, nee, n_brdf, sh, hit, mat, seed);
   if ((F.active & CT_RIS_ENV_LIGHT_BIT) && F.n_env > 0) {
     Res er = res_init();
-    ris_strategy(F, er, ec, n_env, CT_LIGHT_ENV_MAP, F.env, nee, n_brdf, ray_d, hit, mat, seed);
+    ris_strategy(F, er, ec, n_env, CT_LIGHT_ENV_MAP, F.env, nee, n_brdf, sh, hit, mat, seed);
     update_reservoir(r, er.Y, er.type, er.seed, er.F, er.W_sum, er.M, seed);
   }
   if ((F.active & CT_RIS_CUBE_LIGHT_BIT) && F.n_cube > 0) {
     Res cr = res_init();
-    ris_strategy(F, cr, cc, n_cube, CT_LIGHT_CUBE, F.cube, nee, n_brdf, ray_d, hit, mat, seed);
+    ris_strategy(F, cr, cc, n_cube, CT_LIGHT_CUBE, F.cube, nee, n_brdf, sh, hit, mat, seed);
     update_reservoir(r, cr.Y, cr.type, cr.seed, cr.F, cr.W_sum, cr.M, seed);
   }
   return r;
@@ -597,6 +654,7 @@ CT_D void spatial_reuse(const DFrame &F, Res &r, float confidence, uint32_t x, u
   float m_c = 1.0f;
   float M_s = r.M;
   const float fk = (float)num;
+  const Shade sh = shade_of(ray_d, hit);
   for (uint32_t i = 0; i < num; ++i) {
     Res in;
     DI nd;
@@ -607,7 +665,7 @@ CT_D void spatial_reuse(const DFrame &F, Res &r, float confidence, uint32_t x, u
     F3 curr_target = f3(0, 0, 0);
     float m_i = 0.0f;
     if (res_valid(in)) {
-      curr_target = res_radiance(F, in, ray_d, hit, mat, v);
+      curr_target = res_radiance(F, in, sh, hit, mat, v);
       const float tl = luminance(curr_target);
       const float pii = luminance(in.F);
       float mm = in.M * pii;
