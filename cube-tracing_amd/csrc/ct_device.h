@@ -319,28 +319,53 @@ CT_D uint32_t visit_node(const DNode *nodes, uint32_t idx, F3 o, F3 inv, float l
 // Closest modes break equal t by the smaller (instance, primitive).
 constexpr int kTraceAny = 0, kTraceClosest = 1, kTraceBounce = 2;
 
-// Two-level traversal (TLAS in world space, each BLAS in object space).
-// Returns true when an any-hit query is occluded.  `stk` is this lane's LDS
-// stack column.
+// Two-level traversal (TLAS in world space, each BLAS in object space) as
+// a resumable walker: start() takes a ray, each step() visits one TLAS
+// node, instance, BLAS node or voxel leaf, and `done` is set once the ray
+// is finished (for an any-hit query `occluded` tells the outcome).  A
+// kernel can thus refill a lane whose ray finished while the rest of the
+// wave keeps walking.  `stk` is the lane's LDS stack column.
 template <int kMode, bool kCount>
-CT_D bool traverse_mode(const DScene &S, F3 o, F3 d, float limit, uint32_t ti, uint32_t tp,
-                        float &best_t, uint32_t &best_i, uint32_t &best_p, uint32_t *stk,
-                        uint32_t stride, TraceCnt &cnt, bool &overflow) {
-  constexpr bool kAny = kMode == kTraceAny;
-  const F3 winv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-  F3 oo = o, od = d, oinv = winv, dr = d, sg = d, sinv = d;
-  uint32_t first = 0, inst = 0;
-  bool in_blas = false;
-  int base = 0;
+struct Walker {
+  static constexpr bool kAny = kMode == kTraceAny;
+  F3 o, d, winv, oo, od, oinv, dr, sg, sinv;
+  float limit, best_t;
+  uint32_t ti, tp, best_i, best_p, first, inst, ref, steps;
+  int base;
+  bool in_blas, done, occluded, overflow;
   Stack st;
-  st.lds = stk;
-  st.stride = stride;
-  st.n = 0;
-  uint32_t ref = S.tlas_root;
-  for (uint32_t iter = 0;; ++iter) {
-    if (iter > kMaxTraversalSteps) {  // only a malformed tree can get here
+
+  CT_D void start(const DScene &S, F3 o_, F3 d_, float limit_, uint32_t ti_, uint32_t tp_, uint32_t *stk,
+                  uint32_t stride) {
+    o = o_;
+    d = d_;
+    limit = limit_;
+    ti = ti_;
+    tp = tp_;
+    winv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    oo = o; od = d; oinv = winv; dr = d; sg = d; sinv = d;
+    best_t = limit_;
+    best_i = kInvalid;
+    best_p = kInvalid;
+    first = 0;
+    inst = 0;
+    base = 0;
+    steps = 0;
+    in_blas = false;
+    occluded = false;
+    overflow = false;
+    st.lds = stk;
+    st.stride = stride;
+    st.n = 0;
+    ref = S.tlas_root;
+    done = ref == kInvalid;
+  }
+
+  CT_D void step(const DScene &S, TraceCnt &cnt) {
+    if (++steps > kMaxTraversalSteps) {  // only a malformed tree can get here
       overflow = true;
-      return false;
+      done = true;
+      return;
     }
     uint32_t next = kInvalid;
     if (!in_blas) {
@@ -365,24 +390,30 @@ CT_D bool traverse_mode(const DScene &S, F3 o, F3 d, float limit, uint32_t ti, u
         in_blas = true;
         base = st.n;
         ref = I.root;
-        continue;
+        return;
       }
       if (kCount) cnt.nodes++;
       next = visit_node(S.tlas_nodes, ref, o, winv, kAny ? limit : best_t, st, overflow);
     } else {
       if (ref & kLeaf) {
         if (kCount) cnt.leaves++;
-        float4 L = S.blas_leaves[first + (ref & ~kLeaf)];
-        uint32_t pid = __float_as_uint(L.w);
+        const float4 L = S.blas_leaves[first + (ref & ~kLeaf)];
+        const uint32_t pid = __float_as_uint(L.w);
         float t;
         if (kAny) {
-          bool hit = intersect_box<false, false>(f3(L.x, L.y, L.z), oo, dr, sg, sinv, t);
-          if (hit && (t >= 0.0f) && (t <= limit) && !(inst == ti && pid == tp)) return true;
+          const bool hit = intersect_box<false, false>(f3(L.x, L.y, L.z), oo, dr, sg, sinv, t);
+          if (hit && (t >= 0.0f) && (t <= limit) && !(inst == ti && pid == tp)) {
+            occluded = true;
+            done = true;
+            return;
+          }
         } else {
-          bool hit = (kMode == kTraceClosest) ? intersect_box<true, true>(f3(L.x, L.y, L.z), oo, dr, sg, sinv, t)
-                                              : intersect_box<true, false>(f3(L.x, L.y, L.z), oo, dr, sg, sinv, t);
+          const bool hit = (kMode == kTraceClosest)
+                               ? intersect_box<true, true>(f3(L.x, L.y, L.z), oo, dr, sg, sinv, t)
+                               : intersect_box<true, false>(f3(L.x, L.y, L.z), oo, dr, sg, sinv, t);
           if (hit && ((kMode == kTraceClosest) ? (t > 0.0f) : (t >= 0.0f))) {
-            bool better = (t < best_t) || (t == best_t && ((inst < best_i) || (inst == best_i && pid < best_p)));
+            const bool better =
+                (t < best_t) || (t == best_t && ((inst < best_i) || (inst == best_i && pid < best_p)));
             if (better) {
               best_t = t;
               best_i = inst;
@@ -395,16 +426,41 @@ CT_D bool traverse_mode(const DScene &S, F3 o, F3 d, float limit, uint32_t ti, u
         next = visit_node(S.blas_nodes, first + ref, oo, oinv, kAny ? limit : best_t, st, overflow);
       }
     }
-    if (overflow) return false;
+    if (overflow) {
+      done = true;
+      return;
+    }
     if (next != kInvalid) {
       ref = next;
-      continue;
+      return;
     }
     if (in_blas && st.n == base) in_blas = false;
-    if (st.n == 0) break;
+    if (st.n == 0) {
+      done = true;
+      return;
+    }
     ref = st.pop();
   }
-  return false;
+};
+
+// The whole walk of one ray.  Returns true when an any-hit query is
+// occluded; a closest query leaves its hit in best_t/best_i/best_p (which
+// enter as the current limit and an invalid hit).
+template <int kMode, bool kCount>
+CT_D bool traverse_mode(const DScene &S, F3 o, F3 d, float limit, uint32_t ti, uint32_t tp,
+                        float &best_t, uint32_t &best_i, uint32_t &best_p, uint32_t *stk,
+                        uint32_t stride, TraceCnt &cnt, bool &overflow) {
+  Walker<kMode, kCount> w;
+  w.start(S, o, d, limit, ti, tp, stk, stride);
+  w.best_t = best_t;
+  w.best_i = best_i;
+  w.best_p = best_p;
+  while (!w.done) w.step(S, cnt);
+  best_t = w.best_t;
+  best_i = w.best_i;
+  best_p = w.best_p;
+  overflow = overflow || w.overflow;
+  return w.occluded && !w.overflow;
 }
 
 template <bool kAny, bool kCount>
