@@ -1065,6 +1065,26 @@ __global__ void __launch_bounds__(kBlock) k_pt_begin(FrameSet FS) {
   }
 }
 
+// Batches of 64 queued rays for a persistent wave. The first batch is the
+// wave's own index in the grid, taken without an atomic; later batches come
+// from the shared head, offset by the rays the first batches cover. With a
+// grid of resident blocks every first batch starts at once, so a launch with
+// few rays costs about one batch, not one atomic per wave of the grid.
+struct RayBatches {
+  uint32_t base, count, stride;
+  uint32_t *head;
+  CT_D RayBatches(uint32_t count_, uint32_t *head_) : count(count_), head(head_) {
+    stride = gridDim.x * (kBlock / 64u) * 64u;
+    base = (blockIdx.x * (kBlock / 64u) + (threadIdx.x >> 6)) * 64u;
+  }
+  CT_D bool valid() const { return base < count; }
+  CT_D void next() {
+    uint32_t b = 0xFFFFFFFFu;
+    if ((threadIdx.x & 63u) == 0u) b = atomicAdd(head, 64u) + stride;
+    base = __shfl(b, 0);
+  }
+};
+
 // Closest voxel of each queued path segment (intersect() semantics), one
 // 64-ray fetch per wave.  kMulti: segments of several frames (scenes).
 template <bool kCount, bool kMulti>
@@ -1074,12 +1094,8 @@ __global__ void __launch_bounds__(kBlock) k_trace_segments(FrameSet FS, uint32_t
   const uint32_t tid = threadIdx.x, lane = tid & 63u;
   const uint32_t n = F0.counters[kCntPtQueue + (d & 1u)];
   TraceCnt cnt;
-  for (;;) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&F0.counters[kCntPtHead], 64u);
-    base = __shfl(base, 0);
-    if (base >= n) break;
-    const uint32_t q = base + lane;
+  for (RayBatches rb(n, &F0.counters[kCntPtHead]); rb.valid(); rb.next()) {
+    const uint32_t q = rb.base + lane;
     if (q < n) {
       const uint32_t view = kMulti ? (F0.ptq[d & 1u][q] >> kViewShift) : 0u;
       const float4 o = F0.seg_o[q], dd = F0.seg_d[q];
@@ -1325,12 +1341,8 @@ __global__ void __launch_bounds__(kBlock) k_trace_any(DScene S, TraceAnyArgs A) 
   uint32_t count = *A.count;
   if (count > A.max_rays) count = A.max_rays;
   TraceCnt cnt;
-  for (;;) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(A.head, 64u);
-    base = __shfl(base, 0);
-    if (base >= count) break;
-    uint32_t q = base + lane;
+  for (RayBatches rb(count, A.head); rb.valid(); rb.next()) {
+    const uint32_t q = rb.base + lane;
     if (q < count) {
       uint32_t idx = A.queue ? A.queue[q] : q;
       float4 o = A.ray_o[idx], d = A.ray_d[idx];
@@ -1370,12 +1382,8 @@ __global__ void __launch_bounds__(kBlock) k_trace_any_frames(FrameSet FS, uint32
   uint32_t count = F0.counters[kCntQueue];
   if (count > max_rays) count = max_rays;
   TraceCnt cnt;
-  for (;;) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&F0.counters[kCntHead], 64u);
-    base = __shfl(base, 0);
-    if (base >= count) break;
-    const uint32_t q = base + lane;
+  for (RayBatches rb(count, &F0.counters[kCntHead]); rb.valid(); rb.next()) {
+    const uint32_t q = rb.base + lane;
     if (q < count) {
       const uint32_t e = F0.queue[q];
       const DFrame &F = FS.fb[kMulti ? (e >> kViewShift) : 0u];
@@ -1431,14 +1439,45 @@ int trace_grid() {
     hipDeviceProp_t p;
     if (hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
   }
-  return cus * 8;
+  return cus * 5;
+}
+
+// Blocks of a persistent kernel that fit on the device at once (VGPRs and the
+// LDS stack bound it to 5 per CU on gfx950). A grid of at most this many
+// blocks starts every wave together, so each wave's first batch of rays (its
+// own index) is taken at once; blocks beyond it would only start when others
+// end and hold their first batches back to the end of the launch.
+template <class K>
+uint32_t resident_grid(K kernel) {
+  int dev = 0, per_cu = 0;
+  (void)hipGetDevice(&dev);
+  hipDeviceProp_t p;
+  const int cus = hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  return (uint32_t)(per_cu * cus);
+}
+template <bool kCount, bool kMulti>
+uint32_t any_resident() {
+  static const uint32_t n = resident_grid(k_trace_any_frames<kCount, kMulti>);
+  return n;
+}
+template <bool kCount>
+uint32_t any_single_resident() {
+  static const uint32_t n = resident_grid(k_trace_any<kCount>);
+  return n;
+}
+template <bool kCount, bool kMulti>
+uint32_t segments_resident() {
+  static const uint32_t n = resident_grid(k_trace_segments<kCount, kMulti>);
+  return n;
 }
 
 }  // namespace
 
 int launch_trace_any(const DScene &s, const TraceAnyArgs &a, hipStream_t st) {
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, a.head, 0, 1);
-  uint32_t grid = std::min<uint32_t>(std::max<uint32_t>(nblocks(a.max_rays), 1), (uint32_t)trace_grid());
+  const uint32_t grid = std::min<uint32_t>(std::max<uint32_t>(nblocks(a.max_rays), 1),
+                                           a.stats ? any_single_resident<true>() : any_single_resident<false>());
   if (a.stats)
     hipLaunchKernelGGL(k_trace_any<true>, dim3(grid), dim3(kBlock), 0, st, s, a);
   else
@@ -1671,7 +1710,8 @@ struct Launch {
 template <bool kCount, bool kMulti>
 void launch_any_frames(const Launch &L, FrameSet fs, uint32_t max_rays, hipStream_t st) {
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntHead, 1);
-  const uint32_t grid = std::min<uint32_t>(std::max<uint32_t>(nblocks(max_rays), 1), L.trace_blocks);
+  const uint32_t grid = std::min<uint32_t>(std::min<uint32_t>(std::max<uint32_t>(nblocks(max_rays), 1), L.trace_blocks),
+                                           any_resident<kCount, kMulti>());
   hipLaunchKernelGGL((k_trace_any_frames<kCount, kMulti>), dim3(grid), dim3(kBlock), 0, st, fs, max_rays, L.stats);
 }
 void launch_any(const Launch &L, FrameSet fs, uint32_t max_rays, hipStream_t st) {
@@ -1710,7 +1750,9 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark)
   if (L.pt_depth) {
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntPtQueue, 3);
     hipLaunchKernelGGL(k_pt_begin, dim3(nb), dim3(kBlock), 0, st, pix);
-    const uint32_t grid = std::min<uint32_t>(nblocks(np), L.trace_blocks);
+    const uint32_t grid = std::min<uint32_t>(std::min<uint32_t>(std::max<uint32_t>(nblocks(np), 1), L.trace_blocks),
+                                             L.stats ? (L.multi ? segments_resident<true, true>() : segments_resident<true, false>())
+                                                     : (L.multi ? segments_resident<false, true>() : segments_resident<false, false>()));
     for (uint32_t d = 0; d < L.pt_depth; ++d) {
       // the next queue and the fetch counter start at zero
       hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntPtQueue + (int)((d + 1) & 1u), 1);
@@ -1882,7 +1924,7 @@ struct ct_batch {
   int slot = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  uint32_t blocks_per_cu = 8;
+  uint32_t blocks_per_cu = 5;
   uint32_t *counters_pinned = nullptr;
   hipEvent_t t0 = nullptr, t1 = nullptr;
   hipStream_t last_stream = nullptr;
@@ -1945,7 +1987,7 @@ int ct_batch_create(uint32_t n, ct_ctx *const *ctxs, ct_accel *const *accels, ct
   }
   b->pinned = static_cast<DFrame *>(pin);
   b->counters_pinned = static_cast<uint32_t *>(cpin);
-  b->blocks_per_cu = 8;
+  b->blocks_per_cu = 5;
   *out = b;
   return CT_OK;
 }
@@ -2085,7 +2127,7 @@ int ct_batch_get_stats(ct_batch *b, ct_frame_stats *out) {
 
 int ct_batch_autotune(ct_batch *b, const ct_frame_params *params, uint32_t repeats, ct_batch_tuning *out) {
   if (!b || !params || repeats == 0) return fail(CT_E_INVALID, "ct_batch_autotune: bad argument");
-  const uint32_t cand[] = {2, 4, 8, 16, 32};
+  const uint32_t cand[] = {2, 3, 4, 5};
   float best = 0.0f;
   uint32_t best_c = b->blocks_per_cu;
   int rc;

@@ -583,7 +583,8 @@ int ct_batch_render(ct_batch *batch, const ct_frame_params *params, int use_grap
  * ms_total = HIP-event time of the whole batch (other times are 0). */
 int ct_batch_get_stats(ct_batch *batch, ct_frame_stats *out);
 /* Launch-shape autotuner: renders the given frames `repeats` times for
- * each candidate grid of the persistent traversal kernels (blocks per CU)
+ * each candidate grid of the persistent traversal kernels (blocks per CU,
+ * 2 to 5; a grid never exceeds the blocks that can be resident at once)
  * and keeps the fastest; reports the choice. */
 typedef struct ct_batch_tuning {
   uint32_t blocks_per_cu;

@@ -75,4 +75,4 @@ def test_batch_autotune_keeps_results(gpu):
     b = ctamd.Batch(ctxs, [built[k][1] for k, _ in views])
     params = [ctamd.frame_params(w, 0) for _, w in views]
     t = b.autotune(params, repeats=1)
-    assert t.blocks_per_cu in (2, 4, 8, 16, 32) and t.ms > 0 and t.candidates == 5
+    assert t.blocks_per_cu in (2, 3, 4, 5) and t.ms > 0 and t.candidates == 4
