@@ -35,7 +35,7 @@ constexpr float kMinCosTheta = 1e-6f;
 
 // Event marks of a frame (Ctx::ev indices).
 enum {
-  kMarkStart, kMarkPrimary, kMarkRis, kMarkTraceRis, kMarkTemporalEmit, kMarkTraceTemporal, kMarkTemporalFinish,
+  kMarkStart, kMarkPrimary, kMarkRis, kMarkTraceRis, kMarkTemporalFinish,
   kMarkPtEmit, kMarkTracePt, kMarkPt, kMarkSpatialEmit, kMarkTraceSpatial, kMarkSpatialFinish, kMarkPtSpatial, kMarkTaa,
   kMarkCount
 };
@@ -245,11 +245,13 @@ CT_D bool update_reservoir(Res &r, uint32_t X, uint32_t type, uint32_t rs, F3 Fv
 // ------------------------------------------------------- visibility source
 // EMIT: a query that needs a shadow ray writes it to slot (pixel*16 + k)
 // and reports "unoccluded".  FINISH: the k-th query reads bit k.
+// `queries` counts the calls, made or taken from bits.
 struct Vis {
   bool emit;
-  uint32_t px, slot, bits, emitted;
+  uint32_t px, slot, bits, emitted, queries;
 };
 CT_D bool occluded(const DFrame &F, Vis &v, F3 o, F3 d, float dist, uint32_t ti, uint32_t tp) {
+  ++v.queries;
   uint32_t k = v.slot++;
   if (k >= kRaySlots) return false;  // bounded by CT_MAX_NEIGHBORS; never taken
   if (v.emit) {
@@ -363,7 +365,6 @@ CT_D uint32_t lcg_skip33(uint32_t s) {
   constexpr uint32_t A = lcg_mul_pow(33), C = lcg_add_sum(33);
   return A * s + C;
 }
-
 // True when the light voxel lies wholly on the back side of the surface at
 // P (axis-aligned normal n): then dot(normalize(P - x), n) >= 0 for every
 // sample point x of the voxel, light.glsl:343 rejects all draws.  The
@@ -391,6 +392,7 @@ struct LS {
   float dist;
   uint32_t ti, tp;
   F3 l_pos, Le;
+  F3 wi;        // normalize(l_pos - P), the direction calculate_sampled_light shades
   float G;
 };
 
@@ -448,19 +450,30 @@ CT_D void sample_lights(const DFrame &F, const Hit &hit, const PLight &l, float 
     l_wi = normalize(sub(P, l_pos));
     l_pos = compute_ray_origin(l_pos, l_nor);
     l_pos = compute_ray_origin(l_pos, l_nor);
-    distance = length(sub(P, l_pos));
+    // length(P - l_pos), geom_fact_sa(P, l_pos, l_nor) and the shading
+    // direction share one square root: P - l_pos is exactly -(l_pos - P),
+    // and for the axis vector l_nor, dot(l_nor, dir) is the component of
+    // dir on that axis up to sign (the other terms are signed zeros).
+    const F3 dd = sub(l_pos, P);
+    const float d2 = dot(dd, dd);
+    distance = sqrtf(d2);
+    o.wi = divs(dd, distance);
     if (calc_pdf) {
       float area = qs * qs * 6.0f;
       pdf /= area;
     }
     check = true;
-    o.G = geom_fact_sa(P, l_pos, l_nor);
+    o.G = fabsf(l_nor.x != 0.0f ? o.wi.x : (l_nor.y != 0.0f ? o.wi.y : o.wi.z)) / d2;
   } else if (l.type == CT_LIGHT_POINT) {
+    // As above: normalize(l_pos - P) is exactly -normalize(P - l_pos).
     l_pos = l.pos;
-    l_nor = normalize(sub(P, l_pos));
-    distance = length(sub(P, l_pos));
+    const F3 v = sub(P, l_pos);
+    const float d2 = dot(v, v);
+    distance = sqrtf(d2);
+    l_nor = divs(v, distance);
     l_wi = l_nor;
-    o.G = geom_fact_sa(P, l_pos, l_nor);
+    o.wi = neg(l_nor);
+    o.G = fabsf(dot(l_nor, o.wi)) / d2;
   } else if (l.type == CT_LIGHT_ENV_MAP) {
     F3 l_dir = random_on_hemisphere(seed, n);
     l_pos = add(P, muls(l_dir, kMaxDistance));
@@ -471,6 +484,7 @@ CT_D void sample_lights(const DFrame &F, const Hit &hit, const PLight &l, float 
     if (calc_pdf) pdf /= kInv4Pi;
     o.G = 1.0f;
   }
+  if (l.type != CT_LIGHT_CUBE && l.type != CT_LIGHT_POINT) o.wi = normalize(sub(l_pos, P));
   o.ray = visibility && vis;
   o.P = P;
   o.dir = neg(l_wi);
@@ -495,7 +509,7 @@ CT_D F3 sampled_light(const DFrame &F, const Shade &sh, Hit &hit, const Mat &mat
   bool ok = ls.ok;
   if (ls.ray && occluded(F, *v, ls.P, ls.dir, ls.dist, ls.ti, ls.tp)) ok = false;
   if (!ok) return f3(0, 0, 0);
-  F3 wi = normalize(sub(ls.l_pos, hit.P));
+  const F3 wi = ls.wi;  // normalize(ls.l_pos - hit.P)
   F3 brdf = evaluate_material(mat, sh, wi);
   hit.scatter = wi;
   F3 bl = mulv(brdf, ls.Le);
@@ -583,10 +597,14 @@ CT_D Res ris(const DFrame &F, float confidence, F3 ray_d, Hit &hit, const Mat &m
 // previous frame's hit primitive and cube light go through the remapping
 // tables (:338-345, :380-387): a deleted primitive ends the reuse, a moved
 // one or a moved light takes its current index.
+// Its two visibility queries take the slots v.slot and v.slot + 1, whether
+// made or not, so that temporal_emit can queue them before the RIS sample's
+// own visibility is known.
 CT_D void temporal_reuse(const DFrame &F, Res &r, Res prev, uint32_t pidx, F3 ray_d, const Hit &hit, const Mat &mat,
                          uint32_t &seed, Vis &v) {
   const float mcap = F.rp.max_influence_from_the_past;
   const bool remap = (F.active & CT_REMAP_BIT) != 0u;
+  const uint32_t slot0 = v.slot;
   prev.M = gmin(prev.M, mcap * r.M);
   const float prev_M = prev.M;
   const float new_M = r.M + prev_M;
@@ -603,6 +621,7 @@ CT_D void temporal_reuse(const DFrame &F, Res &r, Res prev, uint32_t pidx, F3 ra
       Hit ph = hit_from_di(dp);
       Mat pm = ld_mat(F, dp.mat);
       F3 pd = normalize(sub(ph.P, dp.org));
+      v.slot = slot0;
       tlp = luminance(res_radiance(F, r, pd, ph, pm, v));
     }
     const float p_curr = r.M * luminance(r.F);
@@ -614,6 +633,7 @@ CT_D void temporal_reuse(const DFrame &F, Res &r, Res prev, uint32_t pidx, F3 ra
     if (m != 0u) prev.Y = m;
   }
   if (prev.Y != kInvalid) {
+    v.slot = slot0 + 1u;
     F3 ct = res_radiance(F, prev, ray_d, hit, mat, v);
     const float lc = luminance(ct);
     if (lc > 0.0f) {
@@ -627,6 +647,44 @@ CT_D void temporal_reuse(const DFrame &F, Res &r, Res prev, uint32_t pidx, F3 ra
   float tl = luminance(r.F);
   r.W_y = tl > 0.0f ? r.W_sum / tl : 0.0f;
   r.M = new_M;
+}
+
+// Queues every shadow ray temporal_reuse may query, before the RIS sample's
+// visibility is known: `r` is the RIS reservoir as if its sample were
+// visible. A finish pass that sees the sample occluded makes a subset of
+// these queries (the first needs luminance(r.F) > 0, which occlusion only
+// clears; the second depends on `prev` alone), in the same slots; the rays
+// it does not query are traced and ignored.
+CT_D void temporal_emit(const DFrame &F, const Res &r, Res prev, uint32_t pidx, F3 ray_d, const Hit &hit,
+                        const Mat &mat, Vis &v) {
+  const bool remap = (F.active & CT_REMAP_BIT) != 0u;
+  const uint32_t slot0 = v.slot;
+  prev.M = gmin(prev.M, F.rp.max_influence_from_the_past * r.M);
+  if (r.W_sum > 0.0f && luminance(r.F) > 1e-6f) {
+    DI dp = ld_di(F.dpos_p, F.dnrm_p, F.dorg_p, F.dhit_p, pidx);
+    uint32_t m = 0u;
+    if (remap) {
+      const uint32_t slot = (dp.inst < F.S.n_inst ? F.prev_first[dp.inst] : 0u) + dp.prim;
+      m = slot < F.remap_cap ? F.remap_prim[slot] : 0u;
+      if (m != kInvalid && m != 0u) dp.prim = m;
+    }
+    if (m != kInvalid) {
+      Hit ph = hit_from_di(dp);
+      Mat pm = ld_mat(F, dp.mat);
+      F3 pd = normalize(sub(ph.P, dp.org));
+      v.slot = slot0;
+      (void)res_radiance(F, r, pd, ph, pm, v);
+    }
+  }
+  if (remap && prev.type == CT_LIGHT_CUBE && prev.Y < F.remap_light_cap) {
+    const uint32_t m = F.remap_light[prev.Y];
+    if (m != 0u) prev.Y = m;
+  }
+  if (prev.Y != kInvalid) {
+    v.slot = slot0 + 1u;
+    (void)res_radiance(F, prev, ray_d, hit, mat, v);
+  }
+  v.slot = slot0 + 2u;
 }
 
 // SPATIAL_REUSE, restir_resampling.glsl:418-540, with pairwise_mis.glsl:8-114
@@ -714,10 +772,32 @@ CT_D void camera_ray(const DFrame &F, uint32_t x, uint32_t y, F3 &o, F3 &d) {
 
 #include "restir_pt.h"
 
+// Sums a per-lane count over the wave; lane 0 adds it to `dst`.
+CT_D void wave_add(uint32_t v, uint32_t *dst) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  if (__lane_id() == 0 && v) atomicAdd(dst, v);
+}
+CT_D void wave_add_cnt(const TraceCnt &c, unsigned long long *dst) {
+  unsigned long long a = c.nodes, b = c.leaves, d = c.insts;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off);
+    b += __shfl_down(b, off);
+    d += __shfl_down(d, off);
+  }
+  if (__lane_id() == 0) {
+    atomicAdd(&dst[0], a);
+    atomicAdd(&dst[1], b);
+    atomicAdd(&dst[2], d);
+  }
+}
+
 // ----------------------------------------------------------- queue push
 // Appends this thread's emitted slots (bit k -> ray pixel*16 + k) to the
-// ray queue with one atomic per block.  Every thread of the block calls it.
-CT_D void queue_flush(const DFrame &F, uint32_t px, uint32_t emitted, uint32_t counter) {
+// ray queue with one atomic per block and adds their number to counter
+// `counter` (none when negative).  Every thread of the block calls it.
+CT_D void queue_flush(const DFrame &F, uint32_t px, uint32_t emitted, int counter) {
   __shared__ uint32_t s_wsum[kBlock / 64];
   __shared__ uint32_t s_base;
   const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
@@ -738,7 +818,7 @@ CT_D void queue_flush(const DFrame &F, uint32_t px, uint32_t emitted, uint32_t c
       tot += t;
     }
     s_base = tot ? atomicAdd(&F.counters[kCntQueue], tot) : 0u;
-    if (tot) atomicAdd(&F.counters[counter], tot);
+    if (tot && counter >= 0) atomicAdd(&F.counters[counter], tot);
   }
   __syncthreads();
   uint32_t pos = s_base + s_wsum[w] + incl - c;
@@ -793,7 +873,7 @@ __global__ void __launch_bounds__(kBlock) k_ris(FrameSet FS) {
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
   const uint32_t px = blk_.y + threadIdx.x;
-  uint32_t emitted = 0;
+  uint32_t emitted = 0, ris_ray = 0;
   if (px < F.npix) {
     const uint32_t x = px % F.W, y = px / F.W;
     float4 h = F.hit[px];
@@ -862,9 +942,16 @@ __global__ void __launch_bounds__(kBlock) k_ris(FrameSet FS) {
       float confidence = (prev.W_y > 0.0f) ? predicted : 0.0f;
       di.conf = confidence;
       Res r = ris(F, confidence, rd, hit, mat, di.seed);
-      Vis v{true, px, 0u, 0u, 0u};
+      Vis v{true, px, 0u, 0u, 0u, 0u};
       Res rv = r;
       visibility_pass(F, rv, hit, v);
+      ris_ray = v.emitted;
+      if (prev.W_y > 0.0f) {
+        // The temporal reuse rays (slots 1, 2) go into the same launch; the
+        // reuse takes the prepass camera ray (rgen.glsl:261-262).
+        v.slot = 1u;
+        temporal_emit(F, rv, prev, pidx, rd, hit_from_di(di), mat, v);
+      }
       emitted = v.emitted;
       st_res(F.rA_i, F.rB_i, F.rC_i, px, r);
       F.pidx[px] = pidx;
@@ -872,72 +959,49 @@ __global__ void __launch_bounds__(kBlock) k_ris(FrameSet FS) {
     st_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px, di);
     F.mask[px] = 0u;
   }
-  queue_flush(F, px, emitted, kCntRis);
+  wave_add(ris_ray, &F.counters[kCntRis]);
+  queue_flush(F, px, emitted, -1);
 }
 
-// RIS visibility result (FINISH) and the temporal reuse rays (EMIT).
-__global__ void __launch_bounds__(kBlock) k_temporal_emit(FrameSet FS) {
-  const uint2 blk_ = FS.blk[blockIdx.x];
-  const DFrame &F = FS.fb[blk_.x];
-  const uint32_t px = blk_.y + threadIdx.x;
-  uint32_t emitted = 0;
-  if (px < F.npix) {
-    float4 pd = F.dpos_c[px];
-    if (pd.w > 0.0f) {
-      DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
-      Hit hit = hit_from_di(di);
-      Res r = ld_res(F.rA_i, F.rB_i, F.rC_i, px);
-      Vis v{false, px, 0u, F.mask[px], 0u};
-      visibility_pass(F, r, hit, v);
-      st_res(F.rA_i, F.rB_i, F.rC_i, px, r);
-      uint32_t pidx = F.pidx[px];
-      Res prev = res_init();
-      if (pidx < F.W * F.H) prev = ld_res(F.rA_p, F.rB_p, F.rC_p, pidx);
-      if (prev.W_y > 0.0f) {
-        // TEMPORAL_REUSE takes the prepass camera ray (rgen.glsl:261-262).
-        F3 ro, rd;
-        camera_ray(F, px % F.W, px / F.W, ro, rd);
-        Vis ve{true, px, 0u, 0u, 0u};
-        Mat mat = ld_mat(F, di.mat);
-        uint32_t seed = di.seed;
-        temporal_reuse(F, r, prev, pidx, rd, hit, mat, seed, ve);
-        emitted = ve.emitted;
-      }
-    }
-    F.mask[px] = 0u;
-  }
-  queue_flush(F, px, emitted, kCntTemporal);
-}
-
+// RIS visibility (slot 0) and temporal reuse (slots 1, 2) with the results
+// of the one any-hit launch after k_ris.  kCntTemporal counts the temporal
+// queries made, not the rays traced for them.
 __global__ void __launch_bounds__(kBlock) k_temporal_finish(FrameSet FS) {
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
   const uint32_t px = blk_.y + threadIdx.x;
-  if (px >= F.npix) return;
-  float4 pd = F.dpos_c[px];
-  if (!(pd.w > 0.0f)) {
-    F.gat[(size_t)kGather * px] = pd;  // only the distance of a miss is ever read
-    return;
+  uint32_t queries = 0;
+  if (px < F.npix) {
+    const float4 pd = F.dpos_c[px];
+    if (!(pd.w > 0.0f)) {
+      F.gat[(size_t)kGather * px] = pd;  // only the distance of a miss is ever read
+    } else {
+      const float4 pn = F.dnrm_c[px], po = F.dorg_c[px];
+      const uint4 ph = F.dhit_c[px];
+      const DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
+      const Hit hit = hit_from_di(di);
+      Res r = ld_res(F.rA_i, F.rB_i, F.rC_i, px);
+      Vis v{false, px, 0u, F.mask[px], 0u, 0u};
+      visibility_pass(F, r, hit, v);
+      const uint32_t pidx = F.pidx[px];
+      Res prev = res_init();
+      if (pidx < F.W * F.H) prev = ld_res(F.rA_p, F.rB_p, F.rC_p, pidx);
+      if (prev.W_y > 0.0f) {
+        const Mat mat = ld_mat(F, di.mat);
+        F3 ro, rd;
+        camera_ray(F, px % F.W, px / F.W, ro, rd);
+        v.slot = 1u;
+        v.queries = 0u;
+        uint32_t seed = di.seed;
+        temporal_reuse(F, r, prev, pidx, rd, hit, mat, seed, v);
+        queries = v.queries;
+        F.dorg_c[px].w = __uint_as_float(seed);
+      }
+      st_res(F.rA_i, F.rB_i, F.rC_i, px, r);
+      st_gather(F.gat, px, pd, pn, po, ph.x, ph.y, r);
+    }
   }
-  float4 pn = F.dnrm_c[px], po = F.dorg_c[px];
-  uint4 ph = F.dhit_c[px];
-  Res r = ld_res(F.rA_i, F.rB_i, F.rC_i, px);
-  uint32_t pidx = F.pidx[px];
-  Res prev = res_init();
-  if (pidx < F.W * F.H) prev = ld_res(F.rA_p, F.rB_p, F.rC_p, pidx);
-  if (prev.W_y > 0.0f) {
-    DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
-    Hit hit = hit_from_di(di);
-    Mat mat = ld_mat(F, di.mat);
-    F3 ro, rd;
-    camera_ray(F, px % F.W, px / F.W, ro, rd);
-    Vis v{false, px, 0u, F.mask[px], 0u};
-    uint32_t seed = di.seed;
-    temporal_reuse(F, r, prev, pidx, rd, hit, mat, seed, v);
-    st_res(F.rA_i, F.rB_i, F.rC_i, px, r);
-    F.dorg_c[px].w = __uint_as_float(seed);
-  }
-  st_gather(F.gat, px, pd, pn, po, ph.x, ph.y, r);
+  wave_add(queries, &F.counters[kCntTemporal]);
 }
 
 // Shading pass: spatial reuse (EMIT / FINISH) then shade and store.
@@ -956,7 +1020,7 @@ __global__ void __launch_bounds__(kBlock) k_spatial(FrameSet FS) {
       Mat mat = ld_mat(F, di.mat);
       Hit hit = hit_from_di(di);
       Res sr = ld_res(F.rA_i, F.rB_i, F.rC_i, px);
-      Vis v{kEmit, px, 0u, kEmit ? 0u : F.mask[px], 0u};
+      Vis v{kEmit, px, 0u, kEmit ? 0u : F.mask[px], 0u, 0u};
       uint32_t seed = di.seed;
       spatial_reuse(F, sr, di.conf, x, y, ray_d, hit, mat, seed, v);
       emitted = v.emitted;
@@ -988,28 +1052,6 @@ __global__ void __launch_bounds__(kBlock) k_spatial(FrameSet FS) {
     if (kEmit) F.mask[px] = 0u;
   }
   if (kEmit) queue_flush(F, px, emitted, kCntSpatial);
-}
-
-// taa.glsl:16-108 over the dispatch region.
-// Sums a per-lane count over the wave; lane 0 adds it to `dst`.
-CT_D void wave_add(uint32_t v, uint32_t *dst) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-  if (__lane_id() == 0 && v) atomicAdd(dst, v);
-}
-CT_D void wave_add_cnt(const TraceCnt &c, unsigned long long *dst) {
-  unsigned long long a = c.nodes, b = c.leaves, d = c.insts;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    a += __shfl_down(a, off);
-    b += __shfl_down(b, off);
-    d += __shfl_down(d, off);
-  }
-  if (__lane_id() == 0) {
-    atomicAdd(&dst[0], a);
-    atomicAdd(&dst[1], b);
-    atomicAdd(&dst[2], d);
-  }
 }
 
 // Appends this thread's item (when `active`) to a queue with one atomic per
@@ -1264,6 +1306,7 @@ __global__ void __launch_bounds__(kBlock) k_trace_bounce(DFrame F, const float4 
   if (kCount) wave_add_cnt(tr.cnt, stats);
 }
 
+// taa.glsl:16-108 over the dispatch region.
 __global__ void __launch_bounds__(kBlock) k_taa(FrameSet FS) {
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
@@ -1738,13 +1781,8 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark)
   if ((rc = mark())) return rc;  // kMarkPrimary
   hipLaunchKernelGGL(k_ris, dim3(nb), dim3(kBlock), 0, st, pix);
   if ((rc = mark())) return rc;  // kMarkRis
-  launch_any(L, pix, np, st);
+  launch_any(L, pix, 3 * np, st);  // RIS and temporal reuse visibility
   if ((rc = mark())) return rc;  // kMarkTraceRis
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntQueue, 1);
-  hipLaunchKernelGGL(k_temporal_emit, dim3(nb), dim3(kBlock), 0, st, pix);
-  if ((rc = mark())) return rc;  // kMarkTemporalEmit
-  launch_any(L, pix, 2 * np, st);
-  if ((rc = mark())) return rc;  // kMarkTraceTemporal
   hipLaunchKernelGGL(k_temporal_finish, dim3(nb), dim3(kBlock), 0, st, pix);
   if ((rc = mark())) return rc;  // kMarkTemporalFinish
   if (L.pt_depth) {
@@ -2314,13 +2352,13 @@ int ct_frame_get_stats(ct_ctx *c, ct_frame_stats *out) {
       };
       c->stats.ms_primary = ms(kMarkStart, kMarkPrimary);
       c->stats.ms_ris = ms(kMarkPrimary, kMarkRis);
-      c->stats.ms_temporal = ms(kMarkTraceRis, kMarkTemporalEmit) + ms(kMarkTraceTemporal, kMarkTemporalFinish);
+      c->stats.ms_temporal = ms(kMarkTraceRis, kMarkTemporalFinish);
       c->stats.ms_pt = ms(kMarkTemporalFinish, kMarkPt);
       c->stats.ms_spatial = ms(kMarkPt, kMarkSpatialEmit) + ms(kMarkTraceSpatial, kMarkSpatialFinish);
       c->stats.ms_pt_spatial = ms(kMarkSpatialFinish, kMarkPtSpatial);
       c->stats.ms_taa = ms(kMarkPtSpatial, kMarkTaa);
-      c->stats.ms_trace_any = ms(kMarkRis, kMarkTraceRis) + ms(kMarkTemporalEmit, kMarkTraceTemporal) +
-                              ms(kMarkPtEmit, kMarkTracePt) + ms(kMarkSpatialEmit, kMarkTraceSpatial);
+      c->stats.ms_trace_any =
+          ms(kMarkRis, kMarkTraceRis) + ms(kMarkPtEmit, kMarkTracePt) + ms(kMarkSpatialEmit, kMarkTraceSpatial);
       c->stats.ms_trace_closest = c->stats.ms_primary;
       c->stats.ms_total = ms(kMarkStart, kMarkTaa);
       c->stats.ms_pt_segments = 0.0f;
