@@ -333,6 +333,7 @@ struct Walker {
   uint32_t ti, tp, best_i, best_p, first, inst, ref, steps;
   int base;
   bool in_blas, done, occluded, overflow;
+  const DNode *tlas_nodes, *blas_nodes;  // read once per ray, not per step
   Stack st;
 
   CT_D void start(const DScene &S, F3 o_, F3 d_, float limit_, uint32_t ti_, uint32_t tp_, uint32_t *stk,
@@ -357,6 +358,8 @@ struct Walker {
     st.lds = stk;
     st.stride = stride;
     st.n = 0;
+    tlas_nodes = S.tlas_nodes;
+    blas_nodes = S.blas_nodes;
     ref = S.tlas_root;
     done = ref == kInvalid;
   }
@@ -368,34 +371,44 @@ struct Walker {
       return;
     }
     uint32_t next = kInvalid;
-    if (!in_blas) {
-      if (ref & kLeaf) {
-        inst = S.tlas_inst[ref & ~kLeaf];
-        if (kCount) cnt.insts++;
-        const DInst &I = S.inst[inst];
-        float m[12];
-#pragma unroll
-        for (int k = 0; k < 12; ++k) m[k] = I.w2o[k];
-        oo = xform3(m, o, 1.0f);
-        od = xform3(m, d, 0.0f);
-        oinv = f3(1.0f / od.x, 1.0f / od.y, 1.0f / od.z);
-        if (kMode != kTraceClosest) {
-          dr = od;
-          sinv = f3(safe_inverse(od.x), safe_inverse(od.y), safe_inverse(od.z));
-        } else {
-          dr = mul_rot(od);
-        }
-        sg = f3(-gsign(dr.x), -gsign(dr.y), -gsign(dr.z));
-        first = I.first;
-        in_blas = true;
-        base = st.n;
-        ref = I.root;
-        return;
-      }
+    if (!(ref & kLeaf)) {
+      // TLAS and BLAS nodes take the same code (one pass for a wave whose
+      // lanes are at both levels).
       if (kCount) cnt.nodes++;
-      next = visit_node(S.tlas_nodes, ref, o, winv, kAny ? limit : best_t, st, overflow);
+      const DNode *nodes = in_blas ? blas_nodes + first : tlas_nodes;
+      next = visit_node(nodes, ref, in_blas ? oo : o, in_blas ? oinv : winv, kAny ? limit : best_t, st, overflow);
+    } else if (!in_blas) {
+      inst = S.tlas_inst[ref & ~kLeaf];
+      if (kCount) cnt.insts++;
+      const DInst &I = S.inst[inst];
+      float m[12];
+#pragma unroll
+      for (int k = 0; k < 12; ++k) m[k] = I.w2o[k];
+      oo = xform3(m, o, 1.0f);
+      od = xform3(m, d, 0.0f);
+      // An instance without rotation or scale gives back the direction's
+      // own bits (its zero terms add signed zeros), and then 1 / od is winv:
+      // the three divisions run only for lanes whose direction changed.
+      if (__float_as_uint(od.x) == __float_as_uint(d.x) && __float_as_uint(od.y) == __float_as_uint(d.y) &&
+          __float_as_uint(od.z) == __float_as_uint(d.z))
+        oinv = winv;
+      else
+        oinv = f3(1.0f / od.x, 1.0f / od.y, 1.0f / od.z);
+      if (kMode != kTraceClosest) {
+        dr = od;
+        // safe_inverse(od) with the quotient above
+        sinv = f3(od.x == 0.0f ? 1e12f : oinv.x, od.y == 0.0f ? 1e12f : oinv.y, od.z == 0.0f ? 1e12f : oinv.z);
+      } else {
+        dr = mul_rot(od);
+      }
+      sg = f3(-gsign(dr.x), -gsign(dr.y), -gsign(dr.z));
+      first = I.first;
+      in_blas = true;
+      base = st.n;
+      ref = I.root;
+      return;
     } else {
-      if (ref & kLeaf) {
+      {
         if (kCount) cnt.leaves++;
         const float4 L = S.blas_leaves[first + (ref & ~kLeaf)];
         const uint32_t pid = __float_as_uint(L.w);
@@ -421,9 +434,6 @@ struct Walker {
             }
           }
         }
-      } else {
-        if (kCount) cnt.nodes++;
-        next = visit_node(S.blas_nodes, first + ref, oo, oinv, kAny ? limit : best_t, st, overflow);
       }
     }
     if (overflow) {
