@@ -136,9 +136,11 @@ CT_D bool intersect_box(F3 c, F3 ro, F3 dr, F3 sg, F3 inv, float &dist, F3 *norm
     dtp = f3(dtp.x / dr.x, dtp.y / dr.y, dtp.z / dr.z);
   else
     dtp = mulv(dtp, inv);
-  bool tx = (dtp.x >= 0.0f) && (fabsf(o.y + dr.y * dtp.x) < r) && (fabsf(o.z + dr.z * dtp.x) < r);
-  bool ty = (dtp.y >= 0.0f) && (fabsf(o.z + dr.z * dtp.y) < r) && (fabsf(o.x + dr.x * dtp.y) < r);
-  bool tz = (dtp.z >= 0.0f) && (fabsf(o.x + dr.x * dtp.z) < r) && (fabsf(o.y + dr.y * dtp.z) < r);
+  // Every term is evaluated (no short circuit): the tests are pure
+  // arithmetic, so the result is the same and a wave takes no branches.
+  const bool tx = (int)(dtp.x >= 0.0f) & (int)(fabsf(o.y + dr.y * dtp.x) < r) & (int)(fabsf(o.z + dr.z * dtp.x) < r);
+  const bool ty = (int)(dtp.y >= 0.0f) & (int)(fabsf(o.z + dr.z * dtp.y) < r) & (int)(fabsf(o.x + dr.x * dtp.y) < r);
+  const bool tz = (int)(dtp.z >= 0.0f) & (int)(fabsf(o.x + dr.x * dtp.z) < r) & (int)(fabsf(o.y + dr.y * dtp.z) < r);
   float sx = tx ? sg.x : 0.0f;
   float sy = tx ? 0.0f : (ty ? sg.y : 0.0f);
   float sz = (tx || ty) ? 0.0f : (tz ? sg.z : 0.0f);
