@@ -869,7 +869,9 @@ __global__ void __launch_bounds__(kBlock) k_primary(FrameSet FS, unsigned long l
 // Prepass after the primary hit: rchit (rchit.glsl:18-54), miss
 // (rmiss.glsl:19-29), motion vector, temporal gather, confidence and RIS.
 // EMIT for the RIS visibility ray.
-__global__ void __launch_bounds__(kBlock) k_ris(FrameSet FS) {
+// Five waves per SIMD (at most 96 VGPRs, a few dwords spilled) run this
+// VALU-heavy pass faster than four without spills.
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_ris(FrameSet FS) {
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
   const uint32_t px = blk_.y + threadIdx.x;
