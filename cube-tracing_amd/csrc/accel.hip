@@ -347,6 +347,18 @@ __device__ __forceinline__ float ld_relaxed(const float *p) {
   uint32_t u = __hip_atomic_load(reinterpret_cast<const uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return __uint_as_float(u);
 }
+__device__ __forceinline__ void st_relaxed(float *p, float v) {
+  __hip_atomic_store(reinterpret_cast<uint32_t *>(p), __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Waits until this thread's earlier memory operations are done. Stores made
+// as agent-scope atomics are then visible to every XCD, so a flag set after
+// it publishes them without writing back and invalidating the whole L2
+// (which is what an agent-scope __threadfence costs on a multi-XCD part).
+__device__ __forceinline__ void wait_stores_done() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 // Bottom-up refit: the second child to arrive at a node merges both
 // child boxes (min/max are exact, so the result is order independent).
@@ -382,11 +394,13 @@ __global__ void k_refit(SegView sv, const uint32_t *svals, const float4 *centers
     float *f = reinterpret_cast<float *>(nd);
     uint32_t r0 = nd->d.x;
     int slot = (r0 == ref) ? 0 : 6;
-    for (int a = 0; a < 6; ++a) f[slot + a] = b[a];
-    __threadfence();
+    // Child boxes and heights go through agent-scope atomic stores and
+    // loads, so the flag needs no full fence: the writer waits for its own
+    // stores before the increment, and the second arrival reads after it.
+    for (int a = 0; a < 6; ++a) st_relaxed(f + slot + a, b[a]);
+    wait_stores_done();
     uint32_t old = atomicAdd(&flags[nb + node], 1u);
     if (old != 1u) return;  // first arrival (0) waits for its sibling; >1 cannot happen in a tree
-    __threadfence();
     float c0[6], c1[6];
     for (int a = 0; a < 6; ++a) {
       c0[a] = ld_relaxed(f + a);
@@ -401,7 +415,7 @@ __global__ void k_refit(SegView sv, const uint32_t *svals, const float4 *centers
     uint32_t h0 = (r0c & kLeaf) ? 0u : __hip_atomic_load(heights + nb + r0c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t h1 = (r1c & kLeaf) ? 0u : __hip_atomic_load(heights + nb + r1c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     h = max(h0, h1) + 1u;
-    heights[nb + node] = h;
+    __hip_atomic_store(heights + nb + node, h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     ref = node;
     node = parent_int[nb + node];
   }
