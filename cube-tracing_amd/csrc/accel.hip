@@ -87,24 +87,15 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n) {
   if (i < n) p[i] = v;
 }
 
-// Per-segment centre bounds: one set of 6 atomics per block when the
-// block's items belong to one segment (the common case), else per item.
-__device__ void bounds_accumulate(SegView sv, uint32_t k, bool valid, uint32_t s, float cx, float cy, float cz,
-                                  uint32_t *bounds) {
+// Per-segment centre bounds over a tile of kBoundItems rows of kBlock items:
+// one set of 6 atomics per block when the tile lies in one segment (the
+// common case); a tile across segments reduces row by row, and only a row
+// that itself spans segments falls back to 6 atomics per item. Few, large
+// tiles keep the atomics on a segment's six words from serialising.
+constexpr int kBoundItems = 16;
+constexpr uint32_t kBoundTile = kBlock * kBoundItems;
+__device__ void block_bounds_to(uint32_t v[6], uint32_t *bb) {
   __shared__ uint32_t red[6][kBlock / 64];
-  const uint32_t first = blockIdx.x * blockDim.x;
-  const uint32_t last = min(first + blockDim.x, sv.n) - 1u;
-  const bool uniform = find_seg(sv.off, sv.nseg, first) == find_seg(sv.off, sv.nseg, last);
-  if (!uniform) {
-    if (valid) {
-      uint32_t *bb = bounds + 6 * s;
-      atomicMin(bb + 0, fenc(cx)); atomicMin(bb + 1, fenc(cy)); atomicMin(bb + 2, fenc(cz));
-      atomicMax(bb + 3, fenc(cx)); atomicMax(bb + 4, fenc(cy)); atomicMax(bb + 5, fenc(cz));
-    }
-    return;
-  }
-  uint32_t v[6] = {valid ? fenc(cx) : 0xFFFFFFFFu, valid ? fenc(cy) : 0xFFFFFFFFu, valid ? fenc(cz) : 0xFFFFFFFFu,
-                   valid ? fenc(cx) : 0u, valid ? fenc(cy) : 0u, valid ? fenc(cz) : 0u};
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
@@ -117,45 +108,75 @@ __device__ void bounds_accumulate(SegView sv, uint32_t k, bool valid, uint32_t s
     for (int a = 0; a < 6; ++a) red[a][w] = v[a];
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t sfirst = find_seg(sv.off, sv.nseg, first);
-    uint32_t *bb = bounds + 6 * sfirst;
     for (int a = 0; a < 6; ++a) {
       uint32_t r = red[a][0];
-      for (int q = 1; q < (int)(blockDim.x / 64); ++q) r = a < 3 ? min(r, red[a][q]) : max(r, red[a][q]);
+      for (int q = 1; q < kBlock / 64; ++q) r = a < 3 ? min(r, red[a][q]) : max(r, red[a][q]);
       if (a < 3) atomicMin(bb + a, r); else atomicMax(bb + a, r);
     }
   }
+  __syncthreads();
+}
+template <class Centre>
+__device__ void tile_bounds(SegView sv, Centre centre, uint32_t *bounds) {
+  const uint32_t first = blockIdx.x * kBoundTile;
+  if (first >= sv.n) return;  // the whole block
+  const uint32_t last = min(first + kBoundTile, sv.n) - 1u;
+  const uint32_t s0 = find_seg(sv.off, sv.nseg, first);
+  const bool uniform = s0 == find_seg(sv.off, sv.nseg, last);
+  uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+  for (int i = 0; i < kBoundItems; ++i) {
+    const uint32_t row = first + (uint32_t)i * kBlock;
+    if (row >= sv.n) break;  // block-uniform
+    const uint32_t k = row + threadIdx.x;
+    const bool valid = k < sv.n;
+    const uint32_t rs0 = uniform ? s0 : find_seg(sv.off, sv.nseg, row);
+    const bool row_uniform = uniform || rs0 == find_seg(sv.off, sv.nseg, min(row + kBlock, sv.n) - 1u);
+    const uint32_t s = row_uniform ? rs0 : (valid ? find_seg(sv.off, sv.nseg, k) : 0u);
+    uint32_t e[3] = {0u, 0u, 0u};
+    if (valid) {
+      const float4 c = centre(k, s);
+      e[0] = fenc(c.x);
+      e[1] = fenc(c.y);
+      e[2] = fenc(c.z);
+    }
+    if (uniform) {
+      if (valid)
+        for (int a = 0; a < 3; ++a) {
+          v[a] = min(v[a], e[a]);
+          v[3 + a] = max(v[3 + a], e[a]);
+        }
+    } else if (row_uniform) {
+      uint32_t r[6] = {valid ? e[0] : 0xFFFFFFFFu, valid ? e[1] : 0xFFFFFFFFu, valid ? e[2] : 0xFFFFFFFFu,
+                       valid ? e[0] : 0u, valid ? e[1] : 0u, valid ? e[2] : 0u};
+      block_bounds_to(r, bounds + 6 * rs0);
+    } else if (valid) {
+      uint32_t *bb = bounds + 6 * s;
+      for (int a = 0; a < 3; ++a) {
+        atomicMin(bb + a, e[a]);
+        atomicMax(bb + 3 + a, e[a]);
+      }
+    }
+  }
+  if (uniform) block_bounds_to(v, bounds + 6 * s0);
 }
 
 // Item centres of voxel AABBs and per-segment centre bounds.
 __global__ void __launch_bounds__(kBlock) k_blas_items(SegView sv, const ct_aabb *aabbs, float4 *centers,
                                                         uint32_t *bounds) {
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  bool valid = k < sv.n;
-  uint32_t s = 0;
-  float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-  if (valid) {
-    s = find_seg(sv.off, sv.nseg, k);
-    uint32_t g = sv.src[s] + (k - sv.off[s]);
-    const ct_aabb &b = aabbs[g];
-    cx = (b.minimum[0] + b.maximum[0]) * 0.5f;
-    cy = (b.minimum[1] + b.maximum[1]) * 0.5f;
-    cz = (b.minimum[2] + b.maximum[2]) * 0.5f;
-    centers[k] = make_float4(cx, cy, cz, 0.0f);
-  }
-  bounds_accumulate(sv, k, valid, s, cx, cy, cz, bounds);
+  tile_bounds(
+      sv,
+      [&](uint32_t k, uint32_t s) {
+        const ct_aabb &b = aabbs[sv.src[s] + (k - sv.off[s])];
+        const float4 c = make_float4((b.minimum[0] + b.maximum[0]) * 0.5f, (b.minimum[1] + b.maximum[1]) * 0.5f,
+                                     (b.minimum[2] + b.maximum[2]) * 0.5f, 0.0f);
+        centers[k] = c;
+        return c;
+      },
+      bounds);
 }
 
 __global__ void __launch_bounds__(kBlock) k_item_bounds(SegView sv, const float4 *centers, uint32_t *bounds) {
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  bool valid = k < sv.n;
-  uint32_t s = 0;
-  float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (valid) {
-    s = find_seg(sv.off, sv.nseg, k);
-    c = centers[k];
-  }
-  bounds_accumulate(sv, k, valid, s, c.x, c.y, c.z, bounds);
+  tile_bounds(sv, [&](uint32_t k, uint32_t) { return centers[k]; }, bounds);
 }
 
 __device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
@@ -228,6 +249,58 @@ __global__ void __launch_bounds__(1024) k_rs_scan(uint32_t *hist, uint32_t total
     uint32_t h = hist[i];
     hist[i] = run;
     run += h;
+  }
+}
+
+// Exclusive scan of a long array in three passes: tile sums, a scan of the
+// sums (k_rs_scan, one block), then each tile's own scan plus its offset.
+// A tile is kBlock threads x kScanItems consecutive words per thread.
+constexpr int kScanItems = 16;
+constexpr uint32_t kScanTile = kBlock * kScanItems;
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t &total) {
+  __shared__ uint32_t wsum[kBlock / 64];
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t o = __shfl_up(incl, off);
+    if ((int)lane >= off) incl += o;
+  }
+  if (lane == 63u) wsum[w] = incl;
+  __syncthreads();
+  uint32_t before = 0u;
+  total = 0u;
+  for (uint32_t q = 0; q < kBlock / 64; ++q) {
+    if (q < w) before += wsum[q];
+    total += wsum[q];
+  }
+  __syncthreads();
+  return before + incl - v;
+}
+__global__ void __launch_bounds__(kBlock) k_scan_sums(const uint32_t *a, uint32_t n, uint32_t *sums) {
+  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t t = 0u;
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < n) t += a[base + i];
+  uint32_t total;
+  (void)block_exclusive_scan(t, total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(uint32_t *a, uint32_t n, const uint32_t *offsets) {
+  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t x[kScanItems];
+  uint32_t t = 0u;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    x[i] = base + i < n ? a[base + i] : 0u;
+    t += x[i];
+  }
+  uint32_t total;
+  uint32_t run = offsets[blockIdx.x] + block_exclusive_scan(t, total);
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    if (base + i < n) a[base + i] = run;
+    run += x[i];
   }
 }
 
@@ -517,7 +590,7 @@ int build_segments(Accel &a, hipStream_t st, const std::vector<uint32_t> &off,
       (rc = S.centers.reserve(16 * (size_t)n)) || (rc = S.flags.reserve(4 * (size_t)node_capacity)) ||
       (rc = S.heights.reserve(4 * (size_t)node_capacity)) ||
       (rc = S.root_box.reserve(24 * (size_t)nseg)) || (rc = S.depth.reserve(4 * (size_t)nseg)) ||
-      (rc = S.sorted_keys32.reserve(4 * (size_t)n)) || (rc = S.hist.reserve(4 * 256 * (size_t)(blocks(n, kRsTile) + 1))))
+      (rc = S.sorted_keys32.reserve(4 * (size_t)n)) || (rc = S.hist.reserve(4 * (256 * (size_t)(blocks(n, kRsTile) + 1) + blocks(256u * blocks(n, kRsTile), kScanTile) + 1))))
     return rc;
   uint32_t *d_bounds = S.hist.as<uint32_t>();  // reused before the sort needs it
   std::vector<uint32_t> binit(6 * (size_t)nseg);
@@ -528,10 +601,10 @@ int build_segments(Accel &a, hipStream_t st, const std::vector<uint32_t> &off,
   CT_HIP(hipMemcpyAsync(d_bounds, binit.data(), 24 * (size_t)nseg, hipMemcpyHostToDevice, st));
   float4 *d_centers = S.centers.as<float4>();
   if (voxel_items) {
-    hipLaunchKernelGGL(k_blas_items, dim3(blocks(n)), dim3(kBlock), 0, st, sv, d_aabbs, d_centers, d_bounds);
+    hipLaunchKernelGGL(k_blas_items, dim3(blocks(n, kBoundTile)), dim3(kBlock), 0, st, sv, d_aabbs, d_centers, d_bounds);
   } else {
     CT_HIP(hipMemcpyAsync(d_centers, d_centers_in, 16 * (size_t)n, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_item_bounds, dim3(blocks(n)), dim3(kBlock), 0, st, sv, d_centers, d_bounds);
+    hipLaunchKernelGGL(k_item_bounds, dim3(blocks(n, kBoundTile)), dim3(kBlock), 0, st, sv, d_centers, d_bounds);
   }
   unsigned long long *k0 = S.keys[0].as<unsigned long long>(), *k1 = S.keys[1].as<unsigned long long>();
   uint32_t *v0 = S.vals[0].as<uint32_t>(), *v1 = S.vals[1].as<uint32_t>();
@@ -544,7 +617,10 @@ int build_segments(Accel &a, hipStream_t st, const std::vector<uint32_t> &off,
   for (int p = 0; p < passes; ++p) {
     int shift = p < 4 ? 8 * p : 32 + 8 * (p - 4);
     hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(kBlock), 0, st, k0, n, shift, d_hist);
-    hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, d_hist, 256u * nb);
+    const uint32_t nh = 256u * nb, nt = blocks(nh, kScanTile);
+    hipLaunchKernelGGL(k_scan_sums, dim3(nt), dim3(kBlock), 0, st, d_hist, nh, d_hist + nh);
+    hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, st, d_hist + nh, nt);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kBlock), 0, st, d_hist, nh, d_hist + nh);
     hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(kBlock), 0, st, k0, v0, k1, v1, n, shift, d_hist);
     std::swap(k0, k1);
     std::swap(v0, v1);
@@ -720,7 +796,7 @@ int refit_blas_set(Accel &a, hipStream_t st, const std::vector<uint32_t> &ids) {
   float4 *d_centers = B.centers.as<float4>();
   uint32_t *d_vals = B.vals[0].as<uint32_t>();
   uint32_t *par = S.blas_parent.as<uint32_t>();
-  hipLaunchKernelGGL(k_blas_items, dim3(blocks(n)), dim3(kBlock), 0, st, sv, S.aabbs.as<ct_aabb>(), d_centers,
+  hipLaunchKernelGGL(k_blas_items, dim3(blocks(n, kBoundTile)), dim3(kBlock), 0, st, sv, S.aabbs.as<ct_aabb>(), d_centers,
                      B.lo.as<uint32_t>());
   hipLaunchKernelGGL(k_refit_prep, dim3(blocks(n)), dim3(kBlock), 0, st, sv, S.blas_ids.as<uint32_t>(), d_centers,
                      d_vals, S.leaves.as<float4>());
