@@ -122,7 +122,7 @@ def batched_bench(frames=5, warmup=2, n_views=64):
     hip = ctypes.CDLL("libamdhip64.so")
     params = [[ctamd.frame_params(w, f) for _, w in views] for f in range(warmup + frames)]
     tune = b.autotune(params[0], repeats=2)
-    log(f"batched: autotuned traversal grid {tune.blocks_per_cu} blocks/CU ({tune.ms:.2f} ms)")
+    log(f"batched: autotuned {tune.groups} view groups, traversal grid {tune.blocks_per_cu} blocks/CU ({tune.ms:.2f} ms)")
     # plain: one ct_frame_render per view and frame
     t_plain, t_batch, rays = 0.0, 0.0, 0
     for f in range(warmup + frames):
@@ -147,7 +147,7 @@ def batched_bench(frames=5, warmup=2, n_views=64):
             "plain_ms_per_frame": round(1000 * t_plain / frames, 3),
             "batched_ms_per_frame": round(1000 * t_batch / frames, 3),
             "speedup": round(t_plain / t_batch, 2), "batched_mrays_s": round(rays / t_batch / 1e6, 1),
-            "graph": True, "tuned_blocks_per_cu": tune.blocks_per_cu}
+            "graph": True, "tuned_blocks_per_cu": tune.blocks_per_cu, "tuned_groups": tune.groups}
 
 
 def adjoint_bench(steps=3):

@@ -1949,15 +1949,29 @@ int ct_ctx_reset(ct_ctx *c, void *stream) {
   return zero_frame(*c, st);
 }
 
+// A batch renders its views in up to kMaxGroups groups, each a contiguous
+// range of views with its own ray and segment queues and counters, on its
+// own stream. Groups are independent, so one group's traversal tail runs
+// beside another group's shading passes.
+constexpr uint32_t kMaxGroups = 4;
+struct BatchGroup {
+  uint32_t first = 0, count = 0;  // views [first, first + count)
+  uint64_t cap_px = 0;
+  DevBuf queue, counters, seg_o, seg_d, seg_hit, ptq[2], blk;
+  uint32_t blocks[3] = {0, 0, 0};
+  hipStream_t stream = nullptr;   // group 0 runs on the render stream
+  hipEvent_t done = nullptr;
+};
 struct ct_batch {
   int device = 0;
   hipStream_t stream = nullptr;
   std::vector<ct_ctx *> ctxs;
   std::vector<ct_accel *> accels;
-  DevBuf frames, blk, queue, counters, seg_o, seg_d, seg_hit, ptq[2];
-  uint32_t capacity_px = 0;
+  std::vector<uint64_t> view_cap;   // pixels each view can hold
+  DevBuf frames;
+  BatchGroup groups[kMaxGroups];
+  uint32_t ngroups = 1;
   std::vector<uint32_t> tables_key, graph_key;
-  uint32_t blocks[3] = {0, 0, 0};
   DFrame *pinned = nullptr;  // [2][n] frame records being uploaded
   hipEvent_t slot_free[2] = {};
   bool slot_used[2] = {false, false};
@@ -1965,8 +1979,8 @@ struct ct_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint32_t blocks_per_cu = 5;
-  uint32_t *counters_pinned = nullptr;
-  hipEvent_t t0 = nullptr, t1 = nullptr;
+  uint32_t *counters_pinned = nullptr;  // [kMaxGroups][8]
+  hipEvent_t t0 = nullptr, t1 = nullptr, fork = nullptr;
   hipStream_t last_stream = nullptr;
   bool pending = false;
   uint64_t primary_rays = 0;
@@ -1984,6 +1998,54 @@ void drop_graph(ct_batch *b) {
   b->exec = nullptr;
   b->graph = nullptr;
   b->graph_key.clear();
+}
+}  // namespace
+
+namespace {
+// Splits the views into g contiguous groups of about equal pixel capacity
+// and sizes each group's queues.
+int set_groups(ct_batch *b, uint32_t g) {
+  const uint32_t n = (uint32_t)b->ctxs.size();
+  g = std::max(1u, std::min(std::min(g, kMaxGroups), n));
+  uint64_t total = 0;
+  for (uint64_t c : b->view_cap) total += c;
+  uint32_t v = 0;
+  uint64_t acc = 0;
+  for (uint32_t k = 0; k < g; ++k) {
+    BatchGroup &G = b->groups[k];
+    G.first = v;
+    G.cap_px = 0;
+    const uint64_t target = total * (k + 1) / g;
+    // at least one view per group, and one left for each later group
+    while (v < n && (v == G.first || (acc < target && n - v > g - 1 - k))) {
+      acc += b->view_cap[v];
+      G.cap_px += b->view_cap[v];
+      ++v;
+    }
+    if (k == g - 1)
+      while (v < n) {
+        G.cap_px += b->view_cap[v];
+        ++v;
+      }
+    G.count = v - G.first;
+    int rc;
+    const uint64_t cap = G.cap_px;
+    if ((rc = G.queue.reserve(4 * cap * kRaySlots)) || (rc = G.counters.reserve(64 * sizeof(uint32_t))) ||
+        (rc = G.seg_o.reserve(16 * cap)) || (rc = G.seg_d.reserve(16 * cap)) || (rc = G.seg_hit.reserve(16 * cap)) ||
+        (rc = G.ptq[0].reserve(4 * cap)) || (rc = G.ptq[1].reserve(4 * cap)))
+      return rc;
+    if (k > 0 && !G.stream) {
+      if (hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&G.done, hipEventDisableTiming) != hipSuccess)
+        return fail(CT_E_HIP, "ct_batch: stream creation failed");
+    }
+  }
+  if (g != b->ngroups) {
+    b->ngroups = g;
+    b->tables_key.clear();
+    drop_graph(b);
+  }
+  return CT_OK;
 }
 }  // namespace
 
@@ -2008,18 +2070,16 @@ int ct_batch_create(uint32_t n, ct_ctx *const *ctxs, ct_accel *const *accels, ct
   b->device = ctxs[0]->device;
   b->ctxs.assign(ctxs, ctxs + n);
   b->accels.assign(accels, accels + n);
-  b->capacity_px = (uint32_t)cap;
+  for (uint32_t i = 0; i < n; ++i) b->view_cap.push_back((uint64_t)ctxs[i]->max_w * ctxs[i]->max_h);
   int rc = CT_OK;
   void *pin = nullptr, *cpin = nullptr;
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-      (rc = b->frames.reserve(sizeof(DFrame) * n)) || (rc = b->queue.reserve(4 * cap * kRaySlots)) ||
-      (rc = b->counters.reserve(64 * sizeof(uint32_t))) || (rc = b->seg_o.reserve(16 * cap)) ||
-      (rc = b->seg_d.reserve(16 * cap)) || (rc = b->seg_hit.reserve(16 * cap)) || (rc = b->ptq[0].reserve(4 * cap)) ||
-      (rc = b->ptq[1].reserve(4 * cap)) ||
+      (rc = b->frames.reserve(sizeof(DFrame) * n)) ||
       hipHostMalloc(&pin, 2 * sizeof(DFrame) * n, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(&cpin, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&cpin, kMaxGroups * 8 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
       hipEventCreate(&b->slot_free[0]) != hipSuccess || hipEventCreate(&b->slot_free[1]) != hipSuccess ||
-      hipEventCreate(&b->t0) != hipSuccess || hipEventCreate(&b->t1) != hipSuccess) {
+      hipEventCreate(&b->t0) != hipSuccess || hipEventCreate(&b->t1) != hipSuccess ||
+      hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess || (rc = set_groups(b, 2))) {
     b->pinned = static_cast<DFrame *>(pin);
     b->counters_pinned = static_cast<uint32_t *>(cpin);
     ct_batch_destroy(b);
@@ -2037,9 +2097,15 @@ int ct_batch_destroy(ct_batch *b) {
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->last_stream) (void)hipStreamSynchronize(b->last_stream);
+  for (BatchGroup &G : b->groups)
+    if (G.stream) (void)hipStreamSynchronize(G.stream);
   drop_graph(b);
-  for (hipEvent_t e : {b->slot_free[0], b->slot_free[1], b->t0, b->t1})
+  for (hipEvent_t e : {b->slot_free[0], b->slot_free[1], b->t0, b->t1, b->fork})
     if (e) (void)hipEventDestroy(e);
+  for (BatchGroup &G : b->groups) {
+    if (G.done) (void)hipEventDestroy(G.done);
+    if (G.stream) (void)hipStreamDestroy(G.stream);
+  }
   if (b->pinned) (void)hipHostFree(b->pinned);
   if (b->counters_pinned) (void)hipHostFree(b->counters_pinned);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -2058,22 +2124,36 @@ int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, v
   // the host records of this slot were uploaded two renders ago
   if (b->slot_used[b->slot]) CT_HIP(hipEventSynchronize(b->slot_free[b->slot]));
   DFrame *rec = b->pinned + (size_t)b->slot * n;
-  uint32_t pt_depth = 0, max_nb = 0;
-  for (uint32_t i = 0; i < n; ++i) {
-    DFrame &F = rec[i];
-    if ((rc = prepare_frame(b->ctxs[i], b->accels[i], &params[i], st, F))) return rc;
-    F.queue = b->queue.as<uint32_t>();
-    F.counters = b->counters.as<uint32_t>();
-    F.seg_o = b->seg_o.as<float4>();
-    F.seg_d = b->seg_d.as<float4>();
-    F.seg_hit = b->seg_hit.as<float4>();
-    F.ptq[0] = b->ptq[0].as<uint32_t>();
-    F.ptq[1] = b->ptq[1].as<uint32_t>();
-    F.qtag = n > 1 ? (i << kViewShift) : 0u;
-    if (F.rp.indirect) pt_depth = std::max(pt_depth, F.rp.max_depth);
-    max_nb = std::max(max_nb, F.rp.max_neighbors);
+  const uint32_t ng = b->ngroups;
+  Launch L[kMaxGroups];
+  for (uint32_t g = 0; g < ng; ++g) {
+    BatchGroup &G = b->groups[g];
+    uint32_t pt_depth = 0, max_nb = 0, px = 0;
+    for (uint32_t i = G.first; i < G.first + G.count; ++i) {
+      DFrame &F = rec[i];
+      if ((rc = prepare_frame(b->ctxs[i], b->accels[i], &params[i], st, F))) return rc;
+      F.queue = G.queue.as<uint32_t>();
+      F.counters = G.counters.as<uint32_t>();
+      F.seg_o = G.seg_o.as<float4>();
+      F.seg_d = G.seg_d.as<float4>();
+      F.seg_hit = G.seg_hit.as<float4>();
+      F.ptq[0] = G.ptq[0].as<uint32_t>();
+      F.ptq[1] = G.ptq[1].as<uint32_t>();
+      F.qtag = G.count > 1 ? ((i - G.first) << kViewShift) : 0u;
+      if (F.rp.indirect) pt_depth = std::max(pt_depth, F.rp.max_depth);
+      max_nb = std::max(max_nb, F.rp.max_neighbors);
+      px += F.npix;
+    }
+    L[g].fb = b->frames.as<DFrame>() + G.first;
+    L[g].total_px = px;
+    L[g].pt_depth = pt_depth;
+    L[g].max_neighbors = max_nb;
+    L[g].trace_blocks = (uint32_t)cu_count(b->device) * b->blocks_per_cu;
+    L[g].multi = G.count > 1;
+    L[g].counters = G.counters.as<uint32_t>();
+    L[g].stats = nullptr;
   }
-  std::vector<uint32_t> tkey;
+  std::vector<uint32_t> tkey{ng};
   for (uint32_t i = 0; i < n; ++i) {
     tkey.push_back(rec[i].W);
     tkey.push_back(rec[i].H);
@@ -2081,59 +2161,73 @@ int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, v
     tkey.push_back(rec[i].taa_h);
   }
   if (tkey != b->tables_key) {
-    std::vector<uint2> tbl;
-    build_tables(rec, n, tbl, b->blocks);
     CT_HIP(hipStreamSynchronize(st));
-    if ((rc = b->blk.reserve(sizeof(uint2) * tbl.size()))) return rc;
-    CT_HIP(hipMemcpy(b->blk.p, tbl.data(), sizeof(uint2) * tbl.size(), hipMemcpyHostToDevice));
+    for (uint32_t g = 0; g < ng; ++g) {
+      BatchGroup &G = b->groups[g];
+      std::vector<uint2> tbl;
+      build_tables(rec + G.first, G.count, tbl, G.blocks);
+      if ((rc = G.blk.reserve(sizeof(uint2) * tbl.size()))) return rc;
+      CT_HIP(hipMemcpy(G.blk.p, tbl.data(), sizeof(uint2) * tbl.size(), hipMemcpyHostToDevice));
+    }
     b->tables_key = tkey;
     drop_graph(b);
+  }
+  for (uint32_t g = 0; g < ng; ++g) {
+    L[g].blk = b->groups[g].blk.as<uint2>();
+    std::memcpy(L[g].blocks, b->groups[g].blocks, sizeof(L[g].blocks));
   }
   CT_HIP(hipMemcpyAsync(b->frames.p, rec, sizeof(DFrame) * n, hipMemcpyHostToDevice, st));
   CT_HIP(hipEventRecord(b->slot_free[b->slot], st));
   b->slot_used[b->slot] = true;
   b->slot ^= 1;
-  Launch L;
-  L.fb = b->frames.as<DFrame>();
-  L.blk = b->blk.as<uint2>();
-  std::memcpy(L.blocks, b->blocks, sizeof(L.blocks));
-  L.total_px = 0;
-  for (uint32_t i = 0; i < n; ++i) L.total_px += rec[i].npix;
-  L.pt_depth = pt_depth;
-  L.max_neighbors = max_nb;
-  L.trace_blocks = (uint32_t)cu_count(b->device) * b->blocks_per_cu;
-  L.multi = n > 1;
-  L.counters = b->counters.as<uint32_t>();
-  L.stats = nullptr;
   auto none = []() -> int { return CT_OK; };
+  // Group 0 on `st`, the others on their streams between a fork and a join
+  // on `st` (also inside a graph capture, where they become branches).
+  auto enqueue_all = [&]() -> int {
+    if (ng > 1) CT_HIP(hipEventRecord(b->fork, st));
+    for (uint32_t g = 1; g < ng; ++g) CT_HIP(hipStreamWaitEvent(b->groups[g].stream, b->fork, 0));
+    for (uint32_t g = 0; g < ng; ++g) {
+      int e = enqueue_frames(L[g], g == 0 ? st : b->groups[g].stream, none, none);
+      if (e) return e;
+    }
+    for (uint32_t g = 1; g < ng; ++g) {
+      CT_HIP(hipEventRecord(b->groups[g].done, b->groups[g].stream));
+      CT_HIP(hipStreamWaitEvent(st, b->groups[g].done, 0));
+    }
+    return CT_OK;
+  };
   CT_HIP(hipEventRecord(b->t0, st));
   if (use_graph) {
     std::vector<uint32_t> gkey = tkey;
-    gkey.insert(gkey.end(), {L.total_px, pt_depth, max_nb, L.trace_blocks, (uint32_t)L.multi});
+    for (uint32_t g = 0; g < ng; ++g)
+      gkey.insert(gkey.end(), {L[g].total_px, L[g].pt_depth, L[g].max_neighbors, L[g].trace_blocks, (uint32_t)L[g].multi});
     if (!b->exec || gkey != b->graph_key) {
       drop_graph(b);
       CT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      rc = enqueue_frames(L, st, none, none);
-      hipGraph_t g = nullptr;
-      const hipError_t e = hipStreamEndCapture(st, &g);
+      rc = enqueue_all();
+      hipGraph_t gr = nullptr;
+      const hipError_t e = hipStreamEndCapture(st, &gr);
       if (rc) {
-        if (g) (void)hipGraphDestroy(g);
+        if (gr) (void)hipGraphDestroy(gr);
         return rc;
       }
       if (e != hipSuccess) return fail(CT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-      b->graph = g;
+      b->graph = gr;
       CT_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
       b->graph_key = gkey;
     }
     CT_HIP(hipGraphLaunch(b->exec, st));
-  } else if ((rc = enqueue_frames(L, st, none, none))) {
+  } else if ((rc = enqueue_all())) {
     return rc;
   }
   CT_HIP(hipEventRecord(b->t1, st));
-  CT_HIP(hipMemcpyAsync(b->counters_pinned, b->counters.p, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  for (uint32_t g = 0; g < ng; ++g)
+    CT_HIP(hipMemcpyAsync(b->counters_pinned + 8 * g, b->groups[g].counters.p, 8 * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, st));
   b->pending = true;
   b->last_stream = st;
-  b->primary_rays = L.total_px;
+  b->primary_rays = 0;
+  for (uint32_t g = 0; g < ng; ++g) b->primary_rays += L[g].total_px;
   for (uint32_t i = 0; i < n; ++i) {
     ct_ctx *c = b->ctxs[i];
     c->stats_pending = false;
@@ -2151,14 +2245,16 @@ int ct_batch_get_stats(ct_batch *b, ct_frame_stats *out) {
   if (!b->pending) return CT_OK;
   CT_HIP(hipSetDevice(b->device));
   CT_HIP(hipStreamSynchronize(b->last_stream));
-  const uint32_t *cnt = b->counters_pinned;
   out->primary_rays = b->primary_rays;
-  out->shadow_rays_ris = cnt[kCntRis];
-  out->shadow_rays_temporal = cnt[kCntTemporal];
-  out->shadow_rays_spatial = cnt[kCntSpatial];
-  out->shadow_rays_pt = cnt[kCntPtShadow];
-  out->bounce_rays = cnt[kCntPtBounce];
-  out->shadow_rays = (uint64_t)cnt[kCntRis] + cnt[kCntTemporal] + cnt[kCntSpatial] + cnt[kCntPtShadow];
+  for (uint32_t g = 0; g < b->ngroups; ++g) {
+    const uint32_t *cnt = b->counters_pinned + 8 * g;
+    out->shadow_rays_ris += cnt[kCntRis];
+    out->shadow_rays_temporal += cnt[kCntTemporal];
+    out->shadow_rays_spatial += cnt[kCntSpatial];
+    out->shadow_rays_pt += cnt[kCntPtShadow];
+    out->bounce_rays += cnt[kCntPtBounce];
+  }
+  out->shadow_rays = out->shadow_rays_ris + out->shadow_rays_temporal + out->shadow_rays_spatial + out->shadow_rays_pt;
   float ms = 0.0f;
   CT_HIP(hipEventElapsedTime(&ms, b->t0, b->t1));
   out->ms_total = ms;
@@ -2167,32 +2263,40 @@ int ct_batch_get_stats(ct_batch *b, ct_frame_stats *out) {
 
 int ct_batch_autotune(ct_batch *b, const ct_frame_params *params, uint32_t repeats, ct_batch_tuning *out) {
   if (!b || !params || repeats == 0) return fail(CT_E_INVALID, "ct_batch_autotune: bad argument");
-  const uint32_t cand[] = {2, 3, 4, 5};
+  const uint32_t groups[] = {1, 2, 3, 4}, per_cu[] = {4, 5};
   float best = 0.0f;
-  uint32_t best_c = b->blocks_per_cu;
+  uint32_t best_g = b->ngroups, best_c = b->blocks_per_cu, tried = 0;
   int rc;
-  for (uint32_t c : cand) {
-    b->blocks_per_cu = c;
-    float tot = 0.0f;
-    for (uint32_t r = 0; r < repeats + 1; ++r) {  // the first render captures the graph
-      if ((rc = ct_batch_render(b, params, 1, nullptr))) return rc;
-      ct_frame_stats s;
-      if ((rc = ct_batch_get_stats(b, &s))) return rc;
-      if (r > 0) tot += s.ms_total;
-    }
-    const float ms = tot / (float)repeats;
-    if (best == 0.0f || ms < best) {
-      best = ms;
-      best_c = c;
+  for (uint32_t g : groups) {
+    if (g > b->ctxs.size()) break;
+    for (uint32_t c : per_cu) {
+      if ((rc = set_groups(b, g))) return rc;
+      if (c != b->blocks_per_cu) drop_graph(b);
+      b->blocks_per_cu = c;
+      float tot = 0.0f;
+      for (uint32_t r = 0; r < repeats + 1; ++r) {  // the first render captures the graph
+        if ((rc = ct_batch_render(b, params, 1, nullptr))) return rc;
+        ct_frame_stats s;
+        if ((rc = ct_batch_get_stats(b, &s))) return rc;
+        if (r > 0) tot += s.ms_total;
+      }
+      ++tried;
+      const float ms = tot / (float)repeats;
+      if (best == 0.0f || ms < best) {
+        best = ms;
+        best_g = b->ngroups;
+        best_c = c;
+      }
     }
   }
+  if ((rc = set_groups(b, best_g))) return rc;
   b->blocks_per_cu = best_c;
   drop_graph(b);
   if (out) {
     out->blocks_per_cu = best_c;
     out->ms = best;
-    out->candidates = (uint32_t)(sizeof(cand) / sizeof(cand[0]));
-    out->pad = 0;
+    out->candidates = tried;
+    out->groups = best_g;
   }
   return CT_OK;
 }

@@ -230,7 +230,7 @@ class AdjointDesc(C.Structure):
 
 
 class BatchTuning(C.Structure):
-    _fields_ = [("blocks_per_cu", u32), ("ms", f32), ("candidates", u32), ("pad", u32)]
+    _fields_ = [("blocks_per_cu", u32), ("ms", f32), ("candidates", u32), ("groups", u32)]
 
 
 class SceneDesc(C.Structure):

@@ -566,8 +566,8 @@ int ct_frame_get_stats(ct_ctx *ctx, ct_frame_stats *out);
  * A batch binds n (context, accel) pairs: views of one or several scenes,
  * each view with its own context (its frame history) and resolution.
  * ct_batch_render runs frame params[i] of every view with one launch per
- * pass over all views' pixels (the passes and results of ct_frame_render,
- * view by view).  With use_graph the launch sequence is captured once into
+ * pass over the pixels of each group of views (the passes and results of
+ * ct_frame_render, view by view).  With use_graph the launch sequence is captured once into
  * a hipGraph and replayed; it is captured again when resolutions, path
  * depth, neighbour count or the tuned grid change.  Views must differ in
  * context; contexts and accels must share one device; a context of a
@@ -583,14 +583,18 @@ int ct_batch_render(ct_batch *batch, const ct_frame_params *params, int use_grap
  * ms_total = HIP-event time of the whole batch (other times are 0). */
 int ct_batch_get_stats(ct_batch *batch, ct_frame_stats *out);
 /* Launch-shape autotuner: renders the given frames `repeats` times for
- * each candidate grid of the persistent traversal kernels (blocks per CU,
- * 2 to 5; a grid never exceeds the blocks that can be resident at once)
- * and keeps the fastest; reports the choice. */
+ * each candidate shape and keeps the fastest; reports the choice. A shape
+ * is the number of view groups (1 to 4: contiguous ranges of views of
+ * about equal pixel count, each with its own queues, run on concurrent
+ * streams so one group's traversal tail overlaps another's passes) and
+ * the grid of the persistent traversal kernels (4 or 5 blocks per CU; a
+ * grid never exceeds the blocks that can be resident at once). A new
+ * batch starts with 2 groups and 5 blocks per CU. */
 typedef struct ct_batch_tuning {
   uint32_t blocks_per_cu;
   float ms;            /* mean batch time of the choice */
   uint32_t candidates; /* number of shapes tried */
-  uint32_t pad;
+  uint32_t groups;     /* view groups of the choice */
 } ct_batch_tuning;
 int ct_batch_autotune(ct_batch *batch, const ct_frame_params *params, uint32_t repeats,
                       ct_batch_tuning *out);

@@ -70,9 +70,25 @@ def test_batch_equals_per_view(gpu, use_graph):
 
 
 def test_batch_autotune_keeps_results(gpu):
+    """The tuner tries 1-4 view groups (concurrent streams) x 4-5 blocks/CU;
+    with its choice, a batch still gives each view's bits."""
     scenes, views, built = _views()
     ctxs = [_ctx(*built[k], w) for k, w in views]
+    plain = [_ctx(*built[k], w) for k, w in views]
     b = ctamd.Batch(ctxs, [built[k][1] for k, _ in views])
-    params = [ctamd.frame_params(w, 0) for _, w in views]
-    t = b.autotune(params, repeats=1)
-    assert t.blocks_per_cu in (2, 3, 4, 5) and t.ms > 0 and t.candidates == 4
+    p0 = [ctamd.frame_params(w, 0) for _, w in views]
+    t = b.autotune(p0, repeats=1)
+    assert t.blocks_per_cu in (4, 5) and t.groups in (1, 2, 3, 4) and t.ms > 0 and t.candidates == 8
+    # the tuner rendered frame 0 twice per candidate; give the plain views the same history
+    for _ in range(2 * t.candidates):
+        for (k, w), ctx, p in zip(views, plain, p0):
+            ctx.render(built[k][1], p)
+    for f in (1, 2):
+        ps = [ctamd.frame_params(w, f) for _, w in views]
+        b.render(ps, use_graph=True)
+        for (k, w), ctx, p in zip(views, plain, ps):
+            ctx.render(built[k][1], p)
+        for (k, w), pc, bc in zip(views, plain, ctxs):
+            a, c = _read(pc, w), _read(bc, w)
+            for key in a:
+                assert np.array_equal(a[key], c[key]), (f, t.groups, w.name, key)
