@@ -224,6 +224,15 @@ struct Ctx {
   uint32_t blk_counts[3] = {0, 0, 0};
   const void *lights_accel = nullptr;  // accel and build whose cube lights were uploaded
   uint32_t lights_generation = 0;
+  // A frame can run as two halves of the image on two streams: the second
+  // half has its own ray and segment queues, counters and frame record.
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_half[3] = {};
+  DevBuf h_queue, h_counters, h_ptq[2], h_seg_o, h_seg_d, h_seg_hit, frame_dev2, blk_dev_a, blk_dev2;
+  uint32_t split_key[5] = {0, 0, 0, 0, 0};
+  uint32_t split_counts[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  uint32_t split_px[2] = {0, 0};
+  bool split = false;  // the last frame ran as two halves
 };
 
 // Kernel-side views used by trace.hip and frame.hip.

@@ -1768,11 +1768,19 @@ void launch_any(const Launch &L, FrameSet fs, uint32_t max_rays, hipStream_t st)
 
 // The command list of App::draw (main.cpp:1436-1703) for every frame of the
 // set: the same kernels, each launched once over all frames.
+// `phases` selects parts of the sequence: 1 = primary rays to the path's
+// temporal reuse (every read of the previous frame's buffers), 2 = spatial
+// reuse and spatial path reuse (which rewrite those buffers and read
+// neighbours' phase-1 results), 4 = TAA. Run in order on one stream they
+// are the whole frame; ct_frame_render also runs phases 1 and 2 of two
+// halves of the image on two streams, joined in between.
+constexpr unsigned kPhaseFirst = 1u, kPhaseSpatial = 2u, kPhaseTaa = 4u, kPhasesAll = 7u;
 template <class Mark, class SegMark>
-int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark) {
+int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark, unsigned phases = kPhasesAll) {
   int rc;
   const FrameSet tiles{L.fb, L.blk}, pix{L.fb, L.blk + L.blocks[0]}, taa{L.fb, L.blk + L.blocks[0] + L.blocks[1]};
   const uint32_t np = L.total_px, nb = L.blocks[1];
+  if (phases & kPhaseFirst) {
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, 0, 8);
   if (L.stats) hipLaunchKernelGGL(k_reset_u64, dim3(1), dim3(64), 0, st, L.stats, 12);
   if ((rc = mark())) return rc;  // kMarkStart
@@ -1820,6 +1828,8 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark)
   if ((rc = mark())) return rc;  // kMarkTracePt
   if (L.pt_depth) hipLaunchKernelGGL(k_pt_end<kVisFinish>, dim3(nb), dim3(kBlock), 0, st, pix);
   if ((rc = mark())) return rc;  // kMarkPt
+  }
+  if (phases & kPhaseSpatial) {
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntQueue, 1);
   hipLaunchKernelGGL(k_spatial<true>, dim3(nb), dim3(kBlock), 0, st, pix);
   if ((rc = mark())) return rc;  // kMarkSpatialEmit
@@ -1834,10 +1844,46 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark)
       hipLaunchKernelGGL(k_pt_spatial<false>, dim3(nb), dim3(kBlock), 0, st, pix, L.stats);
   }
   if ((rc = mark())) return rc;  // kMarkPtSpatial
-  if (L.blocks[2]) hipLaunchKernelGGL(k_taa, dim3(L.blocks[2]), dim3(kBlock), 0, st, taa);
-  if ((rc = mark())) return rc;  // kMarkTaa
+  }
+  if (phases & kPhaseTaa) {
+    if (L.blocks[2]) hipLaunchKernelGGL(k_taa, dim3(L.blocks[2]), dim3(kBlock), 0, st, taa);
+    if ((rc = mark())) return rc;  // kMarkTaa
+  }
   CT_HIP(hipGetLastError());
   return CT_OK;
+}
+
+// A frame of at least kSplitMinPixels runs as two halves on two streams:
+// the even and the odd rows of 16-pixel tiles, so both halves hold the same
+// mix of the image. Each tile row must be a whole number of 256-pixel
+// blocks (width a multiple of 16) so that no pixel block spans two rows.
+constexpr uint32_t kSplitMinPixels = 1u << 20;
+bool can_split(const DFrame &F) {
+  return F.npix >= kSplitMinPixels && F.W % 16u == 0u && (F.H + 15) / 16 >= 2u;
+}
+
+// Block tables of the two halves: even tile rows' tiles and pixels plus the
+// whole frame's TAA blocks, then odd tile rows' tiles and pixels.
+void split_tables(const DFrame &F, std::vector<uint2> &a, std::vector<uint2> &b, uint32_t ca[3], uint32_t cb[3],
+                  uint32_t px[2]) {
+  const uint32_t tx = F.W / 16, ty = (F.H + 15) / 16, row_px = 16u * F.W;
+  a.clear();
+  b.clear();
+  px[0] = px[1] = 0u;
+  for (uint32_t r = 0; r < ty; ++r)
+    for (uint32_t t = r * tx; t < (r + 1) * tx; ++t) (r & 1u ? b : a).push_back(make_uint2(0u, t));
+  ca[0] = (uint32_t)a.size();
+  cb[0] = (uint32_t)b.size();
+  for (uint32_t r = 0; r < ty; ++r) {
+    const uint32_t q1 = std::min(F.npix, (r + 1) * row_px);
+    for (uint32_t q = r * row_px; q < q1; q += kBlock) (r & 1u ? b : a).push_back(make_uint2(0u, q));
+    px[r & 1u] += q1 - r * row_px;
+  }
+  ca[1] = (uint32_t)a.size() - ca[0];
+  cb[1] = (uint32_t)b.size() - cb[0];
+  for (uint32_t q = 0; q < F.taa_w * F.taa_h; q += kBlock) a.push_back(make_uint2(0u, q));
+  ca[2] = (uint32_t)a.size() - ca[0] - ca[1];
+  cb[2] = 0u;
 }
 
 }  // namespace
@@ -1882,6 +1928,12 @@ int ct_ctx_destroy(ct_ctx *c) {
   for (int k = 0; k < 48; ++k)
     if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
   if (c->counters_pinned) (void)hipHostFree(c->counters_pinned);
+  if (c->stream2) {
+    (void)hipStreamSynchronize(c->stream2);
+    (void)hipStreamDestroy(c->stream2);
+  }
+  for (hipEvent_t e : {c->ev_fork, c->ev_half[0], c->ev_half[1], c->ev_half[2]})
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return CT_OK;
@@ -2316,48 +2368,131 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
   if ((rc = prepare_frame(c, a, p, st, F))) return rc;
   FrameBufs &b = c->fb;
   unsigned long long *stats = (c->instrument & 2u) ? b.prev_res_copy.as<unsigned long long>() : nullptr;
-  // the frame record and its block tables on the device
   if ((rc = c->frame_dev.reserve(sizeof(DFrame)))) return rc;
-  const uint32_t key[4] = {F.W, F.H, F.taa_w, F.taa_h};
-  if (std::memcmp(key, c->blk_key, sizeof(key)) != 0) {
-    std::vector<uint2> tbl;
-    build_tables(&F, 1, tbl, c->blk_counts);
-    if ((rc = c->blk_dev.reserve(sizeof(uint2) * tbl.size()))) return rc;
-    CT_HIP(hipStreamSynchronize(st));
-    CT_HIP(hipMemcpy(c->blk_dev.p, tbl.data(), sizeof(uint2) * tbl.size(), hipMemcpyHostToDevice));
-    std::memcpy(c->blk_key, key, sizeof(key));
-  }
-  hipLaunchKernelGGL(k_store_frame, dim3(1), dim3(64), 0, st, F, c->frame_dev.as<DFrame>());
-  Launch L;
-  L.fb = c->frame_dev.as<DFrame>();
-  L.blk = c->blk_dev.as<uint2>();
-  std::memcpy(L.blocks, c->blk_counts, sizeof(L.blocks));
-  L.total_px = F.npix;
-  L.pt_depth = F.rp.indirect ? F.rp.max_depth : 0u;
-  L.max_neighbors = F.rp.max_neighbors;
-  L.trace_blocks = (uint32_t)trace_grid();
-  L.multi = false;
-  L.counters = F.counters;
-  L.stats = stats;
-  const bool ev = (c->instrument & 1u) != 0;
-  int ne = 0, nseg = kMarkCount;
-  c->n_seg = 0;
-  auto mark = [&]() -> int {
-    if (ev) CT_HIP(hipEventRecord(c->ev[ne++], st));
-    return CT_OK;
-  };
-  auto seg_mark = [&]() -> int {
-    if (ev && nseg < 48 && c->n_seg < 32) {
-      CT_HIP(hipEventRecord(c->ev[nseg], st));
-      c->ev_seg[c->n_seg++] = nseg++;
+  auto none = []() -> int { return CT_OK; };
+  // Per-pass events and traversal counters describe one sequence: those
+  // frames run whole on one stream.
+  c->split = (c->instrument & 3u) == 0u && can_split(F);
+  if (c->split) {
+    // Two halves of the image, each with its own queues, on `st` and on
+    // stream2: phase 1 of both, a join, phase 2 of both, a join, then TAA.
+    const size_t np = (size_t)c->max_w * c->max_h;
+    if (!c->stream2) {
+      if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+          hipEventCreateWithFlags(&c->ev_half[0], hipEventDisableTiming) != hipSuccess ||
+          hipEventCreateWithFlags(&c->ev_half[1], hipEventDisableTiming) != hipSuccess ||
+          hipEventCreateWithFlags(&c->ev_half[2], hipEventDisableTiming) != hipSuccess)
+        return fail(CT_E_HIP, "ct_frame_render: stream creation failed");
     }
-    return CT_OK;
-  };
-  if ((rc = enqueue_frames(L, st, mark, seg_mark))) return rc;
-  c->n_events = ne;
-  // Counters for the statistics (copied once the frame is done).
-  CT_HIP(hipMemcpyAsync(c->counters_pinned, F.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  if (stats) CT_HIP(hipMemcpyAsync(c->counters_pinned + 8, stats, 12 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    if ((rc = c->h_queue.reserve(4 * np * kRaySlots)) || (rc = c->h_counters.reserve(64 * sizeof(uint32_t))) ||
+        (rc = c->h_ptq[0].reserve(4 * np)) || (rc = c->h_ptq[1].reserve(4 * np)) ||
+        (rc = c->h_seg_o.reserve(16 * np)) || (rc = c->h_seg_d.reserve(16 * np)) ||
+        (rc = c->h_seg_hit.reserve(16 * np)) || (rc = c->frame_dev2.reserve(sizeof(DFrame))))
+      return rc;
+    DFrame FB = F;
+    FB.queue = c->h_queue.as<uint32_t>();
+    FB.counters = c->h_counters.as<uint32_t>();
+    FB.ptq[0] = c->h_ptq[0].as<uint32_t>();
+    FB.ptq[1] = c->h_ptq[1].as<uint32_t>();
+    FB.seg_o = c->h_seg_o.as<float4>();
+    FB.seg_d = c->h_seg_d.as<float4>();
+    FB.seg_hit = c->h_seg_hit.as<float4>();
+    const uint32_t key[5] = {F.W, F.H, F.taa_w, F.taa_h, 1u};
+    if (std::memcmp(key, c->split_key, sizeof(key)) != 0) {
+      std::vector<uint2> ta, tb;
+      split_tables(F, ta, tb, c->split_counts[0], c->split_counts[1], c->split_px);
+      if ((rc = c->blk_dev_a.reserve(sizeof(uint2) * ta.size())) || (rc = c->blk_dev2.reserve(sizeof(uint2) * tb.size())))
+        return rc;
+      CT_HIP(hipStreamSynchronize(st));
+      CT_HIP(hipStreamSynchronize(c->stream2));
+      CT_HIP(hipMemcpy(c->blk_dev_a.p, ta.data(), sizeof(uint2) * ta.size(), hipMemcpyHostToDevice));
+      CT_HIP(hipMemcpy(c->blk_dev2.p, tb.data(), sizeof(uint2) * tb.size(), hipMemcpyHostToDevice));
+      std::memcpy(c->split_key, key, sizeof(key));
+    }
+    hipLaunchKernelGGL(k_store_frame, dim3(1), dim3(64), 0, st, F, c->frame_dev.as<DFrame>());
+    hipLaunchKernelGGL(k_store_frame, dim3(1), dim3(64), 0, st, FB, c->frame_dev2.as<DFrame>());
+    Launch L[2];
+    for (int h = 0; h < 2; ++h) {
+      L[h].fb = h ? c->frame_dev2.as<DFrame>() : c->frame_dev.as<DFrame>();
+      L[h].blk = h ? c->blk_dev2.as<uint2>() : c->blk_dev_a.as<uint2>();
+      std::memcpy(L[h].blocks, c->split_counts[h], sizeof(L[h].blocks));
+      L[h].total_px = c->split_px[h];
+      L[h].pt_depth = F.rp.indirect ? F.rp.max_depth : 0u;
+      L[h].max_neighbors = F.rp.max_neighbors;
+      L[h].trace_blocks = (uint32_t)trace_grid();
+      L[h].multi = false;
+      L[h].counters = h ? FB.counters : F.counters;
+      L[h].stats = nullptr;
+    }
+    hipStream_t s2 = c->stream2;
+    if (c->instrument & 4u) CT_HIP(hipEventRecord(c->ev[kMarkStart], st));
+    CT_HIP(hipEventRecord(c->ev_fork, st));
+    CT_HIP(hipStreamWaitEvent(s2, c->ev_fork, 0));
+    if ((rc = enqueue_frames(L[0], st, none, none, kPhaseFirst)) ||
+        (rc = enqueue_frames(L[1], s2, none, none, kPhaseFirst)))
+      return rc;
+    CT_HIP(hipEventRecord(c->ev_half[0], st));
+    CT_HIP(hipEventRecord(c->ev_half[1], s2));
+    CT_HIP(hipStreamWaitEvent(st, c->ev_half[1], 0));
+    CT_HIP(hipStreamWaitEvent(s2, c->ev_half[0], 0));
+    if ((rc = enqueue_frames(L[0], st, none, none, kPhaseSpatial)) ||
+        (rc = enqueue_frames(L[1], s2, none, none, kPhaseSpatial)))
+      return rc;
+    CT_HIP(hipEventRecord(c->ev_half[2], s2));
+    CT_HIP(hipStreamWaitEvent(st, c->ev_half[2], 0));
+    if ((rc = enqueue_frames(L[0], st, none, none, kPhaseTaa))) return rc;
+    if (c->instrument & 4u) CT_HIP(hipEventRecord(c->ev[kMarkTaa], st));
+    c->n_events = 0;
+    c->n_seg = 0;
+    CT_HIP(hipMemcpyAsync(c->counters_pinned, F.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    CT_HIP(hipMemcpyAsync(c->counters_pinned + 32, FB.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  } else {
+    // the frame record and its block tables on the device
+    const uint32_t key[4] = {F.W, F.H, F.taa_w, F.taa_h};
+    if (std::memcmp(key, c->blk_key, sizeof(key)) != 0) {
+      std::vector<uint2> tbl;
+      build_tables(&F, 1, tbl, c->blk_counts);
+      if ((rc = c->blk_dev.reserve(sizeof(uint2) * tbl.size()))) return rc;
+      CT_HIP(hipStreamSynchronize(st));
+      CT_HIP(hipMemcpy(c->blk_dev.p, tbl.data(), sizeof(uint2) * tbl.size(), hipMemcpyHostToDevice));
+      std::memcpy(c->blk_key, key, sizeof(key));
+    }
+    hipLaunchKernelGGL(k_store_frame, dim3(1), dim3(64), 0, st, F, c->frame_dev.as<DFrame>());
+    Launch L;
+    L.fb = c->frame_dev.as<DFrame>();
+    L.blk = c->blk_dev.as<uint2>();
+    std::memcpy(L.blocks, c->blk_counts, sizeof(L.blocks));
+    L.total_px = F.npix;
+    L.pt_depth = F.rp.indirect ? F.rp.max_depth : 0u;
+    L.max_neighbors = F.rp.max_neighbors;
+    L.trace_blocks = (uint32_t)trace_grid();
+    L.multi = false;
+    L.counters = F.counters;
+    L.stats = stats;
+    const bool ev = (c->instrument & 1u) != 0;
+    int ne = 0, nseg = kMarkCount;
+    c->n_seg = 0;
+    auto mark = [&]() -> int {
+      if (ev) CT_HIP(hipEventRecord(c->ev[ne++], st));
+      return CT_OK;
+    };
+    auto seg_mark = [&]() -> int {
+      if (ev && nseg < 48 && c->n_seg < 32) {
+        CT_HIP(hipEventRecord(c->ev[nseg], st));
+        c->ev_seg[c->n_seg++] = nseg++;
+      }
+      return CT_OK;
+    };
+    if (!ev && (c->instrument & 4u)) CT_HIP(hipEventRecord(c->ev[kMarkStart], st));
+    if ((rc = enqueue_frames(L, st, mark, seg_mark))) return rc;
+    if (!ev && (c->instrument & 4u)) CT_HIP(hipEventRecord(c->ev[kMarkTaa], st));
+    c->n_events = ne;
+    // Counters for the statistics (copied once the frame is done).
+    CT_HIP(hipMemcpyAsync(c->counters_pinned, F.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    if (stats)
+      CT_HIP(hipMemcpyAsync(c->counters_pinned + 8, stats, 12 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  }
   c->stats_pending = true;
   c->stats = ct_frame_stats{};
   c->stats.primary_rays = F.npix;
@@ -2432,12 +2567,20 @@ int ct_frame_get_stats(ct_ctx *c, ct_frame_stats *out) {
   if (c->stats_pending) {
     CT_HIP(hipStreamSynchronize(c->stream_last));
     const uint32_t *cnt = reinterpret_cast<const uint32_t *>(c->counters_pinned);
-    c->stats.shadow_rays_ris = cnt[kCntRis];
-    c->stats.shadow_rays_temporal = cnt[kCntTemporal];
-    c->stats.shadow_rays_spatial = cnt[kCntSpatial];
-    c->stats.shadow_rays_pt = cnt[kCntPtShadow];
-    c->stats.bounce_rays = cnt[kCntPtBounce];
-    c->stats.shadow_rays = (uint64_t)cnt[kCntRis] + cnt[kCntTemporal] + cnt[kCntSpatial] + cnt[kCntPtShadow];
+    for (int h = 0; h < (c->split ? 2 : 1); ++h, cnt += 64) {  // the lower half's counters sit 256 B on
+      c->stats.shadow_rays_ris += cnt[kCntRis];
+      c->stats.shadow_rays_temporal += cnt[kCntTemporal];
+      c->stats.shadow_rays_spatial += cnt[kCntSpatial];
+      c->stats.shadow_rays_pt += cnt[kCntPtShadow];
+      c->stats.bounce_rays += cnt[kCntPtBounce];
+    }
+    c->stats.shadow_rays = c->stats.shadow_rays_ris + c->stats.shadow_rays_temporal + c->stats.shadow_rays_spatial +
+                           c->stats.shadow_rays_pt;
+    if ((c->instrument & 5u) == 4u) {
+      float v = 0.0f;
+      (void)hipEventElapsedTime(&v, c->ev[kMarkStart], c->ev[kMarkTaa]);
+      c->stats.ms_total = v;
+    }
     if (c->instrument & 2u) {
       const unsigned long long *s = reinterpret_cast<const unsigned long long *>(c->counters_pinned + 8);
       c->stats.nodes = s[0];

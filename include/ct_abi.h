@@ -507,7 +507,14 @@ typedef struct ct_frame_params {
   ct_restir_params restir;
 } ct_frame_params;
 
-/* Record the frame's passes on `stream`. */
+/* Record the frame's passes on `stream`. A frame of at least 2^20 pixels
+ * whose width is a multiple of 16 runs as two halves of the image, the
+ * even and the odd rows of 16-pixel tiles: the passes up to temporal path
+ * reuse, then the spatial passes, run for both halves at once, the second
+ * half on a stream of the context that forks from and joins back into
+ * `stream`; TAA follows on `stream`. Per-pass events or traversal counters
+ * (ct_ctx_set_instrumentation 1 or 2) run the frame whole on `stream`.
+ * Results are those of the whole-frame sequence either way. */
 int ct_frame_render(ct_ctx *ctx, ct_accel *accel, const ct_frame_params *params,
                     void *stream);
 
@@ -555,7 +562,9 @@ typedef struct ct_frame_stats {
   float ms_pt_segments;     /* sum over the path-segment traversal launches */
   float pad;
 } ct_frame_stats;
-/* Enable per-pass HIP events (1) and traversal node/leaf counters (2). */
+/* Enable per-pass HIP events (1, frames then run whole on one stream),
+ * traversal node/leaf counters (2, likewise), or only the frame's start
+ * and end events for ms_total (4). */
 int ct_ctx_set_instrumentation(ct_ctx *ctx, uint32_t flags);
 int ct_frame_get_stats(ct_ctx *ctx, ct_frame_stats *out);
 

@@ -161,3 +161,31 @@ def _pt_workload(max_depth=3, radius=1, shift=ctamd.CT_SHIFT_RECONNECTION, rr=0,
                               "no_spatial"])
 def test_frame_parity_pt_variants(gpu, kw):
     _frames_match(_pt_workload(**kw), 3)
+
+
+def test_split_frame_equals_whole_frame(gpu):
+    """A frame of 2^20 pixels or more runs as two halves (even and odd tile
+    rows) on two streams by default; it gives the bits of the same frame run
+    whole on one stream (per-pass events on). The height leaves a partial
+    last tile row."""
+    w = ctamd.hd_di(width=1024, height=1032, grid=64, chunk=32, frames=4, indirect=True, max_depth=2)
+    sc, acc, whole = gpu_scene(w)
+    split = ctamd.Ctx(w.width, w.height)
+    ctamd.bind_scene(sc, split)
+    whole.set_instrumentation(1)
+    for f in range(4):
+        p = ctamd.frame_params(w, f)
+        whole.render(acc, p)
+        split.render(acc, p)
+        for which in (ctamd.CT_BUF_IMAGE, ctamd.CT_BUF_TAA_FRAME, ctamd.CT_BUF_INDIRECT):
+            assert np.array_equal(bits(whole.read(which, w.width, w.height)),
+                                  bits(split.read(which, w.width, w.height))), (f, which)
+        for which in (ctamd.CT_BUF_DI, ctamd.CT_BUF_PREV_DI):
+            assert np.array_equal(di_array(whole.read(which, w.width, w.height)),
+                                  di_array(split.read(which, w.width, w.height))), (f, which)
+        for which in (ctamd.CT_BUF_PATH_RESERVOIR, ctamd.CT_BUF_TEMPORAL_PATH_RESERVOIR):
+            assert np.array_equal(pres_array(whole.read(which, w.width, w.height)),
+                                  pres_array(split.read(which, w.width, w.height))), (f, which)
+        a, b = whole.stats(), split.stats()
+        assert (a.shadow_rays, a.bounce_rays, a.shadow_rays_pt, a.primary_rays) == \
+            (b.shadow_rays, b.bounce_rays, b.shadow_rays_pt, b.primary_rays), f
