@@ -1902,18 +1902,13 @@ int ct_ctx_create(const ct_ctx_desc *d, ct_ctx **out) {
   c->max_w = d->max_width;
   c->max_h = d->max_height;
   int rc;
-  // The two streams of a split frame, created together so that HIP gives
-  // them different hardware queues (it hands queues out in turn).
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
   if (e != hipSuccess) {
-    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return fail(CT_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
   if ((rc = alloc_frame(*c)) || (rc = zero_frame(*c, c->stream))) {
     (void)hipStreamDestroy(c->stream);
-    (void)hipStreamDestroy(c->stream2);
     delete c;
     return rc;
   }
@@ -2382,8 +2377,9 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
     // Two halves of the image, each with its own queues, on `st` and on
     // stream2: phase 1 of both, a join, phase 2 of both, a join, then TAA.
     const size_t np = (size_t)c->max_w * c->max_h;
-    if (!c->ev_fork) {
-      if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+    if (!c->stream2) {  // only contexts that split hold a second stream
+      if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
           hipEventCreateWithFlags(&c->ev_half[0], hipEventDisableTiming) != hipSuccess ||
           hipEventCreateWithFlags(&c->ev_half[1], hipEventDisableTiming) != hipSuccess ||
           hipEventCreateWithFlags(&c->ev_half[2], hipEventDisableTiming) != hipSuccess)
