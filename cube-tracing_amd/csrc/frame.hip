@@ -1366,6 +1366,11 @@ __global__ void k_reset_counters(uint32_t *c, int first, int n) {
   int i = threadIdx.x;
   if (i < n) c[first + i] = 0u;
 }
+// Zeroes counter i for every bit i of `mask` (counters 0..31).
+__global__ void k_reset_mask(uint32_t *c, uint32_t mask) {
+  const uint32_t i = threadIdx.x;
+  if (i < 32u && ((mask >> i) & 1u)) c[i] = 0u;
+}
 __global__ void k_reset_u64(unsigned long long *c, int n) {
   int i = threadIdx.x;
   if (i < n) c[i] = 0ull;
@@ -1754,11 +1759,12 @@ struct Launch {
 
 template <bool kCount, bool kMulti>
 void launch_any_frames(const Launch &L, FrameSet fs, uint32_t max_rays, hipStream_t st) {
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntHead, 1);
   const uint32_t grid = std::min<uint32_t>(std::min<uint32_t>(std::max<uint32_t>(nblocks(max_rays), 1), L.trace_blocks),
                                            any_resident<kCount, kMulti>());
   hipLaunchKernelGGL((k_trace_any_frames<kCount, kMulti>), dim3(grid), dim3(kBlock), 0, st, fs, max_rays, L.stats);
 }
+// The queue length and the fetch head (kCntHead) are zeroed by the caller
+// before the pass that fills the queue.
 void launch_any(const Launch &L, FrameSet fs, uint32_t max_rays, hipStream_t st) {
   if (L.stats)
     L.multi ? launch_any_frames<true, true>(L, fs, max_rays, st) : launch_any_frames<true, false>(L, fs, max_rays, st);
@@ -1781,6 +1787,7 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark,
   const FrameSet tiles{L.fb, L.blk}, pix{L.fb, L.blk + L.blocks[0]}, taa{L.fb, L.blk + L.blocks[0] + L.blocks[1]};
   const uint32_t np = L.total_px, nb = L.blocks[1];
   if (phases & kPhaseFirst) {
+  // queue, fetch head and the ray statistics
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, 0, 8);
   if (L.stats) hipLaunchKernelGGL(k_reset_u64, dim3(1), dim3(64), 0, st, L.stats, 12);
   if ((rc = mark())) return rc;  // kMarkStart
@@ -1796,15 +1803,20 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark,
   hipLaunchKernelGGL(k_temporal_finish, dim3(nb), dim3(kBlock), 0, st, pix);
   if ((rc = mark())) return rc;  // kMarkTemporalFinish
   if (L.pt_depth) {
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntPtQueue, 3);
+    // both segment queues and their head, and the ray queue and head that
+    // path-reuse visibility uses after the path loop
+    hipLaunchKernelGGL(k_reset_mask, dim3(1), dim3(64), 0, st, L.counters,
+                       (1u << kCntPtQueue) | (1u << (kCntPtQueue + 1)) | (1u << kCntPtHead) | (1u << kCntQueue) |
+                           (1u << kCntHead));
     hipLaunchKernelGGL(k_pt_begin, dim3(nb), dim3(kBlock), 0, st, pix);
     const uint32_t grid = std::min<uint32_t>(std::min<uint32_t>(std::max<uint32_t>(nblocks(np), 1), L.trace_blocks),
                                              L.stats ? (L.multi ? segments_resident<true, true>() : segments_resident<true, false>())
                                                      : (L.multi ? segments_resident<false, true>() : segments_resident<false, false>()));
     for (uint32_t d = 0; d < L.pt_depth; ++d) {
-      // the next queue and the fetch counter start at zero
-      hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntPtQueue + (int)((d + 1) & 1u), 1);
-      hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntPtHead, 1);
+      // the next queue and the fetch counter start at zero (already so at depth 0)
+      if (d > 0)
+        hipLaunchKernelGGL(k_reset_mask, dim3(1), dim3(64), 0, st, L.counters,
+                           (1u << (kCntPtQueue + ((d + 1) & 1u))) | (1u << kCntPtHead));
       if ((rc = seg_mark())) return rc;
       if (L.stats && L.multi)
         hipLaunchKernelGGL((k_trace_segments<true, true>), dim3(grid), dim3(kBlock), 0, st, pix, d, L.stats + 6);
@@ -1820,7 +1832,6 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark,
       else
         hipLaunchKernelGGL(k_pt_step<false>, dim3(nblocks(np)), dim3(kBlock), 0, st, pix, d);
     }
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntQueue, 1);
     hipLaunchKernelGGL(k_pt_end<kVisEmit>, dim3(nb), dim3(kBlock), 0, st, pix);
   }
   if ((rc = mark())) return rc;  // kMarkPtEmit
@@ -1830,7 +1841,7 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark,
   if ((rc = mark())) return rc;  // kMarkPt
   }
   if (phases & kPhaseSpatial) {
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, kCntQueue, 1);
+  hipLaunchKernelGGL(k_reset_mask, dim3(1), dim3(64), 0, st, L.counters, (1u << kCntQueue) | (1u << kCntHead));
   hipLaunchKernelGGL(k_spatial<true>, dim3(nb), dim3(kBlock), 0, st, pix);
   if ((rc = mark())) return rc;  // kMarkSpatialEmit
   launch_any(L, pix, 2 * L.max_neighbors * np, st);
