@@ -41,8 +41,16 @@ enum {
 };
 
 // counters[] slots
+// kCntQueue counts the far rays at the front of the ray queue, kCntQueueNear
+// the near rays stored from its back end down (see queue_flush).
 constexpr int kCntQueue = 0, kCntHead = 1, kCntRis = 2, kCntTemporal = 3, kCntSpatial = 4, kCntPtBounce = 5,
-              kCntPtShadow = 6, kCntPtQueue = 8 /* and 9 */, kCntPtHead = 10;
+              kCntPtShadow = 6, kCntQueueNear = 7, kCntPtQueue = 8 /* and 9 */, kCntPtHead = 10;
+#ifndef CT_FAR_RAY
+#define CT_FAR_RAY 2.0f
+#endif
+// A shadow ray at least this long (world units; the 256^3 workload is 8
+// wide) goes to the front of the queue and is traced first.
+constexpr float kFarRay = CT_FAR_RAY;
 
 struct DFrame {
   uint32_t W, H, npix, frame, active, afternoon;
@@ -77,7 +85,8 @@ struct DFrame {
   uint32_t *mask;
   float4 *ray_o, *ray_d;
   uint32_t *ray_t;
-  uint32_t *queue;
+  uint32_t *queue;   // [queue_cap]: far rays from the front, near rays from the back
+  uint32_t queue_cap;
   uint32_t *counters;
   float4 *gat;  // [8 per pixel] neighbour record, see st_gather
   float4 *pres_o;    // [6 per pixel] output path reservoirs (this frame)
@@ -248,7 +257,7 @@ CT_D bool update_reservoir(Res &r, uint32_t X, uint32_t type, uint32_t rs, F3 Fv
 // `queries` counts the calls, made or taken from bits.
 struct Vis {
   bool emit;
-  uint32_t px, slot, bits, emitted, queries;
+  uint32_t px, slot, bits, emitted, queries, far;
 };
 CT_D bool occluded(const DFrame &F, Vis &v, F3 o, F3 d, float dist, uint32_t ti, uint32_t tp) {
   ++v.queries;
@@ -260,6 +269,7 @@ CT_D bool occluded(const DFrame &F, Vis &v, F3 o, F3 d, float dist, uint32_t ti,
     F.ray_d[idx] = make_float4(d.x, d.y, d.z, __uint_as_float(ti));
     F.ray_t[idx] = tp;
     v.emitted |= 1u << k;
+    if (dist >= kFarRay) v.far |= 1u << k;
     return false;
   }
   return (v.bits >> k) & 1u;
@@ -795,13 +805,19 @@ CT_D void wave_add_cnt(const TraceCnt &c, unsigned long long *dst) {
 
 // ----------------------------------------------------------- queue push
 // Appends this thread's emitted slots (bit k -> ray pixel*16 + k) to the
-// ray queue with one atomic per block and adds their number to counter
-// `counter` (none when negative).  Every thread of the block calls it.
-CT_D void queue_flush(const DFrame &F, uint32_t px, uint32_t emitted, int counter) {
+// ray queue with two atomics per block and adds their number to counter
+// `counter` (none when negative). Slots in `far` (rays of at least kFarRay)
+// fill the queue from the front (count kCntQueue), the others from the back
+// end down (count kCntQueueNear): the traversal takes far rays first, so
+// the long walks start early and waves hold rays of like length. Every
+// thread of the block calls it.
+CT_D void queue_flush(const DFrame &F, uint32_t px, uint32_t emitted, uint32_t far, int counter) {
   __shared__ uint32_t s_wsum[kBlock / 64];
-  __shared__ uint32_t s_base;
+  __shared__ uint32_t s_base_far, s_base_near;
   const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
-  const uint32_t c = (uint32_t)__popc(emitted);
+  far &= emitted;
+  const uint32_t nf = (uint32_t)__popc(far), nn = (uint32_t)__popc(emitted) - nf;
+  const uint32_t c = nf | (nn << 16);  // both counts in one word (at most 4096 per block)
   uint32_t incl = c;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
@@ -817,15 +833,19 @@ CT_D void queue_flush(const DFrame &F, uint32_t px, uint32_t emitted, int counte
       s_wsum[i] = tot;
       tot += t;
     }
-    s_base = tot ? atomicAdd(&F.counters[kCntQueue], tot) : 0u;
-    if (tot && counter >= 0) atomicAdd(&F.counters[counter], tot);
+    const uint32_t tf = tot & 0xFFFFu, tn = tot >> 16;
+    s_base_far = tf ? atomicAdd(&F.counters[kCntQueue], tf) : 0u;
+    s_base_near = tn ? atomicAdd(&F.counters[kCntQueueNear], tn) : 0u;
+    if ((tf + tn) && counter >= 0) atomicAdd(&F.counters[counter], tf + tn);
   }
   __syncthreads();
-  uint32_t pos = s_base + s_wsum[w] + incl - c;
+  const uint32_t before = s_wsum[w] + incl - c;
+  uint32_t pf = s_base_far + (before & 0xFFFFu), pn = s_base_near + (before >> 16);
   while (emitted) {
-    uint32_t k = (uint32_t)(__ffs(emitted) - 1);
+    const uint32_t k = (uint32_t)(__ffs(emitted) - 1);
     emitted &= emitted - 1u;
-    F.queue[pos++] = F.qtag | (px * kRaySlots + k);
+    const uint32_t pos = ((far >> k) & 1u) ? pf++ : F.queue_cap - 1u - pn++;
+    F.queue[pos] = F.qtag | (px * kRaySlots + k);
   }
 }
 
@@ -875,7 +895,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
   const uint32_t px = blk_.y + threadIdx.x;
-  uint32_t emitted = 0, ris_ray = 0;
+  uint32_t emitted = 0, far = 0, ris_ray = 0;
   if (px < F.npix) {
     const uint32_t x = px % F.W, y = px / F.W;
     float4 h = F.hit[px];
@@ -955,6 +975,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
         temporal_emit(F, rv, prev, pidx, rd, hit_from_di(di), mat, v);
       }
       emitted = v.emitted;
+      far = v.far;
       st_res(F.rA_i, F.rB_i, F.rC_i, px, r);
       F.pidx[px] = pidx;
     }
@@ -962,7 +983,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     F.mask[px] = 0u;
   }
   wave_add(ris_ray, &F.counters[kCntRis]);
-  queue_flush(F, px, emitted, -1);
+  queue_flush(F, px, emitted, far, -1);
 }
 
 // RIS visibility (slot 0) and temporal reuse (slots 1, 2) with the results
@@ -1012,7 +1033,7 @@ __global__ void __launch_bounds__(kBlock) k_spatial(FrameSet FS) {
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
   const uint32_t px = blk_.y + threadIdx.x;
-  uint32_t emitted = 0;
+  uint32_t emitted = 0, far = 0;
   if (px < F.npix) {
     const uint32_t x = px % F.W, y = px / F.W;
     float4 pd = F.dpos_c[px];
@@ -1026,6 +1047,7 @@ __global__ void __launch_bounds__(kBlock) k_spatial(FrameSet FS) {
       uint32_t seed = di.seed;
       spatial_reuse(F, sr, di.conf, x, y, ray_d, hit, mat, seed, v);
       emitted = v.emitted;
+      far = v.far;
       if (!kEmit && F.rp.indirect) {
         // The indirect term, emission and the NaN scrub follow in k_pt_spatial.
         F3 hv = f3(0, 0, 0);
@@ -1053,7 +1075,7 @@ __global__ void __launch_bounds__(kBlock) k_spatial(FrameSet FS) {
     }
     if (kEmit) F.mask[px] = 0u;
   }
-  if (kEmit) queue_flush(F, px, emitted, kCntSpatial);
+  if (kEmit) queue_flush(F, px, emitted, far, kCntSpatial);
 }
 
 // Appends this thread's item (when `active`) to a queue with one atomic per
@@ -1203,6 +1225,7 @@ __global__ void __launch_bounds__(kBlock) k_pt_end(FrameSet FS) {
   tr.slot = 0u;
   tr.bits = 0u;
   tr.emitted = 0u;
+  tr.far = 0u;
   if (px < F.npix) {
     if (F.dpos_c[px].w > 0.0f) {
       const uint32_t x = px % F.W, y = px / F.W;
@@ -1236,7 +1259,7 @@ __global__ void __launch_bounds__(kBlock) k_pt_end(FrameSet FS) {
     }
     if (kVis == kVisEmit) F.mask[px] = 0u;
   }
-  if (kVis == kVisEmit) queue_flush(F, px, tr.emitted, kCntPtShadow);
+  if (kVis == kVisEmit) queue_flush(F, px, tr.emitted, tr.far, kCntPtShadow);
 }
 
 // Spatial path reuse (indirect_illumination_spatial_reuse, rgen.glsl:427-452)
@@ -1429,13 +1452,15 @@ __global__ void __launch_bounds__(kBlock) k_trace_any_frames(FrameSet FS, uint32
   __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
   const DFrame &F0 = FS.fb[0];
   const uint32_t tid = threadIdx.x, lane = tid & 63u;
-  uint32_t count = F0.counters[kCntQueue];
+  const uint32_t nfar = F0.counters[kCntQueue];
+  uint32_t count = nfar + F0.counters[kCntQueueNear];
   if (count > max_rays) count = max_rays;
   TraceCnt cnt;
   for (RayBatches rb(count, &F0.counters[kCntHead]); rb.valid(); rb.next()) {
     const uint32_t q = rb.base + lane;
     if (q < count) {
-      const uint32_t e = F0.queue[q];
+      // far rays from the front, then near rays from the back end down
+      const uint32_t e = F0.queue[q < nfar ? q : F0.queue_cap - 1u - (q - nfar)];
       const DFrame &F = FS.fb[kMulti ? (e >> kViewShift) : 0u];
       const uint32_t idx = kMulti ? (e & kSlotMask) : e;
       const float4 o = F.ray_o[idx], d = F.ray_d[idx];
@@ -1707,6 +1732,7 @@ int prepare_frame(Ctx *c, Accel *a, const ct_frame_params *p, hipStream_t st, DF
   F.mask = b.mask.as<uint32_t>();
   F.ray_o = b.ray_o.as<float4>(); F.ray_d = b.ray_d.as<float4>(); F.ray_t = b.ray_t.as<uint32_t>();
   F.queue = b.queue.as<uint32_t>();
+  F.queue_cap = (uint32_t)(b.queue.bytes / 4);
   F.counters = b.counters.as<uint32_t>();
   F.gat = b.gather.as<float4>();
   F.pres_o = b.pres[0].as<float4>();
@@ -1807,7 +1833,7 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark,
     // path-reuse visibility uses after the path loop
     hipLaunchKernelGGL(k_reset_mask, dim3(1), dim3(64), 0, st, L.counters,
                        (1u << kCntPtQueue) | (1u << (kCntPtQueue + 1)) | (1u << kCntPtHead) | (1u << kCntQueue) |
-                           (1u << kCntHead));
+                           (1u << kCntQueueNear) | (1u << kCntHead));
     hipLaunchKernelGGL(k_pt_begin, dim3(nb), dim3(kBlock), 0, st, pix);
     const uint32_t grid = std::min<uint32_t>(std::min<uint32_t>(std::max<uint32_t>(nblocks(np), 1), L.trace_blocks),
                                              L.stats ? (L.multi ? segments_resident<true, true>() : segments_resident<true, false>())
@@ -1841,7 +1867,8 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark,
   if ((rc = mark())) return rc;  // kMarkPt
   }
   if (phases & kPhaseSpatial) {
-  hipLaunchKernelGGL(k_reset_mask, dim3(1), dim3(64), 0, st, L.counters, (1u << kCntQueue) | (1u << kCntHead));
+  hipLaunchKernelGGL(k_reset_mask, dim3(1), dim3(64), 0, st, L.counters,
+                     (1u << kCntQueue) | (1u << kCntQueueNear) | (1u << kCntHead));
   hipLaunchKernelGGL(k_spatial<true>, dim3(nb), dim3(kBlock), 0, st, pix);
   if ((rc = mark())) return rc;  // kMarkSpatialEmit
   launch_any(L, pix, 2 * L.max_neighbors * np, st);
@@ -2196,6 +2223,7 @@ int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, v
       DFrame &F = rec[i];
       if ((rc = prepare_frame(b->ctxs[i], b->accels[i], &params[i], st, F))) return rc;
       F.queue = G.queue.as<uint32_t>();
+      F.queue_cap = (uint32_t)(G.queue.bytes / 4);
       F.counters = G.counters.as<uint32_t>();
       F.seg_o = G.seg_o.as<float4>();
       F.seg_d = G.seg_d.as<float4>();
@@ -2403,6 +2431,7 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
       return rc;
     DFrame FB = F;
     FB.queue = c->h_queue.as<uint32_t>();
+    FB.queue_cap = (uint32_t)(c->h_queue.bytes / 4);
     FB.counters = c->h_counters.as<uint32_t>();
     FB.ptq[0] = c->h_ptq[0].as<uint32_t>();
     FB.ptq[1] = c->h_ptq[1].as<uint32_t>();

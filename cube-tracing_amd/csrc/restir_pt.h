@@ -72,7 +72,7 @@ struct PtTrace {
   TraceCnt cnt;
   uint32_t bounce_rays, shadow_rays;
   bool overflow;
-  uint32_t px, slot, bits, emitted;
+  uint32_t px, slot, bits, emitted, far;  // far: emitted rays of at least kFarRay
 };
 
 // ---------------------------------------------------------------- helpers
@@ -305,6 +305,7 @@ CT_D bool vertex_visible(const DFrame &F, F3 o, F3 d, float dist, uint32_t ti, u
     F.ray_d[idx] = make_float4(d.x, d.y, d.z, __uint_as_float(ti));
     F.ray_t[idx] = tp;
     tr.emitted |= 1u << k;
+    if (dist >= kFarRay) tr.far |= 1u << k;
     return true;
   }
   return ((tr.bits >> k) & 1u) == 0u;
