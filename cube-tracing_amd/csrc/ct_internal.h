@@ -224,15 +224,21 @@ struct Ctx {
   uint32_t blk_counts[3] = {0, 0, 0};
   const void *lights_accel = nullptr;  // accel and build whose cube lights were uploaded
   uint32_t lights_generation = 0;
-  // A frame can run as two halves of the image on two streams: the second
-  // half has its own ray and segment queues, counters and frame record.
-  hipStream_t stream2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_half[3] = {};
-  DevBuf h_queue, h_counters, h_ptq[2], h_seg_o, h_seg_d, h_seg_hit, frame_dev2, blk_dev_a, blk_dev2;
-  uint32_t split_key[5] = {0, 0, 0, 0, 0};
-  uint32_t split_counts[2][3] = {{0, 0, 0}, {0, 0, 0}};
-  uint32_t split_px[2] = {0, 0};
-  bool split = false;  // the last frame ran as two halves
+  // A frame can run as kSplitParts interleaved parts of the image on as
+  // many streams (the render stream and part_stream[1..]). Each part has
+  // its own frame record, block table, ray and segment queues, counters and
+  // segment buffers; the per-pixel buffers are shared.
+  static constexpr int kMaxParts = 4;
+  struct PartBufs {
+    DevBuf queue, counters, ptq[2], seg_o, seg_d, seg_hit, frame, blk;
+  };
+  PartBufs part[kMaxParts];
+  hipStream_t part_stream[kMaxParts] = {};
+  hipEvent_t ev_fork = nullptr, ev_p1[kMaxParts] = {}, ev_p2[kMaxParts] = {};
+  uint32_t split_key[6] = {0, 0, 0, 0, 0, 0};
+  uint32_t split_counts[kMaxParts][3] = {};
+  uint32_t split_px[kMaxParts] = {};
+  uint32_t nparts = 0;  // parts of the last frame (0: it ran whole)
 };
 
 // Kernel-side views used by trace.hip and frame.hip.

@@ -1891,37 +1891,43 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark,
   return CT_OK;
 }
 
-// A frame of at least kSplitMinPixels runs as two halves on two streams:
-// the even and the odd rows of 16-pixel tiles, so both halves hold the same
-// mix of the image. Each tile row must be a whole number of 256-pixel
-// blocks (width a multiple of 16) so that no pixel block spans two rows.
+// A frame of at least kSplitMinPixels runs as kSplitParts parts on as many
+// streams: tile row r of 16-pixel tiles goes to part r % kSplitParts, so all
+// parts hold the same mix of the image. Each tile row must be a whole number
+// of 256-pixel blocks (width a multiple of 16) so that no pixel block spans
+// two rows.
 constexpr uint32_t kSplitMinPixels = 1u << 20;
+#ifndef CT_SPLIT_PARTS
+#define CT_SPLIT_PARTS 2
+#endif
+constexpr uint32_t kSplitParts = CT_SPLIT_PARTS;
+static_assert(kSplitParts >= 2 && kSplitParts <= (uint32_t)Ctx::kMaxParts, "split parts");
 bool can_split(const DFrame &F) {
-  return F.npix >= kSplitMinPixels && F.W % 16u == 0u && (F.H + 15) / 16 >= 2u;
+  return F.npix >= kSplitMinPixels && F.W % 16u == 0u && (F.H + 15) / 16 >= kSplitParts;
 }
 
-// Block tables of the two halves: even tile rows' tiles and pixels plus the
-// whole frame's TAA blocks, then odd tile rows' tiles and pixels.
-void split_tables(const DFrame &F, std::vector<uint2> &a, std::vector<uint2> &b, uint32_t ca[3], uint32_t cb[3],
-                  uint32_t px[2]) {
+// Block tables of the parts: each part's tiles and pixels, and for part 0
+// also the whole frame's TAA blocks.
+void split_tables(const DFrame &F, uint32_t parts, std::vector<uint2> *tbl, uint32_t (*cnt)[3], uint32_t *px) {
   const uint32_t tx = F.W / 16, ty = (F.H + 15) / 16, row_px = 16u * F.W;
-  a.clear();
-  b.clear();
-  px[0] = px[1] = 0u;
+  for (uint32_t k = 0; k < parts; ++k) {
+    tbl[k].clear();
+    px[k] = 0u;
+  }
   for (uint32_t r = 0; r < ty; ++r)
-    for (uint32_t t = r * tx; t < (r + 1) * tx; ++t) (r & 1u ? b : a).push_back(make_uint2(0u, t));
-  ca[0] = (uint32_t)a.size();
-  cb[0] = (uint32_t)b.size();
+    for (uint32_t t = r * tx; t < (r + 1) * tx; ++t) tbl[r % parts].push_back(make_uint2(0u, t));
+  for (uint32_t k = 0; k < parts; ++k) cnt[k][0] = (uint32_t)tbl[k].size();
   for (uint32_t r = 0; r < ty; ++r) {
     const uint32_t q1 = std::min(F.npix, (r + 1) * row_px);
-    for (uint32_t q = r * row_px; q < q1; q += kBlock) (r & 1u ? b : a).push_back(make_uint2(0u, q));
-    px[r & 1u] += q1 - r * row_px;
+    for (uint32_t q = r * row_px; q < q1; q += kBlock) tbl[r % parts].push_back(make_uint2(0u, q));
+    px[r % parts] += q1 - r * row_px;
   }
-  ca[1] = (uint32_t)a.size() - ca[0];
-  cb[1] = (uint32_t)b.size() - cb[0];
-  for (uint32_t q = 0; q < F.taa_w * F.taa_h; q += kBlock) a.push_back(make_uint2(0u, q));
-  ca[2] = (uint32_t)a.size() - ca[0] - ca[1];
-  cb[2] = 0u;
+  for (uint32_t k = 0; k < parts; ++k) {
+    cnt[k][1] = (uint32_t)tbl[k].size() - cnt[k][0];
+    cnt[k][2] = 0u;
+  }
+  for (uint32_t q = 0; q < F.taa_w * F.taa_h; q += kBlock) tbl[0].push_back(make_uint2(0u, q));
+  cnt[0][2] = (uint32_t)tbl[0].size() - cnt[0][0] - cnt[0][1];
 }
 
 }  // namespace
@@ -1966,12 +1972,16 @@ int ct_ctx_destroy(ct_ctx *c) {
   for (int k = 0; k < 48; ++k)
     if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
   if (c->counters_pinned) (void)hipHostFree(c->counters_pinned);
-  if (c->stream2) {
-    (void)hipStreamSynchronize(c->stream2);
-    (void)hipStreamDestroy(c->stream2);
+  for (hipStream_t ps : c->part_stream)
+    if (ps) {
+      (void)hipStreamSynchronize(ps);
+      (void)hipStreamDestroy(ps);
+    }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  for (int k = 0; k < Ctx::kMaxParts; ++k) {
+    if (c->ev_p1[k]) (void)hipEventDestroy(c->ev_p1[k]);
+    if (c->ev_p2[k]) (void)hipEventDestroy(c->ev_p2[k]);
   }
-  for (hipEvent_t e : {c->ev_fork, c->ev_half[0], c->ev_half[1], c->ev_half[2]})
-    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return CT_OK;
@@ -2411,82 +2421,93 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
   auto none = []() -> int { return CT_OK; };
   // Per-pass events and traversal counters describe one sequence: those
   // frames run whole on one stream.
-  c->split = (c->instrument & 3u) == 0u && can_split(F);
-  if (c->split) {
-    // Two halves of the image, each with its own queues, on `st` and on
-    // stream2: phase 1 of both, a join, phase 2 of both, a join, then TAA.
+  const uint32_t parts = (c->instrument & 3u) == 0u && can_split(F) ? kSplitParts : 0u;
+  c->nparts = parts;
+  if (parts) {
+    // The parts run phase 1 at once on their streams, join, run phase 2 at
+    // once, join, then TAA runs on `st`. Phase 2 reads neighbours' phase-1
+    // results across parts and rewrites the previous-frame buffers that
+    // phase 1 reads at reprojected pixels, hence the joins. Each part has
+    // its own segment buffers: the whole-frame sequence reuses the ray and
+    // hit buffers for segments, which parts in flight together cannot.
     const size_t np = (size_t)c->max_w * c->max_h;
-    if (!c->stream2) {  // only contexts that split hold a second stream
-      if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
-          hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-          hipEventCreateWithFlags(&c->ev_half[0], hipEventDisableTiming) != hipSuccess ||
-          hipEventCreateWithFlags(&c->ev_half[1], hipEventDisableTiming) != hipSuccess ||
-          hipEventCreateWithFlags(&c->ev_half[2], hipEventDisableTiming) != hipSuccess)
-        return fail(CT_E_HIP, "ct_frame_render: stream creation failed");
+    if (!c->ev_fork) {  // only contexts that split hold extra streams
+      if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess)
+        return fail(CT_E_HIP, "ct_frame_render: event creation failed");
+      for (int k = 0; k < Ctx::kMaxParts; ++k)
+        if ((k > 0 && hipStreamCreateWithFlags(&c->part_stream[k], hipStreamNonBlocking) != hipSuccess) ||
+            hipEventCreateWithFlags(&c->ev_p1[k], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_p2[k], hipEventDisableTiming) != hipSuccess)
+          return fail(CT_E_HIP, "ct_frame_render: stream creation failed");
     }
-    if ((rc = c->h_queue.reserve(4 * np * kRaySlots)) || (rc = c->h_counters.reserve(64 * sizeof(uint32_t))) ||
-        (rc = c->h_ptq[0].reserve(4 * np)) || (rc = c->h_ptq[1].reserve(4 * np)) ||
-        (rc = c->h_seg_o.reserve(16 * np)) || (rc = c->h_seg_d.reserve(16 * np)) ||
-        (rc = c->h_seg_hit.reserve(16 * np)) || (rc = c->frame_dev2.reserve(sizeof(DFrame))))
-      return rc;
-    DFrame FB = F;
-    FB.queue = c->h_queue.as<uint32_t>();
-    FB.queue_cap = (uint32_t)(c->h_queue.bytes / 4);
-    FB.counters = c->h_counters.as<uint32_t>();
-    FB.ptq[0] = c->h_ptq[0].as<uint32_t>();
-    FB.ptq[1] = c->h_ptq[1].as<uint32_t>();
-    FB.seg_o = c->h_seg_o.as<float4>();
-    FB.seg_d = c->h_seg_d.as<float4>();
-    FB.seg_hit = c->h_seg_hit.as<float4>();
-    const uint32_t key[5] = {F.W, F.H, F.taa_w, F.taa_h, 1u};
-    if (std::memcmp(key, c->split_key, sizeof(key)) != 0) {
-      std::vector<uint2> ta, tb;
-      split_tables(F, ta, tb, c->split_counts[0], c->split_counts[1], c->split_px);
-      if ((rc = c->blk_dev_a.reserve(sizeof(uint2) * ta.size())) || (rc = c->blk_dev2.reserve(sizeof(uint2) * tb.size())))
+    DFrame FP[Ctx::kMaxParts];
+    for (uint32_t k = 0; k < parts; ++k) {
+      Ctx::PartBufs &P = c->part[k];
+      if ((rc = P.queue.reserve(4 * np * kRaySlots)) || (rc = P.counters.reserve(64 * sizeof(uint32_t))) ||
+          (rc = P.ptq[0].reserve(4 * np)) || (rc = P.ptq[1].reserve(4 * np)) || (rc = P.seg_o.reserve(16 * np)) ||
+          (rc = P.seg_d.reserve(16 * np)) || (rc = P.seg_hit.reserve(16 * np)) ||
+          (rc = P.frame.reserve(sizeof(DFrame))))
         return rc;
-      CT_HIP(hipStreamSynchronize(st));
-      CT_HIP(hipStreamSynchronize(c->stream2));
-      CT_HIP(hipMemcpy(c->blk_dev_a.p, ta.data(), sizeof(uint2) * ta.size(), hipMemcpyHostToDevice));
-      CT_HIP(hipMemcpy(c->blk_dev2.p, tb.data(), sizeof(uint2) * tb.size(), hipMemcpyHostToDevice));
+      FP[k] = F;
+      FP[k].queue = P.queue.as<uint32_t>();
+      FP[k].queue_cap = (uint32_t)(P.queue.bytes / 4);
+      FP[k].counters = P.counters.as<uint32_t>();
+      FP[k].ptq[0] = P.ptq[0].as<uint32_t>();
+      FP[k].ptq[1] = P.ptq[1].as<uint32_t>();
+      FP[k].seg_o = P.seg_o.as<float4>();
+      FP[k].seg_d = P.seg_d.as<float4>();
+      FP[k].seg_hit = P.seg_hit.as<float4>();
+    }
+    const uint32_t key[6] = {F.W, F.H, F.taa_w, F.taa_h, parts, 1u};
+    if (std::memcmp(key, c->split_key, sizeof(key)) != 0) {
+      std::vector<uint2> tbl[Ctx::kMaxParts];
+      split_tables(F, parts, tbl, c->split_counts, c->split_px);
+      for (uint32_t k = 0; k < parts; ++k)
+        if ((rc = c->part[k].blk.reserve(sizeof(uint2) * tbl[k].size()))) return rc;
+      CT_HIP(hipDeviceSynchronize());
+      for (uint32_t k = 0; k < parts; ++k)
+        CT_HIP(hipMemcpy(c->part[k].blk.p, tbl[k].data(), sizeof(uint2) * tbl[k].size(), hipMemcpyHostToDevice));
       std::memcpy(c->split_key, key, sizeof(key));
     }
-    hipLaunchKernelGGL(k_store_frame, dim3(1), dim3(64), 0, st, F, c->frame_dev.as<DFrame>());
-    hipLaunchKernelGGL(k_store_frame, dim3(1), dim3(64), 0, st, FB, c->frame_dev2.as<DFrame>());
-    Launch L[2];
-    for (int h = 0; h < 2; ++h) {
-      L[h].fb = h ? c->frame_dev2.as<DFrame>() : c->frame_dev.as<DFrame>();
-      L[h].blk = h ? c->blk_dev2.as<uint2>() : c->blk_dev_a.as<uint2>();
-      std::memcpy(L[h].blocks, c->split_counts[h], sizeof(L[h].blocks));
-      L[h].total_px = c->split_px[h];
-      L[h].pt_depth = F.rp.indirect ? F.rp.max_depth : 0u;
-      L[h].max_neighbors = F.rp.max_neighbors;
-      L[h].trace_blocks = (uint32_t)trace_grid();
-      L[h].multi = false;
-      L[h].counters = h ? FB.counters : F.counters;
-      L[h].stats = nullptr;
+    Launch L[Ctx::kMaxParts];
+    for (uint32_t k = 0; k < parts; ++k) {
+      hipLaunchKernelGGL(k_store_frame, dim3(1), dim3(64), 0, st, FP[k], c->part[k].frame.as<DFrame>());
+      L[k].fb = c->part[k].frame.as<DFrame>();
+      L[k].blk = c->part[k].blk.as<uint2>();
+      std::memcpy(L[k].blocks, c->split_counts[k], sizeof(L[k].blocks));
+      L[k].total_px = c->split_px[k];
+      L[k].pt_depth = F.rp.indirect ? F.rp.max_depth : 0u;
+      L[k].max_neighbors = F.rp.max_neighbors;
+      L[k].trace_blocks = (uint32_t)trace_grid();
+      L[k].multi = false;
+      L[k].counters = FP[k].counters;
+      L[k].stats = nullptr;
     }
-    hipStream_t s2 = c->stream2;
+    auto stream_of = [&](uint32_t k) { return k ? c->part_stream[k] : st; };
     if (c->instrument & 4u) CT_HIP(hipEventRecord(c->ev[kMarkStart], st));
     CT_HIP(hipEventRecord(c->ev_fork, st));
-    CT_HIP(hipStreamWaitEvent(s2, c->ev_fork, 0));
-    if ((rc = enqueue_frames(L[0], st, none, none, kPhaseFirst)) ||
-        (rc = enqueue_frames(L[1], s2, none, none, kPhaseFirst)))
-      return rc;
-    CT_HIP(hipEventRecord(c->ev_half[0], st));
-    CT_HIP(hipEventRecord(c->ev_half[1], s2));
-    CT_HIP(hipStreamWaitEvent(st, c->ev_half[1], 0));
-    CT_HIP(hipStreamWaitEvent(s2, c->ev_half[0], 0));
-    if ((rc = enqueue_frames(L[0], st, none, none, kPhaseSpatial)) ||
-        (rc = enqueue_frames(L[1], s2, none, none, kPhaseSpatial)))
-      return rc;
-    CT_HIP(hipEventRecord(c->ev_half[2], s2));
-    CT_HIP(hipStreamWaitEvent(st, c->ev_half[2], 0));
+    for (uint32_t k = 1; k < parts; ++k) CT_HIP(hipStreamWaitEvent(stream_of(k), c->ev_fork, 0));
+    for (uint32_t k = 0; k < parts; ++k)
+      if ((rc = enqueue_frames(L[k], stream_of(k), none, none, kPhaseFirst))) return rc;
+    for (uint32_t k = 1; k < parts; ++k) {
+      CT_HIP(hipEventRecord(c->ev_p1[k], stream_of(k)));
+      CT_HIP(hipStreamWaitEvent(st, c->ev_p1[k], 0));
+    }
+    CT_HIP(hipEventRecord(c->ev_p1[0], st));
+    for (uint32_t k = 1; k < parts; ++k) CT_HIP(hipStreamWaitEvent(stream_of(k), c->ev_p1[0], 0));
+    for (uint32_t k = 0; k < parts; ++k)
+      if ((rc = enqueue_frames(L[k], stream_of(k), none, none, kPhaseSpatial))) return rc;
+    for (uint32_t k = 1; k < parts; ++k) {
+      CT_HIP(hipEventRecord(c->ev_p2[k], stream_of(k)));
+      CT_HIP(hipStreamWaitEvent(st, c->ev_p2[k], 0));
+    }
     if ((rc = enqueue_frames(L[0], st, none, none, kPhaseTaa))) return rc;
     if (c->instrument & 4u) CT_HIP(hipEventRecord(c->ev[kMarkTaa], st));
     c->n_events = 0;
     c->n_seg = 0;
-    CT_HIP(hipMemcpyAsync(c->counters_pinned, F.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    CT_HIP(hipMemcpyAsync(c->counters_pinned + 32, FB.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    for (uint32_t k = 0; k < parts; ++k)  // part k's counters 128 B apart
+      CT_HIP(hipMemcpyAsync(c->counters_pinned + 16 * k, FP[k].counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            st));
   } else {
     // the frame record and its block tables on the device
     const uint32_t key[4] = {F.W, F.H, F.taa_w, F.taa_h};
@@ -2607,7 +2628,7 @@ int ct_frame_get_stats(ct_ctx *c, ct_frame_stats *out) {
   if (c->stats_pending) {
     CT_HIP(hipStreamSynchronize(c->stream_last));
     const uint32_t *cnt = reinterpret_cast<const uint32_t *>(c->counters_pinned);
-    for (int h = 0; h < (c->split ? 2 : 1); ++h, cnt += 64) {  // the lower half's counters sit 256 B on
+    for (uint32_t h = 0; h < std::max(c->nparts, 1u); ++h, cnt += 32) {  // parts' counters 128 B apart
       c->stats.shadow_rays_ris += cnt[kCntRis];
       c->stats.shadow_rays_temporal += cnt[kCntTemporal];
       c->stats.shadow_rays_spatial += cnt[kCntSpatial];
