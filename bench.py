@@ -203,6 +203,44 @@ def adjoint_bench(steps=3):
             "forward_mpaths_s": round(paths / t_fwd / 1e6, 1), "losses": [round(x, 6) for x in losses]}
 
 
+def vox_bench(frames=8, warmup=2, n=128, chunk=64):
+    """A .vox scene end to end: a procedural n^3 MagicaVoxel model (stepped
+    tower, about 1 % emissive voxels) parsed by the library's loader
+    (host), loaded (staging, BLAS/TLAS build, upload), then ReSTIR DI + a
+    3-bounce path at 1920x1080 on it (two-stream frames, GPU events)."""
+    import ctamd
+    from ctamd.voxfile import monument
+    data = monument(n=n)
+    t0 = time.perf_counter()
+    sc = ctamd.Scene.from_vox(data, chunk=(chunk, chunk, chunk))
+    parse_s = time.perf_counter() - t0
+    acc = ctamd.Accel(max_instances=sc.info.instance_count + 8,
+                      max_primitives=sc.info.primitive_count + 32 * sc.info.instance_count + 1024,
+                      max_cube_lights=max(sc.info.cube_light_count, 1) + 8)
+    ctx = ctamd.Ctx(1920, 1080)
+    t0 = time.perf_counter()
+    ctamd.load_scene(sc, acc, ctx)
+    load_s = time.perf_counter() - t0
+    w = ctamd.vox_view(sc, frames=warmup + frames)
+    for f in range(warmup):
+        ctx.render(acc, ctamd.frame_params(w, f))
+    ctx.stats()
+    ctx.set_instrumentation(4)
+    ms, rays = [], 0
+    for f in range(warmup, warmup + frames):
+        ctx.render(acc, ctamd.frame_params(w, f))
+        st = ctx.stats()
+        ms.append(st.ms_total)
+        rays += st.primary_rays + st.shadow_rays + st.bounce_rays
+    med = sorted(ms)[len(ms) // 2]
+    nv = sc.info.primitive_count
+    return {"file_bytes": len(data), "grid": [n, n, n], "voxels": nv, "blas": sc.info.instance_count,
+            "materials": sc.info.material_count, "lights": sc.info.cube_light_count,
+            "parse_ms": round(parse_s * 1e3, 2), "parse_mvoxels_s": round(nv / parse_s / 1e6, 1),
+            "load_ms": round(load_s * 1e3, 1), "frame_ms_median": round(med, 3),
+            "mrays_s": round(rays / (sum(ms) / 1e3) / 1e6, 1)}
+
+
 def update_bench(acc, n_inst, reps=5):
     """Scene updates on the loaded accel (after every other measurement, as
     they move nothing but still rewrite it): UPDATE_BLAS_FROM_CPU tasks with
@@ -237,6 +275,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-view batched comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the inverse-rendering step timing")
+    ap.add_argument("--no-vox", action="store_true", help="skip the .vox scene load and frame timing")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (rocprof counter calibration)")
     args = ap.parse_args()
@@ -358,6 +397,10 @@ def main():
     if not args.no_adjoint:
         adjoint = adjoint_bench()
         log(f"adjoint: {adjoint}")
+    vox = None
+    if not args.no_vox:
+        vox = vox_bench()
+        log(f"vox: {vox}")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, sc, acc, args.cpu_budget)
@@ -386,6 +429,7 @@ def main():
         "batched": batched,
         "adjoint": adjoint,
         "build": build,
+        "vox": vox,
     }
     print(json.dumps(out), flush=True)
 

@@ -275,4 +275,14 @@ int device_count_gfx950();
 
 struct ct_accel : ct::Accel {};
 struct ct_ctx : ct::Ctx {};
-struct ct_scene;
+// A host-side voxel scene (generated or loaded from a .vox file): the
+// arrays the map loader writes into the staging buffers.
+struct ct_scene {
+  ct_scene_desc desc{};
+  std::vector<ct_instance> instances;  // first_primitive_index: host order, contiguous
+  std::vector<ct_aabb> aabbs;
+  std::vector<ct_primitive> prims;
+  std::vector<ct_material> mats;
+  std::vector<ct_light> lights;
+  float gmin[3] = {0, 0, 0}, gmax[3] = {0, 0, 0};
+};

@@ -105,16 +105,6 @@ inline uint32_t ord_bits(float f) {
 
 }  // namespace
 
-struct ct_scene {
-  ct_scene_desc desc{};
-  std::vector<ct_instance> instances;
-  std::vector<ct_aabb> aabbs;
-  std::vector<ct_primitive> prims;
-  std::vector<ct_material> mats;
-  std::vector<ct_light> lights;
-  float gmin[3] = {0, 0, 0}, gmax[3] = {0, 0, 0};
-};
-
 namespace {
 int add_cube_lights(ct_ctx *ctx, uint32_t nl);
 }  // namespace

@@ -62,6 +62,9 @@ CT_MAX_PT_NEIGHBORS = 8
 CT_MAX_PT_DEPTH = 14
 CT_ADJOINT_MAX_DEPTH = 7
 
+(CT_AXIS_X_BOTTOM_TOP, CT_AXIS_Y_BOTTOM_TOP, CT_AXIS_Z_BOTTOM_TOP,
+ CT_AXIS_X_TOP_BOTTOM, CT_AXIS_Y_TOP_BOTTOM, CT_AXIS_Z_TOP_BOTTOM) = range(6)
+
 f32, u32, i32 = C.c_float, C.c_uint32, C.c_int32
 
 
@@ -239,6 +242,11 @@ class SceneDesc(C.Structure):
                 ("palette_size", u32), ("voxel_extent", f32), ("pad", u32)]
 
 
+class VoxDesc(C.Structure):
+    _fields_ = [("axis", u32), ("chunk", u32 * 3), ("model", C.c_int32), ("voxel_extent", f32), ("flux", f32),
+                ("pad", u32)]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [("instance_count", u32), ("primitive_count", u32), ("material_count", u32),
                 ("cube_light_count", u32), ("instances", C.POINTER(Instance)), ("aabbs", C.POINTER(Aabb)),
@@ -251,7 +259,7 @@ STRUCT_SIZES = {"ct_aabb": (Aabb, 24), "ct_instance": (Instance, 72), "ct_primit
                 "ct_reservoir": (Reservoir, 36), "ct_di_info": (DiInfo, 60), "ct_path_reservoir": (PathReservoir, 92),
                 "ct_restir_params": (RestirParams, 108), "ct_adjoint_view": (AdjointView, 136),
                 "ct_adjoint_desc": (AdjointDesc, 16), "ct_batch_tuning": (BatchTuning, 16),
-                "ct_task": (Task, 84), "ct_world": (World, 64)}
+                "ct_task": (Task, 84), "ct_world": (World, 64), "ct_vox_desc": (VoxDesc, 32)}
 
 
 # ----------------------------------------------------------------- loader
@@ -322,6 +330,7 @@ _SIGS = {
     "ct_scene_destroy": ([C.c_void_p], C.c_int),
     "ct_scene_load": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "ct_scene_bind": ([C.c_void_p, C.c_void_p], C.c_int),
+    "ct_scene_from_vox": ([C.c_char_p, C.c_size_t, C.POINTER(VoxDesc), C.POINTER(C.c_void_p)], C.c_int),
 }
 
 
@@ -407,10 +416,13 @@ class Scene:
     instance, AABB, primitive, material and light arrays are copied out of
     the library on first use."""
 
-    def __init__(self, desc: SceneDesc):
+    def __init__(self, desc: SceneDesc = None, handle=None):
         self.desc = desc
         h = C.c_void_p()
-        check(lib().ct_scene_generate(C.byref(desc), C.byref(h)))
+        if handle is None:
+            check(lib().ct_scene_generate(C.byref(desc), C.byref(h)))
+        else:
+            h = handle
         self.handle = h
         info = SceneInfo()
         check(lib().ct_scene_get_info(h, C.byref(info)))
@@ -418,6 +430,24 @@ class Scene:
         self.grid_min = np.array(info.grid_min[:], np.float32)
         self.grid_max = np.array(info.grid_max[:], np.float32)
         self._cache = {}
+
+    @classmethod
+    def from_vox(cls, data, axis=0, chunk=(0, 0, 0), model=-1, voxel_extent=0.0, flux=0.0):
+        """A MagicaVoxel .vox file (bytes, or a path) as a scene, with the
+        map loader's rules (map_loader.cpp:97-325); axis 0 = X_BOTTOM_TOP
+        as main.cpp:1009 loads it."""
+        if not isinstance(data, (bytes, bytearray)):
+            with open(data, "rb") as fh:
+                data = fh.read()
+        d = VoxDesc()
+        d.axis = axis
+        d.chunk[:] = list(chunk)
+        d.model = model
+        d.voxel_extent = voxel_extent
+        d.flux = flux
+        h = C.c_void_p()
+        check(lib().ct_scene_from_vox(bytes(data), len(data), C.byref(d), C.byref(h)))
+        return cls(None, handle=h)
 
     def _get(self, name):
         if name not in self._cache:

@@ -16,7 +16,7 @@ from typing import Callable, Tuple
 
 import numpy as np
 
-__all__ = ["Workload", "small_plain", "hd_di", "frame_params", "scene_desc", "batched_views"]
+__all__ = ["Workload", "small_plain", "hd_di", "frame_params", "scene_desc", "batched_views", "vox_view"]
 
 
 def _lib():
@@ -66,12 +66,13 @@ def _static_camera(pos, fwd, up):
     return lambda f: (pos, fwd, up)
 
 
-def _orbit_camera(radius, height, deg_per_frame, start_deg=0.0):
+def _orbit_camera(radius, height, deg_per_frame, start_deg=0.0, center=(0.0, 0.0, 0.0)):
     def cam(f):
         a = math.radians(start_deg + deg_per_frame * f)
-        pos = (radius * math.sin(a), height, radius * math.cos(a))
-        n = math.sqrt(pos[0] ** 2 + pos[1] ** 2 + pos[2] ** 2)
-        fwd = (-pos[0] / n, -pos[1] / n, -pos[2] / n)
+        rel = (radius * math.sin(a), height, radius * math.cos(a))
+        n = math.sqrt(rel[0] ** 2 + rel[1] ** 2 + rel[2] ** 2)
+        fwd = (-rel[0] / n, -rel[1] / n, -rel[2] / n)
+        pos = tuple(float(c) + r for c, r in zip(center, rel))
         return pos, fwd, (0.0, 1.0, 0.0)
     return cam
 
@@ -119,6 +120,23 @@ def hd_di(frames=16, width=1920, height=1080, grid=256, chunk=64, indirect=True,
         name=f"hd_di_{grid}", grid=(grid, grid, grid), chunk=(chunk, chunk, chunk), occupied=occ,
         emissive=round(0.02 * occ), le=(1.0, 20.0), seed=0xC1, width=width, height=height, frames=frames,
         fov_radians=math.radians(60.0), camera=_orbit_camera(1.5 * extent, 0.375 * extent, 1.0),
+        features=CT_RIS_CUBE_LIGHT_BIT, taa=True, restir=rp)
+
+
+def vox_view(scene, width=1920, height=1080, frames=16, indirect=True, max_depth=3, name="vox"):
+    """Frames of a loaded .vox scene: the camera orbits the scene's centre at
+    1.5x its largest extent, 1 degree/frame, 60 degree field of view;
+    ReSTIR DI over its cube lights plus a max_depth-bounce path, TAA at 0.1."""
+    from . import CT_RIS_CUBE_LIGHT_BIT, restir_params_reference
+    lo, hi = np.asarray(scene.grid_min, np.float64), np.asarray(scene.grid_max, np.float64)
+    ext = float((hi - lo).max())
+    rp = restir_params_reference()
+    rp.taa_alpha = 0.1
+    _indirect(rp, indirect, max_depth)
+    return Workload(
+        name=name, grid=(0, 0, 0), chunk=(0, 0, 0), occupied=scene.info.primitive_count,
+        emissive=scene.info.cube_light_count, le=(0.0, 0.0), seed=0, width=width, height=height, frames=frames,
+        fov_radians=math.radians(60.0), camera=_orbit_camera(1.5 * ext, 0.375 * ext, 1.0, center=(lo + hi) / 2),
         features=CT_RIS_CUBE_LIGHT_BIT, taa=True, restir=rp)
 
 

@@ -665,6 +665,36 @@ int ct_scene_load(const ct_scene *scene, ct_accel *accel, ct_ctx *ctx,
  * and light configuration.  The context must hold no materials yet. */
 int ct_scene_bind(const ct_scene *scene, ct_ctx *ctx);
 
+/* MagicaVoxel .vox files as scenes: the voxel -> AABB / PRIMITIVE /
+ * MATERIAL / LIGHT rules of the map loader (map_loader.cpp:97-325, called
+ * by load_model / load_scene, main.cpp:1006-1126) over the library's own
+ * parser of the VOX chunk format (SIZE, XYZI, RGBA, MATL, MATT, nTRN,
+ * nGRP, nSHP).  The reference parses the file with gvox
+ * (map_loader.cpp:479-493); what gvox decides is fixed here: a model (or
+ * each chunk-aligned box of it) is one region and one instance; with a
+ * scene graph a model of size s sits at v - floor(s/2) + translation;
+ * emissive color indices are MATL `_type` `_emit` with `_emit` > 0 or MATT
+ * type 3 with weight > 0, and emit their palette color.  The result is a
+ * ct_scene for ct_scene_load / ct_scene_bind like a generated one. */
+#define CT_AXIS_X_BOTTOM_TOP 0u /* AXIS_DIRECTION, defines.h:118-125 */
+#define CT_AXIS_Y_BOTTOM_TOP 1u
+#define CT_AXIS_Z_BOTTOM_TOP 2u
+#define CT_AXIS_X_TOP_BOTTOM 3u
+#define CT_AXIS_Y_TOP_BOTTOM 4u
+#define CT_AXIS_Z_TOP_BOTTOM 5u
+typedef struct ct_vox_desc {
+  uint32_t axis;      /* CT_AXIS_*; main.cpp:1009 loads with X_BOTTOM_TOP */
+  uint32_t chunk[3];  /* region size in file voxels per axis; 0: a whole model */
+  int32_t model;      /* -1: every model (placed by the scene graph if any); k: model k alone at the origin */
+  float voxel_extent; /* 0: CT_VOXEL_EXTENT */
+  float flux;         /* emission scale, 0: 1 (the loader's flux, map_loader.cpp:129) */
+  uint32_t pad;
+} ct_vox_desc;
+/* Parses `bytes` bytes at `data` (a whole .vox file).  CT_E_INVALID for a
+ * malformed or empty file, CT_E_UNSUPPORTED for a rotated scene-graph
+ * node, CT_E_CAPACITY beyond CT_MAX_PRIMITIVES voxels. */
+int ct_scene_from_vox(const void *data, size_t bytes, const ct_vox_desc *desc, ct_scene **out);
+
 /* =================================================================
  * Differentiable rendering.  No counterpart in the reference (its
  * renderer has no derivatives); the build's own adjoint of radiance with

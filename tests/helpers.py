@@ -15,8 +15,8 @@ def struct_array(arr, dtype=np.uint8):
     return np.frombuffer(bytes(arr), dtype)
 
 
-def gpu_scene(w, max_prims=None):
-    sc = ctamd.Scene(ctamd.scene_desc(w))
+def gpu_scene(w, max_prims=None, scene=None):
+    sc = scene if scene is not None else ctamd.Scene(ctamd.scene_desc(w))
     n = sc.info.primitive_count
     acc = ctamd.Accel(max_instances=max(sc.info.instance_count, 1) + 8,
                       max_primitives=max_prims or (n + 64 * sc.info.instance_count + 1024),

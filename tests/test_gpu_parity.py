@@ -78,8 +78,8 @@ def test_trace_any_bit_exact(gpu):
     assert 0 < g.mean() < 1
 
 
-def _frames_match(w, frames):
-    sc, acc, ctx = gpu_scene(w)
+def _frames_match(w, frames, scene=None):
+    sc, acc, ctx = gpu_scene(w, scene=scene)
     assert ctx.light_config().cube_light_count == sc.info.cube_light_count
     osc = oracle_scene(sc, acc)
     ofr = O.OFrame(osc, w.width, w.height)
