@@ -365,6 +365,10 @@ int ct_ctx_append_materials(ct_ctx *ctx, const ct_material *m, uint32_t count, u
 int ct_ctx_get_light_config(ct_ctx *ctx, ct_light_config *out);
 
 int ct_scene_load(const ct_scene *s, ct_accel *a, ct_ctx *ctx, void *stream) {
+  return ct_scene_load_transformed(s, a, ctx, nullptr, stream);
+}
+
+int ct_scene_load_transformed(const ct_scene *s, ct_accel *a, ct_ctx *ctx, const float transform[16], void *stream) {
   if (!s || !a || !ctx) return fail(CT_E_INVALID, "ct_scene_load: null argument");
   ct_staging st;
   int rc = ct_accel_get_staging(a, &st);
@@ -404,7 +408,8 @@ int ct_scene_load(const ct_scene *s, ct_accel *a, ct_ctx *ctx, void *stream) {
   t.type = CT_TASK_BUILD_BLAS_FROM_CPU;
   t.u.blas_build_from_cpu.instance_count = ni;
   t.u.blas_build_from_cpu.primitive_count = np;
-  for (int k = 0; k < 16; ++k) t.u.blas_build_from_cpu.transform[k] = (k % 5 == 0) ? 1.0f : 0.0f;
+  for (int k = 0; k < 16; ++k)
+    t.u.blas_build_from_cpu.transform[k] = transform ? transform[k] : ((k % 5 == 0) ? 1.0f : 0.0f);
   if ((rc = ct_accel_task_queue_add(a, &t))) return rc;
   if ((rc = ct_accel_update_scene(a, 1, stream, nullptr))) return rc;
   return add_cube_lights(ctx, nl);

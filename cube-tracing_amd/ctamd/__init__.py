@@ -330,6 +330,7 @@ _SIGS = {
     "ct_scene_destroy": ([C.c_void_p], C.c_int),
     "ct_scene_load": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "ct_scene_bind": ([C.c_void_p, C.c_void_p], C.c_int),
+    "ct_scene_load_transformed": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(f32), C.c_void_p], C.c_int),
     "ct_scene_from_vox": ([C.c_char_p, C.c_size_t, C.POINTER(VoxDesc), C.POINTER(C.c_void_p)], C.c_int),
 }
 
@@ -773,9 +774,15 @@ class Batch:
             pass
 
 
-def load_scene(scene: Scene, accel: Accel, ctx: Ctx):
-    """load_scene (main.cpp:1045-1126): staging, BUILD_BLAS_FROM_CPU, update_scene(true)."""
-    check(lib().ct_scene_load(scene.handle, accel.handle, ctx.handle, None))
+def load_scene(scene: Scene, accel: Accel, ctx: Ctx, transform=None):
+    """load_scene (main.cpp:1045-1126): staging, BUILD_BLAS_FROM_CPU, update_scene(true);
+    with `transform` (4x4, column-major as INSTANCE.transform) it is load_model's
+    placed build (main.cpp:1006-1043)."""
+    if transform is None:
+        check(lib().ct_scene_load(scene.handle, accel.handle, ctx.handle, None))
+    else:
+        t = (f32 * 16)(*np.asarray(transform, np.float32).reshape(16).tolist())
+        check(lib().ct_scene_load_transformed(scene.handle, accel.handle, ctx.handle, t, None))
 
 
 def bind_scene(scene: Scene, ctx: Ctx):

@@ -660,6 +660,11 @@ int ct_scene_destroy(ct_scene *scene);
  * main.cpp:1317-1320). */
 int ct_scene_load(const ct_scene *scene, ct_accel *accel, ct_ctx *ctx,
                   void *stream);
+/* load_model(name, transform) (main.cpp:1006-1043): the same, with
+ * `transform` (column-major 4x4, as INSTANCE.transform) in the
+ * BUILD_BLAS_FROM_CPU task for every new instance; null = identity. */
+int ct_scene_load_transformed(const ct_scene *scene, ct_accel *accel, ct_ctx *ctx,
+                              const float transform[16], void *stream);
 /* The context half of ct_scene_load, for more contexts rendering a scene
  * already loaded into an accel with a fresh context: the same materials
  * and light configuration.  The context must hold no materials yet. */
