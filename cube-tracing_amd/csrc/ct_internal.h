@@ -181,6 +181,9 @@ struct Accel {
   FreeList prim_alloc;
   uint32_t prim_high_water = 0;
   BvhSet set;
+  // voxel-deletion marks (squared_brush.slang): instance bits, slot bits, counters
+  DevBuf brush_imask, brush_pmask, brush_counters;
+  uint32_t brush_imask_words = 0, brush_pmask_words = 0;
   BuildScratch scratch;
   uint32_t status = CT_AS_IDLE;
   uint32_t generation = 0;

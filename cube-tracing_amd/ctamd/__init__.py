@@ -330,6 +330,10 @@ _SIGS = {
     "ct_scene_destroy": ([C.c_void_p], C.c_int),
     "ct_scene_load": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "ct_scene_bind": ([C.c_void_p, C.c_void_p], C.c_int),
+    "ct_brush_apply": ([C.c_void_p, C.c_void_p, u32, u32, u32, C.c_void_p], C.c_int),
+    "ct_brush_mark": ([C.c_void_p, C.POINTER(ObjectInfo), u32, C.c_void_p], C.c_int),
+    "ct_brush_get_counters": ([C.c_void_p, C.POINTER(u32), C.POINTER(u32)], C.c_int),
+    "ct_accel_process_voxel_modifications": ([C.c_void_p, C.c_void_p, C.POINTER(u32), C.POINTER(u32)], C.c_int),
     "ct_scene_load_transformed": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(f32), C.c_void_p], C.c_int),
     "ct_scene_from_vox": ([C.c_char_p, C.c_size_t, C.POINTER(VoxDesc), C.POINTER(C.c_void_p)], C.c_int),
 }
@@ -608,6 +612,26 @@ class Accel:
         t.type = CT_TASK_UNDO_OP_CPU
         self.task_queue_add(t)
 
+    def brush_mark(self, voxels):
+        """Marks (instance id, primitive id) pairs for deletion (delete_primitive_from_instance)."""
+        v = np.ascontiguousarray(voxels, np.uint32).reshape(-1, 2)
+        arr = (ObjectInfo * max(len(v), 1))()
+        C.memmove(arr, v.ctypes.data, 8 * len(v))
+        check(lib().ct_brush_mark(self.handle, arr, len(v), None))
+
+    def brush_counters(self):
+        """(newly marked instances, newly marked primitives), BRUSH_COUNTER."""
+        i, p = u32(0), u32(0)
+        check(lib().ct_brush_get_counters(self.handle, C.byref(i), C.byref(p)))
+        return i.value, p.value
+
+    def process_voxel_modifications(self):
+        """Compacts the marked instances on the device and queues their
+        DELETE_PRIMITIVE_BLAS_FROM_GPU tasks; returns (tasks, voxels)."""
+        i, p = u32(0), u32(0)
+        check(lib().ct_accel_process_voxel_modifications(self.handle, None, C.byref(i), C.byref(p)))
+        return i.value, p.value
+
     def read_bvh(self, instance: int, n: int) -> dict:
         keys = np.zeros(n, np.uint32)
         ids = np.zeros(n, np.uint32)
@@ -783,6 +807,11 @@ def load_scene(scene: Scene, accel: Accel, ctx: Ctx, transform=None):
     else:
         t = (f32 * 16)(*np.asarray(transform, np.float32).reshape(16).tolist())
         check(lib().ct_scene_load_transformed(scene.handle, accel.handle, ctx.handle, t, None))
+
+
+def brush_apply(ctx: Ctx, accel: Accel, x: int, y: int, window: int):
+    """The squared brush (squared_brush.slang) at pixel (x, y) of ctx's last frame."""
+    check(lib().ct_brush_apply(ctx.handle, accel.handle, x, y, window, None))
 
 
 def bind_scene(scene: Scene, ctx: Ctx):

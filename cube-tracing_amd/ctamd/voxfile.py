@@ -65,7 +65,8 @@ def monument(n=48, seed=5, emissive_colors=(200, 201), fill=0.3):
     solid = (r < (n / 2) * (1 - z / (1.3 * n))) & (rng.random((n, n, n)) < fill + 0.5 * (z < 3))
     lamp = solid & (rng.random((n, n, n)) < 0.01)
     col = (1 + (z * 7 + (x // 4) * 3 + (y // 4)) % 190).astype(np.uint8)
-    col[lamp] = rng.choice(emissive_colors, size=int(lamp.sum()))
+    if emissive_colors:
+        col[lamp] = rng.choice(emissive_colors, size=int(lamp.sum()))
     zz, yy, xx = np.nonzero(solid)
     vox = list(zip(xx.tolist(), yy.tolist(), zz.tolist(), col[zz, yy, xx].tolist()))
     pal = [((i * 37) % 256, (i * 91) % 256, (i * 53) % 256, 255) for i in range(1, 257)]

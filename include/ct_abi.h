@@ -701,6 +701,43 @@ typedef struct ct_vox_desc {
 int ct_scene_from_vox(const void *data, size_t bytes, const ct_vox_desc *desc, ct_scene **out);
 
 /* =================================================================
+ * Voxel editing on the GPU: the squared deletion brush
+ * (squared_brush.slang:37-133), the rearrangement pass
+ * (rearrangement.slang:8-204) and check/process_voxel_modifications
+ * (accel_struct_mngr.cpp:2717-2871).
+ *
+ * Marks accumulate in the accel (an instance bitmask, a primitive-slot
+ * bitmask and the BRUSH_COUNTER pair) until processed.  Processing
+ * compacts every marked instance on the device -- the k-th marked local
+ * index below n - d receives the primitive and AABB of the k-th unmarked
+ * local index at or above n - d (n primitives, d marked) -- writes the
+ * remapping table (moved: new local index; deleted: ~0u), queues one
+ * DELETE_PRIMITIVE_BLAS_FROM_GPU task per instance in id order and clears
+ * the marks.  The next ct_accel_update_scene applies the tasks (BLAS
+ * rebuild, or instance deletion when nothing is left).  As in the
+ * reference, cube lights are not touched: deleting an emissive voxel this
+ * way leaves its LIGHT record in place.  All three calls synchronize the
+ * device first; `stream` is accepted for symmetry and unused.
+ * ================================================================= */
+/* entry_brush: every pixel of the square [x-window, x+window] x
+ * [y-window, y+window] (clipped to the last frame, in u32 arithmetic as
+ * the shader: a cursor closer than `window` to the left or top edge
+ * selects nothing) whose stored DI record (the previous DI,
+ * squared_brush.slang:95-96) hits the instance under (x, y) marks its
+ * voxel.  Nothing happens when (x, y) itself shows the sky. */
+int ct_brush_apply(ct_ctx *ctx, ct_accel *accel, uint32_t x, uint32_t y, uint32_t window, void *stream);
+/* delete_primitive_from_instance for a caller's list of voxels (instance
+ * id, primitive id within the instance); ids out of range are ignored. */
+int ct_brush_mark(ct_accel *accel, const ct_object_info *voxels, uint32_t n, void *stream);
+/* BRUSH_COUNTER: newly marked instances and primitives since the last processing. */
+int ct_brush_get_counters(ct_accel *accel, uint32_t *instance_count, uint32_t *primitive_count);
+/* check_voxel_modifications: does nothing (marks kept) unless the accel
+ * is idle (:2855).  *instances / *primitives (may be NULL) receive the
+ * number of tasks queued and of voxels they delete. */
+int ct_accel_process_voxel_modifications(ct_accel *accel, void *stream, uint32_t *instances,
+                                         uint32_t *primitives);
+
+/* =================================================================
  * Differentiable rendering.  No counterpart in the reference (its
  * renderer has no derivatives); the build's own adjoint of radiance with
  * respect to per-voxel albedo and emission.
