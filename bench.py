@@ -241,6 +241,50 @@ def vox_bench(frames=8, warmup=2, n=128, chunk=64):
             "mrays_s": round(rays / (sum(ms) / 1e3) / 1e6, 1)}
 
 
+def brush_bench(ctx, acc, w, f, window=16):
+    """GPU voxel deletion on the 1080p scene (last, as it edits the scene):
+    one squared-brush stroke of half-size `window` at the image centre of
+    the last frame, then processing (device compaction of the marked
+    instance, task queuing) and update_scene (BLAS rebuild); then a
+    bulk deletion of 1 % of the voxels of every BLAS marked from a list.
+    Wall times of each call."""
+    import numpy as np
+    import ctamd
+    ctx.render(acc, ctamd.frame_params(w, f))
+    ctx.stats()
+    acc.update_scene(True)  # idle
+    res = {}
+    t0 = time.perf_counter()
+    ctamd.brush_apply(ctx, acc, w.width // 2, w.height // 2, window)
+    _, marked = acc.brush_counters()
+    t1 = time.perf_counter()
+    tasks, vox = acc.process_voxel_modifications()
+    t2 = time.perf_counter()
+    acc.update_scene(True)
+    t3 = time.perf_counter()
+    acc.update_scene(True)  # settle
+    res["stroke"] = {"window": window, "voxels": vox, "instances": tasks, "mark_ms": round((t1 - t0) * 1e3, 3),
+                     "process_ms": round((t2 - t1) * 1e3, 3), "update_scene_ms": round((t3 - t2) * 1e3, 3)}
+    inst = acc.read_instances()
+    rng = np.random.default_rng(3)
+    marks = []
+    for i, r in enumerate(inst):
+        if r.primitive_count > 100:
+            k = r.primitive_count // 100
+            marks += [(i, int(p)) for p in rng.choice(r.primitive_count, k, replace=False)]
+    marks = np.array(marks, np.uint32)
+    t0 = time.perf_counter()
+    acc.brush_mark(marks)
+    t1 = time.perf_counter()
+    tasks, vox = acc.process_voxel_modifications()
+    t2 = time.perf_counter()
+    acc.update_scene(True)
+    t3 = time.perf_counter()
+    res["bulk_1pct"] = {"voxels": vox, "instances": tasks, "mark_ms": round((t1 - t0) * 1e3, 3),
+                        "process_ms": round((t2 - t1) * 1e3, 3), "update_scene_ms": round((t3 - t2) * 1e3, 3)}
+    return res
+
+
 def update_bench(acc, n_inst, reps=5):
     """Scene updates on the loaded accel (after every other measurement, as
     they move nothing but still rewrite it): UPDATE_BLAS_FROM_CPU tasks with
@@ -276,6 +320,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-view batched comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the inverse-rendering step timing")
     ap.add_argument("--no-vox", action="store_true", help="skip the .vox scene load and frame timing")
+    ap.add_argument("--no-brush", action="store_true", help="skip the GPU voxel-deletion timing")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (rocprof counter calibration)")
     args = ap.parse_args()
@@ -409,6 +454,10 @@ def main():
              "load_mvoxels_s": round(sc.info.primitive_count / load_s / 1e6, 1)}
     build.update(update_bench(acc, sc.info.instance_count))
     log(f"build: {build}")
+    brush = None
+    if not args.no_brush:
+        brush = brush_bench(ctx, acc, w, f)
+        log(f"brush: {brush}")
     ms_sorted = sorted(ms_frames)
     depth = w.restir.max_depth if w.restir.indirect else 0
     out = {
@@ -430,6 +479,7 @@ def main():
         "adjoint": adjoint,
         "build": build,
         "vox": vox,
+        "brush": brush,
     }
     print(json.dumps(out), flush=True)
 
