@@ -185,6 +185,143 @@ struct AdjPlan {
   uint32_t nblocks = 0;
 };
 
+// ----------------------------------------------------------------------
+// Plain (non-ReSTIR) path tracing with the material callables
+// (indirect_illumination_path_tracing, indirect_illumination.glsl:454-539;
+// scatter by type, rcall_scatter.glsl:11-88 via sample_material,
+// light.glsl:231-269).  The path is the adjoint's forward model with a
+// material per voxel (PRIMITIVE.material_index): emission is collected at
+// every vertex and the scattered direction comes from the material's
+// callable; the throughput is multiplied by the callable's attenuation
+// (Lambertian and constant medium: diffuse; metal: specular; dielectric: 1).
+
+struct PMat {  // a MATERIAL packed for the kernel
+  float4 diffuse_type;    // diffuse rgb, type bits
+  float4 specular_rough;  // specular rgb, roughness
+  float4 emission_ior;    // emission rgb, ior
+};
+
+struct PArgs {
+  AArgs A;
+  const PMat *mats;
+  const ct_primitive *prims;
+  uint32_t n_mats;
+};
+
+CT_D F3 reflection(F3 v, F3 n) { return sub(v, muls(n, 2.0f * dot(v, n))); }  // prng.glsl:209-211
+
+CT_D float pow5(float x) {  // pow(x, 5.0) of reflectance, as two squarings and a product
+  const float x2 = x * x;
+  return x2 * x2 * x;
+}
+
+CT_D float reflectance(float cosine, float ref_idx) {  // prng.glsl:214-218 (Schlick)
+  float r0 = (1.0f - ref_idx) / (1.0f + ref_idx);
+  r0 = r0 * r0;
+  return r0 + (1.0f - r0) * pow5(1.0f - cosine);
+}
+
+CT_D F3 refraction(F3 uv, F3 n, float eta) {  // prng.glsl:220-225
+  const float cos_theta = gmin(dot(neg(uv), n), 1.0f);
+  const F3 perp = muls(add(uv, muls(n, cos_theta)), eta);
+  const float ls = perp.x * perp.x + perp.y * perp.y + perp.z * perp.z;
+  const F3 par = muls(n, -sqrtf(fabsf(1.0f - ls)));
+  return add(par, perp);
+}
+
+// random_unit_vector (prng.glsl:63-74): rejection sampling in the unit
+// ball, bounded at 64 draws (then the normal; the chance of reaching the
+// bound is below 1e-20).
+CT_D F3 random_unit_vector(uint32_t &seed, F3 fallback) {
+  for (int k = 0; k < 64; ++k) {
+    const float x = -1.0f + rnd(seed) * 2.0f;
+    const float y = -1.0f + rnd(seed) * 2.0f;
+    const float z = -1.0f + rnd(seed) * 2.0f;
+    const float l2 = x * x + y * y + z * z;
+    if (l2 >= 1.0f) continue;
+    return normalize(f3(x, y, z));
+  }
+  return fallback;
+}
+
+// The callable of the material's type: the new direction, the attenuation,
+// and false when the path is absorbed (a metal ray below the surface).
+CT_D bool scatter_material(const PMat &m, F3 dir, F3 nrm, uint32_t &seed, F3 &sd, F3 &att) {
+  const uint32_t type = __float_as_uint(m.diffuse_type.w) & CT_MATERIAL_TYPE_MASK;
+  if (type == 1u) {  // METAL
+    const float r = m.specular_rough.w;
+    const F3 refl = reflection(dir, nrm);
+    sd = add(muls(random_cosine_direction(seed, refl), gmin(r, 1.0f)), muls(refl, 1.0f - r));
+    att = f3(m.specular_rough.x, m.specular_rough.y, m.specular_rough.z);
+    return dot(sd, nrm) > 0.0f;
+  }
+  if (type == 2u) {  // DIELECTRIC
+    F3 n = nrm;
+    float eta = m.emission_ior.w;
+    if (dot(dir, n) > 0.0f) {
+      n = neg(n);
+      eta = 1.0f / eta;
+    }
+    const float cos_theta = gmin(dot(neg(dir), n), 1.0f);
+    const float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+    const bool cannot_refract = eta * sin_theta > 1.0f;
+    if (cannot_refract || reflectance(cos_theta, eta) > rnd(seed)) sd = reflection(dir, n);
+    else sd = refraction(dir, n, eta);
+    att = f3(1.0f, 1.0f, 1.0f);
+    return true;
+  }
+  if (type == 3u) {  // CONSTANT_MEDIUM
+    sd = random_unit_vector(seed, nrm);
+    if (normal_near_zero(sd)) sd = nrm;
+  } else {  // LAMBERTIAN
+    sd = random_cosine_direction(seed, nrm);
+    if (normal_near_zero(sd)) sd = nrm;
+  }
+  att = f3(m.diffuse_type.x, m.diffuse_type.y, m.diffuse_type.z);
+  return true;
+}
+
+CT_D F3 path_sample(const PArgs &P, const AView &V, uint32_t x, uint32_t y, uint32_t seed, uint32_t *stk) {
+  const AArgs &A = P.A;
+  F3 o, d;
+  view_ray(V, x, y, seed, o, d);
+  F3 beta = f3(1.0f, 1.0f, 1.0f), L = f3(0.0f, 0.0f, 0.0f);
+  for (uint32_t k = 0; k <= A.max_depth; ++k) {
+    uint32_t g;
+    F3 pos, nrm;
+    if (!segment(A, o, d, stk, g, pos, nrm)) break;
+    const uint32_t mi = P.prims[g].material_index;
+    if (mi >= P.n_mats) break;  // no material: absorbs
+    const PMat m = P.mats[mi];
+    L = add(L, mulv(beta, f3(m.emission_ior.x, m.emission_ior.y, m.emission_ior.z)));
+    if (k == A.max_depth) break;
+    F3 sd, att;
+    if (!scatter_material(m, d, nrm, seed, sd, att)) break;
+    beta = mulv(beta, att);
+    const uint32_t type = __float_as_uint(m.diffuse_type.w) & CT_MATERIAL_TYPE_MASK;
+    const F3 side = (type == 0u || dot(sd, nrm) >= 0.0f) ? nrm : neg(nrm);
+    d = sd;
+    o = compute_ray_origin(compute_ray_origin(pos, side), side);
+  }
+  return L;
+}
+
+__global__ void __launch_bounds__(kBlockA) k_path_trace(PArgs P, float *image) {
+  __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlockA];
+  const AArgs &A = P.A;
+  const uint2 b = A.blocks[blockIdx.x];
+  const AView &V = A.views[b.x];
+  const uint32_t p = b.y + threadIdx.x;
+  if (p >= V.width * V.height) return;
+  const uint32_t x = p % V.width, y = p / V.width, gpix = V.base + p;
+  F3 sum = f3(0, 0, 0);
+  for (uint32_t s = 0; s < A.spp; ++s) sum = add(sum, path_sample(P, V, x, y, sample_seed(gpix, A.seed, s), stk + threadIdx.x));
+  const F3 I = divs(sum, (float)A.spp);
+  image[3 * (size_t)gpix] = I.x;
+  image[3 * (size_t)gpix + 1] = I.y;
+  image[3 * (size_t)gpix + 2] = I.z;
+}
+
 int plan(Accel &a, const ct_adjoint_desc *d, const ct_adjoint_view *views, hipStream_t st, AdjPlan &P, AArgs &A) {
   if (!d || !views || d->n_views == 0) return fail(CT_E_INVALID, "adjoint: no views");
   if (d->spp == 0 || d->max_depth > CT_ADJOINT_MAX_DEPTH) return fail(CT_E_INVALID, "adjoint: spp 0 or max_depth too large");
@@ -241,6 +378,40 @@ int ct_adjoint_render(ct_accel *a, const ct_adjoint_desc *d, const ct_adjoint_vi
   A.albedo = albedo;
   A.emission = emission;
   hipLaunchKernelGGL(k_adj_forward, dim3(P.nblocks), dim3(kBlockA), 0, st, A, image);
+  CT_HIP(hipGetLastError());
+  CT_HIP(hipStreamSynchronize(st));  // the plan's buffers are released on return
+  return CT_OK;
+}
+
+int ct_path_trace(ct_accel *a, const ct_adjoint_desc *d, const ct_adjoint_view *views, const ct_material *materials,
+                  uint32_t material_count, float *image, void *stream) {
+  if (!a || !image || (material_count && !materials)) return fail(CT_E_INVALID, "ct_path_trace: null argument");
+  CT_HIP(hipSetDevice(a->device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : a->stream;
+  AdjPlan P;
+  PArgs PA;
+  int rc;
+  if ((rc = plan(*a, d, views, st, P, PA.A))) return rc;
+  std::vector<PMat> pm(std::max<uint32_t>(material_count, 1));
+  for (uint32_t i = 0; i < material_count; ++i) {
+    const ct_material &m = materials[i];
+    const uint32_t type = m.type & CT_MATERIAL_TYPE_MASK;
+    if (type > 3u) return fail(CT_E_INVALID, "ct_path_trace: unknown material type");
+    float tb;
+    std::memcpy(&tb, &m.type, 4);
+    pm[i].diffuse_type = make_float4(m.diffuse[0], m.diffuse[1], m.diffuse[2], tb);
+    pm[i].specular_rough = make_float4(m.specular[0], m.specular[1], m.specular[2], m.roughness);
+    pm[i].emission_ior = make_float4(m.emission[0], m.emission[1], m.emission[2], m.ior);
+  }
+  DevBuf dm;
+  if ((rc = dm.reserve(sizeof(PMat) * pm.size()))) return rc;
+  CT_HIP(hipMemcpyAsync(dm.p, pm.data(), sizeof(PMat) * pm.size(), hipMemcpyHostToDevice, st));
+  PA.mats = dm.as<PMat>();
+  PA.prims = a->set.prims.as<ct_primitive>();
+  PA.n_mats = material_count;
+  PA.A.albedo = nullptr;
+  PA.A.emission = nullptr;
+  hipLaunchKernelGGL(k_path_trace, dim3(P.nblocks), dim3(kBlockA), 0, st, PA, image);
   CT_HIP(hipGetLastError());
   CT_HIP(hipStreamSynchronize(st));  // the plan's buffers are released on return
   return CT_OK;

@@ -46,6 +46,8 @@ CT_AS_IDLE, CT_AS_UPDATING, CT_AS_SWITCHING, CT_AS_SETTLING = 0, 1, 2, 4
 
 CT_LIGHT_NONE, CT_LIGHT_POINT, CT_LIGHT_CUBE, CT_LIGHT_ENV_MAP = 0, 1, 2, 5
 
+CT_MATERIAL_LAMBERTIAN, CT_MATERIAL_METAL, CT_MATERIAL_DIELECTRIC, CT_MATERIAL_CONSTANT_MEDIUM = 0, 1, 2, 3
+
 CT_TASK_BUILD_BLAS_FROM_CPU = 0
 CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_CPU = 1
 CT_TASK_DELETE_BLAS_FROM_CPU = 2
@@ -334,6 +336,8 @@ _SIGS = {
     "ct_brush_mark": ([C.c_void_p, C.POINTER(ObjectInfo), u32, C.c_void_p], C.c_int),
     "ct_brush_get_counters": ([C.c_void_p, C.POINTER(u32), C.POINTER(u32)], C.c_int),
     "ct_accel_process_voxel_modifications": ([C.c_void_p, C.c_void_p, C.POINTER(u32), C.POINTER(u32)], C.c_int),
+    "ct_path_trace": ([C.c_void_p, C.POINTER(AdjointDesc), C.POINTER(AdjointView), C.POINTER(Material), u32,
+                       C.c_void_p, C.c_void_p], C.c_int),
     "ct_scene_load_transformed": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(f32), C.c_void_p], C.c_int),
     "ct_scene_from_vox": ([C.c_char_p, C.c_size_t, C.POINTER(VoxDesc), C.POINTER(C.c_void_p)], C.c_int),
 }

@@ -95,6 +95,21 @@ class DiffRenderer:
         return ga, ge
 
 
+def path_trace(accel: Accel, views, materials, spp=16, max_depth=3, seed=0xC4):
+    """Plain path tracing with the material callables (ct_path_trace) on the
+    current torch device and stream; returns the image [sum of H*W][3]."""
+    import torch
+    from . import Material
+    vv = (AdjointView * len(views))(*views)
+    desc = AdjointDesc(len(views), spp, max_depth, seed)
+    mm = (Material * max(len(materials), 1))(*materials)
+    n = sum(v.width * v.height for v in views)
+    img = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+    check(lib().ct_path_trace(accel.handle, C.byref(desc), vv, mm, len(materials), _ptr(img),
+                              C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    return img
+
+
 def _function():
     import torch
 

@@ -56,6 +56,9 @@ extern "C" {
 
 /* MATERIAL.type (shared.inl:295-301) */
 #define CT_MATERIAL_LAMBERTIAN 0u
+#define CT_MATERIAL_METAL 1u           /* shared.inl:295-298 */
+#define CT_MATERIAL_DIELECTRIC 2u
+#define CT_MATERIAL_CONSTANT_MEDIUM 3u
 #define CT_MATERIAL_TYPE_MASK 0xFFu
 
 /* ------------------------------------------- boundary structs (AoS) */
@@ -785,6 +788,24 @@ int ct_adjoint_backward(ct_accel *accel, const ct_adjoint_desc *desc,
                         const ct_adjoint_view *views, const float *albedo,
                         const float *emission, const float *grad_image,
                         float *grad_albedo, float *grad_emission, void *stream);
+
+/* Plain (non-ReSTIR) path tracing with the material callables
+ * (indirect_illumination_path_tracing, indirect_illumination.glsl:454-539;
+ * METAL / DIELECTRIC / CONSTANT_MEDIUM / LAMBERTIAN scatter,
+ * rcall_scatter.glsl:11-88).  The camera rays, samples and segments are
+ * those of ct_adjoint_render; at every vertex the voxel's material
+ * (materials[PRIMITIVE.material_index]) adds its emission times the
+ * throughput, then its callable gives the next direction and the
+ * throughput is multiplied by the attenuation: LAMBERTIAN cosine-weighted
+ * around the face normal, times diffuse; METAL the mirror direction blended
+ * with a cosine lobe by roughness, times specular, absorbed when it points
+ * below the face; DIELECTRIC Schlick-weighted reflection or Snell
+ * refraction with ior (1/ior leaving the voxel), times 1; CONSTANT_MEDIUM a
+ * uniform direction, times diffuse.  A voxel whose material index is out of
+ * range absorbs.  image: device pointer, views one after another,
+ * [H][W][3].  Calls are ordered on `stream` (null: the accel's stream). */
+int ct_path_trace(ct_accel *accel, const ct_adjoint_desc *desc, const ct_adjoint_view *views,
+                  const ct_material *materials, uint32_t material_count, float *image, void *stream);
 
 #ifdef __cplusplus
 }

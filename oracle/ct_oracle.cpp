@@ -2169,6 +2169,121 @@ static V3 adj_trace(const Scene &sc, const ct_adjoint_desc &d, const ct_adjoint_
   return L;
 }
 
+// ======================================================= plain path tracing
+// indirect_illumination_path_tracing (indirect_illumination.glsl:454-539)
+// with the material callables (rcall_scatter.glsl:11-88): the adjoint's
+// camera rays and segments, a MATERIAL per voxel, emission at every
+// vertex, the callable's direction and attenuation.
+
+static V3 reflection(V3 v, V3 n) { return sub(v, muls(n, 2.0f * dot(v, n))); }  // prng.glsl:209-211
+
+static float pow5(float x) {
+  const float x2 = x * x;
+  return x2 * x2 * x;
+}
+
+static float reflectance(float cosine, float ref_idx) {  // prng.glsl:214-218
+  float r0 = (1.0f - ref_idx) / (1.0f + ref_idx);
+  r0 = r0 * r0;
+  return r0 + (1.0f - r0) * pow5(1.0f - cosine);
+}
+
+static V3 refraction(V3 uv, V3 n, float eta) {  // prng.glsl:220-225
+  const float cos_theta = std::fmin(dot(neg(uv), n), 1.0f);
+  const V3 perp = muls(add(uv, muls(n, cos_theta)), eta);
+  const float ls = perp.x * perp.x + perp.y * perp.y + perp.z * perp.z;
+  const V3 par = muls(n, -std::sqrt(std::fabs(1.0f - ls)));
+  return add(par, perp);
+}
+
+static V3 random_unit_vector(uint32_t &seed, V3 fallback) {  // prng.glsl:63-74, at most 64 draws
+  for (int k = 0; k < 64; ++k) {
+    const float x = -1.0f + rnd(seed) * 2.0f;
+    const float y = -1.0f + rnd(seed) * 2.0f;
+    const float z = -1.0f + rnd(seed) * 2.0f;
+    const float l2 = x * x + y * y + z * z;
+    if (l2 >= 1.0f) continue;
+    return normalize(v3(x, y, z));
+  }
+  return fallback;
+}
+
+static bool near_zero(V3 v) { return std::fabs(v.x) < 1e-8f && std::fabs(v.y) < 1e-8f && std::fabs(v.z) < 1e-8f; }
+
+static bool scatter_material(const ct_material &m, V3 dir, V3 nrm, uint32_t &seed, V3 &sd, V3 &att) {
+  const uint32_t type = m.type & CT_MATERIAL_TYPE_MASK;
+  if (type == CT_MATERIAL_METAL) {
+    const V3 refl = reflection(dir, nrm);
+    sd = add(muls(random_cosine_direction(seed, refl), std::fmin(m.roughness, 1.0f)), muls(refl, 1.0f - m.roughness));
+    att = load3(m.specular);
+    return dot(sd, nrm) > 0.0f;
+  }
+  if (type == CT_MATERIAL_DIELECTRIC) {
+    V3 n = nrm;
+    float eta = m.ior;
+    if (dot(dir, n) > 0.0f) {
+      n = neg(n);
+      eta = 1.0f / eta;
+    }
+    const float cos_theta = std::fmin(dot(neg(dir), n), 1.0f);
+    const float sin_theta = std::sqrt(1.0f - cos_theta * cos_theta);
+    const bool cannot_refract = eta * sin_theta > 1.0f;
+    if (cannot_refract || reflectance(cos_theta, eta) > rnd(seed)) sd = reflection(dir, n);
+    else sd = refraction(dir, n, eta);
+    att = v3(1.0f, 1.0f, 1.0f);
+    return true;
+  }
+  if (type == CT_MATERIAL_CONSTANT_MEDIUM) {
+    sd = random_unit_vector(seed, nrm);
+    if (near_zero(sd)) sd = nrm;
+  } else {
+    sd = random_cosine_direction(seed, nrm);
+    if (near_zero(sd)) sd = nrm;
+  }
+  att = load3(m.diffuse);
+  return true;
+}
+
+static V3 path_trace_sample(const Scene &sc, const ct_adjoint_desc &d, const ct_adjoint_view &V, uint32_t x, uint32_t y,
+                            uint32_t seed, const ct_material *mats, uint32_t n_mats) {
+  const float jx = rnd(seed);
+  const float jy = rnd(seed);
+  float pcx = (float)x + jx, pcy = (float)y + jy;
+  float ux = pcx / (float)V.width, uy = pcy / (float)V.height;
+  float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
+  float org[4] = {0, 0, 0, 1}, ov[4];
+  mat4_mul_vec(V.inv_view, org, ov);
+  float tv[4] = {dx, dy, 1.0f, 1.0f}, t4[4];
+  mat4_mul_vec(V.inv_proj, tv, t4);
+  V3 nt = normalize(v3(t4[0], t4[1], t4[2]));
+  float dv[4] = {nt.x, nt.y, nt.z, 0.0f}, d4[4];
+  mat4_mul_vec(V.inv_view, dv, d4);
+  V3 o = v3(ov[0], ov[1], ov[2]), dir = v3(d4[0], d4[1], d4[2]);
+  V3 beta = v3(1, 1, 1), L = v3(0, 0, 0);
+  for (uint32_t k = 0; k <= d.max_depth; ++k) {
+    V3 ro = compute_ray_origin(o, dir);
+    TraceCount cnt;
+    ClosestState st = trace_bounce(sc, ro, dir, cnt);
+    if (st.inst == kInvalid) break;
+    float t;
+    V3 pos, nrm;
+    hit_from_ray(sc, st.inst, st.prim, ro, dir, t, pos, nrm);
+    const uint32_t g = sc.inst[st.inst].first_primitive_index + st.prim;
+    const uint32_t mi = sc.prims[g].material_index;
+    if (mi >= n_mats) break;
+    const ct_material &m = mats[mi];
+    L = add(L, mulv(beta, load3(m.emission)));
+    if (k == d.max_depth) break;
+    V3 sd, att;
+    if (!scatter_material(m, dir, nrm, seed, sd, att)) break;
+    beta = mulv(beta, att);
+    const V3 side = ((m.type & CT_MATERIAL_TYPE_MASK) == CT_MATERIAL_LAMBERTIAN || dot(sd, nrm) >= 0.0f) ? nrm : neg(nrm);
+    dir = sd;
+    o = compute_ray_origin(compute_ray_origin(pos, side), side);
+  }
+  return L;
+}
+
 // ================================================================ C API
 struct oc_scene : Scene {};
 struct oc_frame : Frame {};
@@ -2412,6 +2527,40 @@ int oc_adjoint_render(oc_scene *scene, const ct_adjoint_desc *d, const ct_adjoin
   }
   return CT_OK;
 }
+
+int oc_path_trace(oc_scene *scene, const ct_adjoint_desc *d, const ct_adjoint_view *views, const ct_material *mats,
+                  uint32_t n_mats, float *image) {
+  if (!scene || !d || !views || d->spp == 0 || d->max_depth > CT_ADJOINT_MAX_DEPTH) return CT_E_INVALID;
+  uint64_t base = 0;
+  for (uint32_t v = 0; v < d->n_views; ++v) {
+    const ct_adjoint_view &V = views[v];
+    const int64_t np = (int64_t)V.width * V.height;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t p = 0; p < np; ++p) {
+      const uint32_t x = (uint32_t)(p % V.width), y = (uint32_t)(p / V.width), gpix = (uint32_t)(base + p);
+      V3 sum = v3(0, 0, 0);
+      for (uint32_t s = 0; s < d->spp; ++s)
+        sum = add(sum, path_trace_sample(*scene, *d, V, x, y, tea(tea(gpix, d->seed), s), mats, n_mats));
+      store3(&image[3 * (size_t)gpix], divs(sum, (float)d->spp));
+    }
+    base += (uint64_t)np;
+  }
+  return CT_OK;
+}
+
+void oc_scatter_material(const ct_material *m, const float dir[3], const float nrm[3], uint32_t *seed, float sd[3],
+                         float att[3], int *ok) {
+  V3 s, a;
+  *ok = scatter_material(*m, load3(dir), load3(nrm), *seed, s, a) ? 1 : 0;
+  store3(sd, s);
+  store3(att, a);
+}
+
+void oc_refraction(const float uv[3], const float n[3], float eta, float out[3]) {
+  store3(out, refraction(load3(uv), load3(n), eta));
+}
+
+float oc_reflectance(float cosine, float ref_idx) { return reflectance(cosine, ref_idx); }
 
 // Gradients accumulated in double, then added into the float outputs.
 int oc_adjoint_backward(oc_scene *scene, const ct_adjoint_desc *d, const ct_adjoint_view *views,

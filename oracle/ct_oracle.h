@@ -94,6 +94,12 @@ void oc_random_cosine_direction(uint32_t *seed, const float n[3], float out[3]);
 
 /* The differentiable path tracer of ct_adjoint_render / _backward on host
  * arrays (gradients accumulated in double, then added). */
+int oc_path_trace(oc_scene *scene, const ct_adjoint_desc *desc, const ct_adjoint_view *views,
+                  const ct_material *materials, uint32_t material_count, float *image);
+void oc_scatter_material(const ct_material *m, const float dir[3], const float nrm[3], uint32_t *seed,
+                         float sd[3], float att[3], int *ok);
+void oc_refraction(const float uv[3], const float n[3], float eta, float out[3]);
+float oc_reflectance(float cosine, float ref_idx);
 int oc_adjoint_render(oc_scene *scene, const ct_adjoint_desc *desc,
                       const ct_adjoint_view *views, const float *albedo,
                       const float *emission, float *image);

@@ -66,6 +66,10 @@ def lib():
             "oc_adjoint_backward": ([C.c_void_p, P(AdjointDesc), P(AdjointView), C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
             "oc_random_cosine_direction": ([P(u32), P(f32), P(f32)], None),
+            "oc_path_trace": ([C.c_void_p, P(AdjointDesc), P(AdjointView), P(Material), u32, C.c_void_p], C.c_int),
+            "oc_scatter_material": ([P(Material), P(f32), P(f32), P(u32), P(f32), P(f32), P(C.c_int)], None),
+            "oc_refraction": ([P(f32), P(f32), f32, P(f32)], None),
+            "oc_reflectance": ([f32, f32], f32),
             "oc_frame_create": ([C.c_void_p, u32, u32], C.c_void_p),
             "oc_frame_set_scene": ([C.c_void_p, C.c_void_p], C.c_int),
             "oc_scene_set_previous": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, u32, C.c_void_p, u32],
@@ -289,6 +293,16 @@ class OScene:
         img = np.zeros((n, 3), np.float32)
         vv = (AdjointView * len(views))(*views)
         rc = lib().oc_adjoint_render(self.handle, C.byref(desc), vv, a.ctypes.data, e.ctypes.data, img.ctypes.data)
+        assert rc == 0
+        return img
+
+    def path_trace(self, desc, views, materials):
+        """Plain path tracing with the material callables: image [sum of H*W][3]."""
+        n = sum(v.width * v.height for v in views)
+        img = np.zeros((n, 3), np.float32)
+        vv = (AdjointView * len(views))(*views)
+        mm = (Material * max(len(materials), 1))(*materials)
+        rc = lib().oc_path_trace(self.handle, C.byref(desc), vv, mm, len(materials), img.ctypes.data)
         assert rc == 0
         return img
 
