@@ -241,6 +241,43 @@ def vox_bench(frames=8, warmup=2, n=128, chunk=64):
             "mrays_s": round(rays / (sum(ms) / 1e3) / 1e6, 1)}
 
 
+def path_trace_bench(acc, sc, w, spp=1, depth=3, reps=3):
+    """Plain path tracing with the material callables on the 1080p scene:
+    its materials with every 4th voxel metal, dielectric and constant
+    medium in turn (the rest Lambertian), one 1920x1080 view from the
+    workload's first camera, `spp` samples and `depth` bounces per pixel
+    (after one untimed call that uploads the materials)."""
+    import ctamd
+    from ctamd import adjoint as AD
+    import numpy as np
+    n = sc.info.material_count
+    mats = (ctamd.Material * n).from_buffer_copy(sc.materials)
+    u = np.frombuffer(mats, np.uint32).reshape(n, 21)  # MATERIAL as 21 words
+    f = u.view(np.float32)
+    kind = np.arange(n, dtype=np.uint32) % 4
+    u[:, 0] = kind
+    metal, dielectric = kind == ctamd.CT_MATERIAL_METAL, kind == ctamd.CT_MATERIAL_DIELECTRIC
+    f[metal, 7:10] = f[metal, 4:7]  # specular = diffuse
+    f[metal, 17] = 0.3              # roughness
+    f[dielectric, 18] = 1.5         # ior
+    p = ctamd.frame_params(w, 0)
+    v = ctamd.AdjointView()
+    v.width, v.height = w.width, w.height
+    v.inv_view[:] = p.inv_view[:]
+    v.inv_proj[:] = p.inv_proj[:]
+    import torch
+    AD.path_trace(acc, [v], mats, spp=spp, max_depth=depth)
+    torch.cuda.synchronize()  # untimed first call
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        img = AD.path_trace(acc, [v], None, spp=spp, max_depth=depth)  # materials kept in the accel
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    samples = w.width * w.height * spp
+    return {"resolution": [w.width, w.height], "spp": spp, "bounces": depth, "ms": round(ms, 2),
+            "msamples_s": round(samples / ms / 1e3, 1), "mean_radiance": round(float(img.mean()), 5)}
+
+
 def brush_bench(ctx, acc, w, f, window=16):
     """GPU voxel deletion on the 1080p scene (last, as it edits the scene):
     one squared-brush stroke of half-size `window` at the image centre of
@@ -319,7 +356,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-view batched comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the inverse-rendering step timing")
-    ap.add_argument("--no-vox", action="store_true", help="skip the .vox scene load and frame timing")
+    ap.add_argument("--no-vox", action="store_true",
+                    help="skip the .vox scene and material path-tracing timings")
     ap.add_argument("--no-brush", action="store_true", help="skip the GPU voxel-deletion timing")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (rocprof counter calibration)")
@@ -454,6 +492,10 @@ def main():
              "load_mvoxels_s": round(sc.info.primitive_count / load_s / 1e6, 1)}
     build.update(update_bench(acc, sc.info.instance_count))
     log(f"build: {build}")
+    ptrace = None
+    if not args.no_vox:
+        ptrace = path_trace_bench(acc, sc, w)
+        log(f"path_trace: {ptrace}")
     brush = None
     if not args.no_brush:
         brush = brush_bench(ctx, acc, w, f)
@@ -479,6 +521,7 @@ def main():
         "adjoint": adjoint,
         "build": build,
         "vox": vox,
+        "path_trace": ptrace,
         "brush": brush,
     }
     print(json.dumps(out), flush=True)

@@ -385,30 +385,34 @@ int ct_adjoint_render(ct_accel *a, const ct_adjoint_desc *d, const ct_adjoint_vi
 
 int ct_path_trace(ct_accel *a, const ct_adjoint_desc *d, const ct_adjoint_view *views, const ct_material *materials,
                   uint32_t material_count, float *image, void *stream) {
-  if (!a || !image || (material_count && !materials)) return fail(CT_E_INVALID, "ct_path_trace: null argument");
+  if (!a || !image) return fail(CT_E_INVALID, "ct_path_trace: null argument");
+  if (!materials && material_count) return fail(CT_E_INVALID, "ct_path_trace: null materials");
+  if (!materials && !a->pt_mats.p) return fail(CT_E_STATE, "ct_path_trace: no materials uploaded yet");
   CT_HIP(hipSetDevice(a->device));
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : a->stream;
   AdjPlan P;
   PArgs PA;
   int rc;
   if ((rc = plan(*a, d, views, st, P, PA.A))) return rc;
-  std::vector<PMat> pm(std::max<uint32_t>(material_count, 1));
-  for (uint32_t i = 0; i < material_count; ++i) {
-    const ct_material &m = materials[i];
-    const uint32_t type = m.type & CT_MATERIAL_TYPE_MASK;
-    if (type > 3u) return fail(CT_E_INVALID, "ct_path_trace: unknown material type");
-    float tb;
-    std::memcpy(&tb, &m.type, 4);
-    pm[i].diffuse_type = make_float4(m.diffuse[0], m.diffuse[1], m.diffuse[2], tb);
-    pm[i].specular_rough = make_float4(m.specular[0], m.specular[1], m.specular[2], m.roughness);
-    pm[i].emission_ior = make_float4(m.emission[0], m.emission[1], m.emission[2], m.ior);
+  if (materials) {  // pack and keep them for later calls with materials == NULL
+    std::vector<PMat> pm(std::max<uint32_t>(material_count, 1));
+    for (uint32_t i = 0; i < material_count; ++i) {
+      const ct_material &m = materials[i];
+      if ((m.type & CT_MATERIAL_TYPE_MASK) > 3u) return fail(CT_E_INVALID, "ct_path_trace: unknown material type");
+      float tb;
+      std::memcpy(&tb, &m.type, 4);
+      pm[i].diffuse_type = make_float4(m.diffuse[0], m.diffuse[1], m.diffuse[2], tb);
+      pm[i].specular_rough = make_float4(m.specular[0], m.specular[1], m.specular[2], m.roughness);
+      pm[i].emission_ior = make_float4(m.emission[0], m.emission[1], m.emission[2], m.ior);
+    }
+    if ((rc = a->pt_mats.reserve(sizeof(PMat) * pm.size()))) return rc;
+    CT_HIP(hipMemcpyAsync(a->pt_mats.p, pm.data(), sizeof(PMat) * pm.size(), hipMemcpyHostToDevice, st));
+    CT_HIP(hipStreamSynchronize(st));  // `pm` is freed on return
+    a->pt_mat_count = material_count;
   }
-  DevBuf dm;
-  if ((rc = dm.reserve(sizeof(PMat) * pm.size()))) return rc;
-  CT_HIP(hipMemcpyAsync(dm.p, pm.data(), sizeof(PMat) * pm.size(), hipMemcpyHostToDevice, st));
-  PA.mats = dm.as<PMat>();
+  PA.mats = a->pt_mats.as<PMat>();
   PA.prims = a->set.prims.as<ct_primitive>();
-  PA.n_mats = material_count;
+  PA.n_mats = a->pt_mat_count;
   PA.A.albedo = nullptr;
   PA.A.emission = nullptr;
   hipLaunchKernelGGL(k_path_trace, dim3(P.nblocks), dim3(kBlockA), 0, st, PA, image);

@@ -184,6 +184,9 @@ struct Accel {
   // voxel-deletion marks (squared_brush.slang): instance bits, slot bits, counters
   DevBuf brush_imask, brush_pmask, brush_counters;
   uint32_t brush_imask_words = 0, brush_pmask_words = 0;
+  // materials of ct_path_trace, packed, kept between calls
+  DevBuf pt_mats;
+  uint32_t pt_mat_count = 0;
   BuildScratch scratch;
   uint32_t status = CT_AS_IDLE;
   uint32_t generation = 0;

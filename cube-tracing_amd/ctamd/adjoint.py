@@ -97,15 +97,22 @@ class DiffRenderer:
 
 def path_trace(accel: Accel, views, materials, spp=16, max_depth=3, seed=0xC4):
     """Plain path tracing with the material callables (ct_path_trace) on the
-    current torch device and stream; returns the image [sum of H*W][3]."""
+    current torch device and stream; returns the image [sum of H*W][3].
+    `materials`: a list of Material, a ctypes Material array, or None to
+    reuse the set the previous call uploaded."""
     import torch
     from . import Material
     vv = (AdjointView * len(views))(*views)
     desc = AdjointDesc(len(views), spp, max_depth, seed)
-    mm = (Material * max(len(materials), 1))(*materials)
+    if materials is None:  # the set uploaded by an earlier call
+        mm, nm = None, 0
+    elif isinstance(materials, C.Array):
+        mm, nm = materials, len(materials)
+    else:
+        mm, nm = (Material * max(len(materials), 1))(*materials), len(materials)
     n = sum(v.width * v.height for v in views)
     img = torch.empty((n, 3), dtype=torch.float32, device="cuda")
-    check(lib().ct_path_trace(accel.handle, C.byref(desc), vv, mm, len(materials), _ptr(img),
+    check(lib().ct_path_trace(accel.handle, C.byref(desc), vv, mm, nm, _ptr(img),
                               C.c_void_p(torch.cuda.current_stream().cuda_stream)))
     return img
 

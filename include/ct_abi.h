@@ -803,7 +803,9 @@ int ct_adjoint_backward(ct_accel *accel, const ct_adjoint_desc *desc,
  * refraction with ior (1/ior leaving the voxel), times 1; CONSTANT_MEDIUM a
  * uniform direction, times diffuse.  A voxel whose material index is out of
  * range absorbs.  image: device pointer, views one after another,
- * [H][W][3].  Calls are ordered on `stream` (null: the accel's stream). */
+ * [H][W][3].  The materials are packed and kept in the accel: a later call
+ * may pass materials = NULL (and material_count 0) to reuse them.  Calls
+ * are ordered on `stream` (null: the accel's stream). */
 int ct_path_trace(ct_accel *accel, const ct_adjoint_desc *desc, const ct_adjoint_view *views,
                   const ct_material *materials, uint32_t material_count, float *image, void *stream);
 

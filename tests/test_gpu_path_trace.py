@@ -68,3 +68,19 @@ def test_lambertian_path_trace_equals_adjoint_forward(gpu):
     assert np.array_equal(sc.primitives[:, 0], np.arange(sc.info.primitive_count))
     p = AD.path_trace(acc, views, mats, spp=w.spp, max_depth=w.max_depth, seed=w.target_seed)
     assert np.array_equal(bits(a.cpu().numpy()), bits(p.cpu().numpy()))
+
+
+def test_materials_kept_between_calls(gpu):
+    w = ctamd.small_plain(width=24, height=16)
+    sc = ctamd.Scene(ctamd.scene_desc(w))
+    acc = ctamd.Accel(max_instances=8, max_primitives=sc.info.primitive_count + 1024,
+                      max_cube_lights=sc.info.cube_light_count + 8)
+    ctx = ctamd.Ctx(8, 8)
+    ctamd.load_scene(sc, acc, ctx)
+    views = AD.orbit_views(1, 24, 16, (0, 0, 0), 3.0, 0.5)
+    with pytest.raises(ctamd.CtError):
+        AD.path_trace(acc, views, None, spp=1, max_depth=2)  # nothing uploaded yet
+    mats = _mixed_materials(sc)
+    a = AD.path_trace(acc, views, mats, spp=2, max_depth=2).cpu().numpy()
+    b = AD.path_trace(acc, views, None, spp=2, max_depth=2).cpu().numpy()
+    assert np.array_equal(bits(a), bits(b))
