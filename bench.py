@@ -123,8 +123,12 @@ def batched_bench(frames=5, warmup=2, n_views=64):
     params = [[ctamd.frame_params(w, f) for _, w in views] for f in range(warmup + frames)]
     tune = b.autotune(params[0], repeats=2)
     log(f"batched: autotuned {tune.groups} view groups, traversal grid {tune.blocks_per_cu} blocks/CU ({tune.ms:.2f} ms)")
-    # plain: one ct_frame_render per view and frame
-    t_plain, t_batch, rays = 0.0, 0.0, 0
+    # plain: one ct_frame_render per view and frame, issued back to back (the
+    # views' own streams overlap on the GPU), then the same with each view
+    # finished before the next starts, as a renderer drawing one view per
+    # frame does (draw(), main.cpp:1436).  Both loops start from the same
+    # history: the serial one renders the same frame numbers on copies.
+    t_plain, t_serial, t_batch, rays = 0.0, 0.0, 0.0, 0
     for f in range(warmup + frames):
         hip.hipDeviceSynchronize()
         t1 = time.perf_counter()
@@ -133,6 +137,19 @@ def batched_bench(frames=5, warmup=2, n_views=64):
         hip.hipDeviceSynchronize()
         if f >= warmup:
             t_plain += time.perf_counter() - t1
+    ctx_serial = []
+    for k, w in views:
+        c = ctamd.Ctx(w.width, w.height)
+        ctamd.bind_scene(built[k][0], c)
+        ctx_serial.append(c)
+    for f in range(warmup + frames):
+        hip.hipDeviceSynchronize()
+        t1 = time.perf_counter()
+        for (k, w), c, p in zip(views, ctx_serial, params[f]):
+            c.render(built[k][1], p)
+            hip.hipDeviceSynchronize()
+        if f >= warmup:
+            t_serial += time.perf_counter() - t1
     for f in range(warmup + frames):
         hip.hipDeviceSynchronize()
         t1 = time.perf_counter()
@@ -145,8 +162,9 @@ def batched_bench(frames=5, warmup=2, n_views=64):
     return {"views": len(views), "scenes": len(scenes), "pixels": sum(w.width * w.height for _, w in views),
             "indirect_bounces": views[0][1].restir.max_depth, "frames": frames,
             "plain_ms_per_frame": round(1000 * t_plain / frames, 3),
+            "plain_serial_ms_per_frame": round(1000 * t_serial / frames, 3),
             "batched_ms_per_frame": round(1000 * t_batch / frames, 3),
-            "speedup": round(t_plain / t_batch, 2), "batched_mrays_s": round(rays / t_batch / 1e6, 1),
+            "speedup": round(t_plain / t_batch, 2), "speedup_vs_serial": round(t_serial / t_batch, 2), "batched_mrays_s": round(rays / t_batch / 1e6, 1),
             "graph": True, "tuned_blocks_per_cu": tune.blocks_per_cu, "tuned_groups": tune.groups}
 
 
