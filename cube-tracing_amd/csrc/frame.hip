@@ -1932,6 +1932,49 @@ void split_tables(const DFrame &F, uint32_t parts, std::vector<uint2> *tbl, uint
 
 }  // namespace
 
+// Random replay over a frame's pixels: pixel (x, y) replays the path
+// reservoir of pixel (x + dx, y + dy) (its init seed and reconnection
+// length) from its own primary hit.  out: 12 u32 per pixel -- ok, length
+// reached, the reservoir's path length and reconnection length, the vertex
+// at that length (instance, primitive, hit xyz bits), the vertex before it
+// (instance, primitive), and 0.
+__global__ void __launch_bounds__(kBlock) k_pt_replay(DFrame F, int32_t dx, int32_t dy, uint32_t *out) {
+  __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  if (px >= F.npix) return;
+  uint32_t *o = out + 12 * (size_t)px;
+  for (int k = 0; k < 12; ++k) o[k] = 0u;
+  const int32_t x = (int32_t)(px % F.W) + dx, y = (int32_t)(px / F.W) + dy;
+  if (!(F.dpos_c[px].w > 0.0f) || x < 0 || y < 0 || x >= (int32_t)F.W || y >= (int32_t)F.H) return;
+  const uint32_t src = (uint32_t)y * F.W + (uint32_t)x;
+  const PRes r = ld_pres(F.pres_o, src);
+  const uint32_t rc_len = pf_rc_length(r.flags);
+  o[2] = pf_path_length(r.flags);
+  o[3] = rc_len;
+  if (!(r.M > 0.0f) || rc_len == 0u || rc_len > F.rp.max_depth) return;
+  PtTrace tr;
+  tr.stk = stk + threadIdx.x;
+  tr.bounce_rays = 0u;
+  tr.shadow_rays = 0u;
+  tr.overflow = false;
+  tr.px = px;
+  tr.slot = 0u;
+  tr.bits = 0u;
+  tr.emitted = 0u;
+  tr.far = 0u;
+  const DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
+  const Replay rp = pt_replay(F, isect_from_di(F, di), r.init_seed, rc_len, tr);
+  o[0] = rp.ok ? 1u : 0u;
+  o[1] = rp.length;
+  o[4] = rp.rc.inst;
+  o[5] = rp.rc.prim;
+  o[6] = __float_as_uint(rp.rc.hit.x);
+  o[7] = __float_as_uint(rp.rc.hit.y);
+  o[8] = __float_as_uint(rp.rc.hit.z);
+  o[9] = rp.rc_prev.inst;
+  o[10] = rp.rc_prev.prim;
+}
+
 extern "C" {
 
 int ct_ctx_create(const ct_ctx_desc *d, ct_ctx **out) {
@@ -2619,6 +2662,25 @@ int ct_trace_bounce(ct_accel *a, uint32_t n, const float *origins, const float *
     stats->leaves = v[1];
     stats->instances = v[2];
   }
+  return CT_OK;
+}
+
+int ct_pt_random_replay(ct_ctx *c, ct_accel *a, const ct_frame_params *p, int32_t dx, int32_t dy, uint32_t *out,
+                        void *stream) {
+  if (!c || !a || !p || !out) return fail(CT_E_INVALID, "ct_pt_random_replay: null argument");
+  if (!p->restir.indirect || !c->fb.pres[0].p) return fail(CT_E_STATE, "ct_pt_random_replay: no path reservoirs");
+  if (p->width != c->last_w || p->height != c->last_h)
+    return fail(CT_E_INVALID, "ct_pt_random_replay: parameters of another size than the last frame");
+  CT_HIP(hipSetDevice(c->device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  CT_HIP(hipDeviceSynchronize());
+  DFrame F;
+  int rc = prepare_frame(c, a, p, st, F);
+  c->cur ^= 1;  // prepare_frame advanced the image ping-pong; a replay renders nothing
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_pt_replay, dim3(nblocks(F.npix)), dim3(kBlock), 0, st, F, dx, dy, out);
+  CT_HIP(hipGetLastError());
+  CT_HIP(hipStreamSynchronize(st));
   return CT_OK;
 }
 

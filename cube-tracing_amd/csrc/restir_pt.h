@@ -589,6 +589,61 @@ CT_D bool pt_step(const DFrame &F, PState &p, Isect &i) {
   return (p.flags & kPathActive) != 0u;
 }
 
+// Random replay (trace_random_replay_path_hybrid_simple,
+// path_tracer.glsl:889-966; generate_random_replay_path): the path a seed
+// draws from a given primary intersection, through the same sampling steps
+// as the frame's paths (pt_begin, then pt_step per segment with the
+// segment's closest voxel), traced in place.  It reports the vertex at
+// index rc_len (the reconnection vertex; index 0 is the primary hit), the
+// vertex before it with its incoming direction and the throughput leaving
+// it (after its scatter), and the length reached.  Replaying a path's own seed from its own
+// primary hit reproduces the path; a replay that stops before rc_len is
+// not invertible (ok = false, zero throughput).
+struct Replay {
+  uint32_t length;  // path length reached
+  bool ok;          // the vertex at rc_len exists
+  ObjHit rc, rc_prev;
+  F3 rc_prev_wo, thp;
+};
+
+CT_D Replay pt_replay(const DFrame &F, Isect i0, uint32_t seed, uint32_t rc_len, PtTrace &tr) {
+  Replay r;
+  r.ok = false;
+  r.rc = objhit_invalid();
+  r.rc_prev = objhit_invalid();
+  r.rc_prev_wo = f3(0, 0, 0);
+  r.thp = f3(0, 0, 0);
+  PState p;
+  bool more = pt_begin(F, p, i0, seed);
+  F3 wo = i0.wo;
+  if (rc_len == 1u) {
+    r.rc_prev = p.hit;
+    r.rc_prev_wo = wo;
+    r.thp = p.thp;
+  }
+  while (more && p.path_length < rc_len) {
+    Isect i = bounce_intersect<false>(F, p.origin, p.dir, tr);
+    wo = i.wo;
+    more = pt_step(F, p, i);
+    if (!i.is_hit) break;
+    if (p.path_length + 1u == rc_len) {
+      r.rc_prev = p.hit;
+      r.rc_prev_wo = wo;
+      r.thp = p.thp;
+    }
+  }
+  r.length = p.path_length;
+  if (p.path_length == rc_len && (p.flags & kPathHit)) {
+    r.ok = true;
+    r.rc = p.hit;
+  } else {
+    r.rc_prev = objhit_invalid();
+    r.rc_prev_wo = f3(0, 0, 0);
+    r.thp = f3(0, 0, 0);
+  }
+  return r;
+}
+
 CT_D void pt_end(PState &p, F3 &color) {
   p.res.M = 1.0f;  // path_builder_finilize
   pres_finalize_ris(p.res);

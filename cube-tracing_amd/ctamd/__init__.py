@@ -338,6 +338,8 @@ _SIGS = {
     "ct_accel_process_voxel_modifications": ([C.c_void_p, C.c_void_p, C.POINTER(u32), C.POINTER(u32)], C.c_int),
     "ct_path_trace": ([C.c_void_p, C.POINTER(AdjointDesc), C.POINTER(AdjointView), C.POINTER(Material), u32,
                        C.c_void_p, C.c_void_p], C.c_int),
+    "ct_pt_random_replay": ([C.c_void_p, C.c_void_p, C.POINTER(FrameParams), C.c_int32, C.c_int32, C.c_void_p,
+                             C.c_void_p], C.c_int),
     "ct_scene_load_transformed": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(f32), C.c_void_p], C.c_int),
     "ct_scene_from_vox": ([C.c_char_p, C.c_size_t, C.POINTER(VoxDesc), C.POINTER(C.c_void_p)], C.c_int),
 }
@@ -816,6 +818,20 @@ def load_scene(scene: Scene, accel: Accel, ctx: Ctx, transform=None):
 def brush_apply(ctx: Ctx, accel: Accel, x: int, y: int, window: int):
     """The squared brush (squared_brush.slang) at pixel (x, y) of ctx's last frame."""
     check(lib().ct_brush_apply(ctx.handle, accel.handle, x, y, window, None))
+
+
+def pt_random_replay(ctx: Ctx, accel: Accel, params, dx=0, dy=0):
+    """Random replay of the last frame's path reservoirs (ct_pt_random_replay):
+    (H*W, 12) u32 per pixel -- ok, length reached, reservoir path length,
+    reconnection length, reconnection vertex (instance, primitive, hit bits),
+    previous vertex (instance, primitive), 0."""
+    import torch
+    n = params.width * params.height
+    out = torch.zeros((n, 12), dtype=torch.int32, device="cuda")
+    check(lib().ct_pt_random_replay(ctx.handle, accel.handle, C.byref(params), dx, dy, C.c_void_p(out.data_ptr()),
+                                    None))
+    torch.cuda.synchronize()
+    return out.cpu().numpy().view(np.uint32)
 
 
 def bind_scene(scene: Scene, ctx: Ctx):

@@ -570,6 +570,20 @@ typedef struct ct_frame_stats {
  * and end events for ms_total (4). */
 int ct_ctx_set_instrumentation(ct_ctx *ctx, uint32_t flags);
 int ct_frame_get_stats(ct_ctx *ctx, ct_frame_stats *out);
+/* Random replay (trace_random_replay_path_hybrid_simple,
+ * path_tracer.glsl:889-966), whose only caller in the reference is
+ * commented out (indirect_illumination.glsl:301-302): after a frame with
+ * paths, pixel (x, y) replays the path reservoir of pixel (x+dx, y+dy) --
+ * its init seed, up to its reconnection length -- from its own primary
+ * hit, with the frame's path sampling steps.  `params` are the frame's
+ * (same size).  out: device pointer, 12 u32 per pixel: ok (the vertex at
+ * the reconnection length exists), length reached, the reservoir's path
+ * and reconnection lengths, that vertex's instance, primitive and hit
+ * position (f32 bits), the vertex before it (instance, primitive), 0.
+ * With dx = dy = 0 and no path reuse the replay reproduces each pixel's own
+ * reconnection vertex. */
+int ct_pt_random_replay(ct_ctx *ctx, ct_accel *accel, const ct_frame_params *params, int32_t dx, int32_t dy,
+                        uint32_t *out, void *stream);
 
 /* =================================================================
  * Batched frames.  No counterpart in the reference, which renders one
