@@ -1393,11 +1393,11 @@ int ct_accel_update_scene(ct_accel *a, int synchronize, void *stream, int *did_w
     CT_HIP(hipMemcpyAsync(&a->aabbs[f], a->set.aabbs.as<ct_aabb>() + f, sizeof(ct_aabb) * n, hipMemcpyDeviceToHost, st));
     CT_HIP(hipMemcpyAsync(&a->prims[f], a->set.prims.as<ct_primitive>() + f, sizeof(ct_primitive) * n,
                           hipMemcpyDeviceToHost, st));
-    std::vector<uint32_t> rm(n);
-    CT_HIP(hipMemcpyAsync(rm.data(), a->set.remap_prim.as<uint32_t>() + f, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
-    CT_HIP(hipStreamSynchronize(st));
-    gpu_remap.emplace_back(f, std::move(rm));
+    gpu_remap.emplace_back(f, std::vector<uint32_t>(n));
+    CT_HIP(hipMemcpyAsync(gpu_remap.back().second.data(), a->set.remap_prim.as<uint32_t>() + f, 4 * (size_t)n,
+                          hipMemcpyDeviceToHost, st));
   }
+  if (!gpu_remap.empty()) CT_HIP(hipStreamSynchronize(st));  // one wait for every instance's readback
   bool worked = false;
   if (a->status == CT_AS_SETTLING) {
     if ((rc = settle(*a, st))) return rc;
