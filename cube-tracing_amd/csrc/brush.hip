@@ -42,17 +42,34 @@ __global__ void k_brush_fill(uint32_t *p, uint32_t v, uint32_t n) {
   if (i < n) p[i] = v;
 }
 
-// delete_primitive_from_instance (squared_brush.slang:37-74).
-__device__ void mark_voxel(uint32_t iid, uint32_t pid, const ct_instance *inst, uint32_t n_inst, uint32_t slot_bits,
-                           uint32_t *imask, uint32_t *pmask, uint32_t *counters) {
-  if (iid >= n_inst) return;
+// delete_primitive_from_instance (squared_brush.slang:37-74): sets the
+// instance and slot bits; returns bit 0 when the instance bit is new and
+// bit 1 when the slot bit is new.
+__device__ uint32_t mark_voxel(uint32_t iid, uint32_t pid, const ct_instance *inst, uint32_t n_inst,
+                               uint32_t slot_bits, uint32_t *imask, uint32_t *pmask) {
+  if (iid >= n_inst) return 0u;
   const uint32_t slot = inst[iid].first_primitive_index + pid;
-  if (pid >= inst[iid].primitive_count || slot >= slot_bits) return;
+  if (pid >= inst[iid].primitive_count || slot >= slot_bits) return 0u;
   const uint32_t ib = 1u << (iid & 31u), pb = 1u << (slot & 31u);
-  const uint32_t old_i = atomicOr(&imask[iid >> 5], ib);
-  const uint32_t old_p = atomicOr(&pmask[slot >> 5], pb);
-  if (!(old_p & pb)) atomicAdd(&counters[1], 1u);
-  if (!(old_i & ib)) atomicAdd(&counters[0], 1u);
+  // Bits are only ever set, so a plain read that already shows the bit is
+  // final and saves the atomic: every voxel of an instance shares its
+  // instance word, and the atomics on it would otherwise serialise.
+  uint32_t old_i = imask[iid >> 5];
+  if (!(old_i & ib)) old_i = atomicOr(&imask[iid >> 5], ib);
+  uint32_t old_p = pmask[slot >> 5];
+  if (!(old_p & pb)) old_p = atomicOr(&pmask[slot >> 5], pb);
+  return ((old_i & ib) ? 0u : 1u) | ((old_p & pb) ? 0u : 2u);
+}
+
+// BRUSH_COUNTER increments of a whole wave in one atomic per counter
+// (increment_brush_counter_*, squared_brush.slang:7-19); every lane of the
+// wave calls it.
+__device__ void count_marks(uint32_t res, uint32_t *counters) {
+  const unsigned long long ni = __ballot((res & 1u) != 0u), np = __ballot((res & 2u) != 0u);
+  if ((threadIdx.x & 63u) == 0u) {
+    if (ni) atomicAdd(&counters[0], (uint32_t)__popcll(ni));
+    if (np) atomicAdd(&counters[1], (uint32_t)__popcll(np));
+  }
 }
 
 // entry_brush (squared_brush.slang:79-133) over the pixels of the window
@@ -62,20 +79,22 @@ __global__ void k_brush(const float4 *di_pos, const uint4 *di_hit, uint32_t W, u
                         uint32_t y0, uint32_t nx, uint32_t ny, const ct_instance *inst, uint32_t n_inst,
                         uint32_t slot_bits, uint32_t *imask, uint32_t *pmask, uint32_t *counters) {
   const uint32_t tx = blockIdx.x * 16u + (threadIdx.x & 15u), ty = blockIdx.y * 16u + (threadIdx.x >> 4);
-  if (tx >= nx || ty >= ny) return;
+  uint32_t res = 0u;
   const size_t centre = (size_t)py * W + px;
-  if (!(di_pos[centre].w > 0.0f)) return;
-  const size_t q = (size_t)(y0 + ty) * W + (x0 + tx);
-  if (!(di_pos[q].w > 0.0f)) return;
-  const uint4 h = di_hit[q];
-  if (h.x != di_hit[centre].x) return;  // only the centre pixel's instance
-  mark_voxel(h.x, h.y, inst, n_inst, slot_bits, imask, pmask, counters);
+  if (tx < nx && ty < ny && di_pos[centre].w > 0.0f) {
+    const size_t q = (size_t)(y0 + ty) * W + (x0 + tx);
+    const uint4 h = di_hit[q];
+    if (di_pos[q].w > 0.0f && h.x == di_hit[centre].x)  // only the centre pixel's instance
+      res = mark_voxel(h.x, h.y, inst, n_inst, slot_bits, imask, pmask);
+  }
+  count_marks(res, counters);
 }
 
 __global__ void k_brush_mark(const uint2 *hits, uint32_t n, const ct_instance *inst, uint32_t n_inst,
                              uint32_t slot_bits, uint32_t *imask, uint32_t *pmask, uint32_t *counters) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) mark_voxel(hits[i].x, hits[i].y, inst, n_inst, slot_bits, imask, pmask, counters);
+  const uint32_t res = i < n ? mark_voxel(hits[i].x, hits[i].y, inst, n_inst, slot_bits, imask, pmask) : 0u;
+  count_marks(res, counters);
 }
 
 struct Marked {
