@@ -2676,8 +2676,8 @@ int ct_pt_random_replay(ct_ctx *c, ct_accel *a, const ct_frame_params *p, int32_
   CT_HIP(hipDeviceSynchronize());
   DFrame F;
   int rc = prepare_frame(c, a, p, st, F);
-  c->cur ^= 1;  // prepare_frame advanced the image ping-pong; a replay renders nothing
   if (rc) return rc;
+  c->cur ^= 1;  // prepare_frame advanced the image ping-pong; a replay renders nothing
   hipLaunchKernelGGL(k_pt_replay, dim3(nblocks(F.npix)), dim3(kBlock), 0, st, F, dx, dy, out);
   CT_HIP(hipGetLastError());
   CT_HIP(hipStreamSynchronize(st));
