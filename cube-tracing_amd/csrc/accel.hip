@@ -1315,6 +1315,61 @@ void task_update(Accel &a, Update &u, const ct_task &t, bool &touched) {
 
 // DELETE_PRIMITIVE_BLAS_FROM_GPU (:2258-2290).  The device slots already
 // hold the compacted primitives; the host mirror was refreshed from them.
+// Cube lights of the GPU deletions of one update (the rearrangement leaves
+// them alone), on indices taken before any of them moves: every light whose
+// voxel survived gets the voxel's new local index, then every light whose
+// voxel was deleted is removed as the CPU deletion removes one (the last
+// light moves into its place and its owner is relinked), highest index
+// first so that a light about to be removed is never the one moved.  The
+// light remapping entries map each original index to its final one.
+void gpu_delete_lights(Accel &a, Update &u, const std::vector<ct_task> &queue,
+                       const std::vector<std::pair<uint32_t, std::vector<ct_primitive>>> &old) {
+  std::vector<uint8_t> alive(a.cube_light_count, 0);
+  std::vector<uint32_t> gone;
+  for (const ct_task &t : queue) {
+    if (t.type != CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_GPU) continue;
+    const uint32_t id = t.u.blas_del_prim_gpu.instance_index;
+    const std::vector<ct_primitive> *o = nullptr;
+    for (const auto &g : old)
+      if (g.first == id) o = &g.second;
+    if (!o || !used(a, id)) continue;
+    const uint32_t f = a.instances[id].first_primitive_index, n = (uint32_t)o->size();
+    const uint32_t keep = n - std::min(t.u.blas_del_prim_gpu.del_prim_count, n);
+    for (uint32_t k = 0; k < keep; ++k) {
+      const uint32_t L = a.prims[f + k].light_index;
+      if (L != kInvalid && L < a.cube_light_count) {
+        a.cube_lights_host[L].instance_info.primitive_id = k;
+        alive[L] = 1;
+      }
+    }
+    for (const ct_primitive &p : *o)
+      if (p.light_index != kInvalid && p.light_index < a.cube_light_count) gone.push_back(p.light_index);
+  }
+  gone.erase(std::remove_if(gone.begin(), gone.end(), [&](uint32_t L) { return alive[L] != 0; }), gone.end());
+  std::sort(gone.begin(), gone.end());
+  gone.erase(std::unique(gone.begin(), gone.end()), gone.end());
+  if (gone.empty()) return;
+  std::vector<uint32_t> origin(a.cube_light_count);  // original index of the light now at each position
+  for (uint32_t l = 0; l < origin.size(); ++l) origin[l] = l;
+  std::vector<uint32_t> final_pos(a.cube_light_count, 0u);  // 0: unchanged
+  std::vector<uint8_t> touched(a.cube_light_count, 0);
+  for (auto it = gone.rbegin(); it != gone.rend(); ++it) {
+    const uint32_t L = *it, Lx = a.cube_light_count - 1;
+    final_pos[origin[L]] = kInvalid;
+    touched[origin[L]] = 1;
+    if (Lx != L) {
+      a.cube_lights_host[L] = a.cube_lights_host[Lx];
+      origin[L] = origin[Lx];
+      relink_light_owner(a, u, L);
+      final_pos[origin[L]] = L;
+      touched[origin[L]] = 1;
+    }
+    --a.cube_light_count;
+  }
+  for (uint32_t l = 0; l < touched.size(); ++l)
+    if (touched[l]) u.remap_lights.push_back(make_uint2(l, final_pos[l]));
+}
+
 void task_delete_from_gpu(Accel &a, Update &u, const ct_task &t) {
   const uint32_t id = t.u.blas_del_prim_gpu.instance_index;
   if (!used(a, id)) return u.error(CT_E_INVALID, "update_scene: DELETE_PRIMITIVE_FROM_GPU of an unused instance");
@@ -1384,12 +1439,14 @@ int ct_accel_update_scene(ct_accel *a, int synchronize, void *stream, int *did_w
     a->tasks.pop();
   }
   std::vector<std::pair<uint32_t, std::vector<uint32_t>>> gpu_remap;
+  std::vector<std::pair<uint32_t, std::vector<ct_primitive>>> gpu_old;  // records before the compaction
   for (const ct_task &t : queue) {
     if (t.type != CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_GPU) continue;
     const uint32_t id = t.u.blas_del_prim_gpu.instance_index;
     if (!used(*a, id) || !a->set.aabbs.p) continue;
     const ct_instance &in = a->instances[id];
     const uint32_t f = in.first_primitive_index, n = in.primitive_count;
+    gpu_old.emplace_back(id, std::vector<ct_primitive>(a->prims.begin() + f, a->prims.begin() + f + n));
     CT_HIP(hipMemcpyAsync(&a->aabbs[f], a->set.aabbs.as<ct_aabb>() + f, sizeof(ct_aabb) * n, hipMemcpyDeviceToHost, st));
     CT_HIP(hipMemcpyAsync(&a->prims[f], a->set.prims.as<ct_primitive>() + f, sizeof(ct_primitive) * n,
                           hipMemcpyDeviceToHost, st));
@@ -1411,6 +1468,7 @@ int ct_accel_update_scene(ct_accel *a, int synchronize, void *stream, int *did_w
   // process_task_queue, accel_struct_mngr.cpp:2115-2443
   a->status = CT_AS_UPDATING;
   Update u;
+  gpu_delete_lights(*a, u, queue, gpu_old);
   uint32_t cursor = 0;
   bool tables = false;
   for (ct_task &t : queue) {

@@ -212,7 +212,9 @@ int ct_accel_request_primitive_index_host_buffer_count(ct_accel *accel, uint32_t
  * DELETE_PRIMITIVE_BLAS_FROM_GPU: the caller's own pass has already
  *   compacted the instance's primitives on the device (and written the
  *   remapping table); the instance loses its last del_prim_count primitives
- *   and is rebuilt, or is deleted when that leaves none (:2258-2290).
+ *   and is rebuilt, or is deleted when that leaves none (:2258-2290).  The
+ *   cube lights of the primitives dropped this way are removed and those of
+ *   moved primitives relinked, over all such tasks of the update at once.
  * UPDATE_BLAS_FROM_CPU: transform = transform * task transform (column-major
  *   product, :2303-2304); then, for primitive_count > 0, the AABBs staged at
  *   aabb_buf_offset replace those of the primitive indices staged at
@@ -731,10 +733,13 @@ int ct_scene_from_vox(const void *data, size_t bytes, const ct_vox_desc *desc, c
  * remapping table (moved: new local index; deleted: ~0u), queues one
  * DELETE_PRIMITIVE_BLAS_FROM_GPU task per instance in id order and clears
  * the marks.  The next ct_accel_update_scene applies the tasks (BLAS
- * rebuild, or instance deletion when nothing is left).  As in the
- * reference, cube lights are not touched: deleting an emissive voxel this
- * way leaves its LIGHT record in place.  All three calls synchronize the
- * device first; `stream` is accepted for symmetry and unused.
+ * rebuild, or instance deletion when nothing is left).  Unlike the
+ * reference, which leaves them in place, that update also removes the
+ * cube lights of deleted voxels (the last light moves into each hole and
+ * its owner is relinked, highest index first), points the lights of moved
+ * voxels at their new primitive ids and fills the light remapping table.
+ * All three calls synchronize the device first; `stream` is accepted for
+ * symmetry and unused.
  * ================================================================= */
 /* entry_brush: every pixel of the square [x-window, x+window] x
  * [y-window, y+window] (clipped to the last frame, in u32 arithmetic as

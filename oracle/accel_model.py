@@ -140,9 +140,42 @@ class AccelModel:
         """One update_scene call: settle the previous update, then apply the
         tasks in order.  Each task is a tuple (kind, args...)."""
         self._settle()
+        self._gpu_deletions([t for t in tasks if t[0] == "delete_from_gpu"])
         for t in tasks:
             getattr(self, "_" + t[0])(*t[1:])
         self.settling = True
+
+    def _gpu_deletions(self, tasks):
+        """The GPU deletions of one update, before the task loop: each
+        instance takes its compacted voxels; lights of surviving voxels get
+        the voxel's new index; lights of deleted voxels are removed as a CPU
+        deletion removes one (last light into the hole, owner relinked),
+        highest index first; the light remapping maps original indices to
+        final ones."""
+        alive, gone = set(), set()
+        for _, i, compacted, count in tasks:
+            inst = self.instances.get(i)
+            if inst is None:
+                continue
+            old = [v.light for v in inst.voxels]
+            keep = len(old) - min(count, len(old))
+            inst.voxels = [copy.copy(v) for v in list(compacted)[:keep]]
+            for k, v in enumerate(inst.voxels):
+                if v.light != INVALID and v.light < len(self.lights):
+                    self.lights[v.light].primitive = k
+                    alive.add(v.light)
+            gone.update(L for L in old if L != INVALID and L < len(self.lights))
+        gone -= alive
+        origin = list(range(len(self.lights)))
+        for L in sorted(gone, reverse=True):
+            Lx = len(self.lights) - 1
+            self.remap_lights[origin[L]] = INVALID
+            if Lx != L:
+                self.lights[L] = copy.deepcopy(self.lights[Lx])
+                origin[L] = origin[Lx]
+                self._relink(L)
+                self.remap_lights[origin[L]] = L
+            self.lights.pop()
 
     def _settle(self):
         if self.settling:
@@ -228,12 +261,10 @@ class AccelModel:
 
     def _delete_from_gpu(self, i, compacted, count):
         """The caller's pass left `compacted` (the instance's voxels after
-        its own compaction) and drops `count` from the end."""
-        inst = self.instances[i]
-        if len(inst.voxels) <= count:
+        its own compaction) and drops `count` from the end; the voxels were
+        taken in _gpu_deletions, an instance left empty is deleted here."""
+        if i in self.instances and not self.instances[i].voxels:
             del self.instances[i]
-        else:
-            inst.voxels = list(compacted)[: len(inst.voxels) - count]
         self.done.append(Done("delete_from_gpu", i))
 
     # ---------------------------------------------------------- export

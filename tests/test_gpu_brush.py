@@ -127,3 +127,35 @@ def test_processing_waits_for_an_idle_accel(gpu):
     assert acc.brush_counters() == (1, 2)
     acc.update_scene()
     assert acc.process_voxel_modifications() == (1, 2)
+
+
+def test_gpu_deletion_removes_the_lights_of_deleted_voxels(gpu):
+    """Lights of deleted voxels leave the light list (last light into the
+    hole, owner relinked), lights of moved voxels follow them, and the light
+    remapping maps original indices to final ones -- as the task model does."""
+    w = _workload()
+    sc, acc, ctx = _load(w)
+    acc.update_scene()
+    m = _model(acc)
+    before = _snapshot(acc)
+    W = acc.read_world()
+    n_lights = W["light_count"]
+    ids = sorted(k for k in before if isinstance(k, int))
+    lit = {i: [k for k in range(before[i][1]) if W["prims"][before[i][0] + k, 1] != 0xFFFFFFFF] for i in ids}
+    a, b = [i for i in ids if len(lit[i]) >= 3][:2]
+    dark_a = [k for k in range(before[a][1]) if k not in lit[a]]
+    plan = {a: lit[a][:2] + dark_a[:3] + [before[a][1] - 1], b: [lit[b][-1]]}
+    acc.brush_mark([(i, p) for i, d in plan.items() for p in d])
+    assert acc.process_voxel_modifications()[0] == 2
+    _check_compaction(acc, m, plan, before)
+    W2 = acc.read_world()
+    gone = {int(W["prims"][before[i][0] + p, 1]) for i, d in plan.items() for p in d} - {0xFFFFFFFF}
+    assert W2["light_count"] == n_lights - len(gone) == len(m.lights)
+    assert np.array_equal(W2["remap_lights"], m.remap_light_table(len(W2["remap_lights"])))
+    # every light's owner points back at it
+    inst = acc.read_instances()
+    for l in range(W2["light_count"]):
+        o = W2["lights"][l].instance_info
+        assert W2["prims"][inst[o.instance_id].first_primitive_index + o.primitive_id, 1] == l
+    osc = _oracle(acc, sc, ctx)
+    _trace_equal(acc, osc, sc, 37)
