@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 
 #include "ct_device.h"
 #include "ct_internal.h"
@@ -1051,6 +1052,8 @@ int ct_accel_get_staging(ct_accel *a, ct_staging *o) {
 int ct_accel_set_cube_light_count(ct_accel *a, uint32_t count) {
   if (!a) return fail(CT_E_INVALID, "ct_accel_set_cube_light_count: null handle");
   if (count > a->max_cube_lights) return fail(CT_E_CAPACITY, "cube light count exceeds max_cube_light_count");
+  if (count != a->cube_light_count && ct::deletions_pending(*a))
+    return fail(CT_E_STATE, "ct_accel_set_cube_light_count: primitive deletions are queued; run update_scene first");
   a->cube_light_count = count;
   return CT_OK;
 }
@@ -1114,9 +1117,25 @@ void mat4_mul_host(const float *m, const float *t, float *out) {
 }
 
 // Per-update state: remapping writes and what the commit must redo.
+// Cube-light moves are tracked in the index space of the list as the update
+// found it, which is the space the previous frame's reservoirs use:
+// origin[l] is the original index of the light now at l (kNoOrigin for a
+// light an undo brought back), final_pos[o] where original light o ended up
+// (kInvalid once removed, kUnmoved if it never moved).  GPU deletions, CPU
+// deletions and undos of one update then compose, and each original index
+// gets one remapping entry.
+// Voxel moves are tracked the same way per instance, in local indices.
+constexpr uint32_t kNoOrigin = 0xFFFFFFFEu, kUnmoved = 0xFFFFFFFEu;
+struct PrimMoves {
+  uint32_t first = 0;
+  std::vector<uint32_t> origin;
+  std::map<uint32_t, uint32_t> final_pos;
+};
 struct Update {
   Dirty dirty;
   std::vector<uint2> remap_prims, remap_lights;  // (index, value)
+  std::vector<uint32_t> light_origin, light_final;
+  std::map<uint32_t, PrimMoves> prim_moves;
   std::vector<DoneTask> completed;
   int first_error = CT_OK;
   std::string message;
@@ -1125,6 +1144,58 @@ struct Update {
       first_error = code;
       message = m;
     }
+  }
+  void lights_begin(uint32_t n) {
+    light_origin.resize(n);
+    for (uint32_t l = 0; l < n; ++l) light_origin[l] = l;
+    light_final.assign(n, kUnmoved);
+  }
+  uint32_t origin_of(uint32_t l) const { return l < light_origin.size() ? light_origin[l] : kNoOrigin; }
+  void light_removed(uint32_t l) {
+    const uint32_t o = origin_of(l);
+    if (o < light_final.size()) light_final[o] = kInvalid;
+  }
+  void light_moved(uint32_t from, uint32_t to) {
+    const uint32_t o = origin_of(from);
+    if (to >= light_origin.size()) light_origin.resize(to + 1, kNoOrigin);
+    light_origin[to] = o;
+    if (o < light_final.size()) light_final[o] = to;
+  }
+  void light_restored(uint32_t l) {
+    if (l >= light_origin.size()) light_origin.resize(l + 1, kNoOrigin);
+    light_origin[l] = kNoOrigin;
+  }
+  void lights_end() {
+    for (uint32_t o = 0; o < light_final.size(); ++o)
+      if (light_final[o] != kUnmoved) remap_lights.push_back(make_uint2(o, light_final[o]));
+  }
+  PrimMoves &moves(uint32_t id, uint32_t first, uint32_t count) {
+    auto it = prim_moves.find(id);
+    if (it != prim_moves.end()) return it->second;
+    PrimMoves &m = prim_moves[id];
+    m.first = first;
+    m.origin.resize(count);
+    for (uint32_t k = 0; k < count; ++k) m.origin[k] = k;
+    return m;
+  }
+  static uint32_t origin_in(const PrimMoves &m, uint32_t l) { return l < m.origin.size() ? m.origin[l] : kNoOrigin; }
+  void prim_removed(PrimMoves &m, uint32_t l) {
+    const uint32_t o = origin_in(m, l);
+    if (o != kNoOrigin) m.final_pos[o] = kInvalid;
+  }
+  void prim_moved(PrimMoves &m, uint32_t from, uint32_t to) {
+    const uint32_t o = origin_in(m, from);
+    if (to >= m.origin.size()) m.origin.resize(to + 1, kNoOrigin);
+    m.origin[to] = o;
+    if (o != kNoOrigin) m.final_pos[o] = to;
+  }
+  void prim_restored(PrimMoves &m, uint32_t l) {
+    if (l >= m.origin.size()) m.origin.resize(l + 1, kNoOrigin);
+    m.origin[l] = kNoOrigin;
+  }
+  void prims_end() {
+    for (const auto &im : prim_moves)
+      for (const auto &of : im.second.final_pos) remap_prims.push_back(make_uint2(im.second.first + of.first, of.second));
   }
 };
 
@@ -1225,8 +1296,10 @@ bool task_delete_primitive(Accel &a, Update &u, ct_task &t, DoneTask &d) {
     d.light_rec = a.cube_lights_host[L];
     Lx = a.cube_light_count - 1;
     if (Lx == L) Lx = kInvalid;
+    u.light_removed(L);
     if (Lx != kInvalid) {
       a.cube_lights_host[L] = a.cube_lights_host[Lx];
+      u.light_moved(Lx, L);
       relink_light_owner(a, u, L);
     }
     --a.cube_light_count;
@@ -1241,17 +1314,29 @@ bool task_delete_primitive(Accel &a, Update &u, ct_task &t, DoneTask &d) {
   in.primitive_count = last;
   d.light = L;
   d.light_moved = Lx;
+  d.light_count_after = a.cube_light_count;
+  if (Lx != kInvalid) d.moved_rec = a.cube_lights_host[L];
   t.u.blas_delete_primitive_from_cpu.remap_primitive_index = last;
   t.u.blas_delete_primitive_from_cpu.del_light_index = L;
   t.u.blas_delete_primitive_from_cpu.remap_light_index = Lx;
   t.u.blas_delete_primitive_from_cpu.remap_primitive_light_index = Lm;
   d.task = t;
-  u.remap_prims.push_back(make_uint2(first + del, kInvalid));
-  if (last != del) u.remap_prims.push_back(make_uint2(first + last, del));
-  if (L != kInvalid) u.remap_lights.push_back(make_uint2(L, kInvalid));
-  if (Lx != kInvalid) u.remap_lights.push_back(make_uint2(Lx, L));
+  PrimMoves &pm = u.moves(id, first, last + 1);
+  u.prim_removed(pm, del);
+  if (last != del) u.prim_moved(pm, last, del);
   mark(u.dirty.rebuild, id);
   return true;
+}
+
+// Whether the light list is still the one a primitive deletion left: same
+// length, and the record it moved into the hole still there.  Any other
+// change to the list (a GPU deletion that removed lights, a load that
+// appended some) invalidates the deletion's saved light indices.
+bool lights_as_left(const Accel &a, const DoneTask &d) {
+  if (d.light == kInvalid) return true;
+  if (a.cube_light_count != d.light_count_after) return false;
+  if (d.light_moved == kInvalid) return true;
+  return std::memcmp(&a.cube_lights_host[d.light], &d.moved_rec, sizeof(ct_light)) == 0;
 }
 
 // Exact inverse of task_delete_primitive (the reference's own undo,
@@ -1262,29 +1347,34 @@ void undo_delete_primitive(Accel &a, Update &u, const DoneTask &d) {
   const uint32_t del = d.task.u.blas_delete_primitive_from_cpu.del_primitive_index;
   if (!used(a, id) || a.instances[id].primitive_count != d.last)
     return u.error(CT_E_STATE, "update_scene: UNDO of a primitive deletion whose instance has changed since");
+  if (!lights_as_left(a, d))
+    return u.error(CT_E_STATE, "update_scene: UNDO of a primitive deletion whose cube light list has changed since");
   if (d.light != kInvalid && a.cube_light_count >= a.max_cube_lights)
     return u.error(CT_E_CAPACITY, "update_scene: UNDO needs a cube light slot");
   ct_instance &in = a.instances[id];
   const uint32_t first = in.first_primitive_index, last = d.last;
   in.primitive_count = last + 1;
   touch(a, u, id, in.primitive_count);
+  PrimMoves &pm = u.moves(id, first, last);
   if (last != del) {
     a.aabbs[first + last] = a.aabbs[first + del];
     a.prims[first + last] = a.prims[first + del];
     const uint32_t Lm = a.prims[first + last].light_index;
     if (Lm != kInvalid && Lm < a.cube_light_count) a.cube_lights_host[Lm].instance_info.primitive_id = last;
-    u.remap_prims.push_back(make_uint2(first + del, last));
+    u.prim_moved(pm, del, last);
   }
+  u.prim_restored(pm, del);
   a.aabbs[first + del] = d.aabb;
   a.prims[first + del] = d.prim;
   if (d.light != kInvalid) {
     ++a.cube_light_count;
     if (d.light_moved != kInvalid) {
       a.cube_lights_host[d.light_moved] = a.cube_lights_host[d.light];
+      u.light_moved(d.light, d.light_moved);
       relink_light_owner(a, u, d.light_moved);
-      u.remap_lights.push_back(make_uint2(d.light, d.light_moved));
     }
     a.cube_lights_host[d.light] = d.light_rec;
+    u.light_restored(d.light);
   }
   mark(u.dirty.rebuild, id);
 }
@@ -1348,26 +1438,16 @@ void gpu_delete_lights(Accel &a, Update &u, const std::vector<ct_task> &queue,
   gone.erase(std::remove_if(gone.begin(), gone.end(), [&](uint32_t L) { return alive[L] != 0; }), gone.end());
   std::sort(gone.begin(), gone.end());
   gone.erase(std::unique(gone.begin(), gone.end()), gone.end());
-  if (gone.empty()) return;
-  std::vector<uint32_t> origin(a.cube_light_count);  // original index of the light now at each position
-  for (uint32_t l = 0; l < origin.size(); ++l) origin[l] = l;
-  std::vector<uint32_t> final_pos(a.cube_light_count, 0u);  // 0: unchanged
-  std::vector<uint8_t> touched(a.cube_light_count, 0);
   for (auto it = gone.rbegin(); it != gone.rend(); ++it) {
     const uint32_t L = *it, Lx = a.cube_light_count - 1;
-    final_pos[origin[L]] = kInvalid;
-    touched[origin[L]] = 1;
+    u.light_removed(L);
     if (Lx != L) {
       a.cube_lights_host[L] = a.cube_lights_host[Lx];
-      origin[L] = origin[Lx];
+      u.light_moved(Lx, L);
       relink_light_owner(a, u, L);
-      final_pos[origin[L]] = L;
-      touched[origin[L]] = 1;
     }
     --a.cube_light_count;
   }
-  for (uint32_t l = 0; l < touched.size(); ++l)
-    if (touched[l]) u.remap_lights.push_back(make_uint2(l, final_pos[l]));
 }
 
 void task_delete_from_gpu(Accel &a, Update &u, const ct_task &t) {
@@ -1403,6 +1483,18 @@ int settle(Accel &a, hipStream_t st) {
     a.prev_first[id] = a.instances[id].first_primitive_index;
   }
   return upload_tables(a, st);
+}
+
+// One write per index, the last one queued: the scatter kernel writes its
+// pairs in parallel, so duplicate indices would race.
+void last_write_wins(std::vector<uint2> &w) {
+  std::stable_sort(w.begin(), w.end(), [](const uint2 &p, const uint2 &q) { return p.x < q.x; });
+  size_t n = 0;
+  for (size_t i = 0; i < w.size(); ++i) {
+    if (i + 1 < w.size() && w[i + 1].x == w[i].x) continue;
+    w[n++] = w[i];
+  }
+  w.resize(n);
 }
 
 int scatter(hipStream_t st, const std::vector<uint2> &w, uint32_t *dst) {
@@ -1468,6 +1560,7 @@ int ct_accel_update_scene(ct_accel *a, int synchronize, void *stream, int *did_w
   // process_task_queue, accel_struct_mngr.cpp:2115-2443
   a->status = CT_AS_UPDATING;
   Update u;
+  u.lights_begin(a->cube_light_count);
   gpu_delete_lights(*a, u, queue, gpu_old);
   uint32_t cursor = 0;
   bool tables = false;
@@ -1529,6 +1622,8 @@ int ct_accel_update_scene(ct_accel *a, int synchronize, void *stream, int *did_w
         break;
     }
   }
+  u.lights_end();
+  u.prims_end();
   a->temp_instance_count = 0;
   a->temp_primitive_count = 0;
   a->temp_prim_index_count = 0;
@@ -1540,10 +1635,14 @@ int ct_accel_update_scene(ct_accel *a, int synchronize, void *stream, int *did_w
   }
   // Remapping writes of this update (and the caller's, re-applied), then
   // remembered for the settle.
+  // The caller's compaction came first, so its entries go first and a later
+  // task's entry for the same slot wins.
+  std::vector<uint2> gpu_entries;
   for (auto &g : gpu_remap) {
     for (uint32_t k = 0; k < g.second.size(); ++k)
-      if (g.second[k] != 0u) u.remap_prims.push_back(make_uint2(g.first + k, g.second[k]));
+      if (g.second[k] != 0u) gpu_entries.push_back(make_uint2(g.first + k, g.second[k]));
   }
+  u.remap_prims.insert(u.remap_prims.begin(), gpu_entries.begin(), gpu_entries.end());
   for (const uint2 &w : u.remap_prims)
     if (w.x < a->set.remap_prim_cap) a->settle_prims.emplace_back(w.x, 1u);
   for (const uint2 &w : u.remap_lights)
@@ -1553,6 +1652,8 @@ int ct_accel_update_scene(ct_accel *a, int synchronize, void *stream, int *did_w
     if (w.x < a->set.remap_prim_cap) wp.push_back(w);
   for (const uint2 &w : u.remap_lights)
     if (w.x < a->set.remap_light_cap) wl.push_back(w);
+  last_write_wins(wp);
+  last_write_wins(wl);
   if ((rc = scatter(st, wp, a->set.remap_prim.as<uint32_t>())) || (rc = scatter(st, wl, a->set.remap_light.as<uint32_t>())))
     return rc;
   for (DoneTask &d : u.completed) {

@@ -149,6 +149,10 @@ struct DoneTask {
   ct_aabb aabb{};
   ct_primitive prim{};
   ct_light light_rec{};
+  // The light list a primitive deletion left: its length and the record the
+  // tail light became at `light`.  An undo applies only to that list.
+  uint32_t light_count_after = 0;
+  ct_light moved_rec{};
 };
 
 struct Accel {
@@ -191,6 +195,18 @@ struct Accel {
   uint32_t status = CT_AS_IDLE;
   uint32_t generation = 0;
 };
+
+// Whether the task queue holds primitive deletions (CPU or GPU).  Their
+// light moves are computed on the light list as update_scene finds it, so
+// nothing may append lights before they run.
+inline bool deletions_pending(const Accel &a) {
+  std::queue<ct_task> q = a.tasks;
+  for (; !q.empty(); q.pop())
+    if (q.front().type == CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_CPU || q.front().type == CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_GPU)
+      return true;
+  return false;
+}
+
 
 
 // ------------------------------------------------------------- ctx

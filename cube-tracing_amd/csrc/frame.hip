@@ -2669,6 +2669,7 @@ int ct_pt_random_replay(ct_ctx *c, ct_accel *a, const ct_frame_params *p, int32_
                         void *stream) {
   if (!c || !a || !p || !out) return fail(CT_E_INVALID, "ct_pt_random_replay: null argument");
   if (!p->restir.indirect || !c->fb.pres[0].p) return fail(CT_E_STATE, "ct_pt_random_replay: no path reservoirs");
+  if (int rc0 = check_frame(c, a, p)) return rc0;
   if (p->width != c->last_w || p->height != c->last_h)
     return fail(CT_E_INVALID, "ct_pt_random_replay: parameters of another size than the last frame");
   CT_HIP(hipSetDevice(c->device));

@@ -370,6 +370,8 @@ int ct_scene_load(const ct_scene *s, ct_accel *a, ct_ctx *ctx, void *stream) {
 
 int ct_scene_load_transformed(const ct_scene *s, ct_accel *a, ct_ctx *ctx, const float transform[16], void *stream) {
   if (!s || !a || !ctx) return fail(CT_E_INVALID, "ct_scene_load: null argument");
+  if (ct::deletions_pending(*a))
+    return fail(CT_E_STATE, "ct_scene_load: primitive deletions are queued; run update_scene first");
   ct_staging st;
   int rc = ct_accel_get_staging(a, &st);
   if (rc) return rc;

@@ -39,6 +39,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -163,27 +164,37 @@ struct Placement {
   int32_t offset[3];
 };
 
-int walk_graph(const std::map<int32_t, Node> &nodes, int32_t id, const int32_t t[3], int depth,
-               const std::vector<Model> &models, std::vector<Placement> &out) {
+// Largest summed translation accepted; placements stay far inside int32.
+constexpr int64_t kMaxTranslation = int64_t(1) << 24;
+
+// Walks the scene graph from `id` with the summed translation `t`.  Every
+// node may be reached once: a graph whose nodes share children (a DAG, or a
+// cycle) is refused, which bounds the walk by the node count.
+int walk_graph(const std::map<int32_t, Node> &nodes, int32_t id, const int64_t t[3], int depth,
+               const std::vector<Model> &models, std::vector<Placement> &out, std::set<int32_t> &seen) {
   if (depth > 64) return fail(CT_E_INVALID, "ct_scene_from_vox: scene graph too deep");
   auto it = nodes.find(id);
   if (it == nodes.end()) return fail(CT_E_INVALID, "ct_scene_from_vox: scene graph names a missing node");
+  if (!seen.insert(id).second) return fail(CT_E_INVALID, "ct_scene_from_vox: scene graph reaches a node twice");
   const Node &n = it->second;
   int rc;
   if (n.kind == 1) {
     if (n.rotated) return fail(CT_E_UNSUPPORTED, "ct_scene_from_vox: rotated nTRN nodes are not supported");
-    const int32_t tt[3] = {t[0] + n.t[0], t[1] + n.t[1], t[2] + n.t[2]};
+    const int64_t tt[3] = {t[0] + n.t[0], t[1] + n.t[1], t[2] + n.t[2]};
+    for (int a = 0; a < 3; ++a)
+      if (tt[a] > kMaxTranslation || tt[a] < -kMaxTranslation)
+        return fail(CT_E_INVALID, "ct_scene_from_vox: scene graph translation out of range");
     for (int32_t c : n.children)
-      if ((rc = walk_graph(nodes, c, tt, depth + 1, models, out))) return rc;
+      if ((rc = walk_graph(nodes, c, tt, depth + 1, models, out, seen))) return rc;
   } else if (n.kind == 2) {
     for (int32_t c : n.children)
-      if ((rc = walk_graph(nodes, c, t, depth + 1, models, out))) return rc;
+      if ((rc = walk_graph(nodes, c, t, depth + 1, models, out, seen))) return rc;
   } else {
     for (int32_t m : n.children) {
       if (m < 0 || (size_t)m >= models.size()) return fail(CT_E_INVALID, "ct_scene_from_vox: nSHP names a missing model");
       Placement pl;
       pl.model = m;
-      for (int a = 0; a < 3; ++a) pl.offset[a] = t[a] - models[m].size[a] / 2;
+      for (int a = 0; a < 3; ++a) pl.offset[a] = (int32_t)(t[a] - models[m].size[a] / 2);
       out.push_back(pl);
     }
   }
@@ -292,7 +303,11 @@ extern "C" int ct_scene_from_vox(const void *data, size_t bytes, const ct_vox_de
               if (endp == s) return fail(CT_E_INVALID, "ct_scene_from_vox: bad nTRN translation");
               s = endp;
             }
-            for (int a = 0; a < 3; ++a) n.t[a] = (int32_t)v[a];
+            for (int a = 0; a < 3; ++a) {
+              if (v[a] > kMaxTranslation || v[a] < -kMaxTranslation)
+                return fail(CT_E_INVALID, "ct_scene_from_vox: nTRN translation out of range");
+              n.t[a] = (int32_t)v[a];
+            }
           }
           auto rr = d.find("_r");
           if (rr != d.end() && std::strtol(rr->second.c_str(), nullptr, 10) != 4) n.rotated = true;  // 4 = identity
@@ -323,8 +338,9 @@ extern "C" int ct_scene_from_vox(const void *data, size_t bytes, const ct_vox_de
     if ((size_t)desc->model >= models.size()) return fail(CT_E_INVALID, "ct_scene_from_vox: model index out of range");
     places.push_back(Placement{desc->model, {0, 0, 0}});
   } else if (!nodes.empty()) {
-    const int32_t zero[3] = {0, 0, 0};
-    int rc = walk_graph(nodes, 0, zero, 0, models, places);
+    const int64_t zero[3] = {0, 0, 0};
+    std::set<int32_t> seen;
+    int rc = walk_graph(nodes, 0, zero, 0, models, places, seen);
     if (rc) return rc;
   } else {
     for (size_t m = 0; m < models.size(); ++m) places.push_back(Placement{(int32_t)m, {0, 0, 0}});

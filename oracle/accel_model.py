@@ -76,6 +76,12 @@ class Done:
     light: int = INVALID
     light_moved: int = INVALID
     light_rec: LightRec | None = None
+    lights_after: int = 0            # length of the light list the deletion left
+    moved_rec: LightRec | None = None  # the record it moved into the hole
+
+
+class StateError(Exception):
+    """A task the library refuses with CT_E_STATE (the update goes on)."""
 
 
 class AccelModel:
@@ -91,6 +97,8 @@ class AccelModel:
         self.prev_transform: dict[int, np.ndarray] = {}
         self.done: list[Done] = []
         self.settling = False
+        # moves of the update in progress (see update)
+        self._origin, self._final, self._porigin, self._pfinal = [], {}, {}, {}
 
     # ---------------------------------------------------------- loading
     @classmethod
@@ -138,12 +146,71 @@ class AccelModel:
     # ---------------------------------------------------------- updates
     def update(self, tasks):
         """One update_scene call: settle the previous update, then apply the
-        tasks in order.  Each task is a tuple (kind, args...)."""
+        tasks in order.  Each task is a tuple (kind, args...).  Light moves
+        are tracked in the index space of the list as the update found it
+        (the previous frame's): origin[l] is the original index of the light
+        now at l, and each original index that moved gets one remapping
+        entry, its final position.  Returns the StateErrors of refused
+        tasks."""
         self._settle()
+        self._origin = list(range(len(self.lights)))
+        self._final = {}
+        self._porigin, self._pfinal = {}, {}
+        refused = []
         self._gpu_deletions([t for t in tasks if t[0] == "delete_from_gpu"])
         for t in tasks:
-            getattr(self, "_" + t[0])(*t[1:])
+            try:
+                getattr(self, "_" + t[0])(*t[1:])
+            except StateError as e:
+                refused.append(e)
+        self.remap_lights.update(self._final)
+        for (i, o), f in self._pfinal.items():
+            self.remap_prims[(i, o)] = f
         self.settling = True
+        return refused
+
+    def _prim_origin(self, i, n):
+        if i not in self._porigin:
+            self._porigin[i] = list(range(n))
+        return self._porigin[i]
+
+    def _prim_removed(self, i, n, l):
+        org = self._prim_origin(i, n)
+        if l < len(org) and org[l] is not None:
+            self._pfinal[(i, org[l])] = INVALID
+
+    def _prim_moved(self, i, n, src, dst):
+        org = self._prim_origin(i, n)
+        o = org[src] if src < len(org) else None
+        while len(org) <= dst:
+            org.append(None)
+        org[dst] = o
+        if o is not None:
+            self._pfinal[(i, o)] = dst
+
+    def _prim_restored(self, i, n, l):
+        org = self._prim_origin(i, n)
+        while len(org) <= l:
+            org.append(None)
+        org[l] = None
+
+    def _light_removed(self, l):
+        o = self._origin[l] if l < len(self._origin) else None
+        if o is not None:
+            self._final[o] = INVALID
+
+    def _light_moved(self, src, dst):
+        o = self._origin[src] if src < len(self._origin) else None
+        while len(self._origin) <= dst:
+            self._origin.append(None)
+        self._origin[dst] = o
+        if o is not None:
+            self._final[o] = dst
+
+    def _light_restored(self, l):
+        while len(self._origin) <= l:
+            self._origin.append(None)
+        self._origin[l] = None
 
     def _gpu_deletions(self, tasks):
         """The GPU deletions of one update, before the task loop: each
@@ -166,15 +233,13 @@ class AccelModel:
                     alive.add(v.light)
             gone.update(L for L in old if L != INVALID and L < len(self.lights))
         gone -= alive
-        origin = list(range(len(self.lights)))
         for L in sorted(gone, reverse=True):
             Lx = len(self.lights) - 1
-            self.remap_lights[origin[L]] = INVALID
+            self._light_removed(L)
             if Lx != L:
                 self.lights[L] = copy.deepcopy(self.lights[Lx])
-                origin[L] = origin[Lx]
+                self._light_moved(Lx, L)
                 self._relink(L)
-                self.remap_lights[origin[L]] = L
             self.lights.pop()
 
     def _settle(self):
@@ -203,8 +268,10 @@ class AccelModel:
             Lx = len(self.lights) - 1
             if Lx == L:
                 Lx = INVALID
+            self._light_removed(L)
             if Lx != INVALID:
                 self.lights[L] = copy.deepcopy(self.lights[Lx])
+                self._light_moved(Lx, L)
                 self._relink(L)
             self.lights.pop()          # the list shrinks by one; its last entry is now stale
         if last != d:
@@ -213,14 +280,22 @@ class AccelModel:
                 self.lights[vox[d].light].primitive = d
         vox.pop()
         rec.light, rec.light_moved = L, Lx
-        self.remap_prims[(i, d)] = INVALID
-        if last != d:
-            self.remap_prims[(i, last)] = d
-        if L != INVALID:
-            self.remap_lights[L] = INVALID
+        rec.lights_after = len(self.lights)
         if Lx != INVALID:
-            self.remap_lights[Lx] = L
+            rec.moved_rec = copy.deepcopy(self.lights[L])
+        self._prim_removed(i, last + 1, d)
+        if last != d:
+            self._prim_moved(i, last + 1, last, d)
         self.done.append(rec)
+
+    def _lights_as_left(self, rec):
+        """The light list is still the one the deletion left: same length
+        and the record it moved into the hole still there."""
+        if rec.light == INVALID:
+            return True
+        if len(self.lights) != rec.lights_after:
+            return False
+        return rec.light_moved == INVALID or self.lights[rec.light] == rec.moved_rec
 
     def _undo(self):
         if not self.done:
@@ -228,24 +303,29 @@ class AccelModel:
         rec = self.done.pop()
         if rec.kind != "delete_primitive":
             return
-        inst = self.instances[rec.instance]
+        inst = self.instances.get(rec.instance)
+        if inst is None or len(inst.voxels) != rec.last:
+            raise StateError("UNDO of a primitive deletion whose instance has changed since")
+        if not self._lights_as_left(rec):
+            raise StateError("UNDO of a primitive deletion whose cube light list has changed since")
         vox = inst.voxels
-        assert len(vox) == rec.last
         d, last = rec.deleted, rec.last
         vox.append(None)
         if last != d:
             vox[last] = vox[d]
             if vox[last].light != INVALID:
                 self.lights[vox[last].light].primitive = last
-            self.remap_prims[(rec.instance, d)] = last
+            self._prim_moved(rec.instance, last, d, last)
+        self._prim_restored(rec.instance, last, d)
         vox[d] = copy.deepcopy(rec.voxel)
         if rec.light != INVALID:
             self.lights.append(None)
             if rec.light_moved != INVALID:
                 self.lights[rec.light_moved] = copy.deepcopy(self.lights[rec.light])
+                self._light_moved(rec.light, rec.light_moved)
                 self._relink(rec.light_moved)
-                self.remap_lights[rec.light] = rec.light_moved
             self.lights[rec.light] = copy.deepcopy(rec.light_rec)
+            self._light_restored(rec.light)
 
     def _delete_blas(self, i):
         del self.instances[i]

@@ -159,3 +159,51 @@ def test_gpu_deletion_removes_the_lights_of_deleted_voxels(gpu):
         assert W2["prims"][inst[o.instance_id].first_primitive_index + o.primitive_id, 1] == l
     osc = _oracle(acc, sc, ctx)
     _trace_equal(acc, osc, sc, 37)
+
+
+def test_undo_after_a_gpu_deletion_that_removed_lights(gpu):
+    """A CPU deletion of a lit voxel, then a GPU deletion that removes lights,
+    then two undos: the first pops the GPU deletion and changes nothing, the
+    second is refused with CT_E_STATE because the light list is no longer
+    the one the CPU deletion left; records stay equal to the task model's.
+    Also: a load while deletions are queued is refused."""
+    w = _workload()
+    sc, acc, ctx = _load(w)
+    acc.update_scene()
+    m = _model(acc)
+    W = acc.read_world()
+    before = _snapshot(acc)
+    ids = sorted(k for k in before if isinstance(k, int))
+    lit = {i: [k for k in range(before[i][1]) if W["prims"][before[i][0] + k, 1] != 0xFFFFFFFF] for i in ids}
+    a, b = [i for i in ids if len(lit[i]) >= 3][:2]
+    acc.delete_primitive(b, lit[b][0])
+    acc.update_scene()
+    m.update([("delete_primitive", b, lit[b][0])])
+    _compare(acc, m)
+    acc.update_scene()  # settle
+    m.update([])
+    before = _snapshot(acc)
+    plan = {a: lit[a][:2]}
+    acc.brush_mark([(a, p) for p in plan[a]])
+    assert acc.process_voxel_modifications()[0] == 1
+    with pytest.raises(ctamd.CtError) as e:
+        ctamd.load_scene(sc, acc, ctx)  # a GPU deletion is queued
+    assert e.value.code == ctamd.CT_E_STATE
+    _check_compaction(acc, m, plan, before)
+    acc.update_scene()  # settle
+    m.update([])
+    acc.undo()  # pops the GPU deletion
+    acc.update_scene()
+    assert m.update([("undo",)]) == []
+    _compare(acc, m)
+    acc.update_scene()
+    m.update([])
+    acc.undo()
+    with pytest.raises(ctamd.CtError) as e:
+        acc.update_scene()
+    assert e.value.code == ctamd.CT_E_STATE
+    refused = m.update([("undo",)])
+    assert len(refused) == 1
+    _compare(acc, m)
+    osc = _oracle(acc, sc, ctx)
+    _trace_equal(acc, osc, sc, 41)

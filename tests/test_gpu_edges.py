@@ -5,6 +5,7 @@ import ctypes as C
 
 import numpy as np
 import pytest
+import torch
 
 import ctamd
 from helpers import gpu_scene
@@ -49,6 +50,18 @@ def test_state_and_argument_errors(gpu):
     p_di = ctamd.frame_params(w, 0)
     p_di.restir.indirect = 0  # parameters without paths: refused before the output is touched
     assert L.ct_pt_random_replay(ctx2.handle, acc.handle, C.byref(p_di), 0, 0, None, None) == ctamd.CT_E_INVALID
-    out = (C.c_uint32 * 1)()  # host memory: the state check must refuse first
-    assert L.ct_pt_random_replay(ctx2.handle, acc.handle, C.byref(p_di), 0, 0, out, None) == ctamd.CT_E_STATE
+    # A device buffer of the full output size, so that a refusal that came
+    # too late would still write in bounds.
+    out = torch.zeros(16 * 8 * 8, dtype=torch.int32, device="cuda")
+    optr = C.c_void_p(out.data_ptr())
+    assert L.ct_pt_random_replay(ctx2.handle, acc.handle, C.byref(p_di), 0, 0, optr, None) == ctamd.CT_E_STATE
+    # the frame's own argument checks apply to a replay as well
+    for field, value, code in (("max_depth", 99, ctamd.CT_E_INVALID), ("pt_neighbor_count", 99, ctamd.CT_E_INVALID),
+                               ("pt_shift_mapping", 7, ctamd.CT_E_UNSUPPORTED),
+                               ("max_neighbors", 99, ctamd.CT_E_INVALID)):
+        bad = ctamd.frame_params(w, 0)
+        setattr(bad.restir, field, value)
+        assert L.ct_pt_random_replay(ctx2.handle, acc.handle, C.byref(bad), 0, 0, optr, None) == code, field
+    torch.cuda.synchronize()
+    assert int(out.abs().sum()) == 0
     assert L.ct_brush_mark(acc.handle, None, 3, None) == ctamd.CT_E_INVALID

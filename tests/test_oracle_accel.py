@@ -208,3 +208,106 @@ def test_update_transform_is_the_column_major_product():
     assert np.array_equal(m.prev_transform[2], np.eye(4, dtype=np.float32).reshape(16))
     m.update([])
     assert np.array_equal(m.prev_transform[2], _translation(0, 1, 0))
+
+
+def _check_light_remap(m, prev_lights):
+    now_l = {r.data: k for k, r in enumerate(m.lights)}
+    for l, data in enumerate(prev_lights):
+        e = m.remap_lights.get(l, 0)
+        if data not in now_l:
+            assert e == INVALID, l
+        elif e == 0:
+            assert now_l[data] in (l, 0), l
+        else:
+            assert e == now_l[data], l
+
+
+def _check_prim_remap(m, prev):
+    for i, inst in m.instances.items():
+        now = {v.aabb: k for k, v in enumerate(inst.voxels)}
+        for p, box in enumerate(prev.get(i, [])):
+            e = m.remap_prims.get((i, p), 0)
+            if box not in now:
+                assert e == INVALID, (i, p)
+            elif e == 0:
+                assert now[box] in (p, 0), (i, p)
+            else:
+                assert e == now[box], (i, p)
+
+
+def _unique_light_scene(seed):
+    m = _scene(np.random.default_rng(seed), light_frac=0.5)
+    for k, r in enumerate(m.lights):  # unique records, so a light can be followed by its bytes
+        r.data = k.to_bytes(4, "little") * 10
+    return m
+
+
+def test_remapping_composes_repeated_deletions_of_one_slot():
+    """Deleting the voxel that the previous deletion of the same update moved
+    into the hole: the first moved voxel is gone too, and its remapping entry
+    must say so (entries are composed over the update, not written per
+    task)."""
+    m = _unique_light_scene(11)
+    lit = [k for k, v in enumerate(m.instances[2].voxels) if v.light != INVALID]
+    for d in (5, lit[0]):
+        prev = {i: [v.aabb for v in inst.voxels] for i, inst in m.instances.items()}
+        prev_lights = [r.data for r in m.lights]
+        m.update([("delete_primitive", 2, d), ("delete_primitive", 2, d), ("delete_primitive", 2, d)])
+        _check_links(m)
+        _check_prim_remap(m, prev)
+        _check_light_remap(m, prev_lights)
+
+
+def test_remapping_composes_gpu_and_cpu_deletions():
+    """GPU deletions of an update are applied first (their compaction already
+    ran); CPU deletions in the same update then move lights that the GPU
+    removal had moved: every previous light index maps to where its record
+    is now."""
+    rng = np.random.default_rng(12)
+    for trial in range(20):
+        m = _unique_light_scene(300 + trial)
+        prev_lights = [r.data for r in m.lights]
+        vox = m.instances[0].voxels
+        keep = [v for k, v in enumerate(vox) if rng.random() < 0.7]
+        dropped = len(vox) - len(keep)
+        compacted = keep + [v for v in vox if v not in keep]
+        tasks = [("delete_from_gpu", 0, compacted, dropped)]
+        for _ in range(3):
+            i = int(rng.integers(1, 3))
+            n = len(m.instances[i].voxels) - sum(1 for t in tasks if t[0] == "delete_primitive" and t[1] == i)
+            tasks.append(("delete_primitive", i, int(rng.integers(0, n))))
+        m.update(tasks)
+        _check_links(m)
+        _check_light_remap(m, prev_lights)
+
+
+def test_undo_refused_after_a_gpu_deletion_removed_lights():
+    """A CPU deletion's undo record names light indices; a GPU deletion
+    that removes lights in between changes them, so the undo is refused
+    (CT_E_STATE in the library) and nothing changes."""
+    m = _unique_light_scene(13)
+    lit = [k for k, v in enumerate(m.instances[1].voxels) if v.light != INVALID]
+    m.update([("delete_primitive", 1, lit[0])])
+    vox = m.instances[0].voxels
+    lit0 = [v for v in vox if v.light != INVALID]
+    keep = [v for v in vox if v is not lit0[0]]
+    m.update([("delete_from_gpu", 0, keep + [lit0[0]], 1)])
+    before = _snapshot(m)
+    refused = m.update([("undo",)])       # pops the GPU deletion: nothing to undo
+    assert not refused and _snapshot(m) == before
+    refused = m.update([("undo",)])       # the CPU deletion: its light list has changed
+    assert len(refused) == 1 and "light list" in str(refused[0])
+    assert _snapshot(m) == before
+    _check_links(m)
+
+
+def test_undo_still_applies_when_the_light_list_is_as_left():
+    m = _unique_light_scene(14)
+    snap = _snapshot(m)
+    lit = [k for k, v in enumerate(m.instances[1].voxels) if v.light != INVALID]
+    m.update([("delete_primitive", 1, lit[0])])
+    m.update([("update_blas", 2, np.eye(4, dtype=np.float32).reshape(16), [])])
+    m.update([("undo",)])
+    assert m.update([("undo",)]) == []
+    assert _snapshot(m) == snap
+    _check_links(m)
