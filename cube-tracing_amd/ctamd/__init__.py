@@ -297,6 +297,8 @@ _SIGS = {
                           C.POINTER(TraceStats)], C.c_int),
     "ct_trace_any": ([C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                       C.POINTER(TraceStats)], C.c_int),
+    "ct_trace_any_noalloc": ([C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                              C.c_void_p], C.c_int),
     "ct_accel_read_instances": ([C.c_void_p, C.POINTER(Instance), u32, C.POINTER(u32)], C.c_int),
     "ct_adjoint_render": ([C.c_void_p, C.POINTER(AdjointDesc), C.POINTER(AdjointView), C.c_void_p, C.c_void_p,
                            C.c_void_p, C.c_void_p], C.c_int),

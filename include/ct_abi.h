@@ -395,6 +395,20 @@ int ct_trace_any(ct_accel *accel, uint32_t n, const float *origins,
                  const uint32_t *targets, uint8_t *occluded_out,
                  ct_trace_stats *stats);
 
+/* ct_trace_any without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Same rays, same result bytes as
+ * ct_trace_any, but every array is in DEVICE memory and the call is only
+ * enqueued on `stream` (null: the accel's stream); nothing is copied back
+ * and nothing is waited for.  The staged rays and the work counters live in
+ * a fixed 36 MiB device array per GPU, so n is at most 2^20 (CT_E_CAPACITY
+ * above); the array is shared by every call on that GPU, so calls that
+ * overlap on one GPU race. */
+int ct_trace_any_noalloc(ct_accel *accel, uint32_t n, const float *origins,
+                         const float *directions, const float *distances,
+                         const uint32_t *targets, uint8_t *occluded_out,
+                         void *stream);
+
 /* Path-segment closest hit, intersect() of bounce.glsl:9-130: the box
  * tests start at compute_ray_origin(origin, direction) and the closest
  * voxel is returned (the shader commits the first one its query
