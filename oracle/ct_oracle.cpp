@@ -1084,6 +1084,9 @@ static void temporal_reuse(const Ctx &c, Res &r, Res prev, uint32_t pidx, V3 ray
   r.M = new_M;
 }
 
+static void pairwise_combine(const Ctx &c, Res &r, uint32_t num, const Res *nres, const ct_di_info *const *ndi,
+                             V3 ray_d, Hit &hit, const ct_material &mat, uint32_t &seed, TraceCount &cnt);
+
 // pairwise_mis.glsl:8-114 and SPATIAL_REUSE, restir_resampling.glsl:418-540
 static void spatial_reuse(const Ctx &c, Res &r, float confidence, uint32_t x, uint32_t y,
                           V3 ray_d, Hit &hit, const ct_material &mat, uint32_t &seed,
@@ -1109,6 +1112,20 @@ static void spatial_reuse(const Ctx &c, Res &r, float confidence, uint32_t x, ui
     if (num < CT_MAX_NEIGHBORS) idx[num++] = l;
   }
   if (num == 0) return;
+  Res nres[CT_MAX_NEIGHBORS];
+  const ct_di_info *ndi[CT_MAX_NEIGHBORS];
+  for (uint32_t i = 0; i < num; ++i) {
+    nres[i] = res_from(c.fr->res_int[idx[i]]);
+    ndi[i] = &c.fr->di_cur[idx[i]];
+  }
+  pairwise_combine(c, r, num, nres, ndi, ray_d, hit, mat, seed, cnt);
+}
+
+// The pairwise MIS combination of spatial reuse over `num` chosen
+// neighbours (pairwise_mis.glsl:8-114): pairwise_init, one pairwise_stream
+// per neighbour, pairwise_end.
+static void pairwise_combine(const Ctx &c, Res &r, uint32_t num, const Res *nres, const ct_di_info *const *ndi,
+                             V3 ray_d, Hit &hit, const ct_material &mat, uint32_t &seed, TraceCount &cnt) {
   // pairwise_init
   Res mres;
   init_res(mres);
@@ -1116,8 +1133,8 @@ static void spatial_reuse(const Ctx &c, Res &r, float confidence, uint32_t x, ui
   float M_s = r.M;
   const float fk = (float)num;
   for (uint32_t i = 0; i < num; ++i) {
-    Res in = res_from(c.fr->res_int[idx[i]]);
-    const ct_di_info &nd = c.fr->di_cur[idx[i]];
+    Res in = nres[i];
+    const ct_di_info &nd = *ndi[i];
     Hit nh = hit_from_di(nd);
     V3 nray_d = normalize(sub(nh.world_hit, load3(nd.ray_origin)));
     const ct_material &nm = material(c, nd.mat_index);
@@ -2788,6 +2805,76 @@ int oc_frame_read(oc_frame *frame, uint32_t which, void *dst, uint64_t bytes) {
 int oc_frame_get_stats(oc_frame *frame, ct_frame_stats *out) {
   if (!frame || !out) return CT_E_INVALID;
   *out = frame->stats;
+  return CT_OK;
+}
+
+// ---------------------------------------------------------------- probes
+// Single passes of the frame on caller-given inputs, for the tests that pin
+// the oracle's ReSTIR arithmetic against independent models.
+
+static Ctx probe_ctx(const oc_scene *scene, const ct_frame_params *p) {
+  Ctx c;
+  c.sc = scene;
+  c.fr = nullptr;
+  c.p = p;
+  return c;
+}
+
+// RIS (restir_resampling.glsl:19-255, with its visibility pass) at the
+// shading point `di` describes, the camera ray from di.ray_origin through
+// di.position, once per seed.
+int oc_probe_ris(oc_scene *scene, const ct_frame_params *p, const ct_di_info *di, float confidence, uint32_t n,
+                 const uint32_t *seeds, ct_reservoir *out) {
+  if (!scene || !p || !di || (n && (!seeds || !out))) return CT_E_INVALID;
+  if (p->is_active & CT_RIS_BRDF_BIT) return CT_E_UNSUPPORTED;
+  Ctx c = probe_ctx(scene, p);
+  const ct_material &mat = material_or_zero(*scene, di->mat_index);
+  const V3 rd = normalize(sub(load3(di->position), load3(di->ray_origin)));
+  for (uint32_t k = 0; k < n; ++k) {
+    Hit hit = hit_from_di(*di);
+    uint32_t seed = seeds[k];
+    TraceCount cnt;
+    out[k] = res_to(ris(c, confidence, rd, hit, mat, seed, cnt));
+  }
+  return CT_OK;
+}
+
+// The pairwise MIS combination of spatial reuse (pairwise_mis.glsl:8-114)
+// of the reservoir `r` at `center` with the given neighbours, as
+// spatial_reuse runs it once its neighbours are chosen.
+int oc_probe_pairwise(oc_scene *scene, const ct_frame_params *p, const ct_di_info *center, const ct_reservoir *r,
+                      uint32_t num, const ct_di_info *ndi, const ct_reservoir *nres, uint32_t seed,
+                      ct_reservoir *out) {
+  if (!scene || !p || !center || !r || !out || num == 0 || num > CT_MAX_NEIGHBORS || !ndi || !nres)
+    return CT_E_INVALID;
+  Ctx c = probe_ctx(scene, p);
+  const ct_material &mat = material_or_zero(*scene, center->mat_index);
+  const V3 rd = normalize(sub(load3(center->position), load3(center->ray_origin)));
+  Res rr = res_from(*r);
+  Res nr[CT_MAX_NEIGHBORS];
+  const ct_di_info *nd[CT_MAX_NEIGHBORS];
+  for (uint32_t i = 0; i < num; ++i) {
+    nr[i] = res_from(nres[i]);
+    nd[i] = &ndi[i];
+  }
+  Hit hit = hit_from_di(*center);
+  TraceCount cnt;
+  pairwise_combine(c, rr, num, nr, nd, rd, hit, mat, seed, cnt);
+  *out = res_to(rr);
+  return CT_OK;
+}
+
+// The shift of path reservoir `res` from the primary vertex `src` to the
+// primary vertex `dst` (compute_shifted_integrand_, shift.glsl:446-483),
+// with visibility: the shifted integrand and its Jacobian.
+int oc_probe_shift(oc_scene *scene, const ct_frame_params *p, const ct_di_info *dst, const ct_di_info *src,
+                   int normalize_dst_wo, const ct_path_reservoir *res, float *jacobian, float integrand[3]) {
+  if (!scene || !p || !dst || !src || !res || !jacobian || !integrand) return CT_E_INVALID;
+  Ctx c = probe_ctx(scene, p);
+  const Isect di = isect_from_di(c, *dst, normalize_dst_wo != 0), si = isect_from_di(c, *src, true);
+  TraceCount cnt;
+  V3 f = shifted_integrand(c, *jacobian, di, si, *res, true, cnt);
+  store3(integrand, f);
   return CT_OK;
 }
 

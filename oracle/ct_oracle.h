@@ -125,6 +125,15 @@ int oc_frame_render(oc_frame *frame, const ct_frame_params *params,
 int oc_frame_read(oc_frame *frame, uint32_t which, void *dst, uint64_t bytes);
 int oc_frame_get_stats(oc_frame *frame, ct_frame_stats *out);
 
+/* Probes (tests only): single passes of the frame on given inputs. */
+int oc_probe_ris(oc_scene *scene, const ct_frame_params *p, const ct_di_info *di, float confidence, uint32_t n,
+                 const uint32_t *seeds, ct_reservoir *out);
+int oc_probe_pairwise(oc_scene *scene, const ct_frame_params *p, const ct_di_info *center, const ct_reservoir *r,
+                      uint32_t num, const ct_di_info *ndi, const ct_reservoir *nres, uint32_t seed,
+                      ct_reservoir *out);
+int oc_probe_shift(oc_scene *scene, const ct_frame_params *p, const ct_di_info *dst, const ct_di_info *src,
+                   int normalize_dst_wo, const ct_path_reservoir *res, float *jacobian, float integrand[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,6 +78,11 @@ def lib():
             "oc_frame_render": ([C.c_void_p, P(FrameParams), C.c_int, u32, u32], C.c_int),
             "oc_frame_read": ([C.c_void_p, u32, C.c_void_p, C.c_uint64], C.c_int),
             "oc_frame_get_stats": ([C.c_void_p, P(FrameStats)], C.c_int),
+            "oc_probe_ris": ([C.c_void_p, P(FrameParams), P(DiInfo), f32, u32, C.c_void_p, C.c_void_p], C.c_int),
+            "oc_probe_pairwise": ([C.c_void_p, P(FrameParams), P(DiInfo), P(Reservoir), u32, C.c_void_p,
+                                   C.c_void_p, u32, P(Reservoir)], C.c_int),
+            "oc_probe_shift": ([C.c_void_p, P(FrameParams), P(DiInfo), P(DiInfo), C.c_int, P(PathReservoir),
+                                P(f32), P(f32)], C.c_int),
         }
         for name, (a, r) in sig.items():
             fn = getattr(L, name)
@@ -230,6 +235,40 @@ class OScene:
                                          self._rl.ctypes.data if self._rl.size else None, self._rl.size)
         if rc != 0:
             raise ValueError(f"oc_scene_set_previous: {rc}")
+
+    # ------------------------------------------------------------ probes
+    def probe_ris(self, params, di, seeds, confidence=0.0):
+        """RIS with its visibility pass at the shading point `di` (a DiInfo),
+        once per seed: (n, 9) uint32 reservoir words (Y, type, seed, W_y,
+        W_sum, M, F.rgb; floats as their bits)."""
+        seeds = np.ascontiguousarray(seeds, np.uint32)
+        out = (Reservoir * len(seeds))()
+        rc = lib().oc_probe_ris(self.handle, C.byref(params), C.byref(di), confidence, len(seeds),
+                                seeds.ctypes.data, C.addressof(out))
+        assert rc == 0, rc
+        return np.frombuffer(bytes(out), np.uint32).reshape(len(seeds), 9).copy()
+
+    def probe_pairwise(self, params, center, res, neighbours, neighbour_res, seed):
+        """Pairwise-MIS combination of spatial reuse for `res` at `center`
+        with the given neighbour DiInfos and Reservoirs."""
+        k = len(neighbours)
+        nd = (DiInfo * k)(*neighbours)
+        nr = (Reservoir * k)(*neighbour_res)
+        out = Reservoir()
+        rc = lib().oc_probe_pairwise(self.handle, C.byref(params), C.byref(center), C.byref(res), k,
+                                     C.addressof(nd), C.addressof(nr), seed, C.byref(out))
+        assert rc == 0, rc
+        return out
+
+    def probe_shift(self, params, dst, src, res, normalize_dst_wo=True):
+        """Shift of path reservoir `res` from primary vertex `src` to `dst`:
+        (jacobian, integrand rgb)."""
+        jac = f32(0.0)
+        f = (f32 * 3)()
+        rc = lib().oc_probe_shift(self.handle, C.byref(params), C.byref(dst), C.byref(src),
+                                  1 if normalize_dst_wo else 0, C.byref(res), C.byref(jac), f)
+        assert rc == 0, rc
+        return float(jac.value), np.array(f[:], np.float32)
 
     def read_bvh(self, instance, n):
         keys = np.zeros(n, np.uint32)
