@@ -2839,6 +2839,29 @@ int oc_probe_ris(oc_scene *scene, const ct_frame_params *p, const ct_di_info *di
   return CT_OK;
 }
 
+// Temporal reuse (restir_resampling.glsl:311-416) of the reservoir `r` at
+// `di` with the previous frame's reservoir `prev` and DI `prev_di` at the
+// predicted pixel, without remapping.
+int oc_probe_temporal(oc_scene *scene, const ct_frame_params *p, const ct_di_info *di, const ct_reservoir *r,
+                      const ct_di_info *prev_di, const ct_reservoir *prev, uint32_t seed, ct_reservoir *out) {
+  if (!scene || !p || !di || !r || !prev_di || !prev || !out) return CT_E_INVALID;
+  if (p->is_active & CT_REMAP_BIT) return CT_E_INVALID;
+  Frame fr;
+  fr.sc = scene;
+  fr.W = fr.H = 1;
+  fr.di_prev.assign(1, *prev_di);
+  Ctx c = probe_ctx(scene, p);
+  c.fr = &fr;
+  const ct_material &mat = material_or_zero(*scene, di->mat_index);
+  const V3 rd = normalize(sub(load3(di->position), load3(di->ray_origin)));
+  Hit hit = hit_from_di(*di);
+  Res rr = res_from(*r);
+  TraceCount cnt;
+  temporal_reuse(c, rr, res_from(*prev), 0, rd, hit, mat, seed, cnt);
+  *out = res_to(rr);
+  return CT_OK;
+}
+
 // The pairwise MIS combination of spatial reuse (pairwise_mis.glsl:8-114)
 // of the reservoir `r` at `center` with the given neighbours, as
 // spatial_reuse runs it once its neighbours are chosen.

@@ -128,6 +128,8 @@ int oc_frame_get_stats(oc_frame *frame, ct_frame_stats *out);
 /* Probes (tests only): single passes of the frame on given inputs. */
 int oc_probe_ris(oc_scene *scene, const ct_frame_params *p, const ct_di_info *di, float confidence, uint32_t n,
                  const uint32_t *seeds, ct_reservoir *out);
+int oc_probe_temporal(oc_scene *scene, const ct_frame_params *p, const ct_di_info *di, const ct_reservoir *r,
+                      const ct_di_info *prev_di, const ct_reservoir *prev, uint32_t seed, ct_reservoir *out);
 int oc_probe_pairwise(oc_scene *scene, const ct_frame_params *p, const ct_di_info *center, const ct_reservoir *r,
                       uint32_t num, const ct_di_info *ndi, const ct_reservoir *nres, uint32_t seed,
                       ct_reservoir *out);

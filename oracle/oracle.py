@@ -79,6 +79,8 @@ def lib():
             "oc_frame_read": ([C.c_void_p, u32, C.c_void_p, C.c_uint64], C.c_int),
             "oc_frame_get_stats": ([C.c_void_p, P(FrameStats)], C.c_int),
             "oc_probe_ris": ([C.c_void_p, P(FrameParams), P(DiInfo), f32, u32, C.c_void_p, C.c_void_p], C.c_int),
+            "oc_probe_temporal": ([C.c_void_p, P(FrameParams), P(DiInfo), P(Reservoir), P(DiInfo), P(Reservoir), u32,
+                                   P(Reservoir)], C.c_int),
             "oc_probe_pairwise": ([C.c_void_p, P(FrameParams), P(DiInfo), P(Reservoir), u32, C.c_void_p,
                                    C.c_void_p, u32, P(Reservoir)], C.c_int),
             "oc_probe_shift": ([C.c_void_p, P(FrameParams), P(DiInfo), P(DiInfo), C.c_int, P(PathReservoir),
@@ -247,6 +249,15 @@ class OScene:
                                 seeds.ctypes.data, C.addressof(out))
         assert rc == 0, rc
         return np.frombuffer(bytes(out), np.uint32).reshape(len(seeds), 9).copy()
+
+    def probe_temporal(self, params, di, res, prev_di, prev_res, seed):
+        """Temporal reuse of `res` at `di` with the previous frame's
+        reservoir and DI at the predicted pixel."""
+        out = Reservoir()
+        rc = lib().oc_probe_temporal(self.handle, C.byref(params), C.byref(di), C.byref(res), C.byref(prev_di),
+                                     C.byref(prev_res), seed, C.byref(out))
+        assert rc == 0, rc
+        return out
 
     def probe_pairwise(self, params, center, res, neighbours, neighbour_res, seed):
         """Pairwise-MIS combination of spatial reuse for `res` at `center`

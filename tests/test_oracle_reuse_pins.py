@@ -1,6 +1,8 @@
 """Pins of the oracle's reuse passes by what fixes them independently of its
 code:
 
+* temporal reuse (restir_resampling.glsl:311-416): a history that holds
+  the pixel's own sample leaves W_y unchanged for any M split;
 * pairwise MIS (pairwise_mis.glsl:8-114, restir_resampling.glsl:418-540):
   k neighbours that are copies of the pixel itself (same surface, same
   reservoir) give m_i = k/(k+1) per copy and m_c = 1 + k/(k+1), so the
@@ -97,6 +99,39 @@ def test_pairwise_mis_with_a_neighbour_that_sees_nothing(floor):
         got = osc.probe_pairwise(p, di, r, [nd], [nr], seed=99 + n)
         assert got.Y == r.Y and got.M == r.M + nr.M
         assert abs(got.W_y - r.W_y) <= 4e-6 * r.W_y, (n, got.W_y, r.W_y)
+
+
+@pytest.mark.parametrize("m_prev", [32.0, 96.0, 640.0, 5000.0])
+def test_temporal_reuse_of_the_own_sample_keeps_W(floor, m_prev):
+    """The previous reservoir holds the same sample at the same surface
+    (static pixel): the balance-heuristic weights are M/(M + M') and
+    M'/(M + M') of the same p_hat, which sum to one, so W_y is unchanged for
+    any M split (M' capped at 20 M, restir_resampling.glsl:317-319) and M
+    becomes M + min(M', 20 M)."""
+    osc, p, points = floor
+    for n, (di, r) in enumerate(points):
+        prev = ctamd.Reservoir.from_buffer_copy(bytes(r))
+        prev.M = m_prev
+        got = osc.probe_temporal(p, di, r, di, prev, seed=77 + n)
+        assert got.Y == r.Y and list(got.F) == list(r.F)
+        assert got.M == np.float32(r.M + min(m_prev, 20.0 * r.M))
+        assert abs(got.W_y - r.W_y) <= 4e-6 * r.W_y, (n, got.W_y, r.W_y)
+
+
+def test_temporal_reuse_with_a_history_without_sample(floor):
+    """A previous reservoir with M' but no sample, at the same surface: the
+    current sample's balance weight is M p / (M p + M' p) (the history could
+    have produced it) and nothing else enters, so W_y scales by M / (M + M')
+    and M becomes M + M'."""
+    osc, p, points = floor
+    for n, (di, r) in enumerate(points[:20]):
+        prev = ctamd.Reservoir()
+        prev.Y = INVALID
+        prev.M = 5.0
+        got = osc.probe_temporal(p, di, r, di, prev, seed=5 + n)
+        assert got.Y == r.Y and got.M == r.M + 5.0
+        expect = r.W_y * r.M / (r.M + 5.0)
+        assert abs(got.W_y - expect) <= 4e-6 * expect, (got.W_y, expect)
 
 
 # ------------------------------------------------------ reconnection shift
