@@ -665,6 +665,21 @@ DScene scene_view(const Accel &a) {
   return s;
 }
 
+DScene previous_scene_view(const Accel &a) {
+  if (!previous_distinct(a)) return scene_view(a);
+  DScene s;
+  s.blas_nodes = a.prev.nodes.as<DNode>();
+  s.blas_leaves = a.prev.leaves.as<float4>();
+  s.inst = a.prev.instances_d.as<DInst>();
+  s.tlas_nodes = a.prev.tlas_nodes.as<DNode>();
+  s.tlas_inst = a.prev.tlas_inst.as<uint32_t>();
+  s.tlas_root = a.prev.tlas_root;
+  s.n_inst = a.prev.n_inst;
+  return s;
+}
+
+bool previous_distinct(const Accel &a) { return a.status == CT_AS_SETTLING && a.prev.valid; }
+
 // What an update changed: instances whose BLAS is rebuilt from scratch,
 // instances refitted in place, slot ranges whose AABBs/primitives are
 // uploaded, and whether buffers must grow (everything is then rebuilt).
@@ -1497,6 +1512,42 @@ void last_write_wins(std::vector<uint2> &w) {
   w.resize(n);
 }
 
+int copy_device(DevBuf &dst, const DevBuf &src, size_t bytes, hipStream_t st) {
+  if (!src.p || bytes == 0) return CT_OK;
+  int rc;
+  if ((rc = dst.reserve(bytes))) return rc;
+  CT_HIP(hipMemcpyAsync(dst.p, src.p, bytes, hipMemcpyDeviceToDevice, st));
+  return CT_OK;
+}
+
+// Copies the committed scene into the previous set before an update
+// changes it (stream-ordered device copies; about 116 B per primitive slot).
+int snapshot_previous(Accel &a, hipStream_t st) {
+  const BvhSet &S = a.set;
+  PrevSet &P = a.prev;
+  P.valid = false;
+  if (!S.nodes.p || S.tlas_count == 0) return CT_OK;
+  const size_t cap = S.prim_capacity, ni = a.instances.size();
+  int rc;
+  if ((rc = copy_device(P.nodes, S.nodes, sizeof(DNode) * cap, st)) ||
+      (rc = copy_device(P.leaves, S.leaves, 16 * cap, st)) ||
+      (rc = copy_device(P.aabbs, S.aabbs, sizeof(ct_aabb) * cap, st)) ||
+      (rc = copy_device(P.prims, S.prims, sizeof(ct_primitive) * cap, st)) ||
+      (rc = copy_device(P.instances_d, S.instances_d, sizeof(DInst) * ni, st)) ||
+      (rc = copy_device(P.inst_aos, S.inst_aos, sizeof(ct_instance) * ni, st)) ||
+      (rc = copy_device(P.tlas_nodes, S.tlas_nodes, S.tlas_nodes.bytes, st)) ||
+      (rc = copy_device(P.tlas_inst, S.tlas_inst, S.tlas_inst.bytes, st)))
+    return rc;
+  P.tlas_root = S.tlas_root;
+  P.tlas_count = S.tlas_count;
+  P.tlas_depth = S.tlas_depth;
+  P.generation = S.generation;
+  P.n_inst = (uint32_t)ni;
+  P.prim_capacity = (uint32_t)cap;
+  P.valid = true;
+  return CT_OK;
+}
+
 int scatter(hipStream_t st, const std::vector<uint2> &w, uint32_t *dst) {
   if (w.empty()) return CT_OK;
   DevBuf buf;
@@ -1551,12 +1602,14 @@ int ct_accel_update_scene(ct_accel *a, int synchronize, void *stream, int *did_w
   if (a->status == CT_AS_SETTLING) {
     if ((rc = settle(*a, st))) return rc;
     a->status = CT_AS_IDLE;
+    a->prev.valid = false;
     worked = true;
   }
   if (queue.empty()) {
     if (did_work) *did_work = worked ? 1 : 0;
     return CT_OK;
   }
+  if ((rc = snapshot_previous(*a, st))) return rc;
   // process_task_queue, accel_struct_mngr.cpp:2115-2443
   a->status = CT_AS_UPDATING;
   Update u;
@@ -1681,7 +1734,15 @@ int ct_accel_is_remapping_active(ct_accel *a, int *active) {
 
 int ct_accel_get_tlas(ct_accel *a, int previous, ct_tlas *o) {
   if (!a || !o) return fail(CT_E_INVALID, "get_tlas: null argument");
-  (void)previous;  // previous == current outside the SETTLING phase
+  if (previous && ct::previous_distinct(*a)) {  // get_previous_tlas, hpp:219-223
+    o->nodes = a->prev.tlas_nodes.p;
+    o->leaves = a->prev.tlas_inst.p;
+    o->root_ref = a->prev.tlas_root;
+    o->instance_count = a->prev.tlas_count;
+    o->depth = a->prev.tlas_depth;
+    o->generation = a->prev.generation;
+    return CT_OK;
+  }
   o->nodes = a->set.tlas_nodes.p;
   o->leaves = a->set.tlas_inst.p;
   o->root_ref = a->set.tlas_root;
@@ -1693,15 +1754,15 @@ int ct_accel_get_tlas(ct_accel *a, int previous, ct_tlas *o) {
 
 int ct_accel_get_world(ct_accel *a, int previous, ct_world *o) {
   if (!a || !o) return fail(CT_E_INVALID, "get_world: null argument");
-  (void)previous;
-  o->instances = a->set.inst_aos.p;
-  o->primitives = a->set.prims.p;
-  o->aabbs = a->set.aabbs.p;
+  const bool prev = previous && ct::previous_distinct(*a);  // get_previous_*_buffer, hpp:225-250
+  o->instances = prev ? a->prev.inst_aos.p : a->set.inst_aos.p;
+  o->primitives = prev ? a->prev.prims.p : a->set.prims.p;
+  o->aabbs = prev ? a->prev.aabbs.p : a->set.aabbs.p;
   o->cube_lights = a->set.cube_lights.p;
   o->remapped_primitives = a->set.remap_prim.p;
   o->remapped_cube_lights = a->set.remap_light.p;
-  o->instance_count = (uint32_t)a->instances.size();
-  o->primitive_capacity = a->set.prim_capacity;
+  o->instance_count = prev ? a->prev.n_inst : (uint32_t)a->instances.size();
+  o->primitive_capacity = prev ? a->prev.prim_capacity : a->set.prim_capacity;
   o->cube_light_count = a->set.cube_light_count;
   o->cube_light_capacity = a->set.remap_light_cap;
   return CT_OK;

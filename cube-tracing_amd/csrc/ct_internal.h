@@ -131,6 +131,18 @@ struct BvhSet {
   uint32_t cube_light_hw = 0;   // entries held: deleted lights leave stale records past the count
 };
 
+// The scene as the frames before an update saw it: the traversable state
+// (BLAS nodes and leaves, instance records, TLAS) and the WORLD instance,
+// primitive and AABB buffers, copied on the device when an update starts.
+// While the accel is SETTLING these are get_previous_tlas and the
+// get_previous_*_buffer set (accel_struct_mngr.hpp:216-250); the cube
+// lights and the remapping tables have one copy, as in the reference.
+struct PrevSet {
+  DevBuf nodes, leaves, instances_d, tlas_nodes, tlas_inst, inst_aos, aabbs, prims;
+  uint32_t tlas_root = kLeaf, tlas_count = 0, tlas_depth = 0, generation = 0, n_inst = 0, prim_capacity = 0;
+  bool valid = false;
+};
+
 struct FreeList {  // first-fit over [0, capacity) (containers/free_list.hpp:103-278)
   struct Range { uint32_t off, size; };
   std::vector<Range> free_ranges;
@@ -185,6 +197,7 @@ struct Accel {
   FreeList prim_alloc;
   uint32_t prim_high_water = 0;
   BvhSet set;
+  PrevSet prev;
   // voxel-deletion marks (squared_brush.slang): instance bits, slot bits, counters
   DevBuf brush_imask, brush_pmask, brush_counters;
   uint32_t brush_imask_words = 0, brush_pmask_words = 0;
@@ -265,6 +278,9 @@ struct Ctx {
 
 // Kernel-side views used by trace.hip and frame.hip.
 DScene scene_view(const Accel &a);
+// The previous set's traversal view while SETTLING (else the current one).
+DScene previous_scene_view(const Accel &a);
+bool previous_distinct(const Accel &a);
 
 // Launch the any-hit traversal over a ray queue (frame.hip / trace.hip).
 struct TraceAnyArgs {

@@ -60,6 +60,8 @@ struct DFrame {
   ct_restir_params rp;
   ct_light_config cfg;
   DScene S;
+  DScene Sp;          // the scene before the last update (get_previous_tlas) when prev_set, else S
+  uint32_t prev_set;  // CT_REMAP_BIT with a distinct previous set: temporal slot-1 queries trace Sp
   const ct_aabb *aabbs;
   const ct_primitive *prims;
   const float4 *mats;   // [2 per material]: (diffuse, type), (emission, roughness)
@@ -1446,7 +1448,12 @@ __global__ void __launch_bounds__(kBlock) k_trace_any(DScene S, TraceAnyArgs A) 
 
 // The frame path's any-hit traversal: the queue holds tagged slots of the
 // frames' ray buffers.
-template <bool kCount, bool kMulti>
+// Queries of the launch after k_ris: kSel 0 traces every ray in the frame's
+// scene; in a frame that traces the previous set (prev_set), kSel 1 skips
+// its slot-1 rays (temporal reuse's target at the previous pixel) and kSel
+// 2 traces only those, in the previous set.
+constexpr int kSelAll = 0, kSelCurrent = 1, kSelPrevious = 2;
+template <bool kCount, bool kMulti, int kSel = kSelAll>
 __global__ void __launch_bounds__(kBlock) k_trace_any_frames(FrameSet FS, uint32_t max_rays,
                                                              unsigned long long *stats) {
   __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
@@ -1463,14 +1470,17 @@ __global__ void __launch_bounds__(kBlock) k_trace_any_frames(FrameSet FS, uint32
       const uint32_t e = F0.queue[q < nfar ? q : F0.queue_cap - 1u - (q - nfar)];
       const DFrame &F = FS.fb[kMulti ? (e >> kViewShift) : 0u];
       const uint32_t idx = kMulti ? (e & kSlotMask) : e;
-      const float4 o = F.ray_o[idx], d = F.ray_d[idx];
-      const uint32_t tp = F.ray_t[idx];
-      float bt = 0.0f;
-      uint32_t bi = 0, bp = 0;
-      bool ovf = false;
-      const bool occ = traverse<true, kCount>(F.S, xyz(o), xyz(d), o.w, __float_as_uint(d.w), tp, bt, bi, bp,
-                                              stk + tid, kBlock, cnt, ovf);
-      if (occ) atomicOr(&F.mask[idx / kRaySlots], 1u << (idx % kRaySlots));
+      const bool mine = kSel == kSelAll || ((F.prev_set != 0u && idx % kRaySlots == 1u) == (kSel == kSelPrevious));
+      if (mine) {
+        const float4 o = F.ray_o[idx], d = F.ray_d[idx];
+        const uint32_t tp = F.ray_t[idx];
+        float bt = 0.0f;
+        uint32_t bi = 0, bp = 0;
+        bool ovf = false;
+        const bool occ = traverse<true, kCount>(kSel == kSelPrevious ? F.Sp : F.S, xyz(o), xyz(d), o.w,
+                                                __float_as_uint(d.w), tp, bt, bi, bp, stk + tid, kBlock, cnt, ovf);
+        if (occ) atomicOr(&F.mask[idx / kRaySlots], 1u << (idx % kRaySlots));
+      }
     }
   }
   if (kCount) wave_add_cnt(cnt, stats);
@@ -1531,9 +1541,9 @@ uint32_t resident_grid(K kernel) {
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu < 1) per_cu = 1;
   return (uint32_t)(per_cu * cus);
 }
-template <bool kCount, bool kMulti>
+template <bool kCount, bool kMulti, int kSel = kSelAll>
 uint32_t any_resident() {
-  static const uint32_t n = resident_grid(k_trace_any_frames<kCount, kMulti>);
+  static const uint32_t n = resident_grid(k_trace_any_frames<kCount, kMulti, kSel>);
   return n;
 }
 template <bool kCount>
@@ -1704,6 +1714,8 @@ int prepare_frame(Ctx *c, Accel *a, const ct_frame_params *p, hipStream_t st, DF
   F.cfg = c->cfg;
   F.cfg.cube_light_count = ncube;  // the manager owns this count (accel_struct_mngr.cpp:2580)
   F.S = scene_view(*a);
+  F.prev_set = ((p->is_active & CT_REMAP_BIT) && previous_distinct(*a)) ? 1u : 0u;
+  F.Sp = F.prev_set ? previous_scene_view(*a) : F.S;
   F.aabbs = a->set.aabbs.as<ct_aabb>();
   F.prims = a->set.prims.as<ct_primitive>();
   F.mats = c->materials.as<float4>();
@@ -1779,23 +1791,28 @@ struct Launch {
   uint32_t max_neighbors = 0;  // most spatial neighbours of the set
   uint32_t trace_blocks = 0;   // grid of the persistent traversal kernels
   bool multi = false;          // queue entries carry the frame index
+  bool prev_trace = false;     // some frame traces the previous set (DFrame::prev_set)
   uint32_t *counters = nullptr;
   unsigned long long *stats = nullptr;
 };
 
-template <bool kCount, bool kMulti>
+template <bool kCount, bool kMulti, int kSel>
 void launch_any_frames(const Launch &L, FrameSet fs, uint32_t max_rays, hipStream_t st) {
   const uint32_t grid = std::min<uint32_t>(std::min<uint32_t>(std::max<uint32_t>(nblocks(max_rays), 1), L.trace_blocks),
-                                           any_resident<kCount, kMulti>());
-  hipLaunchKernelGGL((k_trace_any_frames<kCount, kMulti>), dim3(grid), dim3(kBlock), 0, st, fs, max_rays, L.stats);
+                                           any_resident<kCount, kMulti, kSel>());
+  hipLaunchKernelGGL((k_trace_any_frames<kCount, kMulti, kSel>), dim3(grid), dim3(kBlock), 0, st, fs, max_rays,
+                     L.stats);
 }
 // The queue length and the fetch head (kCntHead) are zeroed by the caller
 // before the pass that fills the queue.
+template <int kSel = kSelAll>
 void launch_any(const Launch &L, FrameSet fs, uint32_t max_rays, hipStream_t st) {
   if (L.stats)
-    L.multi ? launch_any_frames<true, true>(L, fs, max_rays, st) : launch_any_frames<true, false>(L, fs, max_rays, st);
+    L.multi ? launch_any_frames<true, true, kSel>(L, fs, max_rays, st)
+            : launch_any_frames<true, false, kSel>(L, fs, max_rays, st);
   else
-    L.multi ? launch_any_frames<false, true>(L, fs, max_rays, st) : launch_any_frames<false, false>(L, fs, max_rays, st);
+    L.multi ? launch_any_frames<false, true, kSel>(L, fs, max_rays, st)
+            : launch_any_frames<false, false, kSel>(L, fs, max_rays, st);
 }
 
 // The command list of App::draw (main.cpp:1436-1703) for every frame of the
@@ -1824,7 +1841,15 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark,
   if ((rc = mark())) return rc;  // kMarkPrimary
   hipLaunchKernelGGL(k_ris, dim3(nb), dim3(kBlock), 0, st, pix);
   if ((rc = mark())) return rc;  // kMarkRis
-  launch_any(L, pix, 3 * np, st);  // RIS and temporal reuse visibility
+  if (L.prev_trace) {
+    // RIS and temporal reuse visibility; the target-at-previous queries of
+    // frames that trace the previous set in a second pass over the queue
+    launch_any<kSelCurrent>(L, pix, 3 * np, st);
+    hipLaunchKernelGGL(k_reset_mask, dim3(1), dim3(64), 0, st, L.counters, 1u << kCntHead);
+    launch_any<kSelPrevious>(L, pix, 3 * np, st);
+  } else {
+    launch_any(L, pix, 3 * np, st);  // RIS and temporal reuse visibility
+  }
   if ((rc = mark())) return rc;  // kMarkTraceRis
   hipLaunchKernelGGL(k_temporal_finish, dim3(nb), dim3(kBlock), 0, st, pix);
   if ((rc = mark())) return rc;  // kMarkTemporalFinish
@@ -2272,6 +2297,7 @@ int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, v
   for (uint32_t g = 0; g < ng; ++g) {
     BatchGroup &G = b->groups[g];
     uint32_t pt_depth = 0, max_nb = 0, px = 0;
+    bool prev_trace = false;
     for (uint32_t i = G.first; i < G.first + G.count; ++i) {
       DFrame &F = rec[i];
       if ((rc = prepare_frame(b->ctxs[i], b->accels[i], &params[i], st, F))) return rc;
@@ -2287,6 +2313,7 @@ int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, v
       if (F.rp.indirect) pt_depth = std::max(pt_depth, F.rp.max_depth);
       max_nb = std::max(max_nb, F.rp.max_neighbors);
       px += F.npix;
+      prev_trace = prev_trace || F.prev_set != 0u;
     }
     L[g].fb = b->frames.as<DFrame>() + G.first;
     L[g].total_px = px;
@@ -2294,6 +2321,7 @@ int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, v
     L[g].max_neighbors = max_nb;
     L[g].trace_blocks = (uint32_t)cu_count(b->device) * b->blocks_per_cu;
     L[g].multi = G.count > 1;
+    L[g].prev_trace = prev_trace;
     L[g].counters = G.counters.as<uint32_t>();
     L[g].stats = nullptr;
   }
@@ -2344,7 +2372,8 @@ int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, v
   if (use_graph) {
     std::vector<uint32_t> gkey = tkey;
     for (uint32_t g = 0; g < ng; ++g)
-      gkey.insert(gkey.end(), {L[g].total_px, L[g].pt_depth, L[g].max_neighbors, L[g].trace_blocks, (uint32_t)L[g].multi});
+      gkey.insert(gkey.end(), {L[g].total_px, L[g].pt_depth, L[g].max_neighbors, L[g].trace_blocks, (uint32_t)L[g].multi,
+                               (uint32_t)L[g].prev_trace});
     if (!b->exec || gkey != b->graph_key) {
       drop_graph(b);
       CT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -2523,6 +2552,7 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
       L[k].max_neighbors = F.rp.max_neighbors;
       L[k].trace_blocks = (uint32_t)trace_grid();
       L[k].multi = false;
+      L[k].prev_trace = F.prev_set != 0u;
       L[k].counters = FP[k].counters;
       L[k].stats = nullptr;
     }
@@ -2572,6 +2602,7 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
     L.max_neighbors = F.rp.max_neighbors;
     L.trace_blocks = (uint32_t)trace_grid();
     L.multi = false;
+    L.prev_trace = F.prev_set != 0u;
     L.counters = F.counters;
     L.stats = stats;
     const bool ev = (c->instrument & 1u) != 0;
@@ -2630,6 +2661,7 @@ int ct_trace_bounce(ct_accel *a, uint32_t n, const float *origins, const float *
   DFrame F;
   std::memset(&F, 0, sizeof(F));
   F.S = scene_view(*a);
+  F.Sp = F.S;
   F.aabbs = a->set.aabbs.as<ct_aabb>();
   F.prims = a->set.prims.as<ct_primitive>();
   F.n_mats = 0;  // materials live in a frame context; the records here carry indices only

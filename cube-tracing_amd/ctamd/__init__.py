@@ -519,9 +519,11 @@ class Accel:
         check(lib().ct_accel_update_scene(self.handle, 1 if synchronize else 0, None, C.byref(did)))
         return bool(did.value)
 
-    def tlas(self) -> Tlas:
+    def tlas(self, previous=False) -> Tlas:
+        """get_current_tlas, or get_previous_tlas with previous=True (the
+        TLAS from before the last update while SETTLING)."""
         t = Tlas()
-        check(lib().ct_accel_get_tlas(self.handle, 0, C.byref(t)))
+        check(lib().ct_accel_get_tlas(self.handle, 1 if previous else 0, C.byref(t)))
         return t
 
     def world_to_object(self, instance: int) -> np.ndarray:
@@ -546,9 +548,9 @@ class Accel:
         check(lib().ct_accel_is_remapping_active(self.handle, C.byref(a)))
         return bool(a.value)
 
-    def world(self) -> World:
+    def world(self, previous=False) -> World:
         w = World()
-        check(lib().ct_accel_get_world(self.handle, 0, C.byref(w)))
+        check(lib().ct_accel_get_world(self.handle, 1 if previous else 0, C.byref(w)))
         return w
 
     def read_world(self) -> dict:

@@ -288,7 +288,12 @@ int ct_accel_task_queue_add(ct_accel *accel, const ct_task *task);
  * transforms from before the update as the previous frame's
  * (get_geometry_previous_transform_from_instance_id, motion_vectors.glsl:72)
  * and, with CT_REMAP_BIT set, applies the remapping tables in temporal
- * reuse (restir_resampling.glsl:338-387).  The work is ordered on the
+ * reuse (restir_resampling.glsl:338-387) and traces its target-at-previous
+ * visibility against the previous set (restir_resampling.glsl:369 ->
+ * light.glsl:102).  An update that runs tasks first copies the committed
+ * scene on the device (about 116 B per primitive slot): while SETTLING,
+ * `previous` != 0 in ct_accel_get_tlas / ct_accel_get_world names that
+ * copy; otherwise previous and current are the same set.  The work is ordered on the
  * accel's stream after `stream` has drained; with synchronize != 0 the call
  * returns after the build.  *did_work (optional) receives 1 when a task or
  * a settle ran, 0 when idle. */
@@ -305,7 +310,9 @@ int ct_accel_get_status(ct_accel *accel, uint32_t *status);
 int ct_accel_is_remapping_active(ct_accel *accel, int *active);
 
 /* Top-level structure handle; get_current_tlas / get_previous_tlas
- * (accel_struct_mngr.hpp:216-223). Device pointers. */
+ * (accel_struct_mngr.hpp:216-223): previous != 0 while SETTLING gives the
+ * TLAS (and through it the BLASes) from before the last update. Device
+ * pointers. */
 typedef struct ct_tlas {
   const void *nodes;     /* BVH2 nodes, 64 B each */
   const void *leaves;    /* instance ids, Morton order */
@@ -318,7 +325,11 @@ int ct_accel_get_tlas(ct_accel *accel, int previous, ct_tlas *out);
 
 /* Device buffers of the WORLD table (shared.inl:272-286) for the current
  * or previous buffer set; get_{current,previous}_*_buffer,
- * accel_struct_mngr.hpp:225-262. */
+ * accel_struct_mngr.hpp:225-262.  The previous set differs while SETTLING:
+ * its instances, primitives and AABBs are those from before the last update
+ * (a caller's compaction ahead of a GPU deletion task has already written
+ * the AABB and primitive slots by then); cube lights and the remapping
+ * tables have one copy. */
 typedef struct ct_world {
   const void *instances;   /* ct_instance AoS */
   void *primitives;        /* ct_primitive AoS by slot */

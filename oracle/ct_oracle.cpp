@@ -697,6 +697,11 @@ static inline ct_reservoir res_to(const Res &r) {
 
 struct Frame {
   Scene *sc = nullptr;
+  // The scene before the last update (its TLAS, BLASes and instance
+  // records): traced by temporal reuse's target-at-previous query while
+  // remapping is active, get_previous_tlas (restir_resampling.glsl:369 ->
+  // light.glsl:102, accel_struct_mngr.hpp:219-223).  Null: the current one.
+  const Scene *prev_sc = nullptr;
   uint32_t W = 0, H = 0;
   std::vector<ct_di_info> di_cur, di_prev;
   std::vector<ct_reservoir> res_int, res_prev;
@@ -712,6 +717,7 @@ struct Frame {
 
 struct Ctx { // per-frame constants read by every pixel
   const Scene *sc;
+  const Scene *vis_sc = nullptr;  // scene the shadow rays traverse (null: sc)
   Frame *fr;
   const ct_frame_params *p;
   std::vector<float> pt; // per instance: prev_view_proj * geometry_T
@@ -887,7 +893,7 @@ static bool sample_lights(const Ctx &c, const Hit &hit, const ct_light &l, float
   if (visibility && vis) {
     uint32_t ti = check_instance ? l.instance_info.instance_id : kInvalid;
     uint32_t tp = check_instance ? l.instance_info.primitive_id : kInvalid;
-    vis = vis && !trace_any(*c.sc, P, l_v, distance, ti, tp, cnt);
+    vis = vis && !trace_any(c.vis_sc ? *c.vis_sc : *c.sc, P, l_v, distance, ti, tp, cnt);
   }
   out.l_pos = l_pos;
   out.l_nor = l_nor;
@@ -1056,7 +1062,9 @@ static void temporal_reuse(const Ctx &c, Res &r, Res prev, uint32_t pidx, V3 ray
       Hit ph = hit_from_di(dp);
       const ct_material &pm = material(c, dp.mat_index);
       V3 pd = normalize(sub(ph.world_hit, load3(dp.ray_origin)));
-      target_lum_at_prev = luminance(reservoir_get_radiance(c, r, pd, ph, pm, cnt));
+      Ctx cp = c;  // the previous frame's scene while remapping (is_remapping_active)
+      if (remap && c.fr->prev_sc) cp.vis_sc = c.fr->prev_sc;
+      target_lum_at_prev = luminance(reservoir_get_radiance(cp, r, pd, ph, pm, cnt));
     }
     const float p_curr = r.M * luminance(r.F);
     const float m_curr = p_curr / gmax(p_curr + prev_M * target_lum_at_prev, 1e-6f);
@@ -2799,6 +2807,12 @@ int oc_frame_read(oc_frame *frame, uint32_t which, void *dst, uint64_t bytes) {
   }
   if (bytes < sz) return CT_E_INVALID;
   std::memcpy(dst, src, sz);
+  return CT_OK;
+}
+
+int oc_frame_set_previous_scene(oc_frame *frame, oc_scene *scene) {
+  if (!frame) return CT_E_INVALID;
+  frame->prev_sc = scene;
   return CT_OK;
 }
 

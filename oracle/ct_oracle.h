@@ -124,6 +124,9 @@ int oc_frame_render(oc_frame *frame, const ct_frame_params *params,
 /* Same buffer ids and layouts as ct_frame_read. */
 int oc_frame_read(oc_frame *frame, uint32_t which, void *dst, uint64_t bytes);
 int oc_frame_get_stats(oc_frame *frame, ct_frame_stats *out);
+/* The scene before the last update, traced by temporal reuse's
+ * target-at-previous query when CT_REMAP_BIT is set (NULL: the current). */
+int oc_frame_set_previous_scene(oc_frame *frame, oc_scene *scene);
 
 /* Probes (tests only): single passes of the frame on given inputs. */
 int oc_probe_ris(oc_scene *scene, const ct_frame_params *p, const ct_di_info *di, float confidence, uint32_t n,

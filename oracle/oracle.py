@@ -78,6 +78,7 @@ def lib():
             "oc_frame_render": ([C.c_void_p, P(FrameParams), C.c_int, u32, u32], C.c_int),
             "oc_frame_read": ([C.c_void_p, u32, C.c_void_p, C.c_uint64], C.c_int),
             "oc_frame_get_stats": ([C.c_void_p, P(FrameStats)], C.c_int),
+            "oc_frame_set_previous_scene": ([C.c_void_p, C.c_void_p], C.c_int),
             "oc_probe_ris": ([C.c_void_p, P(FrameParams), P(DiInfo), f32, u32, C.c_void_p, C.c_void_p], C.c_int),
             "oc_probe_temporal": ([C.c_void_p, P(FrameParams), P(DiInfo), P(Reservoir), P(DiInfo), P(Reservoir), u32,
                                    P(Reservoir)], C.c_int),
@@ -391,6 +392,14 @@ class OFrame:
         self.scene = scene
         if lib().oc_frame_set_scene(self.handle, scene.handle) != 0:
             raise ValueError("oc_frame_set_scene rejected the scene")
+
+    def set_previous_scene(self, scene):
+        """The scene before the last update (an OScene, or None for the
+        current one): temporal reuse traces its target-at-previous query
+        there while CT_REMAP_BIT is set."""
+        self.prev_scene = scene
+        if lib().oc_frame_set_previous_scene(self.handle, None if scene is None else scene.handle) != 0:
+            raise ValueError("oc_frame_set_previous_scene failed")
 
     def render(self, params, threads=0, rows=None):
         r0, r1 = (0, self.h) if rows is None else rows

@@ -284,6 +284,7 @@ def test_frames_across_an_update_with_remapping(gpu):
     osc2 = _oracle(acc, sc, ctx)
     osc2.set_previous(prev_t, prev_first, W["remap_prims"], W["remap_lights"])
     ofr.set_scene(osc2)
+    ofr.set_previous_scene(osc)
     p = ctamd.frame_params(w, 2)
     p.is_active |= ctamd.CT_REMAP_BIT
     ctx.render(acc, p)
@@ -293,6 +294,7 @@ def test_frames_across_an_update_with_remapping(gpu):
     assert acc.update_scene()  # settle
     osc3 = _oracle(acc, sc, ctx)
     ofr.set_scene(osc3)
+    ofr.set_previous_scene(None)
     p = ctamd.frame_params(w, 3)
     ctx.render(acc, p)
     ofr.render(p)
@@ -339,3 +341,73 @@ def test_delete_from_gpu_keeps_the_callers_compaction(gpu):
     _trace_equal(acc, osc, sc, 31)
     acc.update_scene()  # settle clears the caller's entries too
     assert not acc.read_world()["remap_prims"].any()
+
+
+def test_temporal_visibility_traces_the_previous_set(gpu):
+    """get_previous_tlas while SETTLING: an update that deletes occluders
+    keeps the scene from before it as the previous set; frame 2 (CT_REMAP_BIT)
+    traces temporal reuse's target-at-previous queries there
+    (restir_resampling.glsl:369 -> light.glsl:102) and matches the oracle
+    given the pre-update scene, bit for bit, while an oracle that traces the
+    current scene for them gives other reservoirs.  After the settle the
+    previous set is the current one again."""
+    w = _workload(frames=4, max_depth=1)
+    w.camera = lambda f: ((0.2, 0.35, 2.2), (-0.09, -0.15, -0.98), (0.0, 1.0, 0.0))  # static: reuse everywhere
+    sc, acc, ctx = _load(w)
+    acc.update_scene()
+    osc = _oracle(acc, sc, ctx)
+    ofr, ofr_cur = O.OFrame(osc, w.width, w.height), O.OFrame(osc, w.width, w.height)
+    for f in range(2):
+        p = ctamd.frame_params(w, f)
+        ctx.render(acc, p)
+        ofr.render(p)
+        ofr_cur.render(p)
+        _frames_equal(ctx, ofr, w, f)
+    t0, w0 = acc.tlas(), acc.world()
+    assert acc.tlas(previous=True).nodes == t0.nodes  # idle: previous = current
+    # delete a sixth of the voxels the camera does not see: occluders of shadow rays
+    di = di_array(ctx.read(ctamd.CT_BUF_DI, w.width, w.height))
+    seen = {(int(a), int(b)) for a, b in di[di[:, 3].view(np.float32) > 0][:, 11:13]}
+    inst_before = acc.read_instances()
+    prev_t = np.stack([np.array(r.transform[:], np.float32) for r in inst_before])
+    prev_first = np.array([r.first_primitive_index for r in inst_before], np.uint32)
+    rng = np.random.default_rng(4)
+    n_del = 0
+    for i, r in enumerate(inst_before):
+        cand = [k for k in range(r.primitive_count) if (i, k) not in seen and rng.random() < 1 / 6]
+        for k in sorted(cand, reverse=True)[: r.primitive_count - 1]:
+            acc.delete_primitive(i, k)
+            n_del += 1
+    assert n_del > 1000
+    acc.update_scene()
+    assert acc.status() == ctamd.CT_AS_SETTLING
+    tp, tc = acc.tlas(previous=True), acc.tlas()
+    assert tp.nodes != tc.nodes and tp.generation == t0.generation and tc.generation != t0.generation
+    wp = acc.world(previous=True)
+    assert wp.instances != acc.world().instances and wp.cube_lights == acc.world().cube_lights
+    W = acc.read_world()
+    osc2 = _oracle(acc, sc, ctx)
+    osc2.set_previous(prev_t, prev_first, W["remap_prims"], W["remap_lights"])
+    ofr.set_scene(osc2)
+    ofr.set_previous_scene(osc)
+    ofr_cur.set_scene(osc2)
+    p = ctamd.frame_params(w, 2)
+    p.is_active |= ctamd.CT_REMAP_BIT
+    ctx.render(acc, p)
+    ofr.render(p)
+    ofr_cur.render(p)
+    _frames_equal(ctx, ofr, w, 2)
+    hit = di_array(ofr.read(ctamd.CT_BUF_DI))[:, 3].view(np.float32) > 0
+    a = res_array(ofr.read(ctamd.CT_BUF_INT_RESERVOIR))[hit]
+    b = res_array(ofr_cur.read(ctamd.CT_BUF_INT_RESERVOIR))[hit]
+    assert (a != b).any(axis=1).sum() > 0
+    # settle: the previous set is the current one again
+    assert acc.update_scene()
+    assert acc.tlas(previous=True).nodes == acc.tlas().nodes
+    osc3 = _oracle(acc, sc, ctx)
+    ofr.set_scene(osc3)
+    ofr.set_previous_scene(None)
+    p = ctamd.frame_params(w, 3)
+    ctx.render(acc, p)
+    ofr.render(p)
+    _frames_equal(ctx, ofr, w, 3)
