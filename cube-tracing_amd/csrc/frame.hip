@@ -1213,13 +1213,16 @@ __global__ void __launch_bounds__(kBlock) k_pt_step(FrameSet FS, uint32_t d) {
 // in two passes around one any-hit launch: kVisEmit queues the reuse's
 // visibility rays (taken as visible), kVisFinish recomputes with their
 // results and stores.  No traversal here keeps the kernels' registers low.
-template <int kVis>
+// kHybrid: built with the hybrid shift, whose retrace traverses in place
+// (the stack in LDS); the other build has no in-place traversal.
+template <int kVis, bool kHybrid>
 __global__ void __launch_bounds__(kBlock) k_pt_end(FrameSet FS) {
+  __shared__ uint32_t stk[kHybrid ? (kStackLds > 0 ? kStackLds : 1) * kBlock : 1];
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
   const uint32_t px = blk_.y + threadIdx.x;
   PtTrace tr;
-  tr.stk = nullptr;
+  tr.stk = kHybrid ? stk + threadIdx.x : nullptr;
   tr.bounce_rays = 0u;
   tr.shadow_rays = 0u;
   tr.overflow = false;
@@ -1249,7 +1252,7 @@ __global__ void __launch_bounds__(kBlock) k_pt_end(FrameSet FS) {
         const uint32_t prev = min(F.npix - 1u, pcx + pcy * F.W);
         const PRes temporal = ld_pres(F.pres_t, prev);
         const Isect pi = isect_from_di_raw(F, ld_di(F.dpos_p, F.dnrm_p, F.dorg_p, F.dhit_p, prev));
-        out = pt_temporal<false, kVis>(F, out, temporal, ci, pi, seed, tr);
+        out = pt_temporal<false, kVis, kHybrid>(F, out, temporal, ci, pi, seed, tr);
         ind = muls(out.F, out.weight);
       }
       if (kVis == kVisFinish) {
@@ -1267,7 +1270,7 @@ __global__ void __launch_bounds__(kBlock) k_pt_end(FrameSet FS) {
 // Spatial path reuse (indirect_illumination_spatial_reuse, rgen.glsl:427-452)
 // and the rest of the shading pass: indirect term, emission, NaN scrub,
 // image store (rgen.glsl:449-489).
-template <bool kCount>
+template <bool kCount, bool kHybrid>
 __global__ void __launch_bounds__(kBlock) k_pt_spatial(FrameSet FS, unsigned long long *stats) {
   __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
   const uint2 blk_ = FS.blk[blockIdx.x];
@@ -1286,7 +1289,7 @@ __global__ void __launch_bounds__(kBlock) k_pt_spatial(FrameSet FS, unsigned lon
     if (F.rp.pt_spatial) {
       const Isect ci = isect_from_di(F, di);
       const F3 cam = f3(F.inv_view[12], F.inv_view[13], F.inv_view[14]);
-      const PRes fin = pt_spatial<kCount>(F, x, y, ci, cam, seed, tr);
+      const PRes fin = pt_spatial<kCount, kHybrid>(F, x, y, ci, cam, seed, tr);
       st_pres(F.pres_t, px, fin);
       ind = muls(fin.F, fin.weight);
     } else {
@@ -1792,6 +1795,7 @@ struct Launch {
   uint32_t trace_blocks = 0;   // grid of the persistent traversal kernels
   bool multi = false;          // queue entries carry the frame index
   bool prev_trace = false;     // some frame traces the previous set (DFrame::prev_set)
+  bool hybrid = false;         // some frame's path reuse uses the hybrid shift
   uint32_t *counters = nullptr;
   unsigned long long *stats = nullptr;
 };
@@ -1883,12 +1887,16 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark,
       else
         hipLaunchKernelGGL(k_pt_step<false>, dim3(nblocks(np)), dim3(kBlock), 0, st, pix, d);
     }
-    hipLaunchKernelGGL(k_pt_end<kVisEmit>, dim3(nb), dim3(kBlock), 0, st, pix);
+    if (L.hybrid)
+      hipLaunchKernelGGL((k_pt_end<kVisEmit, true>), dim3(nb), dim3(kBlock), 0, st, pix);
+    else
+      hipLaunchKernelGGL((k_pt_end<kVisEmit, false>), dim3(nb), dim3(kBlock), 0, st, pix);
   }
   if ((rc = mark())) return rc;  // kMarkPtEmit
   if (L.pt_depth) launch_any(L, pix, 2 * np, st);
   if ((rc = mark())) return rc;  // kMarkTracePt
-  if (L.pt_depth) hipLaunchKernelGGL(k_pt_end<kVisFinish>, dim3(nb), dim3(kBlock), 0, st, pix);
+  if (L.pt_depth && L.hybrid) hipLaunchKernelGGL((k_pt_end<kVisFinish, true>), dim3(nb), dim3(kBlock), 0, st, pix);
+  if (L.pt_depth && !L.hybrid) hipLaunchKernelGGL((k_pt_end<kVisFinish, false>), dim3(nb), dim3(kBlock), 0, st, pix);
   if ((rc = mark())) return rc;  // kMarkPt
   }
   if (phases & kPhaseSpatial) {
@@ -1901,10 +1909,14 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark,
   hipLaunchKernelGGL(k_spatial<false>, dim3(nb), dim3(kBlock), 0, st, pix);
   if ((rc = mark())) return rc;  // kMarkSpatialFinish
   if (L.pt_depth) {
-    if (L.stats)
-      hipLaunchKernelGGL(k_pt_spatial<true>, dim3(nb), dim3(kBlock), 0, st, pix, L.stats);
+    if (L.stats && L.hybrid)
+      hipLaunchKernelGGL((k_pt_spatial<true, true>), dim3(nb), dim3(kBlock), 0, st, pix, L.stats);
+    else if (L.stats)
+      hipLaunchKernelGGL((k_pt_spatial<true, false>), dim3(nb), dim3(kBlock), 0, st, pix, L.stats);
+    else if (L.hybrid)
+      hipLaunchKernelGGL((k_pt_spatial<false, true>), dim3(nb), dim3(kBlock), 0, st, pix, L.stats);
     else
-      hipLaunchKernelGGL(k_pt_spatial<false>, dim3(nb), dim3(kBlock), 0, st, pix, L.stats);
+      hipLaunchKernelGGL((k_pt_spatial<false, false>), dim3(nb), dim3(kBlock), 0, st, pix, L.stats);
   }
   if ((rc = mark())) return rc;  // kMarkPtSpatial
   }
@@ -2297,7 +2309,7 @@ int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, v
   for (uint32_t g = 0; g < ng; ++g) {
     BatchGroup &G = b->groups[g];
     uint32_t pt_depth = 0, max_nb = 0, px = 0;
-    bool prev_trace = false;
+    bool prev_trace = false, hybrid = false;
     for (uint32_t i = G.first; i < G.first + G.count; ++i) {
       DFrame &F = rec[i];
       if ((rc = prepare_frame(b->ctxs[i], b->accels[i], &params[i], st, F))) return rc;
@@ -2314,6 +2326,7 @@ int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, v
       max_nb = std::max(max_nb, F.rp.max_neighbors);
       px += F.npix;
       prev_trace = prev_trace || F.prev_set != 0u;
+      hybrid = hybrid || (F.rp.indirect && F.rp.pt_shift_mapping == CT_SHIFT_HYBRID);
     }
     L[g].fb = b->frames.as<DFrame>() + G.first;
     L[g].total_px = px;
@@ -2322,6 +2335,7 @@ int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, v
     L[g].trace_blocks = (uint32_t)cu_count(b->device) * b->blocks_per_cu;
     L[g].multi = G.count > 1;
     L[g].prev_trace = prev_trace;
+    L[g].hybrid = hybrid;
     L[g].counters = G.counters.as<uint32_t>();
     L[g].stats = nullptr;
   }
@@ -2373,7 +2387,7 @@ int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, v
     std::vector<uint32_t> gkey = tkey;
     for (uint32_t g = 0; g < ng; ++g)
       gkey.insert(gkey.end(), {L[g].total_px, L[g].pt_depth, L[g].max_neighbors, L[g].trace_blocks, (uint32_t)L[g].multi,
-                               (uint32_t)L[g].prev_trace});
+                               (uint32_t)L[g].prev_trace, (uint32_t)L[g].hybrid});
     if (!b->exec || gkey != b->graph_key) {
       drop_graph(b);
       CT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -2553,6 +2567,7 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
       L[k].trace_blocks = (uint32_t)trace_grid();
       L[k].multi = false;
       L[k].prev_trace = F.prev_set != 0u;
+      L[k].hybrid = F.rp.indirect && F.rp.pt_shift_mapping == CT_SHIFT_HYBRID;
       L[k].counters = FP[k].counters;
       L[k].stats = nullptr;
     }
@@ -2603,6 +2618,7 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
     L.trace_blocks = (uint32_t)trace_grid();
     L.multi = false;
     L.prev_trace = F.prev_set != 0u;
+    L.hybrid = F.rp.indirect && F.rp.pt_shift_mapping == CT_SHIFT_HYBRID;
     L.counters = F.counters;
     L.stats = stats;
     const bool ev = (c->instrument & 1u) != 0;

@@ -62,6 +62,12 @@ struct PState {  // PATH_STATE, path_state.glsl:23-78 (fields the Lambertian pat
   PBuilder pb;
   PRes res;
   bool last_rough;
+  // Replay of a source path for the hybrid shift (is_replay_for_hybrid_shift):
+  // stops at the vertex before the source's reconnection vertex replay_rc.
+  bool replay, replay_done;
+  uint32_t replay_rc;
+  Isect replay_prev;
+  F3 replay_tp;
 };
 
 // Traversal context of one lane.  The visibility tests of a pass that
@@ -366,11 +372,11 @@ CT_D bool pt_scatter(PState &p, Isect &i) {
 }
 
 // path_builder_add_escape_vertex, path_builder.glsl:36-75
-CT_D void add_escape_vertex(PBuilder &pb, uint32_t plen, F3 wi, F3 path_weight, F3 postfix, float rr_pdf, float mis,
+CT_D bool add_escape_vertex(PBuilder &pb, uint32_t plen, F3 wi, F3 path_weight, F3 postfix, float rr_pdf, float mis,
                             float light_pdf, uint32_t light_type, PRes &res) {
-  if (plen < 1u) return;
+  if (plen < 1u) return false;
   const bool is_rc = plen == pb.rc_len;
-  if (!pres_update(res, path_weight, rr_pdf, pb.seed)) return;
+  if (!pres_update(res, path_weight, rr_pdf, pb.seed)) return false;
   res.flags = pb.flags;
   pf_set_path_length(res.flags, plen);
   pf_set_rc_length(res.flags, pb.rc_len);
@@ -387,7 +393,25 @@ CT_D void add_escape_vertex(PBuilder &pb, uint32_t plen, F3 wi, F3 path_weight, 
   }
   res.light_pdf = light_pdf;
   pf_set_light_type(res.flags, light_type);
+  return true;
 }
+
+// path_builder_mark_escape_vertex_as_rc_vertex, path_builder.glsl:114-139:
+// the selected escape vertex becomes the reconnection vertex of the
+// reservoir's sample (Lambertian: no delta, transmission or specular bits).
+CT_D void mark_escape_rc(PRes &res, uint32_t length, const ObjHit &hit, float light_pdf, uint32_t light_type,
+                         F3 irradiance, F3 wi, float prev_scatter_pdf, float geometry_factor) {
+  pf_set_rc_length(res.flags, length);
+  res.rc = hit;
+  res.flags &= ~((1u << 20) | (1u << 22));
+  res.light_pdf = light_pdf;
+  res.cj.x = prev_scatter_pdf;
+  pf_set_light_type(res.flags, light_type);
+  res.rc_irr = irradiance;
+  res.rc_wi = wi;
+  res.cj.z = geometry_factor;
+}
+CT_D bool hybrid_mode(const DFrame &F) { return F.rp.pt_shift_mapping == CT_SHIFT_HYBRID; }
 
 // eval_voxel_pdf, path_tracer.glsl:185-211
 CT_D float eval_voxel_pdf(F3 pos, const Isect &i) {
@@ -410,10 +434,23 @@ CT_D void pt_emissive(const DFrame &F, PState &p, const Isect &i) {
   const F3 cur = mulv(p.thp, p.prefix_thp);
   const F3 Lr = muls(mulv(cur, i.mat.emission), mis);
   const F3 att = mulv(cur, i.mat.emission);
+  if (p.replay) return;  // a replay only carries throughput to its stop vertex
   if (p.path_length > 1u) p.L = add(p.L, Lr);
   const F3 postfix = muls(mulv(p.thp, att), mis);
   const uint32_t plen = p.path_length >= 1u ? p.path_length - 1u : 0u;
-  add_escape_vertex(p.pb, plen, p.dir, Lr, postfix, p.rr_pdf, mis, light_pdf, kGeomLightCube, p.res);
+  const bool selected = add_escape_vertex(p.pb, plen, p.dir, Lr, postfix, p.rr_pdf, mis, light_pdf, kGeomLightCube, p.res);
+  // an escaped vertex at length >= 2 can be the reconnection vertex
+  // (path_tracer.glsl:303-335)
+  if (selected && hybrid_mode(F) && p.path_length >= 2u && p.path_length < p.pb.rc_len) {
+    const bool far = length(sub(i.P, p.origin)) >= F.rp.near_field_distance;
+    if (far && p.last_rough) {
+      const F3 disp = sub(p.origin, i.P);
+      float g = fabsf(dot(i.n, i.wo)) / dot(disp, disp);
+      if (isnan(g)) g = 0.0f;
+      mark_escape_rc(p.res, p.path_length, p.hit, light_pdf, kGeomLightCube, i.mat.emission, f3(0, 0, 0),
+                     p.prev_scatter_pdf, g);
+    }
+  }
 }
 
 // path_generate_light_sample (path_tracer.glsl:222-256) as far as it is
@@ -440,8 +477,10 @@ CT_D void pt_light_sample(const DFrame &F, const Isect &i, uint32_t &seed) {
   sample_lights(F, h, l, pdf, seed, true, false, ls);
 }
 
-// path_handle_hit, path_tracer.glsl:599-727 (no replay, no hybrid shift:
-// generate_path fixes use_hybrid_shift = false, path_state.glsl:245-247).
+// path_handle_hit, path_tracer.glsl:599-727.  The path builder follows
+// the frame's shift mapping (use_hybrid_shift, which generate_path leaves
+// false, path_state.glsl:245-247, is taken from it); a replay stops at the
+// vertex before the source's reconnection vertex.
 CT_D void pt_handle_hit(const DFrame &F, PState &p, Isect &i) {
   const ct_restir_params &rp = F.rp;
   const bool compute_emissive =
@@ -456,23 +495,49 @@ CT_D void pt_handle_hit(const DFrame &F, PState &p, Isect &i) {
   const F3 prev_origin = p.origin;
   p.origin = compute_ray_origin(compute_ray_origin(i.P, i.n), i.n);
   const bool cur_rough = i.mat.roughness > rp.roughness_threshold;  // classify_as_rough
-  // path_check_rc_vertex, path_tracer.glsl:484-597 (not a replay): the
-  // reconnection vertex is the first secondary hit.
-  if (p.path_length == 1u) {
-    p.pb.rc = p.hit;
-    p.pb.cj.x = p.prev_scatter_pdf;
-    const F3 disp = sub(prev_origin, i.P);
-    p.pb.cj.z = fabsf(dot(i.n, i.wo)) / dot(disp, disp);
-    if (isnan(p.pb.cj.z)) p.pb.cj.z = 0.0f;
-    p.pb.rc_len = p.path_length;
+  // path_check_rc_vertex, path_tracer.glsl:474-597
+  const bool far = length(sub(p.origin, prev_origin)) >= rp.near_field_distance;
+  const bool connectable = far && cur_rough && p.last_rough;
+  if (!p.replay) {
+    const bool hybrid = hybrid_mode(F);
+    const bool can_connect = p.path_length >= 1u && p.path_length < p.pb.rc_len && connectable;
+    if ((!hybrid && p.path_length == 1u) || (hybrid && can_connect)) {
+      p.pb.rc = p.hit;
+      p.pb.cj.x = p.prev_scatter_pdf;
+      const F3 disp = sub(prev_origin, i.P);
+      p.pb.cj.z = fabsf(dot(i.n, i.wo)) / dot(disp, disp);
+      if (isnan(p.pb.cj.z)) p.pb.cj.z = 0.0f;
+      p.pb.rc_len = p.path_length;
+    }
+  } else {
+    // The source path's reconnection vertex is its first connectable one, so
+    // no replayed vertex before it may be connectable, and the one right
+    // before it must be rough to connect from (invertibility, GRIS).
+    bool invertible = !(p.path_length >= 1u && p.path_length + 1u <= p.replay_rc && connectable);
+    if (invertible && p.path_length + 1u == p.replay_rc) {
+      if (!cur_rough) {
+        invertible = false;
+      } else {
+        p.replay_done = true;
+        p.replay_prev = i;
+        p.replay_tp = mulv(p.thp, p.prefix_thp);
+      }
+    }
+    if (!invertible || p.replay_done) {
+      p.flags &= ~kPathActive;
+      return;
+    }
   }
   p.flags &= ~(kPathLightUpper | kPathLightLower);
   // path_handle_sample_light, path_tracer.glsl:339-438
   if (p.path_length == p.pb.rc_len) p.pb.cached_seed = p.seed;
   pt_light_sample(F, i, p.seed);
   p.flags = (p.flags | kPathLightUpper) & ~kPathLightLower;
-  // Russian roulette, terminate_path_by_russian_roulette (path_tracer.glsl:165-177)
-  if (rp.pt_russian_roulette && p.path_length >= 1u) {
+  // Russian roulette, terminate_path_by_russian_roulette (path_tracer.glsl:165-177);
+  // a replay draws its number and goes on (:696-699)
+  if (rp.pt_russian_roulette && p.path_length >= 1u && p.replay) {
+    (void)rnd(p.seed);
+  } else if (rp.pt_russian_roulette && p.path_length >= 1u) {
     const float rr = luminance(mulv(p.thp, p.prefix_thp));
     const float prob = gmax(0.0f, 1.0f - rr);
     if (rnd(p.seed) < prob) {
@@ -492,13 +557,21 @@ CT_D void pt_handle_hit(const DFrame &F, PState &p, Isect &i) {
 
 // path_handle_miss, path_tracer.glsl:729-815 (environment light on)
 CT_D void pt_handle_miss(const DFrame &F, PState &p) {
+  if (p.replay) {  // a replay that escapes before its stop vertex is not invertible
+    p.flags &= ~kPathActive;
+    return;
+  }
   const float light_pdf = kInv4Pi;
   const float mis = eval_mis(p.pdf, light_pdf);
   const F3 Le = sky_color(F.time, F.afternoon != 0, p.dir);
   const F3 Lr = muls(mulv(mulv(p.thp, p.prefix_thp), Le), mis);
   if (p.path_length > 0u) p.L = add(p.L, Lr);
-  add_escape_vertex(p.pb, p.path_length, p.dir, Lr, muls(mulv(p.thp, Le), mis), p.rr_pdf, mis, light_pdf,
-                    kGeomLightEnv, p.res);
+  const bool selected = add_escape_vertex(p.pb, p.path_length, p.dir, Lr, muls(mulv(p.thp, Le), mis), p.rr_pdf, mis,
+                                          light_pdf, kGeomLightEnv, p.res);
+  // escaped to the sky: a reconnection vertex at infinity (path_tracer.glsl:805-833)
+  if (selected && hybrid_mode(F) && p.path_length >= 1u && p.path_length + 1u < p.pb.rc_len && p.last_rough)
+    mark_escape_rc(p.res, p.path_length + 1u, objhit_invalid(), light_pdf, kGeomLightEnv, Le, p.dir,
+                   p.prev_scatter_pdf, 1.0f);
   p.flags &= ~kPathActive;
 }
 
@@ -537,6 +610,9 @@ CT_D bool pt_begin(const DFrame &F, PState &p, Isect i, uint32_t seed) {
   p.pb.cj = f3(0, 0, 0);
   p.res = pres_init(F.rp.max_depth);
   p.last_rough = true;
+  p.replay = false;
+  p.replay_done = false;
+  p.replay_rc = 0u;
   p.origin = compute_ray_origin(compute_ray_origin(i.P, i.n), i.n);
   p.flags &= ~(kPathLightUpper | kPathLightLower);
   p.res.init_seed = p.seed;
@@ -708,16 +784,24 @@ CT_D void ld_pstate(const float4 *S, uint32_t np, uint32_t p, PState &s) {
   s.normal = f3(0, 0, 0);
   s.L = f3(0, 0, 0);
   s.hit = objhit_invalid();
+  s.replay = false;
+  s.replay_done = false;
+  s.replay_rc = 0u;
 }
 
-// compute_shifted_integrand_reconnection, shift.glsl:25-350, for the
-// reconnection shift (rc vertex 1, no cached Jacobian).  `dst` and `src`
-// are the primary intersections; `s` the source reservoir.
+// compute_shifted_integrand_reconnection, shift.glsl:25-350.  `dst` and
+// `src` are the vertices the connection leaves from (the primary ones for
+// the reconnection shift); `s` the source reservoir.  With use_hybrid the
+// reconnection vertex index comes from the path flags and a near-field
+// connection is rejected; with use_cached the source side's geometry term
+// and pdfs come from the reservoir's cached Jacobian, and a successful
+// shift leaves the destination's there.
 template <bool kCount, int kVis = kVisInline>
-CT_D F3 shift_reconnection(const DFrame &F, float &dst_jac, Isect dst, Isect src, const PRes &s, bool eval_vis,
-                           PtTrace &tr) {
+CT_D F3 shift_reconnection(const DFrame &F, float &dst_jac, Isect dst, Isect src, PRes &s, bool eval_vis,
+                           PtTrace &tr, bool use_hybrid = false, bool use_cached = false) {
   dst_jac = 0.0f;
-  const uint32_t rc_len = 1u;
+  F3 dst_cj = f3(0, 0, 0);
+  const uint32_t rc_len = use_hybrid ? pf_rc_length(s.flags) : 1u;
   const F3 irr = s.rc_irr, wi = s.rc_wi;
   const bool exists = instance_hit_exists(s.rc.inst, s.rc.prim);
   const uint32_t plen = pf_path_length(s.flags);
@@ -729,14 +813,16 @@ CT_D F3 shift_reconnection(const DFrame &F, float &dst_jac, Isect dst, Isect src
     F3 integrand = f3(0, 0, 0);
     if (pf_light_type(s.flags) == kGeomLightEnv && plen + 1u == rc_len && !nee) {
       if (vertex_visible<kCount, kVis>(F, dst.P, wi, kMaxDistance, kMaxInstances, kMaxPrimitives, tr)) {
-        const float src_pdf1 = material_pdf(src.n, src.wo, wi);
+        const float src_pdf1 = use_cached ? s.cj.x : material_pdf(src.n, src.wo, wi);
         const float dst_pdf1 = material_pdf(dst.n, dst.wo, wi);
+        dst_cj.x = dst_pdf1;
         const F3 dst_f1 = evaluate_material(dst.mat, dst.n, dst.wo, wi);
         const float mis = eval_mis(dst_pdf1, s.light_pdf);
         integrand = mulv(muls(divs(dst_f1, dst_pdf1), mis), irr);
         dst_jac = dst_pdf1 / src_pdf1;
       }
     }
+    if (use_cached) s.cj = dst_cj;
     if (jacobian_invalid(dst_jac)) dst_jac = 0.0f;
     if (has_nan_inf(integrand)) return f3(0, 0, 0);
     return integrand;
@@ -751,14 +837,22 @@ CT_D F3 shift_reconnection(const DFrame &F, float &dst_jac, Isect dst, Isect src
   const F3 sdisp = sub(rc.P, dst.P);
   const float sd2 = dot(sdisp, sdisp);
   const float scos = fabsf(dot(rc.n, neg(dst_cv)));
+  if (use_hybrid && !(sqrtf(sd2) >= F.rp.near_field_distance)) return f3(0, 0, 0);
   const float cjz = scos / sd2;
-  const F3 odisp = sub(rc.P, src.P);
-  const float od2 = dot(odisp, odisp);
-  const float ocos = fabsf(dot(rc.n, neg(src_cv)));
-  float jac = cjz * od2 / ocos;
+  dst_cj.z = cjz;
+  float jac;
+  if (use_cached) {
+    jac = cjz / s.cj.z;
+  } else {
+    const F3 odisp = sub(rc.P, src.P);
+    const float od2 = dot(odisp, odisp);
+    const float ocos = fabsf(dot(rc.n, neg(src_cv)));
+    jac = cjz * od2 / ocos;
+  }
   if (jacobian_invalid(jac)) return f3(0, 0, 0);
   const float dst_pdf1 = material_pdf(dst.n, dst.wo, dst_cv);
-  const float src_pdf1 = material_pdf(src.n, src.wo, src_cv);
+  dst_cj.x = dst_pdf1;
+  const float src_pdf1 = use_cached ? s.cj.x : material_pdf(src.n, src.wo, src_cv);
   jac *= dst_pdf1 / src_pdf1;
   if (jacobian_invalid(jac)) return f3(0, 0, 0);
   const F3 dst_f1 = evaluate_material(dst.mat, dst.n, dst.wo, dst_cv);
@@ -766,7 +860,8 @@ CT_D F3 shift_reconnection(const DFrame &F, float &dst_jac, Isect dst, Isect src
   if (!escaped) {
     dst_rc = material_pdf(rc.n, rc.wo, wi);
     dst_rc_all = dst_rc;
-    src_rc = material_pdf(rc.n, neg(src_cv), wi);
+    dst_cj.y = dst_rc;
+    src_rc = use_cached ? s.cj.y : material_pdf(rc.n, neg(src_cv), wi);
     dst_pdf2 = !rc_nee ? dst_rc : s.light_pdf;
   }
   F3 dst_f2 = f3(1.0f, 1.0f, 1.0f);
@@ -793,37 +888,115 @@ CT_D F3 shift_reconnection(const DFrame &F, float &dst_jac, Isect dst, Isect src
   }
   if (has_nan_inf(integrand)) return f3(0, 0, 0);
   dst_jac = jac;
+  if (use_cached) s.cj = dst_cj;
   return integrand;
 }
 
-// compute_shifted_integrand_ (shift.glsl:446-483), reconnection or hybrid.
-// The hybrid shift reads RECONNECTION_DATA, whose storage is stubbed in
-// the reference (path_reservoir.glsl:37-58 returns an invalid record);
-// with it the hybrid integrand keeps dst_jacobian = 0 (shift.glsl:363 and
-// the skipped reconnection branch), so every hybrid shift is rejected.
+// RECONNECTION_DATA (shared.inl:380-394) as the retrace produces it: the
+// destination path's vertex before the reconnection vertex and the
+// throughput that reaches it.
+struct RcData {
+  bool ok;
+  Isect prev;
+  F3 tp;
+};
+
+// The retrace of trace_hybrid_shift_rays / trace_random_replay_path_hybrid_simple
+// (indirect_illumination.glsl:12-28, path_tracer.glsl:889-966): replay the
+// source reservoir's random numbers from the destination's primary
+// intersection through the frame's own sampling steps (pt_begin, pt_step
+// over in-place closest hits) until the vertex before the source's
+// reconnection vertex.  Not invertible (ok = false) if it misses, stops
+// early, reaches a connectable vertex before that one or that vertex is
+// not rough.
+CT_D RcData hybrid_retrace(const DFrame &F, Isect i0, const PRes &src, PtTrace &tr) {
+  RcData out;
+  out.ok = false;
+  out.tp = f3(0, 0, 0);
+  PState p;
+  const uint32_t k = pf_rc_length(src.flags);
+  // pt_begin's primary scatter with the path builder's reconnection length
+  // at k instead of max_depth: both exceed 0, so the scatter is the same
+  bool more = pt_begin(F, p, i0, src.init_seed);
+  p.pb.rc_len = k;
+  p.replay = true;
+  p.replay_rc = k;
+  p.last_rough = i0.mat.roughness > F.rp.roughness_threshold;  // generate_replay_primary_scatter_ray
+  while (more) {
+    Isect h = bounce_intersect<false>(F, p.origin, p.dir, tr);
+    more = pt_step(F, p, h);
+  }
+  if (p.replay_done) {
+    out.ok = true;
+    out.prev = p.replay_prev;
+    out.tp = p.replay_tp;
+  }
+  return out;
+}
+
+// compute_shifted_integrand_hybrid, shift.glsl:352-444: random replay up to
+// the vertex before the reconnection vertex, then the reconnection from
+// there with the cached Jacobian (or from the destination's primary vertex
+// when the reconnection vertex is the first bounce).  Two points differ
+// from the text, which as written rejects every shift: the Jacobian starts
+// at 1 (the replayed prefix has unit Jacobian in primary-sample space) where
+// the text sets 0 and multiplies into it (:363, :437), and reconnection
+// length 1 reconnects from the destination's primary vertex where the text
+// passes an invalid hit (:376-378).  A source whose reconnection vertex
+// does not exist (the path ended first) is not shifted.
 template <bool kCount, int kVis = kVisInline>
-CT_D F3 shifted_integrand(const DFrame &F, float &dst_jac, const Isect &dst, const Isect &src, const PRes &s,
+CT_D F3 hybrid_integrand(const DFrame &F, float &dst_jac, const Isect &dst, const Isect &src, PRes &s, bool eval_vis,
+                         PtTrace &tr) {
+  dst_jac = 0.0f;
+  if (s.weight == 0.0f) return f3(0, 0, 0);
+  const uint32_t k = pf_rc_length(s.flags), plen = pf_path_length(s.flags);
+  const bool escaped = plen + 1u == k;
+  if (!(k <= plen || escaped) || k == 0u) return f3(0, 0, 0);
+  F3 tp = f3(1.0f, 1.0f, 1.0f);
+  Isect prev = dst;
+  if (k > 1u) {
+    const RcData rc = hybrid_retrace(F, dst, s, tr);
+    if (!rc.ok) return f3(0, 0, 0);
+    tp = rc.tp;
+    prev = rc.prev;
+  }
+  if (!(tp.x > 0.0f || tp.y > 0.0f || tp.z > 0.0f)) return f3(0, 0, 0);
+  float rj = 0.0f;
+  const F3 rc_tp = shift_reconnection<kCount, kVis>(F, rj, prev, src, s, eval_vis, tr, true, k > 1u);
+  dst_jac = 1.0f * rj;
+  return mulv(tp, rc_tp);
+}
+
+// compute_shifted_integrand_ (shift.glsl:446-483), reconnection or hybrid
+// (the hybrid shift only in the kernels built with kHybrid).  `s` is the
+// caller's copy of the source reservoir: the hybrid shift leaves the
+// destination's cached Jacobian in it.
+template <bool kCount, int kVis = kVisInline, bool kHybrid = false>
+CT_D F3 shifted_integrand(const DFrame &F, float &dst_jac, const Isect &dst, const Isect &src, PRes &s,
                           bool eval_vis, PtTrace &tr) {
   dst_jac = 0.0f;
   if (s.weight == 0.0f) return f3(0, 0, 0);
   if (F.rp.pt_shift_mapping == CT_SHIFT_RECONNECTION)
     return shift_reconnection<kCount, kVis>(F, dst_jac, dst, src, s, eval_vis, tr);
-  if (F.rp.pt_shift_mapping == CT_SHIFT_HYBRID) {
-    // compute_shifted_integrand_hybrid, shift.glsl:352-444, with the stubbed
-    // reconnection data: rc length 1 gives tp = 1 and an invalid previous
-    // vertex, longer ones tp = 0; the reconnection branch is never entered.
-    // (ROUGHNESS_BASED_REJECTION is false, defines.glsl:54.)
-    return pf_rc_length(s.flags) == 1u ? f3(1.0f, 1.0f, 1.0f) : f3(0, 0, 0);
-  }
+  if (kHybrid && F.rp.pt_shift_mapping == CT_SHIFT_HYBRID)
+    return hybrid_integrand<kCount, kVis>(F, dst_jac, dst, src, s, eval_vis, tr);
   return f3(1.0f, 1.0f, 1.0f);
+}
+// compute_shifted_integrand, shift.glsl:485-497: on a copy of the reservoir.
+template <bool kCount, int kVis = kVisInline, bool kHybrid = false>
+CT_D F3 shifted_integrand_copy(const DFrame &F, float &dst_jac, const Isect &dst, const Isect &src, const PRes &s,
+                               bool eval_vis, PtTrace &tr) {
+  PRes t = s;
+  return shifted_integrand<kCount, kVis, kHybrid>(F, dst_jac, dst, src, t, eval_vis, tr);
 }
 
 // shift_and_merge_reservoir, shift.glsl:485-510 (force_merge, mis 1)
-template <bool kCount, int kVis = kVisInline>
+template <bool kCount, int kVis = kVisInline, bool kHybrid = false>
 CT_D bool shift_and_merge(const DFrame &F, float &dst_jac, const Isect &dst, PRes &dst_res, const Isect &src,
                           const PRes &src_res, uint32_t &seed, PtTrace &tr) {
-  const F3 integrand = shifted_integrand<kCount, kVis>(F, dst_jac, dst, src, src_res, true, tr);
-  const bool selected = pres_merge(dst_res, integrand, dst_jac, src_res, seed, 1.0f, true);
+  PRes tmp = src_res;
+  const F3 integrand = shifted_integrand<kCount, kVis, kHybrid>(F, dst_jac, dst, src, tmp, true, tr);
+  const bool selected = pres_merge(dst_res, integrand, dst_jac, tmp, seed, 1.0f, true);
   if (!selected) dst_res.F = f3(0, 0, 0);
   dst_res.M = src_res.M;
   dst_res.weight = src_res.weight;
@@ -835,7 +1008,7 @@ CT_D bool shift_and_merge(const DFrame &F, float &dst_jac, const Isect &dst, PRe
 // then the history shifted here) exist or not by data alone, never by an
 // earlier test's outcome, so a queued pass records the same rays in the
 // same slots as the finishing pass reads.
-template <bool kCount, int kVis = kVisInline>
+template <bool kCount, int kVis = kVisInline, bool kHybrid = false>
 CT_D PRes pt_temporal(const DFrame &F, const PRes &central, PRes temporal, const Isect &cur_i, const Isect &prev_i,
                       uint32_t &seed, PtTrace &tr) {
   PRes dest = pres_init(F.rp.max_depth);
@@ -851,7 +1024,7 @@ CT_D PRes pt_temporal(const DFrame &F, const PRes &central, PRes temporal, const
       dst_jac = 1.0f;
       possible = tdst.weight > 0.0f;
     } else {
-      possible = shift_and_merge<kCount, kVis>(F, dst_jac, cur_i, tdst, prev_i, temporal, seed, tr);
+      possible = shift_and_merge<kCount, kVis, kHybrid>(F, dst_jac, cur_i, tdst, prev_i, temporal, seed, tr);
     }
     if (possible) {
       // j = -1
@@ -864,7 +1037,7 @@ CT_D PRes pt_temporal(const DFrame &F, const PRes &central, PRes temporal, const
         p_sum += p_self;
       } else {
         float tj = 0.0f;
-        const F3 ti = shifted_integrand<kCount, kVis>(F, tj, prev_i, cur_i, tdst, true, tr);
+        const F3 ti = shifted_integrand_copy<kCount, kVis, kHybrid>(F, tj, prev_i, cur_i, tdst, true, tr);
         const float pp = lum601(ti) * tj;
         p_sum += pp * temporal.M;
       }
@@ -914,7 +1087,7 @@ CT_D Isect isect_from_di(const DFrame &F, const DI &d) {
 }
 
 // indirect_illumination_spatial_reuse, indirect_illumination.glsl:312-452
-template <bool kCount>
+template <bool kCount, bool kHybrid = false>
 CT_D PRes pt_spatial(const DFrame &F, uint32_t x, uint32_t y, const Isect &ci, F3 camera_pos, uint32_t &seed,
                      PtTrace &tr) {
   const uint32_t cur = y * F.W + x;
@@ -939,7 +1112,7 @@ CT_D PRes pt_spatial(const DFrame &F, uint32_t x, uint32_t y, const Isect &ci, F
     const PRes nres = ld_pres(F.pres_o, ni);
     valid++;
     float prefix_jac = 0.0f;
-    const F3 prefix = shifted_integrand<kCount>(F, prefix_jac, nis, ci, central, true, tr);
+    const F3 prefix = shifted_integrand_copy<kCount, kVisInline, kHybrid>(F, prefix_jac, nis, ci, central, true, tr);
     const float prefix_pdf = lum601(prefix) * prefix_jac;
     canonical += 1.0f;
     if (prefix_pdf > 0.0f)
@@ -947,7 +1120,7 @@ CT_D PRes pt_spatial(const DFrame &F, uint32_t x, uint32_t y, const Isect &ci, F
                    (prefix_pdf * nres.M + central.M * lum601(central.F) / (float)ncount);
     PRes tdst = dest;
     float dst_jac = 0.0f;
-    const bool possible = shift_and_merge<kCount>(F, dst_jac, ci, tdst, nis, nres, seed, tr);
+    const bool possible = shift_and_merge<kCount, kVisInline, kHybrid>(F, dst_jac, ci, tdst, nis, nres, seed, tr);
     float nw = 0.0f;
     if (possible) {
       nw = lum601(nres.F) / dst_jac * nres.M /

@@ -1430,6 +1430,15 @@ struct PathState {  // path_state.glsl:23-78
   PathBuilder pb;
   ct_path_reservoir res{};
   bool last_rough = true;
+  // Hybrid shift: the path builder picks its reconnection vertex by the
+  // hybrid rule (use_hybrid_shift).  A replay (is_replay_for_hybrid_shift)
+  // retraces a source path's random numbers up to the vertex before the
+  // source's reconnection vertex (replay_rc = its index) and stops there.
+  bool hybrid = false, replay = false;
+  uint32_t replay_rc = 0;
+  bool replay_done = false;  // stopped at vertex replay_rc - 1, invertibly
+  Isect replay_prev;         // that vertex
+  V3 replay_tp{0, 0, 0};     // path_get_current_thp on arriving there
 };
 constexpr uint32_t kPfActive = 0x1, kPfHit = 0x2, kPfTransmission = 0x4, kPfDelta = 0x10, kPfUpper = 0x80,
                    kPfLower = 0x100, kPfDiffusePrimary = 0x200;  // defines.glsl:64-76
@@ -1471,12 +1480,12 @@ static bool path_generate_scatter_ray(PathState &p, Isect &i) {
 }
 
 // path_builder_add_escape_vertex, path_builder.glsl:36-75
-static void add_escape_vertex(PathBuilder &pb, uint32_t path_length, V3 wi, V3 path_weight, V3 postfix,
+static bool add_escape_vertex(PathBuilder &pb, uint32_t path_length, V3 wi, V3 path_weight, V3 postfix,
                               float rr_pdf, float mis_weight, float light_pdf, uint32_t light_type,
                               ct_path_reservoir &res) {
-  if (path_length < 1) return;
+  if (path_length < 1) return false;
   bool is_rc = path_length == pb.rc_vertex_length;
-  if (!pres_update(res, path_weight, rr_pdf, pb.seed)) return;
+  if (!pres_update(res, path_weight, rr_pdf, pb.seed)) return false;
   res.path_flags = pb.path_flags;
   pf_set_path_length(res.path_flags, path_length);
   pf_set_rc_length(res.path_flags, pb.rc_vertex_length);
@@ -1493,6 +1502,24 @@ static void add_escape_vertex(PathBuilder &pb, uint32_t path_length, V3 wi, V3 p
   }
   res.light_pdf = light_pdf;
   pf_set_light_type(res.path_flags, light_type);
+  return true;
+}
+
+// path_builder_mark_escape_vertex_as_rc_vertex, path_builder.glsl:114-139:
+// the selected escape vertex becomes the reconnection vertex of the
+// reservoir's sample (Lambertian: no delta, transmission or specular bits).
+static void mark_escape_vertex_as_rc(ct_path_reservoir &res, uint32_t length, const ct_object_hit &hit,
+                                     float light_pdf, uint32_t light_type, V3 irradiance, V3 wi,
+                                     float prev_scatter_pdf, float geometry_factor) {
+  pf_set_rc_length(res.path_flags, length);
+  res.rc_vertex_hit = hit;
+  res.path_flags &= ~((1u << 20) | (1u << 22));
+  res.light_pdf = light_pdf;
+  res.cached_jacobian[0] = prev_scatter_pdf;
+  pf_set_light_type(res.path_flags, light_type);
+  store3(res.rc_vertex_irradiance, irradiance);
+  store3(res.rc_vertex_wi, wi);
+  res.cached_jacobian[2] = geometry_factor;
 }
 
 // eval_voxel_pdf, path_tracer.glsl:185-211
@@ -1521,10 +1548,23 @@ static void path_handle_emissive_hit(const Ctx &c, PathState &p, const Isect &i)
     Lr = muls(mulv(mulv(p.thp, p.prefix_thp), em), mis);
   }
   V3 att = mulv(mulv(p.thp, p.prefix_thp), em);
+  if (p.replay) return;  // a replay only carries throughput to its stop vertex
   if (p.path_length > 1) p.L = add(p.L, Lr);
   V3 postfix = muls(mulv(p.thp, att), mis);
   uint32_t plen = (uint32_t)std::max(0, (int)p.path_length - 1);
-  add_escape_vertex(p.pb, plen, p.dir, Lr, postfix, p.rr_pdf, mis, light_pdf, CT_LIGHT_CUBE, p.res);
+  const bool selected = add_escape_vertex(p.pb, plen, p.dir, Lr, postfix, p.rr_pdf, mis, light_pdf, CT_LIGHT_CUBE, p.res);
+  // an escaped vertex at length >= 2 can be the reconnection vertex
+  // (path_tracer.glsl:303-335)
+  if (selected && p.hybrid && p.path_length >= 2 && p.path_length < p.pb.rc_vertex_length) {
+    const bool far = length(sub(i.P, p.origin)) >= c.p->restir.near_field_distance;
+    if (far && p.last_rough) {
+      V3 disp = sub(p.origin, i.P);
+      float g = std::fabs(dot(i.n, i.wo)) / dot(disp, disp);
+      if (std::isnan(g)) g = 0.0f;
+      mark_escape_vertex_as_rc(p.res, p.path_length, p.hit, light_pdf, CT_LIGHT_CUBE, em, v3(0, 0, 0),
+                               p.prev_scatter_pdf, g);
+    }
+  }
 }
 
 // path_generate_light_sample, path_tracer.glsl:222-256: the light sample
@@ -1567,14 +1607,37 @@ static void path_handle_hit(const Ctx &c, PathState &p, Isect &i) {
   p.origin = compute_ray_origin(i.P, i.n);
   p.origin = compute_ray_origin(p.origin, i.n);
   bool cur_rough = i.mat.roughness > rp.roughness_threshold;
-  // path_check_rc_vertex, path_tracer.glsl:484-597 (!use_hybrid_shift)
-  if (p.path_length == 1) {
-    p.pb.rc_vertex_hit = p.hit;
-    p.pb.cached_jacobian.x = p.prev_scatter_pdf;
-    V3 disp = sub(prev_origin, i.P);
-    p.pb.cached_jacobian.z = std::fabs(dot(i.n, i.wo)) / dot(disp, disp);
-    if (std::isnan(p.pb.cached_jacobian.z)) p.pb.cached_jacobian.z = 0.0f;
-    p.pb.rc_vertex_length = p.path_length;
+  // path_check_rc_vertex, path_tracer.glsl:474-597
+  const bool far = length(sub(p.origin, prev_origin)) >= rp.near_field_distance;
+  const bool connectable = far && cur_rough && p.last_rough;
+  if (!p.replay) {
+    const bool can_connect = p.path_length >= 1 && p.path_length < p.pb.rc_vertex_length && connectable;
+    if ((!p.hybrid && p.path_length == 1) || (p.hybrid && can_connect)) {
+      p.pb.rc_vertex_hit = p.hit;
+      p.pb.cached_jacobian.x = p.prev_scatter_pdf;
+      V3 disp = sub(prev_origin, i.P);
+      p.pb.cached_jacobian.z = std::fabs(dot(i.n, i.wo)) / dot(disp, disp);
+      if (std::isnan(p.pb.cached_jacobian.z)) p.pb.cached_jacobian.z = 0.0f;
+      p.pb.rc_vertex_length = p.path_length;
+    }
+  } else {
+    // The source path's reconnection vertex is the first connectable one,
+    // so each replayed vertex before it must not be connectable; the one
+    // right before it must be rough to connect from (invertibility, GRIS).
+    bool invertible = !(p.path_length >= 1 && p.path_length + 1 <= p.replay_rc && connectable);
+    if (invertible && p.path_length + 1 == p.replay_rc) {
+      if (!cur_rough) {
+        invertible = false;
+      } else {
+        p.replay_done = true;
+        p.replay_prev = i;
+        p.replay_tp = mulv(p.thp, p.prefix_thp);
+      }
+    }
+    if (!invertible || p.replay_done) {
+      path_set_flag(p, kPfActive, false);
+      return;
+    }
   }
   path_set_flag(p, kPfUpper, false);
   path_set_flag(p, kPfLower, false);
@@ -1583,7 +1646,9 @@ static void path_handle_hit(const Ctx &c, PathState &p, Isect &i) {
   path_light_sample(c, i, p.seed);
   path_set_flag(p, kPfUpper, true);
   path_set_flag(p, kPfLower, false);
-  if (rp.pt_russian_roulette && p.path_length >= 1) {
+  if (rp.pt_russian_roulette && p.path_length >= 1 && p.replay) {
+    (void)rnd(p.seed);  // a replay draws the roulette's number and goes on (path_tracer.glsl:696-699)
+  } else if (rp.pt_russian_roulette && p.path_length >= 1) {
     // terminate_path_by_russian_roulette, path_tracer.glsl:165-177
     float rr = luminance(mulv(p.thp, p.prefix_thp));
     float prob = gmax(0.0f, 1.0f - rr);
@@ -1604,13 +1669,25 @@ static void path_handle_hit(const Ctx &c, PathState &p, Isect &i) {
 
 // path_handle_miss, path_tracer.glsl:729-815
 static void path_handle_miss(const Ctx &c, PathState &p) {
+  if (p.replay) {  // a replay that escapes before its stop vertex is not invertible
+    path_set_flag(p, kPfActive, false);
+    return;
+  }
   float light_pdf = kInv4Pi;  // env_map_sampler_eval_pdf, light.glsl:71
   float mis = eval_mis(1, p.pdf, 1, light_pdf);
   V3 Le = sky_color(c.p->time, c.p->is_afternoon != 0, p.dir);
   V3 Lr = muls(mulv(mulv(p.thp, p.prefix_thp), Le), mis);
   if (p.path_length > 0) p.L = add(p.L, Lr);
-  add_escape_vertex(p.pb, p.path_length, p.dir, Lr, muls(mulv(p.thp, Le), mis), p.rr_pdf, mis, light_pdf,
-                    CT_LIGHT_ENV_MAP, p.res);
+  const bool selected = add_escape_vertex(p.pb, p.path_length, p.dir, Lr, muls(mulv(p.thp, Le), mis), p.rr_pdf,
+                                          mis, light_pdf, CT_LIGHT_ENV_MAP, p.res);
+  // escaped to the sky: a reconnection vertex at infinity (path_tracer.glsl:805-833)
+  if (selected && p.hybrid && p.path_length >= 1 && p.path_length + 1 < p.pb.rc_vertex_length && p.last_rough) {
+    ct_object_hit none{};
+    none.object.instance_id = CT_MAX_INSTANCES;
+    none.object.primitive_id = CT_MAX_PRIMITIVES;
+    mark_escape_vertex_as_rc(p.res, p.path_length + 1, none, light_pdf, CT_LIGHT_ENV_MAP, Le, p.dir,
+                             p.prev_scatter_pdf, 1.0f);
+  }
   path_set_flag(p, kPfActive, false);
 }
 
@@ -1631,6 +1708,7 @@ static ct_path_reservoir trace_restir_path(const Ctx &c, Isect i, uint32_t &seed
   p.pb.rc_vertex_hit.object.instance_id = CT_MAX_INSTANCES;
   p.pb.rc_vertex_hit.object.primitive_id = CT_MAX_PRIMITIVES;
   p.pb.rc_vertex_length = D;
+  p.hybrid = c.p->restir.pt_shift_mapping == CT_SHIFT_HYBRID;
   pres_init(p.res, D);
   // path_handle_primary_hit, path_tracer.glsl:440-469
   p.origin = compute_ray_origin(i.P, i.n);
@@ -1676,12 +1754,17 @@ static V3 compute_new_ray_origin(V3 pos, V3 n, bool view_side) {
   return compute_ray_origin(pos, view_side ? n : neg(n));
 }
 
-// compute_shifted_integrand_reconnection, shift.glsl:25-350
-// (use_hybrid_shift = false, use_cached_jacobian = false)
-static V3 shift_reconnection(const Ctx &c, float &dst_jacobian, Isect dst, Isect src, const ct_path_reservoir &s,
-                             bool eval_visibility, TraceCount &acnt) {
+// compute_shifted_integrand_reconnection, shift.glsl:25-350.  With
+// use_hybrid the reconnection vertex index comes from the path flags and a
+// near-field connection is rejected; with use_cached the source side's
+// geometry term and pdfs come from the reservoir's cached Jacobian, and a
+// successful shift leaves the destination's there.
+static V3 shift_reconnection(const Ctx &c, float &dst_jacobian, Isect dst, Isect src, ct_path_reservoir &s,
+                             bool eval_visibility, TraceCount &acnt, bool use_hybrid = false,
+                             bool use_cached = false) {
   dst_jacobian = 0.0f;
-  const uint32_t rc_vertex_length = 1;
+  float dst_cached[3] = {0.0f, 0.0f, 0.0f};
+  const uint32_t rc_vertex_length = use_hybrid ? pf_rc_length(s.path_flags) : 1u;
   const ct_object_hit &rch = s.rc_vertex_hit;
   V3 irr = load3(s.rc_vertex_irradiance), wi = load3(s.rc_vertex_wi);
   bool exists = rch.object.instance_id < CT_MAX_INSTANCES && rch.object.primitive_id < CT_MAX_PRIMITIVES;
@@ -1694,14 +1777,16 @@ static V3 shift_reconnection(const Ctx &c, float &dst_jacobian, Isect dst, Isect
     V3 integrand = v3(0, 0, 0);
     if (pf_light_type(s.path_flags) == CT_LIGHT_ENV_MAP && path_length + 1 == rc_vertex_length && !last_nee) {
       if (vertex_visible(c, dst.P, wi, kMaxDistance, kInvalid, kInvalid, acnt)) {  // is_segment_visible
-        float src_pdf1 = sample_material_pdf(src.mat, src.n, src.wo, wi);
+        float src_pdf1 = use_cached ? s.cached_jacobian[0] : sample_material_pdf(src.mat, src.n, src.wo, wi);
         float dst_pdf1 = sample_material_pdf(dst.mat, dst.n, dst.wo, wi);
+        dst_cached[0] = dst_pdf1;
         V3 dst_f1 = evaluate_material(dst.mat, dst.n, dst.wo, wi);
         float mis = eval_mis(1, dst_pdf1, 1, s.light_pdf);
         integrand = mulv(muls(divs(dst_f1, dst_pdf1), mis), irr);
         dst_jacobian = dst_pdf1 / src_pdf1;
       }
     }
+    if (use_cached) store3(s.cached_jacobian, v3(dst_cached[0], dst_cached[1], dst_cached[2]));
     if (jac_invalid(dst_jacobian)) dst_jacobian = 0.0f;
     if (bad3(integrand)) return v3(0, 0, 0);
     return integrand;
@@ -1716,15 +1801,23 @@ static V3 shift_reconnection(const Ctx &c, float &dst_jacobian, Isect dst, Isect
   V3 shifted_disp = sub(rc.P, dst.P);
   float shifted_dist2 = dot(shifted_disp, shifted_disp);
   float shifted_cosine = std::fabs(dot(rc.n, neg(dst_cv)));
+  if (use_hybrid && !(std::sqrt(shifted_dist2) >= c.p->restir.near_field_distance)) return v3(0, 0, 0);
   float cj_z = shifted_cosine / shifted_dist2;
-  V3 original_disp = sub(rc.P, src.P);
-  float original_dist2 = dot(original_disp, original_disp);
-  float original_cosine = std::fabs(dot(rc.n, neg(src_cv)));
-  float jacobian = cj_z * original_dist2 / original_cosine;
+  dst_cached[2] = cj_z;
+  float jacobian;
+  if (use_cached) {
+    jacobian = cj_z / s.cached_jacobian[2];
+  } else {
+    V3 original_disp = sub(rc.P, src.P);
+    float original_dist2 = dot(original_disp, original_disp);
+    float original_cosine = std::fabs(dot(rc.n, neg(src_cv)));
+    jacobian = cj_z * original_dist2 / original_cosine;
+  }
   if (jac_invalid(jacobian)) return v3(0, 0, 0);
   float dst_pdf1 = sample_material_pdf(dst.mat, dst.n, dst.wo, dst_cv);
   float dst_pdf1_all = dst_pdf1;
-  float src_pdf1 = sample_material_pdf(src.mat, src.n, src.wo, src_cv);
+  dst_cached[0] = dst_pdf1;
+  float src_pdf1 = use_cached ? s.cached_jacobian[0] : sample_material_pdf(src.mat, src.n, src.wo, src_cv);
   jacobian *= dst_pdf1 / src_pdf1;
   if (jac_invalid(jacobian)) return v3(0, 0, 0);
   V3 dst_f1 = evaluate_material(dst.mat, dst.n, dst.wo, dst_cv);
@@ -1732,7 +1825,8 @@ static V3 shift_reconnection(const Ctx &c, float &dst_jacobian, Isect dst, Isect
   if (!escaped) {
     dst_rc = sample_material_pdf(rc.mat, rc.n, rc.wo, wi);
     dst_rc_all = dst_rc;
-    src_rc = sample_material_pdf(rc.mat, rc.n, neg(src_cv), wi);
+    dst_cached[1] = dst_rc;
+    src_rc = use_cached ? s.cached_jacobian[1] : sample_material_pdf(rc.mat, rc.n, neg(src_cv), wi);
     dst_pdf2 = !rc_nee ? dst_rc : s.light_pdf;
   }
   V3 dst_f2 = v3(1.0f, 1.0f, 1.0f);
@@ -1761,29 +1855,130 @@ static V3 shift_reconnection(const Ctx &c, float &dst_jacobian, Isect dst, Isect
   }
   if (bad3(integrand)) return v3(0, 0, 0);
   dst_jacobian = jacobian;
+  if (use_cached) store3(s.cached_jacobian, v3(dst_cached[0], dst_cached[1], dst_cached[2]));
   return integrand;
 }
 
-// compute_shifted_integrand_, shift.glsl:446-483.  The hybrid shift runs on
-// RECONNECTION_DATA whose storage the reference stubs
-// (path_reservoir.glsl:37-58): with it compute_shifted_integrand_hybrid
-// (shift.glsl:352-444) returns tp = 1 for reconnection length 1, tp = 0
-// otherwise, and leaves dst_jacobian = 0.
+// RECONNECTION_DATA (shared.inl:380-394) as the retrace produces it: the
+// destination path's vertex before the reconnection vertex and the
+// throughput that reaches it.
+struct RcData {
+  bool ok = false;
+  Isect prev;
+  V3 tp{0, 0, 0};
+};
+
+// The retrace of trace_hybrid_shift_rays / trace_random_replay_path_hybrid_simple
+// (indirect_illumination.glsl:12-28, path_tracer.glsl:889-966): replay the
+// source reservoir's random numbers from the destination's primary
+// intersection through the frame's own sampling steps until the vertex
+// before the source's reconnection vertex.  The replay is not invertible
+// (ok = false) if it misses, stops early, reaches a connectable vertex
+// before that one or that vertex is not rough.
+static RcData hybrid_retrace(const Ctx &c, Isect i, const ct_path_reservoir &src, TraceCount &bcnt) {
+  RcData out;
+  const uint32_t D = c.p->restir.max_depth;
+  PathState p;
+  p.max_depth = D;
+  p.flags = 0;
+  p.path_length = 0;
+  path_set_flag(p, kPfActive, true);
+  p.bounce_counters = 0;
+  set_objhit(p.hit, i.inst, i.prim, i.P);
+  path_set_flag(p, kPfHit, true);
+  p.seed = src.init_random_seed;
+  p.pb.seed = p.seed;
+  p.pb.rc_vertex_hit.object.instance_id = CT_MAX_INSTANCES;
+  p.pb.rc_vertex_hit.object.primitive_id = CT_MAX_PRIMITIVES;
+  p.pb.rc_vertex_length = pf_rc_length(src.path_flags);
+  p.hybrid = true;
+  p.replay = true;
+  p.replay_rc = pf_rc_length(src.path_flags);
+  pres_init(p.res, D);
+  p.origin = compute_ray_origin(i.P, i.n);
+  p.origin = compute_ray_origin(p.origin, i.n);
+  bool valid = path_generate_scatter_ray(p, i);
+  p.last_rough = i.mat.roughness > c.p->restir.roughness_threshold;  // generate_replay_primary_scatter_ray
+  if (valid) {
+    do {
+      Isect h = intersect_bounce(c, p.origin, p.dir, bcnt);
+      if (h.is_hit) {
+        set_objhit(p.hit, h.inst, h.prim, h.P);
+        path_set_flag(p, kPfHit, true);
+        p.path_length++;
+        p.scene_length += h.distance;
+        path_handle_hit(c, p, h);
+      } else {
+        path_set_flag(p, kPfHit, false);
+        path_handle_miss(c, p);
+      }
+    } while (p.flags & kPfActive);
+  }
+  if (p.replay_done) {
+    out.ok = true;
+    out.prev = p.replay_prev;
+    out.tp = p.replay_tp;
+  }
+  return out;
+}
+
+// compute_shifted_integrand_hybrid, shift.glsl:352-444: random replay up to
+// the vertex before the reconnection vertex, then the reconnection from
+// there with the cached Jacobian (or from the destination's primary vertex
+// when the reconnection vertex is the first bounce).  Two points differ
+// from the text, which as written rejects every shift: the Jacobian starts
+// at 1 (the replayed prefix has unit Jacobian in primary-sample space) where
+// the text sets 0 and multiplies into it (:363, :437), and reconnection
+// length 1 reconnects from the destination's primary vertex where the text
+// passes an invalid hit (:376-378).  A source whose reconnection vertex
+// does not exist (the path ended first) is not shifted.
+static V3 hybrid_integrand(const Ctx &c, float &dst_jacobian, const Isect &dst, const Isect &src,
+                           ct_path_reservoir &s, bool eval_visibility, TraceCount &acnt) {
+  dst_jacobian = 0.0f;
+  if (s.weight == 0.0f) return v3(0, 0, 0);
+  const uint32_t k = pf_rc_length(s.path_flags), plen = pf_path_length(s.path_flags);
+  const bool escaped = plen + 1 == k;
+  if (!(k <= plen || escaped) || k == 0) return v3(0, 0, 0);
+  V3 tp = v3(1.0f, 1.0f, 1.0f);
+  Isect prev = dst;
+  if (k > 1) {
+    RcData rc = hybrid_retrace(c, dst, s, acnt);
+    if (!rc.ok) return v3(0, 0, 0);
+    tp = rc.tp;
+    prev = rc.prev;
+  }
+  if (!(tp.x > 0.0f || tp.y > 0.0f || tp.z > 0.0f)) return v3(0, 0, 0);
+  float rj = 0.0f;
+  const V3 rc_tp = shift_reconnection(c, rj, prev, src, s, eval_visibility, acnt, true, k > 1);
+  dst_jacobian = 1.0f * rj;
+  return mulv(tp, rc_tp);
+}
+
+// compute_shifted_integrand_, shift.glsl:446-483.  `s` is the caller's
+// copy of the source reservoir: the hybrid shift leaves the destination's
+// cached Jacobian in it.
 static V3 shifted_integrand(const Ctx &c, float &dst_jacobian, const Isect &dst, const Isect &src,
-                            const ct_path_reservoir &s, bool eval_visibility, TraceCount &acnt) {
+                            ct_path_reservoir &s, bool eval_visibility, TraceCount &acnt) {
   dst_jacobian = 0.0f;
   if (s.weight == 0.0f) return v3(0, 0, 0);
   const uint32_t shift = c.p->restir.pt_shift_mapping;
   if (shift == CT_SHIFT_RECONNECTION) return shift_reconnection(c, dst_jacobian, dst, src, s, eval_visibility, acnt);
-  if (shift == CT_SHIFT_HYBRID) return pf_rc_length(s.path_flags) == 1 ? v3(1, 1, 1) : v3(0, 0, 0);
+  if (shift == CT_SHIFT_HYBRID) return hybrid_integrand(c, dst_jacobian, dst, src, s, eval_visibility, acnt);
   return v3(1, 1, 1);
+}
+// compute_shifted_integrand, shift.glsl:485-497: on a copy of the reservoir.
+static V3 shifted_integrand_copy(const Ctx &c, float &dst_jacobian, const Isect &dst, const Isect &src,
+                                 const ct_path_reservoir &s, bool eval_visibility, TraceCount &acnt) {
+  ct_path_reservoir t = s;
+  return shifted_integrand(c, dst_jacobian, dst, src, t, eval_visibility, acnt);
 }
 
 // shift_and_merge_reservoir, shift.glsl:485-510 (force_merge)
 static bool shift_and_merge(const Ctx &c, float &dst_jacobian, const Isect &dst, ct_path_reservoir &dst_res,
                             const Isect &src, const ct_path_reservoir &src_res, uint32_t &seed, TraceCount &acnt) {
-  V3 integrand = shifted_integrand(c, dst_jacobian, dst, src, src_res, true, acnt);
-  bool selected = pres_merge(dst_res, integrand, dst_jacobian, src_res, seed, 1.0f, true);
+  ct_path_reservoir tmp = src_res;
+  V3 integrand = shifted_integrand(c, dst_jacobian, dst, src, tmp, true, acnt);
+  bool selected = pres_merge(dst_res, integrand, dst_jacobian, tmp, seed, 1.0f, true);
   if (!selected) store3(dst_res.F, v3(0, 0, 0));
   dst_res.M = src_res.M;
   dst_res.weight = src_res.weight;
@@ -1829,7 +2024,7 @@ static ct_path_reservoir temporal_path_reuse(const Ctx &c, const ct_path_reservo
             continue;
           }
           float tnj = 0.0f;
-          V3 tni = shifted_integrand(c, tnj, prev_i, cur_i, temp_dst, true, acnt);
+          V3 tni = shifted_integrand_copy(c, tnj, prev_i, cur_i, temp_dst, true, acnt);
           float pp = lum601(tni) * tnj;
           p_sum += pp * temporal.M;
         }
@@ -1887,7 +2082,7 @@ static ct_path_reservoir spatial_path_reuse(const Ctx &c, uint32_t x, uint32_t y
     const ct_path_reservoir &nres = c.fr->pres_o[ni];
     valid++;
     float prefix_jac = 0.0f;
-    V3 prefix = shifted_integrand(c, prefix_jac, nis, ci, central, true, acnt);
+    V3 prefix = shifted_integrand_copy(c, prefix_jac, nis, ci, central, true, acnt);
     float prefix_pdf = lum601(prefix) * prefix_jac;
     canonical += 1.0f;
     if (prefix_pdf > 0.0f)
@@ -2876,6 +3071,26 @@ int oc_probe_temporal(oc_scene *scene, const ct_frame_params *p, const ct_di_inf
   return CT_OK;
 }
 
+// The hybrid shift's retrace of reservoir `res` from the primary vertex
+// `dst`: whether it is invertible, the vertex before the reconnection
+// vertex (position, normal, wo, instance, primitive) and the throughput
+// reaching it.
+int oc_probe_retrace(oc_scene *scene, const ct_frame_params *p, const ct_di_info *dst, const ct_path_reservoir *res,
+                     int *ok, float pos[3], float nrm[3], float wo[3], uint32_t ids[2], float tp[3]) {
+  if (!scene || !p || !dst || !res || !ok || !pos || !nrm || !wo || !ids || !tp) return CT_E_INVALID;
+  Ctx c = probe_ctx(scene, p);
+  TraceCount cnt;
+  const RcData rc = hybrid_retrace(c, isect_from_di(c, *dst, true), *res, cnt);
+  *ok = rc.ok ? 1 : 0;
+  store3(pos, rc.prev.P);
+  store3(nrm, rc.prev.n);
+  store3(wo, rc.prev.wo);
+  ids[0] = rc.prev.inst;
+  ids[1] = rc.prev.prim;
+  store3(tp, rc.tp);
+  return CT_OK;
+}
+
 // The pairwise MIS combination of spatial reuse (pairwise_mis.glsl:8-114)
 // of the reservoir `r` at `center` with the given neighbours, as
 // spatial_reuse runs it once its neighbours are chosen.
@@ -2910,7 +3125,7 @@ int oc_probe_shift(oc_scene *scene, const ct_frame_params *p, const ct_di_info *
   Ctx c = probe_ctx(scene, p);
   const Isect di = isect_from_di(c, *dst, normalize_dst_wo != 0), si = isect_from_di(c, *src, true);
   TraceCount cnt;
-  V3 f = shifted_integrand(c, *jacobian, di, si, *res, true, cnt);
+  V3 f = shifted_integrand_copy(c, *jacobian, di, si, *res, true, cnt);
   store3(integrand, f);
   return CT_OK;
 }

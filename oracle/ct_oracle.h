@@ -136,6 +136,8 @@ int oc_probe_temporal(oc_scene *scene, const ct_frame_params *p, const ct_di_inf
 int oc_probe_pairwise(oc_scene *scene, const ct_frame_params *p, const ct_di_info *center, const ct_reservoir *r,
                       uint32_t num, const ct_di_info *ndi, const ct_reservoir *nres, uint32_t seed,
                       ct_reservoir *out);
+int oc_probe_retrace(oc_scene *scene, const ct_frame_params *p, const ct_di_info *dst, const ct_path_reservoir *res,
+                     int *ok, float pos[3], float nrm[3], float wo[3], uint32_t ids[2], float tp[3]);
 int oc_probe_shift(oc_scene *scene, const ct_frame_params *p, const ct_di_info *dst, const ct_di_info *src,
                    int normalize_dst_wo, const ct_path_reservoir *res, float *jacobian, float integrand[3]);
 

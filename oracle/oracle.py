@@ -84,6 +84,8 @@ def lib():
                                    P(Reservoir)], C.c_int),
             "oc_probe_pairwise": ([C.c_void_p, P(FrameParams), P(DiInfo), P(Reservoir), u32, C.c_void_p,
                                    C.c_void_p, u32, P(Reservoir)], C.c_int),
+            "oc_probe_retrace": ([C.c_void_p, P(FrameParams), P(DiInfo), P(PathReservoir), P(C.c_int), P(f32), P(f32),
+                                  P(f32), P(u32), P(f32)], C.c_int),
             "oc_probe_shift": ([C.c_void_p, P(FrameParams), P(DiInfo), P(DiInfo), C.c_int, P(PathReservoir),
                                 P(f32), P(f32)], C.c_int),
         }
@@ -281,6 +283,19 @@ class OScene:
                                   1 if normalize_dst_wo else 0, C.byref(res), C.byref(jac), f)
         assert rc == 0, rc
         return float(jac.value), np.array(f[:], np.float32)
+
+    def probe_retrace(self, params, dst, res):
+        """The hybrid shift's retrace of `res` from primary vertex `dst`:
+        dict(ok, pos, nrm, wo, ids, tp) of the vertex before the
+        reconnection vertex."""
+        ok = C.c_int(0)
+        pos, nrm, wo, tp = (f32 * 3)(), (f32 * 3)(), (f32 * 3)(), (f32 * 3)()
+        ids = (u32 * 2)()
+        rc = lib().oc_probe_retrace(self.handle, C.byref(params), C.byref(dst), C.byref(res), C.byref(ok), pos, nrm,
+                                    wo, ids, tp)
+        assert rc == 0, rc
+        arr = lambda a: np.array(a[:], np.float32)
+        return dict(ok=bool(ok.value), pos=arr(pos), nrm=arr(nrm), wo=arr(wo), ids=tuple(ids[:]), tp=arr(tp))
 
     def read_bvh(self, instance, n):
         keys = np.zeros(n, np.uint32)
