@@ -4,6 +4,7 @@
 // and load_scene (src/main.cpp:1045-1126).
 
 #include <algorithm>
+#include <omp.h>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -204,6 +205,8 @@ int ct_scene_generate(const ct_scene_desc *d, ct_scene **out) {
   if (d->occupied_count == 0 || d->occupied_count > ncell || d->occupied_count > CT_MAX_PRIMITIVES)
     return fail(CT_E_INVALID, "ct_scene_generate: occupied_count out of range");
   if (d->emissive_count > d->occupied_count) return fail(CT_E_INVALID, "ct_scene_generate: emissive_count > occupied_count");
+  if (d->palette_size > 256 || d->palette_emissive > d->palette_size)
+    return fail(CT_E_INVALID, "ct_scene_generate: palette of more than 256 entries or more emissive entries than entries");
   const float e = d->voxel_extent > 0.0f ? d->voxel_extent : CT_VOXEL_EXTENT;
   std::unique_ptr<ct_scene> sc(new ct_scene());
   sc->desc = *d;
@@ -211,63 +214,119 @@ int ct_scene_generate(const ct_scene_desc *d, ct_scene **out) {
   const double freq[3] = {1.0 / 16.0, 1.0 / 8.0, 1.0 / 4.0}, amp[3] = {1.0, 0.5, 0.25};
   Lattice lat[3];
   for (int o = 0; o < 3; ++o) lat[o].build(d->seed, o, (int)gx, (int)gy, (int)gz, freq[o]);
-  std::vector<uint64_t> keys(ncell);
+  auto noise_key = [&](uint32_t x, uint32_t y, uint32_t z) {
+    double v = 0.0;
+    for (int o = 0; o < 3; ++o)
+      v += amp[o] * lat[o].eval((x + 0.5) * freq[o], (y + 0.5) * freq[o], (z + 0.5) * freq[o]);
+    return ord_bits((float)v);
+  };
+  // Exactly occupied_count cells: the largest noise values, ties to the
+  // smaller linear index.  The cell key is (noise bits << 32) | ~linear
+  // index; the threshold key is the occupied_count-th largest, found from a
+  // histogram of the top 16 noise bits and an exact selection inside the
+  // bin that holds it, so no per-cell array is kept.
+  const int nthr = std::max(1, omp_get_max_threads());
+  std::vector<uint64_t> hist((size_t)nthr << 16, 0);
 #pragma omp parallel for schedule(static)
-  for (int64_t z = 0; z < (int64_t)gz; ++z)
+  for (int64_t z = 0; z < (int64_t)gz; ++z) {
+    uint64_t *h = &hist[(size_t)omp_get_thread_num() << 16];
+    for (uint32_t y = 0; y < gy; ++y)
+      for (uint32_t x = 0; x < gx; ++x) ++h[noise_key(x, y, (uint32_t)z) >> 16];
+  }
+  uint64_t above = 0;
+  uint32_t bin = 0xFFFFu;
+  for (;; --bin) {
+    uint64_t c = 0;
+    for (int t = 0; t < nthr; ++t) c += hist[((size_t)t << 16) + bin];
+    if (above + c >= d->occupied_count || bin == 0) break;
+    above += c;
+  }
+  std::vector<std::vector<uint64_t>> cand(nthr);
+#pragma omp parallel for schedule(static)
+  for (int64_t z = 0; z < (int64_t)gz; ++z) {
+    std::vector<uint64_t> &cd = cand[omp_get_thread_num()];
     for (uint32_t y = 0; y < gy; ++y)
       for (uint32_t x = 0; x < gx; ++x) {
-        double v = 0.0;
-        for (int o = 0; o < 3; ++o)
-          v += amp[o] * lat[o].eval((x + 0.5) * freq[o], (y + 0.5) * freq[o], (z + 0.5) * freq[o]);
+        uint32_t k = noise_key(x, y, (uint32_t)z);
+        if ((k >> 16) != bin) continue;
         uint64_t lin = ((uint64_t)z * gy + y) * gx + x;
-        keys[lin] = ((uint64_t)ord_bits((float)v) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)lin);
+        cd.push_back(((uint64_t)k << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)lin));
       }
-  // Exactly occupied_count cells: the largest noise values, ties to the
-  // smaller linear index.
-  std::vector<uint64_t> sel(keys);
-  std::nth_element(sel.begin(), sel.begin() + (d->occupied_count - 1), sel.end(), std::greater<uint64_t>());
-  const uint64_t thr = sel[d->occupied_count - 1];
-  std::vector<uint8_t> occ(ncell);
-  for (uint64_t i = 0; i < ncell; ++i) occ[i] = keys[i] >= thr ? 1 : 0;
+  }
+  std::vector<uint64_t> sel;
+  for (auto &cd : cand) sel.insert(sel.end(), cd.begin(), cd.end());
+  std::vector<std::vector<uint64_t>>().swap(cand);
+  const uint64_t need = d->occupied_count - above;
+  std::nth_element(sel.begin(), sel.begin() + (need - 1), sel.end(), std::greater<uint64_t>());
+  const uint64_t thr = sel[need - 1];
   std::vector<uint64_t>().swap(sel);
-  std::vector<uint64_t>().swap(keys);
   // Chunks in z-major order become instances; voxels inside a chunk in
-  // z, y, x order become the instance's primitives.
+  // z, y, x order become the instance's primitives.  Chunks are filled in
+  // parallel and concatenated in chunk order.
   const uint32_t cx = d->chunk[0], cy = d->chunk[1], cz = d->chunk[2];
   const uint32_t ncx = (gx + cx - 1) / cx, ncy = (gy + cy - 1) / cy, ncz = (gz + cz - 1) / cz;
   const int hx = (int)gx / 2, hy = (int)gy / 2, hz = (int)gz / 2;
-  std::vector<uint32_t> voxel_inst, voxel_prim;
-  for (uint32_t bz = 0; bz < ncz; ++bz)
-    for (uint32_t by = 0; by < ncy; ++by)
-      for (uint32_t bx = 0; bx < ncx; ++bx) {
-        uint32_t first = (uint32_t)sc->aabbs.size(), count = 0;
-        for (uint32_t z = bz * cz; z < std::min(gz, (bz + 1) * cz); ++z)
-          for (uint32_t y = by * cy; y < std::min(gy, (by + 1) * cy); ++y)
-            for (uint32_t x = bx * cx; x < std::min(gx, (bx + 1) * cx); ++x) {
-              if (!occ[((uint64_t)z * gy + y) * gx + x]) continue;
-              ct_aabb b;
-              b.minimum[0] = (float)((int)x - hx) * e;
-              b.minimum[1] = (float)((int)y - hy) * e;
-              b.minimum[2] = (float)((int)z - hz) * e;
-              b.maximum[0] = (float)((int)x - hx + 1) * e;
-              b.maximum[1] = (float)((int)y - hy + 1) * e;
-              b.maximum[2] = (float)((int)z - hz + 1) * e;
-              sc->aabbs.push_back(b);
-              voxel_inst.push_back((uint32_t)sc->instances.size());
-              voxel_prim.push_back(count);
-              ++count;
-            }
-        if (count == 0) continue;
-        ct_instance in{};
-        for (int k = 0; k < 16; ++k) in.transform[k] = (k % 5 == 0) ? 1.0f : 0.0f;
-        in.first_primitive_index = first;
-        in.primitive_count = count;
-        sc->instances.push_back(in);
-      }
+  const int64_t nchunk = (int64_t)ncx * ncy * ncz;
+  std::vector<std::vector<ct_aabb>> chunk_boxes(nchunk);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t c = 0; c < nchunk; ++c) {
+    const uint32_t bx = (uint32_t)(c % ncx), by = (uint32_t)((c / ncx) % ncy), bz = (uint32_t)(c / ((int64_t)ncx * ncy));
+    std::vector<ct_aabb> &out_boxes = chunk_boxes[c];
+    for (uint32_t z = bz * cz; z < std::min(gz, (bz + 1) * cz); ++z)
+      for (uint32_t y = by * cy; y < std::min(gy, (by + 1) * cy); ++y)
+        for (uint32_t x = bx * cx; x < std::min(gx, (bx + 1) * cx); ++x) {
+          uint64_t lin = ((uint64_t)z * gy + y) * gx + x;
+          uint64_t key = ((uint64_t)noise_key(x, y, z) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)lin);
+          if (key < thr) continue;
+          ct_aabb b;
+          b.minimum[0] = (float)((int)x - hx) * e;
+          b.minimum[1] = (float)((int)y - hy) * e;
+          b.minimum[2] = (float)((int)z - hz) * e;
+          b.maximum[0] = (float)((int)x - hx + 1) * e;
+          b.maximum[1] = (float)((int)y - hy + 1) * e;
+          b.maximum[2] = (float)((int)z - hz + 1) * e;
+          out_boxes.push_back(b);
+        }
+  }
+  uint64_t total = 0;
+  for (auto &cb : chunk_boxes) total += cb.size();
+  if (total != d->occupied_count) return fail(CT_E_STATE, "ct_scene_generate: occupancy selection miscounted");
+  sc->aabbs.reserve(total);
+  for (auto &cb : chunk_boxes) {
+    if (cb.empty()) continue;
+    ct_instance in{};
+    for (int k = 0; k < 16; ++k) in.transform[k] = (k % 5 == 0) ? 1.0f : 0.0f;
+    in.first_primitive_index = (uint32_t)sc->aabbs.size();
+    in.primitive_count = (uint32_t)cb.size();
+    sc->instances.push_back(in);
+    sc->aabbs.insert(sc->aabbs.end(), cb.begin(), cb.end());
+    std::vector<ct_aabb>().swap(cb);
+  }
   const uint32_t np = (uint32_t)sc->aabbs.size();
-  // Emissive voxels: a seeded partial Fisher-Yates draw, kept in voxel order.
+  const bool palette = d->palette_size > 0;
+  // Palette mode: one uint8 entry per voxel (a hash of the seed and the voxel
+  // index), and the palette's emissive entries (a seeded draw).
+  std::vector<uint8_t> entry;
+  std::vector<uint8_t> entry_emits(palette ? d->palette_size : 0, 0);
+  if (palette) {
+    entry.resize(np);
+    for (uint32_t v = 0; v < np; ++v) entry[v] = (uint8_t)(tea_host(d->seed ^ 0x9A1u, v) % d->palette_size);
+    Rng re(tea_host(d->seed, 0xE3Au));
+    std::vector<uint32_t> perm(d->palette_size);
+    for (uint32_t i = 0; i < d->palette_size; ++i) perm[i] = i;
+    for (uint32_t i = 0; i < d->palette_emissive; ++i) {
+      uint32_t j = i + (uint32_t)re.below(d->palette_size - i);
+      std::swap(perm[i], perm[j]);
+      entry_emits[perm[i]] = 1;
+    }
+  }
+  // Emissive voxels: those of an emissive palette entry, or a seeded partial
+  // Fisher-Yates draw; kept in voxel order.
   std::vector<uint32_t> lightsel;
-  {
+  if (palette) {
+    for (uint32_t v = 0; v < np; ++v)
+      if (entry_emits[entry[v]]) lightsel.push_back(v);
+  } else {
     Rng r(tea_host(d->seed, 0xE1u));
     std::vector<uint32_t> perm(np);
     for (uint32_t i = 0; i < np; ++i) perm[i] = i;
@@ -277,22 +336,6 @@ int ct_scene_generate(const ct_scene_desc *d, ct_scene **out) {
     }
     lightsel.assign(perm.begin(), perm.begin() + d->emissive_count);
     std::sort(lightsel.begin(), lightsel.end());
-  }
-  std::vector<uint32_t> light_of(np, CT_INVALID_INDEX);
-  Rng rl(tea_host(d->seed, 0x1E11u));
-  for (uint32_t l = 0; l < (uint32_t)lightsel.size(); ++l) {
-    uint32_t v = lightsel[l];
-    light_of[v] = l;
-    ct_light L{};
-    const ct_aabb &b = sc->aabbs[v];
-    for (int a = 0; a < 3; ++a) L.position[a] = (b.minimum[a] + b.maximum[a]) * 0.5f;
-    float le = (float)(d->le_min + (d->le_max - d->le_min) * rl.uniform());
-    L.emissive[0] = L.emissive[1] = L.emissive[2] = le;
-    L.instance_info.instance_id = voxel_inst[v];
-    L.instance_info.primitive_id = voxel_prim[v];
-    L.size = e;
-    L.type = CT_LIGHT_CUBE;
-    sc->lights.push_back(L);
   }
   // Materials: one per voxel (Lambertian, map_loader.cpp:209-220 fields)
   // or a shared palette.
@@ -305,8 +348,45 @@ int ct_scene_generate(const ct_scene_desc *d, ct_scene **out) {
     return m;
   };
   const float amin = d->albedo_min, aspan = d->albedo_max - d->albedo_min;
+  if (palette) {
+    sc->mats.resize(d->palette_size);
+    Rng rp(tea_host(d->seed, 0xE3Bu));
+    for (uint32_t p = 0; p < d->palette_size; ++p) {
+      float r = (float)(amin + aspan * ra.uniform()), g = (float)(amin + aspan * ra.uniform()),
+            b = (float)(amin + aspan * ra.uniform());
+      sc->mats[p] = make_mat(r, g, b);
+      if (entry_emits[p])
+        for (int a = 0; a < 3; ++a) sc->mats[p].emission[a] = (float)(d->le_min + (d->le_max - d->le_min) * rp.uniform());
+    }
+  }
+  std::vector<uint32_t> light_of(np, CT_INVALID_INDEX);
+  Rng rl(tea_host(d->seed, 0x1E11u));
+  for (uint32_t l = 0; l < (uint32_t)lightsel.size(); ++l) {
+    uint32_t v = lightsel[l];
+    light_of[v] = l;
+    ct_light L{};
+    const ct_aabb &b = sc->aabbs[v];
+    for (int a = 0; a < 3; ++a) L.position[a] = (b.minimum[a] + b.maximum[a]) * 0.5f;
+    if (palette) {
+      for (int a = 0; a < 3; ++a) L.emissive[a] = sc->mats[entry[v]].emission[a];
+    } else {
+      float le = (float)(d->le_min + (d->le_max - d->le_min) * rl.uniform());
+      L.emissive[0] = L.emissive[1] = L.emissive[2] = le;
+    }
+    // the instance holding voxel v: the last one whose first primitive is <= v
+    uint32_t lo = 0, hi = (uint32_t)sc->instances.size();
+    while (hi - lo > 1) {
+      uint32_t mid = (lo + hi) / 2;
+      if (sc->instances[mid].first_primitive_index <= v) lo = mid; else hi = mid;
+    }
+    L.instance_info.instance_id = lo;
+    L.instance_info.primitive_id = v - sc->instances[lo].first_primitive_index;
+    L.size = e;
+    L.type = CT_LIGHT_CUBE;
+    sc->lights.push_back(L);
+  }
   sc->prims.resize(np);
-  if (d->palette_size == 0) {
+  if (!palette) {
     sc->mats.resize(np);
     for (uint32_t v = 0; v < np; ++v) {
       float r = (float)(amin + aspan * ra.uniform()), g = (float)(amin + aspan * ra.uniform()),
@@ -318,14 +398,8 @@ int ct_scene_generate(const ct_scene_desc *d, ct_scene **out) {
       sc->prims[v].light_index = light_of[v];
     }
   } else {
-    sc->mats.resize(d->palette_size);
-    for (uint32_t p = 0; p < d->palette_size; ++p) {
-      float r = (float)(amin + aspan * ra.uniform()), g = (float)(amin + aspan * ra.uniform()),
-            b = (float)(amin + aspan * ra.uniform());
-      sc->mats[p] = make_mat(r, g, b);
-    }
     for (uint32_t v = 0; v < np; ++v) {
-      sc->prims[v].material_index = tea_host(d->seed ^ 0x9A1u, v) % d->palette_size;
+      sc->prims[v].material_index = entry[v];
       sc->prims[v].light_index = light_of[v];
     }
   }

@@ -241,7 +241,7 @@ class BatchTuning(C.Structure):
 class SceneDesc(C.Structure):
     _fields_ = [("grid", u32 * 3), ("chunk", u32 * 3), ("occupied_count", u32), ("emissive_count", u32),
                 ("le_min", f32), ("le_max", f32), ("albedo_min", f32), ("albedo_max", f32), ("seed", u32),
-                ("palette_size", u32), ("voxel_extent", f32), ("pad", u32)]
+                ("palette_size", u32), ("voxel_extent", f32), ("palette_emissive", u32)]
 
 
 class VoxDesc(C.Structure):

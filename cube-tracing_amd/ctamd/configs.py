@@ -16,7 +16,7 @@ from typing import Callable, Tuple
 
 import numpy as np
 
-__all__ = ["Workload", "small_plain", "hd_di", "frame_params", "scene_desc", "batched_views", "vox_view"]
+__all__ = ["Workload", "small_plain", "hd_di", "palette_pt", "frame_params", "scene_desc", "batched_views", "vox_view"]
 
 
 def _lib():
@@ -44,6 +44,7 @@ class Workload:
     albedo: Tuple[float, float] = (0.2, 0.8)
     time: float = 1.0          # Status.time as main.cpp:1289 sets it
     afternoon: int = 1         # Status.is_afternoon, main.cpp:1290
+    palette: Tuple[int, int] = (0, 0)  # (entries, emissive entries); (0, 0): one material per voxel
     extra: dict = field(default_factory=dict)
 
 
@@ -57,7 +58,7 @@ def scene_desc(w: Workload):
     d.le_min, d.le_max = w.le
     d.albedo_min, d.albedo_max = w.albedo
     d.seed = w.seed
-    d.palette_size = 0
+    d.palette_size, d.palette_emissive = w.palette
     d.voxel_extent = CT_VOXEL_EXTENT
     return d
 
@@ -121,6 +122,31 @@ def hd_di(frames=16, width=1920, height=1080, grid=256, chunk=64, indirect=True,
         emissive=round(0.02 * occ), le=(1.0, 20.0), seed=0xC1, width=width, height=height, frames=frames,
         fov_radians=math.radians(60.0), camera=_orbit_camera(1.5 * extent, 0.375 * extent, 1.0),
         features=CT_RIS_CUBE_LIGHT_BIT, taa=True, restir=rp)
+
+
+def palette_pt(frames=16, width=1920, height=1080, grid=512, chunk=64, max_depth=4):
+    """A sparse 512^3 palette scene: 12 % of the cells occupied (16,106,127
+    voxels, u32 voxel indices) in 64^3 chunks, one uint8 entry of a
+    256-entry palette per voxel (albedo U[0.2, 0.8] per channel; 1 % of the
+    entries, 3, emit radiance U[1, 20] per channel, so every voxel of them is
+    a cube light); ReSTIR DI plus ReSTIR PT with the hybrid shift, 4
+    bounces, path M-cap 20 (MAX_INFLUENCE_FROM_THE_PAST_THRESHOLD_PT) and 3
+    spatial neighbours (NEIGHBOR_COUNT), at 1920x1080 with a 1 degree/frame
+    camera orbit and TAA at 0.1 blend."""
+    from . import CT_RIS_CUBE_LIGHT_BIT, CT_SHIFT_HYBRID, restir_params_reference
+    rp = restir_params_reference()
+    rp.taa_alpha = 0.1
+    _indirect(rp, True, max_depth)
+    rp.pt_shift_mapping = CT_SHIFT_HYBRID
+    rp.pt_m_cap = 20.0
+    rp.pt_neighbor_count = 3
+    occ = round(0.12 * grid ** 3)
+    extent = grid * 0.03125
+    return Workload(
+        name=f"palette_pt_{grid}", grid=(grid, grid, grid), chunk=(chunk, chunk, chunk), occupied=occ,
+        emissive=0, le=(1.0, 20.0), seed=0xC2, width=width, height=height, frames=frames,
+        fov_radians=math.radians(60.0), camera=_orbit_camera(1.5 * extent, 0.375 * extent, 1.0),
+        features=CT_RIS_CUBE_LIGHT_BIT, taa=True, restir=rp, palette=(256, 3))
 
 
 def vox_view(scene, width=1920, height=1080, frames=16, indirect=True, max_depth=3, name="vox"):

@@ -676,9 +676,14 @@ typedef struct ct_scene_desc {
   float le_min, le_max;      /* emitted radiance range (white) */
   float albedo_min, albedo_max;
   uint32_t seed;
-  uint32_t palette_size;     /* 0: one material per voxel; else shared palette */
+  uint32_t palette_size;     /* 0: one material per voxel; else a shared palette
+                                of this many entries (at most 256), one uint8
+                                entry per voxel */
   float voxel_extent;        /* 0.03125 */
-  uint32_t pad;
+  uint32_t palette_emissive; /* palette mode: entries that emit, with a
+                                radiance in [le_min, le_max] per channel; every
+                                voxel of such an entry is a cube light and
+                                emissive_count is ignored */
 } ct_scene_desc;
 
 typedef struct ct_scene_info {

@@ -98,3 +98,34 @@ def oracle_scene_from_accel(acc, materials, config):
             w2o[i] = acc.world_to_object(i)
     lights = W["lights"][: W["light_count"]]
     return O.OScene(list(inst), w2o, W["aabbs"], W["prims"], list(materials), lights, config=config), inst, W
+
+
+def all_buffers_equal(ctx, ofr, w, f):
+    """Every buffer of frame f of a workload, GPU against the oracle, bit for
+    bit (reservoirs of DI misses excluded: they hold no sample); returns the
+    hit mask."""
+    n = w.width * w.height
+    for which in (ctamd.CT_BUF_DI, ctamd.CT_BUF_PREV_DI):
+        assert np.array_equal(di_array(ctx.read(which, w.width, w.height)), di_array(ofr.read(which))), (f, which)
+    hit = di_array(ofr.read(ctamd.CT_BUF_DI))[:, 3].view(np.float32) > 0
+    for which in (ctamd.CT_BUF_INT_RESERVOIR, ctamd.CT_BUF_PREV_RESERVOIR):
+        g, o = res_array(ctx.read(which, w.width, w.height)), res_array(ofr.read(which))
+        assert np.array_equal(g[hit], o[hit]), (f, which)
+    if w.restir.indirect:
+        for which in (ctamd.CT_BUF_PATH_RESERVOIR, ctamd.CT_BUF_TEMPORAL_PATH_RESERVOIR):
+            assert np.array_equal(pres_array(ctx.read(which, w.width, w.height)), pres_array(ofr.read(which))), \
+                (f, which)
+        assert np.array_equal(bits(ctx.read(ctamd.CT_BUF_INDIRECT, w.width, w.height)),
+                              bits(ofr.read(ctamd.CT_BUF_INDIRECT))), f
+    assert np.array_equal(bits(ctx.read(ctamd.CT_BUF_VELOCITY, w.width, w.height)[hit]),
+                          bits(ofr.read(ctamd.CT_BUF_VELOCITY)[hit])), f
+    if w.taa and f > 0:
+        assert np.array_equal(bits(ctx.read(ctamd.CT_BUF_TAA_FRAME, w.width, w.height)),
+                              bits(ofr.read(ctamd.CT_BUF_TAA_FRAME))), f
+    img = ctx.read(ctamd.CT_BUF_IMAGE, w.width, w.height)
+    assert img.shape == (n, 4)
+    assert np.array_equal(bits(img), bits(ofr.read(ctamd.CT_BUF_IMAGE))), f
+    gs, os_ = ctx.stats(), ofr.stats()
+    assert (gs.primary_rays, gs.shadow_rays, gs.bounce_rays, gs.shadow_rays_pt) == \
+        (os_.primary_rays, os_.shadow_rays, os_.bounce_rays, os_.shadow_rays_pt), f
+    return hit

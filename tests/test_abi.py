@@ -5,6 +5,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 import ctamd
 
@@ -97,3 +98,72 @@ def test_scene_generator_chunks():
         a = sc.aabbs[inst.first_primitive_index: inst.first_primitive_index + inst.primitive_count]
         cell = np.floor((a[:, :3] + 1.5) / 0.03125 / 32 + 1e-6).astype(int)
         assert len({tuple(x) for x in cell}) == 1             # one chunk per instance
+
+
+def _light_voxel(sc, L):
+    inst = sc.instances[L.instance_info.instance_id]
+    assert L.instance_info.primitive_id < inst.primitive_count
+    return inst.first_primitive_index + L.instance_info.primitive_id
+
+
+def test_scene_generator_palette():
+    """The palette workload's scene at 128^3: 256 shared materials, one entry
+    per voxel, 3 emissive entries, and exactly the voxels of those entries
+    are cube lights, in voxel order, with the entry's emission."""
+    w = ctamd.palette_pt(grid=128, chunk=32, width=16, height=16)
+    assert w.palette == (256, 3)
+    sc = ctamd.Scene(ctamd.scene_desc(w))
+    assert sc.info.primitive_count == round(0.12 * 128 ** 3) and sc.info.material_count == 256
+    mats = sc.materials
+    emi = np.array([m.emission[:] for m in mats], np.float32)
+    dif = np.array([m.diffuse[:] for m in mats], np.float32)
+    emissive_entries = np.nonzero(emi.max(axis=1) > 0)[0]
+    assert len(emissive_entries) == 3
+    assert np.all((dif >= 0.2) & (dif <= 0.8)) and np.all((emi[emissive_entries] >= 1) & (emi[emissive_entries] <= 20))
+    entry = sc.primitives[:, 0]
+    assert entry.max() < 256 and len(np.unique(entry)) == 256
+    lit = np.isin(entry, emissive_entries)
+    assert np.array_equal(np.nonzero(lit)[0], np.nonzero(sc.primitives[:, 1] != 0xFFFFFFFF)[0])
+    assert sc.info.cube_light_count == lit.sum() > 1000
+    a = sc.aabbs
+    vox = [_light_voxel(sc, L) for L in sc.lights]
+    assert vox == sorted(vox) and np.array_equal(vox, np.nonzero(lit)[0])
+    for j, L in enumerate(sc.lights):
+        v = vox[j]
+        assert sc.primitives[v, 1] == j
+        assert np.array_equal(np.float32(L.emissive[:]), emi[entry[v]])
+        assert np.allclose(L.position[:], (a[v, :3] + a[v, 3:]) / 2)
+
+
+def test_scene_generator_light_ids_across_chunks():
+    """Per-voxel materials over several chunks: each light names the
+    instance and primitive of its voxel."""
+    w = ctamd.hd_di(width=16, height=16, grid=96, chunk=32)
+    sc = ctamd.Scene(ctamd.scene_desc(w))
+    a = sc.aabbs
+    ids = {L.instance_info.instance_id for L in sc.lights}
+    assert len(ids) > 10
+    for j, L in enumerate(sc.lights):
+        v = _light_voxel(sc, L)
+        assert sc.primitives[v, 1] == j and sc.primitives[v, 0] == v
+        assert np.allclose(L.position[:], (a[v, :3] + a[v, 3:]) / 2)
+
+
+@pytest.mark.parametrize("count", [1, 7, 12 ** 3])
+def test_scene_generator_occupancy_extremes(count):
+    d = ctamd.scene_desc(ctamd.small_plain())
+    d.grid[:] = [12, 12, 12]
+    d.chunk[:] = [5, 5, 5]
+    d.occupied_count, d.emissive_count = count, 1
+    sc = ctamd.Scene(d)
+    assert sc.info.primitive_count == count
+    g = np.round(sc.aabbs[:, :3] / 0.03125).astype(int)
+    assert len({tuple(x) for x in g}) == count
+
+
+def test_scene_generator_rejects_bad_palettes():
+    h = C.c_void_p()
+    for size, emissive in ((257, 3), (4, 5)):
+        d = ctamd.scene_desc(ctamd.small_plain())
+        d.palette_size, d.palette_emissive = size, emissive
+        assert ctamd.lib().ct_scene_generate(C.byref(d), C.byref(h)) == ctamd.CT_E_INVALID and not h.value

@@ -20,41 +20,13 @@ import pytest
 import ctamd
 import oracle as O
 from ctamd import adjoint as AD
-from helpers import bits, di_array, gpu_scene, oracle_scene, pres_array, res_array
+from helpers import all_buffers_equal, bits, di_array, gpu_scene, oracle_scene, pres_array
 
 pytestmark = pytest.mark.gpu
 
 
 def _say(*a):
     print(*a, flush=True)
-
-
-def _all_buffers_equal(ctx, ofr, w, f):
-    n = w.width * w.height
-    for which in (ctamd.CT_BUF_DI, ctamd.CT_BUF_PREV_DI):
-        assert np.array_equal(di_array(ctx.read(which, w.width, w.height)), di_array(ofr.read(which))), (f, which)
-    hit = di_array(ofr.read(ctamd.CT_BUF_DI))[:, 3].view(np.float32) > 0
-    for which in (ctamd.CT_BUF_INT_RESERVOIR, ctamd.CT_BUF_PREV_RESERVOIR):
-        g, o = res_array(ctx.read(which, w.width, w.height)), res_array(ofr.read(which))
-        assert np.array_equal(g[hit], o[hit]), (f, which)
-    if w.restir.indirect:
-        for which in (ctamd.CT_BUF_PATH_RESERVOIR, ctamd.CT_BUF_TEMPORAL_PATH_RESERVOIR):
-            assert np.array_equal(pres_array(ctx.read(which, w.width, w.height)), pres_array(ofr.read(which))), \
-                (f, which)
-        assert np.array_equal(bits(ctx.read(ctamd.CT_BUF_INDIRECT, w.width, w.height)),
-                              bits(ofr.read(ctamd.CT_BUF_INDIRECT))), f
-    assert np.array_equal(bits(ctx.read(ctamd.CT_BUF_VELOCITY, w.width, w.height)[hit]),
-                          bits(ofr.read(ctamd.CT_BUF_VELOCITY)[hit])), f
-    if w.taa and f > 0:
-        assert np.array_equal(bits(ctx.read(ctamd.CT_BUF_TAA_FRAME, w.width, w.height)),
-                              bits(ofr.read(ctamd.CT_BUF_TAA_FRAME))), f
-    img = ctx.read(ctamd.CT_BUF_IMAGE, w.width, w.height)
-    assert img.shape == (n, 4)
-    assert np.array_equal(bits(img), bits(ofr.read(ctamd.CT_BUF_IMAGE))), f
-    gs, os_ = ctx.stats(), ofr.stats()
-    assert (gs.primary_rays, gs.shadow_rays, gs.bounce_rays, gs.shadow_rays_pt) == \
-        (os_.primary_rays, os_.shadow_rays, os_.bounce_rays, os_.shadow_rays_pt), f
-    return hit
 
 
 @pytest.mark.parametrize("reference_mode", [False, True], ids=["workload", "reference"])
@@ -68,7 +40,7 @@ def test_plain_256_four_frames_match_oracle(gpu, reference_mode):
         p = ctamd.frame_params(w, f)
         ctx.render(acc, p)
         ofr.render(p)
-        _all_buffers_equal(ctx, ofr, w, f)
+        all_buffers_equal(ctx, ofr, w, f)
 
 
 def test_hd_1080p_frames_match_oracle(gpu):
@@ -84,7 +56,7 @@ def test_hd_1080p_frames_match_oracle(gpu):
         p = ctamd.frame_params(w, f)
         ctx.render(acc, p)
         ofr.render(p)
-        hit = _all_buffers_equal(ctx, ofr, w, f)
+        hit = all_buffers_equal(ctx, ofr, w, f)
         assert hit.mean() > 0.3
 
 
