@@ -11,8 +11,14 @@ path reuse, shading and TAA.  The scene is resident in HBM before the timed
 region.
 
 Prints one JSON line: Mrays/s (traversal queries per second), frame ms, the
-roofline of the any-hit traversal kernel and the CPU oracle timed on the
-host cores on a bounded band of rows of the same frame.
+roofline of the dominant traversal kernel (and of the next one and of the
+per-pixel passes), HBM bytes per launch measured by rocprofv3 counter runs
+of this build (child processes started before this one touches the GPU),
+and the CPU oracle timed on the host cores on a bounded band of rows of the
+same frame.  Beside it: the 512^3 palette workload (ReSTIR PT with the
+hybrid shift, 4 bounces), the 64-view batched comparison, the
+inverse-rendering step, a .vox scene, plain path tracing, BLAS/TLAS build
+and refit, and GPU voxel deletion.
 """
 import argparse
 import json
@@ -28,22 +34,6 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_trace_hd_di_pt3.md")
-
-
-def _traffic_summary():
-    """Measured HBM bytes per launch of the traversal kernels on the default
-    workload: the `traffic_bytes_per_launch:` line of the counter summary."""
-    try:
-        with open(TRAFFIC_SUMMARY) as fh:
-            for line in fh:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return {k: int(v) for k, v in (kv.split("=") for kv in line.split(":", 1)[1].split())}
-    except OSError:
-        pass
-    return {}
 
 
 def depth_of(w):
@@ -168,29 +158,16 @@ def batched_bench(frames=5, warmup=2, n_views=64):
             "graph": True, "tuned_blocks_per_cu": tune.blocks_per_cu, "tuned_groups": tune.groups}
 
 
-def adjoint_bench(steps=3):
-    """Inverse-rendering step on 128^3 voxels, 8 views of 512^2 at 16 spp,
-    3 bounces: forward render vs forward + path-replay adjoint (after one
-    untimed step; `losses` lists every step)."""
+def adjoint_bench(steps=3, fit_steps=200):
+    """Inverse rendering on 128^3 voxels, 8 views of 512^2 at 16 spp, 3
+    bounces: forward render vs forward + path-replay adjoint + Adam step
+    (after one untimed step), then the 200-step Adam fit from the initial
+    parameters (learning rates ramped over 10 steps) with its loss curve."""
     import torch
-    import ctamd
     from ctamd import adjoint as AD
-    w = AD.AdjointWorkload()
-    sc, alb, emi = AD.build_scene(w)
-    acc = ctamd.Accel(max_instances=sc.info.instance_count + 8,
-                      max_primitives=sc.info.primitive_count + 32 * sc.info.instance_count + 1024,
-                      max_cube_lights=max(sc.info.cube_light_count, 1) + 8)
-    AD._load_geometry(sc, acc)
-    slots = AD.VoxelSlots(sc, acc)
-    extent = w.grid * ctamd.CT_VOXEL_EXTENT
-    views = AD.orbit_views(w.views, w.width, w.height, (0.0, 0.0, 0.0), 1.6 * extent, 0.4 * extent)
-    r = AD.DiffRenderer(acc, views, w.spp, w.max_depth, w.target_seed)
-    dev = torch.device("cuda")
-    target = r.forward(torch.from_numpy(slots.scatter(alb)).to(dev), torch.from_numpy(slots.scatter(emi)).to(dev))
-    a0, e0 = AD.initial_parameters(w, alb, emi)
-    a = torch.from_numpy(slots.scatter(a0)).to(dev).requires_grad_(True)
-    e = torch.from_numpy(slots.scatter(e0)).to(dev).requires_grad_(True)
-    opt = torch.optim.Adam([{"params": [a], "lr": 0.02}, {"params": [e], "lr": 0.5}])
+    P = AD.c4_problem()
+    w, r, target = P["workload"], P["renderer"], P["target"]
+    a, e = P["albedo"], P["emission"]
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for _ in range(steps):
@@ -198,27 +175,26 @@ def adjoint_bench(steps=3):
             r.forward(a, e)
     torch.cuda.synchronize()
     t_fwd = (time.perf_counter() - t1) / steps
-    losses = []
-
-    def adam_step():
-        opt.zero_grad()
-        loss = ((AD.render(r, a, e) - target) ** 2).mean()
-        loss.backward()
-        opt.step()
-        losses.append(float(loss.detach()))
-
-    adam_step()  # untimed: the process's first backward pays one-time autograd and allocator set-up
+    a2 = a.detach().clone().requires_grad_(True)
+    e2 = e.detach().clone().requires_grad_(True)
+    AD.fit(r, target, a2, e2, 1)  # untimed: one-time autograd and allocator set-up
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    for _ in range(steps):
-        adam_step()
+    AD.fit(r, target, a2, e2, steps)
     torch.cuda.synchronize()
     t_step = (time.perf_counter() - t1) / steps
+    t1 = time.perf_counter()
+    hist = AD.fit(r, target, a, e, fit_steps)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t1
     paths = w.views * w.width * w.height * w.spp
-    return {"voxels": sc.info.primitive_count, "views": w.views, "resolution": [w.width, w.height], "spp": w.spp,
-            "bounces": w.max_depth, "paths_per_render": paths, "forward_ms": round(1000 * t_fwd, 2),
+    curve = {str(i): round(hist[i], 6) for i in sorted(set(list(range(0, fit_steps, 20)) + [1, 2, fit_steps - 1]))}
+    return {"voxels": P["scene"].info.primitive_count, "views": w.views, "resolution": [w.width, w.height],
+            "spp": w.spp, "bounces": w.max_depth, "paths_per_render": paths, "forward_ms": round(1000 * t_fwd, 2),
             "adam_step_ms": round(1000 * t_step, 2), "step_over_forward": round(t_step / t_fwd, 2),
-            "forward_mpaths_s": round(paths / t_fwd / 1e6, 1), "losses": [round(x, 6) for x in losses]}
+            "forward_mpaths_s": round(paths / t_fwd / 1e6, 1), "fit_steps": fit_steps, "fit_s": round(fit_s, 2),
+            "fit_loss_curve": curve, "fit_loss_ratio": round(hist[-1] / hist[0], 6),
+            "fit_first_steps_decrease": all(hist[i + 1] < hist[i] for i in range(10))}
 
 
 def vox_bench(frames=8, warmup=2, n=128, chunk=64):
@@ -361,6 +337,287 @@ def update_bench(acc, n_inst, reps=5):
     return res
 
 
+def setup(sc, w):
+    """Accel sized for the scene, a frame context of the workload's size,
+    and the timed BLAS/TLAS build + upload."""
+    import ctamd
+    acc = ctamd.Accel(max_instances=sc.info.instance_count + 8,
+                      max_primitives=sc.info.primitive_count + 32 * sc.info.instance_count + 1024,
+                      max_cube_lights=sc.info.cube_light_count + 8)
+    ctx = ctamd.Ctx(w.width, w.height)
+    t0 = time.perf_counter()
+    ctamd.load_scene(sc, acc, ctx)
+    return acc, ctx, time.perf_counter() - t0
+
+
+# Algorithmic bytes of the per-pixel passes: the pixel buffers each pass
+# must read and write (structure-of-arrays records: DI 64 B, reservoir
+# 36 B, neighbour record 128 B, hit 16 B, shadow-ray record 36 B) plus
+# the scene records it reads per pixel (AABB 24 B, instance 48 B,
+# primitive 8 B, previous transform 64 B, material 32 B, light 48 B per RIS
+# candidate).
+RIS_HIT_BYTES = 16 + 24 + 48 + 8 + 64 + 32 + 36 + 8 + 36 + 64 + 4 + 4
+RIS_MISS_BYTES = 16 + 16 + 16 + 64 + 4
+SPATIAL_HIT_BYTES = (64 + 32 + 36 + 4) + (64 + 32 + 36 + 4 + 16 + 36 + 64)  # emit + finish, own records
+RAY_BYTES = 36
+
+
+def frame_bench(w, acc, ctx, K, Wm, traffic):
+    """K timed frames of workload w (two halves on two streams, as the
+    library renders a large frame) after Wm warm-up frames and one
+    instrumented frame; then K frames on one stream with events around
+    every pass for the per-kernel rooflines."""
+    import ctamd
+    import numpy as np
+    f = 0
+    for _ in range(Wm):
+        ctx.render(acc, ctamd.frame_params(w, f))
+        f += 1
+    ctx.stats()
+    # One instrumented frame (not timed): traversal node/leaf counters.
+    ctx.set_instrumentation(2)
+    ctx.render(acc, ctamd.frame_params(w, f))
+    f += 1
+    ist = ctx.stats()
+    hits = int((np.frombuffer(bytes(ctx.read(ctamd.CT_BUF_DI, w.width, w.height)), np.float32)
+                .reshape(-1, 15)[:, 3] > 0).sum())
+    # Algorithmic bytes per query: a shadow ray reads its 36 B record, a
+    # path segment reads 32 B and writes a 16 B hit; every BVH2 node
+    # fetched is 64 B, every leaf 28 B (centre + id + instance record
+    # share), every instance entry 64 B.
+    n_any = ist.shadow_rays_ris + ist.shadow_rays_temporal + ist.shadow_rays_pt + ist.shadow_rays_spatial
+    bytes_any = 36 * n_any + 64 * ist.nodes + 28 * ist.leaves + 64 * ist.instances
+    bytes_seg = 48 * ist.bounce_rays + 64 * ist.pt_nodes + 28 * ist.pt_leaves + 64 * ist.pt_instances
+    b_any = bytes_any / max(n_any, 1)
+    b_seg = bytes_seg / max(ist.bounce_rays, 1)
+    npix = w.width * w.height
+    n_c = w.restir.max_ris_cube_samples
+    algo_frame = {
+        "k_primary": 16 * npix + 64 * ist.closest_nodes + 28 * ist.closest_leaves + 64 * ist.closest_instances,
+        "k_ris": hits * (RIS_HIT_BYTES + 48 * n_c) + (npix - hits) * RIS_MISS_BYTES
+                 + RAY_BYTES * (ist.shadow_rays_ris + ist.shadow_rays_temporal),
+        "k_spatial": hits * (SPATIAL_HIT_BYTES + 2 * 128 * w.restir.max_neighbors)
+                     + RAY_BYTES * ist.shadow_rays_spatial,
+    }
+    log(f"{w.name}: instrumented frame: {hits} hit pixels, {n_any} shadow rays ({ist.shadow_rays_ris} RIS, "
+        f"{ist.shadow_rays_temporal} temporal, {ist.shadow_rays_pt} path reuse, {ist.shadow_rays_spatial} "
+        f"spatial) {b_any:.0f} B/ray; {ist.bounce_rays} path segments {b_seg:.0f} B/segment; "
+        f"closest: {ist.closest_nodes} nodes")
+    # Timed region: K frames as the library runs them, with only the
+    # frame's start/end events.
+    ctx.set_instrumentation(4)
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipDeviceSynchronize()
+    t_start = time.perf_counter()
+    rays = 0
+    ms_frames = []
+    for _ in range(K):
+        ctx.render(acc, ctamd.frame_params(w, f))
+        f += 1
+        st = ctx.stats()  # drains the stream, reads the frame's counters and events
+        rays += st.primary_rays + st.shadow_rays + st.bounce_rays
+        ms_frames.append(st.ms_total)
+    hip.hipDeviceSynchronize()
+    wall = time.perf_counter() - t_start
+    # Kernel durations for the roofline: K more frames run whole on one
+    # stream with events around every pass (outside the timed region), so
+    # each traversal launch covers the whole frame's rays on its own.
+    ctx.set_instrumentation(1)
+    ms = {"k_trace_any": 0.0, "k_trace_segments": 0.0, "k_primary": 0.0, "k_ris": 0.0, "k_spatial": 0.0}
+    n_any_t = n_seg_t = 0
+    ms_pt, ms_whole = [], []
+    for _ in range(K):
+        ctx.render(acc, ctamd.frame_params(w, f))
+        f += 1
+        st = ctx.stats()
+        ms["k_trace_any"] += st.ms_trace_any
+        ms["k_trace_segments"] += st.ms_pt_segments
+        ms["k_primary"] += st.ms_primary
+        ms["k_ris"] += st.ms_ris
+        ms["k_spatial"] += st.ms_spatial
+        n_any_t += st.shadow_rays_ris + st.shadow_rays_temporal + st.shadow_rays_pt + st.shadow_rays_spatial
+        n_seg_t += st.bounce_rays
+        ms_pt.append(st.ms_pt)
+        ms_whole.append(st.ms_total)
+    ctx.set_instrumentation(0)
+    depth = depth_of(w)
+    algo = {"k_trace_any": b_any * n_any_t / K, "k_trace_segments": b_seg * n_seg_t / K, **algo_frame}
+    # any-hit launches: RIS and temporal visibility, path-reuse visibility
+    # (with paths), spatial visibility
+    launches = {"k_trace_any": 3 if depth else 2,
+                "k_trace_segments": max(depth, 1), "k_primary": 1, "k_ris": 1, "k_spatial": 2}
+    desc = {"k_trace_any": "k_trace_any_frames (shadow-ray any-hit traversal)",
+            "k_trace_segments": "k_trace_segments (path-segment closest-hit traversal)",
+            "k_primary": "k_primary (camera-ray closest-hit traversal)",
+            "k_ris": "k_ris (hit shading, motion vector, RIS candidates, visibility-ray emission)",
+            "k_spatial": "k_spatial (spatial reuse emit + finish, shading)"}
+
+    def roof(k):
+        per_frame_ms = ms[k] / K
+        achieved = algo[k] / (per_frame_ms / 1e3) / 1e9 if per_frame_ms > 0 else 0.0
+        t = traffic.get(k)
+        r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": t,
+             "algorithmic_bytes_per_launch": round(algo[k] / launches[k]),
+             "kernel": desc[k], "ms_per_frame": round(per_frame_ms, 3), "launches_per_frame": launches[k]}
+        if t:
+            r["hbm_gbs_physical"] = round(t * launches[k] / (per_frame_ms / 1e3) / 1e9, 1)
+            r["frac_physical"] = round(r["hbm_gbs_physical"] / HBM_PEAK_GBS, 4)
+        return r
+
+    order = sorted(("k_trace_any", "k_trace_segments"), key=lambda k: -ms[k])
+    med = lambda v: sorted(v)[len(v) // 2]
+    return {"mrays": round(rays / wall / 1e6, 2), "ms_per_step": round(1000.0 * wall / K, 3),
+            "next_frame": f, "roofline": roof(order[0]), "roofline_next_kernel": roof(order[1]),
+            "frame_kernels": {k: roof(k) for k in ("k_primary", "k_ris", "k_spatial")},
+            "config": {"frame_ms_gpu_events_median": round(med(ms_frames), 3),
+                       "frame_ms_one_stream_median": round(med(ms_whole), 3),
+                       "pt_ms_gpu_events_median": round(med(ms_pt), 3), "rays_per_frame": rays // K}}
+
+
+def palette_bench(K=8, Wm=3, traffic=None):
+    """The 512^3 palette workload: 16,106,127 voxels in 512 chunks, a
+    256-entry palette (3 emissive entries, 187,748 cube lights), ReSTIR DI
+    plus ReSTIR PT with the hybrid shift, 4 bounces, M-cap 20, 3 spatial
+    neighbours, 1920x1080, orbiting camera, TAA."""
+    import ctamd
+    w = ctamd.palette_pt(frames=Wm + 2 * K + 2)
+    t0 = time.time()
+    sc = ctamd.Scene(ctamd.scene_desc(w))
+    gen_s = time.time() - t0
+    acc, ctx, load_s = setup(sc, w)
+    log(f"palette: {sc.info.primitive_count} voxels in {sc.info.instance_count} chunks, "
+        f"{sc.info.cube_light_count} lights; generated in {gen_s:.1f}s, built + uploaded in {load_s:.2f}s")
+    r = frame_bench(w, acc, ctx, K, Wm, traffic or {})
+    out = {"workload": f"restir_pt_hybrid4_{w.width}x{w.height}_grid512_palette256_mcap20_n3_taa",
+           "voxels": sc.info.primitive_count, "blas": sc.info.instance_count,
+           "lights": sc.info.cube_light_count, "palette_entries": sc.info.material_count,
+           "frames": K, "mrays_s": r["mrays"], "ms_per_frame": r["ms_per_step"], **r["config"],
+           "load_ms": round(load_s * 1e3, 1), "roofline": r["roofline"],
+           "roofline_next_kernel": r["roofline_next_kernel"], "roofline_frame_kernels": r["frame_kernels"]}
+    ctx.close()
+    acc.close()
+    sc.close()
+    return out
+
+
+def traffic_probe(which, grid, width, height, depth):
+    """Child of a rocprofv3 counter run: a 1 GiB read + 1 GiB write
+    elementwise pass (the counters' calibration), a separator pass, then
+    one warm-up frame and two whole frames on one stream of the
+    workload."""
+    import torch
+    import ctamd
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.mul(x, 2.0, out=y)
+    torch.cuda.synchronize()
+    del x, y
+    if which == "palette":
+        w = ctamd.palette_pt(frames=4)
+    else:
+        w = ctamd.hd_di(frames=4, width=width, height=height, grid=grid, indirect=depth > 0,
+                        max_depth=max(depth, 1))
+    sc = ctamd.Scene(ctamd.scene_desc(w))
+    acc, ctx, _ = setup(sc, w)
+    ctx.render(acc, ctamd.frame_params(w, 0))
+    ctx.stats()
+    s = torch.ones(1 << 10, device="cuda")
+    torch.mul(s, 2.0, out=s)  # separator: the frames counted follow this pass
+    torch.cuda.synchronize()
+    ctx.set_instrumentation(1)
+    for f in (1, 2):
+        ctx.render(acc, ctamd.frame_params(w, f))
+        ctx.stats()
+    print(json.dumps({"probe": which, "frames": 2}), flush=True)
+
+
+TRAFFIC_KERNELS = {"k_trace_any": "k_trace_any_frames", "k_trace_segments": "k_trace_segments",
+                   "k_primary": "k_primary", "k_ris": "k_ris", "k_spatial": "k_spatial"}
+
+
+def _counter_rows(folder):
+    """Per-dispatch counter sums of a rocprofv3 --pmc run, in dispatch
+    order: its CSV output, else its SQLite database."""
+    import csv
+    import glob
+    rows = {}
+    for path in glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                ent = rows.setdefault(int(r["Dispatch_Id"]), {"name": r["Kernel_Name"], "value": 0.0})
+                ent["value"] += float(r["Counter_Value"])
+    if not rows:
+        import sqlite3
+        for path in glob.glob(os.path.join(folder, "**", "*.db"), recursive=True):
+            con = sqlite3.connect(path)
+            try:
+                for did, name, val in con.execute("select dispatch_id, name, counter_value from pmc_events"):
+                    ent = rows.setdefault(int(did), {"name": name, "value": 0.0})
+                    ent["value"] += float(val)
+            finally:
+                con.close()
+    return [rows[k] for k in sorted(rows)]
+
+
+def measure_traffic(probes, argv_extra, timeout_s=240):
+    """HBM bytes per launch of the frame's kernels, measured by this build:
+    per workload, two rocprofv3 counter runs (FETCH_SIZE, then WRITE_SIZE)
+    over a child process of this script.  Each counter is scaled by what it
+    reads for the child's calibration pass (1 GiB read and 1 GiB written),
+    and averaged over the launches of the two frames after the separator.
+    Runs before this process touches the GPU."""
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return {"note": "rocprofv3 not found: traffic not measured"}
+    out = {}
+    notes = []
+    tmp = tempfile.mkdtemp(prefix="ct_traffic_", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+    for which in probes:
+        res = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            folder = os.path.join(tmp, f"{which}_{counter}")
+            cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", folder, "-o", "c", "--", sys.executable, os.path.abspath(__file__),
+                   "--traffic-probe", which] + argv_extra
+            t0 = time.time()
+            p = None
+            try:
+                p = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                   timeout=timeout_s, start_new_session=True)
+                ok = p.returncode == 0 or b'"probe"' in p.stdout
+            except subprocess.TimeoutExpired:
+                ok = False
+            rows = _counter_rows(folder) if ok else []
+            log(f"traffic: {which} {counter}: {len(rows)} dispatches ({time.time() - t0:.1f}s)")
+            if not rows:
+                if ok or p is not None:
+                    log("traffic: counter run output:\n" + p.stdout.decode(errors="replace")[-3000:])
+                notes.append(f"{which} {counter}: counter run failed")
+                break
+            sep = max(i for i, r in enumerate(rows) if "elementwise" in r["name"])
+            calib = max(r["value"] for r in rows[:sep] if "elementwise" in r["name"])
+            scale = float(1 << 30) / (calib * 1024.0)
+            for k, pat in TRAFFIC_KERNELS.items():
+                vals = [r["value"] for r in rows[sep + 1:] if pat in r["name"]]
+                if vals:
+                    res.setdefault(k, 0.0)
+                    res[k] += scale * 1024.0 * sum(vals) / len(vals)
+            res[f"scale_{counter}"] = round(scale, 4)
+        else:
+            out[which] = {k: (int(v) if k.startswith("k_") else v) for k, v in res.items()}
+    shutil.rmtree(tmp, ignore_errors=True)
+    out["note"] = ("HBM bytes per launch = FETCH_SIZE x scale + WRITE_SIZE x scale (KiB x 1024), each from its "
+                   "own rocprofv3 --pmc run of this build over two whole frames on one stream; scale = 1 GiB / "
+                   "the counter's reading for a 1 GiB elementwise pass in the same run"
+                   + ("; " + "; ".join(notes) if notes else ""))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -377,9 +634,16 @@ def main():
     ap.add_argument("--no-vox", action="store_true",
                     help="skip the .vox scene and material path-tracing timings")
     ap.add_argument("--no-brush", action="store_true", help="skip the GPU voxel-deletion timing")
+    ap.add_argument("--no-palette", action="store_true", help="skip the 512^3 palette workload")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter runs that measure HBM traffic per launch")
+    ap.add_argument("--traffic-probe", choices=("hd", "palette"), help=argparse.SUPPRESS)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (rocprof counter calibration)")
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe(args.traffic_probe, args.grid, args.width, args.height, args.depth)
+        return
     if args.calibrate_copy:
         import torch
         x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
@@ -393,6 +657,14 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the per-frame path is single-GPU "
               f"and multi-GPU sharding is not part of this build", file=sys.stderr)
         sys.exit(2)
+    traffic = {}
+    if not args.no_traffic:
+        # counter runs first: each is a child process, started before this
+        # process initialises the GPU
+        probes = ["hd"] + ([] if args.no_palette else ["palette"])
+        traffic = measure_traffic(probes, ["--grid", str(args.grid), "--width", str(args.width),
+                                           "--height", str(args.height), "--depth", str(args.depth)])
+        log(f"traffic: {traffic}")
     import ctamd
     if not ctamd.device_available():
         raise SystemExit("bench: no gfx950 device visible")
@@ -403,94 +675,14 @@ def main():
     sc = ctamd.Scene(ctamd.scene_desc(w))
     log(f"scene: {sc.info.instance_count} instances, {sc.info.primitive_count} voxels, "
         f"{sc.info.cube_light_count} lights ({time.time() - t0:.1f}s)")
-    acc = ctamd.Accel(max_instances=sc.info.instance_count + 8,
-                      max_primitives=sc.info.primitive_count + 32 * sc.info.instance_count + 1024,
-                      max_cube_lights=sc.info.cube_light_count + 8)
-    ctx = ctamd.Ctx(w.width, w.height)
-    t0 = time.perf_counter()
-    ctamd.load_scene(sc, acc, ctx)
-    load_s = time.perf_counter() - t0
+    acc, ctx, load_s = setup(sc, w)
     log(f"BLAS/TLAS build + upload: {load_s:.3f}s")
-    f = 0
-    for _ in range(Wm):
-        ctx.render(acc, ctamd.frame_params(w, f))
-        f += 1
-    ctx.stats()
-    # One instrumented frame (not timed): traversal node/leaf counters.
-    ctx.set_instrumentation(2)
-    ctx.render(acc, ctamd.frame_params(w, f))
-    f += 1
-    ist = ctx.stats()
-    # Algorithmic bytes per query: a shadow ray reads its
-    # 36 B record, a path segment reads 32 B and writes a 16 B hit; every BVH2
-    # node fetched is 64 B, every leaf 28 B (centre + id + instance record
-    # share), every instance entry 64 B.
-    n_any = ist.shadow_rays_ris + ist.shadow_rays_temporal + ist.shadow_rays_pt + ist.shadow_rays_spatial
-    bytes_any = 36 * n_any + 64 * ist.nodes + 28 * ist.leaves + 64 * ist.instances
-    bytes_seg = 48 * ist.bounce_rays + 64 * ist.pt_nodes + 28 * ist.pt_leaves + 64 * ist.pt_instances
-    b_any = bytes_any / max(n_any, 1)
-    b_seg = bytes_seg / max(ist.bounce_rays, 1)
-    log(f"instrumented frame: {n_any} shadow rays ({ist.shadow_rays_ris} RIS, {ist.shadow_rays_temporal} temporal, "
-        f"{ist.shadow_rays_pt} path reuse, {ist.shadow_rays_spatial} spatial) {b_any:.0f} B/ray; "
-        f"{ist.bounce_rays} path segments {b_seg:.0f} B/segment; closest: {ist.closest_nodes} nodes")
-    # Timed region: K frames as the library runs them (two halves of the
-    # image on two streams), with only the frame's start/end events.
-    ctx.set_instrumentation(4)
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    hip.hipDeviceSynchronize()
-    t_start = time.perf_counter()
-    rays = 0
-    ms_frames = []
-    for _ in range(K):
-        ctx.render(acc, ctamd.frame_params(w, f))
-        f += 1
-        st = ctx.stats()  # drains the stream, reads the frame's counters and events
-        rays += st.primary_rays + st.shadow_rays + st.bounce_rays
-        ms_frames.append(st.ms_total)
-    hip.hipDeviceSynchronize()
-    wall = time.perf_counter() - t_start
-    ms_per_step = 1000.0 * wall / K
-    mrays = rays / wall / 1e6
-    # Kernel durations for the roofline: K more frames run whole on one
-    # stream with events around every pass (outside the timed region), so
-    # each traversal launch covers the whole frame's rays on its own.
-    ctx.set_instrumentation(1)
-    ms_any = ms_seg = 0.0
-    n_any_t = n_seg_t = 0
-    ms_pt, ms_whole = [], []
-    for _ in range(K):
-        ctx.render(acc, ctamd.frame_params(w, f))
-        f += 1
-        st = ctx.stats()
-        ms_any += st.ms_trace_any
-        ms_seg += st.ms_pt_segments
-        n_any_t += st.shadow_rays_ris + st.shadow_rays_temporal + st.shadow_rays_pt + st.shadow_rays_spatial
-        n_seg_t += st.bounce_rays
-        ms_pt.append(st.ms_pt)
-        ms_whole.append(st.ms_total)
-    kern = {
-        "k_trace_any": {"desc": "k_trace_any (shadow-ray any-hit traversal)", "ms": ms_any,
-                        "achieved": b_any * n_any_t / (ms_any / 1000.0) / 1e9 if ms_any > 0 else 0.0},
-        "k_trace_segments": {"desc": "k_trace_segments (path-segment closest-hit traversal)", "ms": ms_seg,
-                             "achieved": b_seg * n_seg_t / (ms_seg / 1000.0) / 1e9 if ms_seg > 0 else 0.0},
-    }
-    order = sorted(kern, key=lambda k: -kern[k]["ms"])
-    # HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE counter
-    # runs of this workload (rocprofv3 cannot run inside the timed process);
-    # the algorithmic bytes per launch of this run are reported beside them.
-    measured = _traffic_summary() if (args.width, args.height, args.grid, args.depth) == (1920, 1080, 256, 3) else {}
-    # any-hit launches per frame: RIS and temporal visibility, path-reuse visibility (with paths), spatial
-    launches = {"k_trace_any": 3 if depth_of(w) else 2, "k_trace_segments": max(depth_of(w), 1)}
-    algo = {"k_trace_any": b_any * n_any_t / K, "k_trace_segments": b_seg * n_seg_t / K}
-
-    def roof(k):
-        v = kern[k]
-        return {"bound": "hbm", "achieved": round(v["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(v["achieved"] / HBM_PEAK_GBS, 4), "traffic": measured.get(k),
-                "algorithmic_bytes_per_launch": round(algo[k] / launches[k]),
-                "kernel": v["desc"], "ms_per_frame": round(v["ms"] / K, 3)}
-
+    hd = frame_bench(w, acc, ctx, K, Wm, traffic.get("hd", {}))
+    f = hd["next_frame"]
+    palette = None
+    if not args.no_palette:
+        palette = palette_bench(traffic=traffic.get("palette"))
+        log(f"palette: {palette}")
     batched = adjoint = None
     if not args.no_batched:
         batched = batched_bench()
@@ -518,23 +710,22 @@ def main():
     if not args.no_brush:
         brush = brush_bench(ctx, acc, w, f)
         log(f"brush: {brush}")
-    ms_sorted = sorted(ms_frames)
-    depth = w.restir.max_depth if w.restir.indirect else 0
+    depth = depth_of(w)
     out = {
         "metric": "Mrays/s + 1080p frame ms (ReSTIR DI/PT), 1 MI355X; batched/adjoint vs plain",
-        "value": round(mrays, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": K, "warmup": Wm,
-        "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+        "value": hd["mrays"], "unit": "Mrays/s", "n_gpus": 1, "steps": K, "warmup": Wm,
+        "ms_per_step": hd["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"restir_di_pt{depth}_{w.width}x{w.height}_grid{args.grid}_64blas_taa",
                    "width": w.width, "height": w.height, "grid": args.grid, "blas": sc.info.instance_count,
                    "voxels": sc.info.primitive_count, "lights": sc.info.cube_light_count,
-                   "frame_ms_gpu_events_median": round(ms_sorted[len(ms_sorted) // 2], 3),
-                   "frame_ms_one_stream_median": round(sorted(ms_whole)[len(ms_whole) // 2], 3),
-                   "pt_ms_gpu_events_median": round(sorted(ms_pt)[len(ms_pt) // 2], 3),
-                   "rays_per_frame": rays // K, "indirect_bounces": depth},
-        "roofline": roof(order[0]),
-        "roofline_next_kernel": roof(order[1]),
+                   **hd["config"], "indirect_bounces": depth},
+        "roofline": hd["roofline"],
+        "roofline_next_kernel": hd["roofline_next_kernel"],
+        "roofline_frame_kernels": hd["frame_kernels"],
+        "traffic_measurement": traffic.get("note"),
         "cpu_baseline": cpu,
+        "palette": palette,
         "batched": batched,
         "adjoint": adjoint,
         "build": build,

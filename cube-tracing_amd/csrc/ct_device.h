@@ -289,12 +289,28 @@ struct Stack {
   }
 };
 
+// A load through a pointer known to address device (global) memory: a
+// global load instead of a flat one, which would also count against the
+// LDS wait counter that the traversal stack uses.
+typedef float gf4 __attribute__((ext_vector_type(4)));
+typedef uint32_t gu4 __attribute__((ext_vector_type(4)));
+CT_D float gld(const float *p) { return *(const __attribute__((address_space(1))) float *)p; }
+CT_D uint32_t gld(const uint32_t *p) { return *(const __attribute__((address_space(1))) uint32_t *)p; }
+CT_D float4 gld(const float4 *p) {
+  const gf4 v = *(const __attribute__((address_space(1))) gf4 *)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+CT_D uint4 gld(const uint4 *p) {
+  const gu4 v = *(const __attribute__((address_space(1))) gu4 *)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // Test both children of a node; returns the next ref (or kInvalid to pop)
 // and pushes the far child when both are hit (near first, ties to child 0).
 CT_D uint32_t visit_node(const DNode *nodes, uint32_t idx, F3 o, F3 inv, float limit, Stack &stk, bool &overflow) {
   const float4 *p = reinterpret_cast<const float4 *>(nodes + idx);
-  float4 a = p[0], b = p[1], c = p[2];
-  uint4 d = reinterpret_cast<const uint4 *>(nodes + idx)[3];
+  float4 a = gld(p), b = gld(p + 1), c = gld(p + 2);
+  uint4 d = gld(reinterpret_cast<const uint4 *>(nodes + idx) + 3);
   float tn0, tn1;
   bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, limit, tn0);
   bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, limit, tn1);
@@ -323,14 +339,20 @@ constexpr int kTraceAny = 0, kTraceClosest = 1, kTraceBounce = 2;
 
 // Two-level traversal (TLAS in world space, each BLAS in object space) as
 // a resumable walker: start() takes a ray, each step() visits one TLAS
-// node, instance, BLAS node or voxel leaf, and `done` is set once the ray
-// is finished (for an any-hit query `occluded` tells the outcome).  A
-// kernel can thus refill a lane whose ray finished while the rest of the
-// wave keeps walking.  `stk` is the lane's LDS stack column.
+// node, BLAS node or voxel leaf (a step that reaches an instance also
+// enters it and visits the BLAS root), and `done` is set once the ray is
+// finished (for an any-hit query `occluded` tells the outcome).  `stk` is
+// the lane's LDS stack column.
+// The ray a node test uses (cur_o, cur_inv) is the world ray in the TLAS
+// and the object-space ray in a BLAS; it is switched on entering and on
+// leaving an instance.  Instance entry comes first in a step and the
+// stack update last, so in a wave whose lanes take different branches each
+// state register is written after every read of its old value: the
+// branches' results merge without register copies.
 template <int kMode, bool kCount>
 struct Walker {
   static constexpr bool kAny = kMode == kTraceAny;
-  F3 o, d, winv, oo, od, oinv, dr, sg, sinv;
+  F3 o, d, winv, cur_o, cur_inv, dr, sg, sinv;
   float limit, best_t;
   uint32_t ti, tp, best_i, best_p, first, inst, ref, steps;
   int base;
@@ -346,7 +368,7 @@ struct Walker {
     ti = ti_;
     tp = tp_;
     winv = f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    oo = o; od = d; oinv = winv; dr = d; sg = d; sinv = d;
+    cur_o = o; cur_inv = winv; dr = d; sg = d; sinv = d;
     best_t = limit_;
     best_i = kInvalid;
     best_p = kInvalid;
@@ -372,29 +394,21 @@ struct Walker {
       done = true;
       return;
     }
-    uint32_t next = kInvalid;
-    if (!(ref & kLeaf)) {
-      // TLAS and BLAS nodes take the same code (one pass for a wave whose
-      // lanes are at both levels).
-      if (kCount) cnt.nodes++;
-      const DNode *nodes = in_blas ? blas_nodes + first : tlas_nodes;
-      next = visit_node(nodes, ref, in_blas ? oo : o, in_blas ? oinv : winv, kAny ? limit : best_t, st, overflow);
-    } else if (!in_blas) {
-      inst = S.tlas_inst[ref & ~kLeaf];
+    if ((ref & kLeaf) && !in_blas) {
+      // Enter the instance: the object-space ray, then its BLAS root below.
+      inst = gld(S.tlas_inst + (ref & ~kLeaf));
       if (kCount) cnt.insts++;
-      const DInst &I = S.inst[inst];
+      const DInst *I = S.inst + inst;
       float m[12];
 #pragma unroll
-      for (int k = 0; k < 12; ++k) m[k] = I.w2o[k];
-      oo = xform3(m, o, 1.0f);
-      od = xform3(m, d, 0.0f);
+      for (int k = 0; k < 12; ++k) m[k] = gld(I->w2o + k);
+      const F3 oo = xform3(m, o, 1.0f), od = xform3(m, d, 0.0f);
       // An instance without rotation or scale gives back the direction's
       // own bits (its zero terms add signed zeros), and then 1 / od is winv:
       // the three divisions run only for lanes whose direction changed.
-      if (__float_as_uint(od.x) == __float_as_uint(d.x) && __float_as_uint(od.y) == __float_as_uint(d.y) &&
-          __float_as_uint(od.z) == __float_as_uint(d.z))
-        oinv = winv;
-      else
+      F3 oinv = winv;
+      if (__float_as_uint(od.x) != __float_as_uint(d.x) || __float_as_uint(od.y) != __float_as_uint(d.y) ||
+          __float_as_uint(od.z) != __float_as_uint(d.z))
         oinv = f3(1.0f / od.x, 1.0f / od.y, 1.0f / od.z);
       if (kMode != kTraceClosest) {
         dr = od;
@@ -404,54 +418,61 @@ struct Walker {
         dr = mul_rot(od);
       }
       sg = f3(-gsign(dr.x), -gsign(dr.y), -gsign(dr.z));
-      first = I.first;
+      cur_o = oo;
+      cur_inv = oinv;
+      first = gld(&I->first);
       in_blas = true;
       base = st.n;
-      ref = I.root;
-      return;
+      ref = gld(&I->root);
+    }
+    uint32_t next = kInvalid;
+    bool finished = false;
+    if (!(ref & kLeaf)) {
+      // TLAS and BLAS nodes take the same code.
+      if (kCount) cnt.nodes++;
+      const DNode *nodes = in_blas ? blas_nodes + first : tlas_nodes;
+      next = visit_node(nodes, ref, cur_o, cur_inv, kAny ? limit : best_t, st, overflow);
     } else {
-      {
-        if (kCount) cnt.leaves++;
-        const float4 L = S.blas_leaves[first + (ref & ~kLeaf)];
-        const uint32_t pid = __float_as_uint(L.w);
-        float t;
-        if (kAny) {
-          const bool hit = intersect_box<false, false>(f3(L.x, L.y, L.z), oo, dr, sg, sinv, t);
-          if (hit && (t >= 0.0f) && (t <= limit) && !(inst == ti && pid == tp)) {
-            occluded = true;
-            done = true;
-            return;
-          }
-        } else {
-          const bool hit = (kMode == kTraceClosest)
-                               ? intersect_box<true, true>(f3(L.x, L.y, L.z), oo, dr, sg, sinv, t)
-                               : intersect_box<true, false>(f3(L.x, L.y, L.z), oo, dr, sg, sinv, t);
-          if (hit && ((kMode == kTraceClosest) ? (t > 0.0f) : (t >= 0.0f))) {
-            const bool better =
-                (t < best_t) || (t == best_t && ((inst < best_i) || (inst == best_i && pid < best_p)));
-            if (better) {
-              best_t = t;
-              best_i = inst;
-              best_p = pid;
-            }
+      if (kCount) cnt.leaves++;
+      const float4 L = gld(S.blas_leaves + first + (ref & ~kLeaf));
+      const uint32_t pid = __float_as_uint(L.w);
+      float t;
+      if (kAny) {
+        const bool hit = intersect_box<false, false>(f3(L.x, L.y, L.z), cur_o, dr, sg, sinv, t);
+        if (hit && (t >= 0.0f) && (t <= limit) && !(inst == ti && pid == tp)) {
+          occluded = true;
+          finished = true;
+        }
+      } else {
+        const bool hit = (kMode == kTraceClosest)
+                             ? intersect_box<true, true>(f3(L.x, L.y, L.z), cur_o, dr, sg, sinv, t)
+                             : intersect_box<true, false>(f3(L.x, L.y, L.z), cur_o, dr, sg, sinv, t);
+        if (hit && ((kMode == kTraceClosest) ? (t > 0.0f) : (t >= 0.0f))) {
+          const bool better =
+              (t < best_t) || (t == best_t && ((inst < best_i) || (inst == best_i && pid < best_p)));
+          if (better) {
+            best_t = t;
+            best_i = inst;
+            best_p = pid;
           }
         }
       }
     }
-    if (overflow) {
+    if (finished || overflow) {
       done = true;
-      return;
-    }
-    if (next != kInvalid) {
+    } else if (next != kInvalid) {
       ref = next;
-      return;
+    } else {
+      if (in_blas && st.n == base) {
+        in_blas = false;
+        cur_o = o;
+        cur_inv = winv;
+      }
+      if (st.n == 0)
+        done = true;
+      else
+        ref = st.pop();
     }
-    if (in_blas && st.n == base) in_blas = false;
-    if (st.n == 0) {
-      done = true;
-      return;
-    }
-    ref = st.pop();
   }
 };
 
