@@ -17,7 +17,7 @@ import numpy as np
 
 from . import (AdjointDesc, AdjointView, Accel, Scene, SceneDesc, CT_VOXEL_EXTENT, camera_matrices, check, lib)
 
-__all__ = ["AdjointWorkload", "inverse_rendering_c4", "orbit_views", "VoxelSlots", "DiffRenderer", "render",
+__all__ = ["AdjointWorkload", "inverse_rendering_c4", "c4_problem", "fit", "orbit_views", "VoxelSlots", "DiffRenderer", "render",
            "build_scene"]
 
 
@@ -193,9 +193,37 @@ def initial_parameters(w: AdjointWorkload, hidden_albedo, hidden_emission):
     return alb, emi
 
 
-def inverse_rendering_c4(steps=200, w: AdjointWorkload | None = None, lr_albedo=0.02, lr_emission=0.5, log=None):
-    """Fits albedo and emission to target images rendered from the hidden
-    parameters with an L2 loss and Adam; returns the loss history."""
+def fit(r, target, albedo, emission, steps, lr_albedo=0.02, lr_emission=0.5, warmup=10, log=None):
+    """Adam on (albedo, emission) against `target` with an L2 loss; the
+    learning rates ramp linearly over the first `warmup` steps (without the
+    ramp, emission's first full-size step overshoots and the loss rises).
+    Albedo is clamped to [0, 1] and emission to >= 0 after each step.
+    Returns the loss of every step."""
+    import torch
+    opt = torch.optim.Adam([{"params": [albedo], "lr": lr_albedo}, {"params": [emission], "lr": lr_emission}])
+    hist = []
+    for it in range(steps):
+        s = min(1.0, (it + 1) / warmup) if warmup else 1.0
+        opt.param_groups[0]["lr"] = lr_albedo * s
+        opt.param_groups[1]["lr"] = lr_emission * s
+        opt.zero_grad()
+        img = render(r, albedo, emission)
+        loss = ((img - target) ** 2).mean()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            albedo.clamp_(0.0, 1.0)
+            emission.clamp_(min=0.0)
+        hist.append(float(loss.detach()))
+        if log and (it % 20 == 0 or it == steps - 1):
+            log(f"step {it}: loss {hist[-1]:.6g}")
+    return hist
+
+
+def c4_problem(w: AdjointWorkload | None = None):
+    """The inverse-rendering workload set up on the GPU: the renderer over
+    its views, the target images of the hidden parameters, and the initial
+    parameters as leaf tensors (per voxel slot)."""
     import torch
     w = w or AdjointWorkload()
     sc, alb_t, emi_t = build_scene(w)
@@ -212,21 +240,16 @@ def inverse_rendering_c4(steps=200, w: AdjointWorkload | None = None, lr_albedo=
     a0, e0 = initial_parameters(w, alb_t, emi_t)
     albedo = torch.from_numpy(slots.scatter(a0)).to(dev).requires_grad_(True)
     emission = torch.from_numpy(slots.scatter(e0)).to(dev).requires_grad_(True)
-    opt = torch.optim.Adam([{"params": [albedo], "lr": lr_albedo}, {"params": [emission], "lr": lr_emission}])
-    hist = []
-    for it in range(steps):
-        opt.zero_grad()
-        img = render(r, albedo, emission)
-        loss = ((img - target) ** 2).mean()
-        loss.backward()
-        opt.step()
-        with torch.no_grad():
-            albedo.clamp_(0.0, 1.0)
-            emission.clamp_(min=0.0)
-        hist.append(float(loss.detach()))
-        if log and (it % 20 == 0 or it == steps - 1):
-            log(f"step {it}: loss {hist[-1]:.6g}")
-    return hist
+    return {"workload": w, "scene": sc, "accel": acc, "slots": slots, "renderer": r, "target": target,
+            "albedo": albedo, "emission": emission, "hidden": (alb_t, emi_t)}
+
+
+def inverse_rendering_c4(steps=200, w: AdjointWorkload | None = None, lr_albedo=0.02, lr_emission=0.5, warmup=10,
+                         log=None):
+    """Fits albedo and emission to target images rendered from the hidden
+    parameters with an L2 loss and Adam; returns the loss history."""
+    P = c4_problem(w)
+    return fit(P["renderer"], P["target"], P["albedo"], P["emission"], steps, lr_albedo, lr_emission, warmup, log)
 
 
 def _load_geometry(sc: Scene, acc: Accel):

@@ -173,3 +173,14 @@ def test_inverse_rendering_workload_gradients_match_finite_differences(gpu):
         # central difference's own resolution on this loss
         assert abs(fd - grad) <= 1e-3 * abs(grad) + 2e-6 * gmax, (kind, i, fd, float(grad))
     assert len(checks) == 64
+
+
+def test_inverse_rendering_workload_200_adam_steps(gpu):
+    """The 200-step fit of the inverse-rendering workload: the loss falls at
+    every one of the first 10 steps (learning rates ramped over them) and
+    ends below 1 % of where it starts."""
+    _say("adjoint: 200 Adam steps")
+    hist = AD.inverse_rendering_c4(steps=200, log=_say)
+    assert len(hist) == 200
+    assert all(hist[i + 1] < hist[i] for i in range(10)), hist[:11]
+    assert hist[-1] < 0.01 * hist[0], (hist[0], hist[-1])
