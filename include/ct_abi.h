@@ -498,7 +498,14 @@ typedef struct ct_restir_params {
   uint32_t pad;
 } ct_restir_params;
 #define CT_SHIFT_RECONNECTION 0u  /* SHIFT_MAPPING_RECONNECTION, defines.glsl:85 */
-#define CT_SHIFT_HYBRID 2u        /* SHIFT_MAPPING_HYBRID, defines.glsl:87 */
+#define CT_SHIFT_HYBRID 2u        /* SHIFT_MAPPING_HYBRID, defines.glsl:87.  The path
+                                     builder marks the first connectable vertex (far field,
+                                     rough after rough); a shift replays the source's seed
+                                     from the destination's primary hit to the vertex before
+                                     the reconnection vertex (no vertex before it connectable,
+                                     that one rough, else rejected), then reconnects with the
+                                     cached Jacobian, its replayed prefix counting 1;
+                                     near-field connections are rejected. */
 #define CT_MAX_PT_NEIGHBORS 8u
 #define CT_MAX_PT_DEPTH 14u       /* path lengths are stored in 4 bits */
 
