@@ -82,6 +82,50 @@ def cpu_baseline(w, sc, acc, budget_s):
                       f"total ({total_rays} traversal queries in {total_t:.1f}s, OpenMP {cores} threads)"}
 
 
+def c0_bench(sets=5):
+    """The 64^3 plain workload (31,457 voxels, 8 lights, 256x256, ReSTIR DI
+    + 1 bounce, frames 0-3): GPU frame time by events over `sets` runs of
+    the four frames (context reset between runs), and the oracle on the
+    host cores rendering the same four frames once."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ctamd
+    import oracle as O
+    w = ctamd.small_plain()
+    sc = ctamd.Scene(ctamd.scene_desc(w))
+    acc, ctx, _ = setup(sc, w)
+    params = [ctamd.frame_params(w, f) for f in range(4)]
+    ms, rays = [], 0
+    for k in range(sets + 1):
+        ctx.reset()
+        ctx.set_instrumentation(4)
+        for p in params:
+            ctx.render(acc, p)
+            st = ctx.stats()
+            if k > 0:  # the first set is a warm-up
+                ms.append(st.ms_total)
+                rays += st.primary_rays + st.shadow_rays + st.bounce_rays
+    ctx.set_instrumentation(0)
+    gpu_ms = sum(ms) / len(ms)
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    fr = O.OFrame(O.OScene.from_scene(sc), w.width, w.height)
+    t0 = time.perf_counter()
+    crays = 0
+    for p in params:
+        fr.render(p, threads=cores)
+        st = fr.stats()
+        crays += st.primary_rays + st.shadow_rays + st.bounce_rays
+    cpu_s = time.perf_counter() - t0
+    out = {"workload": "small_plain_64^3_256x256_di_pt1_4frames", "voxels": sc.info.primitive_count,
+           "gpu_frame_ms_mean": round(gpu_ms, 4), "gpu_mrays_s": round(rays / (sum(ms) / 1e3) / 1e6, 1),
+           "cpu_frame_ms_mean": round(cpu_s * 1e3 / 4, 2), "cpu_mrays_s": round(crays / cpu_s / 1e6, 2),
+           "cpu_cores": cores, "cpu_kind": "port", "rays_per_frame": crays // 4,
+           "gpu_over_cpu": round((cpu_s * 1e3 / 4) / gpu_ms, 1)}
+    ctx.close()
+    acc.close()
+    sc.close()
+    return out
+
+
 def batched_bench(frames=5, warmup=2, n_views=64):
     """Batched views (4 scenes, 64 views of uneven size): per-view
     ct_frame_render against one batched, graph-replayed render."""
@@ -635,6 +679,7 @@ def main():
                     help="skip the .vox scene and material path-tracing timings")
     ap.add_argument("--no-brush", action="store_true", help="skip the GPU voxel-deletion timing")
     ap.add_argument("--no-palette", action="store_true", help="skip the 512^3 palette workload")
+    ap.add_argument("--no-c0", action="store_true", help="skip the 64^3 plain workload (GPU and oracle)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter runs that measure HBM traffic per launch")
     ap.add_argument("--traffic-probe", choices=("hd", "palette"), help=argparse.SUPPRESS)
@@ -683,6 +728,10 @@ def main():
     if not args.no_palette:
         palette = palette_bench(traffic=traffic.get("palette"))
         log(f"palette: {palette}")
+    c0 = None
+    if not args.no_c0:
+        c0 = c0_bench()
+        log(f"c0: {c0}")
     batched = adjoint = None
     if not args.no_batched:
         batched = batched_bench()
@@ -726,6 +775,7 @@ def main():
         "traffic_measurement": traffic.get("note"),
         "cpu_baseline": cpu,
         "palette": palette,
+        "c0": c0,
         "batched": batched,
         "adjoint": adjoint,
         "build": build,
