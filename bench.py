@@ -160,8 +160,9 @@ def batched_bench(frames=5, warmup=2, n_views=64):
     # plain: one ct_frame_render per view and frame, issued back to back (the
     # views' own streams overlap on the GPU), then the same with each view
     # finished before the next starts, as a renderer drawing one view per
-    # frame does (draw(), main.cpp:1436).  Both loops start from the same
-    # history: the serial one renders the same frame numbers on copies.
+    # frame does (draw(), main.cpp:1436), then the same renders enqueued on
+    # one stream.  Every loop starts from the same history: the later ones
+    # render the same frame numbers on copies.
     t_plain, t_serial, t_batch, rays = 0.0, 0.0, 0.0, 0
     for f in range(warmup + frames):
         hip.hipDeviceSynchronize()
@@ -184,6 +185,27 @@ def batched_bench(frames=5, warmup=2, n_views=64):
             hip.hipDeviceSynchronize()
         if f >= warmup:
             t_serial += time.perf_counter() - t1
+    # plain on one stream: the 64 renders enqueued one after another on a
+    # single stream with no host synchronisation between them (each view's
+    # launches follow the previous view's, as one command queue runs them)
+    import torch
+    one = torch.cuda.Stream()
+    ctx_one = []
+    for k, w in views:
+        c = ctamd.Ctx(w.width, w.height)
+        ctamd.bind_scene(built[k][0], c)
+        ctx_one.append(c)
+    t_one = 0.0
+    for f in range(warmup + frames):
+        hip.hipDeviceSynchronize()
+        t1 = time.perf_counter()
+        for (k, w), c, p in zip(views, ctx_one, params[f]):
+            c.render(built[k][1], p, stream=one.cuda_stream)
+        hip.hipDeviceSynchronize()
+        if f >= warmup:
+            t_one += time.perf_counter() - t1
+    for c in ctx_one:
+        c.close()
     for f in range(warmup + frames):
         hip.hipDeviceSynchronize()
         t1 = time.perf_counter()
@@ -197,6 +219,8 @@ def batched_bench(frames=5, warmup=2, n_views=64):
             "indirect_bounces": views[0][1].restir.max_depth, "frames": frames,
             "plain_ms_per_frame": round(1000 * t_plain / frames, 3),
             "plain_serial_ms_per_frame": round(1000 * t_serial / frames, 3),
+            "plain_one_stream_ms_per_frame": round(1000 * t_one / frames, 3),
+            "speedup_vs_one_stream": round(t_one / t_batch, 2),
             "batched_ms_per_frame": round(1000 * t_batch / frames, 3),
             "speedup": round(t_plain / t_batch, 2), "speedup_vs_serial": round(t_serial / t_batch, 2), "batched_mrays_s": round(rays / t_batch / 1e6, 1),
             "graph": True, "tuned_blocks_per_cu": tune.blocks_per_cu, "tuned_groups": tune.groups}

@@ -730,8 +730,10 @@ class Ctx:
     def set_instrumentation(self, flags: int):
         check(lib().ct_ctx_set_instrumentation(self.handle, flags))
 
-    def render(self, accel: Accel, params: FrameParams):
-        check(lib().ct_frame_render(self.handle, accel.handle, C.byref(params), None))
+    def render(self, accel: Accel, params: FrameParams, stream=None):
+        """Renders one frame on `stream` (a hipStream_t as an int; None: the
+        context's own stream)."""
+        check(lib().ct_frame_render(self.handle, accel.handle, C.byref(params), C.c_void_p(stream) if stream else None))
 
     def stats(self) -> FrameStats:
         s = FrameStats()
