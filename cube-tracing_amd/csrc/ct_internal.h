@@ -231,6 +231,7 @@ struct FrameBufs {
   DevBuf ray_o, ray_d, ray_t, queue, counters, counters_host_staging;
   DevBuf pres[2], indirect;  // ReSTIR PT: output / temporal path reservoirs, indirect colour
   DevBuf pstate, ptq[2];     // ReSTIR PT wavefront: path state, segment queues
+  DevBuf retrace;            // hybrid shift: retrace records of temporal path reuse (2 per pixel)
   bool pt_ready = false;     // allocated (and zeroed) on the first frame with indirect light
 };
 

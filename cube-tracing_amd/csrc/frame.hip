@@ -95,6 +95,7 @@ struct DFrame {
   float4 *pres_t;    // [6 per pixel] temporal path reservoirs (history)
   float4 *indirect;  // indirect colour of the first pass
   float4 *pst;       // path state between the wavefront kernels (14 float4 per pixel)
+  float4 *rc;        // hybrid shift: retrace records, 2 slots of 5 float4 per pixel
   uint32_t *ptq[2];  // path-segment queues (tagged pixel ids) of even / odd depths
   float4 *seg_o, *seg_d, *seg_hit;  // path segments and their hits by queue position
   uint32_t qtag;     // frame index << kViewShift when frames are batched, else 0
@@ -1213,16 +1214,16 @@ __global__ void __launch_bounds__(kBlock) k_pt_step(FrameSet FS, uint32_t d) {
 // in two passes around one any-hit launch: kVisEmit queues the reuse's
 // visibility rays (taken as visible), kVisFinish recomputes with their
 // results and stores.  No traversal here keeps the kernels' registers low.
-// kHybrid: built with the hybrid shift, whose retrace traverses in place
-// (the stack in LDS); the other build has no in-place traversal.
+// kHybrid: built with the hybrid shift.  Its retraces are found by a third
+// pass before them (kVisRequest, which stores nothing else) and traced by
+// k_pt_retrace; both passes read the results.
 template <int kVis, bool kHybrid>
 __global__ void __launch_bounds__(kBlock) k_pt_end(FrameSet FS) {
-  __shared__ uint32_t stk[kHybrid ? (kStackLds > 0 ? kStackLds : 1) * kBlock : 1];
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
   const uint32_t px = blk_.y + threadIdx.x;
   PtTrace tr;
-  tr.stk = kHybrid ? stk + threadIdx.x : nullptr;
+  tr.stk = nullptr;
   tr.bounce_rays = 0u;
   tr.shadow_rays = 0u;
   tr.overflow = false;
@@ -1231,6 +1232,9 @@ __global__ void __launch_bounds__(kBlock) k_pt_end(FrameSet FS) {
   tr.bits = 0u;
   tr.emitted = 0u;
   tr.far = 0u;
+  tr.rc_slot = 0u;
+  tr.rc_prev = 0u;
+  tr.rc_req = 0u;
   if (px < F.npix) {
     if (F.dpos_c[px].w > 0.0f) {
       const uint32_t x = px % F.W, y = px / F.W;
@@ -1252,6 +1256,7 @@ __global__ void __launch_bounds__(kBlock) k_pt_end(FrameSet FS) {
         const uint32_t prev = min(F.npix - 1u, pcx + pcy * F.W);
         const PRes temporal = ld_pres(F.pres_t, prev);
         const Isect pi = isect_from_di_raw(F, ld_di(F.dpos_p, F.dnrm_p, F.dorg_p, F.dhit_p, prev));
+        tr.rc_prev = prev;
         out = pt_temporal<false, kVis, kHybrid>(F, out, temporal, ci, pi, seed, tr);
         ind = muls(out.F, out.weight);
       }
@@ -1265,6 +1270,49 @@ __global__ void __launch_bounds__(kBlock) k_pt_end(FrameSet FS) {
     if (kVis == kVisEmit) F.mask[px] = 0u;
   }
   if (kVis == kVisEmit) queue_flush(F, px, tr.emitted, tr.far, kCntPtShadow);
+  if (kVis == kVisRequest) {
+    // the segment queues are idle after the path loop: slot k's retraces
+    // go to queue k
+    const bool r0 = (tr.rc_req & 1u) != 0u, r1 = (tr.rc_req & 2u) != 0u;
+    const uint32_t p0 = queue_push(r0, &F.counters[kCntPtQueue], nullptr);
+    if (r0) F.ptq[0][p0] = F.qtag | px;
+    const uint32_t p1 = queue_push(r1, &F.counters[kCntPtQueue + 1], nullptr);
+    if (r1) F.ptq[1][p1] = F.qtag | px;
+  }
+}
+
+// The hybrid retraces that temporal path reuse requested, at full waves:
+// queue k holds the pixels whose slot-k shift replays a source path (slot
+// 0 from the pixel's own primary hit, slot 1 from the previous pixel's);
+// each result replaces the request in the pixel's retrace record.
+template <bool kMulti>
+__global__ void __launch_bounds__(kBlock) k_pt_retrace(FrameSet FS) {
+  __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
+  const DFrame &F0 = FS.fb[0];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
+  const uint32_t n0 = F0.counters[kCntPtQueue], n = n0 + F0.counters[kCntPtQueue + 1];
+  PtTrace tr;
+  tr.stk = stk + tid;
+  tr.bounce_rays = 0u;
+  tr.shadow_rays = 0u;
+  tr.overflow = false;
+  for (RayBatches rb(n, &F0.counters[kCntPtHead]); rb.valid(); rb.next()) {
+    const uint32_t q = rb.base + lane;
+    if (q < n) {
+      const uint32_t slot = q < n0 ? 0u : 1u;
+      const uint32_t e = F0.ptq[slot][slot ? q - n0 : q];
+      const DFrame &F = FS.fb[kMulti ? (e >> kViewShift) : 0u];
+      const uint32_t px = kMulti ? (e & kSlotMask) : e;
+      const float4 req = rc_rec(F, px, slot)[0];
+      const uint32_t di_index = __float_as_uint(req.x);
+      PRes src;
+      src.init_seed = __float_as_uint(req.y);
+      src.flags = __float_as_uint(req.z);
+      const Isect dst = slot == 0u ? isect_from_di(F, ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, di_index))
+                                   : isect_from_di_raw(F, ld_di(F.dpos_p, F.dnrm_p, F.dorg_p, F.dhit_p, di_index));
+      st_rc(F, px, slot, hybrid_retrace(F, dst, src, tr));
+    }
+  }
 }
 
 // Spatial path reuse (indirect_illumination_spatial_reuse, rgen.glsl:427-452)
@@ -1559,6 +1607,11 @@ uint32_t segments_resident() {
   static const uint32_t n = resident_grid(k_trace_segments<kCount, kMulti>);
   return n;
 }
+template <bool kMulti>
+uint32_t retrace_resident() {
+  static const uint32_t n = resident_grid(k_pt_retrace<kMulti>);
+  return n;
+}
 
 }  // namespace
 
@@ -1692,6 +1745,10 @@ int prepare_frame(Ctx *c, Accel *a, const ct_frame_params *p, hipStream_t st, DF
   int rc;
   const bool pt = p->restir.indirect != 0;
   if (pt && (rc = alloc_pt(*c, st))) return rc;
+  // retrace records exist for contexts that render the hybrid shift; each
+  // frame writes every record it reads, so they need no clearing
+  const bool hybrid = pt && p->restir.pt_temporal && p->restir.pt_shift_mapping == CT_SHIFT_HYBRID;
+  if (hybrid && (rc = c->fb.retrace.reserve(16 * 2 * kRcVec * (size_t)c->max_w * c->max_h))) return rc;
   // Cube lights from the manager (packed), uploaded after each build.
   const uint32_t ncube = a->set.cube_light_count;
   DevBuf &cube = c->inst_pt;
@@ -1754,6 +1811,7 @@ int prepare_frame(Ctx *c, Accel *a, const ct_frame_params *p, hipStream_t st, DF
   F.pres_t = b.pres[1].as<float4>();
   F.indirect = b.indirect.as<float4>();
   F.pst = b.pstate.as<float4>();
+  F.rc = b.retrace.as<float4>();
   F.ptq[0] = b.ptq[0].as<uint32_t>();
   F.ptq[1] = b.ptq[1].as<uint32_t>();
   // path segments reuse the primary-hit and shadow-ray buffers, which are
@@ -1887,10 +1945,23 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark,
       else
         hipLaunchKernelGGL(k_pt_step<false>, dim3(nblocks(np)), dim3(kBlock), 0, st, pix, d);
     }
-    if (L.hybrid)
+    if (L.hybrid) {
+      // the retraces temporal path reuse needs: requested into the idle
+      // segment queues, traced at full waves, read by both passes below
+      hipLaunchKernelGGL(k_reset_mask, dim3(1), dim3(64), 0, st, L.counters,
+                         (1u << kCntPtQueue) | (1u << (kCntPtQueue + 1)) | (1u << kCntPtHead));
+      hipLaunchKernelGGL((k_pt_end<kVisRequest, true>), dim3(nb), dim3(kBlock), 0, st, pix);
+      const uint32_t rgrid =
+          std::min<uint32_t>(std::min<uint32_t>(std::max<uint32_t>(nblocks(2 * np), 1), L.trace_blocks),
+                             L.multi ? retrace_resident<true>() : retrace_resident<false>());
+      if (L.multi)
+        hipLaunchKernelGGL(k_pt_retrace<true>, dim3(rgrid), dim3(kBlock), 0, st, pix);
+      else
+        hipLaunchKernelGGL(k_pt_retrace<false>, dim3(rgrid), dim3(kBlock), 0, st, pix);
       hipLaunchKernelGGL((k_pt_end<kVisEmit, true>), dim3(nb), dim3(kBlock), 0, st, pix);
-    else
+    } else {
       hipLaunchKernelGGL((k_pt_end<kVisEmit, false>), dim3(nb), dim3(kBlock), 0, st, pix);
+    }
   }
   if ((rc = mark())) return rc;  // kMarkPtEmit
   if (L.pt_depth) launch_any(L, pix, 2 * np, st);

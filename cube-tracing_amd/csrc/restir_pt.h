@@ -79,6 +79,11 @@ struct PtTrace {
   uint32_t bounce_rays, shadow_rays;
   bool overflow;
   uint32_t px, slot, bits, emitted, far;  // far: emitted rays of at least kFarRay
+  // Hybrid retraces of temporal path reuse: which of the pixel's two (0: the
+  // history shifted to this pixel's primary hit, 1: this pixel's path
+  // shifted to the previous pixel's), the previous pixel, and the retraces
+  // a request pass asked for (bit per slot).
+  uint32_t rc_slot, rc_prev, rc_req;
 };
 
 // ---------------------------------------------------------------- helpers
@@ -293,10 +298,14 @@ CT_D Isect bounce_intersect(const DFrame &F, F3 origin, F3 dir, PtTrace &tr) {
 // while the ray is recorded in the pixel's next slot), or read back from the
 // pixel's occlusion bits after the any-hit launch.
 constexpr int kVisInline = 0, kVisEmit = 1, kVisFinish = 2;
+// A pass that only finds the hybrid retraces temporal path reuse needs:
+// visibility is taken as given and nothing but the requests is stored.
+constexpr int kVisRequest = 3;
 
 // is_vertex_visible (light.glsl:82-142) on the current TLAS.
 template <bool kCount, int kVis = kVisInline>
 CT_D bool vertex_visible(const DFrame &F, F3 o, F3 d, float dist, uint32_t ti, uint32_t tp, PtTrace &tr) {
+  if (kVis == kVisRequest) return true;
   if (kVis == kVisInline) {
     float bt = 0.0f;
     uint32_t bi = kInvalid, bp = kInvalid;
@@ -901,6 +910,45 @@ struct RcData {
   F3 tp;
 };
 
+// The frame's retrace records: 5 float4 per pixel and slot.  A request pass
+// writes (DI index of the destination, the source's init seed and flags)
+// into the first; the retrace pass overwrites the record with its result.
+constexpr uint32_t kRcVec = 5;
+CT_D float4 *rc_rec(const DFrame &F, uint32_t px, uint32_t slot) { return F.rc + (size_t)kRcVec * (2u * px + slot); }
+CT_D void st_rc_request(const DFrame &F, uint32_t px, uint32_t slot, uint32_t di_index, const PRes &src) {
+  rc_rec(F, px, slot)[0] = make_float4(__uint_as_float(di_index), __uint_as_float(src.init_seed),
+                                       __uint_as_float(src.flags), 0.0f);
+}
+CT_D void st_rc(const DFrame &F, uint32_t px, uint32_t slot, const RcData &r) {
+  float4 *o = rc_rec(F, px, slot);
+  const Isect &i = r.prev;
+  o[0] = make_float4(i.P.x, i.P.y, i.P.z, i.distance);
+  o[1] = make_float4(i.n.x, i.n.y, i.n.z, i.wo.x);
+  o[2] = make_float4(i.wo.y, i.wo.z, i.wi.x, i.wi.y);
+  o[3] = make_float4(i.wi.z, __uint_as_float(i.inst), __uint_as_float(i.prim), __uint_as_float(i.mat_idx));
+  o[4] = make_float4(r.tp.x, r.tp.y, r.tp.z, __uint_as_float((r.ok ? 1u : 0u) | (i.is_hit ? 2u : 0u)));
+}
+CT_D RcData ld_rc(const DFrame &F, uint32_t px, uint32_t slot) {
+  const float4 *o = rc_rec(F, px, slot);
+  const float4 a = o[0], b = o[1], c = o[2], d = o[3], e = o[4];
+  RcData r;
+  const uint32_t fl = __float_as_uint(e.w);
+  r.ok = (fl & 1u) != 0u;
+  r.tp = f3(e.x, e.y, e.z);
+  Isect &i = r.prev;
+  i.is_hit = (fl & 2u) != 0u;
+  i.P = f3(a.x, a.y, a.z);
+  i.distance = a.w;
+  i.n = f3(b.x, b.y, b.z);
+  i.wo = f3(b.w, c.x, c.y);
+  i.wi = f3(c.z, c.w, d.x);
+  i.inst = __float_as_uint(d.y);
+  i.prim = __float_as_uint(d.z);
+  i.mat_idx = __float_as_uint(d.w);
+  i.mat = ld_mat(F, i.mat_idx);
+  return r;
+}
+
 // The retrace of trace_hybrid_shift_rays / trace_random_replay_path_hybrid_simple
 // (indirect_illumination.glsl:12-28, path_tracer.glsl:889-966): replay the
 // source reservoir's random numbers from the destination's primary
@@ -955,7 +1003,15 @@ CT_D F3 hybrid_integrand(const DFrame &F, float &dst_jac, const Isect &dst, cons
   F3 tp = f3(1.0f, 1.0f, 1.0f);
   Isect prev = dst;
   if (k > 1u) {
-    const RcData rc = hybrid_retrace(F, dst, s, tr);
+    // Temporal path reuse (the passes around the any-hit launch) takes the
+    // retrace from the frame's records, filled by a request pass and the
+    // retrace kernel in between; spatial path reuse retraces in place.
+    if (kVis == kVisRequest) {
+      st_rc_request(F, tr.px, tr.rc_slot, tr.rc_slot == 0u ? tr.px : tr.rc_prev, s);
+      tr.rc_req |= 1u << tr.rc_slot;
+      return f3(0, 0, 0);
+    }
+    const RcData rc = kVis == kVisInline ? hybrid_retrace(F, dst, s, tr) : ld_rc(F, tr.px, tr.rc_slot);
     if (!rc.ok) return f3(0, 0, 0);
     tp = rc.tp;
     prev = rc.prev;
@@ -1024,6 +1080,7 @@ CT_D PRes pt_temporal(const DFrame &F, const PRes &central, PRes temporal, const
       dst_jac = 1.0f;
       possible = tdst.weight > 0.0f;
     } else {
+      tr.rc_slot = 0u;  // the history shifted to this pixel's primary hit
       possible = shift_and_merge<kCount, kVis, kHybrid>(F, dst_jac, cur_i, tdst, prev_i, temporal, seed, tr);
     }
     if (possible) {
@@ -1037,6 +1094,7 @@ CT_D PRes pt_temporal(const DFrame &F, const PRes &central, PRes temporal, const
         p_sum += p_self;
       } else {
         float tj = 0.0f;
+        tr.rc_slot = 1u;  // this pixel's path shifted to the previous pixel's primary hit
         const F3 ti = shifted_integrand_copy<kCount, kVis, kHybrid>(F, tj, prev_i, cur_i, tdst, true, tr);
         const float pp = lum601(ti) * tj;
         p_sum += pp * temporal.M;
