@@ -1032,11 +1032,43 @@ CT_D F3 shifted_integrand(const DFrame &F, float &dst_jac, const Isect &dst, con
                           bool eval_vis, PtTrace &tr) {
   dst_jac = 0.0f;
   if (s.weight == 0.0f) return f3(0, 0, 0);
-  if (F.rp.pt_shift_mapping == CT_SHIFT_RECONNECTION)
-    return shift_reconnection<kCount, kVis>(F, dst_jac, dst, src, s, eval_vis, tr);
-  if (kHybrid && F.rp.pt_shift_mapping == CT_SHIFT_HYBRID)
-    return hybrid_integrand<kCount, kVis>(F, dst_jac, dst, src, s, eval_vis, tr);
-  return f3(1.0f, 1.0f, 1.0f);
+  const uint32_t mapping = F.rp.pt_shift_mapping;
+  if (!kHybrid) {
+    if (mapping == CT_SHIFT_RECONNECTION) return shift_reconnection<kCount, kVis>(F, dst_jac, dst, src, s, eval_vis, tr);
+    return f3(1.0f, 1.0f, 1.0f);
+  }
+  // The hybrid build reaches one reconnection for both mappings (it serves
+  // frame sets that mix them): the hybrid shift's checks and retrace first,
+  // then the reconnection from the retraced vertex or from dst.
+  const bool hyb = mapping == CT_SHIFT_HYBRID;
+  if (!hyb && mapping != CT_SHIFT_RECONNECTION) return f3(1.0f, 1.0f, 1.0f);
+  F3 tp = f3(1.0f, 1.0f, 1.0f);
+  RcData rc;
+  bool retraced = false;
+  if (hyb) {
+    const uint32_t k = pf_rc_length(s.flags), plen = pf_path_length(s.flags);
+    const bool escaped = plen + 1u == k;
+    if (!(k <= plen || escaped) || k == 0u) return f3(0, 0, 0);
+    if (k > 1u) {
+      // Temporal path reuse (the passes around the any-hit launch) takes the
+      // retrace from the frame's records, filled by a request pass and the
+      // retrace kernel in between; spatial path reuse retraces in place.
+      if (kVis == kVisRequest) {
+        st_rc_request(F, tr.px, tr.rc_slot, tr.rc_slot == 0u ? tr.px : tr.rc_prev, s);
+        tr.rc_req |= 1u << tr.rc_slot;
+        return f3(0, 0, 0);
+      }
+      rc = kVis == kVisInline ? hybrid_retrace(F, dst, s, tr) : ld_rc(F, tr.px, tr.rc_slot);
+      if (!rc.ok) return f3(0, 0, 0);
+      tp = rc.tp;
+      retraced = true;
+    }
+    if (!(tp.x > 0.0f || tp.y > 0.0f || tp.z > 0.0f)) return f3(0, 0, 0);
+  }
+  float rj = 0.0f;
+  const F3 r = shift_reconnection<kCount, kVis>(F, rj, retraced ? rc.prev : dst, src, s, eval_vis, tr, hyb, retraced);
+  dst_jac = rj;  // the hybrid shift's 1.0f * rj
+  return hyb ? mulv(tp, r) : r;
 }
 // compute_shifted_integrand, shift.glsl:485-497: on a copy of the reservoir.
 template <bool kCount, int kVis = kVisInline, bool kHybrid = false>
