@@ -982,51 +982,21 @@ CT_D RcData hybrid_retrace(const DFrame &F, Isect i0, const PRes &src, PtTrace &
   return out;
 }
 
-// compute_shifted_integrand_hybrid, shift.glsl:352-444: random replay up to
-// the vertex before the reconnection vertex, then the reconnection from
-// there with the cached Jacobian (or from the destination's primary vertex
-// when the reconnection vertex is the first bounce).  Two points differ
-// from the text, which as written rejects every shift: the Jacobian starts
-// at 1 (the replayed prefix has unit Jacobian in primary-sample space) where
-// the text sets 0 and multiplies into it (:363, :437), and reconnection
-// length 1 reconnects from the destination's primary vertex where the text
-// passes an invalid hit (:376-378).  A source whose reconnection vertex
-// does not exist (the path ended first) is not shifted.
-template <bool kCount, int kVis = kVisInline>
-CT_D F3 hybrid_integrand(const DFrame &F, float &dst_jac, const Isect &dst, const Isect &src, PRes &s, bool eval_vis,
-                         PtTrace &tr) {
-  dst_jac = 0.0f;
-  if (s.weight == 0.0f) return f3(0, 0, 0);
-  const uint32_t k = pf_rc_length(s.flags), plen = pf_path_length(s.flags);
-  const bool escaped = plen + 1u == k;
-  if (!(k <= plen || escaped) || k == 0u) return f3(0, 0, 0);
-  F3 tp = f3(1.0f, 1.0f, 1.0f);
-  Isect prev = dst;
-  if (k > 1u) {
-    // Temporal path reuse (the passes around the any-hit launch) takes the
-    // retrace from the frame's records, filled by a request pass and the
-    // retrace kernel in between; spatial path reuse retraces in place.
-    if (kVis == kVisRequest) {
-      st_rc_request(F, tr.px, tr.rc_slot, tr.rc_slot == 0u ? tr.px : tr.rc_prev, s);
-      tr.rc_req |= 1u << tr.rc_slot;
-      return f3(0, 0, 0);
-    }
-    const RcData rc = kVis == kVisInline ? hybrid_retrace(F, dst, s, tr) : ld_rc(F, tr.px, tr.rc_slot);
-    if (!rc.ok) return f3(0, 0, 0);
-    tp = rc.tp;
-    prev = rc.prev;
-  }
-  if (!(tp.x > 0.0f || tp.y > 0.0f || tp.z > 0.0f)) return f3(0, 0, 0);
-  float rj = 0.0f;
-  const F3 rc_tp = shift_reconnection<kCount, kVis>(F, rj, prev, src, s, eval_vis, tr, true, k > 1u);
-  dst_jac = 1.0f * rj;
-  return mulv(tp, rc_tp);
-}
-
 // compute_shifted_integrand_ (shift.glsl:446-483), reconnection or hybrid
 // (the hybrid shift only in the kernels built with kHybrid).  `s` is the
 // caller's copy of the source reservoir: the hybrid shift leaves the
 // destination's cached Jacobian in it.
+// The hybrid shift, compute_shifted_integrand_hybrid (shift.glsl:352-444):
+// random replay up to the vertex before the reconnection vertex, then the
+// reconnection from there with the cached Jacobian (or from the
+// destination's primary vertex when the reconnection vertex is the first
+// bounce).  Two points differ from the text, which as written rejects every
+// shift: the Jacobian starts at 1 (the replayed prefix has unit Jacobian in
+// primary-sample space) where the text sets 0 and multiplies into it (:363,
+// :437), and reconnection length 1 reconnects from the destination's
+// primary vertex where the text passes an invalid hit (:376-378).  A source
+// whose reconnection vertex does not exist (the path ended first) is not
+// shifted.
 template <bool kCount, int kVis = kVisInline, bool kHybrid = false>
 CT_D F3 shifted_integrand(const DFrame &F, float &dst_jac, const Isect &dst, const Isect &src, PRes &s,
                           bool eval_vis, PtTrace &tr) {
