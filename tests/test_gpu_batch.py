@@ -92,3 +92,32 @@ def test_batch_autotune_keeps_results(gpu):
             a, c = _read(pc, w), _read(bc, w)
             for key in a:
                 assert np.array_equal(a[key], c[key]), (f, t.groups, w.name, key)
+
+
+def test_batch_mixing_shift_mappings(gpu):
+    """Views with the hybrid shift beside views with the reconnection shift
+    in one batch (the hybrid kernels, their retrace requests in the group's
+    segment queues and the retrace kernel over several frames' pixels): four
+    frames, each view's bits equal its own render."""
+    scenes, views = ctamd.batched_views(n_views=6, seed=0xC3, grids=(64, 96), max_depth=3, size_scale=0.15)
+    built = [_scene(s) for s in scenes]
+    for n, (k, w) in enumerate(views):
+        w.restir = ctamd.RestirParams.from_buffer_copy(bytes(w.restir))
+        w.restir.pt_shift_mapping = ctamd.CT_SHIFT_HYBRID if n % 2 == 0 else ctamd.CT_SHIFT_RECONNECTION
+    plain = [_ctx(*built[k], w) for k, w in views]
+    batch_ctx = [_ctx(*built[k], w) for k, w in views]
+    b = ctamd.Batch(batch_ctx, [built[k][1] for k, _ in views])
+    long_rc = 0
+    for f in range(4):
+        params = [ctamd.frame_params(w, f) for _, w in views]
+        b.render(params, use_graph=True)
+        for (k, w), ctx, p in zip(views, plain, params):
+            ctx.render(built[k][1], p)
+        for n, ((k, w), pc, bc) in enumerate(zip(views, plain, batch_ctx)):
+            a, c = _read(pc, w), _read(bc, w)
+            for key in a:
+                assert np.array_equal(a[key], c[key]), (f, n, w.name, key)
+            if n % 2 == 0:
+                pres = a["pres"]
+                long_rc += int((((pres[:, 2] >> 4) & 0xF) >= 2)[pres[:, 1].view(np.float32) > 0].sum())
+    assert long_rc > 0  # the hybrid views hold paths with reconnection lengths above 1
