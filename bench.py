@@ -420,13 +420,18 @@ def setup(sc, w):
 
 # Algorithmic bytes of the per-pixel passes: the pixel buffers each pass
 # must read and write (structure-of-arrays records: DI 64 B, reservoir
-# 36 B, neighbour record 128 B, hit 16 B, shadow-ray record 36 B) plus
+# 36 B, neighbour record 128 B, spatial reuse's per-neighbour result 48 B,
+# hit 16 B, shadow-ray record 36 B) plus
 # the scene records it reads per pixel (AABB 24 B, instance 48 B,
 # primitive 8 B, previous transform 64 B, material 32 B, light 48 B per RIS
 # candidate).
 RIS_HIT_BYTES = 16 + 24 + 48 + 8 + 64 + 32 + 36 + 8 + 36 + 64 + 4 + 4
 RIS_MISS_BYTES = 16 + 16 + 16 + 64 + 4
-SPATIAL_HIT_BYTES = (64 + 32 + 36 + 4) + (64 + 32 + 36 + 4 + 16 + 36 + 64)  # emit + finish, own records
+# spatial reuse: own records in both passes, the 16 B header of the
+# per-neighbour results written by the emit pass and read by the finish pass
+SPATIAL_HIT_BYTES = (64 + 32 + 36 + 4) + (64 + 32 + 36 + 4 + 16 + 36 + 64) + 2 * 16
+# per neighbour: its 128 B record gathered once, its 48 B result written and read
+SPATIAL_NEIGHBOUR_BYTES = 128 + 2 * 48
 RAY_BYTES = 36
 
 
@@ -464,7 +469,7 @@ def frame_bench(w, acc, ctx, K, Wm, traffic):
         "k_primary": 16 * npix + 64 * ist.closest_nodes + 28 * ist.closest_leaves + 64 * ist.closest_instances,
         "k_ris": hits * (RIS_HIT_BYTES + 48 * n_c) + (npix - hits) * RIS_MISS_BYTES
                  + RAY_BYTES * (ist.shadow_rays_ris + ist.shadow_rays_temporal),
-        "k_spatial": hits * (SPATIAL_HIT_BYTES + 2 * 128 * w.restir.max_neighbors)
+        "k_spatial": hits * (SPATIAL_HIT_BYTES + SPATIAL_NEIGHBOUR_BYTES * w.restir.max_neighbors)
                      + RAY_BYTES * ist.shadow_rays_spatial,
     }
     log(f"{w.name}: instrumented frame: {hits} hit pixels, {n_any} shadow rays ({ist.shadow_rays_ris} RIS, "

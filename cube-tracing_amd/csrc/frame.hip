@@ -91,6 +91,7 @@ struct DFrame {
   uint32_t queue_cap;
   uint32_t *counters;
   float4 *gat;  // [8 per pixel] neighbour record, see st_gather
+  float4 *spc;  // spatial reuse's per-neighbour results, see spc_at
   float4 *pres_o;    // [6 per pixel] output path reservoirs (this frame)
   float4 *pres_t;    // [6 per pixel] temporal path reservoirs (history)
   float4 *indirect;  // indirect colour of the first pass
@@ -700,9 +701,71 @@ CT_D void temporal_emit(const DFrame &F, const Res &r, Res prev, uint32_t pidx, 
   v.slot = slot0 + 2u;
 }
 
-// SPATIAL_REUSE, restir_resampling.glsl:418-540, with pairwise_mis.glsl:8-114
+// The emit pass of spatial reuse keeps, per pixel, what the finish pass needs of each
+// neighbour in place of a second gather: header spc[px] = (neighbour count,
+// the seed after the neighbour draw); neighbour i's fields j = 0..2 at
+// spc[(1 + 3 i + j) * npix + px]: (target of its sample here, luminance of
+// this pixel's sample's target there), (M, luminance of its F, W_y, flags),
+// (Y, type, seed).  The targets are the unoccluded ones; flag bits 1 and 2
+// say whether each took a visibility slot, whose bit the finish pass reads.
+CT_D float4 *spc_at(const DFrame &F, uint32_t field, uint32_t px) { return F.spc + (size_t)field * F.npix + px; }
+
+// SPATIAL_REUSE, restir_resampling.glsl:418-540, with pairwise_mis.glsl:8-114.  kEmit:
+// the pass that queues the visibility rays (taken as unoccluded) and keeps
+// the per-neighbour results; else the pass that finishes from them with the
+// rays' bits.
+template <bool kEmit>
 CT_D void spatial_reuse(const DFrame &F, Res &r, float confidence, uint32_t x, uint32_t y, F3 ray_d, const Hit &hit,
                         const Mat &mat, uint32_t &seed, Vis &v) {
+  const uint32_t px = y * F.W + x;
+  if (!kEmit) {
+    const float4 hd = *spc_at(F, 0, px);
+    const uint32_t num = __float_as_uint(hd.x);
+    seed = __float_as_uint(hd.y);
+    if (num == 0) return;
+    Res mres = res_init();
+    float m_c = 1.0f;
+    float M_s = r.M;
+    const float fk = (float)num;
+    for (uint32_t i = 0; i < num; ++i) {
+      const float4 a = *spc_at(F, 1 + 3 * i, px), b = *spc_at(F, 2 + 3 * i, px), c = *spc_at(F, 3 + 3 * i, px);
+      const uint32_t fl = __float_as_uint(b.w);
+      const float in_M = b.x;
+      bool occ_a = false, occ_b = false;
+      if (fl & 2u) occ_a = occluded(F, v, f3(0, 0, 0), f3(0, 0, 0), 0.0f, 0u, 0u);
+      if (fl & 4u) occ_b = occluded(F, v, f3(0, 0, 0), f3(0, 0, 0), 0.0f, 0u, 0u);
+      const F3 curr_target = occ_a ? f3(0, 0, 0) : xyz(a);
+      float m_i = 0.0f;
+      if (fl & 1u) {
+        const float tl = luminance(curr_target);
+        const float mm = in_M * b.y;
+        const float den = mm + (r.M / fk) * tl;
+        m_i = den > 0.0f ? mm / den : 0.0f;
+      }
+      {
+        const float pic = occ_b ? 0.0f : a.w;
+        const float pcc = luminance(r.F);
+        const float numer = in_M * pic;
+        const bool den_ok = (pcc + numer) > 0.0f;
+        m_c += 1.0f;
+        m_c -= den_ok ? numer / (numer + (r.M / fk) * pcc) : 0.0f;
+      }
+      if (fl & 1u) {
+        const float w_i = luminance(curr_target) * b.z * m_i;
+        update_reservoir(mres, __float_as_uint(c.x), __float_as_uint(c.y), __float_as_uint(c.z), curr_target, w_i,
+                         1.0f, seed);
+      }
+      M_s += in_M;
+    }
+    uint32_t s2 = seed;
+    const float w_c = luminance(r.F) * r.W_y * m_c;
+    update_reservoir(mres, r.Y, r.type, r.seed, r.F, w_c, 1.0f, s2);
+    mres.M = M_s;
+    const float tl = luminance(mres.F);
+    mres.W_y = tl > 0.0f ? mres.W_sum / (tl * (float)(1u + num)) : 0.0f;
+    r = mres;
+    return;
+  }
   const ct_restir_params &rp = F.rp;
   if (r.W_y == 0.0f) confidence = 0.0f;
   float radius = mixf(rp.max_neighbor_radius, rp.min_neighbor_radius, clampf(hit.distance / rp.max_distance_to_hit, 0.0f, 1.0f));
@@ -720,11 +783,8 @@ CT_D void spatial_reuse(const DFrame &F, Res &r, float confidence, uint32_t x, u
     if (gather_distance(F.gat, l) <= 0.0f) continue;
     if (num < CT_MAX_NEIGHBORS) idx[num++] = l;
   }
+  *spc_at(F, 0, px) = make_float4(__uint_as_float(num), __uint_as_float(seed), 0.0f, 0.0f);
   if (num == 0) return;
-  Res mres = res_init();
-  float m_c = 1.0f;
-  float M_s = r.M;
-  const float fk = (float)num;
   const Shade sh = shade_of(ray_d, hit);
   for (uint32_t i = 0; i < num; ++i) {
     Res in;
@@ -734,38 +794,23 @@ CT_D void spatial_reuse(const DFrame &F, Res &r, float confidence, uint32_t x, u
     F3 nray_d = normalize(sub(nh.P, nd.org));
     Mat nm = ld_mat(F, nd.mat);
     F3 curr_target = f3(0, 0, 0);
-    float m_i = 0.0f;
+    uint32_t fl = 0u;
+    float pii = 0.0f;
+    const uint32_t s0 = v.slot;
     if (res_valid(in)) {
       curr_target = res_radiance(F, in, sh, hit, mat, v);
-      const float tl = luminance(curr_target);
-      const float pii = luminance(in.F);
-      float mm = in.M * pii;
-      float den = mm + (r.M / fk) * tl;
-      m_i = den > 0.0f ? mm / den : 0.0f;
+      pii = luminance(in.F);
+      fl |= 1u;
     }
+    const uint32_t s1 = v.slot;
     F3 input_target = f3(0, 0, 0);
     if (res_valid(r)) input_target = res_radiance(F, r, nray_d, nh, nm, v);
-    {
-      const float pic = luminance(input_target);
-      const float pcc = luminance(r.F);
-      const float numer = in.M * pic;
-      const bool den_ok = (pcc + numer) > 0.0f;
-      m_c += 1.0f;
-      m_c -= den_ok ? numer / (numer + (r.M / fk) * pcc) : 0.0f;
-    }
-    if (res_valid(in)) {
-      const float w_i = luminance(curr_target) * in.W_y * m_i;
-      update_reservoir(mres, in.Y, in.type, in.seed, curr_target, w_i, 1.0f, seed);
-    }
-    M_s += in.M;
+    if (s1 != s0) fl |= 2u;
+    if (v.slot != s1) fl |= 4u;
+    *spc_at(F, 1 + 3 * i, px) = make_float4(curr_target.x, curr_target.y, curr_target.z, luminance(input_target));
+    *spc_at(F, 2 + 3 * i, px) = make_float4(in.M, pii, in.W_y, __uint_as_float(fl));
+    *spc_at(F, 3 + 3 * i, px) = make_float4(__uint_as_float(in.Y), __uint_as_float(in.type), __uint_as_float(in.seed), 0.0f);
   }
-  uint32_t s2 = seed;
-  const float w_c = luminance(r.F) * r.W_y * m_c;
-  update_reservoir(mres, r.Y, r.type, r.seed, r.F, w_c, 1.0f, s2);
-  mres.M = M_s;
-  const float tl = luminance(mres.F);
-  mres.W_y = tl > 0.0f ? mres.W_sum / (tl * (float)(1u + num)) : 0.0f;
-  r = mres;
 }
 
 // Camera ray of pixel (x, y): get_ray_from_current_pixel, rgen.glsl:18-42
@@ -1048,7 +1093,7 @@ __global__ void __launch_bounds__(kBlock) k_spatial(FrameSet FS) {
       Res sr = ld_res(F.rA_i, F.rB_i, F.rC_i, px);
       Vis v{kEmit, px, 0u, kEmit ? 0u : F.mask[px], 0u, 0u};
       uint32_t seed = di.seed;
-      spatial_reuse(F, sr, di.conf, x, y, ray_d, hit, mat, seed, v);
+      spatial_reuse<kEmit>(F, sr, di.conf, x, y, ray_d, hit, mat, seed, v);
       emitted = v.emitted;
       far = v.far;
       if (!kEmit && F.rp.indirect) {
@@ -1749,6 +1794,10 @@ int prepare_frame(Ctx *c, Accel *a, const ct_frame_params *p, hipStream_t st, DF
   // frame writes every record it reads, so they need no clearing
   const bool hybrid = pt && p->restir.pt_temporal && p->restir.pt_shift_mapping == CT_SHIFT_HYBRID;
   if (hybrid && (rc = c->fb.retrace.reserve(16 * 2 * kRcVec * (size_t)c->max_w * c->max_h))) return rc;
+  // spatial reuse's per-neighbour results: a header and 3 float4 per neighbour
+  const size_t nfield =
+      1 + 3 * (size_t)std::max<uint32_t>(std::max(p->restir.max_neighbors, p->restir.min_neighbors), 1u);
+  if ((rc = c->fb.spatial.reserve(16 * nfield * (size_t)c->max_w * c->max_h))) return rc;
   // Cube lights from the manager (packed), uploaded after each build.
   const uint32_t ncube = a->set.cube_light_count;
   DevBuf &cube = c->inst_pt;
@@ -1807,6 +1856,7 @@ int prepare_frame(Ctx *c, Accel *a, const ct_frame_params *p, hipStream_t st, DF
   F.queue_cap = (uint32_t)(b.queue.bytes / 4);
   F.counters = b.counters.as<uint32_t>();
   F.gat = b.gather.as<float4>();
+  F.spc = b.spatial.as<float4>();
   F.pres_o = b.pres[0].as<float4>();
   F.pres_t = b.pres[1].as<float4>();
   F.indirect = b.indirect.as<float4>();
