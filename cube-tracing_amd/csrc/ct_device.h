@@ -306,7 +306,12 @@ CT_D uint4 gld(const uint4 *p) {
 }
 
 // Test both children of a node; returns the next ref (or kInvalid to pop)
-// and pushes the far child when both are hit (near first, ties to child 0).
+// and pushes the other child when both are hit.  Closest-hit walks take the
+// near child first (ties to child 0), so the best hit prunes early.  Any-hit
+// walks take the far child first (ties to child 1): a shadow ray's
+// occluders cluster at its target end, where lights sit among other
+// voxels, and the outcome does not depend on the order.
+template <bool kFarFirst>
 CT_D uint32_t visit_node(const DNode *nodes, uint32_t idx, F3 o, F3 inv, float limit, Stack &stk, bool &overflow) {
   const float4 *p = reinterpret_cast<const float4 *>(nodes + idx);
   float4 a = gld(p), b = gld(p + 1), c = gld(p + 2);
@@ -315,7 +320,7 @@ CT_D uint32_t visit_node(const DNode *nodes, uint32_t idx, F3 o, F3 inv, float l
   bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, limit, tn0);
   bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, limit, tn1);
   if (h0 && h1) {
-    bool swap = tn1 < tn0;
+    const bool swap = kFarFirst ? !(tn1 < tn0) : (tn1 < tn0);
     stk.push(swap ? d.x : d.y, overflow);
     return swap ? d.y : d.x;
   }
@@ -431,7 +436,7 @@ struct Walker {
       // TLAS and BLAS nodes take the same code.
       if (kCount) cnt.nodes++;
       const DNode *nodes = in_blas ? blas_nodes + first : tlas_nodes;
-      next = visit_node(nodes, ref, cur_o, cur_inv, kAny ? limit : best_t, st, overflow);
+      next = visit_node<kAny>(nodes, ref, cur_o, cur_inv, kAny ? limit : best_t, st, overflow);
     } else {
       if (kCount) cnt.leaves++;
       const float4 L = gld(S.blas_leaves + first + (ref & ~kLeaf));

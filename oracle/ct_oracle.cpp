@@ -521,8 +521,11 @@ static bool any_leaf(const Scene &sc, uint32_t ii, uint32_t pid, V3 oo, V3 od,
   return true;
 }
 
-// Two-level traversal: TLAS in world space, each BLAS in object space,
-// near child first (ties go to child 0), far child pushed.
+// Two-level traversal: TLAS in world space, each BLAS in object space.
+// Closest-hit walks visit the near child first (ties go to child 0) and push
+// the far one; any-hit walks visit the far child first (ties go to child 1)
+// and push the near one, the build's order for shadow rays (the outcome of
+// an any-hit query does not depend on it; the node counts do).
 template <bool kAny, bool kBounce = false>
 static bool traverse(const Scene &sc, V3 o, V3 d, float limit, uint32_t ti, uint32_t tp,
                      ClosestState &st, TraceCount &cnt) {
@@ -558,7 +561,7 @@ static bool traverse(const Scene &sc, V3 o, V3 d, float limit, uint32_t ti, uint
           bool h0 = slab(bx, oo, oinv, lim, tn0), h1 = slab(bx + 6, oo, oinv, lim, tn1);
           uint32_t c0 = b.child[2 * r], c1 = b.child[2 * r + 1];
           if (h0 && h1) {
-            bool swap = tn1 < tn0;
+            const bool swap = kAny ? !(tn1 < tn0) : (tn1 < tn0);
             stack[sp++] = swap ? c0 : c1;
             if (sp >= kStack) { std::fprintf(stderr, "oracle stack overflow\n"); std::abort(); }
             r = swap ? c1 : c0;
@@ -578,7 +581,7 @@ static bool traverse(const Scene &sc, V3 o, V3 d, float limit, uint32_t ti, uint
       bool h0 = slab(bx, o, winv, lim, tn0), h1 = slab(bx + 6, o, winv, lim, tn1);
       uint32_t c0 = sc.tlas.child[2 * ref], c1 = sc.tlas.child[2 * ref + 1];
       if (h0 && h1) {
-        bool swap = tn1 < tn0;
+        const bool swap = kAny ? !(tn1 < tn0) : (tn1 < tn0);
         stack[sp++] = swap ? c0 : c1;
         ref = swap ? c1 : c0;
         continue;
