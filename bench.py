@@ -425,7 +425,7 @@ def setup(sc, w):
 # the scene records it reads per pixel (AABB 24 B, instance 48 B,
 # primitive 8 B, previous transform 64 B, material 32 B, light 48 B per RIS
 # candidate).
-RIS_HIT_BYTES = 16 + 24 + 48 + 8 + 64 + 32 + 36 + 8 + 36 + 64 + 4 + 4
+RIS_HIT_BYTES = 16 + 24 + 48 + 8 + 64 + 32 + 36 + 8 + 36 + 64 + 4 + 4 + 32  # + temporal reuse's two targets
 RIS_MISS_BYTES = 16 + 16 + 16 + 64 + 4
 # spatial reuse: own records in both passes, the 16 B header of the
 # per-neighbour results written by the emit pass and read by the finish pass

@@ -227,7 +227,7 @@ struct FrameBufs {
   DevBuf di_pos[2], di_nrm[2], di_org[2], di_hit[2];  // DI, current/previous
   DevBuf resA[2], resB[2], resC[2];                   // intermediate / previous
   DevBuf velocity, img[2], taa, hit, pidx, prev_res_copy;
-  DevBuf mask, gather, spatial;                       // per-pixel pass state
+  DevBuf mask, gather, spatial, temporal;             // per-pixel pass state
   DevBuf ray_o, ray_d, ray_t, queue, counters, counters_host_staging;
   DevBuf pres[2], indirect;  // ReSTIR PT: output / temporal path reservoirs, indirect colour
   DevBuf pstate, ptq[2];     // ReSTIR PT wavefront: path state, segment queues

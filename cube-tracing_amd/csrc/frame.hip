@@ -92,6 +92,7 @@ struct DFrame {
   uint32_t *counters;
   float4 *gat;  // [8 per pixel] neighbour record, see st_gather
   float4 *spc;  // spatial reuse's per-neighbour results, see spc_at
+  float4 *tpc;  // temporal reuse's two targets and flags, [2 per pixel], see temporal_emit
   float4 *pres_o;    // [6 per pixel] output path reservoirs (this frame)
   float4 *pres_t;    // [6 per pixel] temporal path reservoirs (history)
   float4 *indirect;  // indirect colour of the first pass
@@ -613,7 +614,9 @@ CT_D Res ris(const DFrame &F, float confidence, F3 ray_d, Hit &hit, const Mat &m
 // one or a moved light takes its current index.
 // Its two visibility queries take the slots v.slot and v.slot + 1, whether
 // made or not, so that temporal_emit can queue them before the RIS sample's
-// own visibility is known.
+// own visibility is known.  temporal_emit also evaluates both targets; this
+// pass takes them from the pixel's record (tpc) and applies the rays' bits.
+constexpr uint32_t kTpRayA = 2u, kTpDeleted = 4u, kTpRayB = 16u;
 CT_D void temporal_reuse(const DFrame &F, Res &r, Res prev, uint32_t pidx, F3 ray_d, const Hit &hit, const Mat &mat,
                          uint32_t &seed, Vis &v) {
   const float mcap = F.rp.max_influence_from_the_past;
@@ -622,21 +625,17 @@ CT_D void temporal_reuse(const DFrame &F, Res &r, Res prev, uint32_t pidx, F3 ra
   prev.M = gmin(prev.M, mcap * r.M);
   const float prev_M = prev.M;
   const float new_M = r.M + prev_M;
+  // the two targets temporal_emit evaluated (unoccluded) and its flags
+  const float4 ta = F.tpc[v.px], tb = F.tpc[F.npix + v.px];
+  const uint32_t fl = __float_as_uint(ta.w);
   if (r.W_sum > 0.0f) {
     float tlp = 0.0f;
     if (luminance(r.F) > 1e-6f) {
-      DI dp = ld_di(F.dpos_p, F.dnrm_p, F.dorg_p, F.dhit_p, pidx);
-      if (remap) {
-        const uint32_t slot = (dp.inst < F.S.n_inst ? F.prev_first[dp.inst] : 0u) + dp.prim;
-        const uint32_t m = slot < F.remap_cap ? F.remap_prim[slot] : 0u;
-        if (m == kInvalid) return;
-        if (m != 0u) dp.prim = m;
-      }
-      Hit ph = hit_from_di(dp);
-      Mat pm = ld_mat(F, dp.mat);
-      F3 pd = normalize(sub(ph.P, dp.org));
+      if (fl & kTpDeleted) return;  // the previous hit primitive was deleted
+      F3 t = xyz(ta);
       v.slot = slot0;
-      tlp = luminance(res_radiance(F, r, pd, ph, pm, v));
+      if ((fl & kTpRayA) && occluded(F, v, f3(0, 0, 0), f3(0, 0, 0), 0.0f, 0u, 0u)) t = f3(0, 0, 0);
+      tlp = luminance(t);
     }
     const float p_curr = r.M * luminance(r.F);
     const float m_curr = p_curr / gmax(p_curr + prev_M * tlp, 1e-6f);
@@ -648,7 +647,8 @@ CT_D void temporal_reuse(const DFrame &F, Res &r, Res prev, uint32_t pidx, F3 ra
   }
   if (prev.Y != kInvalid) {
     v.slot = slot0 + 1u;
-    F3 ct = res_radiance(F, prev, ray_d, hit, mat, v);
+    F3 ct = xyz(tb);
+    if ((fl & kTpRayB) && occluded(F, v, f3(0, 0, 0), f3(0, 0, 0), 0.0f, 0u, 0u)) ct = f3(0, 0, 0);
     const float lc = luminance(ct);
     if (lc > 0.0f) {
       const float lp = luminance(prev.F);
@@ -674,6 +674,8 @@ CT_D void temporal_emit(const DFrame &F, const Res &r, Res prev, uint32_t pidx, 
   const bool remap = (F.active & CT_REMAP_BIT) != 0u;
   const uint32_t slot0 = v.slot;
   prev.M = gmin(prev.M, F.rp.max_influence_from_the_past * r.M);
+  uint32_t fl = 0u;
+  F3 ta = f3(0, 0, 0), tb = f3(0, 0, 0);
   if (r.W_sum > 0.0f && luminance(r.F) > 1e-6f) {
     DI dp = ld_di(F.dpos_p, F.dnrm_p, F.dorg_p, F.dhit_p, pidx);
     uint32_t m = 0u;
@@ -687,7 +689,10 @@ CT_D void temporal_emit(const DFrame &F, const Res &r, Res prev, uint32_t pidx, 
       Mat pm = ld_mat(F, dp.mat);
       F3 pd = normalize(sub(ph.P, dp.org));
       v.slot = slot0;
-      (void)res_radiance(F, r, pd, ph, pm, v);
+      ta = res_radiance(F, r, pd, ph, pm, v);
+      if (v.slot != slot0) fl |= kTpRayA;
+    } else {
+      fl |= kTpDeleted;
     }
   }
   if (remap && prev.type == CT_LIGHT_CUBE && prev.Y < F.remap_light_cap) {
@@ -696,9 +701,12 @@ CT_D void temporal_emit(const DFrame &F, const Res &r, Res prev, uint32_t pidx, 
   }
   if (prev.Y != kInvalid) {
     v.slot = slot0 + 1u;
-    (void)res_radiance(F, prev, ray_d, hit, mat, v);
+    tb = res_radiance(F, prev, ray_d, hit, mat, v);
+    if (v.slot != slot0 + 1u) fl |= kTpRayB;
   }
   v.slot = slot0 + 2u;
+  F.tpc[v.px] = make_float4(ta.x, ta.y, ta.z, __uint_as_float(fl));
+  F.tpc[F.npix + v.px] = make_float4(tb.x, tb.y, tb.z, 0.0f);
 }
 
 // The emit pass of spatial reuse keeps, per pixel, what the finish pass needs of each
@@ -1798,6 +1806,7 @@ int prepare_frame(Ctx *c, Accel *a, const ct_frame_params *p, hipStream_t st, DF
   const size_t nfield =
       1 + 3 * (size_t)std::max<uint32_t>(std::max(p->restir.max_neighbors, p->restir.min_neighbors), 1u);
   if ((rc = c->fb.spatial.reserve(16 * nfield * (size_t)c->max_w * c->max_h))) return rc;
+  if ((rc = c->fb.temporal.reserve(32 * (size_t)c->max_w * c->max_h))) return rc;
   // Cube lights from the manager (packed), uploaded after each build.
   const uint32_t ncube = a->set.cube_light_count;
   DevBuf &cube = c->inst_pt;
@@ -1857,6 +1866,7 @@ int prepare_frame(Ctx *c, Accel *a, const ct_frame_params *p, hipStream_t st, DF
   F.counters = b.counters.as<uint32_t>();
   F.gat = b.gather.as<float4>();
   F.spc = b.spatial.as<float4>();
+  F.tpc = b.temporal.as<float4>();
   F.pres_o = b.pres[0].as<float4>();
   F.pres_t = b.pres[1].as<float4>();
   F.indirect = b.indirect.as<float4>();
