@@ -156,7 +156,8 @@ def batched_bench(frames=5, warmup=2, n_views=64):
     hip = ctypes.CDLL("libamdhip64.so")
     params = [[ctamd.frame_params(w, f) for _, w in views] for f in range(warmup + frames)]
     tune = b.autotune(params[0], repeats=2)
-    log(f"batched: autotuned {tune.groups} view groups, traversal grid {tune.blocks_per_cu} blocks/CU ({tune.ms:.2f} ms)")
+    log(f"batched: autotuned over {tune.candidates} shapes: {tune.groups} view groups, traversal grid "
+        f"{tune.blocks_per_cu} blocks/CU, far rays from {tune.far_ray:g} units ({tune.ms:.2f} ms)")
     # plain: one ct_frame_render per view and frame, issued back to back (the
     # views' own streams overlap on the GPU), then the same with each view
     # finished before the next starts, as a renderer drawing one view per
@@ -223,7 +224,8 @@ def batched_bench(frames=5, warmup=2, n_views=64):
             "speedup_vs_one_stream": round(t_one / t_batch, 2),
             "batched_ms_per_frame": round(1000 * t_batch / frames, 3),
             "speedup": round(t_plain / t_batch, 2), "speedup_vs_serial": round(t_serial / t_batch, 2), "batched_mrays_s": round(rays / t_batch / 1e6, 1),
-            "graph": True, "tuned_blocks_per_cu": tune.blocks_per_cu, "tuned_groups": tune.groups}
+            "graph": True, "tuned_blocks_per_cu": tune.blocks_per_cu, "tuned_groups": tune.groups,
+            "tuned_far_ray": tune.far_ray, "tuner_candidates": tune.candidates}
 
 
 def adjoint_bench(steps=3, fit_steps=200):

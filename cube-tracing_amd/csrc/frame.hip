@@ -49,7 +49,8 @@ constexpr int kCntQueue = 0, kCntHead = 1, kCntRis = 2, kCntTemporal = 3, kCntSp
 #define CT_FAR_RAY 2.0f
 #endif
 // A shadow ray at least this long (world units; the 256^3 workload is 8
-// wide) goes to the front of the queue and is traced first.
+// wide) goes to the front of the queue and is traced first (the frame's
+// far_ray; a batch's autotuner picks its own).
 constexpr float kFarRay = CT_FAR_RAY;
 
 struct DFrame {
@@ -102,6 +103,7 @@ struct DFrame {
   float4 *seg_o, *seg_d, *seg_hit;  // path segments and their hits by queue position
   uint32_t qtag;     // frame index << kViewShift when frames are batched, else 0
   uint32_t taa_w, taa_h;  // TAA dispatch size (main.cpp:1642-1643 truncation applied)
+  float far_ray;     // shadow rays at least this long are queued (and traced) first
 };
 
 // Several frames in one launch: every block knows its frame and first
@@ -274,7 +276,7 @@ CT_D bool occluded(const DFrame &F, Vis &v, F3 o, F3 d, float dist, uint32_t ti,
     F.ray_d[idx] = make_float4(d.x, d.y, d.z, __uint_as_float(ti));
     F.ray_t[idx] = tp;
     v.emitted |= 1u << k;
-    if (dist >= kFarRay) v.far |= 1u << k;
+    if (dist >= F.far_ray) v.far |= 1u << k;
     return false;
   }
   return (v.bits >> k) & 1u;
@@ -1882,6 +1884,7 @@ int prepare_frame(Ctx *c, Accel *a, const ct_frame_params *p, hipStream_t st, DF
   F.qtag = 0u;
   F.taa_w = p->restir.taa_truncate_dispatch ? (F.W / 8) * 8 : F.W;
   F.taa_h = p->restir.taa_truncate_dispatch ? (F.H / 8) * 8 : F.H;
+  F.far_ray = kFarRay;
   return CT_OK;
 }
 
@@ -2290,6 +2293,7 @@ struct ct_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint32_t blocks_per_cu = 5;
+  float far_ray = kFarRay;              // the queue's far/near split (world units)
   uint32_t *counters_pinned = nullptr;  // [kMaxGroups][8]
   hipEvent_t t0 = nullptr, t1 = nullptr, fork = nullptr;
   hipStream_t last_stream = nullptr;
@@ -2444,6 +2448,7 @@ int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, v
     for (uint32_t i = G.first; i < G.first + G.count; ++i) {
       DFrame &F = rec[i];
       if ((rc = prepare_frame(b->ctxs[i], b->accels[i], &params[i], st, F))) return rc;
+      F.far_ray = b->far_ray;
       F.queue = G.queue.as<uint32_t>();
       F.queue_cap = (uint32_t)(G.queue.bytes / 4);
       F.counters = G.counters.as<uint32_t>();
@@ -2581,40 +2586,51 @@ int ct_batch_get_stats(ct_batch *b, ct_frame_stats *out) {
 
 int ct_batch_autotune(ct_batch *b, const ct_frame_params *params, uint32_t repeats, ct_batch_tuning *out) {
   if (!b || !params || repeats == 0) return fail(CT_E_INVALID, "ct_batch_autotune: bad argument");
-  const uint32_t groups[] = {1, 2, 3, 4}, per_cu[] = {4, 5};
+  // the split of the views into concurrent groups, the persistent traversal
+  // grid (blocks per CU; its waves per CU are 4x that), and the far/near
+  // split of the ray queue (the order the traversal takes its rays in)
+  const uint32_t groups[] = {1, 2, 3, 4}, per_cu[] = {3, 4, 5};
+  const float far[] = {1.0f, 2.0f, 4.0f, 1e30f};
   float best = 0.0f;
   uint32_t best_g = b->ngroups, best_c = b->blocks_per_cu, tried = 0;
+  float best_far = b->far_ray;
   int rc;
   for (uint32_t g : groups) {
     if (g > b->ctxs.size()) break;
     for (uint32_t c : per_cu) {
-      if ((rc = set_groups(b, g))) return rc;
-      if (c != b->blocks_per_cu) drop_graph(b);
-      b->blocks_per_cu = c;
-      float tot = 0.0f;
-      for (uint32_t r = 0; r < repeats + 1; ++r) {  // the first render captures the graph
-        if ((rc = ct_batch_render(b, params, 1, nullptr))) return rc;
-        ct_frame_stats s;
-        if ((rc = ct_batch_get_stats(b, &s))) return rc;
-        if (r > 0) tot += s.ms_total;
-      }
-      ++tried;
-      const float ms = tot / (float)repeats;
-      if (best == 0.0f || ms < best) {
-        best = ms;
-        best_g = b->ngroups;
-        best_c = c;
+      for (float fr : far) {
+        if ((rc = set_groups(b, g))) return rc;
+        if (c != b->blocks_per_cu) drop_graph(b);
+        b->blocks_per_cu = c;
+        b->far_ray = fr;
+        float tot = 0.0f;
+        for (uint32_t r = 0; r < repeats + 1; ++r) {  // the first render captures the graph
+          if ((rc = ct_batch_render(b, params, 1, nullptr))) return rc;
+          ct_frame_stats s;
+          if ((rc = ct_batch_get_stats(b, &s))) return rc;
+          if (r > 0) tot += s.ms_total;
+        }
+        ++tried;
+        const float ms = tot / (float)repeats;
+        if (best == 0.0f || ms < best) {
+          best = ms;
+          best_g = b->ngroups;
+          best_c = c;
+          best_far = fr;
+        }
       }
     }
   }
   if ((rc = set_groups(b, best_g))) return rc;
   b->blocks_per_cu = best_c;
+  b->far_ray = best_far;
   drop_graph(b);
   if (out) {
     out->blocks_per_cu = best_c;
     out->ms = best;
     out->candidates = tried;
     out->groups = best_g;
+    out->far_ray = best_far;
   }
   return CT_OK;
 }

@@ -78,7 +78,7 @@ struct PtTrace {
   TraceCnt cnt;
   uint32_t bounce_rays, shadow_rays;
   bool overflow;
-  uint32_t px, slot, bits, emitted, far;  // far: emitted rays of at least kFarRay
+  uint32_t px, slot, bits, emitted, far;  // far: emitted rays of at least F.far_ray
   // Hybrid retraces of temporal path reuse: which of the pixel's two (0: the
   // history shifted to this pixel's primary hit, 1: this pixel's path
   // shifted to the previous pixel's), the previous pixel, and the retraces
@@ -320,7 +320,7 @@ CT_D bool vertex_visible(const DFrame &F, F3 o, F3 d, float dist, uint32_t ti, u
     F.ray_d[idx] = make_float4(d.x, d.y, d.z, __uint_as_float(ti));
     F.ray_t[idx] = tp;
     tr.emitted |= 1u << k;
-    if (dist >= kFarRay) tr.far |= 1u << k;
+    if (dist >= F.far_ray) tr.far |= 1u << k;
     return true;
   }
   return ((tr.bits >> k) & 1u) == 0u;

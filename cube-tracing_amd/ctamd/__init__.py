@@ -235,7 +235,7 @@ class AdjointDesc(C.Structure):
 
 
 class BatchTuning(C.Structure):
-    _fields_ = [("blocks_per_cu", u32), ("ms", f32), ("candidates", u32), ("groups", u32)]
+    _fields_ = [("blocks_per_cu", u32), ("ms", f32), ("candidates", u32), ("groups", u32), ("far_ray", f32)]
 
 
 class SceneDesc(C.Structure):
@@ -260,7 +260,7 @@ STRUCT_SIZES = {"ct_aabb": (Aabb, 24), "ct_instance": (Instance, 72), "ct_primit
                 "ct_material": (Material, 84), "ct_light": (Light, 40), "ct_light_config": (LightConfig, 52),
                 "ct_reservoir": (Reservoir, 36), "ct_di_info": (DiInfo, 60), "ct_path_reservoir": (PathReservoir, 92),
                 "ct_restir_params": (RestirParams, 108), "ct_adjoint_view": (AdjointView, 136),
-                "ct_adjoint_desc": (AdjointDesc, 16), "ct_batch_tuning": (BatchTuning, 16),
+                "ct_adjoint_desc": (AdjointDesc, 16), "ct_batch_tuning": (BatchTuning, 20),
                 "ct_task": (Task, 84), "ct_world": (World, 64), "ct_vox_desc": (VoxDesc, 32)}
 
 

@@ -646,15 +646,18 @@ int ct_batch_get_stats(ct_batch *batch, ct_frame_stats *out);
  * each candidate shape and keeps the fastest; reports the choice. A shape
  * is the number of view groups (1 to 4: contiguous ranges of views of
  * about equal pixel count, each with its own queues, run on concurrent
- * streams so one group's traversal tail overlaps another's passes) and
- * the grid of the persistent traversal kernels (4 or 5 blocks per CU; a
- * grid never exceeds the blocks that can be resident at once). A new
- * batch starts with 2 groups and 5 blocks per CU. */
+ * streams so one group's traversal tail overlaps another's passes), the
+ * grid of the persistent traversal kernels (3, 4 or 5 blocks per CU; a
+ * grid never exceeds the blocks that can be resident at once) and the ray
+ * queue's order (shadow rays at least far_ray long are queued and traced
+ * first: 1, 2, 4 world units or none): 48 shapes.  No shape changes a
+ * result.  A new batch starts with 2 groups, 5 blocks per CU and 2 units. */
 typedef struct ct_batch_tuning {
   uint32_t blocks_per_cu;
   float ms;            /* mean batch time of the choice */
   uint32_t candidates; /* number of shapes tried */
   uint32_t groups;     /* view groups of the choice */
+  float far_ray;       /* the queue's far/near split of the choice */
 } ct_batch_tuning;
 int ct_batch_autotune(ct_batch *batch, const ct_frame_params *params, uint32_t repeats,
                       ct_batch_tuning *out);

@@ -70,15 +70,17 @@ def test_batch_equals_per_view(gpu, use_graph):
 
 
 def test_batch_autotune_keeps_results(gpu):
-    """The tuner tries 1-4 view groups (concurrent streams) x 4-5 blocks/CU;
-    with its choice, a batch still gives each view's bits."""
+    """The tuner tries 1-4 view groups (concurrent streams) x 3-5 blocks/CU
+    x 4 far/near splits of the ray queue; with its choice, a batch still
+    gives each view's bits."""
     scenes, views, built = _views()
     ctxs = [_ctx(*built[k], w) for k, w in views]
     plain = [_ctx(*built[k], w) for k, w in views]
     b = ctamd.Batch(ctxs, [built[k][1] for k, _ in views])
     p0 = [ctamd.frame_params(w, 0) for _, w in views]
     t = b.autotune(p0, repeats=1)
-    assert t.blocks_per_cu in (4, 5) and t.groups in (1, 2, 3, 4) and t.ms > 0 and t.candidates == 8
+    assert t.blocks_per_cu in (3, 4, 5) and t.groups in (1, 2, 3, 4) and t.ms > 0 and t.candidates == 48
+    assert t.far_ray in (np.float32(1.0), np.float32(2.0), np.float32(4.0), np.float32(1e30))
     # the tuner rendered frame 0 twice per candidate; give the plain views the same history
     for _ in range(2 * t.candidates):
         for (k, w), ctx, p in zip(views, plain, p0):
