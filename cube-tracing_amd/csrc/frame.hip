@@ -582,8 +582,15 @@ CT_D void ris_strategy(const DFrame &F, Res &target, uint32_t count, uint32_t ns
   }
 }
 
-// RIS, restir_resampling.glsl:19-255 (point, environment, cube strategies)
-CT_D Res ris(const DFrame &F, float confidence, F3 ray_d, Hit &hit, const Mat &mat, uint32_t &seed) {
+// Candidate counts of RIS (restir_resampling.glsl:50-65) that its BRDF
+// strategy's weights use.
+struct RisCounts {
+  uint32_t nee, n_brdf, cube_lights;
+};
+
+// RIS, restir_resampling.glsl:19-255: the point, environment and cube
+// strategies (the BRDF strategy follows in ris_brdf).
+CT_D Res ris(const DFrame &F, float confidence, F3 ray_d, Hit &hit, const Mat &mat, uint32_t &seed, RisCounts &rc) {
   const ct_restir_params &rp = F.rp;
   confidence = clampf(confidence, 0.0f, 1.0f);
   const uint32_t pc = F.cfg.point_light_count, ec = F.cfg.env_map_count;
@@ -593,6 +600,9 @@ CT_D Res ris(const DFrame &F, float confidence, F3 ray_d, Hit &hit, const Mat &m
   uint32_t n_cube = max(f2u_sat(gmin((float)rp.max_ris_cube_samples * (1.0f - confidence), (float)cc)), rp.min_ris_cube_samples);
   uint32_t n_brdf = F.cfg.brdf_count + n_env;
   uint32_t nee = n_point + n_cube;
+  rc.nee = nee;
+  rc.n_brdf = n_brdf;
+  rc.cube_lights = cc;
   const Shade sh = shade_of(ray_d, hit);
   Res r = res_init();
   if ((F.active & CT_RIS_POINT_LIGHT_BIT) && F.n_point > 0)
@@ -944,16 +954,82 @@ __global__ void __launch_bounds__(kBlock) k_primary(FrameSet FS, unsigned long l
   }
 }
 
+// The BRDF strategy of RIS, restir_resampling.glsl:190-244: brdf_count
+// directions from sample_material (light.glsl:232-270 with the Lambertian
+// callable, rcall_scatter.glsl:73-84, which also leaves the direction in
+// hit.scatter), each traced to its closest voxel in place (intersect(),
+// bounce.glsl:9-130; `stk` is the lane's LDS stack column).  A candidate
+// whose voxel emits has F = brdf * Le * G with G = geom_fact_sa at the hit
+// (NaN, inf and negative G taken as 0) and m_i = 1 / max(n_brdf * pdf * G +
+// nee * p_light * G, 1e-6), p_light = (1 / cube lights) / (6 VOXEL_EXTENT^2)
+// (sample_lights_pdf, light.glsl:272-293), and names the voxel's cube light
+// (PRIMITIVE.light_index); any other candidate has weight 0 and light -1.
+// All are CUBE candidates.  Returns whether the merge into `r` selected the
+// strategy's reservoir: then the reservoir skips the visibility pass.
+CT_D bool ris_brdf(const DFrame &F, Res &r, F3 ray_o, Hit &hit, const Mat &mat, const RisCounts &rc, uint32_t &seed,
+                   uint32_t *stk, uint32_t &rays) {
+  Res br = res_init();
+  for (uint32_t l = 0; l < F.cfg.brdf_count; ++l) {
+    float w_i = 0.0f;
+    F3 Fv = f3(0, 0, 0);
+    const F3 P = hit.P, n = hit.n;
+    const F3 wo = normalize(sub(ray_o, P));
+    uint32_t light_index = kInvalid;
+    const uint32_t current_seed = seed;
+    F3 wi = random_cosine_direction(seed, n);
+    if (normal_near_zero(wi)) wi = n;
+    const float mat_pdf = material_pdf(n, wo, wi);
+    hit.scatter = wi;
+    const F3 ro = compute_ray_origin(P, wi);
+    float bt = kMaxDistance;
+    uint32_t bi = kInvalid, bp = kInvalid;
+    bool ovf = false;
+    TraceCnt cnt;
+    traverse_mode<kTraceBounce, false>(F.S, ro, wi, kMaxDistance, kInvalid, kInvalid, bt, bi, bp, stk, kBlock, cnt,
+                                       ovf);
+    ++rays;
+    if (bi != kInvalid) {
+      const uint32_t gp = F.S.inst[bi].first + bp;
+      const Mat hm = ld_mat(F, F.prims[gp].material_index);
+      if (hm.emission.x > 0.0f || hm.emission.y > 0.0f || hm.emission.z > 0.0f) {
+        float t;
+        F3 pos, hn;
+        hit_from_ray(F, bi, bp, ro, wi, t, pos, hn);
+        float G = geom_fact_sa(P, pos, hn);
+        if (isnan(G) || isinf(G) || G < 0.0f) G = 0.0f;
+        const float light_pdf = (1.0f / (float)rc.cube_lights) / (kVoxelExtent * kVoxelExtent * 6.0f);
+        const float m_i = 1.0f / gmax((float)rc.n_brdf * mat_pdf * G + (float)rc.nee * light_pdf * G, 1e-6f);
+        const F3 brdf = evaluate_material(mat, n, wo, wi);
+        Fv = muls(mulv(brdf, hm.emission), G);
+        w_i = luminance(Fv) * m_i;
+        light_index = F.prims[gp].light_index;
+      }
+    }
+    update_reservoir(br, light_index, CT_LIGHT_CUBE, current_seed, Fv, w_i, 1.0f, seed);
+  }
+  return update_reservoir(r, br.Y, br.type, br.seed, br.F, br.W_sum, br.M, seed);
+}
+
+// An intermediate reservoir that skips the visibility pass (its BRDF
+// candidate was selected) carries these bits as its W_y from k_ris to
+// k_temporal_finish, which stores W_y = 0 (initialise_reservoir's value,
+// reservoir.glsl:17-25) in their place.  Every other reservoir carries +0
+// there until the pass.
+constexpr uint32_t kNoVisibilityPass = 0x80000000u;
+
 // Prepass after the primary hit: rchit (rchit.glsl:18-54), miss
 // (rmiss.glsl:19-29), motion vector, temporal gather, confidence and RIS.
 // EMIT for the RIS visibility ray.
 // Five waves per SIMD (at most 96 VGPRs, a few dwords spilled) run this
-// VALU-heavy pass faster than four without spills.
+// VALU-heavy pass faster than four without spills.  kBrdf: some frame of
+// the set has the RIS BRDF strategy on (its rays need an LDS stack).
+template <bool kBrdf>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_ris(FrameSet FS) {
+  __shared__ uint32_t stk[kBrdf ? (kStackLds > 0 ? kStackLds : 1) * kBlock : 1];
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
   const uint32_t px = blk_.y + threadIdx.x;
-  uint32_t emitted = 0, far = 0, ris_ray = 0;
+  uint32_t emitted = 0, far = 0, ris_ray = 0, brdf_rays = 0;
   if (px < F.npix) {
     const uint32_t x = px % F.W, y = px / F.W;
     float4 h = F.hit[px];
@@ -1021,10 +1097,16 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
       float predicted = clampf(prev.M / F.rp.max_influence_from_the_past, 0.0f, 1.0f);
       float confidence = (prev.W_y > 0.0f) ? predicted : 0.0f;
       di.conf = confidence;
-      Res r = ris(F, confidence, rd, hit, mat, di.seed);
+      RisCounts rc;
+      Res r = ris(F, confidence, rd, hit, mat, di.seed, rc);
+      bool needs_visibility = true;
+      if (kBrdf && (F.active & CT_RIS_BRDF_BIT) &&
+          ris_brdf(F, r, ro, hit, mat, rc, di.seed, stk + threadIdx.x, brdf_rays))
+        needs_visibility = false;
       Vis v{true, px, 0u, 0u, 0u, 0u};
       Res rv = r;
-      visibility_pass(F, rv, hit, v);
+      if (needs_visibility) visibility_pass(F, rv, hit, v);
+      else r.W_y = __uint_as_float(kNoVisibilityPass);
       ris_ray = v.emitted;
       if (prev.W_y > 0.0f) {
         // The temporal reuse rays (slots 1, 2) go into the same launch; the
@@ -1041,6 +1123,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     F.mask[px] = 0u;
   }
   wave_add(ris_ray, &F.counters[kCntRis]);
+  if (kBrdf) wave_add(brdf_rays, &F.counters[kCntPtBounce]);
   queue_flush(F, px, emitted, far, -1);
 }
 
@@ -1063,7 +1146,8 @@ __global__ void __launch_bounds__(kBlock) k_temporal_finish(FrameSet FS) {
       const Hit hit = hit_from_di(di);
       Res r = ld_res(F.rA_i, F.rB_i, F.rC_i, px);
       Vis v{false, px, 0u, F.mask[px], 0u, 0u};
-      visibility_pass(F, r, hit, v);
+      if (__float_as_uint(r.W_y) == kNoVisibilityPass) r.W_y = 0.0f;
+      else visibility_pass(F, r, hit, v);
       const uint32_t pidx = F.pidx[px];
       Res prev = res_init();
       if (pidx < F.W * F.H) prev = ld_res(F.rA_p, F.rB_p, F.rC_p, pidx);
@@ -1776,7 +1860,6 @@ int check_frame(const Ctx *c, const Accel *a, const ct_frame_params *p) {
   if (!c || !a || !p) return fail(CT_E_INVALID, "ct_frame_render: null argument");
   if (p->width == 0 || p->height == 0 || p->width > c->max_w || p->height > c->max_h)
     return fail(CT_E_INVALID, "ct_frame_render: resolution exceeds the context");
-  if (p->is_active & CT_RIS_BRDF_BIT) return fail(CT_E_UNSUPPORTED, "ct_frame_render: BRDF sampling in RIS is not supported");
   if (p->restir.max_neighbors > CT_MAX_NEIGHBORS || p->restir.min_neighbors > CT_MAX_NEIGHBORS)
     return fail(CT_E_INVALID, "ct_frame_render: neighbour count above CT_MAX_NEIGHBORS");
   if ((p->is_active & CT_TAA_BIT) && !p->has_previous)
@@ -1917,6 +2000,7 @@ struct Launch {
   bool multi = false;          // queue entries carry the frame index
   bool prev_trace = false;     // some frame traces the previous set (DFrame::prev_set)
   bool hybrid = false;         // some frame's path reuse uses the hybrid shift
+  bool ris_brdf = false;       // some frame's RIS has the BRDF strategy on
   uint32_t *counters = nullptr;
   unsigned long long *stats = nullptr;
 };
@@ -1964,7 +2048,10 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark,
   else
     hipLaunchKernelGGL(k_primary<false>, dim3(L.blocks[0]), dim3(kBlock), 0, st, tiles, L.stats);
   if ((rc = mark())) return rc;  // kMarkPrimary
-  hipLaunchKernelGGL(k_ris, dim3(nb), dim3(kBlock), 0, st, pix);
+  if (L.ris_brdf)
+    hipLaunchKernelGGL(k_ris<true>, dim3(nb), dim3(kBlock), 0, st, pix);
+  else
+    hipLaunchKernelGGL(k_ris<false>, dim3(nb), dim3(kBlock), 0, st, pix);
   if ((rc = mark())) return rc;  // kMarkRis
   if (L.prev_trace) {
     // RIS and temporal reuse visibility; the target-at-previous queries of
@@ -2444,7 +2531,7 @@ int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, v
   for (uint32_t g = 0; g < ng; ++g) {
     BatchGroup &G = b->groups[g];
     uint32_t pt_depth = 0, max_nb = 0, px = 0;
-    bool prev_trace = false, hybrid = false;
+    bool prev_trace = false, hybrid = false, ris_brdf = false;
     for (uint32_t i = G.first; i < G.first + G.count; ++i) {
       DFrame &F = rec[i];
       if ((rc = prepare_frame(b->ctxs[i], b->accels[i], &params[i], st, F))) return rc;
@@ -2463,6 +2550,7 @@ int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, v
       px += F.npix;
       prev_trace = prev_trace || F.prev_set != 0u;
       hybrid = hybrid || (F.rp.indirect && F.rp.pt_shift_mapping == CT_SHIFT_HYBRID);
+      ris_brdf = ris_brdf || (F.active & CT_RIS_BRDF_BIT) != 0u;
     }
     L[g].fb = b->frames.as<DFrame>() + G.first;
     L[g].total_px = px;
@@ -2472,6 +2560,7 @@ int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, v
     L[g].multi = G.count > 1;
     L[g].prev_trace = prev_trace;
     L[g].hybrid = hybrid;
+    L[g].ris_brdf = ris_brdf;
     L[g].counters = G.counters.as<uint32_t>();
     L[g].stats = nullptr;
   }
@@ -2523,7 +2612,7 @@ int ct_batch_render(ct_batch *b, const ct_frame_params *params, int use_graph, v
     std::vector<uint32_t> gkey = tkey;
     for (uint32_t g = 0; g < ng; ++g)
       gkey.insert(gkey.end(), {L[g].total_px, L[g].pt_depth, L[g].max_neighbors, L[g].trace_blocks, (uint32_t)L[g].multi,
-                               (uint32_t)L[g].prev_trace, (uint32_t)L[g].hybrid});
+                               (uint32_t)L[g].prev_trace, (uint32_t)L[g].hybrid, (uint32_t)L[g].ris_brdf});
     if (!b->exec || gkey != b->graph_key) {
       drop_graph(b);
       CT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -2715,6 +2804,7 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
       L[k].multi = false;
       L[k].prev_trace = F.prev_set != 0u;
       L[k].hybrid = F.rp.indirect && F.rp.pt_shift_mapping == CT_SHIFT_HYBRID;
+      L[k].ris_brdf = (F.active & CT_RIS_BRDF_BIT) != 0u;
       L[k].counters = FP[k].counters;
       L[k].stats = nullptr;
     }
@@ -2766,6 +2856,7 @@ int ct_frame_render(ct_ctx *c, ct_accel *a, const ct_frame_params *p, void *stre
     L.multi = false;
     L.prev_trace = F.prev_set != 0u;
     L.hybrid = F.rp.indirect && F.rp.pt_shift_mapping == CT_SHIFT_HYBRID;
+    L.ris_brdf = (F.active & CT_RIS_BRDF_BIT) != 0u;
     L.counters = F.counters;
     L.stats = stats;
     const bool ev = (c->instrument & 1u) != 0;

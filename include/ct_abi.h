@@ -45,6 +45,10 @@ extern "C" {
 #define CT_RIS_POINT_LIGHT_BIT (1u << 1)
 #define CT_RIS_ENV_LIGHT_BIT (1u << 2)
 #define CT_RIS_CUBE_LIGHT_BIT (1u << 3)
+/* The BRDF strategy of RIS (restir_resampling.glsl:190-244): brdf_count
+ * cosine-distributed candidates per pixel, each traced to its closest voxel;
+ * a candidate selected last skips the visibility pass and keeps W_y = 0.
+ * Its rays count into ct_frame_stats.bounce_rays. */
 #define CT_RIS_BRDF_BIT (1u << 4)
 #define CT_REMAP_BIT (1u << 30)
 
@@ -589,7 +593,8 @@ typedef struct ct_frame_stats {
   float ms_taa;
   float ms_trace_any;       /* sum over the any-hit traversal launches */
   float ms_trace_closest;
-  uint64_t bounce_rays;     /* ReSTIR PT path segments (closest hit) */
+  uint64_t bounce_rays;     /* closest-hit rays of intersect(): ReSTIR PT path
+                               segments and the RIS BRDF strategy's rays */
   uint64_t shadow_rays_pt;  /* ReSTIR PT reconnection visibility queries */
   uint64_t pt_nodes;        /* path-segment traversal (counters on) */
   uint64_t pt_leaves;

@@ -37,6 +37,7 @@ namespace {
 constexpr uint32_t kLeaf = 0x80000000u;
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 constexpr float kHalfVoxel = 0.015625f;        // HALF_VOXEL_EXTENT, shared.inl:46
+constexpr float kVoxelExtent = 0.03125f;       // VOXEL_EXTENT, shared.inl:45
 constexpr float kLeafPad = 0.000244140625f;    // 2^-12 conservative node padding
 constexpr float kInvPi = 0.31830988618379067153776752674503f;   // shared.inl:52
 constexpr float kInv4Pi = 0.079577471545947667884441881686255f; // shared.inl:54
@@ -958,10 +959,16 @@ static void reservoir_visibility_pass(const Ctx &c, Res &r, const Hit &hit, Trac
   r.F = muls(r.F, visibility ? 1.0f : 0.0f);
 }
 
-// RIS, restir_resampling.glsl:19-255 (point, environment and cube
-// strategies; the BRDF strategy is rejected by the caller).
-static Res ris(const Ctx &c, float confidence, V3 ray_d, Hit &hit, const ct_material &mat,
-               uint32_t &seed, TraceCount &cnt) {
+static bool ris_brdf(const Ctx &c, Res &reservoir, V3 ray_o, Hit &hit, const ct_material &mat, uint32_t nee,
+                     uint32_t n_brdf, uint32_t cube_light_count, uint32_t &seed, TraceCount &bcnt);
+
+// RIS, restir_resampling.glsl:19-255: the point, environment and cube
+// strategies, the BRDF strategy with CT_RIS_BRDF_BIT (its closest-hit rays
+// count into bcnt.bounce), then the visibility pass unless the BRDF
+// strategy's candidate was the last one selected (:240-252).  ray_o, ray_d:
+// the camera ray.
+static Res ris(const Ctx &c, float confidence, V3 ray_o, V3 ray_d, Hit &hit, const ct_material &mat,
+               uint32_t &seed, TraceCount &cnt, TraceCount &bcnt) {
   const ct_restir_params &rp = c.p->restir;
   const ct_light_config &lc = c.sc->cfg;
   uint32_t active = c.p->is_active;
@@ -1015,7 +1022,11 @@ static Res ris(const Ctx &c, float confidence, V3 ray_d, Hit &hit, const ct_mate
     strategy(cr, cube_light_count, n_cube, CT_LIGHT_CUBE, c.sc->cube);
     update_reservoir(reservoir, cr.Y, cr.l_type, cr.seed, cr.F, cr.W_sum, cr.M, seed);
   }
-  reservoir_visibility_pass(c, reservoir, hit, cnt);
+  bool requires_visibility = true;
+  if ((active & CT_RIS_BRDF_BIT) &&
+      ris_brdf(c, reservoir, ray_o, hit, mat, nee, n_brdf, cube_light_count, seed, bcnt))
+    requires_visibility = false;
+  if (requires_visibility) reservoir_visibility_pass(c, reservoir, hit, cnt);
   return reservoir;
 }
 
@@ -1196,7 +1207,6 @@ static void pairwise_combine(const Ctx &c, Res &r, uint32_t num, const Res *nres
 constexpr float kHlfMax = 6.5504e+4f;        // shared.inl:56
 constexpr float kHlfMin = 1.175494351e-38f;  // shared.inl:57
 constexpr float kPiF = 3.1415926535897932384626433832795f;  // DAXA_PI, shared.inl:49
-constexpr float kVoxelExtent = 0.03125f;     // shared.inl:45
 
 // sin/cos of x in [0, 2*pi).  GLSL leaves their accuracy to the
 // implementation; this build fixes one: Cody-Waite reduction by pi/2 and
@@ -1412,6 +1422,52 @@ static V3 random_cosine_direction(uint32_t &seed, V3 normal) {
   u = normalize(cross(normal, u));
   V3 v = cross(normal, u);
   return add(add(muls(u, x), muls(v, y)), muls(normal, z));
+}
+
+// The BRDF strategy of RIS, restir_resampling.glsl:190-244: brdf_count
+// directions from sample_material (light.glsl:232-270 with the Lambertian
+// callable, rcall_scatter.glsl:73-84, which also leaves the direction in
+// hit.scatter_dir), each traced by intersect() (bounce.glsl:9-130).  A
+// candidate whose closest voxel emits has F = brdf * Le * G with G =
+// geom_fact_sa at the hit (NaN, inf and negative G taken as 0) and
+// m_i = 1 / max(n_brdf * pdf * G + nee * p_light * G, 1e-6), p_light =
+// (1 / cube lights) / (6 VOXEL_EXTENT^2) (sample_lights_pdf,
+// light.glsl:272-293), and names the voxel's cube light
+// (PRIMITIVE.light_index); any other candidate has weight 0 and light -1.
+// All are CUBE candidates.  Returns whether the merge into `reservoir`
+// selected the strategy's reservoir.
+static bool ris_brdf(const Ctx &c, Res &reservoir, V3 ray_o, Hit &hit, const ct_material &mat, uint32_t nee,
+                     uint32_t n_brdf, uint32_t cube_light_count, uint32_t &seed, TraceCount &bcnt) {
+  const Scene &sc = *c.sc;
+  Res br;
+  init_res(br);
+  for (uint32_t l = 0; l < sc.cfg.brdf_count; ++l) {
+    float w_i = 0.0f;
+    V3 F = v3(0, 0, 0);
+    const V3 P = hit.world_hit, n = hit.world_nrm;
+    const V3 wo = normalize(sub(ray_o, P));
+    uint32_t light_index = kInvalid;
+    const uint32_t current_seed = seed;
+    V3 m_wi = random_cosine_direction(seed, n);
+    if (std::fabs(m_wi.x) < 1e-8f && std::fabs(m_wi.y) < 1e-8f && std::fabs(m_wi.z) < 1e-8f) m_wi = n;
+    const float mat_pdf = sample_material_pdf(mat, n, wo, m_wi);
+    hit.scatter_dir = m_wi;
+    TraceCount tc;
+    const Isect i = intersect_bounce(c, P, m_wi, tc);
+    bcnt.bounce += tc.bounce;
+    if (i.is_hit && (i.mat.emission[0] > 0.0f || i.mat.emission[1] > 0.0f || i.mat.emission[2] > 0.0f)) {
+      float G = geom_fact_sa(P, i.P, i.n);
+      if (std::isnan(G) || std::isinf(G) || G < 0.0f) G = 0.0f;
+      const float light_pdf = (1.0f / (float)cube_light_count) / (kVoxelExtent * kVoxelExtent * 6.0f);
+      const float m_i = 1.0f / gmax((float)n_brdf * mat_pdf * G + (float)nee * light_pdf * G, 1e-6f);
+      const V3 brdf = evaluate_material(mat, n, wo, m_wi);
+      F = muls(mulv(brdf, load3(i.mat.emission)), G);
+      w_i = luminance(F) * m_i;
+      light_index = sc.prims[sc.inst[i.inst].first_primitive_index + i.prim].light_index;
+    }
+    update_reservoir(br, light_index, CT_LIGHT_CUBE, current_seed, F, w_i, 1.0f, seed);
+  }
+  return update_reservoir(reservoir, br.Y, br.l_type, br.seed, br.F, br.W_sum, br.M, seed);
 }
 
 struct PathBuilder {  // path_builder.glsl:8-17
@@ -2212,7 +2268,7 @@ static void prepass_pixel(const Ctx &c, uint32_t x, uint32_t y, TraceCount &cnt,
     float predicted = clampf(prev.M / p.restir.max_influence_from_the_past, 0.0f, 1.0f);
     float confidence = (prev.W_y > 0.0f) ? predicted : 0.0f;
     di.confidence = confidence;
-    Res r = ris(c, confidence, rd, hit, mat, di.seed, cnt);
+    Res r = ris(c, confidence, ro, rd, hit, mat, di.seed, cnt, ptc);
     if (prev.W_y > 0.0f) temporal_reuse(c, r, prev, pidx, rd, hit, mat, di.seed, cnt);
     fr.res_int[idx] = res_to(r);
     if (p.restir.indirect) {
@@ -2893,7 +2949,6 @@ int oc_frame_render(oc_frame *frame, const ct_frame_params *params, int threads,
   if (!frame || !params) return CT_E_INVALID;
   const ct_frame_params &p = *params;
   if (p.width != frame->W || p.height != frame->H) return CT_E_INVALID;
-  if (p.is_active & CT_RIS_BRDF_BIT) return CT_E_UNSUPPORTED;
   if (p.restir.max_neighbors > CT_MAX_NEIGHBORS || p.restir.min_neighbors > CT_MAX_NEIGHBORS)
     return CT_E_INVALID;
   if ((p.is_active & CT_TAA_BIT) && !p.has_previous) return CT_E_INVALID;
@@ -3038,15 +3093,15 @@ static Ctx probe_ctx(const oc_scene *scene, const ct_frame_params *p) {
 int oc_probe_ris(oc_scene *scene, const ct_frame_params *p, const ct_di_info *di, float confidence, uint32_t n,
                  const uint32_t *seeds, ct_reservoir *out) {
   if (!scene || !p || !di || (n && (!seeds || !out))) return CT_E_INVALID;
-  if (p->is_active & CT_RIS_BRDF_BIT) return CT_E_UNSUPPORTED;
   Ctx c = probe_ctx(scene, p);
   const ct_material &mat = material_or_zero(*scene, di->mat_index);
-  const V3 rd = normalize(sub(load3(di->position), load3(di->ray_origin)));
+  const V3 ro = load3(di->ray_origin);
+  const V3 rd = normalize(sub(load3(di->position), ro));
   for (uint32_t k = 0; k < n; ++k) {
     Hit hit = hit_from_di(*di);
     uint32_t seed = seeds[k];
-    TraceCount cnt;
-    out[k] = res_to(ris(c, confidence, rd, hit, mat, seed, cnt));
+    TraceCount cnt, bcnt;
+    out[k] = res_to(ris(c, confidence, ro, rd, hit, mat, seed, cnt, bcnt));
   }
   return CT_OK;
 }

@@ -78,7 +78,7 @@ def test_trace_any_bit_exact(gpu):
     assert 0 < g.mean() < 1
 
 
-def _frames_match(w, frames, scene=None):
+def _frames_match(w, frames, scene=None, check=None):
     sc, acc, ctx = gpu_scene(w, scene=scene)
     assert ctx.light_config().cube_light_count == sc.info.cube_light_count
     osc = oracle_scene(sc, acc)
@@ -108,6 +108,8 @@ def _frames_match(w, frames, scene=None):
         assert gs.shadow_rays == os_.shadow_rays, (f, gs.shadow_rays, os_.shadow_rays)
         assert gs.bounce_rays == os_.bounce_rays, (f, gs.bounce_rays, os_.bounce_rays)
         assert gs.shadow_rays_pt == os_.shadow_rays_pt, (f, gs.shadow_rays_pt, os_.shadow_rays_pt)
+        if check:
+            check(f, ctx, ofr)
     return sc
 
 
@@ -121,6 +123,31 @@ def test_frame_parity_reference_mode_taa(gpu):
 
 def test_frame_parity_chunked_orbit(gpu):
     _frames_match(ctamd.hd_di(width=96, height=64, grid=96, chunk=32, frames=3), 3)
+
+
+@pytest.mark.parametrize("kind", ["direct", "pt"])
+def test_frame_parity_ris_brdf(gpu, kind):
+    """RIS with its BRDF strategy (CT_RIS_BRDF_BIT, restir_resampling.glsl:
+    190-252): a closest-hit ray per shaded pixel inside the RIS pass, every
+    buffer and ray count bit for bit against the oracle.  A fifth of the
+    voxels emit (many lights make the strategy's candidates competitive), so
+    frame 0 (no temporal history) holds reservoirs whose BRDF candidate won
+    the merge: W_y = 0 with F > 0, no visibility pass."""
+    w = ctamd.hd_di(width=96, height=64, grid=96, chunk=32, frames=3, indirect=kind == "pt")
+    w.emissive *= 10
+    w.features |= ctamd.CT_RIS_BRDF_BIT
+
+    def check(f, ctx, ofr):
+        st = ctx.stats()
+        hits = int((di_array(ofr.read(ctamd.CT_BUF_DI))[:, 3].view(np.float32) > 0).sum())
+        if not w.restir.indirect:
+            assert st.bounce_rays == hits  # brdf_count = 1 ray per shaded pixel
+        if f == 0:
+            r = res_array(ctx.read(ctamd.CT_BUF_INT_RESERVOIR, w.width, w.height))
+            wy, fr = r[:, 3].view(np.float32), r[:, 6:9].view(np.float32)
+            assert ((wy == 0) & (fr.max(axis=1) > 0)).sum() > 10
+
+    _frames_match(w, w.frames, check=check)
 
 
 @pytest.mark.parametrize("grid,chunk", [(64, 64), (96, 32)])
@@ -163,12 +190,15 @@ def test_frame_parity_pt_variants(gpu, kw):
     _frames_match(_pt_workload(**kw), 3)
 
 
-def test_split_frame_equals_whole_frame(gpu):
+@pytest.mark.parametrize("brdf", [False, True], ids=["nee", "ris_brdf"])
+def test_split_frame_equals_whole_frame(gpu, brdf):
     """A frame of 2^20 pixels or more runs as two halves (even and odd tile
     rows) on two streams by default; it gives the bits of the same frame run
     whole on one stream (per-pass events on). The height leaves a partial
     last tile row."""
     w = ctamd.hd_di(width=1024, height=1032, grid=64, chunk=32, frames=4, indirect=True, max_depth=2)
+    if brdf:
+        w.features |= ctamd.CT_RIS_BRDF_BIT
     sc, acc, whole = gpu_scene(w)
     split = ctamd.Ctx(w.width, w.height)
     ctamd.bind_scene(sc, split)
