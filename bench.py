@@ -228,6 +228,31 @@ def batched_bench(frames=5, warmup=2, n_views=64):
             "tuned_far_ray": tune.far_ray, "tuner_candidates": tune.candidates}
 
 
+def frame_adjoint_bench(w, acc, ctx, sc, f, reps=10):
+    """The frame adjoint (ct_frame_adjoint) on the 1080p workload: one frame
+    rendered, then the gradient of sum(g * shading output) with its final
+    reservoirs held fixed, with respect to every material's albedo and
+    emission (one material per voxel), timed over `reps` calls."""
+    import torch
+    import ctamd
+    from ctamd.adjoint import frame_adjoint
+    p = ctamd.frame_params(w, f)
+    ctx.render(acc, p)
+    ctx.stats()
+    nm = sc.info.material_count
+    g = torch.randn((w.width * w.height, 4), dtype=torch.float32, device="cuda")
+    gd, ge = frame_adjoint(ctx, acc, p, g, nm)  # untimed: allocation of the gradients
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        gd, ge = frame_adjoint(ctx, acc, p, g, nm)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t1) / reps
+    return {"workload": w.name, "materials": nm, "pixels": w.width * w.height, "ms_per_call": round(1000 * t, 3),
+            "nonzero_albedo_grads": int(torch.count_nonzero(gd).item()),
+            "note": "includes zeroing the two [materials][3] gradient tensors and a device sync per call"}
+
+
 def adjoint_bench(steps=3, fit_steps=200):
     """Inverse rendering on 128^3 voxels, 8 views of 512^2 at 16 spp, 3
     bounces: forward render vs forward + path-replay adjoint + Adam step
@@ -767,7 +792,11 @@ def main():
     if not args.no_batched:
         batched = batched_bench()
         log(f"batched: {batched}")
+    frame_adj = None
     if not args.no_adjoint:
+        frame_adj = frame_adjoint_bench(w, acc, ctx, sc, f)
+        f += 1
+        log(f"frame adjoint: {frame_adj}")
         adjoint = adjoint_bench()
         log(f"adjoint: {adjoint}")
     vox = None
@@ -809,6 +838,7 @@ def main():
         "c0": c0,
         "batched": batched,
         "adjoint": adjoint,
+        "frame_adjoint": frame_adj,
         "build": build,
         "vox": vox,
         "path_trace": ptrace,

@@ -275,6 +275,12 @@ struct Ctx {
   uint32_t split_counts[kMaxParts][3] = {};
   uint32_t split_px[kMaxParts] = {};
   uint32_t nparts = 0;  // parts of the last frame (0: it ran whole)
+  // The last frame's record (its final DI and reservoirs stay in fb until
+  // the next frame), for ct_frame_adjoint: valid while the accel keeps the
+  // build it was rendered with.
+  std::vector<unsigned char> last_frame;
+  const void *last_frame_accel = nullptr;
+  uint32_t last_frame_generation = 0;
 };
 
 // Kernel-side views used by trace.hip and frame.hip.

@@ -1855,6 +1855,72 @@ int upload_lights(Ctx &c, DevBuf &dst, const std::vector<ct_light> &src) {
   return CT_OK;
 }
 
+// ------------------------------------------------------------ frame adjoint
+// The frame's shading output with its final reservoirs held fixed, as a
+// function of the materials' diffuse albedo d and emission e: a shaded
+// pixel shows F_y W_y + I_ind + e[m_px] (e with direct emittance), where
+// F_y = brdf(d[m_px]) * Le * G replays the final reservoir's light sample
+// from its seed (reservoir_get_radiance, reservoir.glsl:98-112) with the
+// visibility the frame found, and Le is the emission of the light voxel's
+// material (a cube light; other lights keep their own).  W_y and the
+// indirect term I_ind are held.  F_y is linear in d and in Le, so its
+// partials are F_y re-evaluated with a unit albedo or a unit Le.  A channel
+// whose F_y W_y + e is not finite (the frame's NaN scrub) has no gradient.
+// Adds g . dI/dd into gd[m_px] and g . dI/de into ge[m_light] and ge[m_px].
+__global__ void __launch_bounds__(kBlock) k_frame_adjoint(const DFrame *Fp, const float4 *grad_img, float *gd,
+                                                          float *ge) {
+  const DFrame &F = *Fp;
+  const uint32_t px = blockIdx.x * kBlock + threadIdx.x;
+  if (px >= F.npix) return;
+  if (!(F.dpos_p[px].w > 0.0f)) return;
+  const DI di = ld_di(F.dpos_p, F.dnrm_p, F.dorg_p, F.dhit_p, px);
+  if (di.mat >= F.n_mats) return;
+  const Res r = ld_res(F.rA_p, F.rB_p, F.rC_p, px);
+  const Mat mat = ld_mat(F, di.mat);
+  const float4 g = grad_img[px];
+  F3 Fv = f3(0, 0, 0), Fd = f3(0, 0, 0), Fe = f3(0, 0, 0);
+  uint32_t ml = kInvalid;
+  bool lit = false;
+  PLight l;
+  if (r.W_y != 0.0f && !(r.F.x == 0.0f && r.F.y == 0.0f && r.F.z == 0.0f) && res_valid(r) &&
+      light_by_type(F, r.type, r.Y, l)) {
+    if (l.type == CT_LIGHT_CUBE && l.inst < F.S.n_inst) {
+      const DInst &I = F.S.inst[l.inst];
+      if (l.prim < I.count) ml = F.prims[I.first + l.prim].material_index;
+      if (ml >= F.n_mats) ml = kInvalid;
+    }
+    const F3 ray_d = normalize(sub(di.pos, di.org));
+    auto eval = [&](F3 d, F3 le) {
+      Mat m = mat;
+      m.diffuse = d;
+      PLight L = l;
+      L.emissive = le;
+      Hit hit = hit_from_di(di);
+      const Shade sh = shade_of(ray_d, hit);
+      float pdf_out, G;
+      uint32_t s = r.seed;
+      return sampled_light(F, sh, hit, m, L, 1.0f, pdf_out, G, s, false, false, false, nullptr);
+    };
+    Fv = eval(mat.diffuse, l.emissive);
+    Fd = eval(f3(1.0f, 1.0f, 1.0f), l.emissive);
+    if (ml != kInvalid) Fe = eval(mat.diffuse, f3(1.0f, 1.0f, 1.0f));
+    lit = true;
+  }
+  const bool emit = F.rp.direct_emittance != 0u;
+  const float gc[3] = {g.x, g.y, g.z}, fv[3] = {Fv.x, Fv.y, Fv.z}, fd[3] = {Fd.x, Fd.y, Fd.z},
+              fe[3] = {Fe.x, Fe.y, Fe.z}, em[3] = {mat.emission.x, mat.emission.y, mat.emission.z};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float v = (lit ? fv[k] * r.W_y : 0.0f) + (emit ? em[k] : 0.0f);
+    if (isinf(v) || isnan(v)) continue;
+    if (lit) {
+      atomicAdd(&gd[3 * (size_t)di.mat + k], gc[k] * (fd[k] * r.W_y));
+      if (ml != kInvalid) atomicAdd(&ge[3 * (size_t)ml + k], gc[k] * (fe[k] * r.W_y));
+    }
+    if (emit) atomicAdd(&ge[3 * (size_t)di.mat + k], gc[k]);
+  }
+}
+
 // ------------------------------------------------------------ frame plans
 int check_frame(const Ctx *c, const Accel *a, const ct_frame_params *p) {
   if (!c || !a || !p) return fail(CT_E_INVALID, "ct_frame_render: null argument");
@@ -1968,6 +2034,9 @@ int prepare_frame(Ctx *c, Accel *a, const ct_frame_params *p, hipStream_t st, DF
   F.taa_w = p->restir.taa_truncate_dispatch ? (F.W / 8) * 8 : F.W;
   F.taa_h = p->restir.taa_truncate_dispatch ? (F.H / 8) * 8 : F.H;
   F.far_ray = kFarRay;
+  c->last_frame.assign(reinterpret_cast<const unsigned char *>(&F), reinterpret_cast<const unsigned char *>(&F + 1));
+  c->last_frame_accel = a;
+  c->last_frame_generation = a->set.generation;
   return CT_OK;
 }
 
@@ -2968,6 +3037,29 @@ int ct_pt_random_replay(ct_ctx *c, ct_accel *a, const ct_frame_params *p, int32_
   hipLaunchKernelGGL(k_pt_replay, dim3(nblocks(F.npix)), dim3(kBlock), 0, st, F, dx, dy, out);
   CT_HIP(hipGetLastError());
   CT_HIP(hipStreamSynchronize(st));
+  return CT_OK;
+}
+
+int ct_frame_adjoint(ct_ctx *c, ct_accel *a, const ct_frame_params *p, const float *grad_image, float *grad_diffuse,
+                     float *grad_emission, void *stream) {
+  if (!c || !a || !p || !grad_image || !grad_diffuse || !grad_emission)
+    return fail(CT_E_INVALID, "ct_frame_adjoint: null argument");
+  if (c->last_frame.size() != sizeof(DFrame) || c->last_frame_accel != a ||
+      c->last_frame_generation != a->set.generation)
+    return fail(CT_E_STATE, "ct_frame_adjoint: no frame rendered with this accel's current build");
+  DFrame F;
+  std::memcpy(&F, c->last_frame.data(), sizeof(F));
+  if (F.W != p->width || F.H != p->height || F.n_mats != (uint32_t)c->mats_host.size())
+    return fail(CT_E_INVALID, "ct_frame_adjoint: parameters differ from the last frame's");
+  CT_HIP(hipSetDevice(c->device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  int rc;
+  if ((rc = c->frame_dev.reserve(sizeof(DFrame)))) return rc;
+  // the record goes through a kernel argument copy, ordered on the stream
+  hipLaunchKernelGGL(k_store_frame, dim3(1), dim3(64), 0, st, F, c->frame_dev.as<DFrame>());
+  hipLaunchKernelGGL(k_frame_adjoint, dim3(nblocks(F.npix)), dim3(kBlock), 0, st, c->frame_dev.as<DFrame>(),
+                     reinterpret_cast<const float4 *>(grad_image), grad_diffuse, grad_emission);
+  CT_HIP(hipGetLastError());
   return CT_OK;
 }
 

@@ -320,6 +320,8 @@ _SIGS = {
     "ct_frame_read": ([C.c_void_p, u32, C.c_void_p, C.c_size_t], C.c_int),
     "ct_ctx_set_instrumentation": ([C.c_void_p, u32], C.c_int),
     "ct_frame_get_stats": ([C.c_void_p, C.POINTER(FrameStats)], C.c_int),
+    "ct_frame_adjoint": ([C.c_void_p, C.c_void_p, C.POINTER(FrameParams), C.c_void_p, C.c_void_p, C.c_void_p,
+                          C.c_void_p], C.c_int),
     "ct_batch_create": ([u32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)], C.c_int),
     "ct_batch_destroy": ([C.c_void_p], C.c_int),
     "ct_batch_render": ([C.c_void_p, C.POINTER(FrameParams), C.c_int, C.c_void_p], C.c_int),

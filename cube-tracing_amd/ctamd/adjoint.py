@@ -1,6 +1,8 @@
 """Differentiable voxel rendering: a torch.autograd.Function over
 ct_adjoint_render / ct_adjoint_backward (path-replay adjoint, O(1) memory
-per path), and the inverse-rendering workload on a 128^3 grid.
+per path), and the inverse-rendering workload on a 128^3 grid; and
+frame_adjoint, the gradient of a rendered ReSTIR frame with its final
+reservoirs held fixed (ct_frame_adjoint).
 
 Parameters are per voxel: albedo [slots, 3] and scalar emission [slots],
 indexed by the accel's primitive slot (``VoxelSlots`` maps between the
@@ -18,7 +20,7 @@ import numpy as np
 from . import (AdjointDesc, AdjointView, Accel, Scene, SceneDesc, CT_VOXEL_EXTENT, camera_matrices, check, lib)
 
 __all__ = ["AdjointWorkload", "inverse_rendering_c4", "c4_problem", "fit", "orbit_views", "VoxelSlots", "DiffRenderer", "render",
-           "build_scene"]
+           "build_scene", "frame_adjoint"]
 
 
 def orbit_views(n, width, height, center, radius, height_offset, fov_radians=math.radians(60.0), start_deg=0.0):
@@ -115,6 +117,25 @@ def path_trace(accel: Accel, views, materials, spp=16, max_depth=3, seed=0xC4):
     check(lib().ct_path_trace(accel.handle, C.byref(desc), vv, mm, nm, _ptr(img),
                               C.c_void_p(torch.cuda.current_stream().cuda_stream)))
     return img
+
+
+def frame_adjoint(ctx, accel: Accel, params, grad_image, n_materials):
+    """Gradients of sum(grad_image * shading output) of the frame `ctx` last
+    rendered with `params`, its final reservoirs held fixed
+    (ct_frame_adjoint), with respect to every material's diffuse albedo and
+    emission.  grad_image: a float32 device tensor [H*W][4].  Returns
+    (grad_diffuse, grad_emission), float32 device tensors [n_materials][3],
+    on the current torch stream."""
+    import torch
+    g = grad_image.contiguous()
+    if g.dtype != torch.float32 or g.numel() != 4 * params.width * params.height:
+        raise ValueError("grad_image must be float32 [H*W][4]")
+    gd = torch.zeros((n_materials, 3), dtype=torch.float32, device=g.device)
+    ge = torch.zeros((n_materials, 3), dtype=torch.float32, device=g.device)
+    torch.cuda.synchronize()  # the frame ran on the context's own stream
+    check(lib().ct_frame_adjoint(ctx.handle, accel.handle, C.byref(params), _ptr(g), _ptr(gd), _ptr(ge),
+                                  C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    return gd, ge
 
 
 def _function():

@@ -609,6 +609,26 @@ typedef struct ct_frame_stats {
  * and end events for ms_total (4). */
 int ct_ctx_set_instrumentation(ct_ctx *ctx, uint32_t flags);
 int ct_frame_get_stats(ct_ctx *ctx, ct_frame_stats *out);
+/* Adjoint of the last rendered frame with its final reservoirs held fixed
+ * (the checkpoint the frame leaves in its buffers).  No counterpart in the
+ * reference (its renderer has no derivatives).  The function
+ * differentiated is the frame's shading output (the image, or with TAA the
+ * TAA input) as a function of every material's diffuse albedo d and
+ * emission e: a shaded pixel shows F_y W_y + I_ind + e[m_px] (e with
+ * direct emittance), where F_y = brdf(d[m_px]) * Le * G replays the final
+ * reservoir's light sample from its seed with the visibility the frame
+ * found, Le is the emission of the light voxel's material for a cube light
+ * (a point or environment light keeps its own), and W_y and the indirect
+ * term I_ind are held fixed (a biased gradient; the resampling weights are
+ * not differentiated).  Channels the frame's NaN scrub zeroes have no
+ * gradient.  grad_image: device [H][W][4] (dLoss/dimage, alpha ignored);
+ * the gradients are added into grad_diffuse and grad_emission, device
+ * [materials][3] (zero them first).  Call after ct_frame_render (or a batch
+ * render of this context) with the same params, before the context renders
+ * again and before the accel or the materials change (CT_E_STATE once the
+ * accel has a newer build).  Ordered on `stream` (null: the context's). */
+int ct_frame_adjoint(ct_ctx *ctx, ct_accel *accel, const ct_frame_params *params, const float *grad_image,
+                     float *grad_diffuse, float *grad_emission, void *stream);
 /* Random replay (trace_random_replay_path_hybrid_simple,
  * path_tracer.glsl:889-966), whose only caller in the reference is
  * commented out (indirect_illumination.glsl:301-302): after a frame with

@@ -3039,6 +3039,142 @@ int oc_frame_render(oc_frame *frame, const ct_frame_params *params, int threads,
   return CT_OK;
 }
 
+// The frame's direct light with its final reservoirs held fixed, as a
+// function of the materials' diffuse albedo d[m] and emission e[m]: pixel
+// p with shading material m_p and final reservoir (y, W_y) shows
+//   I_p = F_y(d[m_p], Le) * W_y + I_ind,p + e[m_p]   (e: with direct emittance)
+// where F_y = brdf * Le * G replays the reservoir's light sample from its
+// seed (reservoir_get_radiance, reservoir.glsl:98-112) with the visibility
+// the frame found, Le = e[material of the light's voxel] for a cube light
+// (a point or environment light keeps its own), and I_ind is the frame's
+// indirect term, held fixed.  A channel whose F_y W_y + e is not finite
+// (the frame's NaN scrub, rgen.glsl:445-450) shows 0 and has no gradient.
+// The image is the shading output, before TAA (the frame's TAA input when
+// TAA runs).
+struct DiTerm {
+  bool lit = false;          // a visible sample with W_y != 0
+  uint32_t m_px = kInvalid;  // material of the shading voxel
+  uint32_t m_light = kInvalid;
+  V3 F{0, 0, 0}, dF_dd{0, 0, 0}, dF_de{0, 0, 0};  // F_y, dF_y/d d[m_px], dF_y/d Le (per channel)
+  float W_y = 0.0f;
+};
+static uint32_t light_material(const Scene &sc, const ct_light &l) {
+  if (l.type != CT_LIGHT_CUBE || l.instance_info.instance_id >= sc.inst.size()) return kInvalid;
+  const uint64_t slot = (uint64_t)sc.inst[l.instance_info.instance_id].first_primitive_index + l.instance_info.primitive_id;
+  return slot < sc.prims.size() ? sc.prims[slot].material_index : kInvalid;
+}
+// `emission` null: the light's own emissive as Le (the adjoint's point, the
+// frame's values).
+static DiTerm di_term(const Ctx &c, uint32_t idx, const float *diffuse, const float *emission) {
+  const Scene &sc = *c.sc;
+  const Frame &fr = *c.fr;
+  DiTerm t;
+  const ct_di_info &di = fr.di_prev[idx];
+  t.m_px = di.mat_index;
+  const Res r = res_from(fr.res_prev[idx]);
+  t.W_y = r.W_y;
+  const V3 Fs = r.F;
+  if (r.W_y == 0.0f || (Fs.x == 0.0f && Fs.y == 0.0f && Fs.z == 0.0f) || !res_valid(r)) return t;
+  const ct_light *l = light_by_type(c, r.l_type, r.Y);
+  if (!l || t.m_px >= sc.mats.size()) return t;
+  t.m_light = light_material(sc, *l);
+  const V3 ray_d = normalize(sub(load3(di.position), load3(di.ray_origin)));
+  auto eval = [&](V3 dv, bool own_le, V3 le) {
+    ct_material m = sc.mats[t.m_px];
+    store3(m.diffuse, dv);
+    ct_light L = *l;
+    if (!own_le) store3(L.emissive, le);
+    Hit hit = hit_from_di(di);
+    float pdf_out, G;
+    uint32_t s = r.seed;
+    TraceCount cnt;
+    return calculate_sampled_light(c, ray_d, hit, m, L, 1.0f, pdf_out, G, s, false, false, false, cnt);
+  };
+  const bool own = t.m_light == kInvalid || !emission;
+  const V3 d = load3(&diffuse[3 * (size_t)t.m_px]);
+  const V3 le = own ? load3(l->emissive) : load3(&emission[3 * (size_t)t.m_light]);
+  t.F = eval(d, own, le);
+  t.dF_dd = eval(v3(1, 1, 1), own, le);
+  t.dF_de = t.m_light == kInvalid ? v3(0, 0, 0) : eval(d, false, v3(1, 1, 1));
+  t.lit = true;
+  return t;
+}
+static Ctx reshade_ctx(oc_frame *frame, const ct_frame_params *params) {
+  Ctx c;
+  c.sc = frame->sc;
+  c.fr = frame;
+  c.p = params;
+  return c;
+}
+
+int oc_frame_reshade(oc_frame *frame, const ct_frame_params *params, const float *diffuse, const float *emission,
+                     uint32_t n_mats, float *image) {
+  if (!frame || !params || !diffuse || !emission || !image) return CT_E_INVALID;
+  if (params->width != frame->W || params->height != frame->H || n_mats != frame->sc->mats.size())
+    return CT_E_INVALID;
+  const Ctx c = reshade_ctx(frame, params);
+  const ct_frame_params &p = *params;
+  const size_t np = (size_t)frame->W * frame->H;
+  for (size_t idx = 0; idx < np; ++idx) {
+    const ct_di_info &di = frame->di_prev[idx];
+    float *o = &image[4 * idx];
+    if (!(di.distance > 0.0f)) {
+      const float *src = (p.is_active & CT_TAA_BIT) ? &frame->taa[4 * idx] : &frame->img[frame->cur][4 * idx];
+      for (int k = 0; k < 4; ++k) o[k] = src[k];
+      continue;
+    }
+    const DiTerm t = di_term(c, (uint32_t)idx, diffuse, emission);
+    V3 hv = v3(0, 0, 0);
+    if (t.lit) hv = add(hv, muls(t.F, t.W_y));
+    if (p.restir.indirect) {
+      const ct_path_reservoir &fin = frame->pres_t[idx];
+      hv = add(hv, p.restir.pt_spatial ? muls(load3(fin.F), fin.weight) : load3(&frame->indirect[3 * idx]));
+    }
+    if (p.restir.direct_emittance && t.m_px < n_mats) hv = add(hv, load3(&emission[3 * (size_t)t.m_px]));
+    if (std::isinf(hv.x) || std::isnan(hv.x)) hv.x = 0.0f;
+    if (std::isinf(hv.y) || std::isnan(hv.y)) hv.y = 0.0f;
+    if (std::isinf(hv.z) || std::isnan(hv.z)) hv.z = 0.0f;
+    store3(o, hv);
+    o[3] = 1.0f;
+  }
+  return CT_OK;
+}
+
+int oc_frame_adjoint(oc_frame *frame, const ct_frame_params *params, const float *grad_image, float *grad_diffuse,
+                     float *grad_emission) {
+  if (!frame || !params || !grad_image || !grad_diffuse || !grad_emission) return CT_E_INVALID;
+  if (params->width != frame->W || params->height != frame->H) return CT_E_INVALID;
+  const Ctx c = reshade_ctx(frame, params);
+  const Scene &sc = *frame->sc;
+  const size_t nm = sc.mats.size();
+  std::vector<float> d(3 * nm), e(3 * nm);
+  for (size_t m = 0; m < nm; ++m)
+    for (int k = 0; k < 3; ++k) {
+      d[3 * m + k] = sc.mats[m].diffuse[k];
+      e[3 * m + k] = sc.mats[m].emission[k];
+    }
+  const size_t np = (size_t)frame->W * frame->H;
+  const bool emit = params->restir.direct_emittance != 0;
+  for (size_t idx = 0; idx < np; ++idx) {
+    if (!(frame->di_prev[idx].distance > 0.0f)) continue;
+    const DiTerm t = di_term(c, (uint32_t)idx, d.data(), nullptr);
+    if (t.m_px >= nm) continue;
+    const float F[3] = {t.F.x, t.F.y, t.F.z}, Fd[3] = {t.dF_dd.x, t.dF_dd.y, t.dF_dd.z},
+                Fe[3] = {t.dF_de.x, t.dF_de.y, t.dF_de.z};
+    for (int k = 0; k < 3; ++k) {
+      const float g = grad_image[4 * idx + k];
+      const float v = (t.lit ? F[k] * t.W_y : 0.0f) + (emit ? e[3 * t.m_px + k] : 0.0f);
+      if (std::isinf(v) || std::isnan(v)) continue;
+      if (t.lit) {
+        grad_diffuse[3 * (size_t)t.m_px + k] += g * (Fd[k] * t.W_y);
+        if (t.m_light < nm) grad_emission[3 * (size_t)t.m_light + k] += g * (Fe[k] * t.W_y);
+      }
+      if (emit) grad_emission[3 * (size_t)t.m_px + k] += g;
+    }
+  }
+  return CT_OK;
+}
+
 int oc_frame_read(oc_frame *frame, uint32_t which, void *dst, uint64_t bytes) {
   if (!frame || !dst) return CT_E_INVALID;
   size_t np = (size_t)frame->W * frame->H;

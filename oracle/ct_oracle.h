@@ -127,6 +127,16 @@ int oc_frame_get_stats(oc_frame *frame, ct_frame_stats *out);
 /* The scene before the last update, traced by temporal reuse's
  * target-at-previous query when CT_REMAP_BIT is set (NULL: the current). */
 int oc_frame_set_previous_scene(oc_frame *frame, oc_scene *scene);
+/* The last frame's shading output (its image, or with TAA the TAA input)
+ * with its final reservoirs held fixed, for the materials' diffuse albedo
+ * and emission [n_mats][3] (the scene's count); image [H][W][4].  Equals
+ * the frame's shading output at the scene's own values. */
+int oc_frame_reshade(oc_frame *frame, const ct_frame_params *params, const float *diffuse, const float *emission,
+                     uint32_t n_mats, float *image);
+/* Gradient of sum(grad_image * image) of that function at the scene's own
+ * values, added into grad_diffuse / grad_emission [materials][3]. */
+int oc_frame_adjoint(oc_frame *frame, const ct_frame_params *params, const float *grad_image, float *grad_diffuse,
+                     float *grad_emission);
 
 /* Probes (tests only): single passes of the frame on given inputs. */
 int oc_probe_ris(oc_scene *scene, const ct_frame_params *p, const ct_di_info *di, float confidence, uint32_t n,

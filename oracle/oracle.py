@@ -77,6 +77,8 @@ def lib():
             "oc_frame_destroy": ([C.c_void_p], None),
             "oc_frame_render": ([C.c_void_p, P(FrameParams), C.c_int, u32, u32], C.c_int),
             "oc_frame_read": ([C.c_void_p, u32, C.c_void_p, C.c_uint64], C.c_int),
+            "oc_frame_reshade": ([C.c_void_p, P(FrameParams), C.c_void_p, C.c_void_p, u32, C.c_void_p], C.c_int),
+            "oc_frame_adjoint": ([C.c_void_p, P(FrameParams), C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
             "oc_frame_get_stats": ([C.c_void_p, P(FrameStats)], C.c_int),
             "oc_frame_set_previous_scene": ([C.c_void_p, C.c_void_p], C.c_int),
             "oc_probe_ris": ([C.c_void_p, P(FrameParams), P(DiInfo), f32, u32, C.c_void_p, C.c_void_p], C.c_int),
@@ -446,6 +448,29 @@ class OFrame:
         s = FrameStats()
         lib().oc_frame_get_stats(self.handle, C.byref(s))
         return s
+
+    def reshade(self, params, diffuse, emission):
+        """The last frame's image [W*H][4] with its final reservoirs held
+        fixed, for materials' diffuse and emission arrays [materials][3]."""
+        d = np.ascontiguousarray(diffuse, np.float32)
+        e = np.ascontiguousarray(emission, np.float32)
+        out = np.zeros((self.w * self.h, 4), np.float32)
+        rc = lib().oc_frame_reshade(self.handle, C.byref(params), d.ctypes.data, e.ctypes.data, d.shape[0],
+                                    out.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"oc_frame_reshade failed: {rc}")
+        return out
+
+    def adjoint(self, params, grad_image, n_mats):
+        """d sum(grad_image * reshade) / d (diffuse, emission) at the scene's
+        own materials: two [n_mats][3] arrays."""
+        g = np.ascontiguousarray(grad_image, np.float32).reshape(-1, 4)
+        gd = np.zeros((n_mats, 3), np.float32)
+        ge = np.zeros((n_mats, 3), np.float32)
+        rc = lib().oc_frame_adjoint(self.handle, C.byref(params), g.ctypes.data, gd.ctypes.data, ge.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"oc_frame_adjoint failed: {rc}")
+        return gd, ge
 
     def close(self):
         if getattr(self, "handle", None):
