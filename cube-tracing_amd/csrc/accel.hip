@@ -554,6 +554,16 @@ __global__ void k_scatter_u32(const uint2 *w, uint32_t n, uint32_t *dst) {
   if (i < n) dst[w[i].x] = w[i].y;
 }
 
+// Single slots uploaded as packed records: slot[i] takes aabb[i], prim[i].
+__global__ void k_scatter_slots(const uint32_t *slot, const ct_aabb *aabb, const ct_primitive *prim, uint32_t n,
+                                ct_aabb *dst_aabb, ct_primitive *dst_prim) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    dst_aabb[slot[i]] = aabb[i];
+    dst_prim[slot[i]] = prim[i];
+  }
+}
+
 inline uint32_t blocks(uint64_t n, uint32_t b = kBlock) { return (uint32_t)((n + b - 1) / b); }
 
 // Host copy of a column-major mat4 * (x, y, z, 1), summed as the kernels do.
@@ -938,12 +948,54 @@ int commit(Accel &a, const Dirty &d) {
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks(cap)), dim3(kBlock), 0, st, S.remap_prim.as<uint32_t>(), 0u, cap);
     S.remap_prim_cap = cap;
   } else {
-    for (const auto &r : d.ranges) {
-      if (r.second == 0) continue;
+    // Touched slot ranges, sorted and merged; runs of a few slots (a
+    // relinked light owner, one per moved light) go up together as packed
+    // records with one copy and one scatter kernel, longer runs by copies.
+    std::vector<std::pair<uint32_t, uint32_t>> runs;
+    for (const auto &r : d.ranges)
+      if (r.second) runs.push_back(r);
+    std::sort(runs.begin(), runs.end());
+    size_t m = 0;
+    for (size_t k = 1; k < runs.size(); ++k) {
+      const uint64_t end = (uint64_t)runs[m].first + runs[m].second;
+      if ((uint64_t)runs[k].first <= end)
+        runs[m].second = (uint32_t)std::max<uint64_t>(end, (uint64_t)runs[k].first + runs[k].second) - runs[m].first;
+      else
+        runs[++m] = runs[k];
+    }
+    if (!runs.empty()) runs.resize(m + 1);
+    constexpr uint32_t kPackedRun = 8;
+    std::vector<uint32_t> one;
+    for (const auto &r : runs) {
+      if (r.second <= kPackedRun) {
+        for (uint32_t k = 0; k < r.second; ++k) one.push_back(r.first + k);
+        continue;
+      }
       CT_HIP(hipMemcpyAsync(S.aabbs.as<ct_aabb>() + r.first, &a.aabbs[r.first], sizeof(ct_aabb) * (size_t)r.second,
                             hipMemcpyHostToDevice, st));
       CT_HIP(hipMemcpyAsync(S.prims.as<ct_primitive>() + r.first, &a.prims[r.first],
                             sizeof(ct_primitive) * (size_t)r.second, hipMemcpyHostToDevice, st));
+    }
+    if (!one.empty()) {
+      const size_t n = one.size();
+      std::vector<unsigned char> pk((4 + sizeof(ct_aabb) + sizeof(ct_primitive)) * n);
+      uint32_t *ps = reinterpret_cast<uint32_t *>(pk.data());
+      ct_aabb *pa = reinterpret_cast<ct_aabb *>(ps + n);
+      ct_primitive *pp = reinterpret_cast<ct_primitive *>(pa + n);
+      for (size_t k = 0; k < n; ++k) {
+        ps[k] = one[k];
+        pa[k] = a.aabbs[one[k]];
+        pp[k] = a.prims[one[k]];
+      }
+      if ((rc = a.upload.reserve(pk.size()))) return rc;
+      CT_HIP(hipMemcpyAsync(a.upload.p, pk.data(), pk.size(), hipMemcpyHostToDevice, st));
+      const uint32_t *ds = a.upload.as<uint32_t>();
+      const ct_aabb *da = reinterpret_cast<const ct_aabb *>(ds + n);
+      const ct_primitive *dp = reinterpret_cast<const ct_primitive *>(da + n);
+      hipLaunchKernelGGL(k_scatter_slots, dim3(blocks(n)), dim3(kBlock), 0, st, ds, da, dp, (uint32_t)n,
+                         S.aabbs.as<ct_aabb>(), S.prims.as<ct_primitive>());
+      CT_HIP(hipGetLastError());
+      CT_HIP(hipStreamSynchronize(st));  // `pk` is released on return
     }
   }
   const uint32_t lcap = std::max<uint32_t>(a.max_cube_lights, 1);
@@ -1430,7 +1482,9 @@ void task_update(Accel &a, Update &u, const ct_task &t, bool &touched) {
 void gpu_delete_lights(Accel &a, Update &u, const std::vector<ct_task> &queue,
                        const std::vector<std::pair<uint32_t, std::vector<ct_primitive>>> &old) {
   std::vector<uint8_t> alive(a.cube_light_count, 0);
-  std::vector<uint32_t> gone;
+  // one entry per instance (its first task); the instances own disjoint
+  // lights, so they are scanned in parallel
+  std::vector<std::pair<const ct_task *, const std::vector<ct_primitive> *>> work;
   for (const ct_task &t : queue) {
     if (t.type != CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_GPU) continue;
     const uint32_t id = t.u.blas_del_prim_gpu.instance_index;
@@ -1438,7 +1492,18 @@ void gpu_delete_lights(Accel &a, Update &u, const std::vector<ct_task> &queue,
     for (const auto &g : old)
       if (g.first == id) o = &g.second;
     if (!o || !used(a, id)) continue;
-    const uint32_t f = a.instances[id].first_primitive_index, n = (uint32_t)o->size();
+    if (std::any_of(work.begin(), work.end(),
+                    [&](const auto &w) { return w.first->u.blas_del_prim_gpu.instance_index == id; }))
+      continue;
+    work.emplace_back(&t, o);
+  }
+  std::vector<std::vector<uint32_t>> gone_of(work.size());
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t q = 0; q < (int64_t)work.size(); ++q) {
+    const ct_task &t = *work[q].first;
+    const std::vector<ct_primitive> &o = *work[q].second;
+    const uint32_t id = t.u.blas_del_prim_gpu.instance_index;
+    const uint32_t f = a.instances[id].first_primitive_index, n = (uint32_t)o.size();
     const uint32_t keep = n - std::min(t.u.blas_del_prim_gpu.del_prim_count, n);
     for (uint32_t k = 0; k < keep; ++k) {
       const uint32_t L = a.prims[f + k].light_index;
@@ -1447,9 +1512,11 @@ void gpu_delete_lights(Accel &a, Update &u, const std::vector<ct_task> &queue,
         alive[L] = 1;
       }
     }
-    for (const ct_primitive &p : *o)
-      if (p.light_index != kInvalid && p.light_index < a.cube_light_count) gone.push_back(p.light_index);
+    for (const ct_primitive &p : o)
+      if (p.light_index != kInvalid && p.light_index < a.cube_light_count) gone_of[q].push_back(p.light_index);
   }
+  std::vector<uint32_t> gone;
+  for (const auto &g : gone_of) gone.insert(gone.end(), g.begin(), g.end());
   gone.erase(std::remove_if(gone.begin(), gone.end(), [&](uint32_t L) { return alive[L] != 0; }), gone.end());
   std::sort(gone.begin(), gone.end());
   gone.erase(std::unique(gone.begin(), gone.end()), gone.end());
@@ -1474,23 +1541,35 @@ void task_delete_from_gpu(Accel &a, Update &u, const ct_task &t) {
     return;
   }
   in.primitive_count -= t.u.blas_del_prim_gpu.del_prim_count;
-  touch(a, u, id, in.primitive_count);
+  // No upload: the device slots are the compaction's, which the host
+  // mirror was refreshed from.
   mark(u.dirty.rebuild, id);
 }
 
 // process_settling_task_queue (:2586-2680): the remapping tables are
 // cleared and the previous instance state becomes the current one.
+int scatter(hipStream_t st, const std::vector<uint2> &w, uint32_t *dst, DevBuf &buf);
+
 int settle(Accel &a, hipStream_t st) {
   BvhSet &S = a.set;
+  // Single entries (most of them) are cleared by one scatter per table,
+  // longer ranges by a fill each.
+  std::vector<uint2> zp, zl;
   for (const auto &r : a.settle_prims) {
     if (!S.remap_prim.p || r.first >= S.remap_prim_cap) continue;
     const uint32_t n = std::min(r.second, S.remap_prim_cap - r.first);
-    if (n) hipLaunchKernelGGL(k_fill_u32, dim3(blocks(n)), dim3(kBlock), 0, st, S.remap_prim.as<uint32_t>() + r.first, 0u, n);
+    if (n == 1)
+      zp.push_back(make_uint2(r.first, 0u));
+    else if (n)
+      hipLaunchKernelGGL(k_fill_u32, dim3(blocks(n)), dim3(kBlock), 0, st, S.remap_prim.as<uint32_t>() + r.first, 0u, n);
   }
   for (uint32_t l : a.settle_lights)
-    if (S.remap_light.p && l < S.remap_light_cap)
-      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, st, S.remap_light.as<uint32_t>() + l, 0u, 1u);
+    if (S.remap_light.p && l < S.remap_light_cap) zl.push_back(make_uint2(l, 0u));
   CT_HIP(hipGetLastError());
+  int rc;
+  if ((rc = scatter(st, zp, S.remap_prim.as<uint32_t>(), a.upload)) ||
+      (rc = scatter(st, zl, S.remap_light.as<uint32_t>(), a.upload)))
+    return rc;
   a.settle_prims.clear();
   a.settle_lights.clear();
   for (size_t id = 0; id < a.instances.size(); ++id) {
@@ -1548,15 +1627,15 @@ int snapshot_previous(Accel &a, hipStream_t st) {
   return CT_OK;
 }
 
-int scatter(hipStream_t st, const std::vector<uint2> &w, uint32_t *dst) {
+// Writes dst[w.x] = w.y for every pair through `buf` (device scratch).
+int scatter(hipStream_t st, const std::vector<uint2> &w, uint32_t *dst, DevBuf &buf) {
   if (w.empty()) return CT_OK;
-  DevBuf buf;
   int rc;
   if ((rc = buf.reserve(sizeof(uint2) * w.size()))) return rc;
   CT_HIP(hipMemcpyAsync(buf.p, w.data(), sizeof(uint2) * w.size(), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_scatter_u32, dim3(blocks(w.size())), dim3(kBlock), 0, st, buf.as<uint2>(), (uint32_t)w.size(), dst);
   CT_HIP(hipGetLastError());
-  CT_HIP(hipStreamSynchronize(st));  // `buf` is freed on return
+  CT_HIP(hipStreamSynchronize(st));  // `w` and the next use of `buf` wait for the kernel
   return CT_OK;
 }
 
@@ -1581,23 +1660,78 @@ int ct_accel_update_scene(ct_accel *a, int synchronize, void *stream, int *did_w
     queue.push_back(a->tasks.front());
     a->tasks.pop();
   }
-  std::vector<std::pair<uint32_t, std::vector<uint32_t>>> gpu_remap;
+  std::vector<uint2> gpu_entries;  // the caller's compaction's remapping entries (slot, value), value != 0
   std::vector<std::pair<uint32_t, std::vector<ct_primitive>>> gpu_old;  // records before the compaction
+  // Each GPU-deleted instance's slots (AABB, primitive, remapping entry) come
+  // back through one pinned staging buffer, then into the host mirror.  Slot
+  // ranges separated only by alignment padding are read as one run (the
+  // padding belongs to no primitive).
+  std::vector<std::pair<uint32_t, uint32_t>> rb;  // (first slot, count) per instance
   for (const ct_task &t : queue) {
     if (t.type != CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_GPU) continue;
     const uint32_t id = t.u.blas_del_prim_gpu.instance_index;
     if (!used(*a, id) || !a->set.aabbs.p) continue;
     const ct_instance &in = a->instances[id];
     const uint32_t f = in.first_primitive_index, n = in.primitive_count;
+    if (std::any_of(gpu_old.begin(), gpu_old.end(), [&](const auto &g) { return g.first == id; })) continue;
     gpu_old.emplace_back(id, std::vector<ct_primitive>(a->prims.begin() + f, a->prims.begin() + f + n));
-    CT_HIP(hipMemcpyAsync(&a->aabbs[f], a->set.aabbs.as<ct_aabb>() + f, sizeof(ct_aabb) * n, hipMemcpyDeviceToHost, st));
-    CT_HIP(hipMemcpyAsync(&a->prims[f], a->set.prims.as<ct_primitive>() + f, sizeof(ct_primitive) * n,
-                          hipMemcpyDeviceToHost, st));
-    gpu_remap.emplace_back(f, std::vector<uint32_t>(n));
-    CT_HIP(hipMemcpyAsync(gpu_remap.back().second.data(), a->set.remap_prim.as<uint32_t>() + f, 4 * (size_t)n,
-                          hipMemcpyDeviceToHost, st));
+    rb.emplace_back(f, n);
   }
-  if (!gpu_remap.empty()) CT_HIP(hipStreamSynchronize(st));  // one wait for every instance's readback
+  if (!rb.empty()) {
+    std::vector<std::pair<uint32_t, uint32_t>> runs(rb);  // merged (first, count)
+    std::sort(runs.begin(), runs.end());
+    size_t m = 0;
+    for (size_t k = 1; k < runs.size(); ++k) {
+      const uint64_t end = (uint64_t)runs[m].first + runs[m].second;
+      // a gap of fewer than CT_PRIMITIVE_ALIGNMENT slots is an instance's
+      // alignment padding (instances start aligned): no other instance
+      if ((uint64_t)runs[k].first < end + CT_PRIMITIVE_ALIGNMENT) {
+        runs[m].second = (uint32_t)std::max<uint64_t>(end, (uint64_t)runs[k].first + runs[k].second) - runs[m].first;
+      } else {
+        runs[++m] = runs[k];
+      }
+    }
+    runs.resize(m + 1);
+    size_t slots = 0;
+    for (const auto &r : runs) slots += r.second;
+    constexpr size_t kSlotBytes = sizeof(ct_aabb) + sizeof(ct_primitive) + 4;
+    if ((rc = a->readback.reserve(kSlotBytes * slots))) return rc;
+    ct_aabb *pa = a->readback.as<ct_aabb>();
+    ct_primitive *pp = reinterpret_cast<ct_primitive *>(pa + slots);
+    uint32_t *pr = reinterpret_cast<uint32_t *>(pp + slots);
+    std::vector<size_t> at(runs.size());
+    size_t off = 0;
+    for (size_t k = 0; k < runs.size(); ++k) {
+      const uint32_t f = runs[k].first, n = runs[k].second;
+      at[k] = off;
+      CT_HIP(hipMemcpyAsync(pa + off, a->set.aabbs.as<ct_aabb>() + f, sizeof(ct_aabb) * n, hipMemcpyDeviceToHost, st));
+      CT_HIP(hipMemcpyAsync(pp + off, a->set.prims.as<ct_primitive>() + f, sizeof(ct_primitive) * n,
+                            hipMemcpyDeviceToHost, st));
+      CT_HIP(hipMemcpyAsync(pr + off, a->set.remap_prim.as<uint32_t>() + f, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
+      off += n;
+    }
+    CT_HIP(hipStreamSynchronize(st));  // one wait for every range
+    // into the host mirror, and the compaction's remapping entries, in
+    // chunks of at most 64 Ki slots over the host threads
+    constexpr uint32_t kChunk = 1u << 16;
+    std::vector<std::pair<size_t, uint32_t>> chunks;  // (run, first slot of the chunk)
+    for (size_t k = 0; k < runs.size(); ++k)
+      for (uint64_t c = runs[k].first; c < (uint64_t)runs[k].first + runs[k].second; c += kChunk)
+        chunks.emplace_back(k, (uint32_t)c);
+    std::vector<std::vector<uint2>> found(chunks.size());
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t q = 0; q < (int64_t)chunks.size(); ++q) {
+      const size_t k = chunks[q].first;
+      const uint32_t c = chunks[q].second;
+      const uint32_t n = (uint32_t)std::min<uint64_t>(kChunk, (uint64_t)runs[k].first + runs[k].second - c);
+      const size_t src = at[k] + (c - runs[k].first);
+      std::memcpy(&a->aabbs[c], pa + src, sizeof(ct_aabb) * n);
+      std::memcpy(&a->prims[c], pp + src, sizeof(ct_primitive) * n);
+      for (uint32_t i = 0; i < n; ++i)
+        if (pr[src + i] != 0u) found[q].push_back(make_uint2(c + i, pr[src + i]));
+    }
+    for (const auto &f : found) gpu_entries.insert(gpu_entries.end(), f.begin(), f.end());
+  }
   bool worked = false;
   if (a->status == CT_AS_SETTLING) {
     if ((rc = settle(*a, st))) return rc;
@@ -1690,11 +1824,6 @@ int ct_accel_update_scene(ct_accel *a, int synchronize, void *stream, int *did_w
   // remembered for the settle.
   // The caller's compaction came first, so its entries go first and a later
   // task's entry for the same slot wins.
-  std::vector<uint2> gpu_entries;
-  for (auto &g : gpu_remap) {
-    for (uint32_t k = 0; k < g.second.size(); ++k)
-      if (g.second[k] != 0u) gpu_entries.push_back(make_uint2(g.first + k, g.second[k]));
-  }
   u.remap_prims.insert(u.remap_prims.begin(), gpu_entries.begin(), gpu_entries.end());
   for (const uint2 &w : u.remap_prims)
     if (w.x < a->set.remap_prim_cap) a->settle_prims.emplace_back(w.x, 1u);
@@ -1707,7 +1836,8 @@ int ct_accel_update_scene(ct_accel *a, int synchronize, void *stream, int *did_w
     if (w.x < a->set.remap_light_cap) wl.push_back(w);
   last_write_wins(wp);
   last_write_wins(wl);
-  if ((rc = scatter(st, wp, a->set.remap_prim.as<uint32_t>())) || (rc = scatter(st, wl, a->set.remap_light.as<uint32_t>())))
+  if ((rc = scatter(st, wp, a->set.remap_prim.as<uint32_t>(), a->upload)) ||
+      (rc = scatter(st, wl, a->set.remap_light.as<uint32_t>(), a->upload)))
     return rc;
   for (DoneTask &d : u.completed) {
     a->done.push_back(d);

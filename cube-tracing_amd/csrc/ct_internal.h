@@ -106,6 +106,36 @@ struct DevBuf {
   template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
+// Pinned host memory (hipHostMalloc) with RAII: device-to-host copies into
+// it run at the link's rate, where pageable memory is staged by the runtime.
+struct PinnedBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf &) = delete;
+  PinnedBuf &operator=(const PinnedBuf &) = delete;
+  ~PinnedBuf() { release(); }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // Grow to at least `n` bytes (contents are not preserved).
+  int reserve(size_t n) {
+    if (n <= bytes && p) return CT_OK;
+    release();
+    size_t m = n ? n : 16;
+    hipError_t e = hipHostMalloc(&p, m, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(CT_E_HIP, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    }
+    bytes = m;
+    return CT_OK;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
 // Build scratch for the LBVH kernels (reused between builds).
 struct BuildScratch {
   DevBuf keys[2], vals[2], centers, lo, hi, parent_int, parent_leaf, flags, heights, hist, seg,
@@ -205,6 +235,8 @@ struct Accel {
   DevBuf pt_mats;
   uint32_t pt_mat_count = 0;
   BuildScratch scratch;
+  PinnedBuf readback;                       // GPU deletions' slots on their way to the host mirror
+  DevBuf upload;                            // packed slot records and remapping writes on their way in
   uint32_t status = CT_AS_IDLE;
   uint32_t generation = 0;
 };
