@@ -1019,12 +1019,10 @@ constexpr uint32_t kNoVisibilityPass = 0x80000000u;
 
 // Prepass after the primary hit: rchit (rchit.glsl:18-54), miss
 // (rmiss.glsl:19-29), motion vector, temporal gather, confidence and RIS.
-// EMIT for the RIS visibility ray.
-// Five waves per SIMD (at most 96 VGPRs, a few dwords spilled) run this
-// VALU-heavy pass faster than four without spills.  kBrdf: some frame of
-// the set has the RIS BRDF strategy on (its rays need an LDS stack).
+// EMIT for the RIS visibility ray.  kBrdf: some frame of the set has the
+// RIS BRDF strategy on (its rays need an LDS stack).
 template <bool kBrdf>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_ris(FrameSet FS) {
+CT_D void ris_pass(const FrameSet &FS) {
   __shared__ uint32_t stk[kBrdf ? (kStackLds > 0 ? kStackLds : 1) * kBlock : 1];
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
@@ -1125,6 +1123,20 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
   wave_add(ris_ray, &F.counters[kCntRis]);
   if (kBrdf) wave_add(brdf_rays, &F.counters[kCntPtBounce]);
   queue_flush(F, px, emitted, far, -1);
+}
+// Five waves per SIMD (at most 96 VGPRs, a few dwords spilled) run the
+// VALU-heavy pass faster than four without spills.  With the BRDF
+// strategy's traversal inside, five waves would spill ~150 dwords; four
+// (128 VGPRs) spill 32.
+#ifndef CT_RIS_BRDF_WAVES
+#define CT_RIS_BRDF_WAVES 4
+#endif
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_ris(FrameSet FS) {
+  ris_pass<false>(FS);
+}
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CT_RIS_BRDF_WAVES)))
+k_ris_brdf(FrameSet FS) {
+  ris_pass<true>(FS);
 }
 
 // RIS visibility (slot 0) and temporal reuse (slots 1, 2) with the results
@@ -2118,9 +2130,9 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark,
     hipLaunchKernelGGL(k_primary<false>, dim3(L.blocks[0]), dim3(kBlock), 0, st, tiles, L.stats);
   if ((rc = mark())) return rc;  // kMarkPrimary
   if (L.ris_brdf)
-    hipLaunchKernelGGL(k_ris<true>, dim3(nb), dim3(kBlock), 0, st, pix);
+    hipLaunchKernelGGL(k_ris_brdf, dim3(nb), dim3(kBlock), 0, st, pix);
   else
-    hipLaunchKernelGGL(k_ris<false>, dim3(nb), dim3(kBlock), 0, st, pix);
+    hipLaunchKernelGGL(k_ris, dim3(nb), dim3(kBlock), 0, st, pix);
   if ((rc = mark())) return rc;  // kMarkRis
   if (L.prev_trace) {
     // RIS and temporal reuse visibility; the target-at-previous queries of
