@@ -5,7 +5,8 @@
 * the 1080p workload (256^3 in 64 BLAS of 64^3, ReSTIR DI + 3-bounce path,
   TAA 0.1, orbiting camera) rendered as the library renders it by default
   (two interleaved halves on two streams), three frames, every buffer bit
-  for bit against the oracle;
+  for bit against the oracle; then two frames with the RIS BRDF strategy
+  on, and the frame adjoint of the last one against the oracle's;
 * the batched workload (64 views over the 64^3, 128^3, 256^3 and 384^3
   scenes, DI + 2 bounces): the graph-replayed batch equals each view
   rendered on its own, and one view of each scene equals the oracle;
@@ -58,6 +59,25 @@ def test_hd_1080p_frames_match_oracle(gpu):
         ofr.render(p)
         hit = all_buffers_equal(ctx, ofr, w, f)
         assert hit.mean() > 0.3
+    # two more frames with the RIS BRDF strategy on (temporal history carried
+    # over), then the frame adjoint of the last one against the oracle's
+    w.features |= ctamd.CT_RIS_BRDF_BIT
+    for f in range(3, 5):
+        _say("1080p frame", f, "with the RIS BRDF strategy")
+        p = ctamd.frame_params(w, f)
+        ctx.render(acc, p)
+        ofr.render(p)
+        all_buffers_equal(ctx, ofr, w, f)
+    import torch
+    nm = sc.info.material_count
+    g = np.random.default_rng(2).standard_normal((w.width * w.height, 4)).astype(np.float32)
+    _say("1080p frame adjoint")
+    gd, ge = AD.frame_adjoint(ctx, acc, p, torch.from_numpy(g).cuda(), nm)
+    torch.cuda.synchronize()
+    od, oe = ofr.adjoint(p, g, nm)
+    for a, b in ((gd.cpu().numpy(), od), (ge.cpu().numpy(), oe)):
+        assert np.count_nonzero(b) > 10000
+        assert np.all(np.abs(a - b) <= 1e-5 * np.abs(b) + 1e-6 * np.abs(b).max())
 
 
 def test_batched_64_views_equal_per_view_and_oracle(gpu):
