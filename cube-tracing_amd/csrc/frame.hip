@@ -954,6 +954,33 @@ __global__ void __launch_bounds__(kBlock) k_primary(FrameSet FS, unsigned long l
   }
 }
 
+// Pixel of this thread in a pass that walks the image by tiles: a block
+// covers a 16x16 tile (entry `tile` of the tile table), each wave an 8x8
+// quarter, as the primary rays do.  A wave's pixels then read neighbouring
+// previous-frame pixels, neighbour records and voxels that its 2D
+// neighbours share in the caches, and the rays it queues start close
+// together.  On the 1080p frame this is 4% faster than runs of 256
+// consecutive pixels (3.41 against 3.55 ms; k_ris 0.47 against 0.54 ms,
+// spatial reuse 0.36 against 0.39 ms).  CT_TILED=0 builds the runs.
+#ifndef CT_TILED
+#define CT_TILED 1
+#endif
+constexpr bool kTiled = CT_TILED != 0;
+CT_D bool tile_pixel(const DFrame &F, uint32_t tile, uint32_t &px) {
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  const uint32_t tiles_x = (F.W + 15) / 16;
+  const uint32_t x = (tile % tiles_x) * 16 + (w & 1u) * 8 + (lane & 7u);
+  const uint32_t y = (tile / tiles_x) * 16 + (w >> 1) * 8 + (lane >> 3);
+  px = y * F.W + x;
+  return x < F.W && y < F.H;
+}
+// This thread's pixel: tiled, or the block's run of consecutive pixels.
+CT_D bool pass_pixel(const DFrame &F, uint2 blk, uint32_t &px) {
+  if (kTiled) return tile_pixel(F, blk.y, px);
+  px = blk.y + threadIdx.x;
+  return px < F.npix;
+}
+
 // The BRDF strategy of RIS, restir_resampling.glsl:190-244: brdf_count
 // directions from sample_material (light.glsl:232-270 with the Lambertian
 // callable, rcall_scatter.glsl:73-84, which also leaves the direction in
@@ -1026,9 +1053,10 @@ CT_D void ris_pass(const FrameSet &FS) {
   __shared__ uint32_t stk[kBrdf ? (kStackLds > 0 ? kStackLds : 1) * kBlock : 1];
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
-  const uint32_t px = blk_.y + threadIdx.x;
+  uint32_t px;
+  const bool mine = pass_pixel(F, blk_, px);
   uint32_t emitted = 0, far = 0, ris_ray = 0, brdf_rays = 0;
-  if (px < F.npix) {
+  if (mine) {
     const uint32_t x = px % F.W, y = px / F.W;
     float4 h = F.hit[px];
     uint32_t hi = __float_as_uint(h.y), hp = __float_as_uint(h.z);
@@ -1145,9 +1173,10 @@ k_ris_brdf(FrameSet FS) {
 __global__ void __launch_bounds__(kBlock) k_temporal_finish(FrameSet FS) {
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
-  const uint32_t px = blk_.y + threadIdx.x;
+  uint32_t px;
+  const bool mine = pass_pixel(F, blk_, px);
   uint32_t queries = 0;
-  if (px < F.npix) {
+  if (mine) {
     const float4 pd = F.dpos_c[px];
     if (!(pd.w > 0.0f)) {
       F.gat[(size_t)kGather * px] = pd;  // only the distance of a miss is ever read
@@ -1186,9 +1215,10 @@ template <bool kEmit>
 __global__ void __launch_bounds__(kBlock) k_spatial(FrameSet FS) {
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
-  const uint32_t px = blk_.y + threadIdx.x;
+  uint32_t px;
+  const bool mine = pass_pixel(F, blk_, px);
   uint32_t emitted = 0, far = 0;
-  if (px < F.npix) {
+  if (mine) {
     const uint32_t x = px % F.W, y = px / F.W;
     float4 pd = F.dpos_c[px];
     if (pd.w > 0.0f) {
@@ -1266,10 +1296,11 @@ CT_D uint32_t queue_push(bool active, uint32_t *count, uint32_t *also) {
 __global__ void __launch_bounds__(kBlock) k_pt_begin(FrameSet FS) {
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
-  const uint32_t px = blk_.y + threadIdx.x;
+  uint32_t px;
+  const bool mine = pass_pixel(F, blk_, px);
   bool more = false;
   F3 ro, rd;
-  if (px < F.npix && F.dpos_c[px].w > 0.0f) {
+  if (mine && F.dpos_c[px].w > 0.0f) {
     const DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
     PState p;
     more = pt_begin(F, p, isect_from_di(F, di), di.seed);
@@ -1372,7 +1403,8 @@ template <int kVis, bool kHybrid>
 __global__ void __launch_bounds__(kBlock) k_pt_end(FrameSet FS) {
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
-  const uint32_t px = blk_.y + threadIdx.x;
+  uint32_t px;
+  const bool mine = pass_pixel(F, blk_, px);
   PtTrace tr;
   tr.stk = nullptr;
   tr.bounce_rays = 0u;
@@ -1386,7 +1418,7 @@ __global__ void __launch_bounds__(kBlock) k_pt_end(FrameSet FS) {
   tr.rc_slot = 0u;
   tr.rc_prev = 0u;
   tr.rc_req = 0u;
-  if (px < F.npix) {
+  if (mine) {
     if (F.dpos_c[px].w > 0.0f) {
       const uint32_t x = px % F.W, y = px / F.W;
       if (kVis == kVisFinish) tr.bits = F.mask[px];
@@ -1474,13 +1506,14 @@ __global__ void __launch_bounds__(kBlock) k_pt_spatial(FrameSet FS, unsigned lon
   __shared__ uint32_t stk[(kStackLds > 0 ? kStackLds : 1) * kBlock];
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
-  const uint32_t px = blk_.y + threadIdx.x;
+  uint32_t px;
+  const bool mine = pass_pixel(F, blk_, px);
   PtTrace tr;
   tr.stk = stk + threadIdx.x;
   tr.bounce_rays = 0u;
   tr.shadow_rays = 0u;
   tr.overflow = false;
-  if (px < F.npix && F.dpos_c[px].w > 0.0f) {
+  if (mine && F.dpos_c[px].w > 0.0f) {
     const uint32_t x = px % F.W, y = px / F.W;
     const DI di = ld_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px);
     uint32_t seed = __float_as_uint(F.dorg_p[px].w);  // after the DI spatial reuse
@@ -2117,8 +2150,11 @@ constexpr unsigned kPhaseFirst = 1u, kPhaseSpatial = 2u, kPhaseTaa = 4u, kPhases
 template <class Mark, class SegMark>
 int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark, unsigned phases = kPhasesAll) {
   int rc;
-  const FrameSet tiles{L.fb, L.blk}, pix{L.fb, L.blk + L.blocks[0]}, taa{L.fb, L.blk + L.blocks[0] + L.blocks[1]};
-  const uint32_t np = L.total_px, nb = L.blocks[1];
+  const FrameSet tiles{L.fb, L.blk}, runs{L.fb, L.blk + L.blocks[0]}, taa{L.fb, L.blk + L.blocks[0] + L.blocks[1]};
+  // the per-pixel passes walk the image by tiles (kTiled) or by runs of
+  // consecutive pixels; `pix` and `nb` are their table and block count
+  const FrameSet pix = kTiled ? tiles : runs;
+  const uint32_t np = L.total_px, nb = kTiled ? L.blocks[0] : L.blocks[1];
   if (phases & kPhaseFirst) {
   // queue, fetch head and the ray statistics
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, L.counters, 0, 8);
