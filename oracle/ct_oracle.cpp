@@ -3058,10 +3058,14 @@ struct DiTerm {
   V3 F{0, 0, 0}, dF_dd{0, 0, 0}, dF_de{0, 0, 0};  // F_y, dF_y/d d[m_px], dF_y/d Le (per channel)
   float W_y = 0.0f;
 };
+// The material of a cube light's voxel (its owner primitive), or kInvalid.
 static uint32_t light_material(const Scene &sc, const ct_light &l) {
   if (l.type != CT_LIGHT_CUBE || l.instance_info.instance_id >= sc.inst.size()) return kInvalid;
-  const uint64_t slot = (uint64_t)sc.inst[l.instance_info.instance_id].first_primitive_index + l.instance_info.primitive_id;
-  return slot < sc.prims.size() ? sc.prims[slot].material_index : kInvalid;
+  const ct_instance &in = sc.inst[l.instance_info.instance_id];
+  if (l.instance_info.primitive_id >= in.primitive_count) return kInvalid;
+  const uint64_t slot = (uint64_t)in.first_primitive_index + l.instance_info.primitive_id;
+  const uint32_t m = slot < sc.prims.size() ? sc.prims[slot].material_index : kInvalid;
+  return m < sc.mats.size() ? m : kInvalid;
 }
 // `emission` null: the light's own emissive as Le (the adjoint's point, the
 // frame's values).
@@ -3223,9 +3227,9 @@ static Ctx probe_ctx(const oc_scene *scene, const ct_frame_params *p) {
   return c;
 }
 
-// RIS (restir_resampling.glsl:19-255, with its visibility pass) at the
-// shading point `di` describes, the camera ray from di.ray_origin through
-// di.position, once per seed.
+// RIS (restir_resampling.glsl:19-255, with its visibility pass unless the
+// BRDF strategy's candidate won) at the shading point `di` describes, the
+// camera ray from di.ray_origin through di.position, once per seed.
 int oc_probe_ris(oc_scene *scene, const ct_frame_params *p, const ct_di_info *di, float confidence, uint32_t n,
                  const uint32_t *seeds, ct_reservoir *out) {
   if (!scene || !p || !di || (n && (!seeds || !out))) return CT_E_INVALID;
