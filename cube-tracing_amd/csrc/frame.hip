@@ -547,8 +547,10 @@ CT_D F3 res_radiance(const DFrame &F, const Res &r, F3 ray_d, Hit hit, const Mat
   return res_radiance(F, r, shade_of(ray_d, hit), hit, mat, v);
 }
 
-// reservoir_visibility_pass, reservoir.glsl:76-96, 114-126
-CT_D void visibility_pass(const DFrame &F, Res &r, const Hit &hit, Vis &v) {
+// reservoir_visibility_pass, reservoir.glsl:76-96, 114-126.  Returns the
+// visibility it applied (in an emitting pass: whether the light sample is
+// valid, its ray queued).
+CT_D bool visibility_pass(const DFrame &F, Res &r, const Hit &hit, Vis &v) {
   bool visibility = false;
   if (res_valid(r)) {
     PLight l;
@@ -561,6 +563,20 @@ CT_D void visibility_pass(const DFrame &F, Res &r, const Hit &hit, Vis &v) {
       if (ls.ray && occluded(F, v, ls.P, ls.dir, ls.dist, ls.ti, ls.tp)) visibility = false;
     }
   }
+  r.W_y = visibility ? (r.W_sum / luminance(r.F)) : 0.0f;
+  r.F = muls(r.F, visibility ? 1.0f : 0.0f);
+  return visibility;
+}
+
+// k_ris records the outcome of its visibility pass's light sample in the
+// pixel's occlusion mask, above the ray slots' bits: kRisOk (the sample is
+// valid) and kRisRay (its ray took slot 0).  The finishing pass then applies
+// the ray's bit without drawing the sample again.
+constexpr uint32_t kRisOk = 1u << 31, kRisRay = 1u << 30;
+static_assert(kRaySlots <= 30, "the RIS outcome bits sit above the ray slots");
+CT_D void visibility_finish(Res &r, uint32_t mask) {
+  bool visibility = (mask & kRisOk) != 0u;
+  if ((mask & kRisRay) && (mask & 1u)) visibility = false;
   r.W_y = visibility ? (r.W_sum / luminance(r.F)) : 0.0f;
   r.F = muls(r.F, visibility ? 1.0f : 0.0f);
 }
@@ -1055,7 +1071,7 @@ CT_D void ris_pass(const FrameSet &FS) {
   const DFrame &F = FS.fb[blk_.x];
   uint32_t px;
   const bool mine = pass_pixel(F, blk_, px);
-  uint32_t emitted = 0, far = 0, ris_ray = 0, brdf_rays = 0;
+  uint32_t emitted = 0, far = 0, ris_ray = 0, brdf_rays = 0, ris_mask = 0;
   if (mine) {
     const uint32_t x = px % F.W, y = px / F.W;
     float4 h = F.hit[px];
@@ -1131,8 +1147,12 @@ CT_D void ris_pass(const FrameSet &FS) {
         needs_visibility = false;
       Vis v{true, px, 0u, 0u, 0u, 0u};
       Res rv = r;
-      if (needs_visibility) visibility_pass(F, rv, hit, v);
-      else r.W_y = __uint_as_float(kNoVisibilityPass);
+      if (needs_visibility) {
+        if (visibility_pass(F, rv, hit, v)) ris_mask |= kRisOk;
+        if (v.emitted & 1u) ris_mask |= kRisRay;
+      } else {
+        r.W_y = __uint_as_float(kNoVisibilityPass);
+      }
       ris_ray = v.emitted;
       if (prev.W_y > 0.0f) {
         // The temporal reuse rays (slots 1, 2) go into the same launch; the
@@ -1146,7 +1166,7 @@ CT_D void ris_pass(const FrameSet &FS) {
       F.pidx[px] = pidx;
     }
     st_di(F.dpos_c, F.dnrm_c, F.dorg_c, F.dhit_c, px, di);
-    F.mask[px] = 0u;
+    F.mask[px] = ris_mask;
   }
   wave_add(ris_ray, &F.counters[kCntRis]);
   if (kBrdf) wave_add(brdf_rays, &F.counters[kCntPtBounce]);
@@ -1188,7 +1208,7 @@ __global__ void __launch_bounds__(kBlock) k_temporal_finish(FrameSet FS) {
       Res r = ld_res(F.rA_i, F.rB_i, F.rC_i, px);
       Vis v{false, px, 0u, F.mask[px], 0u, 0u};
       if (__float_as_uint(r.W_y) == kNoVisibilityPass) r.W_y = 0.0f;
-      else visibility_pass(F, r, hit, v);
+      else visibility_finish(r, v.bits);
       const uint32_t pidx = F.pidx[px];
       Res prev = res_init();
       if (pidx < F.W * F.H) prev = ld_res(F.rA_p, F.rB_p, F.rC_p, pidx);
