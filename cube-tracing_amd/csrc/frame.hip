@@ -1231,12 +1231,15 @@ __global__ void __launch_bounds__(kBlock) k_temporal_finish(FrameSet FS) {
 }
 
 // Shading pass: spatial reuse (EMIT / FINISH) then shade and store.
+// The finish pass streams its pixel's own records (the emit pass's
+// per-neighbour results, neighbour-major): it takes runs of consecutive
+// pixels, which read them in whole cache lines.
 template <bool kEmit>
 __global__ void __launch_bounds__(kBlock) k_spatial(FrameSet FS) {
   const uint2 blk_ = FS.blk[blockIdx.x];
   const DFrame &F = FS.fb[blk_.x];
-  uint32_t px;
-  const bool mine = pass_pixel(F, blk_, px);
+  uint32_t px = blk_.y + threadIdx.x;
+  const bool mine = kEmit ? pass_pixel(F, blk_, px) : px < F.npix;
   uint32_t emitted = 0, far = 0;
   if (mine) {
     const uint32_t x = px % F.W, y = px / F.W;
@@ -2264,7 +2267,7 @@ int enqueue_frames(const Launch &L, hipStream_t st, Mark mark, SegMark seg_mark,
   if ((rc = mark())) return rc;  // kMarkSpatialEmit
   launch_any(L, pix, 2 * L.max_neighbors * np, st);
   if ((rc = mark())) return rc;  // kMarkTraceSpatial
-  hipLaunchKernelGGL(k_spatial<false>, dim3(nb), dim3(kBlock), 0, st, pix);
+  hipLaunchKernelGGL(k_spatial<false>, dim3(L.blocks[1]), dim3(kBlock), 0, st, runs);
   if ((rc = mark())) return rc;  // kMarkSpatialFinish
   if (L.pt_depth) {
     if (L.stats && L.hybrid)
