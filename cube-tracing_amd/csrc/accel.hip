@@ -1692,43 +1692,50 @@ int ct_accel_update_scene(ct_accel *a, int synchronize, void *stream, int *did_w
       }
     }
     runs.resize(m + 1);
-    size_t slots = 0;
-    for (const auto &r : runs) slots += r.second;
+    // Chunks of at most 64 Ki slots, copied in rounds through at most
+    // kStageSlots slots of pinned staging (72 MiB), each round's chunks then
+    // into the host mirror and the compaction's remapping entries collected,
+    // over the host threads.
     constexpr size_t kSlotBytes = sizeof(ct_aabb) + sizeof(ct_primitive) + 4;
-    if ((rc = a->readback.reserve(kSlotBytes * slots))) return rc;
-    ct_aabb *pa = a->readback.as<ct_aabb>();
-    ct_primitive *pp = reinterpret_cast<ct_primitive *>(pa + slots);
-    uint32_t *pr = reinterpret_cast<uint32_t *>(pp + slots);
-    std::vector<size_t> at(runs.size());
-    size_t off = 0;
-    for (size_t k = 0; k < runs.size(); ++k) {
-      const uint32_t f = runs[k].first, n = runs[k].second;
-      at[k] = off;
-      CT_HIP(hipMemcpyAsync(pa + off, a->set.aabbs.as<ct_aabb>() + f, sizeof(ct_aabb) * n, hipMemcpyDeviceToHost, st));
-      CT_HIP(hipMemcpyAsync(pp + off, a->set.prims.as<ct_primitive>() + f, sizeof(ct_primitive) * n,
-                            hipMemcpyDeviceToHost, st));
-      CT_HIP(hipMemcpyAsync(pr + off, a->set.remap_prim.as<uint32_t>() + f, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
-      off += n;
-    }
-    CT_HIP(hipStreamSynchronize(st));  // one wait for every range
-    // into the host mirror, and the compaction's remapping entries, in
-    // chunks of at most 64 Ki slots over the host threads
     constexpr uint32_t kChunk = 1u << 16;
-    std::vector<std::pair<size_t, uint32_t>> chunks;  // (run, first slot of the chunk)
-    for (size_t k = 0; k < runs.size(); ++k)
-      for (uint64_t c = runs[k].first; c < (uint64_t)runs[k].first + runs[k].second; c += kChunk)
-        chunks.emplace_back(k, (uint32_t)c);
+    constexpr size_t kStageSlots = (size_t)1 << 21;
+    std::vector<std::pair<uint32_t, uint32_t>> chunks;  // (first slot, count)
+    size_t slots = 0;
+    for (const auto &r : runs)
+      for (uint64_t c = r.first; c < (uint64_t)r.first + r.second; c += kChunk) {
+        chunks.emplace_back((uint32_t)c, (uint32_t)std::min<uint64_t>(kChunk, (uint64_t)r.first + r.second - c));
+        slots += chunks.back().second;
+      }
+    const size_t stage = std::min(slots, kStageSlots);
+    if ((rc = a->readback.reserve(kSlotBytes * stage))) return rc;
+    ct_aabb *pa = a->readback.as<ct_aabb>();
+    ct_primitive *pp = reinterpret_cast<ct_primitive *>(pa + stage);
+    uint32_t *pr = reinterpret_cast<uint32_t *>(pp + stage);
     std::vector<std::vector<uint2>> found(chunks.size());
+    std::vector<size_t> at(chunks.size());
+    for (size_t q0 = 0; q0 < chunks.size();) {
+      size_t q1 = q0, off = 0;
+      for (; q1 < chunks.size() && off + chunks[q1].second <= stage; ++q1) {
+        const uint32_t f = chunks[q1].first, n = chunks[q1].second;
+        at[q1] = off;
+        CT_HIP(hipMemcpyAsync(pa + off, a->set.aabbs.as<ct_aabb>() + f, sizeof(ct_aabb) * n, hipMemcpyDeviceToHost, st));
+        CT_HIP(hipMemcpyAsync(pp + off, a->set.prims.as<ct_primitive>() + f, sizeof(ct_primitive) * n,
+                              hipMemcpyDeviceToHost, st));
+        CT_HIP(hipMemcpyAsync(pr + off, a->set.remap_prim.as<uint32_t>() + f, 4 * (size_t)n, hipMemcpyDeviceToHost,
+                              st));
+        off += n;
+      }
+      CT_HIP(hipStreamSynchronize(st));  // one wait per round
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int64_t q = 0; q < (int64_t)chunks.size(); ++q) {
-      const size_t k = chunks[q].first;
-      const uint32_t c = chunks[q].second;
-      const uint32_t n = (uint32_t)std::min<uint64_t>(kChunk, (uint64_t)runs[k].first + runs[k].second - c);
-      const size_t src = at[k] + (c - runs[k].first);
-      std::memcpy(&a->aabbs[c], pa + src, sizeof(ct_aabb) * n);
-      std::memcpy(&a->prims[c], pp + src, sizeof(ct_primitive) * n);
-      for (uint32_t i = 0; i < n; ++i)
-        if (pr[src + i] != 0u) found[q].push_back(make_uint2(c + i, pr[src + i]));
+      for (int64_t q = (int64_t)q0; q < (int64_t)q1; ++q) {
+        const uint32_t c = chunks[q].first, n = chunks[q].second;
+        const size_t src = at[q];
+        std::memcpy(&a->aabbs[c], pa + src, sizeof(ct_aabb) * n);
+        std::memcpy(&a->prims[c], pp + src, sizeof(ct_primitive) * n);
+        for (uint32_t i = 0; i < n; ++i)
+          if (pr[src + i] != 0u) found[q].push_back(make_uint2(c + i, pr[src + i]));
+      }
+      q0 = q1;
     }
     for (const auto &f : found) gpu_entries.insert(gpu_entries.end(), f.begin(), f.end());
   }
