@@ -1403,7 +1403,7 @@ __global__ void __launch_bounds__(kBlock) k_pt_step(FrameSet FS, uint32_t d) {
     const F3 box_o = compute_ray_origin(p.origin, p.dir);
     Isect i = segment_record(F, p.origin, box_o, p.dir, __float_as_uint(h.y), __float_as_uint(h.z));
     more = pt_step(F, p, i);
-    st_pstate(F.pst, F.npix, px, p);
+    st_pstate(F.pst, F.npix, px, p, false);
     ro = compute_ray_origin(p.origin, p.dir);
     rd = p.dir;
   }

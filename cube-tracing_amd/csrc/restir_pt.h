@@ -49,6 +49,7 @@ struct PBuilder {  // PATH_BUILDER, path_builder.glsl:8-17
   uint32_t cached_seed, seed, rc_len, flags;
   F3 cj;
 };
+constexpr uint32_t kDirtyRc = 1u, kDirtyRes = 2u;
 struct PState {  // PATH_STATE, path_state.glsl:23-78 (fields the Lambertian path reads)
   uint32_t max_depth, flags, path_length, bounce_counters;
   float scene_length;
@@ -62,6 +63,10 @@ struct PState {  // PATH_STATE, path_state.glsl:23-78 (fields the Lambertian pat
   PBuilder pb;
   PRes res;
   bool last_rough;
+  // Parts of the stored state a path-loop step changed besides those every
+  // step rewrites (kDirtyRc: the path builder's reconnection vertex and
+  // Jacobian terms; kDirtyRes: the path reservoir).  Not stored.
+  uint32_t dirty;
   // Replay of a source path for the hybrid shift (is_replay_for_hybrid_shift):
   // stops at the vertex before the source's reconnection vertex replay_rc.
   bool replay, replay_done;
@@ -367,7 +372,10 @@ CT_D bool pt_scatter(PState &p, Isect &i) {
   if (p.path_length == 0u) p.flags |= kPathDiffusePrimary;
   p.thp = mulv(p.thp, weight);
   p.pdf = pdf;
-  if (p.path_length == p.pb.rc_len) p.pb.cj.y = pdf;
+  if (p.path_length == p.pb.rc_len) {
+    p.pb.cj.y = pdf;
+    p.dirty |= kDirtyRc;
+  }
   p.prev_scatter_pdf = pdf;
   if (pdf <= 0.0f) return false;
   p.bounce_counters += 1u;
@@ -445,6 +453,7 @@ CT_D void pt_emissive(const DFrame &F, PState &p, const Isect &i) {
   const F3 att = mulv(cur, i.mat.emission);
   if (p.replay) return;  // a replay only carries throughput to its stop vertex
   if (p.path_length > 1u) p.L = add(p.L, Lr);
+  p.dirty |= kDirtyRes;
   const F3 postfix = muls(mulv(p.thp, att), mis);
   const uint32_t plen = p.path_length >= 1u ? p.path_length - 1u : 0u;
   const bool selected = add_escape_vertex(p.pb, plen, p.dir, Lr, postfix, p.rr_pdf, mis, light_pdf, kGeomLightCube, p.res);
@@ -512,6 +521,7 @@ CT_D void pt_handle_hit(const DFrame &F, PState &p, Isect &i) {
     const bool can_connect = p.path_length >= 1u && p.path_length < p.pb.rc_len && connectable;
     if ((!hybrid && p.path_length == 1u) || (hybrid && can_connect)) {
       p.pb.rc = p.hit;
+      p.dirty |= kDirtyRc;
       p.pb.cj.x = p.prev_scatter_pdf;
       const F3 disp = sub(prev_origin, i.P);
       p.pb.cj.z = fabsf(dot(i.n, i.wo)) / dot(disp, disp);
@@ -575,6 +585,7 @@ CT_D void pt_handle_miss(const DFrame &F, PState &p) {
   const F3 Le = sky_color(F.time, F.afternoon != 0, p.dir);
   const F3 Lr = muls(mulv(mulv(p.thp, p.prefix_thp), Le), mis);
   if (p.path_length > 0u) p.L = add(p.L, Lr);
+  p.dirty |= kDirtyRes;
   const bool selected = add_escape_vertex(p.pb, p.path_length, p.dir, Lr, muls(mulv(p.thp, Le), mis), p.rr_pdf, mis,
                                           light_pdf, kGeomLightEnv, p.res);
   // escaped to the sky: a reconnection vertex at infinity (path_tracer.glsl:805-833)
@@ -606,6 +617,7 @@ CT_D bool pt_begin(const DFrame &F, PState &p, Isect i, uint32_t seed) {
   p.hit.hit = i.P;
   p.thp = f3(1.0f, 1.0f, 1.0f);
   p.prefix_thp = f3(1.0f, 1.0f, 1.0f);
+  p.dirty = 0u;
   p.L = f3(0, 0, 0);
   p.rr_pdf = 1.0f;
   p.prev_scatter_pdf = 1.0f;
@@ -740,7 +752,11 @@ CT_D void pt_end(PState &p, F3 &color) {
 // field-major (field k of path p at S[k * np + p]).  Fields the outputs
 // never read (L, normal, scene_length, hit) are not kept.
 constexpr int kPathStateVec = 14;
-CT_D void st_pstate(float4 *S, uint32_t np, uint32_t p, const PState &s) {
+// Stores the state; with `all` false only the vectors a path-loop step
+// changes: the seeds, flags, ray, throughputs and the builder's latest
+// direction always, the reconnection vertex and the reservoir when marked
+// dirty.
+CT_D void st_pstate(float4 *S, uint32_t np, uint32_t p, const PState &s, bool all = true) {
   const uint32_t packed = (s.path_length & 0xFFu) | ((s.bounce_counters & 0xFFu) << 8) | ((s.pb.rc_len & 0xFFu) << 16) |
                           ((s.last_rough ? 1u : 0u) << 24) | ((s.max_depth & 0x7Fu) << 25);
   S[0 * (size_t)np + p] = make_float4(__uint_as_float(s.seed), __uint_as_float(s.pb.seed), __uint_as_float(s.flags),
@@ -749,10 +765,13 @@ CT_D void st_pstate(float4 *S, uint32_t np, uint32_t p, const PState &s) {
   S[2 * (size_t)np + p] = make_float4(s.dir.x, s.dir.y, s.dir.z, s.prev_scatter_pdf);
   S[3 * (size_t)np + p] = make_float4(s.thp.x, s.thp.y, s.thp.z, s.rr_pdf);
   S[4 * (size_t)np + p] = make_float4(s.prefix_thp.x, s.prefix_thp.y, s.prefix_thp.z, __uint_as_float(s.pb.cached_seed));
-  S[5 * (size_t)np + p] = make_float4(__uint_as_float(s.pb.rc.inst), __uint_as_float(s.pb.rc.prim), s.pb.rc.hit.x,
-                                      s.pb.rc.hit.y);
+  if (all || (s.dirty & kDirtyRc))
+    S[5 * (size_t)np + p] = make_float4(__uint_as_float(s.pb.rc.inst), __uint_as_float(s.pb.rc.prim), s.pb.rc.hit.x,
+                                        s.pb.rc.hit.y);
   S[6 * (size_t)np + p] = make_float4(s.pb.rc.hit.z, s.pb.rc_wi.x, s.pb.rc_wi.y, s.pb.rc_wi.z);
-  S[7 * (size_t)np + p] = make_float4(s.pb.cj.x, s.pb.cj.y, s.pb.cj.z, __uint_as_float(s.pb.flags));
+  if (all || (s.dirty & kDirtyRc))
+    S[7 * (size_t)np + p] = make_float4(s.pb.cj.x, s.pb.cj.y, s.pb.cj.z, __uint_as_float(s.pb.flags));
+  if (!all && !(s.dirty & kDirtyRes)) return;
   const PRes &r = s.res;
   S[8 * (size_t)np + p] = make_float4(r.M, r.weight, __uint_as_float(r.flags), __uint_as_float(r.rc_seed));
   S[9 * (size_t)np + p] = make_float4(r.F.x, r.F.y, r.F.z, r.light_pdf);
@@ -792,6 +811,7 @@ CT_D void ld_pstate(const float4 *S, uint32_t np, uint32_t p, PState &s) {
   s.scene_length = 0.0f;
   s.normal = f3(0, 0, 0);
   s.L = f3(0, 0, 0);
+  s.dirty = 0u;
   s.hit = objhit_invalid();
   s.replay = false;
   s.replay_done = false;
