@@ -1398,10 +1398,14 @@ __global__ void __launch_bounds__(kBlock) k_pt_step(FrameSet FS, uint32_t d) {
     const DFrame &F = FS.fb[kMulti ? (entry >> kViewShift) : 0u];
     const uint32_t px = kMulti ? (entry & kSlotMask) : entry;
     PState p;
-    ld_pstate(F.pst, F.npix, px, p);
+    ld_pstate(F.pst, F.npix, px, p, false);
     const float4 h = F0.seg_hit[q];
     const F3 box_o = compute_ray_origin(p.origin, p.dir);
     Isect i = segment_record(F, p.origin, box_o, p.dir, __float_as_uint(h.y), __float_as_uint(h.z));
+    // only an escape or an emitting voxel reaches the path reservoir
+    // (pt_handle_miss, pt_emissive): load it for those steps
+    if (!i.is_hit || i.mat.emission.x > 0.0f || i.mat.emission.y > 0.0f || i.mat.emission.z > 0.0f)
+      ld_pstate_res(F.pst, F.npix, px, p);
     more = pt_step(F, p, i);
     st_pstate(F.pst, F.npix, px, p, false);
     ro = compute_ray_origin(p.origin, p.dir);

@@ -780,10 +780,25 @@ CT_D void st_pstate(float4 *S, uint32_t np, uint32_t p, const PState &s, bool al
   S[12 * (size_t)np + p] = make_float4(r.rc.hit.z, r.rc_wi.x, r.rc_wi.y, r.rc_wi.z);
   S[13 * (size_t)np + p] = make_float4(r.rc_irr.x, r.rc_irr.y, r.rc_irr.z, 0.0f);
 }
-CT_D void ld_pstate(const float4 *S, uint32_t np, uint32_t p, PState &s) {
+// The stored path reservoir (vectors 8..13 of the state).
+CT_D void ld_pstate_res(const float4 *S, uint32_t np, uint32_t p, PState &s) {
   float4 q[kPathStateVec];
 #pragma unroll
-  for (int k = 0; k < kPathStateVec; ++k) q[k] = S[(size_t)k * np + p];
+  for (int k = 8; k < kPathStateVec; ++k) q[k] = S[(size_t)k * np + p];
+  PRes &r = s.res;
+  r.M = q[8].x; r.weight = q[8].y; r.flags = __float_as_uint(q[8].z); r.rc_seed = __float_as_uint(q[8].w);
+  r.F = f3(q[9].x, q[9].y, q[9].z); r.light_pdf = q[9].w;
+  r.cj = f3(q[10].x, q[10].y, q[10].z); r.init_seed = __float_as_uint(q[10].w);
+  r.rc.inst = __float_as_uint(q[11].x); r.rc.prim = __float_as_uint(q[11].y); r.rc.hit = f3(q[11].z, q[11].w, q[12].x);
+  r.rc_wi = f3(q[12].y, q[12].z, q[12].w);
+  r.rc_irr = f3(q[13].x, q[13].y, q[13].z);
+}
+// Loads the state; with `res` false all but the path reservoir, which a
+// caller that needs it then takes with ld_pstate_res.
+CT_D void ld_pstate(const float4 *S, uint32_t np, uint32_t p, PState &s, bool res = true) {
+  float4 q[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) q[k] = S[(size_t)k * np + p];
   s.seed = __float_as_uint(q[0].x);
   s.pb.seed = __float_as_uint(q[0].y);
   s.flags = __float_as_uint(q[0].z);
@@ -801,13 +816,7 @@ CT_D void ld_pstate(const float4 *S, uint32_t np, uint32_t p, PState &s) {
   s.pb.rc.hit = f3(q[5].z, q[5].w, q[6].x);
   s.pb.rc_wi = f3(q[6].y, q[6].z, q[6].w);
   s.pb.cj = f3(q[7].x, q[7].y, q[7].z); s.pb.flags = __float_as_uint(q[7].w);
-  PRes &r = s.res;
-  r.M = q[8].x; r.weight = q[8].y; r.flags = __float_as_uint(q[8].z); r.rc_seed = __float_as_uint(q[8].w);
-  r.F = f3(q[9].x, q[9].y, q[9].z); r.light_pdf = q[9].w;
-  r.cj = f3(q[10].x, q[10].y, q[10].z); r.init_seed = __float_as_uint(q[10].w);
-  r.rc.inst = __float_as_uint(q[11].x); r.rc.prim = __float_as_uint(q[11].y); r.rc.hit = f3(q[11].z, q[11].w, q[12].x);
-  r.rc_wi = f3(q[12].y, q[12].z, q[12].w);
-  r.rc_irr = f3(q[13].x, q[13].y, q[13].z);
+  if (res) ld_pstate_res(S, np, p, s);
   s.scene_length = 0.0f;
   s.normal = f3(0, 0, 0);
   s.L = f3(0, 0, 0);
