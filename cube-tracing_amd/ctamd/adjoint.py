@@ -67,6 +67,21 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr())
 
 
+def _on_torch_stream(call):
+    """Runs call(stream) in order with torch's current stream: on that stream,
+    or, when torch works on the null stream (handle 0, which the library's
+    own non-blocking streams do not wait for and which does not wait for
+    them), on the library's stream between two device synchronisations."""
+    import torch
+    h = torch.cuda.current_stream().cuda_stream
+    if h:
+        return call(C.c_void_p(h))
+    torch.cuda.synchronize()
+    r = call(None)
+    torch.cuda.synchronize()
+    return r
+
+
 class DiffRenderer:
     """Views, sample count, depth and seed of one differentiable render."""
 
@@ -77,23 +92,20 @@ class DiffRenderer:
         self.desc = AdjointDesc(len(views), spp, max_depth, seed)
         self.n_pixels = sum(v.width * v.height for v in views)
 
-    def _stream(self):
-        import torch
-        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
-
     def forward(self, albedo, emission):
         import torch
         img = torch.empty((self.n_pixels, 3), dtype=torch.float32, device=albedo.device)
-        check(lib().ct_adjoint_render(self.accel.handle, C.byref(self.desc), self.views, _ptr(albedo), _ptr(emission),
-                                      _ptr(img), self._stream()))
+        _on_torch_stream(lambda st: check(lib().ct_adjoint_render(
+            self.accel.handle, C.byref(self.desc), self.views, _ptr(albedo), _ptr(emission), _ptr(img), st)))
         return img
 
     def backward(self, albedo, emission, grad_image):
         import torch
         ga = torch.zeros_like(albedo)
         ge = torch.zeros_like(emission)
-        check(lib().ct_adjoint_backward(self.accel.handle, C.byref(self.desc), self.views, _ptr(albedo),
-                                        _ptr(emission), _ptr(grad_image), _ptr(ga), _ptr(ge), self._stream()))
+        _on_torch_stream(lambda st: check(lib().ct_adjoint_backward(
+            self.accel.handle, C.byref(self.desc), self.views, _ptr(albedo), _ptr(emission), _ptr(grad_image),
+            _ptr(ga), _ptr(ge), st)))
         return ga, ge
 
 
@@ -114,8 +126,7 @@ def path_trace(accel: Accel, views, materials, spp=16, max_depth=3, seed=0xC4):
         mm, nm = (Material * max(len(materials), 1))(*materials), len(materials)
     n = sum(v.width * v.height for v in views)
     img = torch.empty((n, 3), dtype=torch.float32, device="cuda")
-    check(lib().ct_path_trace(accel.handle, C.byref(desc), vv, mm, nm, _ptr(img),
-                              C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    _on_torch_stream(lambda st: check(lib().ct_path_trace(accel.handle, C.byref(desc), vv, mm, nm, _ptr(img), st)))
     return img
 
 
@@ -133,8 +144,8 @@ def frame_adjoint(ctx, accel: Accel, params, grad_image, n_materials):
     gd = torch.zeros((n_materials, 3), dtype=torch.float32, device=g.device)
     ge = torch.zeros((n_materials, 3), dtype=torch.float32, device=g.device)
     torch.cuda.synchronize()  # the frame ran on the context's own stream
-    check(lib().ct_frame_adjoint(ctx.handle, accel.handle, C.byref(params), _ptr(g), _ptr(gd), _ptr(ge),
-                                  C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    _on_torch_stream(lambda st: check(lib().ct_frame_adjoint(ctx.handle, accel.handle, C.byref(params), _ptr(g),
+                                                             _ptr(gd), _ptr(ge), st)))
     return gd, ge
 
 

@@ -10,6 +10,8 @@ material's diffuse albedo and emission.
   L(d, e) - L(d, 0) for L = sum(g * image).
 * Without a frame rendered on the accel's current build the call is
   refused (CT_E_STATE), and parameters of another size are rejected.
+* After a batch render each view's context holds the same checkpoint as
+  after rendering the view alone.
 """
 import numpy as np
 import pytest
@@ -88,3 +90,43 @@ def test_frame_adjoint_refusals(gpu):
     gd, ge = frame_adjoint(ctx, acc, p, g, nm)
     torch.cuda.synchronize()
     assert float(gd.abs().sum()) == 0.0 and float(ge.abs().sum()) == 0.0
+
+
+def test_frame_adjoint_after_a_batch_render(gpu):
+    """A context rendered as part of a batch holds the same checkpoint as one
+    that rendered its frame alone: the frame adjoint of each view after two
+    batched frames equals the adjoint after the same frames rendered view by
+    view (float atomics: relative 1e-5)."""
+    import torch
+    scenes, views = ctamd.batched_views(n_views=3, seed=0xC5, grids=(64,), max_depth=2, size_scale=0.1)
+    sw = scenes[0]
+    sc = ctamd.Scene(ctamd.scene_desc(sw))
+    acc = ctamd.Accel(max_instances=sc.info.instance_count + 8,
+                      max_primitives=sc.info.primitive_count + 32 * sc.info.instance_count + 1024,
+                      max_cube_lights=sc.info.cube_light_count + 8)
+    ctamd.load_scene(sc, acc, ctamd.Ctx(8, 8))
+
+    def ctx_for(w):
+        c = ctamd.Ctx(w.width, w.height)
+        ctamd.bind_scene(sc, c)
+        return c
+
+    plain = [ctx_for(w) for _, w in views]
+    batched = [ctx_for(w) for _, w in views]
+    b = ctamd.Batch(batched, [acc] * len(views))
+    for f in range(2):
+        params = [ctamd.frame_params(w, f) for _, w in views]
+        b.render(params)
+        b.stats()
+        for ctx, p in zip(plain, params):
+            ctx.render(acc, p)
+            ctx.stats()
+    nm = sc.info.material_count
+    rng = np.random.default_rng(4)
+    for (_, w), cp, cb, p in zip(views, plain, batched, params):
+        g = torch.from_numpy(rng.standard_normal((w.width * w.height, 4)).astype(np.float32)).cuda()
+        gp = [t.cpu().numpy() for t in frame_adjoint(cp, acc, p, g, nm)]
+        gb = [t.cpu().numpy() for t in frame_adjoint(cb, acc, p, g, nm)]
+        for a, c in zip(gp, gb):
+            assert np.count_nonzero(a) > 0
+            assert np.all(np.abs(a - c) <= 1e-5 * np.abs(a) + 1e-6 * np.abs(a).max())
