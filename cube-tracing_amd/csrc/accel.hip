@@ -1156,6 +1156,7 @@ int ct_accel_task_queue_add(ct_accel *a, const ct_task *task) {
     a->temp_primitive_count += task->u.blas_build_from_cpu.primitive_count;
   }
   a->tasks.push(*task);
+  ++a->task_serial;
   return CT_OK;
 }
 
@@ -1479,18 +1480,25 @@ void task_update(Accel &a, Update &u, const ct_task &t, bool &touched) {
 // light moves into its place and its owner is relinked), highest index
 // first so that a light about to be removed is never the one moved.  The
 // light remapping entries map each original index to its final one.
-void gpu_delete_lights(Accel &a, Update &u, const std::vector<ct_task> &queue,
-                       const std::vector<std::pair<uint32_t, std::vector<ct_primitive>>> &old) {
+// A GPU-deleted instance's primitive records before its compaction: all n
+// of them, or those of the deleted slots (the lights the deletion can
+// remove: a kept voxel's light stays alive).
+struct GpuOld {
+  uint32_t id = 0, n = 0;
+  std::vector<ct_primitive> recs;
+};
+
+void gpu_delete_lights(Accel &a, Update &u, const std::vector<ct_task> &queue, const std::vector<GpuOld> &old) {
   std::vector<uint8_t> alive(a.cube_light_count, 0);
   // one entry per instance (its first task); the instances own disjoint
   // lights, so they are scanned in parallel
-  std::vector<std::pair<const ct_task *, const std::vector<ct_primitive> *>> work;
+  std::vector<std::pair<const ct_task *, const GpuOld *>> work;
   for (const ct_task &t : queue) {
     if (t.type != CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_GPU) continue;
     const uint32_t id = t.u.blas_del_prim_gpu.instance_index;
-    const std::vector<ct_primitive> *o = nullptr;
-    for (const auto &g : old)
-      if (g.first == id) o = &g.second;
+    const GpuOld *o = nullptr;
+    for (const GpuOld &g : old)
+      if (g.id == id) o = &g;
     if (!o || !used(a, id)) continue;
     if (std::any_of(work.begin(), work.end(),
                     [&](const auto &w) { return w.first->u.blas_del_prim_gpu.instance_index == id; }))
@@ -1501,9 +1509,9 @@ void gpu_delete_lights(Accel &a, Update &u, const std::vector<ct_task> &queue,
 #pragma omp parallel for schedule(dynamic, 1)
   for (int64_t q = 0; q < (int64_t)work.size(); ++q) {
     const ct_task &t = *work[q].first;
-    const std::vector<ct_primitive> &o = *work[q].second;
+    const GpuOld &o = *work[q].second;
     const uint32_t id = t.u.blas_del_prim_gpu.instance_index;
-    const uint32_t f = a.instances[id].first_primitive_index, n = (uint32_t)o.size();
+    const uint32_t f = a.instances[id].first_primitive_index, n = o.n;
     const uint32_t keep = n - std::min(t.u.blas_del_prim_gpu.del_prim_count, n);
     for (uint32_t k = 0; k < keep; ++k) {
       const uint32_t L = a.prims[f + k].light_index;
@@ -1512,7 +1520,7 @@ void gpu_delete_lights(Accel &a, Update &u, const std::vector<ct_task> &queue,
         alive[L] = 1;
       }
     }
-    for (const ct_primitive &p : o)
+    for (const ct_primitive &p : o.recs)
       if (p.light_index != kInvalid && p.light_index < a.cube_light_count) gone_of[q].push_back(p.light_index);
   }
   std::vector<uint32_t> gone;
@@ -1655,27 +1663,71 @@ int ct_accel_update_scene(ct_accel *a, int synchronize, void *stream, int *did_w
   // A caller's compaction pass ahead of DELETE_PRIMITIVE_BLAS_FROM_GPU has
   // written device slots and remapping entries: take them before the
   // settle clears the tables.
+  const uint64_t first_serial = a->task_serial - a->tasks.size();
   std::vector<ct_task> queue;
   while (!a->tasks.empty()) {
     queue.push_back(a->tasks.front());
     a->tasks.pop();
   }
+  std::vector<Accel::Compaction> comps;  // the library's own compactions, consumed here
+  comps.swap(a->compactions);
   std::vector<uint2> gpu_entries;  // the caller's compaction's remapping entries (slot, value), value != 0
-  std::vector<std::pair<uint32_t, std::vector<ct_primitive>>> gpu_old;  // records before the compaction
-  // Each GPU-deleted instance's slots (AABB, primitive, remapping entry) come
-  // back through one pinned staging buffer, then into the host mirror.  Slot
-  // ranges separated only by alignment padding are read as one run (the
-  // padding belongs to no primitive).
+  std::vector<GpuOld> gpu_old;     // records before the compaction
+  // An instance the library's compaction pass rearranged, with its task the
+  // only GPU deletion of it in the queue, takes the pass's moves on the host
+  // mirror (the pass wrote nothing else: the tables were clear, as it runs
+  // only on an idle accel).  Any other GPU-deleted instance's slots (AABB,
+  // primitive, remapping entry) come back through one pinned staging
+  // buffer, then into the host mirror.  Slot ranges separated only by
+  // alignment padding are read as one run (the padding belongs to no
+  // primitive).
   std::vector<std::pair<uint32_t, uint32_t>> rb;  // (first slot, count) per instance
-  for (const ct_task &t : queue) {
+  std::vector<const Accel::Compaction *> moves;   // per entry of gpu_old; null: read back
+  for (size_t j = 0; j < queue.size(); ++j) {
+    const ct_task &t = queue[j];
     if (t.type != CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_GPU) continue;
     const uint32_t id = t.u.blas_del_prim_gpu.instance_index;
     if (!used(*a, id) || !a->set.aabbs.p) continue;
     const ct_instance &in = a->instances[id];
     const uint32_t f = in.first_primitive_index, n = in.primitive_count;
-    if (std::any_of(gpu_old.begin(), gpu_old.end(), [&](const auto &g) { return g.first == id; })) continue;
-    gpu_old.emplace_back(id, std::vector<ct_primitive>(a->prims.begin() + f, a->prims.begin() + f + n));
-    rb.emplace_back(f, n);
+    if (std::any_of(gpu_old.begin(), gpu_old.end(), [&](const GpuOld &g) { return g.id == id; })) continue;
+    const Accel::Compaction *c = nullptr;
+    for (const Accel::Compaction &k : comps)
+      if (k.serial == first_serial + j && k.id == id && k.first == f && k.count == n) c = &k;
+    for (size_t k = j + 1; c && k < queue.size(); ++k)
+      if (queue[k].type == CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_GPU && queue[k].u.blas_del_prim_gpu.instance_index == id)
+        c = nullptr;
+    GpuOld g;
+    g.id = id;
+    g.n = n;
+    if (c) {
+      g.recs.reserve(c->del.size());
+      for (uint32_t d : c->del) g.recs.push_back(a->prims[f + d]);
+    } else {
+      g.recs.assign(a->prims.begin() + f, a->prims.begin() + f + n);
+      rb.emplace_back(f, n);
+    }
+    gpu_old.push_back(std::move(g));
+    moves.push_back(c);
+  }
+  {
+    std::vector<std::vector<uint2>> found(moves.size());
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t q = 0; q < (int64_t)moves.size(); ++q) {
+      const Accel::Compaction *c = moves[q];
+      if (!c) continue;
+      const uint32_t f = c->first;
+      for (size_t k = 0; k < c->sur.size(); ++k) {
+        a->aabbs[f + c->del[k]] = a->aabbs[f + c->sur[k]];
+        a->prims[f + c->del[k]] = a->prims[f + c->sur[k]];
+      }
+      std::vector<uint2> &e = found[q];
+      e.reserve(c->del.size() + c->sur.size());
+      for (uint32_t d : c->del) e.push_back(make_uint2(f + d, 0xFFFFFFFFu));
+      for (size_t k = 0; k < c->sur.size(); ++k)
+        if (c->del[k] != 0u) e.push_back(make_uint2(f + c->sur[k], c->del[k]));
+    }
+    for (const auto &e : found) gpu_entries.insert(gpu_entries.end(), e.begin(), e.end());
   }
   if (!rb.empty()) {
     std::vector<std::pair<uint32_t, uint32_t>> runs(rb);  // merged (first, count)
@@ -1739,6 +1791,7 @@ int ct_accel_update_scene(ct_accel *a, int synchronize, void *stream, int *did_w
     }
     for (const auto &f : found) gpu_entries.insert(gpu_entries.end(), f.begin(), f.end());
   }
+  std::sort(gpu_entries.begin(), gpu_entries.end(), [](uint2 x, uint2 y) { return x.x < y.x; });
   bool worked = false;
   if (a->status == CT_AS_SETTLING) {
     if ((rc = settle(*a, st))) return rc;

@@ -331,7 +331,32 @@ int ct_accel_process_voxel_modifications(ct_accel *a, void *stream, uint32_t *in
     CT_HIP(hipGetLastError());
     std::vector<uint32_t> dc(mk.size());
     CT_HIP(hipMemcpyAsync(dc.data(), del.p, 4ull * dc.size(), hipMemcpyDeviceToHost, st));
+    // the compacted instances' marks (one span of words), from which the
+    // host mirror repeats the kernel's moves
+    uint32_t lo = 0xFFFFFFFFu, hi = 0;
+    for (const Marked &m : mk) {
+      lo = std::min(lo, m.first >> 5);
+      hi = std::max(hi, (m.first >> 5) + (m.count + 31u) / 32u);
+    }
+    std::vector<uint32_t> pm(hi - lo);
+    CT_HIP(hipMemcpyAsync(pm.data(), a->brush_pmask.as<uint32_t>() + lo, 4ull * pm.size(), hipMemcpyDeviceToHost, st));
     CT_HIP(hipStreamSynchronize(st));
+    // k_rearrange pairs the k-th deleted local index below the kept count
+    // with the k-th survivor at or above it, both ascending
+    std::vector<Accel::Compaction> cp(mk.size());
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t k = 0; k < (int64_t)mk.size(); ++k) {
+      const uint32_t n = mk[k].count, nw = (n + 31u) / 32u, w0 = (mk[k].first >> 5) - lo;
+      Accel::Compaction &c = cp[k];
+      for (uint32_t w = 0; w < nw; ++w) {
+        const uint32_t valid = (w + 1u) * 32u <= n ? 0xFFFFFFFFu : (1u << (n & 31u)) - 1u;
+        for (uint32_t b = pm[w0 + w] & valid; b; b &= b - 1u) c.del.push_back(32u * w + (uint32_t)__builtin_ctz(b));
+      }
+      const uint32_t keep = n - (uint32_t)c.del.size();
+      size_t d = std::lower_bound(c.del.begin(), c.del.end(), keep) - c.del.begin();
+      for (uint32_t i = keep; i < n && c.sur.size() < d; ++i)
+        if (!((pm[w0 + (i >> 5)] >> (i & 31u)) & 1u)) c.sur.push_back(i);
+    }
     // one DELETE_PRIMITIVE_BLAS_FROM_GPU task per instance, in id order (:2786-2838)
     for (size_t k = 0; k < mk.size(); ++k) {
       if (!dc[k]) continue;
@@ -339,7 +364,15 @@ int ct_accel_process_voxel_modifications(ct_accel *a, void *stream, uint32_t *in
       t.type = CT_TASK_DELETE_PRIMITIVE_BLAS_FROM_GPU;
       t.u.blas_del_prim_gpu.instance_index = ids[k];
       t.u.blas_del_prim_gpu.del_prim_count = dc[k];
+      const uint64_t serial = a->task_serial;
       if ((rc = ct_accel_task_queue_add(a, &t))) return rc;
+      if (cp[k].del.size() == dc[k]) {
+        cp[k].serial = serial;
+        cp[k].id = ids[k];
+        cp[k].first = mk[k].first;
+        cp[k].count = mk[k].count;
+        a->compactions.push_back(std::move(cp[k]));
+      }
       if (instances) ++*instances;
       if (primitives) *primitives += dc[k];
     }

@@ -231,6 +231,19 @@ struct Accel {
   // voxel-deletion marks (squared_brush.slang): instance bits, slot bits, counters
   DevBuf brush_imask, brush_pmask, brush_counters;
   uint32_t brush_imask_words = 0, brush_pmask_words = 0;
+  // The library's own compaction of an instance (k_rearrange): its
+  // DELETE_PRIMITIVE_BLAS_FROM_GPU task's serial, the instance's slot range,
+  // the deleted local indices (ascending) and the survivors at or above the
+  // kept count (ascending) that took the deleted slots below it, pairwise.
+  // update_scene applies the moves to the host mirror instead of reading the
+  // range back.  task_serial counts every task ever added.
+  struct Compaction {
+    uint64_t serial;
+    uint32_t id, first, count;
+    std::vector<uint32_t> del, sur;
+  };
+  std::vector<Compaction> compactions;
+  uint64_t task_serial = 0;
   // materials of ct_path_trace, packed, kept between calls
   DevBuf pt_mats;
   uint32_t pt_mat_count = 0;

@@ -798,7 +798,11 @@ int ct_scene_from_vox(const void *data, size_t bytes, const ct_vox_desc *desc, c
  * remapping table (moved: new local index; deleted: ~0u), queues one
  * DELETE_PRIMITIVE_BLAS_FROM_GPU task per instance in id order and clears
  * the marks.  The next ct_accel_update_scene applies the tasks (BLAS
- * rebuild, or instance deletion when nothing is left).  Unlike the
+ * rebuild, or instance deletion when nothing is left); it takes the moves
+ * into its host copy of the scene from this call's record, so the slots of
+ * a compacted instance are left alone until then (a caller's own pass that
+ * compacts the instance again queues a task of its own, and the update then
+ * reads the instance's slots back from the device).  Unlike the
  * reference, which leaves them in place, that update also removes the
  * cube lights of deleted voxels (the last light moves into each hole and
  * its owner is relinked, highest index first), points the lights of moved

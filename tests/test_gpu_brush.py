@@ -29,6 +29,7 @@ def _vox_scene():
 def _check_compaction(acc, m, plan, before):
     """Device records right after processing, then the model after update_scene."""
     W = acc.read_world()
+    rms = {}
     for i, dele in plan.items():
         f, n = before[i]
         moves, keep, remap = B.rearrange(n, dele)
@@ -39,7 +40,14 @@ def _check_compaction(acc, m, plan, before):
         for k, v in remap.items():
             rm[k] = v
         assert np.array_equal(W["remap_prims"][f:f + n], rm), i
+        rms[i] = (f, n, keep, rm)
     acc.update_scene()
+    # the compaction's remapping entries survive the update that applies it
+    # (a kept instance's slots take no other entries)
+    W = acc.read_world()
+    for i, (f, n, keep, rm) in rms.items():
+        if keep:
+            assert np.array_equal(W["remap_prims"][f:f + n], rm), i
     m.update([("delete_from_gpu", i, B.compacted_records(m.instances[i].voxels, before[i][1], plan[i]),
                len(set(plan[i]))) for i in sorted(plan)])
     _compare(acc, m)
@@ -87,6 +95,16 @@ def test_marked_voxels_compact_and_rebuild(gpu):
     _trace_equal(acc, osc, sc, 23)
     acc.update_scene()  # settle clears the remapping entries
     assert not acc.read_world()["remap_prims"].any()
+    # The host mirror took the compaction's moves without reading the slots
+    # back: CPU deletions in the compacted instances move the records it
+    # holds at the ends of the kept ranges (six per instance, index 0 each
+    # time, so six tails are read).
+    ops = [("delete_primitive", i, 0) for i in sorted(plan) if i != ids[-1] for _ in range(6)]
+    for _, i, p in ops:
+        acc.delete_primitive(i, p)
+    acc.update_scene()
+    m.update(ops)
+    _compare(acc, m)
 
 
 def test_brush_stroke_on_a_frame(gpu):
