@@ -2817,8 +2817,19 @@ int ct_batch_get_stats(ct_batch *b, ct_frame_stats *out) {
   return CT_OK;
 }
 
+// The tuner is switched off (CT_BATCH_AUTOTUNE 0): one test run of it ended
+// in a segmentation fault of the host process, inside this call, whose
+// cause reading the code has not found.  Built with CT_BATCH_AUTOTUNE=1 it
+// runs as described in ct_abi.h.
+#ifndef CT_BATCH_AUTOTUNE
+#define CT_BATCH_AUTOTUNE 0
+#endif
 int ct_batch_autotune(ct_batch *b, const ct_frame_params *params, uint32_t repeats, ct_batch_tuning *out) {
   if (!b || !params || repeats == 0) return fail(CT_E_INVALID, "ct_batch_autotune: bad argument");
+#if !CT_BATCH_AUTOTUNE
+  (void)out;
+  return fail(CT_E_UNSUPPORTED, "ct_batch_autotune: switched off (a host segmentation fault inside it is unexplained)");
+#else
   // the split of the views into concurrent groups, the persistent traversal
   // grid (blocks per CU; its waves per CU are 4x that), and the far/near
   // split of the ray queue (the order the traversal takes its rays in)
@@ -2866,6 +2877,7 @@ int ct_batch_autotune(ct_batch *b, const ct_frame_params *params, uint32_t repea
     out->far_ray = best_far;
   }
   return CT_OK;
+#endif
 }
 
 int ct_ctx_set_instrumentation(ct_ctx *c, uint32_t flags) {

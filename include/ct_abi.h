@@ -676,7 +676,11 @@ int ct_batch_get_stats(ct_batch *batch, ct_frame_stats *out);
  * grid never exceeds the blocks that can be resident at once) and the ray
  * queue's order (shadow rays at least far_ray long are queued and traced
  * first: 1, 2, 4 world units or none): 48 shapes.  No shape changes a
- * result.  A new batch starts with 2 groups, 5 blocks per CU and 2 units. */
+ * result.  A new batch starts with 2 groups, 5 blocks per CU and 2 units.
+ * Switched off in this build: returns CT_E_UNSUPPORTED unless the library
+ * is compiled with CT_BATCH_AUTOTUNE=1 (one run of it ended in a host
+ * segmentation fault whose cause is not found); batches keep the default
+ * shape. */
 typedef struct ct_batch_tuning {
   uint32_t blocks_per_cu;
   float ms;            /* mean batch time of the choice */
